@@ -1,0 +1,237 @@
+#!/usr/bin/env python3
+"""Benchmark of the pairwise alignment hot path on one MI355X.
+
+One step = one pass of the path over one batch of synthetic, seeded input that
+is already resident in HBM:
+
+  headline  Smith-Waterman, score + span + CIGAR: 50,000 pairs, query 256 bp
+            i.i.d. ACGT, target 1,024 bp i.i.d. ACGT with a 10%-mutated copy
+            of the query (sub:ins:del = 2:1:1) written in at a uniform offset;
+            match 2 / mismatch -3 / gap open 3 / gap extend 1.  value = GCUPS
+            (cells = sum of text_len * pattern_len per step / step time).
+  secondary WFA edit distance, score + CIGAR: 10,000 pairs, pattern 150 bp,
+            text = pattern with 2% per-base edits (2:1:1), smax 64 (pairs/s).
+
+Also printed on the same JSON line: the dominant kernel's roofline (timed live
+with HIP events on the launch stream), per-step latency p50/p99, the number of
+results that differ from the CPU oracle on a checked sample, and the oracle
+(the CPU restatement of the reference path) timed on this host's cores on a
+bounded sample of the same workload.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BASELINE_METRIC = "WFA pairs/s + SW GCUPS on 1 MI355X, routed vs plain, 0 mismatches; p99 ms"
+SW_SCHEME = (2, -3, 3, 1)
+SW_OPS_PER_CELL = 12            # E: 2 sub + 1 max; F: 2 sub + 1 max; H: add + 3 max; best: 1 max
+PEAK_INT16_TOPS = 78.6          # gfx950 packed-int16 VALU: 256 CU x 2.4 GHz x 64 lanes x 2
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pctl(xs, q):
+    return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
+
+
+def time_steps(run, steps, warmup, lib):
+    import torch
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    kern = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record()
+        run()
+        b.record()
+        kern.append(lib.ga_last_kernel_ms())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = [a.elapsed_time(b) for a, b in ev]
+    return wall, step_ms, kern
+
+
+def check_sw(ps, dev, sample, oracle):
+    out, cig = dev.results()
+    bad = 0
+    for i in sample:
+        r = oracle.sw_trace(ps.a_seq(i), ps.b_seq(i), SW_SCHEME)
+        if tuple(int(x) for x in out[i]) != r[:5] or cig[i].decode() != r[5]:
+            bad += 1
+    return bad
+
+
+def check_wfa(ps, dev, sample, oracle):
+    sc, cig = dev.results()
+    bad = 0
+    for i in sample:
+        osc, ocg = oracle.wfa(ps.a_seq(i), ps.b_seq(i), int(ps.smax[i]))
+        if int(sc[i]) != osc or cig[i].decode() != ocg:
+            bad += 1
+    return bad
+
+
+def cpu_baseline_sw(ps, seconds, threads, oracle):
+    """The oracle's SW trace path on `threads` host threads over a sample of the
+    same workload, sized to run about `seconds`."""
+    import ctypes as C
+    L = oracle.lib()
+    tab = oracle.mm_table(SW_SCHEME[0], SW_SCHEME[1])
+
+    def run(n):
+        sub = ps.subset(np.arange(n))
+        out = np.zeros(5 * n, dtype=np.int32)
+        cig = np.zeros(int((sub.a_len.astype(np.int64) + sub.b_len).sum()) + 1, dtype=np.uint8)
+        coff = np.zeros(n, dtype=np.int64)
+        coff[1:] = np.cumsum(sub.a_len.astype(np.int64) + sub.b_len)[:-1]
+        clen = np.zeros(n, dtype=np.int32)
+        t0 = time.perf_counter()
+        rc = L.ga_oracle_sw_batch(sub.a.ctypes.data, sub.a_off.ctypes.data, sub.a_len.ctypes.data,
+                                  sub.b.ctypes.data, sub.b_off.ctypes.data, sub.b_len.ctypes.data,
+                                  n, tab.ctypes.data, SW_SCHEME[2], SW_SCHEME[3], 1,
+                                  out.ctypes.data, cig.ctypes.data, coff.ctypes.data,
+                                  clen.ctypes.data, threads)
+        assert rc == 0
+        return time.perf_counter() - t0, int((sub.a_len.astype(np.int64) * sub.b_len).sum())
+
+    probe = max(threads * 4, 32)
+    dt, _ = run(probe)
+    n = int(min(ps.n, max(probe, probe * seconds / max(dt, 1e-3))))
+    dt, cells = run(n)
+    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} of the {ps.n} pairs (256 x 1024, score + span + CIGAR), "
+                      f"{dt:.1f} s on {threads} threads of the benchmark host"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-wfa", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} is not supported; pairs are independent, see DESIGN.md "
+              f"for how the batch would be sharded)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from genoaligner_amd import _native as N
+    from genoaligner_amd import workloads as W
+    from genoaligner_amd.device import SwDeviceBatch, WfaDeviceBatch
+    from oracle import oracle as O
+
+    lib = N.lib()
+    lib.ga_set_kernel_timing(1)
+    torch.cuda.init()
+    dev_name = torch.cuda.get_device_name(0)
+    log(f"device: {dev_name}")
+
+    # ---------------- SW headline ----------------
+    t0 = time.perf_counter()
+    sw = W.sw_embedded(n=50_000, qlen=256, tlen=1024, rate=0.10)
+    cells = int((sw.a_len.astype(np.int64) * sw.b_len).sum())
+    swd = SwDeviceBatch(sw, scoring=SW_SCHEME, trace=True)
+    log(f"sw workload ready ({time.perf_counter() - t0:.1f} s), {cells / 1e9:.2f} Gcells/step")
+    wall, step_ms, kern = time_steps(swd.run, args.steps, args.warmup, lib)
+    ms_per_step = wall * 1000.0 / args.steps
+    gcups = cells / (wall / args.steps) / 1e9
+    kern_ms = float(np.mean([k for k in kern if k > 0])) if any(k > 0 for k in kern) else None
+    log(f"sw: {ms_per_step:.2f} ms/step, {gcups:.1f} GCUPS, forward kernel {kern_ms} ms")
+    rng = np.random.default_rng(1)
+    sample = rng.choice(sw.n, size=min(300, sw.n), replace=False)
+    sw_bad = check_sw(sw, swd, sample, O)
+    log(f"sw: {sw_bad} mismatches in {len(sample)} checked pairs")
+
+    roof = None
+    if kern_ms:
+        achieved = SW_OPS_PER_CELL * cells / (kern_ms * 1e-3) / 1e12
+        roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS,
+                "unit": "TOP/s", "frac": round(achieved / PEAK_INT16_TOPS, 4),
+                "traffic": None, "kernel": "sw_kernel<trace> (forward)",
+                "kernel_ms": round(kern_ms, 3)}
+
+    # ---------------- WFA secondary ----------------
+    wfa_line = None
+    if not args.no_wfa:
+        wf = W.wfa_plain(n=10_000, length=150, rate=0.02, smax=64)
+        wfd = WfaDeviceBatch(wf, trace=True)
+        wwall, wstep, wkern = time_steps(wfd.run, args.steps, args.warmup, lib)
+        wsample = rng.choice(wf.n, size=min(1000, wf.n), replace=False)
+        wbad = check_wfa(wf, wfd, wsample, O)
+        wk = float(np.mean([k for k in wkern if k > 0])) if any(k > 0 for k in wkern) else None
+        in_bytes = int(((wf.a_len.astype(np.int64) + 3) // 4 + (wf.b_len.astype(np.int64) + 3) // 4).sum())
+        sc, cg = wfd.results()
+        out_bytes = 5 * wf.n + int(sum(len(c) for c in cg))
+        wfa_line = {
+            "workload": "wfa_plain 10000 x 150 bp, 2% edits (2:1:1), smax 64, score + CIGAR",
+            "pairs_per_s": round(wf.n / (wwall / args.steps), 1),
+            "ms_per_step": round(wwall * 1000.0 / args.steps, 3),
+            "p50_ms": pctl(wstep, 50), "p99_ms": pctl(wstep, 99),
+            "kernel_ms": round(wk, 4) if wk else None,
+            "hbm_roofline": {"bound": "hbm", "algorithmic_bytes": in_bytes + out_bytes,
+                             "achieved_GBs": round((in_bytes + out_bytes) / (wk * 1e-3) / 1e9, 2) if wk else None,
+                             "peak_GBs": PEAK_HBM_GBS},
+            "mismatches": wbad, "checked": int(len(wsample)),
+            "tcups_ref_style": round(float((wf.a_len.astype(np.int64) * wf.b_len).sum())
+                                     / (wwall / args.steps) / 1e12, 4),
+        }
+        log(f"wfa: {wfa_line['pairs_per_s']:.0f} pairs/s, {wbad} mismatches")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(os.cpu_count() or 1, 16)
+        cpu = cpu_baseline_sw(sw, args.cpu_seconds, threads, O)
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": BASELINE_METRIC,
+        "value": round(gcups, 3),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded)",
+        "config": {"workload": "sw_embedded: 50000 pairs, query 256 bp, target 1024 bp with a "
+                               "10%-mutated query copy (2:1:1), scoring 2/-3/3/1, score + span + CIGAR",
+                   "pairs": sw.n, "cells_per_step": cells, "device": dev_name},
+        "p50_ms": pctl(step_ms, 50),
+        "p99_ms": pctl(step_ms, 99),
+        "mismatches": sw_bad,
+        "checked": int(len(sample)),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "wfa": wfa_line,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

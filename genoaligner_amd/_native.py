@@ -1,0 +1,112 @@
+"""ctypes binding of libgenoaligner_amd.so (the C ABI in
+include/genoaligner/genoaligner_c.h).
+
+The library is required: importing a compute entry point without it raises,
+there is no CPU fallback.  It is built in-tree by ``python -m
+genoaligner_amd.build`` (or ``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from . import build as _build
+
+LIB_PATH = _build.LIB
+
+GA_OK, GA_INVALID_ARGUMENT, GA_DEVICE_ERROR = 0, 1, 2
+GA_MAX_SMAX = 511
+GA_MAX_SMAX_EXT = 4000
+GA_SW_MAX_TRACE_CELLS = 1 << 26
+
+# Every symbol include/genoaligner/genoaligner_c.h declares.
+EXPORTS = (
+    "ga_align_batch", "ga_align_batch_ext", "ga_align_sw_batch",
+    "ga_backend_name", "ga_device_name", "ga_device_available",
+    "ga_cigar_arena_bytes_wfa", "ga_cigar_arena_bytes_sw",
+    "ga_wfa_device", "ga_sw_device", "ga_set_kernel_timing", "ga_last_kernel_ms",
+)
+
+
+class AlignReq(C.Structure):
+    _fields_ = [("pattern", C.c_void_p), ("pattern_len", C.c_int32),
+                ("text", C.c_void_p), ("text_len", C.c_int32),
+                ("smax", C.c_int32), ("with_cigar", C.c_int32)]
+
+
+class AlignRes(C.Structure):
+    _fields_ = [("score", C.c_int32), ("resolved", C.c_int32), ("cigar_off", C.c_int64),
+                ("cigar_len", C.c_int32), ("rescore_ok", C.c_int32), ("wellformed_ok", C.c_int32)]
+
+
+class SWScoringC(C.Structure):
+    _fields_ = [("match", C.c_int32), ("mismatch", C.c_int32),
+                ("gap_open", C.c_int32), ("gap_extend", C.c_int32)]
+
+
+class SWReq(C.Structure):
+    _fields_ = [("text", C.c_void_p), ("text_len", C.c_int32),
+                ("pattern", C.c_void_p), ("pattern_len", C.c_int32),
+                ("scoring", SWScoringC), ("with_cigar", C.c_int32)]
+
+
+class SWRes(C.Structure):
+    _fields_ = [("score", C.c_int32), ("resolved", C.c_int32), ("cigar_off", C.c_int64),
+                ("cigar_len", C.c_int32), ("start_i", C.c_int32), ("start_j", C.c_int32),
+                ("end_i", C.c_int32), ("end_j", C.c_int32), ("rescore_ok", C.c_int32),
+                ("wellformed_ok", C.c_int32), ("too_large", C.c_int32)]
+
+
+class WfaDevBatch(C.Structure):
+    _fields_ = [("pattern", C.c_void_p), ("pattern_off", C.c_void_p), ("pattern_len", C.c_void_p),
+                ("text", C.c_void_p), ("text_off", C.c_void_p), ("text_len", C.c_void_p),
+                ("smax", C.c_void_p), ("n_pairs", C.c_int64), ("smax_max", C.c_int32),
+                ("max_len", C.c_int32), ("score", C.c_void_p), ("cigar", C.c_void_p),
+                ("cigar_off", C.c_void_p), ("cigar_len", C.c_void_p)]
+
+
+class SwDevBatch(C.Structure):
+    _fields_ = [("text", C.c_void_p), ("text_off", C.c_void_p), ("text_len", C.c_void_p),
+                ("pattern", C.c_void_p), ("pattern_off", C.c_void_p), ("pattern_len", C.c_void_p),
+                ("n_pairs", C.c_int64), ("max_text_len", C.c_int32), ("max_pattern_len", C.c_int32),
+                ("scoring", SWScoringC), ("with_cigar", C.c_int32), ("out", C.c_void_p),
+                ("cigar", C.c_void_p), ("cigar_off", C.c_void_p), ("cigar_len", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m genoaligner_amd.build` "
+                "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        i32p, cpp = C.POINTER(C.c_int32), C.POINTER(C.c_char_p)
+        for name, req, res in (("ga_align_batch", AlignReq, AlignRes),
+                               ("ga_align_batch_ext", AlignReq, AlignRes),
+                               ("ga_align_sw_batch", SWReq, SWRes)):
+            f = getattr(L, name)
+            f.argtypes = [C.POINTER(req), C.c_int64, C.POINTER(res), C.c_void_p, C.c_int64,
+                          i32p, i32p, cpp]
+            f.restype = C.c_int
+        L.ga_backend_name.restype = C.c_char_p
+        L.ga_device_name.restype = C.c_char_p
+        L.ga_device_available.restype = C.c_int
+        L.ga_cigar_arena_bytes_wfa.argtypes = [C.POINTER(AlignReq), C.c_int64]
+        L.ga_cigar_arena_bytes_wfa.restype = C.c_int64
+        L.ga_cigar_arena_bytes_sw.argtypes = [C.POINTER(SWReq), C.c_int64]
+        L.ga_cigar_arena_bytes_sw.restype = C.c_int64
+        L.ga_wfa_device.argtypes = [C.POINTER(WfaDevBatch), C.c_void_p, cpp]
+        L.ga_wfa_device.restype = C.c_int
+        L.ga_sw_device.argtypes = [C.POINTER(SwDevBatch), C.c_void_p, cpp]
+        L.ga_sw_device.restype = C.c_int
+        L.ga_set_kernel_timing.argtypes = [C.c_int]
+        L.ga_set_kernel_timing.restype = None
+        L.ga_last_kernel_ms.argtypes = []
+        L.ga_last_kernel_ms.restype = C.c_float
+        _lib = L
+    return _lib

@@ -1,0 +1,817 @@
+// host.cpp -- the C ABI (include/genoaligner/genoaligner_c.h) over the gfx950
+// kernels: request validation, batch packing, device buffers on a per-call
+// stream, launches, and result decoding with the library-side CIGAR checks.
+//
+// Batch and result rules restate the reference API (src/api/api.cpp):
+//   WFA   validation :245-266, min smax :271-272, any_cigar :273-274 and the
+//         mixed-batch rule that every resolved pair then gets a CIGAR :456-469,
+//         decode + validate :71-106
+//   SW    validation :600-616 with the scheme rules :515-536, buffer sizing
+//         :636-647, result fill :542-580 and the score-only fill :829-838
+// The error strings are the reference's, so callers that log or match them see
+// the same text.
+#include "ga_device.hpp"
+#include "../../include/genoaligner/genoaligner_c.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+using ga::kSeqPad;
+
+// ---------------------------------------------------------------------------
+// device plumbing
+// ---------------------------------------------------------------------------
+std::once_flag g_pool_once;
+
+void init_pool()
+{
+    // Keep freed stream-ordered allocations cached in the device's default pool
+    // instead of returning them to the driver at every synchronisation.
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t thr = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+}
+
+// A small pool of streams (default priority), reused across calls.
+std::mutex g_stream_mu;
+std::vector<hipStream_t> g_streams;
+
+hipStream_t acquire_stream()
+{
+    {
+        std::lock_guard<std::mutex> lk(g_stream_mu);
+        if (!g_streams.empty()) {
+            hipStream_t s = g_streams.back();
+            g_streams.pop_back();
+            return s;
+        }
+    }
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    return s;
+}
+
+void release_stream(hipStream_t s)
+{
+    if (!s) return;
+    std::lock_guard<std::mutex> lk(g_stream_mu);
+    g_streams.push_back(s);
+}
+
+// Stream-ordered device allocations freed together at the end of a call.
+struct DevArena {
+    hipStream_t st = nullptr;
+    std::vector<void*> ptrs;
+    bool ok = true;
+    template <class T> T* alloc(size_t count)
+    {
+        void* p = nullptr;
+        const size_t bytes = std::max<size_t>(count * sizeof(T), 64);
+        if (hipMallocAsync(&p, bytes, st) != hipSuccess) { ok = false; return nullptr; }
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+    template <class T> T* upload(const std::vector<T>& v)
+    {
+        T* d = alloc<T>(v.size());
+        if (d && !v.empty() &&
+            hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess)
+            ok = false;
+        return d;
+    }
+    void release()
+    {
+        for (void* p : ptrs) (void)hipFreeAsync(p, st);
+        ptrs.clear();
+    }
+};
+
+// Packed byte buffer: kSeqPad bytes before the first and after the last
+// sequence, sequences back to back, int64 offsets.
+struct Packed {
+    std::vector<uint8_t> bytes;
+    std::vector<int64_t> off;
+    void reserve(size_t n, size_t total) { off.reserve(n); bytes.reserve(total + 2 * kSeqPad); }
+    void begin() { bytes.assign(kSeqPad, 0); }
+    void add(const char* p, int len)
+    {
+        off.push_back(int64_t(bytes.size()));
+        if (len > 0) bytes.insert(bytes.end(), p, p + len);
+    }
+    void end() { bytes.insert(bytes.end(), kSeqPad, 0); }
+};
+
+// Run fn(i) for i in [0, n) on up to 16 host threads.
+template <class F> void parallel_for(int64_t n, F&& fn)
+{
+    const int64_t grain = 2048;
+    unsigned hw = std::thread::hardware_concurrency();
+    int nt = int(std::min<int64_t>((n + grain - 1) / grain, std::min<unsigned>(hw ? hw : 1, 16u)));
+    if (nt <= 1) { for (int64_t i = 0; i < n; ++i) fn(i); return; }
+    std::atomic<int64_t> next{0};
+    auto work = [&] {
+        for (;;) {
+            const int64_t i0 = next.fetch_add(grain);
+            if (i0 >= n) break;
+            const int64_t i1 = std::min(n, i0 + grain);
+            for (int64_t i = i0; i < i1; ++i) fn(i);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+}
+
+// ---------------------------------------------------------------------------
+// optional kernel timing (per calling thread)
+// ---------------------------------------------------------------------------
+struct KernelTimer {
+    bool enabled = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
+    size_t used = 0;
+    void reset() { used = 0; }
+    void begin(hipStream_t st)
+    {
+        if (!enabled) return;
+        if (used == pairs.size()) {
+            hipEvent_t a = nullptr, b = nullptr;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+            pairs.emplace_back(a, b);
+        }
+        (void)hipEventRecord(pairs[used].first, st);
+    }
+    void end(hipStream_t st)
+    {
+        if (!enabled || used >= pairs.size()) return;
+        (void)hipEventRecord(pairs[used].second, st);
+        ++used;
+    }
+    float total_ms()
+    {
+        if (used == 0) return -1.0f;
+        float sum = 0.0f;
+        for (size_t i = 0; i < used; ++i) {
+            if (hipEventSynchronize(pairs[i].second) != hipSuccess) return -1.0f;
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, pairs[i].first, pairs[i].second) != hipSuccess) return -1.0f;
+            sum += ms;
+        }
+        return sum;
+    }
+};
+thread_local KernelTimer t_timer;
+
+// Brackets the SW forward kernel inside sw_launch (the walk is not timed).
+struct TimerHooks : ga::LaunchHooks {
+    void before(hipStream_t st) override { t_timer.begin(st); }
+    void after(hipStream_t st) override { t_timer.end(st); }
+};
+thread_local TimerHooks t_timer_hooks;
+
+// ---------------------------------------------------------------------------
+// the library-side CIGAR checks (api.cpp:84-106 and :114-156)
+// ---------------------------------------------------------------------------
+void wfa_validate(const char* cg, int len, int score, const char* pattern, int m,
+                  const char* text, int n, int32_t* rescore_ok, int32_t* wellformed_ok)
+{
+    int s = 0, pi = 0, ti = 0;
+    bool wf = true;
+    for (int c = 0; c < len; ++c) {
+        switch (cg[c]) {
+            case 'M': if (pi >= m || ti >= n || pattern[pi] != text[ti]) wf = false; ++pi; ++ti; break;
+            case 'X': if (pi >= m || ti >= n || pattern[pi] == text[ti]) wf = false; ++pi; ++ti; ++s; break;
+            case 'I': ++ti; ++s; break;
+            case 'D': ++pi; ++s; break;
+            default: wf = false; break;
+        }
+    }
+    if (pi != m || ti != n) wf = false;
+    *wellformed_ok = wf;
+    *rescore_ok = (s == score);
+}
+
+void sw_validate(const char* cg, int len, const char* text, int text_len, int start_i, int end_i,
+                 const char* pattern, int pattern_len, int start_j, int end_j,
+                 const ga_sw_scoring& sc, int score, int32_t* rescore_ok, int32_t* wellformed_ok)
+{
+    int i = start_i, j = start_j, s = 0, run = 0;
+    bool wf = true;
+    auto close_gap = [&] {
+        if (run != 0) {
+            const int L = run > 0 ? run : -run;
+            s -= sc.gap_open + (L - 1) * sc.gap_extend;
+            run = 0;
+        }
+    };
+    for (int c = 0; c < len; ++c) {
+        switch (cg[c]) {
+            case 'M':
+                close_gap();
+                if (i > end_i || j > end_j || i >= text_len || j >= pattern_len || text[i] != pattern[j]) wf = false;
+                ++i; ++j; s += sc.match; break;
+            case 'X':
+                close_gap();
+                if (i > end_i || j > end_j || i >= text_len || j >= pattern_len || text[i] == pattern[j]) wf = false;
+                ++i; ++j; s += sc.mismatch; break;
+            case 'I':
+                if (run < 0) close_gap();
+                if (i > end_i || i >= text_len) wf = false;
+                ++i; ++run; break;
+            case 'D':
+                if (run > 0) close_gap();
+                if (j > end_j || j >= pattern_len) wf = false;
+                ++j; --run; break;
+            default: wf = false; break;
+        }
+    }
+    close_gap();
+    if (i != end_i + 1 || j != end_j + 1) wf = false;
+    *wellformed_ok = wf;
+    *rescore_ok = wf && (s == score);
+}
+
+// ---------------------------------------------------------------------------
+// WFA batch on the device
+// ---------------------------------------------------------------------------
+// Workspace budget for the trace variant's wavefront stacks; batches whose
+// stacks exceed it run in consecutive chunks that reuse one buffer.
+constexpr int64_t kWfaWsBudget = int64_t(4) << 30;
+
+struct WfaOut {
+    std::vector<int32_t> score, cig_len;
+    std::vector<int64_t> slot_off;   // slot of pair i in the arena
+};
+
+// Runs every pair with its own smax; trace[i] == 0 means score only (no CIGAR
+// for that pair).  The CIGAR bytes land in `arena` (slot layout).
+const char* run_wfa(const ga_align_req* reqs, int64_t n, const std::vector<int32_t>& smax,
+                    const std::vector<uint8_t>& trace, char* arena, WfaOut& out)
+{
+    std::call_once(g_pool_once, init_pool);
+    Packed P, T;
+    P.begin(); T.begin();
+    out.slot_off.resize(static_cast<size_t>(n));
+    int64_t slot = 0;
+    int32_t smax_max = 0;
+    std::vector<int32_t> plen(static_cast<size_t>(n)), tlen(static_cast<size_t>(n));
+    std::vector<int64_t> trace_ids, score_ids;
+    for (int64_t i = 0; i < n; ++i) {
+        const ga_align_req& r = reqs[i];
+        plen[size_t(i)] = r.pattern_len;
+        tlen[size_t(i)] = r.text_len;
+        P.add(r.pattern, r.pattern_len);
+        T.add(r.text, r.text_len);
+        out.slot_off[size_t(i)] = slot;
+        slot += std::max(r.pattern_len, 0) + std::max(r.text_len, 0);
+        smax_max = std::max(smax_max, smax[size_t(i)]);
+        (trace[size_t(i)] ? trace_ids : score_ids).push_back(i);
+    }
+    P.end(); T.end();
+
+    hipStream_t st = acquire_stream();
+    if (!st) return "hipStreamCreate";
+    DevArena D;
+    D.st = st;
+    const char* err = nullptr;
+    auto* d_pat = D.upload(P.bytes);
+    auto* d_txt = D.upload(T.bytes);
+    auto* d_poff = D.upload(P.off);
+    auto* d_toff = D.upload(T.off);
+    auto* d_plen = D.upload(plen);
+    auto* d_tlen = D.upload(tlen);
+    auto* d_smax = D.upload(smax);
+    auto* d_score = D.alloc<int32_t>(static_cast<size_t>(n));
+    uint8_t* d_cig = nullptr;
+    int64_t* d_cigoff = nullptr;
+    int32_t* d_ciglen = nullptr;
+    int32_t* d_ws = nullptr;
+    const int64_t ws_stride = int64_t(smax_max + 1) * int64_t(smax_max + 1);
+    int64_t chunk = 0;
+    if (!trace_ids.empty()) {
+        d_cig = D.alloc<uint8_t>(size_t(slot));
+        d_cigoff = D.upload(out.slot_off);
+        d_ciglen = D.alloc<int32_t>(static_cast<size_t>(n));
+        chunk = std::max<int64_t>(1, kWfaWsBudget / (ws_stride * 4));
+        chunk = std::min<int64_t>(chunk, int64_t(trace_ids.size()));
+        d_ws = D.alloc<int32_t>(size_t(chunk * ws_stride));
+    }
+    if (!D.ok) err = "hipMalloc/hipMemcpy (WFA buffers)";
+
+    // Pairs are launched by id ranges; a batch is traced or score-only per pair,
+    // so each run of consecutive same-kind ids is one launch (or several chunks).
+    auto launch_range = [&](int64_t a, int64_t b, bool tr) -> hipError_t {
+        while (a < b) {
+            const int64_t len = tr ? std::min(chunk, b - a) : std::min<int64_t>(b - a, 1 << 30);
+            ga::WfaLaunch L{};
+            L.pat = d_pat; L.poff = d_poff; L.plen = d_plen;
+            L.txt = d_txt; L.toff = d_toff; L.tlen = d_tlen;
+            L.smax = d_smax; L.n = int32_t(len); L.smax_max = smax_max; L.score = d_score;
+            L.cigar = d_cig; L.cig_off = d_cigoff; L.cig_len = d_ciglen;
+            L.ws = d_ws; L.ws_stride = ws_stride; L.first_pair = int32_t(a);
+            hipError_t e = ga::wfa_launch(L, tr, st);
+            if (e != hipSuccess) return e;
+            a += len;
+        }
+        return hipSuccess;
+    };
+    if (!err) {
+        int64_t a = 0;
+        while (a < n && !err) {
+            int64_t b = a;
+            while (b < n && trace[size_t(b)] == trace[size_t(a)]) ++b;
+            if (launch_range(a, b, trace[size_t(a)] != 0) != hipSuccess) err = "WFA kernel launch";
+            a = b;
+        }
+    }
+    out.score.assign(static_cast<size_t>(n), -1);
+    out.cig_len.assign(static_cast<size_t>(n), 0);
+    if (!err) {
+        if (hipMemcpyAsync(out.score.data(), d_score, size_t(n) * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+            err = "hipMemcpy(results)";
+        if (!err && d_cig) {
+            if (hipMemcpyAsync(out.cig_len.data(), d_ciglen, size_t(n) * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                (slot > 0 && hipMemcpyAsync(arena, d_cig, size_t(slot), hipMemcpyDeviceToHost, st) != hipSuccess))
+                err = "hipMemcpy(results)";
+        }
+    }
+    D.release();
+    const hipError_t se = hipStreamSynchronize(st);
+    if (!err && se != hipSuccess) err = hipGetErrorString(se);
+    release_stream(st);
+    return err;
+}
+
+void reset_wfa(ga_align_res* res, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) res[i] = ga_align_res{-1, 0, 0, 0, 0, 0};
+}
+
+int wfa_common(const ga_align_req* reqs, int64_t n, ga_align_res* res, char* arena,
+               int64_t arena_bytes, int32_t* resolved_count, int32_t* unresolved_count,
+               const char** error, bool ext)
+{
+    if (error) *error = nullptr;
+    if (resolved_count) *resolved_count = 0;
+    if (unresolved_count) *unresolved_count = 0;
+    if (n <= 0) return GA_OK;
+    reset_wfa(res, n);
+    auto refuse = [&](const char* msg) {
+        if (error) *error = msg;
+        reset_wfa(res, n);
+        return GA_INVALID_ARGUMENT;
+    };
+    if (!reqs || !res) return refuse("null request or result array");
+    const int max_smax = ext ? GA_MAX_SMAX_EXT : GA_MAX_SMAX;
+    for (int64_t i = 0; i < n; ++i) {
+        const ga_align_req& r = reqs[i];
+        if ((r.pattern_len > 0 && r.pattern == nullptr) || (r.text_len > 0 && r.text == nullptr))
+            return refuse("null pointer with non-zero length");
+        if (r.pattern_len < 0 || r.text_len < 0) return refuse("negative length");
+        if (r.smax < 0 || r.smax > max_smax)
+            return refuse(ext ? "smax out of range [0, 4000]" : "smax out of range [0, 511]");
+    }
+    const int64_t need = ga_cigar_arena_bytes_wfa(reqs, n);
+    if (need > 0 && (arena == nullptr || arena_bytes < need))
+        return refuse("CIGAR arena smaller than the batch needs (ga_cigar_arena_bytes_wfa)");
+
+    std::vector<int32_t> smax(static_cast<size_t>(n));
+    std::vector<uint8_t> trace(static_cast<size_t>(n));
+    bool any_cigar = false;
+    int32_t smin = reqs[0].smax;
+    for (int64_t i = 0; i < n; ++i) {
+        any_cigar |= reqs[i].with_cigar != 0;
+        smin = std::min(smin, reqs[i].smax);
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        if (ext) {
+            smax[size_t(i)] = reqs[i].smax;
+            trace[size_t(i)] = reqs[i].with_cigar != 0;
+        } else {
+            smax[size_t(i)] = smin;
+            trace[size_t(i)] = any_cigar;
+        }
+    }
+    WfaOut out;
+    const char* err = run_wfa(reqs, n, smax, trace, arena, out);
+    if (err) {
+        if (error) *error = err;
+        reset_wfa(res, n);
+        return GA_DEVICE_ERROR;
+    }
+    parallel_for(n, [&](int64_t i) {
+        const ga_align_req& r = reqs[i];
+        ga_align_res& o = res[i];
+        o.score = out.score[size_t(i)];
+        o.resolved = o.score >= 0;
+        const int len = out.cig_len[size_t(i)];
+        const int slot_len = std::max(r.pattern_len, 0) + std::max(r.text_len, 0);
+        o.cigar_off = out.slot_off[size_t(i)] + slot_len - len;
+        if (o.resolved && trace[size_t(i)] && len > 0) {
+            o.cigar_len = len;
+            wfa_validate(arena + o.cigar_off, len, o.score, r.pattern, r.pattern_len, r.text,
+                         r.text_len, &o.rescore_ok, &o.wellformed_ok);
+        } else {
+            o.cigar_len = 0;
+            o.cigar_off = out.slot_off[size_t(i)];
+        }
+    });
+    int32_t rc = 0;
+    for (int64_t i = 0; i < n; ++i) rc += res[i].resolved ? 1 : 0;
+    if (resolved_count) *resolved_count = rc;
+    if (unresolved_count) *unresolved_count = int32_t(n) - rc;
+    return GA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// SW batch on the device
+// ---------------------------------------------------------------------------
+constexpr int64_t kSwDirBudget = int64_t(16) << 30;     // direction nibbles per chunk
+constexpr int kLdsBndMaxRows = 8192;                     // 2 ints per row in LDS
+
+const char* sw_scheme_error(const ga_sw_scoring& s, bool with_cigar)
+{
+    if (s.match <= 0) return "SW scoring: match must be > 0 (the score is 0 for every input otherwise)";
+    if (s.mismatch >= s.match) return "SW scoring: mismatch must be < match";
+    if (s.gap_open <= 0 || s.gap_extend <= 0) return "SW scoring: gap costs must be > 0";
+    if (with_cigar && s.gap_extend > s.gap_open)
+        return "SW scoring: gap_extend > gap_open with with_cigar cannot re-score "
+               "to its own score (api.hpp; RESULTADO_H9_SW2_TRACE.md). Score-only "
+               "requests may use it";
+    return nullptr;
+}
+
+// Device SW over pairs given as pointers; res8 receives 8 ints per pair (see
+// ga::SwLaunch).  walk[i] selects pairs whose span + CIGAR are produced.
+const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>& skip,
+                   bool trace, const std::vector<uint8_t>& walk, char* arena,
+                   std::vector<int32_t>& res8, std::vector<int64_t>& slot_off)
+{
+    std::call_once(g_pool_once, init_pool);
+    Packed T, P;
+    T.begin(); P.begin();
+    std::vector<int32_t> tl(static_cast<size_t>(n)), pl(static_cast<size_t>(n));
+    std::vector<int64_t> dir_off(static_cast<size_t>(n));
+    slot_off.resize(static_cast<size_t>(n));
+    int64_t slot = 0, dir_words = 0;
+    int32_t max_m = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const ga_sw_req& r = reqs[i];
+        const bool sk = skip[size_t(i)] != 0;
+        tl[size_t(i)] = sk ? 0 : r.text_len;
+        pl[size_t(i)] = sk ? 0 : r.pattern_len;
+        T.add(r.text, sk ? 0 : r.text_len);
+        P.add(r.pattern, sk ? 0 : r.pattern_len);
+        slot_off[size_t(i)] = slot;
+        slot += std::max(r.text_len, 0) + std::max(r.pattern_len, 0);
+        dir_off[size_t(i)] = dir_words;
+        if (trace) dir_words += ga::sw_dir_words(tl[size_t(i)], pl[size_t(i)]);
+        max_m = std::max(max_m, tl[size_t(i)]);
+    }
+    T.end(); P.end();
+
+    hipStream_t st = acquire_stream();
+    if (!st) return "hipStreamCreate";
+    DevArena D;
+    D.st = st;
+    const char* err = nullptr;
+    auto* d_txt = D.upload(T.bytes);
+    auto* d_pat = D.upload(P.bytes);
+    auto* d_toff = D.upload(T.off);
+    auto* d_poff = D.upload(P.off);
+    auto* d_tl = D.upload(tl);
+    auto* d_pl = D.upload(pl);
+    auto* d_res = D.alloc<int32_t>(size_t(n) * 8);
+    uint8_t* d_cig = nullptr;
+    int64_t* d_slot = nullptr;
+    int64_t* d_diroff = nullptr;
+    uint8_t* d_walk = nullptr;
+    uint32_t* d_dirs = nullptr;
+    // chunking by the direction-table budget; dir offsets restart per chunk
+    std::vector<std::pair<int64_t, int64_t>> chunks;
+    if (trace) {
+        d_cig = D.alloc<uint8_t>(size_t(slot));
+        d_slot = D.upload(slot_off);
+        d_walk = D.upload(walk);
+        int64_t a = 0, words = 0, max_words = 0;
+        std::vector<int64_t> local(static_cast<size_t>(n));
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t w = ga::sw_dir_words(tl[size_t(i)], pl[size_t(i)]);
+            if (i > a && (words + w) * 4 > kSwDirBudget) {
+                chunks.emplace_back(a, i);
+                max_words = std::max(max_words, words);
+                a = i; words = 0;
+            }
+            local[size_t(i)] = words;
+            words += w;
+        }
+        chunks.emplace_back(a, n);
+        max_words = std::max(max_words, words);
+        d_diroff = D.upload(local);
+        d_dirs = D.alloc<uint32_t>(size_t(max_words));
+    } else {
+        chunks.emplace_back(0, n);
+    }
+    const bool lds_bnd = max_m <= kLdsBndMaxRows;
+    int32_t* d_bnd = nullptr;
+    const int64_t bnd_chunk = int64_t(1) << 16;
+    if (!lds_bnd) d_bnd = D.alloc<int32_t>(size_t(std::min<int64_t>(n, bnd_chunk)) * 2 * size_t(max_m));
+    if (!D.ok) err = "hipMalloc(dir table) -- batch workspace exceeds device memory; split the batch";
+
+    const ga_sw_scoring sc = reqs[0].scoring;
+    for (auto& ch : chunks) {
+        if (err) break;
+        for (int64_t a = ch.first; a < ch.second && !err;) {
+            const int64_t len = std::min(ch.second - a, lds_bnd ? (int64_t(1) << 30) : bnd_chunk);
+            ga::SwLaunch L{};
+            L.txt = d_txt; L.toff = d_toff; L.tlen = d_tl;
+            L.pat = d_pat; L.poff = d_poff; L.plen = d_pl;
+            L.n = int32_t(len); L.max_m = max_m;
+            L.match = sc.match; L.mismatch = sc.mismatch; L.go = sc.gap_open; L.ge = sc.gap_extend;
+            L.res = d_res; L.dirs = d_dirs; L.dir_off = d_diroff; L.walk = d_walk;
+            L.cigar = d_cig; L.cig_off = d_slot; L.bnd = d_bnd; L.first_pair = int32_t(a);
+            if (ga::sw_launch(L, trace, st) != hipSuccess) err = "SW kernel launch";
+            a += len;
+        }
+    }
+    res8.assign(size_t(n) * 8, -1);
+    if (!err) {
+        if (hipMemcpyAsync(res8.data(), d_res, size_t(n) * 32, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            (trace && slot > 0 &&
+             hipMemcpyAsync(arena, d_cig, size_t(slot), hipMemcpyDeviceToHost, st) != hipSuccess))
+            err = "hipMemcpy(results)";
+    }
+    D.release();
+    const hipError_t se = hipStreamSynchronize(st);
+    if (!err && se != hipSuccess) err = hipGetErrorString(se);
+    release_stream(st);
+    return err;
+}
+
+void reset_sw(ga_sw_res* res, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) res[i] = ga_sw_res{-1, 0, 0, 0, -1, -1, -1, -1, 0, 0, 0};
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t ga_cigar_arena_bytes_wfa(const ga_align_req* reqs, int64_t n)
+{
+    int64_t s = 0;
+    for (int64_t i = 0; i < n; ++i) s += std::max(reqs[i].pattern_len, 0) + std::max(reqs[i].text_len, 0);
+    return s;
+}
+
+int64_t ga_cigar_arena_bytes_sw(const ga_sw_req* reqs, int64_t n)
+{
+    int64_t s = 0;
+    for (int64_t i = 0; i < n; ++i) s += std::max(reqs[i].pattern_len, 0) + std::max(reqs[i].text_len, 0);
+    return s;
+}
+
+int ga_align_batch(const ga_align_req* reqs, int64_t n, ga_align_res* res, char* arena,
+                   int64_t arena_bytes, int32_t* resolved_count, int32_t* unresolved_count,
+                   const char** error)
+{
+    return wfa_common(reqs, n, res, arena, arena_bytes, resolved_count, unresolved_count, error, false);
+}
+
+int ga_align_batch_ext(const ga_align_req* reqs, int64_t n, ga_align_res* res, char* arena,
+                       int64_t arena_bytes, int32_t* resolved_count, int32_t* unresolved_count,
+                       const char** error)
+{
+    return wfa_common(reqs, n, res, arena, arena_bytes, resolved_count, unresolved_count, error, true);
+}
+
+int ga_align_sw_batch(const ga_sw_req* reqs, int64_t n, ga_sw_res* res, char* arena,
+                      int64_t arena_bytes, int32_t* resolved_count, int32_t* unresolved_count,
+                      const char** error)
+{
+    if (error) *error = nullptr;
+    if (resolved_count) *resolved_count = 0;
+    if (unresolved_count) *unresolved_count = 0;
+    if (n <= 0) return GA_OK;
+    reset_sw(res, n);
+    auto refuse = [&](const char* msg) {
+        if (error) *error = msg;
+        reset_sw(res, n);
+        return GA_INVALID_ARGUMENT;
+    };
+    if (!reqs || !res) return refuse("null request or result array");
+    const ga_sw_scoring s0 = reqs[0].scoring;
+    for (int64_t i = 0; i < n; ++i) {
+        const ga_sw_req& r = reqs[i];
+        if ((r.text_len > 0 && r.text == nullptr) || (r.pattern_len > 0 && r.pattern == nullptr))
+            return refuse("null pointer with non-zero length");
+        if (r.text_len < 0 || r.pattern_len < 0) return refuse("negative length");
+        if (const char* e = sw_scheme_error(r.scoring, r.with_cigar != 0)) return refuse(e);
+        if (r.scoring.match != s0.match || r.scoring.mismatch != s0.mismatch ||
+            r.scoring.gap_open != s0.gap_open || r.scoring.gap_extend != s0.gap_extend)
+            return refuse("all requests in an SW batch must share one scoring scheme");
+    }
+    bool any_cigar = false;
+    int64_t max_cells = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        any_cigar |= reqs[i].with_cigar != 0;
+        max_cells = std::max(max_cells, (int64_t(reqs[i].text_len) + 1) * (int64_t(reqs[i].pattern_len) + 1));
+    }
+    const int64_t need = ga_cigar_arena_bytes_sw(reqs, n);
+    if (any_cigar && need > 0 && (arena == nullptr || arena_bytes < need))
+        return refuse("CIGAR arena smaller than the batch needs (ga_cigar_arena_bytes_sw)");
+    // api.cpp:644-645 and sw_kernel_impl.hip:276-280: a pair whose matrix exceeds
+    // the batch's direction stride is too_large.
+    const int64_t dir_stride = std::min<int64_t>(max_cells, GA_SW_MAX_TRACE_CELLS);
+    std::vector<uint8_t> skip(static_cast<size_t>(n), 0), walk(static_cast<size_t>(n), 0);
+    for (int64_t i = 0; i < n; ++i) {
+        const ga_sw_req& r = reqs[i];
+        if (any_cigar && r.text_len > 0 && r.pattern_len > 0 &&
+            (int64_t(r.text_len) + 1) * (int64_t(r.pattern_len) + 1) > dir_stride)
+            skip[size_t(i)] = 1;
+        walk[size_t(i)] = r.with_cigar != 0;
+    }
+    std::vector<int32_t> r8;
+    std::vector<int64_t> slot_off;
+    const char* err = run_sw(reqs, n, skip, any_cigar, walk, arena, r8, slot_off);
+    if (err) {
+        if (error) *error = err;
+        reset_sw(res, n);
+        return GA_DEVICE_ERROR;
+    }
+    parallel_for(n, [&](int64_t i) {
+        const ga_sw_req& r = reqs[i];
+        ga_sw_res& o = res[i];
+        const int32_t* q = &r8[size_t(i) * 8];
+        o.cigar_off = slot_off[size_t(i)];
+        if (skip[size_t(i)]) { o.too_large = 1; return; }
+        if (q[0] < 0) return;
+        o.score = q[0];
+        o.resolved = 1;
+        if (!any_cigar) {                         // score-only batch (api.cpp:829-838)
+            o.end_i = q[3]; o.end_j = q[4];
+            if (o.score == 0) o.rescore_ok = o.wellformed_ok = 1;
+            return;
+        }
+        o.end_i = q[1]; o.end_j = q[2];           // trace fill (api.cpp:542-580)
+        if (o.score == 0) { o.rescore_ok = o.wellformed_ok = 1; return; }
+        if (!r.with_cigar) return;
+        o.start_i = q[5]; o.start_j = q[6];
+        const int len = q[7];
+        if (len > 0) {
+            const int slot_len = std::max(r.text_len, 0) + std::max(r.pattern_len, 0);
+            o.cigar_off = slot_off[size_t(i)] + slot_len - len;
+            o.cigar_len = len;
+            sw_validate(arena + o.cigar_off, len, r.text, r.text_len, o.start_i, o.end_i,
+                        r.pattern, r.pattern_len, o.start_j, o.end_j, r.scoring, o.score,
+                        &o.rescore_ok, &o.wellformed_ok);
+        }
+    });
+    int32_t rc = 0;
+    for (int64_t i = 0; i < n; ++i) rc += res[i].resolved ? 1 : 0;
+    if (resolved_count) *resolved_count = rc;
+    if (unresolved_count) *unresolved_count = int32_t(n) - rc;
+    return GA_OK;
+}
+
+void ga_set_kernel_timing(int enabled) { t_timer.enabled = enabled != 0; }
+
+float ga_last_kernel_ms(void) { return t_timer.total_ms(); }
+
+const char* ga_backend_name(void) { return "rocm"; }
+
+int ga_device_available(void)
+{
+    int n = 0;
+    return hipGetDeviceCount(&n) == hipSuccess && n > 0;
+}
+
+const char* ga_device_name(void)
+{
+    static const std::string name = [] {
+        hipDeviceProp_t p{};
+        if (hipGetDeviceProperties(&p, 0) != hipSuccess) return std::string("none");
+        return std::string(p.name);
+    }();
+    return name.c_str();
+}
+
+// ---- device-resident batches ------------------------------------------------
+int ga_wfa_device(const ga_wfa_dev_batch* b, void* stream, const char** error)
+{
+    if (error) *error = nullptr;
+    if (!b) { if (error) *error = "null batch"; return GA_INVALID_ARGUMENT; }
+    if (b->n_pairs <= 0) return GA_OK;
+    if (b->smax_max < 0 || b->smax_max > GA_MAX_SMAX_EXT) {
+        if (error) *error = "smax_max out of range [0, 4000]";
+        return GA_INVALID_ARGUMENT;
+    }
+    if (b->n_pairs > INT32_MAX) { if (error) *error = "too many pairs"; return GA_INVALID_ARGUMENT; }
+    std::call_once(g_pool_once, init_pool);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const bool trace = b->cigar != nullptr;
+    const int64_t ws_stride = int64_t(b->smax_max + 1) * int64_t(b->smax_max + 1);
+    int64_t chunk = trace ? std::max<int64_t>(1, kWfaWsBudget / (ws_stride * 4)) : b->n_pairs;
+    chunk = std::min(chunk, b->n_pairs);
+    int32_t* ws = nullptr;
+    if (trace && hipMallocAsync(reinterpret_cast<void**>(&ws), size_t(chunk * ws_stride) * 4, st) != hipSuccess) {
+        if (error) *error = "hipMalloc(wavefront stack)";
+        return GA_DEVICE_ERROR;
+    }
+    int rc = GA_OK;
+    t_timer.reset();
+    for (int64_t a = 0; a < b->n_pairs; a += chunk) {
+        ga::WfaLaunch L{};
+        L.pat = b->pattern; L.poff = b->pattern_off; L.plen = b->pattern_len;
+        L.txt = b->text; L.toff = b->text_off; L.tlen = b->text_len;
+        L.smax = b->smax; L.n = int32_t(std::min(chunk, b->n_pairs - a)); L.smax_max = b->smax_max;
+        L.score = b->score; L.cigar = b->cigar; L.cig_off = b->cigar_off; L.cig_len = b->cigar_len;
+        L.ws = ws; L.ws_stride = ws_stride; L.first_pair = int32_t(a);
+        t_timer.begin(st);
+        const hipError_t e = ga::wfa_launch(L, trace, st);
+        t_timer.end(st);
+        if (e != hipSuccess) {
+            if (error) *error = "WFA kernel launch";
+            rc = GA_DEVICE_ERROR;
+            break;
+        }
+    }
+    if (ws) (void)hipFreeAsync(ws, st);
+    return rc;
+}
+
+int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)
+{
+    if (error) *error = nullptr;
+    if (!b) { if (error) *error = "null batch"; return GA_INVALID_ARGUMENT; }
+    if (b->n_pairs <= 0) return GA_OK;
+    if (b->n_pairs > INT32_MAX) { if (error) *error = "too many pairs"; return GA_INVALID_ARGUMENT; }
+    if (const char* e = sw_scheme_error(b->scoring, b->with_cigar != 0)) {
+        if (error) *error = e;
+        return GA_INVALID_ARGUMENT;
+    }
+    const bool trace = b->with_cigar != 0;
+    if (!b->out || (trace && (!b->cigar || !b->cigar_off))) {
+        if (error) *error = "null output buffer";
+        return GA_INVALID_ARGUMENT;
+    }
+    std::call_once(g_pool_once, init_pool);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // Every pair gets a direction table of the batch's largest shape.
+    const int64_t per = ga::sw_dir_words(b->max_text_len, b->max_pattern_len);
+    const bool lds_bnd = b->max_text_len <= kLdsBndMaxRows;
+    int64_t chunk = b->n_pairs;
+    if (trace) chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, kSwDirBudget / std::max<int64_t>(per * 4, 1)));
+    if (!lds_bnd) chunk = std::min<int64_t>(chunk, int64_t(1) << 16);
+    std::vector<void*> owned;
+    auto dalloc = [&](void** p, size_t bytes) {
+        if (hipMallocAsync(p, std::max<size_t>(bytes, 64), st) != hipSuccess) return false;
+        owned.push_back(*p);
+        return true;
+    };
+    int32_t* d_res = nullptr;
+    uint32_t* d_dirs = nullptr;
+    int32_t* d_bnd = nullptr;
+    bool ok = dalloc(reinterpret_cast<void**>(&d_res), size_t(b->n_pairs) * 32);
+    if (ok && trace) ok = dalloc(reinterpret_cast<void**>(&d_dirs), size_t(chunk * per) * 4);
+    if (ok && !lds_bnd)
+        ok = dalloc(reinterpret_cast<void**>(&d_bnd), size_t(chunk) * 2 * size_t(b->max_text_len) * 4);
+    int rc = GA_OK;
+    if (!ok) { if (error) *error = "hipMalloc(SW workspace)"; rc = GA_DEVICE_ERROR; }
+    t_timer.reset();
+    for (int64_t a = 0; a < b->n_pairs && rc == GA_OK; a += chunk) {
+        ga::SwLaunch L{};
+        L.txt = b->text; L.toff = b->text_off; L.tlen = b->text_len;
+        L.pat = b->pattern; L.poff = b->pattern_off; L.plen = b->pattern_len;
+        L.n = int32_t(std::min(chunk, b->n_pairs - a)); L.max_m = b->max_text_len;
+        L.match = b->scoring.match; L.mismatch = b->scoring.mismatch;
+        L.go = b->scoring.gap_open; L.ge = b->scoring.gap_extend;
+        L.res = d_res; L.dirs = d_dirs; L.dir_off = nullptr; L.dir_stride = per;
+        L.walk = nullptr; L.cigar = b->cigar; L.cig_off = b->cigar_off; L.bnd = d_bnd;
+        L.first_pair = int32_t(a);
+        if (ga::sw_launch(L, trace, st, &t_timer_hooks) != hipSuccess) {
+            if (error) *error = "SW kernel launch";
+            rc = GA_DEVICE_ERROR;
+        }
+    }
+    if (rc == GA_OK && ga::sw_gather(d_res, b->n_pairs, trace, b->out, b->cigar_len, st) != hipSuccess) {
+        if (error) *error = "SW gather launch";
+        rc = GA_DEVICE_ERROR;
+    }
+    for (void* p : owned) (void)hipFreeAsync(p, st);
+    return rc;
+}
+
+}  // extern "C"

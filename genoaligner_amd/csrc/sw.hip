@@ -1,0 +1,252 @@
+// sw.hip -- gfx950 kernels for local alignment with affine gaps (Gotoh
+// Smith-Waterman): score, end coordinate, and (trace variant) a 4-bit direction
+// code per cell in HBM that a walk kernel turns into span + CIGAR.
+//
+// Recurrence and tie rules (reference: include/genoaligner/backend/
+// sw_kernel_impl.hip:292-322, sw_kernel.hip:76-91), text = row axis i,
+// pattern = column axis j, both 1-based in the matrix:
+//   E[i][j] = max(H[i][j-1] - go, E[i][j-1] - ge)   extend only if strictly greater
+//   F[i][j] = max(H[i-1][j] - go, F[i-1][j] - ge)   extend only if strictly greater
+//   H[i][j] = diag + s, then E if E > H, then F if F > H; H <= 0 -> 0 (STOP)
+//   boundaries H = E = F = 0 on row 0 and column 0 (:286-296)
+//   best = first strict maximum in row-major order (:320)
+// The direction code is the reference's byte (hsrc | EEXT 4 | FEXT 8) kept in
+// a nibble, and the walk is the reference's three-state walk (:198-242).
+//
+// Mapping to CDNA4: one wave64 per pair.  The pattern is cut into strips of 64
+// columns; in a strip lane l owns column j = 64*strip + l + 1 and the text is
+// streamed through the wave with a one-step skew per lane, so at step t lane l
+// computes row i = t - l + 1.  Everything a cell needs from its left neighbour
+// (H and E of the same row, and H of the row above) is what lane l-1 held one
+// and two steps earlier, moved by one cross-lane shift per step; F and H of the
+// row above stay in the lane's own registers.  Lane 0 takes its left column
+// from a boundary buffer that lane 63 of the previous strip filled (in LDS when
+// it fits).  Each lane packs 8 direction nibbles per 32-bit word and stores one
+// word every 8 steps; a wave's 64 words form one contiguous 256-byte store.
+//
+// End coordinates.  Each lane keeps the first strict maximum of its own columns
+// in (row, strip) order.  That per-lane record is exactly the reference score
+// kernel's per-lane record at warp width 64 (its lane (j-1) % 64 owns the same
+// columns), so the score-only end follows its xor-butterfly argmax
+// (sw_kernel_impl.hip:146,161-167); the trace end is the row-major minimum over
+// lanes holding the maximum.
+#include "ga_device.hpp"
+
+namespace ga {
+namespace {
+
+__device__ __forceinline__ uint64_t load8(const uint8_t* p)
+{
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    const uint64_t* w = reinterpret_cast<const uint64_t*>(a & ~uintptr_t(7));
+    const unsigned sh = unsigned(a & 7u) * 8u;
+    const uint64_t lo = w[0];
+    const uint64_t hi = w[1];
+    return sh ? ((lo >> sh) | (hi << (64u - sh))) : lo;
+}
+
+template <bool TRACE, bool LDS_BND>
+__global__ __launch_bounds__(64) void sw_kernel(SwLaunch L)
+{
+    extern __shared__ int lds[];
+    const int slot = blockIdx.x;
+    const int pid = L.first_pair + slot;
+    const int lane = threadIdx.x;
+    const int m = L.tlen[pid];
+    const int n = L.plen[pid];
+    int32_t* res = L.res + int64_t(pid) * 8;
+    if (m <= 0 || n <= 0) {
+        if (lane == 0) {
+            res[0] = 0;
+            for (int q = 1; q < 7; ++q) res[q] = -1;
+            res[7] = 0;
+        }
+        return;
+    }
+    const uint8_t* T = L.txt + L.toff[pid];
+    const uint8_t* P = L.pat + L.poff[pid];
+    int* bH = LDS_BND ? lds : L.bnd + int64_t(slot) * 2 * L.max_m;
+    int* bE = bH + L.max_m;
+    for (int q = lane; q < m; q += 64) { bH[q] = 0; bE[q] = 0; }
+    __syncthreads();
+
+    const int go = L.go, ge = L.ge, sm = L.match, sx = L.mismatch;
+    const int nstrips = (n + 63) >> 6;
+    const int nchunks = (m + 63 + 7) >> 3;          // 8-step chunks per strip
+    uint32_t* dirs = nullptr;
+    if (TRACE) dirs = L.dirs + (L.dir_off ? L.dir_off[pid] : int64_t(slot) * L.dir_stride);
+    int gH = 0, gI = -1, gJ = -1;
+
+    for (int s = 0; s < nstrips; ++s) {
+        const int j = (s << 6) + lane;               // 0-based column of this lane
+        const bool jvalid = j < n;
+        const uint8_t pc = jvalid ? P[j] : uint8_t(0);
+        int h = 0, e = 0, f = 0;                     // this lane's last row
+        int hl_prev = 0;                             // H[i-1][j-1]
+        int bHv = 0, bIv = -1;
+        // text bytes for rows t - lane .. t - lane + 7, prefetched a chunk ahead
+        uint64_t tcur = load8(T - lane);
+        for (int c = 0; c < nchunks; ++c) {
+            const int t0 = c << 3;
+            const uint64_t tnext = load8(T + t0 + 8 - lane);
+            uint32_t acc = 0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = t0 + u - lane;         // 0-based row of this step
+                // Left neighbour's H and E, one step old, by a whole-wave shift
+                // right (DPP wave_shr:1); lane 0 receives the strip boundary
+                // (row t0+u), read at a wave-uniform LDS address.
+                const int i0 = t0 + u;
+                const int bh = i0 < m ? bH[i0] : 0;
+                const int be = i0 < m ? bE[i0] : 0;
+                const int hl = __builtin_amdgcn_update_dpp(bh, h, 0x138, 0xf, 0xf, false);
+                const int el = __builtin_amdgcn_update_dpp(be, e, 0x138, 0xf, 0xf, false);
+                const int diag = hl_prev;
+                hl_prev = hl;
+                const bool active = (i >= 0) && (i < m) && jvalid;
+                if (active) {
+                    const uint8_t tc = uint8_t(tcur >> (8 * u));
+                    int ee = hl - go;
+                    uint32_t code = 0;
+                    { const int e2 = el - ge; if (e2 > ee) { ee = e2; code |= 4u; } }
+                    int ff = h - go;
+                    { const int f2 = f - ge; if (f2 > ff) { ff = f2; code |= 8u; } }
+                    int hv = diag + (tc == pc ? sm : sx);
+                    uint32_t hsrc = 1u;
+                    if (ee > hv) { hv = ee; hsrc = 2u; }
+                    if (ff > hv) { hv = ff; hsrc = 3u; }
+                    if (hv <= 0) { hv = 0; hsrc = 0u; }
+                    h = hv; e = ee; f = ff;
+                    if (hv > bHv) { bHv = hv; bIv = i; }
+                    if (TRACE) acc |= (code | hsrc) << (4 * u);
+                    if (lane == 63) { bH[i] = hv; bE[i] = ee; }
+                }
+            }
+            if (TRACE) dirs[(int64_t(s) * nchunks + c) * 64 + lane] = acc;
+            tcur = tnext;
+        }
+        if (bHv > gH || (bHv == gH && bHv > 0 && bIv < gI)) { gH = bHv; gI = bIv; gJ = j; }
+        __syncthreads();
+    }
+
+    // Score-only end: the reference's xor-butterfly argmax at width 64.
+    int lH = gH, lI = gI, lJ = gJ;
+    // Trace end: the row-major first cell among lanes holding the maximum.
+    int rH = gH, rI = gI, rJ = gJ;
+    for (int off = 32; off > 0; off >>= 1) {
+        const int ob = __shfl_xor(lH, off), oi = __shfl_xor(lI, off), oj = __shfl_xor(lJ, off);
+        if (ob > lH) { lH = ob; lI = oi; lJ = oj; }
+        const int rb = __shfl_xor(rH, off), ri = __shfl_xor(rI, off), rj = __shfl_xor(rJ, off);
+        if (rb > rH || (rb == rH && rb > 0 && (ri < rI || (ri == rI && rj < rJ)))) {
+            rH = rb; rI = ri; rJ = rj;
+        }
+    }
+    if (lane == 0) {
+        res[0] = lH;          // lH == rH: both are the maximum
+        res[1] = rI; res[2] = rJ;
+        res[3] = lI; res[4] = lJ;
+        res[5] = -1; res[6] = -1; res[7] = 0;
+    }
+}
+
+// One thread per pair: the reference's walk over the nibble table, writing the
+// CIGAR right-aligned into the pair's slot of text_len + pattern_len bytes.
+__global__ __launch_bounds__(64) void sw_walk_kernel(SwLaunch L)
+{
+    const int q = int(blockIdx.x) * 64 + int(threadIdx.x);
+    if (q >= L.n) return;
+    const int pid = L.first_pair + q;
+    int32_t* res = L.res + int64_t(pid) * 8;
+    const int score = res[0];
+    if (score <= 0 || (L.walk && !L.walk[pid])) return;
+    const int m = L.tlen[pid], n = L.plen[pid];
+    const uint8_t* T = L.txt + L.toff[pid];
+    const uint8_t* P = L.pat + L.poff[pid];
+    const uint32_t* dirs = L.dirs + (L.dir_off ? L.dir_off[pid] : int64_t(q) * L.dir_stride);
+    const int nchunks = (m + 63 + 7) >> 3;
+    uint8_t* cg = L.cigar + L.cig_off[pid];
+    int pos = m + n;
+    int i = res[1] + 1, j = res[2] + 1, state = 0;
+    while (i > 0 && j > 0 && pos > 0) {
+        const int j0 = j - 1, sidx = j0 >> 6, l = j0 & 63, t = (i - 1) + l;
+        const uint32_t w = dirs[(int64_t(sidx) * nchunks + (t >> 3)) * 64 + l];
+        const uint32_t b = (w >> (4 * (t & 7))) & 15u;
+        if (state == 0) {
+            const uint32_t hsrc = b & 3u;
+            if (hsrc == 0u) break;
+            if (hsrc == 1u) {
+                cg[--pos] = (T[i - 1] == P[j - 1]) ? 'M' : 'X';
+                --i; --j;
+            } else if (hsrc == 2u) {
+                cg[--pos] = 'D';
+                state = (b & 4u) ? 1 : 0;
+                --j;
+            } else {
+                cg[--pos] = 'I';
+                state = (b & 8u) ? 2 : 0;
+                --i;
+            }
+        } else if (state == 1) {
+            cg[--pos] = 'D';
+            --j;
+            if (!(b & 4u)) state = 0;
+        } else {
+            cg[--pos] = 'I';
+            --i;
+            if (!(b & 8u)) state = 0;
+        }
+    }
+    res[5] = i;
+    res[6] = j;
+    res[7] = m + n - pos;
+}
+
+__global__ __launch_bounds__(256) void sw_gather_kernel(const int32_t* res8, int64_t n, int trace,
+                                                       int32_t* out5, int32_t* cig_len)
+{
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int32_t* q = res8 + i * 8;
+    int32_t* o = out5 + i * 5;
+    o[0] = q[0];
+    if (trace) { o[1] = q[5]; o[2] = q[6]; o[3] = q[1]; o[4] = q[2]; }
+    else       { o[1] = -1;   o[2] = -1;   o[3] = q[3]; o[4] = q[4]; }
+    if (cig_len) cig_len[i] = trace ? q[7] : 0;
+}
+
+}  // namespace
+
+hipError_t sw_gather(const int32_t* res8, int64_t n, bool trace, int32_t* out5, int32_t* cig_len,
+                     hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sw_gather_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, res8, n,
+                       int(trace), out5, cig_len);
+    return hipGetLastError();
+}
+
+hipError_t sw_launch(const SwLaunch& L, bool trace, hipStream_t st, LaunchHooks* hooks)
+{
+    if (L.n <= 0) return hipSuccess;
+    const size_t bnd_bytes = size_t(2) * size_t(L.max_m) * sizeof(int);
+    const bool in_lds = L.bnd == nullptr;
+    const size_t lds = in_lds ? bnd_bytes : 0;
+    const dim3 grid(unsigned(L.n)), block(64);
+    if (hooks) hooks->before(st);
+    if (trace) {
+        if (in_lds) hipLaunchKernelGGL((sw_kernel<true, true>), grid, block, lds, st, L);
+        else        hipLaunchKernelGGL((sw_kernel<true, false>), grid, block, lds, st, L);
+        if (hooks) hooks->after(st);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        const dim3 wgrid(unsigned((L.n + 63) / 64));
+        hipLaunchKernelGGL(sw_walk_kernel, wgrid, block, 0, st, L);
+    } else {
+        if (in_lds) hipLaunchKernelGGL((sw_kernel<false, true>), grid, block, lds, st, L);
+        else        hipLaunchKernelGGL((sw_kernel<false, false>), grid, block, lds, st, L);
+        if (hooks) hooks->after(st);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace ga

@@ -1,0 +1,149 @@
+"""Seeded synthetic workloads for the pairwise path (vectorised numpy).
+
+Each generator returns a ``PairSet``: two packed byte arrays with int64 offsets
+and int32 lengths, plus per-pair metadata.  All sequences are uppercase ACGT
+bytes (uint8).
+
+Workloads (sizes as used by the benchmark and tests):
+  wfa_plain   n pairs; pattern L bp i.i.d. uniform ACGT; text = pattern with
+              per-base edit probability p, sub:ins:del = 2:1:1 (a substitution
+              picks one of the 3 other bases, an insertion adds a uniform base
+              after the current one); smax per pair.
+  sw_embedded n pairs; query (pattern, columns) Lq bp i.i.d.; target (text,
+              rows) Lt bp i.i.d. with a copy of the query mutated at rate p
+              (2:1:1) written over it at a uniform offset, so every target is
+              exactly Lt bp.
+  wfa_mixed   n pairs; length log-uniform in [lo, hi]; divergence uniform in
+              [0, dmax] per pair (2:1:1); smax = min(0.2 * len, cap).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+ACGT = np.frombuffer(b"ACGT", dtype=np.uint8)
+
+
+@dataclass
+class PairSet:
+    a: np.ndarray          # packed bytes, first sequence of each pair (WFA: pattern; SW: text)
+    a_off: np.ndarray      # int64
+    a_len: np.ndarray      # int32
+    b: np.ndarray          # packed bytes, second sequence (WFA: text; SW: pattern)
+    b_off: np.ndarray
+    b_len: np.ndarray
+    smax: Optional[np.ndarray] = None      # int32 per pair (WFA)
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.a_len.shape[0])
+
+    def a_seq(self, i: int) -> bytes:
+        o = int(self.a_off[i])
+        return self.a[o: o + int(self.a_len[i])].tobytes()
+
+    def b_seq(self, i: int) -> bytes:
+        o = int(self.b_off[i])
+        return self.b[o: o + int(self.b_len[i])].tobytes()
+
+    def subset(self, idx) -> "PairSet":
+        idx = np.asarray(idx)
+        a, ao, al = _gather(self.a, self.a_off, self.a_len, idx)
+        b, bo, bl = _gather(self.b, self.b_off, self.b_len, idx)
+        sm = None if self.smax is None else self.smax[idx].copy()
+        return PairSet(a, ao, al, b, bo, bl, sm, dict(self.meta))
+
+
+def _gather(buf, off, ln, idx):
+    lens = ln[idx].astype(np.int64)
+    new_off = np.zeros(len(idx), dtype=np.int64)
+    if len(idx) > 1:
+        new_off[1:] = np.cumsum(lens)[:-1]
+    total = int(lens.sum())
+    src = np.repeat(off[idx] - new_off, lens) + np.arange(total, dtype=np.int64)
+    return buf[src].copy(), new_off, ln[idx].astype(np.int32)
+
+
+def _offsets(lens: np.ndarray) -> np.ndarray:
+    off = np.zeros(len(lens), dtype=np.int64)
+    if len(lens) > 1:
+        off[1:] = np.cumsum(lens.astype(np.int64))[:-1]
+    return off
+
+
+def random_bases(rng: np.random.Generator, n: int) -> np.ndarray:
+    return ACGT[rng.integers(0, 4, size=n)]
+
+
+def mutate(rng: np.random.Generator, seq: np.ndarray, lens: np.ndarray, rate: np.ndarray):
+    """Apply per-base edits (sub:ins:del = 2:1:1 at total probability rate[pair])
+    to every sequence of a packed array.  Returns (bytes, offsets, lengths)."""
+    lens = lens.astype(np.int64)
+    total = int(lens.sum())
+    pair = np.repeat(np.arange(len(lens)), lens)
+    r = rng.random(total)
+    p = rate[pair]
+    sub = r < p * 0.5
+    ins = (r >= p * 0.5) & (r < p * 0.75)
+    dele = (r >= p * 0.75) & (r < p)
+    base = seq.copy()
+    # substitution: one of the three other bases
+    codes = np.searchsorted(ACGT, base)
+    shift = rng.integers(1, 4, size=total)
+    base[sub] = ACGT[(codes[sub] + shift[sub]) % 4]
+    count = 1 + ins.astype(np.int64) - dele.astype(np.int64)
+    out_len = np.bincount(pair, weights=count, minlength=len(lens)).astype(np.int64)
+    src = np.repeat(np.arange(total), count)
+    out = base[src]
+    # the second copy of an inserted base becomes a uniform random base
+    first = np.ones(len(src), dtype=bool)
+    first[1:] = src[1:] != src[:-1]
+    dup = ~first
+    out[dup] = random_bases(rng, int(dup.sum()))
+    return out, _offsets(out_len), out_len.astype(np.int32)
+
+
+def wfa_plain(n: int = 10_000, length: int = 150, rate: float = 0.02, smax: int = 64,
+              seed: int = 20261202) -> PairSet:
+    rng = np.random.default_rng(seed)
+    pl = np.full(n, length, dtype=np.int32)
+    pat = random_bases(rng, n * length)
+    txt, toff, tl = mutate(rng, pat, pl, np.full(n, rate))
+    return PairSet(pat, _offsets(pl), pl, txt, toff, tl, np.full(n, smax, dtype=np.int32),
+                   meta={"workload": "wfa_plain", "n": n, "length": length, "rate": rate,
+                         "smax": smax, "seed": seed})
+
+
+def sw_embedded(n: int = 50_000, qlen: int = 256, tlen: int = 1024, rate: float = 0.10,
+                seed: int = 20261203) -> PairSet:
+    rng = np.random.default_rng(seed)
+    ql = np.full(n, qlen, dtype=np.int32)
+    q = random_bases(rng, n * qlen)
+    mut, moff, ml = mutate(rng, q, ql, np.full(n, rate))
+    ml = np.minimum(ml, tlen)
+    t = random_bases(rng, n * tlen).reshape(n, tlen)
+    off = (rng.random(n) * (tlen - ml + 1)).astype(np.int64)
+    rows = np.repeat(np.arange(n), ml)
+    within = np.arange(int(ml.sum())) - np.repeat(_offsets(ml), ml)
+    t[rows, np.repeat(off, ml) + within] = mut[np.repeat(moff, ml) + within]
+    tl = np.full(n, tlen, dtype=np.int32)
+    return PairSet(t.reshape(-1), _offsets(tl), tl, q, _offsets(ql), ql,
+                   meta={"workload": "sw_embedded", "n": n, "qlen": qlen, "tlen": tlen,
+                         "rate": rate, "seed": seed, "offset": off})
+
+
+def wfa_mixed(n: int = 200_000, lo: int = 100, hi: int = 10_000, dmax: float = 0.15,
+              smax_frac: float = 0.2, smax_cap: int = 2048, seed: int = 20261204,
+              div: Optional[np.ndarray] = None) -> PairSet:
+    rng = np.random.default_rng(seed)
+    pl = np.exp(rng.uniform(np.log(lo), np.log(hi), size=n)).astype(np.int32)
+    rate = rng.uniform(0.0, dmax, size=n) if div is None else div
+    pat = random_bases(rng, int(pl.astype(np.int64).sum()))
+    txt, toff, tl = mutate(rng, pat, pl, rate)
+    smax = np.minimum((smax_frac * pl).astype(np.int32), smax_cap).astype(np.int32)
+    return PairSet(pat, _offsets(pl), pl, txt, toff, tl, smax,
+                   meta={"workload": "wfa_mixed", "n": n, "lo": lo, "hi": hi, "dmax": dmax,
+                         "seed": seed, "rate": rate})

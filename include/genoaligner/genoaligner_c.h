@@ -1,0 +1,184 @@
+/*
+ * genoaligner_c.h -- C ABI of the MI355X (gfx950) pairwise aligner.
+ *
+ * Plain C, POD structs, plain pointers and sizes.  Every entry point below
+ * either replaces one function of the reference's public C++ API
+ * (include/genoaligner/api.hpp, implemented in src/api/api.cpp) with the same
+ * argument meaning, result rules and error behaviour, or is an explicitly
+ * marked extension with no reference counterpart.  include/genoaligner/api.hpp
+ * in this repository re-exposes the reference's C++ surface on top of this ABI.
+ *
+ * Conventions shared by all entry points
+ *   - Status codes mirror BatchResult::Status / SWBatchResult::Status
+ *     (api.hpp:158, :306): GA_OK, GA_INVALID_ARGUMENT, GA_DEVICE_ERROR.
+ *     `*error` receives a static string, non-NULL iff the status is not GA_OK.
+ *   - A batch is refused as a whole, before any device work, on any invalid
+ *     request; results are then reset to their defaults.
+ *   - CIGARs are byte strings over {M,X,I,D}; I consumes text, D consumes
+ *     pattern.  They are written into a caller-owned arena: pair i's CIGAR is
+ *     cigar_arena[res[i].cigar_off .. + res[i].cigar_len).  The arena must hold
+ *     at least sum(pattern_len + text_len) bytes over the batch
+ *     (ga_cigar_arena_bytes_wfa/_sw compute it); a smaller arena is refused with
+ *     GA_INVALID_ARGUMENT before any device work.
+ *   - Calls are synchronous and thread-safe for disjoint inputs (api.hpp:91-111):
+ *     each call runs on its own HIP stream; the input bytes must stay alive and
+ *     unmodified until the call returns.
+ */
+#ifndef GENOALIGNER_C_H
+#define GENOALIGNER_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { GA_OK = 0, GA_INVALID_ARGUMENT = 1, GA_DEVICE_ERROR = 2 };
+
+/* api.cpp:227 -- the reference contract's smax bound for ga_align_batch. */
+#define GA_MAX_SMAX 511
+/* Extension bound for ga_align_batch_ext (per-pair smax). */
+#define GA_MAX_SMAX_EXT 4000
+/* api.hpp:256 -- largest SW matrix ((text_len+1)*(pattern_len+1)) traced. */
+#define GA_SW_MAX_TRACE_CELLS (1 << 26)
+
+/* ---- WFA edit distance ---------------------------------------------------- */
+
+/* AlignRequest, api.hpp:81-88 (same fields, same defaults: smax 64, with_cigar 1). */
+typedef struct ga_align_req {
+    const char* pattern;
+    int32_t     pattern_len;
+    const char* text;
+    int32_t     text_len;
+    int32_t     smax;
+    int32_t     with_cigar;
+} ga_align_req;
+
+/* AlignResult, api.hpp:64-76. */
+typedef struct ga_align_res {
+    int32_t score;          /* edit distance; -1 = unresolved (distance > smax) */
+    int32_t resolved;
+    int64_t cigar_off;      /* into the caller's arena */
+    int32_t cigar_len;      /* 0 when unresolved */
+    int32_t rescore_ok;
+    int32_t wellformed_ok;
+} ga_align_res;
+
+/* Replaces genoaligner::align_batch (api.hpp:164, api.cpp:234-497).
+ * One smax for the batch: the minimum over the requests, each of which must lie
+ * in [0, GA_MAX_SMAX].  If any request has with_cigar, every resolved pair gets
+ * a CIGAR and both validators (api.cpp:380-470); otherwise scores only. */
+int ga_align_batch(const ga_align_req* reqs, int64_t n, ga_align_res* res,
+                   char* cigar_arena, int64_t arena_bytes,
+                   int32_t* resolved_count, int32_t* unresolved_count,
+                   const char** error);
+
+/* EXTENSION (no reference counterpart): per-pair smax in [0, GA_MAX_SMAX_EXT]
+ * and per-pair with_cigar (score-only pairs get no CIGAR).  Otherwise the same
+ * per-pair results as ga_align_batch with that pair's smax. */
+int ga_align_batch_ext(const ga_align_req* reqs, int64_t n, ga_align_res* res,
+                       char* cigar_arena, int64_t arena_bytes,
+                       int32_t* resolved_count, int32_t* unresolved_count,
+                       const char** error);
+
+/* ---- Smith-Waterman, affine gaps ------------------------------------------ */
+
+/* SWScoring, api.hpp:246-251 (defaults 2, -3, 5, 2). */
+typedef struct ga_sw_scoring {
+    int32_t match;
+    int32_t mismatch;
+    int32_t gap_open;
+    int32_t gap_extend;
+} ga_sw_scoring;
+
+/* SWRequest, api.hpp:261-268: text is the row axis, pattern the column axis. */
+typedef struct ga_sw_req {
+    const char*   text;
+    int32_t       text_len;
+    const char*   pattern;
+    int32_t       pattern_len;
+    ga_sw_scoring scoring;
+    int32_t       with_cigar;
+} ga_sw_req;
+
+/* SWAlignResult, api.hpp:273-294. */
+typedef struct ga_sw_res {
+    int32_t score;
+    int32_t resolved;
+    int64_t cigar_off;
+    int32_t cigar_len;
+    int32_t start_i, start_j, end_i, end_j;
+    int32_t rescore_ok;
+    int32_t wellformed_ok;
+    int32_t too_large;
+} ga_sw_res;
+
+/* Replaces genoaligner::align_sw_batch (api.hpp:312, api.cpp:584-847). */
+int ga_align_sw_batch(const ga_sw_req* reqs, int64_t n, ga_sw_res* res,
+                      char* cigar_arena, int64_t arena_bytes,
+                      int32_t* resolved_count, int32_t* unresolved_count,
+                      const char** error);
+
+/* ---- device queries (api.hpp:485-487) -------------------------------------- */
+const char* ga_backend_name(void);
+const char* ga_device_name(void);
+int         ga_device_available(void);
+
+/* Bytes of CIGAR arena a batch needs: sum over pairs of pattern_len + text_len
+ * (negative lengths count as 0). */
+int64_t ga_cigar_arena_bytes_wfa(const ga_align_req* reqs, int64_t n);
+int64_t ga_cigar_arena_bytes_sw(const ga_sw_req* reqs, int64_t n);
+
+/* ---- device-resident batches (EXTENSION: no reference counterpart) ---------
+ * For callers whose sequences already sit in device memory (the benchmark,
+ * the request loop).  All pointers are device pointers.  Sequences are packed
+ * byte arrays addressed by int64 offsets; the packed arrays must be readable
+ * for 128 bytes before their first and after their last sequence.  `cigar_off[i]` gives pair i's slot of
+ * pattern_len + text_len bytes in `cigar`; the CIGAR is right-aligned in its
+ * slot: it occupies cigar[cigar_off[i] + slot - cigar_len[i] .. + slot).
+ * `stream` is a hipStream_t (NULL = the null stream).  Work is enqueued
+ * asynchronously; results are valid once the stream is synchronised. */
+typedef struct ga_wfa_dev_batch {
+    const uint8_t* pattern; const int64_t* pattern_off; const int32_t* pattern_len;
+    const uint8_t* text;    const int64_t* text_off;    const int32_t* text_len;
+    const int32_t* smax;            /* per pair */
+    int64_t        n_pairs;
+    int32_t        smax_max;        /* >= every smax[i]; <= GA_MAX_SMAX_EXT */
+    int32_t        max_len;         /* >= every pattern_len and text_len */
+    int32_t*       score;           /* out */
+    uint8_t*       cigar;           /* NULL = score only */
+    const int64_t* cigar_off;
+    int32_t*       cigar_len;       /* out */
+} ga_wfa_dev_batch;
+
+int ga_wfa_device(const ga_wfa_dev_batch* b, void* stream, const char** error);
+
+typedef struct ga_sw_dev_batch {
+    const uint8_t* text;    const int64_t* text_off;    const int32_t* text_len;
+    const uint8_t* pattern; const int64_t* pattern_off; const int32_t* pattern_len;
+    int64_t        n_pairs;
+    int32_t        max_text_len, max_pattern_len;
+    ga_sw_scoring  scoring;
+    int32_t        with_cigar;      /* 1: trace path (row-major end, span, CIGAR) */
+    int32_t*       out;             /* out: 5 ints per pair: score, start_i, start_j, end_i, end_j */
+    uint8_t*       cigar;           /* with_cigar only */
+    const int64_t* cigar_off;
+    int32_t*       cigar_len;       /* out */
+} ga_sw_dev_batch;
+
+int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error);
+
+/* ---- kernel timing (EXTENSION: measurement support) ------------------------
+ * When enabled (per calling thread), ga_wfa_device and ga_sw_device record HIP
+ * events on their stream around their dominant kernel (the WFA kernel; the SW
+ * forward kernel) and ga_last_kernel_ms() returns the summed duration of those
+ * launches in the calling thread's last device call, waiting for it to finish.
+ * Returns a negative value when nothing was recorded. */
+void  ga_set_kernel_timing(int enabled);
+float ga_last_kernel_ms(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GENOALIGNER_C_H */

@@ -1,0 +1,400 @@
+"""Python face of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg may
+import this module.  It loads
+
+* ``oracle/_build/libga_oracle.so`` -- the C restatement (oracle/ga_oracle.c) of
+  the reference's per-pair algorithms, and
+* ``oracle/_ref/libga_ref.so`` (optional) -- the reference's own kernel bodies
+  and API compiled for the CPU through its HIP CPU shim (oracle/build_ref.sh),
+
+and restates the batch rules of the reference API (src/api/api.cpp:234-497 for
+WFA, :584-847 for Smith-Waterman) on top of the per-pair functions, so a test
+can ask "what would the reference return for this whole batch".
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "_build", "libga_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libga_ref.so")
+
+# api.cpp:227 and api.hpp:256
+K_MAX_SMAX = 511
+SW_MAX_TRACE_CELLS = 1 << 26
+
+_lib = None
+_ref = None
+
+
+def build(quiet: bool = True) -> None:
+    """Compile the C restatement (and the reference shim library when the
+    reference tree is present)."""
+    out = subprocess.DEVNULL if quiet else None
+    subprocess.check_call(["make", "-s", "-C", HERE], stdout=out)
+    subprocess.check_call([os.path.join(HERE, "build_ref.sh")], stdout=out, stderr=out)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        L = C.CDLL(ORACLE_SO)
+        cp, i32p = C.c_char_p, C.POINTER(C.c_int)
+        L.ga_oracle_wfa.argtypes = [cp, C.c_int, cp, C.c_int, C.c_int, i32p, C.c_void_p, i32p]
+        L.ga_oracle_wfa_score.argtypes = [cp, C.c_int, cp, C.c_int, C.c_int]
+        L.ga_oracle_levenshtein.argtypes = [cp, C.c_int, cp, C.c_int]
+        L.ga_oracle_wfa_validate.argtypes = [cp, C.c_int, C.c_int, cp, C.c_int, cp, C.c_int, i32p, i32p]
+        L.ga_oracle_sw_table_mm.argtypes = [C.c_int, C.c_int, C.c_void_p]
+        sw_args = [cp, C.c_int, cp, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                   i32p, i32p, i32p, i32p, i32p, C.c_void_p, i32p]
+        L.ga_oracle_sw_trace.argtypes = sw_args
+        L.ga_oracle_sw_value_walk.argtypes = sw_args
+        L.ga_oracle_sw_score_lane.argtypes = [cp, C.c_int, cp, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                              C.c_int, i32p, i32p, i32p]
+        L.ga_oracle_sw_validate.argtypes = [cp, C.c_int, cp, C.c_int, C.c_int, C.c_int,
+                                            cp, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                            C.c_int, C.c_int, C.c_int, i32p, i32p]
+        vp = C.c_void_p
+        L.ga_oracle_wfa_batch.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.c_int64,
+                                          vp, vp, vp, vp, C.c_int]
+        L.ga_oracle_sw_batch.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int64, vp, C.c_int, C.c_int,
+                                         C.c_int, vp, vp, vp, vp, C.c_int]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    """The reference's own code compiled for the CPU (None when not built)."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            return None
+        R = C.CDLL(REF_SO)
+        cp, i32p, vp = C.c_char_p, C.POINTER(C.c_int), C.c_void_p
+        R.ga_ref_wfa_trace.argtypes = [cp, C.c_int, cp, C.c_int, C.c_int, i32p, vp, i32p, i32p, i32p]
+        R.ga_ref_wfa_score_flat.argtypes = [cp, C.c_int, cp, C.c_int, C.c_int]
+        R.ga_ref_edit_distance.argtypes = [cp, C.c_int, cp, C.c_int]
+        R.ga_ref_sw_trace.argtypes = [cp, C.c_int, cp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      vp, vp, i32p]
+        R.ga_ref_sw_score.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.c_int, C.c_int, vp]
+        R.ga_ref_align_sw_batch.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int,
+                                            vp, vp, vp, vp]
+        R.ga_ref_align_batch_status.argtypes = [vp, vp, vp, vp, vp, C.c_int]
+        _ref = R
+    return _ref
+
+
+def _b(s) -> bytes:
+    return s if isinstance(s, (bytes, bytearray)) else s.encode("latin-1")
+
+
+# ---------------------------------------------------------------------------
+# per-pair WFA
+# ---------------------------------------------------------------------------
+def wfa(pattern, text, smax: int):
+    """(score, cigar) as wfa_trace_kernel + decode_cigar produce them; cigar is
+    '' when unresolved or when the walk emitted nothing."""
+    p, t = _b(pattern), _b(text)
+    sc, ln = C.c_int(), C.c_int()
+    buf = C.create_string_buffer(len(p) + len(t) + 4)
+    if lib().ga_oracle_wfa(p, len(p), t, len(t), smax, C.byref(sc), buf, C.byref(ln)) != 0:
+        raise MemoryError("oracle wfa")
+    return sc.value, buf.raw[: ln.value].decode("latin-1")
+
+
+def wfa_score(pattern, text, smax: int) -> int:
+    p, t = _b(pattern), _b(text)
+    return lib().ga_oracle_wfa_score(p, len(p), t, len(t), smax)
+
+
+def levenshtein(pattern, text) -> int:
+    p, t = _b(pattern), _b(text)
+    return lib().ga_oracle_levenshtein(p, len(p), t, len(t))
+
+
+def wfa_validate(cigar: str, score: int, pattern, text):
+    p, t, c = _b(pattern), _b(text), _b(cigar)
+    r, w = C.c_int(), C.c_int()
+    lib().ga_oracle_wfa_validate(c, len(c), score, p, len(p), t, len(t), C.byref(r), C.byref(w))
+    return bool(r.value), bool(w.value)
+
+
+# ---------------------------------------------------------------------------
+# per-pair SW
+# ---------------------------------------------------------------------------
+def mm_table(match: int, mismatch: int) -> np.ndarray:
+    tab = np.empty(256 * 256, dtype=np.int32)
+    lib().ga_oracle_sw_table_mm(match, mismatch, tab.ctypes.data)
+    return tab
+
+
+def _table(scoring) -> np.ndarray:
+    if isinstance(scoring, np.ndarray):
+        return np.ascontiguousarray(scoring, dtype=np.int32).reshape(-1)
+    return mm_table(scoring[0], scoring[1])
+
+
+def _gaps(scoring, gap_open=None, gap_extend=None):
+    if isinstance(scoring, np.ndarray):
+        return gap_open, gap_extend
+    return scoring[2], scoring[3]
+
+
+def sw_trace(text, pattern, scoring, gap_open=None, gap_extend=None):
+    """sw_trace_kernel + sw_trace_walk: (score, start_i, start_j, end_i, end_j, cigar).
+    ``scoring`` is (match, mismatch, gap_open, gap_extend) or a 256x256 table
+    (then pass gap_open/gap_extend)."""
+    t, p = _b(text), _b(pattern)
+    tab = _table(scoring)
+    go, ge = _gaps(scoring, gap_open, gap_extend)
+    o = [C.c_int() for _ in range(6)]
+    buf = C.create_string_buffer(len(t) + len(p) + 4)
+    if lib().ga_oracle_sw_trace(t, len(t), p, len(p), tab.ctypes.data, go, ge,
+                                *[C.byref(x) for x in o[:5]], buf, C.byref(o[5])) != 0:
+        raise MemoryError("oracle sw_trace")
+    return (o[0].value, o[1].value, o[2].value, o[3].value, o[4].value,
+            buf.raw[: o[5].value].decode("latin-1"))
+
+
+def sw_value_walk(text, pattern, scoring, gap_open=None, gap_extend=None):
+    t, p = _b(text), _b(pattern)
+    tab = _table(scoring)
+    go, ge = _gaps(scoring, gap_open, gap_extend)
+    o = [C.c_int() for _ in range(6)]
+    buf = C.create_string_buffer(len(t) + len(p) + 4)
+    if lib().ga_oracle_sw_value_walk(t, len(t), p, len(p), tab.ctypes.data, go, ge,
+                                     *[C.byref(x) for x in o[:5]], buf, C.byref(o[5])) != 0:
+        raise MemoryError("oracle sw_value_walk")
+    return (o[0].value, o[1].value, o[2].value, o[3].value, o[4].value,
+            buf.raw[: o[5].value].decode("latin-1"))
+
+
+def sw_score_lane(text, pattern, scoring, gap_open=None, gap_extend=None, wsize: int = 64):
+    """sw_score_kernel's (score, end_i, end_j) at warp width ``wsize``."""
+    t, p = _b(text), _b(pattern)
+    tab = _table(scoring)
+    go, ge = _gaps(scoring, gap_open, gap_extend)
+    s, i, j = C.c_int(), C.c_int(), C.c_int()
+    lib().ga_oracle_sw_score_lane(t, len(t), p, len(p), tab.ctypes.data, go, ge, wsize,
+                                  C.byref(s), C.byref(i), C.byref(j))
+    return s.value, i.value, j.value
+
+
+def sw_validate(cigar, text, start_i, end_i, pattern, start_j, end_j, scoring, score,
+                gap_open=None, gap_extend=None):
+    t, p, c = _b(text), _b(pattern), _b(cigar)
+    tab = _table(scoring)
+    go, ge = _gaps(scoring, gap_open, gap_extend)
+    r, w = C.c_int(), C.c_int()
+    lib().ga_oracle_sw_validate(c, len(c), t, len(t), start_i, end_i, p, len(p), start_j, end_j,
+                                tab.ctypes.data, go, ge, score, C.byref(r), C.byref(w))
+    return bool(r.value), bool(w.value)
+
+
+# ---------------------------------------------------------------------------
+# batch rules of the reference API, restated over the per-pair functions
+# ---------------------------------------------------------------------------
+@dataclass
+class AlignResult:
+    score: int = -1
+    resolved: bool = False
+    cigar: str = ""
+    rescore_ok: bool = False
+    wellformed_ok: bool = False
+
+
+@dataclass
+class BatchResult:
+    results: List[AlignResult] = field(default_factory=list)
+    resolved_count: int = 0
+    unresolved_count: int = 0
+    status: str = "ok"           # ok | invalid_argument | device_error
+    error: Optional[str] = None
+
+
+def align_batch(reqs: Sequence) -> BatchResult:
+    """src/api/api.cpp:234-497.  Each request has .pattern .pattern_len .text
+    .text_len .smax .with_cigar (pattern/text may be None)."""
+    out = BatchResult(results=[AlignResult() for _ in reqs])
+    if not reqs:
+        return out
+    for r in reqs:                                   # api.cpp:246-266
+        if (r.pattern_len > 0 and r.pattern is None) or (r.text_len > 0 and r.text is None):
+            return BatchResult([AlignResult() for _ in reqs], status="invalid_argument",
+                               error="null pointer with non-zero length")
+        if r.pattern_len < 0 or r.text_len < 0:
+            return BatchResult([AlignResult() for _ in reqs], status="invalid_argument",
+                               error="negative length")
+        if r.smax < 0 or r.smax > K_MAX_SMAX:
+            return BatchResult([AlignResult() for _ in reqs], status="invalid_argument",
+                               error="smax out of range [0, 511]")
+    smax = min(r.smax for r in reqs)                 # api.cpp:271-272
+    any_cigar = any(r.with_cigar for r in reqs)      # api.cpp:273-274
+    for k, r in enumerate(reqs):
+        p = _b(r.pattern or b"")[: r.pattern_len]
+        t = _b(r.text or b"")[: r.text_len]
+        res = out.results[k]
+        if any_cigar:                                # api.cpp:456-469
+            sc, cg = wfa(p, t, smax)
+            res.score, res.resolved = sc, sc >= 0
+            if res.resolved and len(cg) > 0:
+                res.cigar = cg
+                res.rescore_ok, res.wellformed_ok = wfa_validate(cg, sc, p, t)
+        else:                                        # api.cpp:471-488
+            sc = wfa_score(p, t, smax)
+            res.score, res.resolved = sc, sc >= 0
+    out.resolved_count = sum(1 for r in out.results if r.resolved)
+    out.unresolved_count = len(out.results) - out.resolved_count
+    return out
+
+
+@dataclass
+class SWAlignResult:
+    score: int = -1
+    resolved: bool = False
+    cigar: str = ""
+    start_i: int = -1
+    start_j: int = -1
+    end_i: int = -1
+    end_j: int = -1
+    rescore_ok: bool = False
+    wellformed_ok: bool = False
+    too_large: bool = False
+
+
+@dataclass
+class SWBatchResult:
+    results: List[SWAlignResult] = field(default_factory=list)
+    resolved_count: int = 0
+    unresolved_count: int = 0
+    status: str = "ok"
+    error: Optional[str] = None
+
+
+def sw_scheme_error(sc, with_cigar: bool) -> Optional[str]:
+    """api.cpp:515-536"""
+    match, mismatch, go, ge = sc
+    if match <= 0:
+        return "SW scoring: match must be > 0 (the score is 0 for every input otherwise)"
+    if mismatch >= match:
+        return "SW scoring: mismatch must be < match"
+    if go <= 0 or ge <= 0:
+        return "SW scoring: gap costs must be > 0"
+    if with_cigar and ge > go:
+        return "SW scoring: gap_extend > gap_open with with_cigar"
+    return None
+
+
+def align_sw_batch(reqs: Sequence) -> SWBatchResult:
+    """src/api/api.cpp:584-847 (device path semantics).  Each request has .text
+    .text_len .pattern .pattern_len .scoring (4-tuple) .with_cigar."""
+    n = len(reqs)
+    out = SWBatchResult(results=[SWAlignResult() for _ in reqs])
+    if n == 0:
+        return out
+
+    def invalid(msg):
+        return SWBatchResult([SWAlignResult() for _ in reqs], status="invalid_argument", error=msg)
+
+    s0 = tuple(reqs[0].scoring)
+    for r in reqs:                                   # api.cpp:600-616
+        if (r.text_len > 0 and r.text is None) or (r.pattern_len > 0 and r.pattern is None):
+            return invalid("null pointer with non-zero length")
+        if r.text_len < 0 or r.pattern_len < 0:
+            return invalid("negative length")
+        e = sw_scheme_error(tuple(r.scoring), r.with_cigar)
+        if e:
+            return invalid(e)
+        if tuple(r.scoring) != s0:
+            return invalid("all requests in an SW batch must share one scoring scheme")
+    any_cigar = any(r.with_cigar for r in reqs)
+    max_cells = max((r.text_len + 1) * (r.pattern_len + 1) for r in reqs)
+    dir_stride = min(max_cells, SW_MAX_TRACE_CELLS)  # api.cpp:644-645
+    for k, r in enumerate(reqs):
+        t = _b(r.text or b"")[: r.text_len]
+        p = _b(r.pattern or b"")[: r.pattern_len]
+        res = out.results[k]
+        if any_cigar:                                # api.cpp:750-784, 542-580
+            m, nn = len(t), len(p)
+            if m > 0 and nn > 0 and (m + 1) * (nn + 1) > dir_stride:
+                res.too_large = True
+                continue
+            sc, si, sj, ei, ej, cg = sw_trace(t, p, s0)
+            res.score, res.resolved, res.end_i, res.end_j = sc, True, ei, ej
+            if sc == 0:
+                res.rescore_ok = res.wellformed_ok = True
+                continue
+            if not r.with_cigar:
+                continue
+            res.start_i, res.start_j = si, sj
+            if len(cg) > 0:
+                res.cigar = cg
+                res.rescore_ok, res.wellformed_ok = sw_validate(cg, t, si, ei, p, sj, ej, s0, sc)
+        else:                                        # api.cpp:785-838
+            sc, ei, ej = sw_score_lane(t, p, s0, wsize=64)
+            res.score, res.resolved, res.end_i, res.end_j = sc, True, ei, ej
+            if sc == 0:
+                res.rescore_ok = res.wellformed_ok = True
+    out.resolved_count = sum(1 for r in out.results if r.resolved)
+    out.unresolved_count = n - out.resolved_count
+    return out
+
+
+# ---------------------------------------------------------------------------
+# reference (compiled through its CPU shim) -- available only where built
+# ---------------------------------------------------------------------------
+def ref_wfa(pattern, text, smax: int):
+    R = ref()
+    p, t = _b(pattern), _b(text)
+    sc, ln, em, tr = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    buf = C.create_string_buffer(len(p) + len(t) + 4)
+    R.ga_ref_wfa_trace(p, len(p), t, len(t), smax, C.byref(sc), buf, C.byref(ln), C.byref(em),
+                       C.byref(tr))
+    return sc.value, buf.raw[: ln.value].decode("latin-1")
+
+
+def ref_wfa_score_flat(pattern, text, smax: int) -> int:
+    p, t = _b(pattern), _b(text)
+    return ref().ga_ref_wfa_score_flat(p, len(p), t, len(t), smax)
+
+
+def ref_sw_trace(text, pattern, scoring):
+    t, p = _b(text), _b(pattern)
+    o = (C.c_int * 5)()
+    ln = C.c_int()
+    buf = C.create_string_buffer(len(t) + len(p) + 4)
+    ref().ga_ref_sw_trace(t, len(t), p, len(p), *scoring, o, buf, C.byref(ln))
+    return (o[0], o[1], o[2], o[3], o[4], buf.raw[: ln.value].decode("latin-1"))
+
+
+def ref_sw_score(pairs, scoring, wsize: int = 64):
+    """sw_score_kernel under the shim's warp emulation; pairs = [(text, pattern)]."""
+    texts = [_b(t) for t, _ in pairs]
+    pats = [_b(p) for _, p in pairs]
+    T = b"".join(texts) or b"\0"
+    P = b"".join(pats) or b"\0"
+    toff = np.cumsum([0] + [len(x) for x in texts[:-1]]).astype(np.int64)
+    poff = np.cumsum([0] + [len(x) for x in pats[:-1]]).astype(np.int64)
+    tl = np.array([len(x) for x in texts], dtype=np.int32)
+    pl = np.array([len(x) for x in pats], dtype=np.int32)
+    out = np.zeros(3 * len(pairs), dtype=np.int32)
+    Tb = C.create_string_buffer(T, len(T))
+    Pb = C.create_string_buffer(P, len(P))
+    ref().ga_ref_sw_score(C.addressof(Tb), toff.ctypes.data, tl.ctypes.data, C.addressof(Pb),
+                          poff.ctypes.data, pl.ctypes.data, len(pairs), *scoring, wsize,
+                          out.ctypes.data)
+    return [tuple(int(v) for v in out[3 * i: 3 * i + 3]) for i in range(len(pairs))]

@@ -1,0 +1,141 @@
+"""The C-ABI library without a GPU: it loads, exports every function that
+include/genoaligner/genoaligner_c.h declares, its struct layouts match the
+Python binding, and requests that the reference refuses are refused before any
+device work (src/api/api.cpp:245-266, :600-616, :515-536)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "genoaligner", "genoaligner_c.h")
+
+
+def _declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ga_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from genoaligner_amd import _native as N
+    L = N.lib()
+    declared = _declared_functions()
+    assert set(declared) == set(N.EXPORTS)
+    for name in declared:
+        assert hasattr(L, name), name
+    nm = subprocess.run(["nm", "-D", "--defined-only", N.LIB_PATH], capture_output=True, text=True).stdout
+    for name in declared:
+        assert re.search(rf" T {name}$", nm, re.M), name
+
+
+def test_struct_layouts_match_header(tmp_path):
+    from genoaligner_amd import _native as N
+    src = tmp_path / "layout.c"
+    checks = {
+        "ga_align_req": N.AlignReq, "ga_align_res": N.AlignRes, "ga_sw_scoring": N.SWScoringC,
+        "ga_sw_req": N.SWReq, "ga_sw_res": N.SWRes, "ga_wfa_dev_batch": N.WfaDevBatch,
+        "ga_sw_dev_batch": N.SwDevBatch,
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
+    for cname, py in checks.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0;}")
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-std=c99", str(src), "-o", str(exe)])
+    got = dict(l.split() for l in subprocess.check_output([str(exe)], text=True).splitlines())
+    for cname, py in checks.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(py, fname).offset, (cname, fname)
+
+
+def test_backend_name():
+    import genoaligner_amd as ga
+    assert ga.backend_name() == "rocm"
+
+
+@pytest.mark.parametrize("mutate,msg", [
+    (lambda r: setattr(r, "smax", 512), "smax out of range [0, 511]"),
+    (lambda r: setattr(r, "smax", -1), "smax out of range [0, 511]"),
+    (lambda r: setattr(r, "pattern_len", -1), "negative length"),
+    (lambda r: setattr(r, "pattern", None), "null pointer with non-zero length"),
+])
+def test_wfa_refusals_before_device_work(mutate, msg):
+    import genoaligner_amd as ga
+    good = ga.AlignRequest(pattern="ACGT", pattern_len=4, text="ACGA", text_len=4, smax=8)
+    bad = ga.AlignRequest(pattern="ACGT", pattern_len=4, text="ACGA", text_len=4, smax=8)
+    mutate(bad)
+    b = ga.align_batch([good, bad])
+    assert b.status == ga.Status.invalid_argument and b.error == msg
+    assert all(r.score == -1 and not r.resolved for r in b.results)
+    assert b.resolved_count == 0 and b.unresolved_count == 0
+
+
+def test_wfa_ext_bound():
+    import genoaligner_amd as ga
+    b = ga.align_batch_ext([ga.AlignRequest(pattern="A", pattern_len=1, text="A", text_len=1, smax=4001)])
+    assert b.status == ga.Status.invalid_argument
+
+
+@pytest.mark.parametrize("scoring,with_cigar,msg", [
+    ((0, -3, 5, 2), True, "SW scoring: match must be > 0"),
+    ((2, 2, 5, 2), True, "SW scoring: mismatch must be < match"),
+    ((2, -3, 0, 2), True, "SW scoring: gap costs must be > 0"),
+    ((2, -3, 5, 0), True, "SW scoring: gap costs must be > 0"),
+    ((2, -3, 1, 2), True, "SW scoring: gap_extend > gap_open with with_cigar"),
+])
+def test_sw_refusals_before_device_work(scoring, with_cigar, msg):
+    import genoaligner_amd as ga
+    r = ga.SWRequest(text="ACGT", text_len=4, pattern="ACGT", pattern_len=4,
+                     scoring=ga.SWScoring(*scoring), with_cigar=with_cigar)
+    b = ga.align_sw_batch([r])
+    assert b.status == ga.Status.invalid_argument and b.error.startswith(msg)
+
+
+def test_sw_mixed_schemes_refused():
+    import genoaligner_amd as ga
+    a = ga.SWRequest(text="ACGT", text_len=4, pattern="ACGT", pattern_len=4)
+    b = ga.SWRequest(text="ACGT", text_len=4, pattern="ACGT", pattern_len=4,
+                     scoring=ga.SWScoring(2, -3, 5, 1))
+    out = ga.align_sw_batch([a, b])
+    assert out.status == ga.Status.invalid_argument
+    assert out.error == "all requests in an SW batch must share one scoring scheme"
+
+
+def test_empty_batches():
+    import genoaligner_amd as ga
+    assert ga.align_batch([]).ok() and ga.align_sw_batch([]).ok()
+
+
+needs_ref = pytest.mark.skipif("not __import__('oracle.oracle', fromlist=['x']).ref_available()",
+                               reason="reference shim library not built")
+
+
+@needs_ref
+def test_refusals_match_reference():
+    """The same malformed batches the reference API refuses are refused here."""
+    import numpy as np
+    from oracle import oracle as O
+    import genoaligner_amd as ga
+    R = O.ref()
+    cases = [([4, 4], [4, 4], [0, 0], [0, 0], [8, 8], 0), ([4, 4], [4, 4], [0, 0], [0, 0], [8, 600], 1),
+             ([4, -1], [4, 4], [0, 0], [0, 0], [8, 8], 1), ([4, 4], [4, 4], [0, 1], [0, 0], [8, 8], 1)]
+    for pl, tl, npat, ntxt, smax, want in cases:
+        a = [np.array(x, dtype=np.int32) for x in (pl, tl, smax)]
+        nb = [np.array(x, dtype=np.uint8) for x in (npat, ntxt)]
+        st = R.ga_ref_align_batch_status(a[0].ctypes.data, a[1].ctypes.data, nb[0].ctypes.data,
+                                         nb[1].ctypes.data, a[2].ctypes.data, 2)
+        reqs = [ga.AlignRequest(pattern=None if npat[k] else "A" * max(pl[k], 0), pattern_len=pl[k],
+                                text=None if ntxt[k] else "A" * max(tl[k], 0), text_len=tl[k],
+                                smax=smax[k]) for k in range(2)]
+        if want == 1:
+            assert st == 1 and ga.align_batch(reqs).status == ga.Status.invalid_argument
+        else:
+            assert st == 0

@@ -1,0 +1,225 @@
+"""GPU parity: the HIP path through the C ABI against the oracle, the golden
+vectors written from the reference's own code, and the reference's result
+rules.  Integer work, so every comparison is exact."""
+import random
+
+import numpy as np
+import pytest
+
+from conftest import golden_sw_score, golden_sw_trace, golden_wfa
+
+pytestmark = pytest.mark.gpu
+
+
+def _mutate(rng, s, rate, alpha="ACGT"):
+    out = []
+    for c in s:
+        r = rng.random()
+        if r < rate / 2:
+            out.append(rng.choice([x for x in alpha if x != c] or [c]))
+        elif r < rate * 3 / 4:
+            out += [c, rng.choice(alpha)]
+        elif r >= rate:
+            out.append(c)
+    return "".join(out)
+
+
+def _areq(ga, p, t, smax, with_cigar=True):
+    return ga.AlignRequest(pattern=p, pattern_len=len(p), text=t, text_len=len(t), smax=smax,
+                           with_cigar=with_cigar)
+
+
+def _sreq(ga, t, p, sc, with_cigar=True):
+    return ga.SWRequest(text=t, text_len=len(t), pattern=p, pattern_len=len(p),
+                        scoring=ga.SWScoring(*sc), with_cigar=with_cigar)
+
+
+# ------------------------------------------------------------------ WFA
+def test_wfa_golden_api(gpu):
+    import genoaligner_amd as ga
+    rows = golden_wfa()
+    by_smax = {}
+    for r in rows:
+        by_smax.setdefault(r[0], []).append(r)
+    checked = 0
+    for smax, rs in sorted(by_smax.items()):
+        b = ga.align_batch([_areq(ga, p, t, smax) for _, p, t, _, _, _ in rs])
+        assert b.ok(), b.error
+        for (s, p, t, score, cigar, _), res in zip(rs, b.results):
+            assert (res.score, res.cigar) == (score, cigar), (s, p, t)
+            if cigar:
+                assert res.rescore_ok and res.wellformed_ok
+            checked += 1
+        # score-only batch: same scores, no CIGARs
+        bs = ga.align_batch([_areq(ga, p, t, smax, False) for _, p, t, _, _, _ in rs])
+        assert [r.score for r in bs.results] == [r[3] for r in rs]
+        assert all(r.cigar == "" for r in bs.results)
+    assert checked == len(rows)
+
+
+def test_wfa_random_vs_oracle(gpu, oracle):
+    import genoaligner_amd as ga
+    rng = random.Random(2026)
+    for smax in (0, 5, 64, 300, 511):
+        reqs, pairs = [], []
+        for _ in range(300):
+            p = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 400)))
+            t = _mutate(rng, p, rng.choice([0.0, 0.01, 0.05, 0.2, 0.6]))
+            pairs.append((p, t))
+            reqs.append(_areq(ga, p, t, smax))
+        b = ga.align_batch(reqs)
+        assert b.ok()
+        for (p, t), res in zip(pairs, b.results):
+            assert (res.score, res.cigar) == oracle.wfa(p, t, smax), (smax, p, t)
+
+
+def test_wfa_batch_rules(gpu, oracle):
+    import genoaligner_amd as ga
+    # min smax and the mixed-batch CIGAR rule (src/api/api.cpp:271-274, :456-469)
+    reqs = [_areq(ga, "ACGT", "ACGA", 8, False), _areq(ga, "AAAA", "TTTT", 2, True),
+            _areq(ga, "", "", 5, False), _areq(ga, "ACGTNNacgt", "ACGTNNACGT", 9, False)]
+    b = ga.align_batch(reqs)
+    want = oracle.align_batch(reqs)
+    assert b.ok()
+    for got, exp in zip(b.results, want.results):
+        assert (got.score, got.resolved, got.cigar, got.rescore_ok, got.wellformed_ok) == \
+               (exp.score, exp.resolved, exp.cigar, exp.rescore_ok, exp.wellformed_ok)
+    assert (b.resolved_count, b.unresolved_count) == (want.resolved_count, want.unresolved_count)
+    # documented example (api.hpp:41-46)
+    r = ga.align(_areq(ga, "ACGTACGT", "ACGTTCGT", 32))
+    assert r.resolved and r.score == 1 and r.cigar == "MMMMXMMM"
+
+
+def test_wfa_ext_per_pair_smax(gpu, oracle):
+    import genoaligner_amd as ga
+    rng = random.Random(7)
+    reqs, exp = [], []
+    for _ in range(200):
+        L = rng.randint(50, 3000)
+        p = "".join(rng.choice("ACGT") for _ in range(L))
+        t = _mutate(rng, p, rng.uniform(0, 0.15))
+        smax = min(int(0.2 * L), 2048)
+        wc = rng.random() < 0.7
+        reqs.append(_areq(ga, p, t, smax, wc))
+        sc, cg = oracle.wfa(p, t, smax)
+        exp.append((sc, cg if wc else ""))
+    b = ga.align_batch_ext(reqs)
+    assert b.ok()
+    assert [(r.score, r.cigar) for r in b.results] == exp
+
+
+def test_wfa_device_resident_plain(gpu, oracle):
+    from genoaligner_amd import workloads as W
+    from genoaligner_amd.device import WfaDeviceBatch
+    ps = W.wfa_plain(n=2000)
+    dev = WfaDeviceBatch(ps, trace=True)
+    dev.run()
+    sc, cig = dev.results()
+    for i in range(ps.n):
+        p, t = ps.a_seq(i), ps.b_seq(i)
+        osc, ocg = oracle.wfa(p, t, int(ps.smax[i]))
+        assert (int(sc[i]), cig[i].decode()) == (osc, ocg), i
+
+
+# ------------------------------------------------------------------ SW
+def test_sw_trace_golden_api(gpu):
+    import genoaligner_amd as ga
+    rows = golden_sw_trace()
+    by = {}
+    for r in rows:
+        by.setdefault(r[0], []).append(r)
+    for sc, rs in by.items():
+        b = ga.align_sw_batch([_sreq(ga, t, p, sc) for _, t, p, *_ in rs])
+        assert b.ok(), b.error
+        for (_, t, p, score, si, sj, ei, ej, cigar), res in zip(rs, b.results):
+            assert (res.score, res.start_i, res.start_j, res.end_i, res.end_j, res.cigar) == \
+                   (score, si, sj, ei, ej, cigar), (sc, t, p)
+            assert res.resolved
+            if score > 0:
+                assert res.rescore_ok and res.wellformed_ok
+
+
+def test_sw_score_only_golden_api(gpu):
+    """Score-only batches report the end cell by the wave64 lane rule."""
+    import genoaligner_amd as ga
+    rows = golden_sw_score()
+    by = {}
+    for r in rows:
+        by.setdefault(r[0], []).append(r)
+    for sc, rs in by.items():
+        b = ga.align_sw_batch([_sreq(ga, t, p, sc, False) for _, t, p, *_ in rs])
+        assert b.ok(), b.error
+        for (_, t, p, score, ei, ej), res in zip(rs, b.results):
+            assert (res.score, res.end_i, res.end_j, res.start_i, res.start_j, res.cigar) == \
+                   (score, ei, ej, -1, -1, ""), (sc, t, p)
+
+
+def test_sw_random_vs_oracle(gpu, oracle):
+    import genoaligner_amd as ga
+    rng = random.Random(31)
+    for sc in ((2, -3, 5, 2), (2, -3, 3, 1), (1, -1, 2, 1)):
+        reqs, pairs = [], []
+        for _ in range(150):
+            t = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 700)))
+            a = rng.randint(0, len(t))
+            p = _mutate(rng, t[a: a + rng.randint(0, 300)], 0.1)
+            if rng.random() < 0.2:
+                p = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 200)))
+            pairs.append((t, p))
+            reqs.append(_sreq(ga, t, p, sc, rng.random() < 0.8))
+        b = ga.align_sw_batch(reqs)
+        want = oracle.align_sw_batch(reqs)
+        assert b.ok()
+        for got, exp, (t, p) in zip(b.results, want.results, pairs):
+            assert (got.score, got.resolved, got.cigar, got.start_i, got.start_j, got.end_i,
+                    got.end_j, got.rescore_ok, got.wellformed_ok, got.too_large) == \
+                   (exp.score, exp.resolved, exp.cigar, exp.start_i, exp.start_j, exp.end_i,
+                    exp.end_j, exp.rescore_ok, exp.wellformed_ok, exp.too_large), (sc, t, p)
+
+
+def test_sw_long_text_global_boundary(gpu, oracle):
+    """Texts longer than the LDS boundary buffer take the HBM boundary path."""
+    import genoaligner_amd as ga
+    rng = random.Random(3)
+    t = "".join(rng.choice("ACGT") for _ in range(9000))
+    p = _mutate(rng, t[4000:4150], 0.1)
+    reqs = [_sreq(ga, t, p, (2, -3, 3, 1)), _sreq(ga, t[:100], p, (2, -3, 3, 1))]
+    b = ga.align_sw_batch(reqs)
+    want = oracle.align_sw_batch(reqs)
+    for got, exp in zip(b.results, want.results):
+        assert (got.score, got.cigar, got.start_i, got.end_i, got.start_j, got.end_j) == \
+               (exp.score, exp.cigar, exp.start_i, exp.end_i, exp.start_j, exp.end_j)
+
+
+def test_sw_too_large(gpu):
+    import genoaligner_amd as ga
+    big = "A" * 8192
+    reqs = [_sreq(ga, big, big[:8193], (2, -3, 5, 2)), _sreq(ga, "TTTACGTGTT", "ACGTGT", (2, -3, 5, 2))]
+    reqs[0] = _sreq(ga, big + "C", big, (2, -3, 5, 2))        # (8194)(8193) > 2^26 cells
+    b = ga.align_sw_batch(reqs)
+    assert b.ok()
+    assert b.results[0].too_large and not b.results[0].resolved and b.results[0].score == -1
+    r = b.results[1]
+    assert (r.score, r.cigar, r.start_i, r.end_i, r.start_j, r.end_j) == (12, "MMMMMM", 3, 8, 0, 5)
+    assert (b.resolved_count, b.unresolved_count) == (1, 1)
+
+
+def test_sw_device_resident_embedded(gpu, oracle):
+    from genoaligner_amd import workloads as W
+    from genoaligner_amd.device import SwDeviceBatch
+    ps = W.sw_embedded(n=256)
+    dev = SwDeviceBatch(ps, scoring=(2, -3, 3, 1), trace=True)
+    dev.run()
+    out, cig = dev.results()
+    for i in range(ps.n):
+        t, p = ps.a_seq(i), ps.b_seq(i)
+        r = oracle.sw_trace(t, p, (2, -3, 3, 1))
+        assert tuple(int(x) for x in out[i]) == r[:5], i
+        assert cig[i].decode() == r[5], i
+    dev2 = SwDeviceBatch(ps, scoring=(2, -3, 3, 1), trace=False)
+    dev2.run()
+    out2, _ = dev2.results()
+    for i in range(0, ps.n, 7):
+        t, p = ps.a_seq(i), ps.b_seq(i)
+        s, ei, ej = oracle.sw_score_lane(t, p, (2, -3, 3, 1))
+        assert tuple(int(x) for x in out2[i]) == (s, -1, -1, ei, ej)
