@@ -36,7 +36,7 @@ hipError_t wfa_launch(const WfaLaunch& L, bool trace, hipStream_t st);
 struct SwLaunch {
     const uint8_t* txt; const int64_t* toff; const int32_t* tlen;   // rows
     const uint8_t* pat; const int64_t* poff; const int32_t* plen;   // columns
-    int32_t n;
+    int32_t n;              // blocks of the forward kernel / threads of the walk
     int32_t max_m;          // max text_len in the launch
     int32_t match, mismatch, go, ge;
     // out: 8 ints per pair: score, end_i, end_j (row-major first), lane_end_i,
@@ -49,7 +49,48 @@ struct SwLaunch {
     uint8_t* cigar; const int64_t* cig_off;
     int32_t* bnd;           // global strip-boundary scratch (max_m*2 ints per pair) or null
     int32_t first_pair;
+    // Re-run mode: block/thread q serves slot (wave q/2, half q%2) of a packed
+    // launch (pairs from pair_a/pair_b, or first_pair + q when those are null),
+    // only when rerun[pid] is set; its direction words start at
+    // dirs + (q/2) * dir_stride + (q%2) * (dir_stride/2).
+    const uint8_t* rerun;
+    const int32_t* pair_a; const int32_t* pair_b;
+    int32_t n_total;
 };
+
+// Marks, in res[0] of a pair, that the packed 16-bit kernel left it to the int32 kernel.
+constexpr int kSwRerun = -2;
+
+struct Sw16Launch {
+    const uint8_t* txt; const int64_t* toff; const int32_t* tlen;
+    const uint8_t* pat; const int64_t* poff; const int32_t* plen;
+    const int32_t* pair_a; const int32_t* pair_b;   // per wave; null: first_pair + 2w, + 1
+    int32_t first_pair;
+    int32_t n_total;        // pairs in the arrays (a half whose pair id >= n_total is empty)
+    int32_t n_waves;
+    int32_t max_m;          // LDS sizing: max text_len over the launch's pairs
+    int32_t match, mismatch, go, ge;
+    int32_t* res;           // 8 ints per pair, as SwLaunch
+    uint8_t* rerun;         // out: 1 = needs the int32 kernel
+    uint32_t* dirs; int64_t dir_stride;             // words per wave (trace)
+    int32_t* wave_m;        // out: per wave, rows streamed (trace walk)
+    const uint8_t* walk;    // per pair: emit span + CIGAR; null = all
+    uint8_t* cigar; const int64_t* cig_off;
+};
+
+// Largest text length the packed kernel takes (LDS: 12 bytes per row + guards).
+constexpr int kSw16MaxRows = 4096;
+
+inline int64_t sw16_dir_words_per_wave(int max_m, int max_n)
+{
+    if (max_m <= 0 || max_n <= 0) return 0;
+    const int64_t strips = (max_n + 63) / 64;
+    const int64_t nq = ((max_m + 63 + 3) & ~3) / 4;
+    const int64_t own = strips * nq * 64;
+    // room for the int32 layout of both halves when they are re-run
+    const int64_t alt = 2 * strips * int64_t((max_m + 63 + 7) / 8) * 64;
+    return own > alt ? own : alt;
+}
 
 // Words of direction nibbles one pair needs for the trace path.
 inline int64_t sw_dir_words(int m, int n)
@@ -68,6 +109,11 @@ struct LaunchHooks {
 };
 
 hipError_t sw_launch(const SwLaunch& L, bool trace, hipStream_t st, LaunchHooks* hooks = nullptr);
+
+// Packed kernel over L.n_waves waves (two pairs each), then -- separately --
+// its walk over 2 * n_waves slots.
+hipError_t sw16_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHooks* hooks = nullptr);
+hipError_t sw16_walk_launch(const Sw16Launch& L, hipStream_t st);
 
 // Reduce the 8-int per-pair records to the device-resident result layout:
 // out5 = score, start_i, start_j, end_i, end_j (trace: row-major end and span;

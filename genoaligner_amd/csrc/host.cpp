@@ -453,6 +453,28 @@ const char* sw_scheme_error(const ga_sw_scoring& s, bool with_cigar)
     return nullptr;
 }
 
+// Enqueue the packed 16-bit path for one chunk of waves: the packed forward
+// kernel, the int32 kernel over the pairs it flagged (same direction region),
+// and both walks.  All on `st`, no host synchronisation.
+hipError_t enqueue_sw16_chunk(const ga::Sw16Launch& P16, bool trace, hipStream_t st,
+                              ga::LaunchHooks* hooks)
+{
+    hipError_t e = ga::sw16_launch(P16, trace, st, hooks);
+    if (e != hipSuccess) return e;
+    ga::SwLaunch R{};
+    R.txt = P16.txt; R.toff = P16.toff; R.tlen = P16.tlen;
+    R.pat = P16.pat; R.poff = P16.poff; R.plen = P16.plen;
+    R.n = 2 * P16.n_waves; R.max_m = P16.max_m;
+    R.match = P16.match; R.mismatch = P16.mismatch; R.go = P16.go; R.ge = P16.ge;
+    R.res = P16.res; R.dirs = P16.dirs; R.dir_off = nullptr; R.dir_stride = P16.dir_stride;
+    R.walk = P16.walk; R.cigar = P16.cigar; R.cig_off = P16.cig_off; R.bnd = nullptr;
+    R.first_pair = P16.first_pair; R.rerun = P16.rerun;
+    R.pair_a = P16.pair_a; R.pair_b = P16.pair_b; R.n_total = P16.n_total;
+    e = ga::sw_launch(R, trace, st);
+    if (e != hipSuccess || !trace) return e;
+    return ga::sw16_walk_launch(P16, st);
+}
+
 // Device SW over pairs given as pointers; res8 receives 8 ints per pair (see
 // ga::SwLaunch).  walk[i] selects pairs whose span + CIGAR are produced.
 const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>& skip,
@@ -460,25 +482,48 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
                    std::vector<int32_t>& res8, std::vector<int64_t>& slot_off)
 {
     std::call_once(g_pool_once, init_pool);
+    // Order on the device: pairs for the int32 kernel first (texts longer than
+    // the packed kernel takes, and empty or skipped pairs), then the packed
+    // kernel's pairs sorted by shape so each wave's two pairs are alike.
+    std::vector<int64_t> order;
+    order.reserve(size_t(n));
+    std::vector<int64_t> packed;
+    slot_off.resize(static_cast<size_t>(n));
+    int64_t slot = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const ga_sw_req& r = reqs[i];
+        slot_off[size_t(i)] = slot;
+        slot += std::max(r.text_len, 0) + std::max(r.pattern_len, 0);
+        const bool sk = skip[size_t(i)] != 0;
+        if (!sk && r.text_len > 0 && r.pattern_len > 0 && r.text_len <= ga::kSw16MaxRows)
+            packed.push_back(i);
+        else
+            order.push_back(i);
+    }
+    const int64_t n32 = int64_t(order.size());
+    std::stable_sort(packed.begin(), packed.end(), [&](int64_t a, int64_t b) {
+        if (reqs[a].text_len != reqs[b].text_len) return reqs[a].text_len > reqs[b].text_len;
+        return reqs[a].pattern_len > reqs[b].pattern_len;
+    });
+    order.insert(order.end(), packed.begin(), packed.end());
+
     Packed T, P;
     T.begin(); P.begin();
     std::vector<int32_t> tl(static_cast<size_t>(n)), pl(static_cast<size_t>(n));
-    std::vector<int64_t> dir_off(static_cast<size_t>(n));
-    slot_off.resize(static_cast<size_t>(n));
-    int64_t slot = 0, dir_words = 0;
-    int32_t max_m = 0;
-    for (int64_t i = 0; i < n; ++i) {
+    std::vector<int64_t> cig_off(static_cast<size_t>(n));
+    std::vector<uint8_t> walk_p(static_cast<size_t>(n));
+    int32_t max_m32 = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t i = order[size_t(k)];
         const ga_sw_req& r = reqs[i];
         const bool sk = skip[size_t(i)] != 0;
-        tl[size_t(i)] = sk ? 0 : r.text_len;
-        pl[size_t(i)] = sk ? 0 : r.pattern_len;
+        tl[size_t(k)] = sk ? 0 : r.text_len;
+        pl[size_t(k)] = sk ? 0 : r.pattern_len;
         T.add(r.text, sk ? 0 : r.text_len);
         P.add(r.pattern, sk ? 0 : r.pattern_len);
-        slot_off[size_t(i)] = slot;
-        slot += std::max(r.text_len, 0) + std::max(r.pattern_len, 0);
-        dir_off[size_t(i)] = dir_words;
-        if (trace) dir_words += ga::sw_dir_words(tl[size_t(i)], pl[size_t(i)]);
-        max_m = std::max(max_m, tl[size_t(i)]);
+        cig_off[size_t(k)] = slot_off[size_t(i)];
+        walk_p[size_t(k)] = walk[size_t(i)];
+        if (k < n32) max_m32 = std::max(max_m32, tl[size_t(k)]);
     }
     T.end(); P.end();
 
@@ -494,61 +539,119 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
     auto* d_tl = D.upload(tl);
     auto* d_pl = D.upload(pl);
     auto* d_res = D.alloc<int32_t>(size_t(n) * 8);
+    auto* d_rerun = D.alloc<uint8_t>(size_t(n));
     uint8_t* d_cig = nullptr;
     int64_t* d_slot = nullptr;
-    int64_t* d_diroff = nullptr;
     uint8_t* d_walk = nullptr;
-    uint32_t* d_dirs = nullptr;
-    // chunking by the direction-table budget; dir offsets restart per chunk
-    std::vector<std::pair<int64_t, int64_t>> chunks;
     if (trace) {
         d_cig = D.alloc<uint8_t>(size_t(slot));
-        d_slot = D.upload(slot_off);
-        d_walk = D.upload(walk);
-        int64_t a = 0, words = 0, max_words = 0;
-        std::vector<int64_t> local(static_cast<size_t>(n));
-        for (int64_t i = 0; i < n; ++i) {
-            const int64_t w = ga::sw_dir_words(tl[size_t(i)], pl[size_t(i)]);
-            if (i > a && (words + w) * 4 > kSwDirBudget) {
-                chunks.emplace_back(a, i);
-                max_words = std::max(max_words, words);
-                a = i; words = 0;
-            }
-            local[size_t(i)] = words;
-            words += w;
-        }
-        chunks.emplace_back(a, n);
-        max_words = std::max(max_words, words);
-        d_diroff = D.upload(local);
-        d_dirs = D.alloc<uint32_t>(size_t(max_words));
-    } else {
-        chunks.emplace_back(0, n);
+        d_slot = D.upload(cig_off);
+        d_walk = D.upload(walk_p);
     }
-    const bool lds_bnd = max_m <= kLdsBndMaxRows;
-    int32_t* d_bnd = nullptr;
-    const int64_t bnd_chunk = int64_t(1) << 16;
-    if (!lds_bnd) d_bnd = D.alloc<int32_t>(size_t(std::min<int64_t>(n, bnd_chunk)) * 2 * size_t(max_m));
-    if (!D.ok) err = "hipMalloc(dir table) -- batch workspace exceeds device memory; split the batch";
-
     const ga_sw_scoring sc = reqs[0].scoring;
-    for (auto& ch : chunks) {
-        if (err) break;
-        for (int64_t a = ch.first; a < ch.second && !err;) {
-            const int64_t len = std::min(ch.second - a, lds_bnd ? (int64_t(1) << 30) : bnd_chunk);
-            ga::SwLaunch L{};
+
+    // ---- int32 kernel: pairs [0, n32) -------------------------------------
+    if (n32 > 0 && D.ok) {
+        std::vector<std::pair<int64_t, int64_t>> chunks;
+        int64_t* d_diroff = nullptr;
+        uint32_t* d_dirs = nullptr;
+        if (trace) {
+            int64_t a = 0, words = 0, max_words = 0;
+            std::vector<int64_t> local(static_cast<size_t>(n32));
+            for (int64_t k = 0; k < n32; ++k) {
+                const int64_t w = ga::sw_dir_words(tl[size_t(k)], pl[size_t(k)]);
+                if (k > a && (words + w) * 4 > kSwDirBudget) {
+                    chunks.emplace_back(a, k);
+                    max_words = std::max(max_words, words);
+                    a = k; words = 0;
+                }
+                local[size_t(k)] = words;
+                words += w;
+            }
+            chunks.emplace_back(a, n32);
+            max_words = std::max(max_words, words);
+            d_diroff = D.upload(local);
+            d_dirs = D.alloc<uint32_t>(size_t(max_words));
+        } else {
+            chunks.emplace_back(0, n32);
+        }
+        const bool lds_bnd = max_m32 <= kLdsBndMaxRows;
+        int32_t* d_bnd = nullptr;
+        const int64_t bnd_chunk = int64_t(1) << 16;
+        if (!lds_bnd) d_bnd = D.alloc<int32_t>(size_t(std::min<int64_t>(n32, bnd_chunk)) * 2 * size_t(max_m32));
+        if (!D.ok) err = "hipMalloc(dir table) -- batch workspace exceeds device memory; split the batch";
+        for (auto& ch : chunks) {
+            if (err) break;
+            for (int64_t a = ch.first; a < ch.second && !err;) {
+                const int64_t len = std::min(ch.second - a, lds_bnd ? (int64_t(1) << 30) : bnd_chunk);
+                ga::SwLaunch L{};
+                L.txt = d_txt; L.toff = d_toff; L.tlen = d_tl;
+                L.pat = d_pat; L.poff = d_poff; L.plen = d_pl;
+                L.n = int32_t(len); L.max_m = max_m32;
+                L.match = sc.match; L.mismatch = sc.mismatch; L.go = sc.gap_open; L.ge = sc.gap_extend;
+                L.res = d_res; L.dirs = d_dirs; L.dir_off = d_diroff; L.walk = d_walk;
+                L.cigar = d_cig; L.cig_off = d_slot; L.bnd = d_bnd; L.first_pair = int32_t(a);
+                L.n_total = int32_t(n);
+                if (ga::sw_launch(L, trace, st) != hipSuccess) err = "SW kernel launch";
+                a += len;
+            }
+        }
+    }
+
+    // ---- packed kernel: pairs [n32, n), two per wave ------------------------
+    const int64_t npk = n - n32;
+    if (npk > 0 && !err) {
+        const int64_t n_waves = (npk + 1) / 2;
+        // chunk waves by the direction budget (stride from each chunk's largest shape)
+        std::vector<std::pair<int64_t, int64_t>> wchunks;
+        int64_t max_stride = 0;
+        {
+            int64_t w0 = 0, mm = 0, mn = 0;
+            for (int64_t w = 0; w < n_waves; ++w) {
+                int64_t wm = 0, wn = 0;
+                for (int h = 0; h < 2; ++h) {
+                    const int64_t k = n32 + 2 * w + h;
+                    if (k < n) { wm = std::max<int64_t>(wm, tl[size_t(k)]); wn = std::max<int64_t>(wn, pl[size_t(k)]); }
+                }
+                const int64_t nm = std::max(mm, wm), nn = std::max(mn, wn);
+                const int64_t stride = trace ? ga::sw16_dir_words_per_wave(int(nm), int(nn)) : 0;
+                if (w > w0 && stride * (w - w0 + 1) * 4 > kSwDirBudget) {
+                    wchunks.emplace_back(w0, w);
+                    max_stride = std::max(max_stride, trace ? ga::sw16_dir_words_per_wave(int(mm), int(mn)) * (w - w0) : 0);
+                    w0 = w; mm = wm; mn = wn;
+                } else {
+                    mm = nm; mn = nn;
+                }
+            }
+            wchunks.emplace_back(w0, n_waves);
+            max_stride = std::max(max_stride, trace ? ga::sw16_dir_words_per_wave(int(mm), int(mn)) * (n_waves - w0) : 0);
+        }
+        uint32_t* d_dirs16 = trace ? D.alloc<uint32_t>(size_t(max_stride)) : nullptr;
+        int32_t* d_wave_m = D.alloc<int32_t>(size_t(n_waves));
+        if (!D.ok) err = "hipMalloc(dir table) -- batch workspace exceeds device memory; split the batch";
+        for (auto& ch : wchunks) {
+            if (err) break;
+            int32_t cm = 0, cn = 0;
+            for (int64_t k = n32 + 2 * ch.first; k < std::min(n, n32 + 2 * ch.second); ++k) {
+                cm = std::max(cm, tl[size_t(k)]);
+                cn = std::max(cn, pl[size_t(k)]);
+            }
+            ga::Sw16Launch L{};
             L.txt = d_txt; L.toff = d_toff; L.tlen = d_tl;
             L.pat = d_pat; L.poff = d_poff; L.plen = d_pl;
-            L.n = int32_t(len); L.max_m = max_m;
+            L.first_pair = int32_t(n32 + 2 * ch.first); L.n_total = int32_t(n);
+            L.n_waves = int32_t(ch.second - ch.first); L.max_m = cm;
             L.match = sc.match; L.mismatch = sc.mismatch; L.go = sc.gap_open; L.ge = sc.gap_extend;
-            L.res = d_res; L.dirs = d_dirs; L.dir_off = d_diroff; L.walk = d_walk;
-            L.cigar = d_cig; L.cig_off = d_slot; L.bnd = d_bnd; L.first_pair = int32_t(a);
-            if (ga::sw_launch(L, trace, st) != hipSuccess) err = "SW kernel launch";
-            a += len;
+            L.res = d_res; L.rerun = d_rerun;
+            L.dirs = d_dirs16; L.dir_stride = trace ? ga::sw16_dir_words_per_wave(cm, cn) : 0;
+            L.wave_m = d_wave_m; L.walk = d_walk; L.cigar = d_cig; L.cig_off = d_slot;
+            if (enqueue_sw16_chunk(L, trace, st, nullptr) != hipSuccess) err = "SW kernel launch";
         }
     }
-    res8.assign(size_t(n) * 8, -1);
+
+    std::vector<int32_t> r8p(static_cast<size_t>(n) * 8, -1);
     if (!err) {
-        if (hipMemcpyAsync(res8.data(), d_res, size_t(n) * 32, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        if (hipMemcpyAsync(r8p.data(), d_res, size_t(n) * 32, hipMemcpyDeviceToHost, st) != hipSuccess ||
             (trace && slot > 0 &&
              hipMemcpyAsync(arena, d_cig, size_t(slot), hipMemcpyDeviceToHost, st) != hipSuccess))
             err = "hipMemcpy(results)";
@@ -557,6 +660,9 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
     const hipError_t se = hipStreamSynchronize(st);
     if (!err && se != hipSuccess) err = hipGetErrorString(se);
     release_stream(st);
+    res8.assign(static_cast<size_t>(n) * 8, -1);
+    for (int64_t k = 0; k < n; ++k)
+        std::copy(r8p.begin() + k * 8, r8p.begin() + k * 8 + 8, res8.begin() + order[size_t(k)] * 8);
     return err;
 }
 
@@ -769,19 +875,58 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)
     }
     std::call_once(g_pool_once, init_pool);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    // Every pair gets a direction table of the batch's largest shape.
-    const int64_t per = ga::sw_dir_words(b->max_text_len, b->max_pattern_len);
-    const bool lds_bnd = b->max_text_len <= kLdsBndMaxRows;
-    int64_t chunk = b->n_pairs;
-    if (trace) chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, kSwDirBudget / std::max<int64_t>(per * 4, 1)));
-    if (!lds_bnd) chunk = std::min<int64_t>(chunk, int64_t(1) << 16);
     std::vector<void*> owned;
     auto dalloc = [&](void** p, size_t bytes) {
         if (hipMallocAsync(p, std::max<size_t>(bytes, 64), st) != hipSuccess) return false;
         owned.push_back(*p);
         return true;
     };
+    auto release = [&] { for (void* p : owned) (void)hipFreeAsync(p, st); };
     int32_t* d_res = nullptr;
+    if (b->max_text_len <= ga::kSw16MaxRows) {
+        // packed path: pairs (2w, 2w+1) share a wave
+        const int64_t n_waves = (b->n_pairs + 1) / 2;
+        const int64_t stride = trace ? ga::sw16_dir_words_per_wave(b->max_text_len, b->max_pattern_len) : 0;
+        int64_t cw = n_waves;
+        if (trace && stride > 0) cw = std::max<int64_t>(1, std::min<int64_t>(cw, kSwDirBudget / (stride * 4)));
+        uint8_t* d_rerun = nullptr;
+        uint32_t* d_dirs = nullptr;
+        int32_t* d_wave_m = nullptr;
+        bool ok = dalloc(reinterpret_cast<void**>(&d_res), size_t(b->n_pairs) * 32) &&
+                  dalloc(reinterpret_cast<void**>(&d_rerun), size_t(b->n_pairs)) &&
+                  dalloc(reinterpret_cast<void**>(&d_wave_m), size_t(cw) * 4);
+        if (ok && trace) ok = dalloc(reinterpret_cast<void**>(&d_dirs), size_t(cw * stride) * 4);
+        int rc = GA_OK;
+        if (!ok) { if (error) *error = "hipMalloc(SW workspace)"; rc = GA_DEVICE_ERROR; }
+        t_timer.reset();
+        for (int64_t w0 = 0; w0 < n_waves && rc == GA_OK; w0 += cw) {
+            ga::Sw16Launch L{};
+            L.txt = b->text; L.toff = b->text_off; L.tlen = b->text_len;
+            L.pat = b->pattern; L.poff = b->pattern_off; L.plen = b->pattern_len;
+            L.first_pair = int32_t(2 * w0); L.n_total = int32_t(b->n_pairs);
+            L.n_waves = int32_t(std::min(cw, n_waves - w0)); L.max_m = b->max_text_len;
+            L.match = b->scoring.match; L.mismatch = b->scoring.mismatch;
+            L.go = b->scoring.gap_open; L.ge = b->scoring.gap_extend;
+            L.res = d_res; L.rerun = d_rerun; L.dirs = d_dirs; L.dir_stride = stride;
+            L.wave_m = d_wave_m; L.walk = nullptr; L.cigar = b->cigar; L.cig_off = b->cigar_off;
+            if (enqueue_sw16_chunk(L, trace, st, &t_timer_hooks) != hipSuccess) {
+                if (error) *error = "SW kernel launch";
+                rc = GA_DEVICE_ERROR;
+            }
+        }
+        if (rc == GA_OK && ga::sw_gather(d_res, b->n_pairs, trace, b->out, b->cigar_len, st) != hipSuccess) {
+            if (error) *error = "SW gather launch";
+            rc = GA_DEVICE_ERROR;
+        }
+        release();
+        return rc;
+    }
+    // Every pair gets a direction table of the batch's largest shape.
+    const int64_t per = ga::sw_dir_words(b->max_text_len, b->max_pattern_len);
+    const bool lds_bnd = b->max_text_len <= kLdsBndMaxRows;
+    int64_t chunk = b->n_pairs;
+    if (trace) chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, kSwDirBudget / std::max<int64_t>(per * 4, 1)));
+    if (!lds_bnd) chunk = std::min<int64_t>(chunk, int64_t(1) << 16);
     uint32_t* d_dirs = nullptr;
     int32_t* d_bnd = nullptr;
     bool ok = dalloc(reinterpret_cast<void**>(&d_res), size_t(b->n_pairs) * 32);
@@ -800,7 +945,7 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)
         L.go = b->scoring.gap_open; L.ge = b->scoring.gap_extend;
         L.res = d_res; L.dirs = d_dirs; L.dir_off = nullptr; L.dir_stride = per;
         L.walk = nullptr; L.cigar = b->cigar; L.cig_off = b->cigar_off; L.bnd = d_bnd;
-        L.first_pair = int32_t(a);
+        L.first_pair = int32_t(a); L.n_total = int32_t(b->n_pairs);
         if (ga::sw_launch(L, trace, st, &t_timer_hooks) != hipSuccess) {
             if (error) *error = "SW kernel launch";
             rc = GA_DEVICE_ERROR;
@@ -810,7 +955,7 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)
         if (error) *error = "SW gather launch";
         rc = GA_DEVICE_ERROR;
     }
-    for (void* p : owned) (void)hipFreeAsync(p, st);
+    release();
     return rc;
 }
 

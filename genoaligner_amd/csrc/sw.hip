@@ -35,6 +35,25 @@
 namespace ga {
 namespace {
 
+// Re-run mode: the pair held by slot q of a packed launch (-1 when empty).
+__device__ __forceinline__ int slot_pair(const SwLaunch& L, int q)
+{
+    const int w = q >> 1;
+    int pid;
+    if (L.pair_a) pid = (q & 1) ? L.pair_b[w] : L.pair_a[w];
+    else          pid = L.first_pair + q;
+    return pid < L.n_total ? pid : -1;
+}
+
+// First direction word of a pair: explicit offsets, or its packed slot's
+// region in re-run mode, or a fixed stride per block.
+__device__ __forceinline__ int64_t dir_base(const SwLaunch& L, int pid, int q)
+{
+    if (L.dir_off) return L.dir_off[pid];
+    if (L.rerun) return int64_t(q >> 1) * L.dir_stride + int64_t(q & 1) * (L.dir_stride / 2);
+    return int64_t(q) * L.dir_stride;
+}
+
 __device__ __forceinline__ uint64_t load8(const uint8_t* p)
 {
     const uintptr_t a = reinterpret_cast<uintptr_t>(p);
@@ -50,7 +69,11 @@ __global__ __launch_bounds__(64) void sw_kernel(SwLaunch L)
 {
     extern __shared__ int lds[];
     const int slot = blockIdx.x;
-    const int pid = L.first_pair + slot;
+    int pid = L.first_pair + slot;
+    if (L.rerun) {
+        pid = slot_pair(L, slot);
+        if (pid < 0 || !L.rerun[pid]) return;
+    }
     const int lane = threadIdx.x;
     const int m = L.tlen[pid];
     const int n = L.plen[pid];
@@ -74,7 +97,7 @@ __global__ __launch_bounds__(64) void sw_kernel(SwLaunch L)
     const int nstrips = (n + 63) >> 6;
     const int nchunks = (m + 63 + 7) >> 3;          // 8-step chunks per strip
     uint32_t* dirs = nullptr;
-    if (TRACE) dirs = L.dirs + (L.dir_off ? L.dir_off[pid] : int64_t(slot) * L.dir_stride);
+    if (TRACE) dirs = L.dirs + dir_base(L, pid, slot);
     int gH = 0, gI = -1, gJ = -1;
 
     for (int s = 0; s < nstrips; ++s) {
@@ -155,14 +178,18 @@ __global__ __launch_bounds__(64) void sw_walk_kernel(SwLaunch L)
 {
     const int q = int(blockIdx.x) * 64 + int(threadIdx.x);
     if (q >= L.n) return;
-    const int pid = L.first_pair + q;
+    int pid = L.first_pair + q;
+    if (L.rerun) {
+        pid = slot_pair(L, q);
+        if (pid < 0 || !L.rerun[pid]) return;
+    }
     int32_t* res = L.res + int64_t(pid) * 8;
     const int score = res[0];
     if (score <= 0 || (L.walk && !L.walk[pid])) return;
     const int m = L.tlen[pid], n = L.plen[pid];
     const uint8_t* T = L.txt + L.toff[pid];
     const uint8_t* P = L.pat + L.poff[pid];
-    const uint32_t* dirs = L.dirs + (L.dir_off ? L.dir_off[pid] : int64_t(q) * L.dir_stride);
+    const uint32_t* dirs = L.dirs + dir_base(L, pid, q);
     const int nchunks = (m + 63 + 7) >> 3;
     uint8_t* cg = L.cigar + L.cig_off[pid];
     int pos = m + n;

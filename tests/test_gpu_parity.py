@@ -223,3 +223,23 @@ def test_sw_device_resident_embedded(gpu, oracle):
         t, p = ps.a_seq(i), ps.b_seq(i)
         s, ei, ej = oracle.sw_score_lane(t, p, (2, -3, 3, 1))
         assert tuple(int(x) for x in out2[i]) == (s, -1, -1, ei, ej)
+
+
+def test_sw_packed_overflow_rerun(gpu, oracle):
+    """Scores beyond the packed kernel's 16-bit range are re-run in int32 and
+    still match; neighbours in the same wave are unaffected."""
+    import genoaligner_amd as ga
+    rng = random.Random(17)
+    big = "".join(rng.choice("ACGT") for _ in range(4000))
+    sc = (10, -3, 12, 2)
+    pairs = [(big, big), (big[:300], _mutate(rng, big[:300], 0.1)), (big[:3000], big[100:2900])]
+    reqs = [_sreq(ga, t, p, sc) for t, p in pairs]
+    b = ga.align_sw_batch(reqs)
+    want = oracle.align_sw_batch(reqs)
+    assert b.results[0].score == 40000
+    for got, exp in zip(b.results, want.results):
+        assert (got.score, got.cigar, got.start_i, got.end_i, got.start_j, got.end_j) == \
+               (exp.score, exp.cigar, exp.start_i, exp.end_i, exp.start_j, exp.end_j)
+    bs = ga.align_sw_batch([_sreq(ga, t, p, sc, False) for t, p in pairs])
+    for got, (t, p) in zip(bs.results, pairs):
+        assert (got.score, got.end_i, got.end_j) == oracle.sw_score_lane(t, p, sc)
