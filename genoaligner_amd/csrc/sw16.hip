@@ -47,7 +47,7 @@ typedef unsigned short v2u16 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint32_t as_u(v2u16 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ v2u16 as_v(uint32_t x) { return __builtin_bit_cast(v2u16, x); }
 
-// per-half unsigned max, saturating subtract, wrapping add; signed difference
+// per-half unsigned max, saturating subtract and add; signed difference
 __device__ __forceinline__ uint32_t pmax(uint32_t a, uint32_t b)
 {
     return as_u(__builtin_elementwise_max(as_v(a), as_v(b)));
@@ -56,7 +56,12 @@ __device__ __forceinline__ uint32_t psubs(uint32_t a, uint32_t b)
 {
     return as_u(__builtin_elementwise_sub_sat(as_v(a), as_v(b)));
 }
-__device__ __forceinline__ uint32_t padd(uint32_t a, uint32_t b) { return as_u(as_v(a) + as_v(b)); }
+// saturating at 65535, so a pair whose scores outgrow 16 bits keeps a maximum
+// above the re-run threshold instead of wrapping below it
+__device__ __forceinline__ uint32_t padd(uint32_t a, uint32_t b)
+{
+    return as_u(__builtin_elementwise_add_sat(as_v(a), as_v(b)));
+}
 __device__ __forceinline__ uint32_t pdiff(uint32_t a, uint32_t b)   // a - b per half, signed
 {
     return __builtin_bit_cast(uint32_t, __builtin_bit_cast(v2i16, a) - __builtin_bit_cast(v2i16, b));
