@@ -299,6 +299,13 @@ def sw_scheme_error(sc, with_cigar: bool) -> Optional[str]:
     return None
 
 
+def _sc4(sc):
+    """A scoring scheme as a 4-tuple (accepts tuples or objects with fields)."""
+    if hasattr(sc, "match"):
+        return (sc.match, sc.mismatch, sc.gap_open, sc.gap_extend)
+    return tuple(sc)
+
+
 def align_sw_batch(reqs: Sequence) -> SWBatchResult:
     """src/api/api.cpp:584-847 (device path semantics).  Each request has .text
     .text_len .pattern .pattern_len .scoring (4-tuple) .with_cigar."""
@@ -310,16 +317,16 @@ def align_sw_batch(reqs: Sequence) -> SWBatchResult:
     def invalid(msg):
         return SWBatchResult([SWAlignResult() for _ in reqs], status="invalid_argument", error=msg)
 
-    s0 = tuple(reqs[0].scoring)
+    s0 = _sc4(reqs[0].scoring)
     for r in reqs:                                   # api.cpp:600-616
         if (r.text_len > 0 and r.text is None) or (r.pattern_len > 0 and r.pattern is None):
             return invalid("null pointer with non-zero length")
         if r.text_len < 0 or r.pattern_len < 0:
             return invalid("negative length")
-        e = sw_scheme_error(tuple(r.scoring), r.with_cigar)
+        e = sw_scheme_error(_sc4(r.scoring), r.with_cigar)
         if e:
             return invalid(e)
-        if tuple(r.scoring) != s0:
+        if _sc4(r.scoring) != s0:
             return invalid("all requests in an SW batch must share one scoring scheme")
     any_cigar = any(r.with_cigar for r in reqs)
     max_cells = max((r.text_len + 1) * (r.pattern_len + 1) for r in reqs)
