@@ -454,13 +454,19 @@ const char* sw_scheme_error(const ga_sw_scoring& s, bool with_cigar)
 }
 
 // Enqueue the packed 16-bit path for one chunk of waves: the packed forward
-// kernel, the int32 kernel over the pairs it flagged (same direction region),
-// and both walks.  All on `st`, no host synchronisation.
+// kernel, its walk, then the int32 kernel (forward + walk) over the pairs it
+// flagged.  A flagged pair reuses its wave's direction region, which still holds
+// the partner half's packed flags until the packed walk has run -- hence this
+// order.  All on `st`, no host synchronisation.
 hipError_t enqueue_sw16_chunk(const ga::Sw16Launch& P16, bool trace, hipStream_t st,
                               ga::LaunchHooks* hooks)
 {
     hipError_t e = ga::sw16_launch(P16, trace, st, hooks);
     if (e != hipSuccess) return e;
+    if (trace) {
+        e = ga::sw16_walk_launch(P16, st);
+        if (e != hipSuccess) return e;
+    }
     ga::SwLaunch R{};
     R.txt = P16.txt; R.toff = P16.toff; R.tlen = P16.tlen;
     R.pat = P16.pat; R.poff = P16.poff; R.plen = P16.plen;
@@ -470,9 +476,7 @@ hipError_t enqueue_sw16_chunk(const ga::Sw16Launch& P16, bool trace, hipStream_t
     R.walk = P16.walk; R.cigar = P16.cigar; R.cig_off = P16.cig_off; R.bnd = nullptr;
     R.first_pair = P16.first_pair; R.rerun = P16.rerun;
     R.pair_a = P16.pair_a; R.pair_b = P16.pair_b; R.n_total = P16.n_total;
-    e = ga::sw_launch(R, trace, st);
-    if (e != hipSuccess || !trace) return e;
-    return ga::sw16_walk_launch(P16, st);
+    return ga::sw_launch(R, trace, st);
 }
 
 // Device SW over pairs given as pointers; res8 receives 8 ints per pair (see
