@@ -112,8 +112,12 @@ def cpu_baseline_sw(ps, seconds, threads, oracle):
         assert rc == 0
         return time.perf_counter() - t0, int((sub.a_len.astype(np.int64) * sub.b_len).sum())
 
+    # calibrate on growing samples until one takes ~1 s, then size the timed run
     probe = max(threads * 4, 32)
     dt, _ = run(probe)
+    while dt < 1.0 and probe < ps.n:
+        probe = int(min(ps.n, probe * max(2.0, 1.2 / max(dt, 1e-3))))
+        dt, _ = run(probe)
     n = int(min(ps.n, max(probe, probe * seconds / max(dt, 1e-3))))
     dt, cells = run(n)
     return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads,
@@ -146,7 +150,8 @@ def main():
     lib = N.lib()
     lib.ga_set_kernel_timing(1)
     torch.cuda.init()
-    dev_name = torch.cuda.get_device_name(0)
+    props = torch.cuda.get_device_properties(0)
+    dev_name = f"{torch.cuda.get_device_name(0)} ({getattr(props, 'gcnArchName', '?')})"
     log(f"device: {dev_name}")
 
     # ---------------- SW headline ----------------
@@ -217,7 +222,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32",
+        "dtype": "int16",
         "data": "synthetic (seeded)",
         "config": {"workload": "sw_embedded: 50000 pairs, query 256 bp, target 1024 bp with a "
                                "10%-mutated query copy (2:1:1), scoring 2/-3/3/1, score + span + CIGAR",
