@@ -134,6 +134,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-wfa", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a kernel of "
+                         "known traffic, to scale hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -153,6 +156,15 @@ def main():
     props = torch.cuda.get_device_properties(0)
     dev_name = f"{torch.cuda.get_device_name(0)} ({getattr(props, 'gcnArchName', '?')})"
     log(f"device: {dev_name}")
+
+    if args.copy_probe_mib > 0:
+        nbytes = args.copy_probe_mib << 20
+        src = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {nbytes} bytes read + {nbytes} bytes written")
+        del src, dst
 
     # ---------------- SW headline ----------------
     t0 = time.perf_counter()
