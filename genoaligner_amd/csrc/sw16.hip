@@ -70,9 +70,12 @@ __device__ __forceinline__ uint32_t psign_mask(uint32_t d)          // 0xffff wh
 {
     return __builtin_bit_cast(uint32_t, __builtin_bit_cast(v2i16, d) >> short(15));
 }
-__device__ __forceinline__ uint32_t shr1(uint32_t old, uint32_t v)  // lane l <- lane l-1; lane 0 <- old
+// lane l <- lane l-1 (DPP wave_shr:1, lane 0 gets 0) plus `add`, which is the
+// boundary value on lane 0 and 0 on every other lane; folds into one
+// v_add_u32_dpp.
+__device__ __forceinline__ uint32_t shr1(uint32_t add, uint32_t v)
 {
-    return uint32_t(__builtin_amdgcn_update_dpp(int(old), int(v), 0x138, 0xf, 0xf, false));
+    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x138, 0xf, 0xf, true)) + add;
 }
 
 __device__ __forceinline__ int dna_code(uint8_t c)
@@ -93,6 +96,81 @@ __device__ __forceinline__ uint32_t u_table(uint8_t q, int match, int mismatch, 
 
 constexpr uint32_t kSelNone = 0x0c0c0c0cu;   // v_perm selector: both halves u = 0
 
+// LDS layout: selectors at [0, max_m + 136) dwords (64 guard rows before,
+// 72 after), then the boundary column, 16-byte aligned: max_m + 8 rows of uint2
+// followed by 8 zero rows that lanes 1..63 read in place of the boundary.
+__host__ __device__ inline int sw16_bnd_offset(int max_m) { return (max_m + 136 + 3) & ~3; }
+__host__ __device__ inline size_t sw16_lds_bytes(int max_m)
+{
+    return size_t(sw16_bnd_offset(max_m)) * 4 + size_t(max_m + 16) * 8;
+}
+
+// Per-lane state of one strip.
+struct Strip {
+    uint32_t H, Eh, Fh, diag;      // last row's H, Eh, Fh; H of the left column one row up
+    uint32_t bestH, bestT;         // per half: first maximum of this column and its step
+    uint32_t tabA, tabB;           // u tables of this lane's pattern bytes
+    uint32_t colmask;              // 0xffff per half whose column exists
+};
+
+// Four steps t0 .. t0+3 of a strip.  sel4/bnd4 hold this group's selectors and
+// boundary rows (prefetched by the caller).  ROWS: rows past a pair's end are
+// masked out of the maximum (drain tail only).  COLS: columns past a pair's
+// end are masked (partial strips only).
+template <bool TRACE, bool ROWS, bool COLS>
+__device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const uint32_t (&sv)[4],
+                                            const uint2 (&bv)[4], uint32_t go2, uint32_t ge2,
+                                            uint32_t m02, int mAp, int mBp, uint32_t& acc,
+                                            uint32_t (&hout)[4], uint32_t (&eout)[4])
+{
+    uint32_t Hb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const uint32_t Hl = shr1(bv[u].x, S.H);
+        const uint32_t El = shr1(bv[u].y, S.Eh);
+        const uint32_t uu = __builtin_amdgcn_perm(S.tabB, S.tabA, sv[u]);
+        const uint32_t xe = psubs(El, ge2);
+        const uint32_t En = pmax(Hl, xe);
+        const uint32_t xf = psubs(S.Fh, ge2);
+        const uint32_t Fn = pmax(S.H, xf);
+        const uint32_t A1 = psubs(padd(S.diag, uu), m02);
+        const uint32_t A2 = psubs(pmax(En, Fn), go2);
+        const uint32_t Hn = pmax(A1, A2);
+        uint32_t hb = Hn;
+        if (COLS) hb &= S.colmask;
+        if (ROWS) {
+            const int i = t0 + u - lane;
+            hb &= (i < mAp ? 0xffffu : 0u) | (i < mBp ? 0xffff0000u : 0u);
+        }
+        Hb[u] = hb;
+        if (TRACE) {
+            const uint32_t P1 = __builtin_amdgcn_perm(pdiff(En, Fn), pdiff(A1, A2), 0x07050301u);
+            const uint32_t P2 = __builtin_amdgcn_perm(pdiff(S.H, xf), pdiff(Hl, xe), 0x07050301u);
+            const uint32_t Z = (P1 & 0x80808080u) | ((P2 >> 4) & 0x08080808u);
+            acc = (u == 0) ? Z : ((acc >> 1) | Z);
+        }
+        hout[u] = Hn;
+        eout[u] = En;
+        S.diag = Hl;
+        S.H = Hn;
+        S.Eh = En;
+        S.Fh = Fn;
+    }
+    // First maxima: one max per step; the exact per-step update runs only for a
+    // group in which some lane's maximum grew.
+    const uint32_t nb = pmax(pmax(S.bestH, Hb[0]), pmax(pmax(Hb[1], Hb[2]), Hb[3]));
+    if (__any(nb != S.bestH)) {
+        uint32_t b = S.bestH;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t up = psign_mask(pdiff(b, Hb[u]));
+            b = pmax(b, Hb[u]);
+            S.bestT = (up & (uint32_t(t0 + u) * 0x10001u)) | (~up & S.bestT);
+        }
+    }
+    S.bestH = nb;
+}
+
 template <bool TRACE>
 __global__ __launch_bounds__(64) void sw16_kernel(Sw16Launch L)
 {
@@ -112,10 +190,10 @@ __global__ __launch_bounds__(64) void sw16_kernel(Sw16Launch L)
     const uint8_t* TB = ib >= 0 ? L.txt + L.toff[ib] : TA;
     const uint8_t* PB = ib >= 0 ? L.pat + L.poff[ib] : PA;
 
-    // LDS: per-row selectors with 64 guard rows on each side, then the strip
+    // LDS: per-row selectors with guard rows on both sides, then the strip
     // boundary column (H and Eh of both pairs, one uint2 per text row).
     uint32_t* sel = lds16 + 64;
-    uint2* bnd = reinterpret_cast<uint2*>(lds16 + (L.max_m + 136));
+    uint2* bnd = reinterpret_cast<uint2*>(lds16 + sw16_bnd_offset(L.max_m));
     const int go = L.go, ge = L.ge;
     const uint32_t go2 = uint32_t(go) * 0x10001u, ge2 = uint32_t(ge) * 0x10001u;
     const int m0 = L.mismatch < 0 ? -L.mismatch : 0;
@@ -123,16 +201,18 @@ __global__ __launch_bounds__(64) void sw16_kernel(Sw16Launch L)
 
     bool badA = false, badB = false;
     for (int i = lane - 64; i < M + 72; i += 64) {
-        uint32_t s = kSelNone;
+        uint32_t sv = kSelNone;
         if (i >= 0 && i < M) {
             uint32_t lo = 12u, hi = 12u;
             if (i < mAp) { const int c = dna_code(TA[i]); badA |= c < 0; lo = uint32_t(c < 0 ? 12 : c); }
             if (i < mBp) { const int c = dna_code(TB[i]); badB |= c < 0; hi = uint32_t(c < 0 ? 12 : 4 + c); }
-            s = lo | (12u << 8) | (hi << 16) | (12u << 24);
+            sv = lo | (12u << 8) | (hi << 16) | (12u << 24);
         }
-        sel[i] = s;
+        sel[i] = sv;
     }
-    for (int i = lane; i < M; i += 64) bnd[i] = make_uint2(0u, go2);
+    for (int i = lane; i < M + 8; i += 64) bnd[i] = make_uint2(0u, go2);
+    const int zrow = L.max_m + 8;
+    if (lane < 8) bnd[zrow + lane] = make_uint2(0u, 0u);
     for (int j = lane; j < N; j += 64) {
         if (j < nA && mAp > 0) badA |= dna_code(PA[j]) < 0;
         if (j < nB && mBp > 0) badB |= dna_code(PB[j]) < 0;
@@ -145,72 +225,58 @@ __global__ __launch_bounds__(64) void sw16_kernel(Sw16Launch L)
     const int nstrips = (N + 63) >> 6;
     const int nsteps = (M + 63 + 3) & ~3;
     const int nq = nsteps >> 2;
+    const int mmin = mAp < mBp ? mAp : mBp;
+    const int t_main = mmin & ~3;                 // groups [0, t_main) need no row mask
+    const int nmin = nA < nB ? nA : nB;
     uint32_t* dirs = TRACE ? L.dirs + int64_t(w) * L.dir_stride : nullptr;
     int gH[2] = {0, 0}, gI[2] = {-1, -1}, gJ[2] = {-1, -1};
 
     for (int s = 0; s < nstrips; ++s) {
         const int j = (s << 6) + lane;
-        const uint8_t qa = j < nA ? PA[j] : uint8_t(0);
-        const uint8_t qb = j < nB ? PB[j] : uint8_t(0);
-        const uint32_t tabA = u_table(qa, L.match, L.mismatch, m0);
-        const uint32_t tabB = u_table(qb, L.match, L.mismatch, m0);
-        const uint32_t colmask = (j < nA ? 0xffffu : 0u) | (j < nB ? 0xffff0000u : 0u);
-        uint32_t H = 0, Eh = go2, Fh = go2, diag = 0;
-        uint32_t bestH = 0, bestT = 0xffffffffu;
-        for (int t0 = 0; t0 < nsteps; t0 += 4) {
-            uint32_t acc = 0;
+        Strip S;
+        S.tabA = u_table(j < nA ? PA[j] : uint8_t(0), L.match, L.mismatch, m0);
+        S.tabB = u_table(j < nB ? PB[j] : uint8_t(0), L.match, L.mismatch, m0);
+        S.colmask = (j < nA ? 0xffffu : 0u) | (j < nB ? 0xffff0000u : 0u);
+        S.H = 0; S.Eh = go2; S.Fh = go2; S.diag = 0;
+        S.bestH = 0; S.bestT = 0xffffffffu;
+        const bool full = (s << 6) + 64 <= nmin;  // every column of the strip exists in both pairs
+
+        uint32_t sv[4], svn[4];
+        uint2 bv[4], bvn[4];
+        // lane 0 reads the boundary rows, the other lanes the zero rows
+        const uint2* bsrc = lane == 0 ? bnd : bnd + zrow;
+        const int bstep = lane == 0 ? 4 : 0;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int t = t0 + u;
-                const uint2 b = t < M ? bnd[t] : make_uint2(0u, go2);
-                const uint32_t Hl = shr1(b.x, H);
-                const uint32_t El = shr1(b.y, Eh);
-                const uint32_t sv = sel[t - lane];
-                const uint32_t uu = __builtin_amdgcn_perm(tabB, tabA, sv);
-                const uint32_t xe = psubs(El, ge2);
-                const uint32_t En = pmax(Hl, xe);
-                const uint32_t xf = psubs(Fh, ge2);
-                const uint32_t Fn = pmax(H, xf);
-                const uint32_t A1 = psubs(padd(diag, uu), m02);
-                const uint32_t G = pmax(En, Fn);
-                const uint32_t A2 = psubs(G, go2);
-                const uint32_t Hn = pmax(A1, A2);
-                // live cells only: column in range, row in range (rows < 0 hold 0)
-                uint32_t live = colmask;
-                if (t >= M) {
-                    const int i = t - lane;
-                    live &= (i < mAp ? 0xffffu : 0u) | (i < mBp ? 0xffff0000u : 0u);
-                } else if (t >= (mAp < mBp ? mAp : mBp)) {
-                    const int i = t - lane;
-                    live &= (i < mAp ? 0xffffu : 0u) | (i < mBp ? 0xffff0000u : 0u);
-                }
-                const uint32_t Hb = Hn & live;
-                const uint32_t up = psign_mask(pdiff(bestH, Hb));
-                bestH = pmax(bestH, Hb);
-                bestT = (up & (uint32_t(t) * 0x10001u)) | (~up & bestT);
-                if (TRACE) {
-                    const uint32_t f_nd = pdiff(A1, A2);
-                    const uint32_t f_fe = pdiff(En, Fn);
-                    const uint32_t f_ee = pdiff(Hl, xe);
-                    const uint32_t f_fx = pdiff(H, xf);
-                    const uint32_t P1 = __builtin_amdgcn_perm(f_fe, f_nd, 0x07050301u);
-                    const uint32_t P2 = __builtin_amdgcn_perm(f_fx, f_ee, 0x07050301u);
-                    const uint32_t Z = (P1 & 0x80808080u) | ((P2 >> 4) & 0x08080808u);
-                    acc = (u == 0) ? Z : ((acc >> 1) | Z);
-                }
-                if (lane == 63 && t >= 63 && t - 63 < M) bnd[t - 63] = make_uint2(Hn, En);
-                diag = Hl;
-                H = Hn;
-                Eh = En;
-                Fh = Fn;
+        for (int u = 0; u < 4; ++u) { sv[u] = sel[u - lane]; bv[u] = bsrc[u]; }
+        for (int t0 = 0; t0 < nsteps; t0 += 4) {
+            // prefetch the next group (selector guards and 8 spare boundary rows
+            // keep these reads inside the buffers)
+            bsrc += bstep;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) { svn[u] = sel[t0 + 4 + u - lane]; bvn[u] = bsrc[u]; }
+            uint32_t acc = 0, hout[4], eout[4];
+            if (t0 < t_main) {
+                if (full) strip_group<TRACE, false, false>(S, t0, lane, sv, bv, go2, ge2, m02, mAp, mBp, acc, hout, eout);
+                else      strip_group<TRACE, false, true>(S, t0, lane, sv, bv, go2, ge2, m02, mAp, mBp, acc, hout, eout);
+            } else {
+                strip_group<TRACE, true, true>(S, t0, lane, sv, bv, go2, ge2, m02, mAp, mBp, acc, hout, eout);
             }
             if (TRACE) dirs[(int64_t(s) * nq + (t0 >> 2)) * 64 + lane] = acc;
+            if (lane == 63) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int r = t0 + u - 63;
+                    if (r >= 0 && r < M) bnd[r] = make_uint2(hout[u], eout[u]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) { sv[u] = svn[u]; bv[u] = bvn[u]; }
         }
         // fold this strip's first maxima (per half) into the running records
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int bh = int((bestH >> (16 * h)) & 0xffffu);
-            const int bt = int((bestT >> (16 * h)) & 0xffffu);
+            const int bh = int((S.bestH >> (16 * h)) & 0xffffu);
+            const int bt = int((S.bestT >> (16 * h)) & 0xffffu);
             const int bi = bh > 0 ? bt - lane : -1;
             if (bh > gH[h] || (bh == gH[h] && bh > 0 && bi < gI[h])) { gH[h] = bh; gI[h] = bi; gJ[h] = j; }
         }
@@ -327,7 +393,7 @@ __global__ __launch_bounds__(64) void sw16_walk_kernel(Sw16Launch L)
 hipError_t sw16_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHooks* hooks)
 {
     if (L.n_waves <= 0) return hipSuccess;
-    const size_t lds = size_t(L.max_m + 136) * 4 + size_t(L.max_m) * 8;
+    const size_t lds = sw16_lds_bytes(L.max_m);
     const dim3 grid(unsigned(L.n_waves)), block(64);
     if (hooks) hooks->before(st);
     if (trace) hipLaunchKernelGGL(sw16_kernel<true>, grid, block, lds, st, L);
