@@ -73,13 +73,18 @@ struct Sw16Launch {
     int32_t* res;           // 8 ints per pair, as SwLaunch
     uint8_t* rerun;         // out: 1 = needs the int32 kernel
     uint32_t* dirs; int64_t dir_stride;             // words per wave (trace)
+    uint32_t* selbuf;       // per-wave text selector rows (scratch), sw16_sel_stride dwords per wave
     int32_t* wave_m;        // out: per wave, rows streamed (trace walk)
     const uint8_t* walk;    // per pair: emit span + CIGAR; null = all
     uint8_t* cigar; const int64_t* cig_off;
 };
 
-// Largest text length the packed kernel takes (LDS: 12 bytes per row + guards).
+// Largest text length the packed kernel takes (LDS: 8 bytes per text row).
 constexpr int kSw16MaxRows = 4096;
+
+// Selector scratch per wave: 64 guard rows before and 80 after max_m rows
+// (reads reach 63 rows before row 0 and 74 rows past the last row).
+__host__ __device__ inline int64_t sw16_sel_stride(int max_m) { return (int64_t(max_m) + 144 + 3) & ~int64_t(3); }
 
 inline int64_t sw16_dir_words_per_wave(int max_m, int max_n)
 {

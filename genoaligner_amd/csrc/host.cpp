@@ -632,6 +632,11 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
         }
         uint32_t* d_dirs16 = trace ? D.alloc<uint32_t>(size_t(max_stride)) : nullptr;
         int32_t* d_wave_m = D.alloc<int32_t>(size_t(n_waves));
+        int32_t pk_max_m = 0;
+        for (int64_t k = n32; k < n; ++k) pk_max_m = std::max(pk_max_m, tl[size_t(k)]);
+        int64_t max_cw = 0;
+        for (auto& ch : wchunks) max_cw = std::max(max_cw, ch.second - ch.first);
+        uint32_t* d_sel = D.alloc<uint32_t>(size_t(max_cw * ga::sw16_sel_stride(pk_max_m)));
         if (!D.ok) err = "hipMalloc(dir table) -- batch workspace exceeds device memory; split the batch";
         for (auto& ch : wchunks) {
             if (err) break;
@@ -648,6 +653,7 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
             L.match = sc.match; L.mismatch = sc.mismatch; L.go = sc.gap_open; L.ge = sc.gap_extend;
             L.res = d_res; L.rerun = d_rerun;
             L.dirs = d_dirs16; L.dir_stride = trace ? ga::sw16_dir_words_per_wave(cm, cn) : 0;
+            L.selbuf = d_sel;
             L.wave_m = d_wave_m; L.walk = d_walk; L.cigar = d_cig; L.cig_off = d_slot;
             if (enqueue_sw16_chunk(L, trace, st, nullptr) != hipSuccess) err = "SW kernel launch";
         }
@@ -900,6 +906,8 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)
                   dalloc(reinterpret_cast<void**>(&d_rerun), size_t(b->n_pairs)) &&
                   dalloc(reinterpret_cast<void**>(&d_wave_m), size_t(cw) * 4);
         if (ok && trace) ok = dalloc(reinterpret_cast<void**>(&d_dirs), size_t(cw * stride) * 4);
+        uint32_t* d_sel = nullptr;
+        if (ok) ok = dalloc(reinterpret_cast<void**>(&d_sel), size_t(cw * ga::sw16_sel_stride(b->max_text_len)) * 4);
         int rc = GA_OK;
         if (!ok) { if (error) *error = "hipMalloc(SW workspace)"; rc = GA_DEVICE_ERROR; }
         t_timer.reset();
@@ -911,7 +919,7 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)
             L.n_waves = int32_t(std::min(cw, n_waves - w0)); L.max_m = b->max_text_len;
             L.match = b->scoring.match; L.mismatch = b->scoring.mismatch;
             L.go = b->scoring.gap_open; L.ge = b->scoring.gap_extend;
-            L.res = d_res; L.rerun = d_rerun; L.dirs = d_dirs; L.dir_stride = stride;
+            L.res = d_res; L.rerun = d_rerun; L.dirs = d_dirs; L.dir_stride = stride; L.selbuf = d_sel;
             L.wave_m = d_wave_m; L.walk = nullptr; L.cigar = b->cigar; L.cig_off = b->cigar_off;
             if (enqueue_sw16_chunk(L, trace, st, &t_timer_hooks) != hipSuccess) {
                 if (error) *error = "SW kernel launch";

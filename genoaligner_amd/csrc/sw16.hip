@@ -96,14 +96,9 @@ __device__ __forceinline__ uint32_t u_table(uint8_t q, int match, int mismatch, 
 
 constexpr uint32_t kSelNone = 0x0c0c0c0cu;   // v_perm selector: both halves u = 0
 
-// LDS layout: selectors at [0, max_m + 136) dwords (64 guard rows before,
-// 72 after), then the boundary column, 16-byte aligned: max_m + 8 rows of uint2
-// followed by 8 zero rows that lanes 1..63 read in place of the boundary.
-__host__ __device__ inline int sw16_bnd_offset(int max_m) { return (max_m + 136 + 3) & ~3; }
-__host__ __device__ inline size_t sw16_lds_bytes(int max_m)
-{
-    return size_t(sw16_bnd_offset(max_m)) * 4 + size_t(max_m + 16) * 8;
-}
+// LDS layout: the boundary column, max_m + 8 rows of uint2, followed by 8 zero
+// rows that lanes 1..63 read in place of the boundary.
+__host__ __device__ inline size_t sw16_lds_bytes(int max_m) { return size_t(max_m + 16) * 8; }
 
 // Per-lane state of one strip.
 struct Strip {
@@ -174,7 +169,7 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
 template <bool TRACE>
 __global__ __launch_bounds__(64) void sw16_kernel(Sw16Launch L)
 {
-    extern __shared__ uint32_t lds16[];
+    extern __shared__ __align__(16) uint32_t lds16[];
     const int w = blockIdx.x;
     const int lane = threadIdx.x;
     const int ia = L.pair_a ? L.pair_a[w] : L.first_pair + 2 * w;
@@ -190,17 +185,17 @@ __global__ __launch_bounds__(64) void sw16_kernel(Sw16Launch L)
     const uint8_t* TB = ib >= 0 ? L.txt + L.toff[ib] : TA;
     const uint8_t* PB = ib >= 0 ? L.pat + L.poff[ib] : PA;
 
-    // LDS: per-row selectors with guard rows on both sides, then the strip
-    // boundary column (H and Eh of both pairs, one uint2 per text row).
-    uint32_t* sel = lds16 + 64;
-    uint2* bnd = reinterpret_cast<uint2*>(lds16 + sw16_bnd_offset(L.max_m));
+    // Per-row selectors (scratch in HBM/L2, guard rows on both sides) and, in
+    // LDS, the strip boundary column (H and Eh of both pairs per text row).
+    uint32_t* sel = L.selbuf + int64_t(w) * sw16_sel_stride(L.max_m) + 64;
+    uint2* bnd = reinterpret_cast<uint2*>(lds16);
     const int go = L.go, ge = L.ge;
     const uint32_t go2 = uint32_t(go) * 0x10001u, ge2 = uint32_t(ge) * 0x10001u;
     const int m0 = L.mismatch < 0 ? -L.mismatch : 0;
     const uint32_t m02 = uint32_t(m0) * 0x10001u;
 
     bool badA = false, badB = false;
-    for (int i = lane - 64; i < M + 72; i += 64) {
+    for (int i = lane - 64; i < M + 80; i += 64) {
         uint32_t sv = kSelNone;
         if (i >= 0 && i < M) {
             uint32_t lo = 12u, hi = 12u;
@@ -241,19 +236,20 @@ __global__ __launch_bounds__(64) void sw16_kernel(Sw16Launch L)
         S.bestH = 0; S.bestT = 0xffffffffu;
         const bool full = (s << 6) + 64 <= nmin;  // every column of the strip exists in both pairs
 
-        uint32_t sv[4], svn[4];
+        uint32_t sv[4], svn[4], svnn[4];
         uint2 bv[4], bvn[4];
         // lane 0 reads the boundary rows, the other lanes the zero rows
         const uint2* bsrc = lane == 0 ? bnd : bnd + zrow;
         const int bstep = lane == 0 ? 4 : 0;
+        const uint32_t* sp = sel - lane;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) { sv[u] = sel[u - lane]; bv[u] = bsrc[u]; }
+        for (int u = 0; u < 4; ++u) { sv[u] = sp[u]; svn[u] = sp[4 + u]; bv[u] = bsrc[u]; }
         for (int t0 = 0; t0 < nsteps; t0 += 4) {
-            // prefetch the next group (selector guards and 8 spare boundary rows
-            // keep these reads inside the buffers)
+            // prefetch: selectors two groups ahead (HBM/L2), boundary one group
+            // ahead (LDS); guard rows keep every read inside its buffer
             bsrc += bstep;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) { svn[u] = sel[t0 + 4 + u - lane]; bvn[u] = bsrc[u]; }
+            for (int u = 0; u < 4; ++u) { svnn[u] = sp[t0 + 8 + u]; bvn[u] = bsrc[u]; }
             uint32_t acc = 0, hout[4], eout[4];
             if (t0 < t_main) {
                 if (full) strip_group<TRACE, false, false>(S, t0, lane, sv, bv, go2, ge2, m02, mAp, mBp, acc, hout, eout);
@@ -270,7 +266,7 @@ __global__ __launch_bounds__(64) void sw16_kernel(Sw16Launch L)
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) { sv[u] = svn[u]; bv[u] = bvn[u]; }
+            for (int u = 0; u < 4; ++u) { sv[u] = svn[u]; svn[u] = svnn[u]; bv[u] = bvn[u]; }
         }
         // fold this strip's first maxima (per half) into the running records
 #pragma unroll
@@ -317,13 +313,15 @@ __global__ __launch_bounds__(64) void sw16_kernel(Sw16Launch L)
     }
 }
 
-__device__ __forceinline__ uint32_t flag16(const uint32_t* dirs, int nq, int i, int j, int h, int f)
+// The 4 flags of cell (i, j) of half h, in bits 0..3 = nd, fe, ee, fx.
+__device__ __forceinline__ uint32_t flags16(const uint32_t* dirs, int nq, int i, int j, int h)
 {
     const int j0 = j - 1, s = j0 >> 6, l = j0 & 63, t = (i - 1) + l;
     const uint32_t w = dirs[(int64_t(s) * nq + (t >> 2)) * 64 + l];
-    const int byte = (f == 0 || f == 2) ? h : 2 + h;
-    const int bit = (f < 2) ? 4 + (t & 3) : (t & 3);
-    return (w >> (8 * byte + bit)) & 1u;
+    const int u = t & 3;
+    const uint32_t lo = w >> (8 * h), hi = w >> (8 * (2 + h));   // bytes h (nd, ee) and 2+h (fe, fx)
+    return ((lo >> (4 + u)) & 1u) | (((hi >> (4 + u)) & 1u) << 1) |
+           (((lo >> u) & 1u) << 2) | (((hi >> u) & 1u) << 3);
 }
 
 // One thread per pair: the reference's three-state walk driven by the 4 flags
@@ -349,35 +347,36 @@ __global__ __launch_bounds__(64) void sw16_walk_kernel(Sw16Launch L)
     int pos = m + n;
     int i = res[1] + 1, j = res[2] + 1, state = 0, v = score;
     while (i > 0 && j > 0 && pos > 0) {
+        if (state == 0 && v == 0) break;
+        const uint32_t f = flags16(dirs, nq, i, j, h);
         if (state == 0) {
-            if (v == 0) break;
-            if (!flag16(dirs, nq, i, j, h, 0)) {
+            if (!(f & 1u)) {
                 const bool eq = T[i - 1] == P[j - 1];
                 cg[--pos] = eq ? 'M' : 'X';
                 v -= eq ? L.match : L.mismatch;
                 --i; --j;
-            } else if (!flag16(dirs, nq, i, j, h, 1)) {
+            } else if (!(f & 2u)) {
                 cg[--pos] = 'D';
-                const bool ext = flag16(dirs, nq, i, j, h, 2);
+                const bool ext = f & 4u;
                 state = ext ? 1 : 0;
                 v += ext ? L.ge : L.go;
                 --j;
             } else {
                 cg[--pos] = 'I';
-                const bool ext = flag16(dirs, nq, i, j, h, 3);
+                const bool ext = f & 8u;
                 state = ext ? 2 : 0;
                 v += ext ? L.ge : L.go;
                 --i;
             }
         } else if (state == 1) {
             cg[--pos] = 'D';
-            const bool ext = flag16(dirs, nq, i, j, h, 2);
+            const bool ext = f & 4u;
             if (!ext) state = 0;
             v += ext ? L.ge : L.go;
             --j;
         } else {
             cg[--pos] = 'I';
-            const bool ext = flag16(dirs, nq, i, j, h, 3);
+            const bool ext = f & 8u;
             if (!ext) state = 0;
             v += ext ? L.ge : L.go;
             --i;
