@@ -453,20 +453,17 @@ const char* sw_scheme_error(const ga_sw_scoring& s, bool with_cigar)
     return nullptr;
 }
 
-// Enqueue the packed 16-bit path for one chunk of waves: the packed forward
-// kernel, its walk, then the int32 kernel (forward + walk) over the pairs it
+// Enqueue the packed 16-bit path for one chunk of waves: the packed kernel
+// (forward and walk), then the int32 kernel (forward + walk) over the pairs it
 // flagged.  A flagged pair reuses its wave's direction region, which still holds
 // the partner half's packed flags until the packed walk has run -- hence this
 // order.  All on `st`, no host synchronisation.
 hipError_t enqueue_sw16_chunk(const ga::Sw16Launch& P16, bool trace, hipStream_t st,
                               ga::LaunchHooks* hooks)
 {
+    // (the packed kernel's trace variant walks its own pairs before it exits)
     hipError_t e = ga::sw16_launch(P16, trace, st, hooks);
     if (e != hipSuccess) return e;
-    if (trace) {
-        e = ga::sw16_walk_launch(P16, st);
-        if (e != hipSuccess) return e;
-    }
     ga::SwLaunch R{};
     R.txt = P16.txt; R.toff = P16.toff; R.tlen = P16.tlen;
     R.pat = P16.pat; R.poff = P16.poff; R.plen = P16.plen;

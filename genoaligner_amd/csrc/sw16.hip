@@ -100,6 +100,90 @@ constexpr uint32_t kSelNone = 0x0c0c0c0cu;   // v_perm selector: both halves u =
 // rows that lanes 1..63 read in place of the boundary.
 __host__ __device__ inline size_t sw16_lds_bytes(int max_m) { return size_t(max_m + 16) * 8; }
 
+// The 4 flags of cell (i, j) of half h, in bits 0..3 = nd, fe, ee, fx.
+__device__ __forceinline__ uint32_t flags16(const uint32_t* dirs, int nq, int i, int j, int h)
+{
+    const int j0 = j - 1, s = j0 >> 6, l = j0 & 63, t = (i - 1) + l;
+    const uint32_t w = dirs[(int64_t(s) * nq + (t >> 2)) * 64 + l];
+    const int u = t & 3;
+    const uint32_t lo = w >> (8 * h), hi = w >> (8 * (2 + h));   // bytes h (nd, ee) and 2+h (fe, fx)
+    return ((lo >> (4 + u)) & 1u) | (((hi >> (4 + u)) & 1u) << 1) |
+           (((lo >> u) & 1u) << 2) | (((hi >> u) & 1u) << 3);
+}
+
+// The reference's three-state walk (sw_kernel_impl.hip:198-242) driven by the 4
+// flags and the carried cell value, from the 1-based end cell (i, j) with H =
+// score.  Writes the CIGAR right-aligned into cg[0 .. m+n) and returns its
+// length; *si, *sj receive the 0-based start.
+__device__ int walk16(const uint32_t* dirs, int nq, int h, const uint8_t* T, const uint8_t* P,
+                      int m, int n, int i, int j, int score, int match, int mismatch, int go,
+                      int ge, uint8_t* cg, int* si, int* sj)
+{
+    int pos = m + n, state = 0, v = score;
+    while (i > 0 && j > 0 && pos > 0) {
+        if (state == 0 && v == 0) break;
+        const uint32_t f = flags16(dirs, nq, i, j, h);
+        if (state == 0) {
+            if (!(f & 1u)) {
+                const bool eq = T[i - 1] == P[j - 1];
+                cg[--pos] = eq ? 'M' : 'X';
+                v -= eq ? match : mismatch;
+                --i; --j;
+            } else if (!(f & 2u)) {
+                cg[--pos] = 'D';
+                const bool ext = f & 4u;
+                state = ext ? 1 : 0;
+                v += ext ? ge : go;
+                --j;
+            } else {
+                cg[--pos] = 'I';
+                const bool ext = f & 8u;
+                state = ext ? 2 : 0;
+                v += ext ? ge : go;
+                --i;
+            }
+        } else if (state == 1) {
+            cg[--pos] = 'D';
+            const bool ext = f & 4u;
+            if (!ext) state = 0;
+            v += ext ? ge : go;
+            --j;
+        } else {
+            cg[--pos] = 'I';
+            const bool ext = f & 8u;
+            if (!ext) state = 0;
+            v += ext ? ge : go;
+            --i;
+        }
+    }
+    *si = i;
+    *sj = j;
+    return m + n - pos;
+}
+
+// Stand-alone walk over 2 * n_waves slots (kept for launches that separate it).
+__global__ __launch_bounds__(64) void sw16_walk_kernel(Sw16Launch L)
+{
+    const int q = int(blockIdx.x) * 64 + int(threadIdx.x);
+    if (q >= 2 * L.n_waves) return;
+    const int w = q >> 1, h = q & 1;
+    int pid = L.pair_a ? (h ? L.pair_b[w] : L.pair_a[w]) : L.first_pair + q;
+    if (pid >= L.n_total || pid < 0) return;
+    if (L.rerun[pid]) return;
+    int32_t* res = L.res + int64_t(pid) * 8;
+    const int score = res[0];
+    if (score <= 0 || (L.walk && !L.walk[pid])) return;
+    const int Mw = L.wave_m[w];
+    const int nq = ((Mw + 63 + 3) & ~3) >> 2;
+    int si, sj;
+    const int len = walk16(L.dirs + int64_t(w) * L.dir_stride, nq, h, L.txt + L.toff[pid],
+                           L.pat + L.poff[pid], L.tlen[pid], L.plen[pid], res[1] + 1, res[2] + 1,
+                           score, L.match, L.mismatch, L.go, L.ge, L.cigar + L.cig_off[pid], &si, &sj);
+    res[5] = si;
+    res[6] = sj;
+    res[7] = len;
+}
+
 // Per-lane state of one strip.
 struct Strip {
     uint32_t H, Eh, Fh, diag;      // last row's H, Eh, Fh; H of the left column one row up
@@ -280,6 +364,8 @@ __global__ __launch_bounds__(64) void sw16_kernel(Sw16Launch L)
     }
 
     const int limit = 32767 - 255 - go;
+    int endI[2] = {-1, -1}, endJ[2] = {-1, -1}, endS[2] = {0, 0};
+    bool redo[2] = {false, false};
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int pid = h == 0 ? ia : ib;
@@ -294,9 +380,11 @@ __global__ __launch_bounds__(64) void sw16_kernel(Sw16Launch L)
                 rH = rb; rI = ri; rJ = rj;
             }
         }
+        endI[h] = rI; endJ[h] = rJ; endS[h] = lH;
+        redo[h] = (h == 0 ? badA : badB) || lH >= limit;
         if (lane == 0) {
             int32_t* res = L.res + int64_t(pid) * 8;
-            const bool rerun = (h == 0 ? badA : badB) || lH >= limit;
+            const bool rerun = redo[h];
             const int m = h == 0 ? mA : mB, n = h == 0 ? nA : nB;
             if (m <= 0 || n <= 0) {
                 res[0] = 0;
@@ -311,80 +399,26 @@ __global__ __launch_bounds__(64) void sw16_kernel(Sw16Launch L)
             }
         }
     }
-}
-
-// The 4 flags of cell (i, j) of half h, in bits 0..3 = nd, fe, ee, fx.
-__device__ __forceinline__ uint32_t flags16(const uint32_t* dirs, int nq, int i, int j, int h)
-{
-    const int j0 = j - 1, s = j0 >> 6, l = j0 & 63, t = (i - 1) + l;
-    const uint32_t w = dirs[(int64_t(s) * nq + (t >> 2)) * 64 + l];
-    const int u = t & 3;
-    const uint32_t lo = w >> (8 * h), hi = w >> (8 * (2 + h));   // bytes h (nd, ee) and 2+h (fe, fx)
-    return ((lo >> (4 + u)) & 1u) | (((hi >> (4 + u)) & 1u) << 1) |
-           (((lo >> u) & 1u) << 2) | (((hi >> u) & 1u) << 3);
-}
-
-// One thread per pair: the reference's three-state walk driven by the 4 flags
-// and the carried cell value.
-__global__ __launch_bounds__(64) void sw16_walk_kernel(Sw16Launch L)
-{
-    const int q = int(blockIdx.x) * 64 + int(threadIdx.x);
-    if (q >= 2 * L.n_waves) return;
-    const int w = q >> 1, h = q & 1;
-    int pid = L.pair_a ? (h ? L.pair_b[w] : L.pair_a[w]) : L.first_pair + q;
-    if (pid >= L.n_total || pid < 0) return;
-    if (L.rerun[pid]) return;
-    int32_t* res = L.res + int64_t(pid) * 8;
-    const int score = res[0];
-    if (score <= 0 || (L.walk && !L.walk[pid])) return;
-    const int m = L.tlen[pid], n = L.plen[pid];
-    const uint8_t* T = L.txt + L.toff[pid];
-    const uint8_t* P = L.pat + L.poff[pid];
-    const int Mw = L.wave_m[w];
-    const int nq = ((Mw + 63 + 3) & ~3) >> 2;
-    const uint32_t* dirs = L.dirs + int64_t(w) * L.dir_stride;
-    uint8_t* cg = L.cigar + L.cig_off[pid];
-    int pos = m + n;
-    int i = res[1] + 1, j = res[2] + 1, state = 0, v = score;
-    while (i > 0 && j > 0 && pos > 0) {
-        if (state == 0 && v == 0) break;
-        const uint32_t f = flags16(dirs, nq, i, j, h);
-        if (state == 0) {
-            if (!(f & 1u)) {
-                const bool eq = T[i - 1] == P[j - 1];
-                cg[--pos] = eq ? 'M' : 'X';
-                v -= eq ? L.match : L.mismatch;
-                --i; --j;
-            } else if (!(f & 2u)) {
-                cg[--pos] = 'D';
-                const bool ext = f & 4u;
-                state = ext ? 1 : 0;
-                v += ext ? L.ge : L.go;
-                --j;
-            } else {
-                cg[--pos] = 'I';
-                const bool ext = f & 8u;
-                state = ext ? 2 : 0;
-                v += ext ? L.ge : L.go;
-                --i;
-            }
-        } else if (state == 1) {
-            cg[--pos] = 'D';
-            const bool ext = f & 4u;
-            if (!ext) state = 0;
-            v += ext ? L.ge : L.go;
-            --j;
-        } else {
-            cg[--pos] = 'I';
-            const bool ext = f & 8u;
-            if (!ext) state = 0;
-            v += ext ? L.ge : L.go;
-            --i;
+    if (!TRACE) return;
+    // The walks, fused: lane 0 walks pair A, lane 32 pair B, while the wave's
+    // slot would otherwise sit idle; their load latency overlaps the forward
+    // work of the other waves on the SIMD.
+    __syncthreads();
+    if ((lane & 31) == 0) {
+        const int h = lane >> 5;
+        const int pid = h == 0 ? ia : ib;
+        if (pid >= 0 && !redo[h] && endS[h] > 0 && (!L.walk || L.walk[pid])) {
+            const int m = h == 0 ? mA : mB, n = h == 0 ? nA : nB;
+            int si, sj;
+            const int len = walk16(dirs, nq, h, h == 0 ? TA : TB, h == 0 ? PA : PB, m, n,
+                                   endI[h] + 1, endJ[h] + 1, endS[h], L.match, L.mismatch, go, ge,
+                                   L.cigar + L.cig_off[pid], &si, &sj);
+            int32_t* res = L.res + int64_t(pid) * 8;
+            res[5] = si;
+            res[6] = sj;
+            res[7] = len;
         }
     }
-    res[5] = i;
-    res[6] = j;
-    res[7] = m + n - pos;
 }
 
 }  // namespace
