@@ -69,6 +69,7 @@ struct Sw16Launch {
     int32_t n_total;        // pairs in the arrays (a half whose pair id >= n_total is empty)
     int32_t n_waves;
     int32_t max_m;          // LDS sizing: max text_len over the launch's pairs
+    int32_t max_n;          //   and max pattern_len (trace: staging for the walk)
     int32_t match, mismatch, go, ge;
     int32_t* res;           // 8 ints per pair, as SwLaunch
     uint8_t* rerun;         // out: 1 = needs the int32 kernel

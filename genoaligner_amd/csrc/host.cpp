@@ -646,7 +646,7 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
             L.txt = d_txt; L.toff = d_toff; L.tlen = d_tl;
             L.pat = d_pat; L.poff = d_poff; L.plen = d_pl;
             L.first_pair = int32_t(n32 + 2 * ch.first); L.n_total = int32_t(n);
-            L.n_waves = int32_t(ch.second - ch.first); L.max_m = cm;
+            L.n_waves = int32_t(ch.second - ch.first); L.max_m = cm; L.max_n = cn;
             L.match = sc.match; L.mismatch = sc.mismatch; L.go = sc.gap_open; L.ge = sc.gap_extend;
             L.res = d_res; L.rerun = d_rerun;
             L.dirs = d_dirs16; L.dir_stride = trace ? ga::sw16_dir_words_per_wave(cm, cn) : 0;
@@ -914,6 +914,7 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)
             L.pat = b->pattern; L.poff = b->pattern_off; L.plen = b->pattern_len;
             L.first_pair = int32_t(2 * w0); L.n_total = int32_t(b->n_pairs);
             L.n_waves = int32_t(std::min(cw, n_waves - w0)); L.max_m = b->max_text_len;
+            L.max_n = b->max_pattern_len;
             L.match = b->scoring.match; L.mismatch = b->scoring.mismatch;
             L.go = b->scoring.gap_open; L.ge = b->scoring.gap_extend;
             L.res = d_res; L.rerun = d_rerun; L.dirs = d_dirs; L.dir_stride = stride; L.selbuf = d_sel;

@@ -98,7 +98,12 @@ constexpr uint32_t kSelNone = 0x0c0c0c0cu;   // v_perm selector: both halves u =
 
 // LDS layout: the boundary column, max_m + 8 rows of uint2, followed by 8 zero
 // rows that lanes 1..63 read in place of the boundary.
-__host__ __device__ inline size_t sw16_lds_bytes(int max_m) { return size_t(max_m + 16) * 8; }
+__host__ __device__ inline size_t sw16_lds_bytes(int max_m, int max_n)
+{
+    const size_t bnd = size_t(max_m + 16) * 8;
+    const size_t stage = size_t(4) * size_t(max_m + max_n) + 16;   // trace: 2 CIGAR slots + 2 pairs
+    return bnd > stage ? bnd : stage;
+}
 
 // The 4 flags of cell (i, j) of half h, in bits 0..3 = nd, fe, ee, fx.
 __device__ __forceinline__ uint32_t flags16(const uint32_t* dirs, int nq, int i, int j, int h)
@@ -113,48 +118,41 @@ __device__ __forceinline__ uint32_t flags16(const uint32_t* dirs, int nq, int i,
 
 // The reference's three-state walk (sw_kernel_impl.hip:198-242) driven by the 4
 // flags and the carried cell value, from the 1-based end cell (i, j) with H =
-// score.  Writes the CIGAR right-aligned into cg[0 .. m+n) and returns its
-// length; *si, *sj receive the 0-based start.
-__device__ int walk16(const uint32_t* dirs, int nq, int h, const uint8_t* T, const uint8_t* P,
-                      int m, int n, int i, int j, int score, int match, int mismatch, int go,
-                      int ge, uint8_t* cg, int* si, int* sj)
+// score.  T, P and cg may live in LDS or global memory.  Writes the CIGAR
+// right-aligned into cg[0 .. m+n) and returns its length; *si, *sj receive the
+// 0-based start.  The flag word of the diagonal successor is loaded one step
+// ahead, since most moves of a local alignment are diagonal.
+template <class CP, class SP>
+__device__ int walk16(const uint32_t* dirs, int nq, int h, SP T, SP P, int m, int n, int i, int j,
+                      int score, int match, int mismatch, int go, int ge, CP cg, int* si, int* sj)
 {
     int pos = m + n, state = 0, v = score;
+    uint32_t f = (i > 0 && j > 0) ? flags16(dirs, nq, i, j, h) : 0u;
     while (i > 0 && j > 0 && pos > 0) {
         if (state == 0 && v == 0) break;
-        const uint32_t f = flags16(dirs, nq, i, j, h);
-        if (state == 0) {
-            if (!(f & 1u)) {
-                const bool eq = T[i - 1] == P[j - 1];
-                cg[--pos] = eq ? 'M' : 'X';
-                v -= eq ? match : mismatch;
-                --i; --j;
-            } else if (!(f & 2u)) {
-                cg[--pos] = 'D';
-                const bool ext = f & 4u;
-                state = ext ? 1 : 0;
-                v += ext ? ge : go;
-                --j;
-            } else {
-                cg[--pos] = 'I';
-                const bool ext = f & 8u;
-                state = ext ? 2 : 0;
-                v += ext ? ge : go;
-                --i;
-            }
-        } else if (state == 1) {
+        const uint32_t fd = (i > 1 && j > 1) ? flags16(dirs, nq, i - 1, j - 1, h) : 0u;
+        if (state == 0 && !(f & 1u)) {
+            const bool eq = T[i - 1] == P[j - 1];
+            cg[--pos] = eq ? 'M' : 'X';
+            v -= eq ? match : mismatch;
+            --i; --j;
+            f = fd;
+            continue;
+        }
+        if (state == 1 || (state == 0 && !(f & 2u))) {
             cg[--pos] = 'D';
             const bool ext = f & 4u;
-            if (!ext) state = 0;
+            state = ext ? 1 : 0;
             v += ext ? ge : go;
             --j;
         } else {
             cg[--pos] = 'I';
             const bool ext = f & 8u;
-            if (!ext) state = 0;
+            state = ext ? 2 : 0;
             v += ext ? ge : go;
             --i;
         }
+        if (i > 0 && j > 0) f = flags16(dirs, nq, i, j, h);
     }
     *si = i;
     *sj = j;
@@ -402,22 +400,48 @@ __global__ __launch_bounds__(64) void sw16_kernel(Sw16Launch L)
     if (!TRACE) return;
     // The walks, fused: lane 0 walks pair A, lane 32 pair B, while the wave's
     // slot would otherwise sit idle; their load latency overlaps the forward
-    // work of the other waves on the SIMD.
+    // work of the other waves on the SIMD.  Both pairs' sequences and CIGARs are
+    // staged in the LDS (free once the strips are done): the walk's only global
+    // accesses are the flag words, and the CIGARs leave in coalesced stores.
     __syncthreads();
+    uint8_t* st8 = reinterpret_cast<uint8_t*>(lds16);
+    const int slotA = mA + nA, slotB = mB + nB;
+    uint8_t* cgA = st8;
+    uint8_t* cgB = cgA + slotA;
+    uint8_t* sTA = cgB + slotB;
+    uint8_t* sPA = sTA + mA;
+    uint8_t* sTB = sPA + nA;
+    uint8_t* sPB = sTB + mB;
+    for (int q = lane; q < mA; q += 64) sTA[q] = TA[q];
+    for (int q = lane; q < nA; q += 64) sPA[q] = PA[q];
+    for (int q = lane; q < mB; q += 64) sTB[q] = TB[q];
+    for (int q = lane; q < nB; q += 64) sPB[q] = PB[q];
+    __syncthreads();
+    int len_h = 0;
     if ((lane & 31) == 0) {
         const int h = lane >> 5;
         const int pid = h == 0 ? ia : ib;
         if (pid >= 0 && !redo[h] && endS[h] > 0 && (!L.walk || L.walk[pid])) {
             const int m = h == 0 ? mA : mB, n = h == 0 ? nA : nB;
             int si, sj;
-            const int len = walk16(dirs, nq, h, h == 0 ? TA : TB, h == 0 ? PA : PB, m, n,
-                                   endI[h] + 1, endJ[h] + 1, endS[h], L.match, L.mismatch, go, ge,
-                                   L.cigar + L.cig_off[pid], &si, &sj);
+            len_h = walk16(dirs, nq, h, h == 0 ? sTA : sTB, h == 0 ? sPA : sPB, m, n,
+                           endI[h] + 1, endJ[h] + 1, endS[h], L.match, L.mismatch, go, ge,
+                           h == 0 ? cgA : cgB, &si, &sj);
             int32_t* res = L.res + int64_t(pid) * 8;
             res[5] = si;
             res[6] = sj;
-            res[7] = len;
+            res[7] = len_h;
         }
+    }
+    __syncthreads();
+    const int lenA = __shfl(len_h, 0), lenB = __shfl(len_h, 32);
+    if (lenA > 0) {
+        uint8_t* dst = L.cigar + L.cig_off[ia] + (slotA - lenA);
+        for (int q = lane; q < lenA; q += 64) dst[q] = cgA[slotA - lenA + q];
+    }
+    if (lenB > 0) {
+        uint8_t* dst = L.cigar + L.cig_off[ib] + (slotB - lenB);
+        for (int q = lane; q < lenB; q += 64) dst[q] = cgB[slotB - lenB + q];
     }
 }
 
@@ -426,7 +450,7 @@ __global__ __launch_bounds__(64) void sw16_kernel(Sw16Launch L)
 hipError_t sw16_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHooks* hooks)
 {
     if (L.n_waves <= 0) return hipSuccess;
-    const size_t lds = sw16_lds_bytes(L.max_m);
+    const size_t lds = sw16_lds_bytes(L.max_m, trace ? L.max_n : 0);
     const dim3 grid(unsigned(L.n_waves)), block(64);
     if (hooks) hooks->before(st);
     if (trace) hipLaunchKernelGGL(sw16_kernel<true>, grid, block, lds, st, L);
