@@ -120,23 +120,26 @@ __device__ __forceinline__ uint32_t flags16(const uint32_t* dirs, int nq, int i,
 // flags and the carried cell value, from the 1-based end cell (i, j) with H =
 // score.  T, P and cg may live in LDS or global memory.  Writes the CIGAR
 // right-aligned into cg[0 .. m+n) and returns its length; *si, *sj receive the
-// 0-based start.  The flag word of the diagonal successor is loaded one step
+// 0-based start.  Flag words along the current diagonal are loaded four steps
 // ahead, since most moves of a local alignment are diagonal.
 template <class CP, class SP>
 __device__ int walk16(const uint32_t* dirs, int nq, int h, SP T, SP P, int m, int n, int i, int j,
                       int score, int match, int mismatch, int go, int ge, CP cg, int* si, int* sj)
 {
+    // fq[k] = flags of (i-k, j-k): a window along the current diagonal, so a
+    // diagonal run waits for one load every 4 steps
+    auto fl = [&](int a, int b) -> uint32_t { return (a > 0 && b > 0) ? flags16(dirs, nq, a, b, h) : 0u; };
+    uint32_t fq0 = fl(i, j), fq1 = fl(i - 1, j - 1), fq2 = fl(i - 2, j - 2), fq3 = fl(i - 3, j - 3);
     int pos = m + n, state = 0, v = score;
-    uint32_t f = (i > 0 && j > 0) ? flags16(dirs, nq, i, j, h) : 0u;
     while (i > 0 && j > 0 && pos > 0) {
         if (state == 0 && v == 0) break;
-        const uint32_t fd = (i > 1 && j > 1) ? flags16(dirs, nq, i - 1, j - 1, h) : 0u;
+        const uint32_t f = fq0;
         if (state == 0 && !(f & 1u)) {
             const bool eq = T[i - 1] == P[j - 1];
             cg[--pos] = eq ? 'M' : 'X';
             v -= eq ? match : mismatch;
             --i; --j;
-            f = fd;
+            fq0 = fq1; fq1 = fq2; fq2 = fq3; fq3 = fl(i - 3, j - 3);
             continue;
         }
         if (state == 1 || (state == 0 && !(f & 2u))) {
@@ -152,7 +155,7 @@ __device__ int walk16(const uint32_t* dirs, int nq, int h, SP T, SP P, int m, in
             v += ext ? ge : go;
             --i;
         }
-        if (i > 0 && j > 0) f = flags16(dirs, nq, i, j, h);
+        fq0 = fl(i, j); fq1 = fl(i - 1, j - 1); fq2 = fl(i - 2, j - 2); fq3 = fl(i - 3, j - 3);
     }
     *si = i;
     *sj = j;
