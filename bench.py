@@ -190,6 +190,27 @@ def main():
                 "traffic": None, "kernel": "sw_kernel<trace> (forward)",
                 "kernel_ms": round(kern_ms, 3)}
 
+    # score-only on the same pairs: score + end cell (the lane-rule end)
+    swd_s = SwDeviceBatch(sw, scoring=SW_SCHEME, trace=False)
+    swall, sstep, skern = time_steps(swd_s.run, args.steps, args.warmup, lib)
+    sk = float(np.mean([k for k in skern if k > 0])) if any(k > 0 for k in skern) else None
+    out_s, _ = swd_s.results()
+    s_bad = 0
+    for i in sample[:100]:
+        s_, ei, ej = O.sw_score_lane(sw.a_seq(i), sw.b_seq(i), SW_SCHEME)
+        if tuple(int(x) for x in out_s[i]) != (s_, -1, -1, ei, ej):
+            s_bad += 1
+    sw_score_line = {
+        "workload": "same pairs, score + end cell only",
+        "gcups": round(cells / (swall / args.steps) / 1e9, 3),
+        "ms_per_step": round(swall * 1000.0 / args.steps, 3),
+        "kernel_ms": round(sk, 4) if sk else None,
+        "roofline_frac": round(SW_OPS_PER_CELL * cells / (sk * 1e-3) / 1e12 / PEAK_INT16_TOPS, 4) if sk else None,
+        "mismatches": s_bad, "checked": 100,
+    }
+    log(f"sw score-only: {sw_score_line}")
+    del swd_s
+
     # ---------------- WFA secondary ----------------
     wfa_line = None
     if not args.no_wfa:
@@ -245,6 +266,7 @@ def main():
         "checked": int(len(sample)),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "sw_score_only": sw_score_line,
         "wfa": wfa_line,
     }
     print(json.dumps(line), flush=True)

@@ -251,8 +251,10 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
     S.bestH = nb;
 }
 
+// At most 96 VGPRs (5 waves per SIMD) for the trace variant.
 template <bool TRACE>
-__global__ __launch_bounds__(64) void sw16_kernel(Sw16Launch L)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TRACE ? 5 : 6)))
+void sw16_kernel(Sw16Launch L)
 {
     extern __shared__ __align__(16) uint32_t lds16[];
     const int w = blockIdx.x;
@@ -323,18 +325,22 @@ __global__ __launch_bounds__(64) void sw16_kernel(Sw16Launch L)
 
         uint32_t sv[4], svn[4], svnn[4];
         uint2 bv[4], bvn[4];
-        // lane 0 reads the boundary rows, the other lanes the zero rows
-        const uint2* bsrc = lane == 0 ? bnd : bnd + zrow;
-        const int bstep = lane == 0 ? 4 : 0;
+        // Lane 0 reads the boundary rows (rows past the text repeat row M's
+        // spare entries, values that only reach drained cells); the other lanes
+        // read the zero rows.
         const uint32_t* sp = sel - lane;
+        const int brow_max = M + 4;                // rows M .. M+7 are spare
 #pragma unroll
-        for (int u = 0; u < 4; ++u) { sv[u] = sp[u]; svn[u] = sp[4 + u]; bv[u] = bsrc[u]; }
+        for (int u = 0; u < 4; ++u) {
+            sv[u] = sp[u]; svn[u] = sp[4 + u];
+            bv[u] = bnd[lane == 0 ? u : zrow + u];
+        }
         for (int t0 = 0; t0 < nsteps; t0 += 4) {
             // prefetch: selectors two groups ahead (HBM/L2), boundary one group
             // ahead (LDS); guard rows keep every read inside its buffer
-            bsrc += bstep;
+            const int brow = lane == 0 ? min(t0 + 4, brow_max) : zrow;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) { svnn[u] = sp[t0 + 8 + u]; bvn[u] = bsrc[u]; }
+            for (int u = 0; u < 4; ++u) { svnn[u] = sp[t0 + 8 + u]; bvn[u] = bnd[brow + u]; }
             uint32_t acc = 0, hout[4], eout[4];
             if (t0 < t_main) {
                 if (full) strip_group<TRACE, false, false>(S, t0, lane, sv, bv, go2, ge2, m02, mAp, mBp, acc, hout, eout);
