@@ -70,12 +70,16 @@ __device__ __forceinline__ uint32_t psign_mask(uint32_t d)          // 0xffff wh
 {
     return __builtin_bit_cast(uint32_t, __builtin_bit_cast(v2i16, d) >> short(15));
 }
-// lane l <- lane l-1 (DPP wave_shr:1, lane 0 gets 0) plus `add`, which is the
-// boundary value on lane 0 and 0 on every other lane; folds into one
-// v_add_u32_dpp.
-__device__ __forceinline__ uint32_t shr1(uint32_t add, uint32_t v)
+// lane l <- lane l-1, lane 0 <- bnd: a DPP wave_shr:1 move with bound_ctrl off,
+// so lane 0 (which has no source lane) keeps the boundary value preloaded into
+// the destination.  Written in asm so the destination is the preloaded
+// register; "s_nop 1" provides the two wait states a DPP read of a VGPR needs
+// after a VALU write of it.
+__device__ __forceinline__ uint32_t shr1(uint32_t bnd, uint32_t v)
 {
-    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x138, 0xf, 0xf, true)) + add;
+    uint32_t d = bnd;
+    asm("s_nop 1\n\tv_mov_b32_dpp %0, %1 wave_shr:1 row_mask:0xf bank_mask:0xf" : "+v"(d) : "v"(v));
+    return d;
 }
 
 __device__ __forceinline__ int dna_code(uint8_t c)
@@ -96,11 +100,11 @@ __device__ __forceinline__ uint32_t u_table(uint8_t q, int match, int mismatch, 
 
 constexpr uint32_t kSelNone = 0x0c0c0c0cu;   // v_perm selector: both halves u = 0
 
-// LDS layout: the boundary column, max_m + 8 rows of uint2, followed by 8 zero
-// rows that lanes 1..63 read in place of the boundary.
+// LDS layout: the boundary column, max(max_m, 4) rows of uint2 (H and Eh of
+// both pairs per text row); after the strips, the walk's staging area.
 __host__ __device__ inline size_t sw16_lds_bytes(int max_m, int max_n)
 {
-    const size_t bnd = size_t(max_m + 16) * 8;
+    const size_t bnd = size_t(max_m > 4 ? max_m : 4) * 8;
     const size_t stage = size_t(4) * size_t(max_m + max_n) + 16;   // trace: 2 CIGAR slots + 2 pairs
     return bnd > stage ? bnd : stage;
 }
@@ -253,7 +257,7 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
 
 // At most 96 VGPRs (5 waves per SIMD) for the trace variant.
 template <bool TRACE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TRACE ? 5 : 6)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
 void sw16_kernel(Sw16Launch L)
 {
     extern __shared__ __align__(16) uint32_t lds16[];
@@ -292,9 +296,7 @@ void sw16_kernel(Sw16Launch L)
         }
         sel[i] = sv;
     }
-    for (int i = lane; i < M + 8; i += 64) bnd[i] = make_uint2(0u, go2);
-    const int zrow = L.max_m + 8;
-    if (lane < 8) bnd[zrow + lane] = make_uint2(0u, 0u);
+    for (int i = lane; i < M; i += 64) bnd[i] = make_uint2(0u, go2);
     for (int j = lane; j < N; j += 64) {
         if (j < nA && mAp > 0) badA |= dna_code(PA[j]) < 0;
         if (j < nB && mBp > 0) badB |= dna_code(PB[j]) < 0;
@@ -325,20 +327,20 @@ void sw16_kernel(Sw16Launch L)
 
         uint32_t sv[4], svn[4], svnn[4];
         uint2 bv[4], bvn[4];
-        // Lane 0 reads the boundary rows (rows past the text repeat row M's
-        // spare entries, values that only reach drained cells); the other lanes
-        // read the zero rows.
+        // Boundary rows are read at a wave-uniform address (only lane 0 keeps
+        // them); past the text the row index is clamped inside the buffer --
+        // those values only reach drained cells.
         const uint32_t* sp = sel - lane;
-        const int brow_max = M + 4;                // rows M .. M+7 are spare
+        const int brow_max = M > 4 ? M - 4 : 0;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             sv[u] = sp[u]; svn[u] = sp[4 + u];
-            bv[u] = bnd[lane == 0 ? u : zrow + u];
+            bv[u] = bnd[u];
         }
         for (int t0 = 0; t0 < nsteps; t0 += 4) {
             // prefetch: selectors two groups ahead (HBM/L2), boundary one group
             // ahead (LDS); guard rows keep every read inside its buffer
-            const int brow = lane == 0 ? min(t0 + 4, brow_max) : zrow;
+            const int brow = min(t0 + 4, brow_max);
 #pragma unroll
             for (int u = 0; u < 4; ++u) { svnn[u] = sp[t0 + 8 + u]; bvn[u] = bnd[brow + u]; }
             uint32_t acc = 0, hout[4], eout[4];
