@@ -70,16 +70,12 @@ __device__ __forceinline__ uint32_t psign_mask(uint32_t d)          // 0xffff wh
 {
     return __builtin_bit_cast(uint32_t, __builtin_bit_cast(v2i16, d) >> short(15));
 }
-// lane l <- lane l-1, lane 0 <- bnd: a DPP wave_shr:1 move with bound_ctrl off,
-// so lane 0 (which has no source lane) keeps the boundary value preloaded into
-// the destination.  Written in asm so the destination is the preloaded
-// register; "s_nop 1" provides the two wait states a DPP read of a VGPR needs
-// after a VALU write of it.
-__device__ __forceinline__ uint32_t shr1(uint32_t bnd, uint32_t v)
+// lane l <- lane l-1 (DPP wave_shr:1, lane 0 gets 0) plus `add`, which is the
+// boundary value on lane 0 and 0 on every other lane; folds into one
+// v_add_u32_dpp.
+__device__ __forceinline__ uint32_t shr1(uint32_t add, uint32_t v)
 {
-    uint32_t d = bnd;
-    asm("s_nop 1\n\tv_mov_b32_dpp %0, %1 wave_shr:1 row_mask:0xf bank_mask:0xf" : "+v"(d) : "v"(v));
-    return d;
+    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x138, 0xf, 0xf, true)) + add;
 }
 
 __device__ __forceinline__ int dna_code(uint8_t c)
@@ -87,10 +83,10 @@ __device__ __forceinline__ int dna_code(uint8_t c)
     return c == 'A' ? 0 : c == 'C' ? 1 : c == 'G' ? 2 : c == 'T' ? 3 : -1;
 }
 
-// Score-table dword for one pattern byte: byte k = (q == "ACGT"[k] ? match : mismatch) + m0.
-__device__ __forceinline__ uint32_t u_table(uint8_t q, int match, int mismatch, int m0)
+// Score-table dword for one pattern byte: byte k = (q == "ACGT"[k] ? match : mismatch) + bias.
+__device__ __forceinline__ uint32_t u_table(uint8_t q, int match, int mismatch, int bias)
 {
-    const uint32_t um = uint32_t(match + m0) & 0xffu, ux = uint32_t(mismatch + m0) & 0xffu;
+    const uint32_t um = uint32_t(match + bias) & 0xffu, ux = uint32_t(mismatch + bias) & 0xffu;
     uint32_t t = 0;
     const uint8_t acgt[4] = {'A', 'C', 'G', 'T'};
 #pragma unroll
@@ -100,11 +96,11 @@ __device__ __forceinline__ uint32_t u_table(uint8_t q, int match, int mismatch, 
 
 constexpr uint32_t kSelNone = 0x0c0c0c0cu;   // v_perm selector: both halves u = 0
 
-// LDS layout: the boundary column, max(max_m, 4) rows of uint2 (H and Eh of
-// both pairs per text row); after the strips, the walk's staging area.
+// LDS layout: the boundary column, max_m + 8 rows of uint2, followed by 8 zero
+// rows that lanes 1..63 read in place of the boundary.
 __host__ __device__ inline size_t sw16_lds_bytes(int max_m, int max_n)
 {
-    const size_t bnd = size_t(max_m > 4 ? max_m : 4) * 8;
+    const size_t bnd = size_t(max_m + 16) * 8;
     const size_t stage = size_t(4) * size_t(max_m + max_n) + 16;   // trace: 2 CIGAR slots + 2 pairs
     return bnd > stage ? bnd : stage;
 }
@@ -201,10 +197,10 @@ struct Strip {
 // boundary rows (prefetched by the caller).  ROWS: rows past a pair's end are
 // masked out of the maximum (drain tail only).  COLS: columns past a pair's
 // end are masked (partial strips only).
-template <bool TRACE, bool ROWS, bool COLS>
+template <bool TRACE, bool ROWS, bool COLS, bool GK>
 __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const uint32_t (&sv)[4],
-                                            const uint2 (&bv)[4], uint32_t go2, uint32_t ge2,
-                                            uint32_t m02, int mAp, int mBp, uint32_t& acc,
+                                            const uint2 (&bv)[4], uint32_t b2, uint32_t gk2,
+                                            uint32_t ge2, int mAp, int mBp, uint32_t& acc,
                                             uint32_t (&hout)[4], uint32_t (&eout)[4])
 {
     uint32_t Hb[4];
@@ -217,9 +213,13 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
         const uint32_t En = pmax(Hl, xe);
         const uint32_t xf = psubs(S.Fh, ge2);
         const uint32_t Fn = pmax(S.H, xf);
-        const uint32_t A1 = psubs(padd(S.diag, uu), m02);
-        const uint32_t A2 = psubs(pmax(En, Fn), go2);
-        const uint32_t Hn = pmax(A1, A2);
+        // H = max(0, diag + s, max(E, F)) with u = s + b and Eh, Fh = E, F + go:
+        //   = sat(max(diag + u, max(Eh, Fh) + (b - go)) - b), and b = go in the
+        // usual case gap_open >= -mismatch, where the middle add vanishes.
+        const uint32_t A1 = padd(S.diag, uu);
+        const uint32_t G = pmax(En, Fn);
+        const uint32_t A2 = GK ? padd(G, gk2) : G;
+        const uint32_t Hn = psubs(pmax(A1, A2), b2);
         uint32_t hb = Hn;
         if (COLS) hb &= S.colmask;
         if (ROWS) {
@@ -228,6 +228,7 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
         }
         Hb[u] = hb;
         if (TRACE) {
+            // nd: A2 > A1 (both carry the same bias, so the comparison is exact)
             const uint32_t P1 = __builtin_amdgcn_perm(pdiff(En, Fn), pdiff(A1, A2), 0x07050301u);
             const uint32_t P2 = __builtin_amdgcn_perm(pdiff(S.H, xf), pdiff(Hl, xe), 0x07050301u);
             const uint32_t Z = (P1 & 0x80808080u) | ((P2 >> 4) & 0x08080808u);
@@ -257,7 +258,7 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
 
 // At most 96 VGPRs (5 waves per SIMD) for the trace variant.
 template <bool TRACE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TRACE ? 5 : 6)))
 void sw16_kernel(Sw16Launch L)
 {
     extern __shared__ __align__(16) uint32_t lds16[];
@@ -283,7 +284,9 @@ void sw16_kernel(Sw16Launch L)
     const int go = L.go, ge = L.ge;
     const uint32_t go2 = uint32_t(go) * 0x10001u, ge2 = uint32_t(ge) * 0x10001u;
     const int m0 = L.mismatch < 0 ? -L.mismatch : 0;
-    const uint32_t m02 = uint32_t(m0) * 0x10001u;
+    const int bias = go > m0 ? go : m0;            // u = s + bias >= 0
+    const uint32_t b2 = uint32_t(bias) * 0x10001u, gk2 = uint32_t(bias - go) * 0x10001u;
+    const bool gk = bias != go;
 
     bool badA = false, badB = false;
     for (int i = lane - 64; i < M + 80; i += 64) {
@@ -296,7 +299,9 @@ void sw16_kernel(Sw16Launch L)
         }
         sel[i] = sv;
     }
-    for (int i = lane; i < M; i += 64) bnd[i] = make_uint2(0u, go2);
+    for (int i = lane; i < M + 8; i += 64) bnd[i] = make_uint2(0u, go2);
+    const int zrow = L.max_m + 8;
+    if (lane < 8) bnd[zrow + lane] = make_uint2(0u, 0u);
     for (int j = lane; j < N; j += 64) {
         if (j < nA && mAp > 0) badA |= dna_code(PA[j]) < 0;
         if (j < nB && mBp > 0) badB |= dna_code(PB[j]) < 0;
@@ -318,8 +323,8 @@ void sw16_kernel(Sw16Launch L)
     for (int s = 0; s < nstrips; ++s) {
         const int j = (s << 6) + lane;
         Strip S;
-        S.tabA = u_table(j < nA ? PA[j] : uint8_t(0), L.match, L.mismatch, m0);
-        S.tabB = u_table(j < nB ? PB[j] : uint8_t(0), L.match, L.mismatch, m0);
+        S.tabA = u_table(j < nA ? PA[j] : uint8_t(0), L.match, L.mismatch, bias);
+        S.tabB = u_table(j < nB ? PB[j] : uint8_t(0), L.match, L.mismatch, bias);
         S.colmask = (j < nA ? 0xffffu : 0u) | (j < nB ? 0xffff0000u : 0u);
         S.H = 0; S.Eh = go2; S.Fh = go2; S.diag = 0;
         S.bestH = 0; S.bestT = 0xffffffffu;
@@ -327,28 +332,28 @@ void sw16_kernel(Sw16Launch L)
 
         uint32_t sv[4], svn[4], svnn[4];
         uint2 bv[4], bvn[4];
-        // Boundary rows are read at a wave-uniform address (only lane 0 keeps
-        // them); past the text the row index is clamped inside the buffer --
-        // those values only reach drained cells.
+        // Lane 0 reads the boundary rows (rows past the text repeat row M's
+        // spare entries, values that only reach drained cells); the other lanes
+        // read the zero rows.
         const uint32_t* sp = sel - lane;
-        const int brow_max = M > 4 ? M - 4 : 0;
+        const int brow_max = M + 4;                // rows M .. M+7 are spare
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             sv[u] = sp[u]; svn[u] = sp[4 + u];
-            bv[u] = bnd[u];
+            bv[u] = bnd[lane == 0 ? u : zrow + u];
         }
         for (int t0 = 0; t0 < nsteps; t0 += 4) {
             // prefetch: selectors two groups ahead (HBM/L2), boundary one group
             // ahead (LDS); guard rows keep every read inside its buffer
-            const int brow = min(t0 + 4, brow_max);
+            const int brow = lane == 0 ? min(t0 + 4, brow_max) : zrow;
 #pragma unroll
             for (int u = 0; u < 4; ++u) { svnn[u] = sp[t0 + 8 + u]; bvn[u] = bnd[brow + u]; }
             uint32_t acc = 0, hout[4], eout[4];
-            if (t0 < t_main) {
-                if (full) strip_group<TRACE, false, false>(S, t0, lane, sv, bv, go2, ge2, m02, mAp, mBp, acc, hout, eout);
-                else      strip_group<TRACE, false, true>(S, t0, lane, sv, bv, go2, ge2, m02, mAp, mBp, acc, hout, eout);
+            if (t0 < t_main && full) {
+                if (!gk) strip_group<TRACE, false, false, false>(S, t0, lane, sv, bv, b2, gk2, ge2, mAp, mBp, acc, hout, eout);
+                else     strip_group<TRACE, false, false, true>(S, t0, lane, sv, bv, b2, gk2, ge2, mAp, mBp, acc, hout, eout);
             } else {
-                strip_group<TRACE, true, true>(S, t0, lane, sv, bv, go2, ge2, m02, mAp, mBp, acc, hout, eout);
+                strip_group<TRACE, true, true, true>(S, t0, lane, sv, bv, b2, gk2, ge2, mAp, mBp, acc, hout, eout);
             }
             if (TRACE) dirs[(int64_t(s) * nq + (t0 >> 2)) * 64 + lane] = acc;
             if (lane == 63) {
@@ -372,7 +377,7 @@ void sw16_kernel(Sw16Launch L)
         __syncthreads();
     }
 
-    const int limit = 32767 - 255 - go;
+    const int limit = 32767 - 255 - bias;
     int endI[2] = {-1, -1}, endJ[2] = {-1, -1}, endS[2] = {0, 0};
     bool redo[2] = {false, false};
 #pragma unroll
