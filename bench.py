@@ -11,6 +11,10 @@ is already resident in HBM:
             (cells = sum of text_len * pattern_len per step / step time).
   secondary WFA edit distance, score + CIGAR: 10,000 pairs, pattern 150 bp,
             text = pattern with 2% per-base edits (2:1:1), smax 64 (pairs/s).
+  wfa_mixed WFA over 200,000 pairs, pattern lengths log-uniform 100 bp..10 kbp,
+            per-pair divergence uniform 0..15% (2:1:1), smax = min(0.2 len,
+            2048), 70% of pairs with CIGAR and 30% score-only (pairs/s), with
+            the oracle timed on a random sample of the same pairs.
 
 Also printed on the same JSON line: the dominant kernel's roofline (timed live
 with HIP events on the launch stream), per-step latency p50/p99, the number of
@@ -126,6 +130,44 @@ def cpu_baseline_sw(ps, seconds, threads, oracle):
                       f"{dt:.1f} s on {threads} threads of the benchmark host"}
 
 
+def cpu_baseline_wfa(ps, want, seconds, threads, oracle):
+    """The oracle's WFA batch (score, CIGAR where wanted) on `threads` host
+    threads over a random sample of the pairs, sized to run about `seconds`."""
+    L = oracle.lib()
+    rng = np.random.default_rng(3)
+    perm = rng.permutation(ps.n)
+
+    def run(n):
+        idx = np.sort(perm[:n])
+        sub = ps.subset(idx)
+        w = np.ascontiguousarray(want[idx], dtype=np.uint8)
+        slot = sub.a_len.astype(np.int64) + sub.b_len
+        cig = np.zeros(int(slot.sum()) + 1, dtype=np.uint8)
+        coff = np.zeros(n, dtype=np.int64)
+        coff[1:] = np.cumsum(slot)[:-1]
+        sc = np.zeros(n, dtype=np.int32)
+        clen = np.zeros(n, dtype=np.int32)
+        smax = np.ascontiguousarray(sub.smax, dtype=np.int32)
+        t0 = time.perf_counter()
+        rc = L.ga_oracle_wfa_batch(sub.a.ctypes.data, sub.a_off.ctypes.data, sub.a_len.ctypes.data,
+                                   sub.b.ctypes.data, sub.b_off.ctypes.data, sub.b_len.ctypes.data,
+                                   smax.ctypes.data, w.ctypes.data, n, sc.ctypes.data,
+                                   cig.ctypes.data, coff.ctypes.data, clen.ctypes.data, threads)
+        assert rc == 0
+        return time.perf_counter() - t0
+
+    probe = max(threads * 8, 64)
+    dt = run(probe)
+    while dt < 1.0 and probe < ps.n:
+        probe = int(min(ps.n, probe * max(2.0, 1.2 / max(dt, 1e-3))))
+        dt = run(probe)
+    n = int(min(ps.n, max(probe, probe * seconds / max(dt, 1e-3))))
+    dt = run(n)
+    return {"value": round(n / dt, 1), "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"{n} pairs drawn at random from the {ps.n}, {dt:.1f} s on "
+                      f"{threads} threads of the benchmark host"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +176,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-wfa", action="store_true")
+    ap.add_argument("--no-wfa-mixed", action="store_true")
+    ap.add_argument("--wfa-mixed-pairs", type=int, default=200_000)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, to scale hardware byte counters)")
@@ -238,6 +282,39 @@ def main():
         }
         log(f"wfa: {wfa_line['pairs_per_s']:.0f} pairs/s, {wbad} mismatches")
 
+    # ---------------- WFA, mixed lengths / distances ----------------
+    mixed_line = None
+    if not args.no_wfa_mixed:
+        t0 = time.perf_counter()
+        wm = W.wfa_mixed(n=args.wfa_mixed_pairs)
+        want = (np.random.default_rng(4).random(wm.n) < 0.7).astype(np.uint8)
+        wmd = WfaDeviceBatch(wm, trace=True, want=want)
+        log(f"wfa_mixed workload ready ({time.perf_counter() - t0:.1f} s)")
+        mwall, mstep, mkern = time_steps(wmd.run, args.steps, args.warmup, lib)
+        msample = rng.choice(wm.n, size=min(300, wm.n), replace=False)
+        sc, cg = wmd.results()
+        mbad = 0
+        for i in msample:
+            osc, ocg = O.wfa(wm.a_seq(i), wm.b_seq(i), int(wm.smax[i]))
+            if int(sc[i]) != osc or cg[i].decode() != (ocg if want[i] else ""):
+                mbad += 1
+        mk = float(np.mean([k for k in mkern if k > 0])) if any(k > 0 for k in mkern) else None
+        mixed_line = {
+            "workload": f"wfa_mixed {wm.n} pairs, lengths log-uniform 100..10000 bp, divergence "
+                        "0..15% (2:1:1), smax min(0.2 len, 2048), 70% CIGAR / 30% score-only",
+            "pairs_per_s": round(wm.n / (mwall / args.steps), 1),
+            "ms_per_step": round(mwall * 1000.0 / args.steps, 3),
+            "p50_ms": pctl(mstep, 50), "p99_ms": pctl(mstep, 99),
+            "kernel_ms": round(mk, 3) if mk else None,
+            "resolved": int((sc >= 0).sum()), "mismatches": mbad, "checked": int(len(msample)),
+        }
+        log(f"wfa_mixed: {mixed_line}")
+        if not args.no_cpu_baseline:
+            mixed_line["cpu_baseline"] = cpu_baseline_wfa(wm, want, args.cpu_seconds,
+                                                          min(os.cpu_count() or 1, 16), O)
+            log(f"wfa_mixed cpu baseline: {mixed_line['cpu_baseline']}")
+        del wmd
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
@@ -268,6 +345,7 @@ def main():
         "cpu_baseline": cpu,
         "sw_score_only": sw_score_line,
         "wfa": wfa_line,
+        "wfa_mixed": mixed_line,
     }
     print(json.dumps(line), flush=True)
 

@@ -62,7 +62,9 @@ class WfaDevBatch(C.Structure):
                 ("text", C.c_void_p), ("text_off", C.c_void_p), ("text_len", C.c_void_p),
                 ("smax", C.c_void_p), ("n_pairs", C.c_int64), ("smax_max", C.c_int32),
                 ("max_len", C.c_int32), ("score", C.c_void_p), ("cigar", C.c_void_p),
-                ("cigar_off", C.c_void_p), ("cigar_len", C.c_void_p)]
+                ("cigar_off", C.c_void_p), ("cigar_len", C.c_void_p),
+                ("want_cigar", C.c_void_p), ("host_smax", C.c_void_p),
+                ("host_pattern_len", C.c_void_p), ("host_text_len", C.c_void_p)]
 
 
 class SwDevBatch(C.Structure):

@@ -16,22 +16,30 @@ constexpr int kSeqPad = 128;
 // (reference: include/genoaligner/backend/wfa_kernel.hip:103,110).
 constexpr int kWfaNeg = INT32_MIN / 2;
 
-struct WfaLaunch {
+// Persistent WFA over one size class: n_waves waves claim pairs order[0..n)
+// from an atomic counter (zeroed by the launcher) and align them one at a time;
+// want[pid] selects score + CIGAR (else score only).  Each wave owns a
+// wavefront stack of (smax_cls+1)^2 offsets, 16-bit unless `wide` (needed
+// when a text is 32767 bytes or longer).
+struct WfaQueueLaunch {
     const uint8_t* pat; const int64_t* poff; const int32_t* plen;
     const uint8_t* txt; const int64_t* toff; const int32_t* tlen;
     const int32_t* smax;
+    const uint8_t* want;     // null: every pair gets a CIGAR when cigar != null
+    const int32_t* order;    // claim order (pair ids); null: pairs 0 .. n-1
     int32_t n;
-    int32_t smax_max;
+    int32_t* next;           // claim counter, 0 at launch
+    int32_t smax_cls;        // >= smax of every pair in the class (LDS + stack sizing)
+    int32_t n_waves;
     int32_t* score;
-    // trace only
     uint8_t* cigar; const int64_t* cig_off; int32_t* cig_len;
-    int32_t* ws;            // wavefront stack, ws_stride ints per pair slot
-    int64_t ws_stride;
-    int32_t first_pair;     // pairs [first_pair, first_pair + n) of the arrays
+    void* ws;                // n_waves * (smax_cls+1)^2 entries of int16 (int32 if wide)
+    bool wide;
 };
 
-// Enqueue the WFA kernel for pairs [L.first_pair, L.first_pair + L.n).
-hipError_t wfa_launch(const WfaLaunch& L, bool trace, hipStream_t st);
+int wfa_queue_waves(int smax_cls, int n);     // waves to launch for a class
+size_t wfa_queue_ws_bytes(int smax_cls, int n_waves, bool wide);
+hipError_t wfa_queue_launch(const WfaQueueLaunch& L, hipStream_t st);
 
 struct SwLaunch {
     const uint8_t* txt; const int64_t* toff; const int32_t* tlen;   // rows

@@ -250,6 +250,109 @@ void sw_validate(const char* cg, int len, const char* text, int text_len, int st
 // stacks exceed it run in consecutive chunks that reuse one buffer.
 constexpr int64_t kWfaWsBudget = int64_t(4) << 30;
 
+// Size classes of the persistent WFA kernel, by a pair's smax: each class is
+// one launch whose LDS wavefronts and per-wave stacks are sized for its bound.
+constexpr int kWfaClassBound[] = {64, 256, 1024, GA_MAX_SMAX_EXT};
+constexpr int kWfaNumClasses = 4;
+constexpr int64_t kWfaStackBudget = int64_t(24) << 30;   // bytes of stacks per class launch
+
+struct WfaPlan {
+    std::vector<int32_t> order;                 // pair ids, class by class
+    std::vector<int> bound;                     // per class: smax bound
+    std::vector<int64_t> first, count;          // per class: slice of `order`
+};
+
+// Classes by smax; inside a class the longest pairs are claimed first.
+WfaPlan plan_wfa(int64_t n, const int32_t* smax, const int32_t* plen, const int32_t* tlen)
+{
+    WfaPlan P;
+    std::vector<std::vector<int32_t>> cls(kWfaNumClasses);
+    std::vector<int> cmax(kWfaNumClasses, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        int c = 0;
+        while (c < kWfaNumClasses - 1 && smax[i] > kWfaClassBound[c]) ++c;
+        cls[size_t(c)].push_back(int32_t(i));
+        cmax[size_t(c)] = std::max(cmax[size_t(c)], int(smax[i]));
+    }
+    for (int c = 0; c < kWfaNumClasses; ++c) {
+        auto& v = cls[size_t(c)];
+        if (v.empty()) continue;
+        std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) {
+            return int64_t(plen[a]) + tlen[a] > int64_t(plen[b]) + tlen[b];
+        });
+        P.bound.push_back(std::max(cmax[size_t(c)], 0));
+        P.first.push_back(int64_t(P.order.size()));
+        P.count.push_back(int64_t(v.size()));
+        P.order.insert(P.order.end(), v.begin(), v.end());
+    }
+    return P;
+}
+
+// Enqueue every class of `plan` (pair data and outputs in `base`) with the
+// classes running concurrently on forked streams joined back into `st`.
+// Device buffers are stream-ordered allocations released after use.
+const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, bool trace,
+                        hipStream_t st, KernelTimer* timer)
+{
+    const size_t nc = plan.bound.size();
+    if (nc == 0) return nullptr;
+    int32_t* d_order = nullptr;
+    int32_t* d_next = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void**>(&d_order), plan.order.size() * 4 + 64, st) != hipSuccess ||
+        hipMallocAsync(reinterpret_cast<void**>(&d_next), nc * 4 + 64, st) != hipSuccess)
+        return "hipMalloc(WFA queue)";
+    if (hipMemcpyAsync(d_order, plan.order.data(), plan.order.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+        return "hipMemcpy(WFA queue)";
+    const char* err = nullptr;
+    if (timer) timer->begin(st);
+    hipEvent_t fork = nullptr;
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> joins;
+    if (nc > 1 && (hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess ||
+                   hipEventRecord(fork, st) != hipSuccess))
+        err = "hipEventRecord";
+    for (size_t c = 0; c < nc && !err; ++c) {
+        hipStream_t cs = st;
+        if (c > 0) {
+            cs = acquire_stream();
+            if (!cs || hipStreamWaitEvent(cs, fork, 0) != hipSuccess) { err = "hipStreamWaitEvent"; break; }
+            side.push_back(cs);
+        }
+        ga::WfaQueueLaunch L = base;
+        L.order = d_order + plan.first[c];
+        L.n = int32_t(plan.count[c]);
+        L.next = d_next + c;
+        L.smax_cls = plan.bound[c];
+        int waves = ga::wfa_queue_waves(L.smax_cls, L.n);
+        const int64_t per_wave = int64_t(ga::wfa_queue_ws_bytes(L.smax_cls, 1, L.wide));
+        if (trace) waves = int(std::max<int64_t>(1, std::min<int64_t>(waves, kWfaStackBudget / per_wave)));
+        L.n_waves = waves;
+        void* ws = nullptr;
+        if (trace && hipMallocAsync(&ws, size_t(per_wave) * size_t(waves), cs) != hipSuccess) {
+            err = "hipMalloc(wavefront stacks)";
+            break;
+        }
+        L.ws = ws;
+        if (ga::wfa_queue_launch(L, cs) != hipSuccess) err = "WFA kernel launch";
+        if (ws) (void)hipFreeAsync(ws, cs);
+        if (c > 0) {
+            hipEvent_t j = nullptr;
+            if (hipEventCreateWithFlags(&j, hipEventDisableTiming) == hipSuccess &&
+                hipEventRecord(j, cs) == hipSuccess && hipStreamWaitEvent(st, j, 0) == hipSuccess)
+                joins.push_back(j);
+            else if (!err)
+                err = "hipEventRecord";
+        }
+    }
+    if (timer) timer->end(st);
+    for (hipEvent_t j : joins) (void)hipEventDestroy(j);
+    if (fork) (void)hipEventDestroy(fork);
+    for (hipStream_t cs : side) release_stream(cs);
+    (void)hipFreeAsync(d_order, st);
+    (void)hipFreeAsync(d_next, st);
+    return err;
+}
+
 struct WfaOut {
     std::vector<int32_t> score, cig_len;
     std::vector<int64_t> slot_off;   // slot of pair i in the arena
@@ -265,85 +368,50 @@ const char* run_wfa(const ga_align_req* reqs, int64_t n, const std::vector<int32
     P.begin(); T.begin();
     out.slot_off.resize(static_cast<size_t>(n));
     int64_t slot = 0;
-    int32_t smax_max = 0;
     std::vector<int32_t> plen(static_cast<size_t>(n)), tlen(static_cast<size_t>(n));
-    std::vector<int64_t> trace_ids, score_ids;
+    bool any_trace = false;
+    int32_t max_text = 0;
     for (int64_t i = 0; i < n; ++i) {
         const ga_align_req& r = reqs[i];
+        max_text = std::max(max_text, r.text_len);
         plen[size_t(i)] = r.pattern_len;
         tlen[size_t(i)] = r.text_len;
         P.add(r.pattern, r.pattern_len);
         T.add(r.text, r.text_len);
         out.slot_off[size_t(i)] = slot;
         slot += std::max(r.pattern_len, 0) + std::max(r.text_len, 0);
-        smax_max = std::max(smax_max, smax[size_t(i)]);
-        (trace[size_t(i)] ? trace_ids : score_ids).push_back(i);
+        any_trace |= trace[size_t(i)] != 0;
     }
     P.end(); T.end();
+    const WfaPlan plan = plan_wfa(n, smax.data(), plen.data(), tlen.data());
 
     hipStream_t st = acquire_stream();
     if (!st) return "hipStreamCreate";
     DevArena D;
     D.st = st;
     const char* err = nullptr;
-    auto* d_pat = D.upload(P.bytes);
-    auto* d_txt = D.upload(T.bytes);
-    auto* d_poff = D.upload(P.off);
-    auto* d_toff = D.upload(T.off);
-    auto* d_plen = D.upload(plen);
-    auto* d_tlen = D.upload(tlen);
-    auto* d_smax = D.upload(smax);
-    auto* d_score = D.alloc<int32_t>(static_cast<size_t>(n));
-    uint8_t* d_cig = nullptr;
-    int64_t* d_cigoff = nullptr;
-    int32_t* d_ciglen = nullptr;
-    int32_t* d_ws = nullptr;
-    const int64_t ws_stride = int64_t(smax_max + 1) * int64_t(smax_max + 1);
-    int64_t chunk = 0;
-    if (!trace_ids.empty()) {
-        d_cig = D.alloc<uint8_t>(size_t(slot));
-        d_cigoff = D.upload(out.slot_off);
-        d_ciglen = D.alloc<int32_t>(static_cast<size_t>(n));
-        chunk = std::max<int64_t>(1, kWfaWsBudget / (ws_stride * 4));
-        chunk = std::min<int64_t>(chunk, int64_t(trace_ids.size()));
-        d_ws = D.alloc<int32_t>(size_t(chunk * ws_stride));
+    ga::WfaQueueLaunch L{};
+    L.pat = D.upload(P.bytes); L.poff = D.upload(P.off); L.plen = D.upload(plen);
+    L.txt = D.upload(T.bytes); L.toff = D.upload(T.off); L.tlen = D.upload(tlen);
+    L.smax = D.upload(smax);
+    L.wide = max_text > 32766;
+    L.score = D.alloc<int32_t>(size_t(n));
+    if (any_trace) {
+        L.want = D.upload(trace);
+        L.cigar = D.alloc<uint8_t>(size_t(slot));
+        L.cig_off = D.upload(out.slot_off);
+        L.cig_len = D.alloc<int32_t>(size_t(n));
     }
     if (!D.ok) err = "hipMalloc/hipMemcpy (WFA buffers)";
-
-    // Pairs are launched by id ranges; a batch is traced or score-only per pair,
-    // so each run of consecutive same-kind ids is one launch (or several chunks).
-    auto launch_range = [&](int64_t a, int64_t b, bool tr) -> hipError_t {
-        while (a < b) {
-            const int64_t len = tr ? std::min(chunk, b - a) : std::min<int64_t>(b - a, 1 << 30);
-            ga::WfaLaunch L{};
-            L.pat = d_pat; L.poff = d_poff; L.plen = d_plen;
-            L.txt = d_txt; L.toff = d_toff; L.tlen = d_tlen;
-            L.smax = d_smax; L.n = int32_t(len); L.smax_max = smax_max; L.score = d_score;
-            L.cigar = d_cig; L.cig_off = d_cigoff; L.cig_len = d_ciglen;
-            L.ws = d_ws; L.ws_stride = ws_stride; L.first_pair = int32_t(a);
-            hipError_t e = ga::wfa_launch(L, tr, st);
-            if (e != hipSuccess) return e;
-            a += len;
-        }
-        return hipSuccess;
-    };
-    if (!err) {
-        int64_t a = 0;
-        while (a < n && !err) {
-            int64_t b = a;
-            while (b < n && trace[size_t(b)] == trace[size_t(a)]) ++b;
-            if (launch_range(a, b, trace[size_t(a)] != 0) != hipSuccess) err = "WFA kernel launch";
-            a = b;
-        }
-    }
+    if (!err) err = enqueue_wfa(L, plan, any_trace, st, nullptr);
     out.score.assign(static_cast<size_t>(n), -1);
     out.cig_len.assign(static_cast<size_t>(n), 0);
     if (!err) {
-        if (hipMemcpyAsync(out.score.data(), d_score, size_t(n) * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+        if (hipMemcpyAsync(out.score.data(), L.score, size_t(n) * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
             err = "hipMemcpy(results)";
-        if (!err && d_cig) {
-            if (hipMemcpyAsync(out.cig_len.data(), d_ciglen, size_t(n) * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                (slot > 0 && hipMemcpyAsync(arena, d_cig, size_t(slot), hipMemcpyDeviceToHost, st) != hipSuccess))
+        if (!err && any_trace) {
+            if (hipMemcpyAsync(out.cig_len.data(), L.cig_len, size_t(n) * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                (slot > 0 && hipMemcpyAsync(arena, L.cigar, size_t(slot), hipMemcpyDeviceToHost, st) != hipSuccess))
                 err = "hipMemcpy(results)";
         }
     }
@@ -834,35 +902,29 @@ int ga_wfa_device(const ga_wfa_dev_batch* b, void* stream, const char** error)
     if (b->n_pairs > INT32_MAX) { if (error) *error = "too many pairs"; return GA_INVALID_ARGUMENT; }
     std::call_once(g_pool_once, init_pool);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const bool trace = b->cigar != nullptr;
-    const int64_t ws_stride = int64_t(b->smax_max + 1) * int64_t(b->smax_max + 1);
-    int64_t chunk = trace ? std::max<int64_t>(1, kWfaWsBudget / (ws_stride * 4)) : b->n_pairs;
-    chunk = std::min(chunk, b->n_pairs);
-    int32_t* ws = nullptr;
-    if (trace && hipMallocAsync(reinterpret_cast<void**>(&ws), size_t(chunk * ws_stride) * 4, st) != hipSuccess) {
-        if (error) *error = "hipMalloc(wavefront stack)";
+    WfaPlan plan;
+    if (b->host_smax && b->host_pattern_len && b->host_text_len) {
+        plan = plan_wfa(b->n_pairs, b->host_smax, b->host_pattern_len, b->host_text_len);
+    } else {
+        plan.order.resize(size_t(b->n_pairs));
+        for (int64_t i = 0; i < b->n_pairs; ++i) plan.order[size_t(i)] = int32_t(i);
+        plan.bound.push_back(b->smax_max);
+        plan.first.push_back(0);
+        plan.count.push_back(b->n_pairs);
+    }
+    ga::WfaQueueLaunch L{};
+    L.pat = b->pattern; L.poff = b->pattern_off; L.plen = b->pattern_len;
+    L.txt = b->text; L.toff = b->text_off; L.tlen = b->text_len;
+    L.smax = b->smax; L.want = b->want_cigar;
+    L.score = b->score; L.cigar = b->cigar; L.cig_off = b->cigar_off; L.cig_len = b->cigar_len;
+    L.wide = b->max_len > 32766;
+    t_timer.reset();
+    const char* err = enqueue_wfa(L, plan, b->cigar != nullptr, st, &t_timer);
+    if (err) {
+        if (error) *error = err;
         return GA_DEVICE_ERROR;
     }
-    int rc = GA_OK;
-    t_timer.reset();
-    for (int64_t a = 0; a < b->n_pairs; a += chunk) {
-        ga::WfaLaunch L{};
-        L.pat = b->pattern; L.poff = b->pattern_off; L.plen = b->pattern_len;
-        L.txt = b->text; L.toff = b->text_off; L.tlen = b->text_len;
-        L.smax = b->smax; L.n = int32_t(std::min(chunk, b->n_pairs - a)); L.smax_max = b->smax_max;
-        L.score = b->score; L.cigar = b->cigar; L.cig_off = b->cigar_off; L.cig_len = b->cigar_len;
-        L.ws = ws; L.ws_stride = ws_stride; L.first_pair = int32_t(a);
-        t_timer.begin(st);
-        const hipError_t e = ga::wfa_launch(L, trace, st);
-        t_timer.end(st);
-        if (e != hipSuccess) {
-            if (error) *error = "WFA kernel launch";
-            rc = GA_DEVICE_ERROR;
-            break;
-        }
-    }
-    if (ws) (void)hipFreeAsync(ws, st);
-    return rc;
+    return GA_OK;
 }
 
 int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)

@@ -54,48 +54,60 @@ __device__ __forceinline__ int match_run(const uint8_t* P, const uint8_t* T, int
     return run < lim ? run : lim;
 }
 
-template <bool TRACE>
-__global__ __launch_bounds__(64) void wfa_kernel(WfaLaunch L)
+// ---------------------------------------------------------------------------
+// Persistent, queue-fed kernel.  One wave aligns one pair at a time and claims
+// the next from an atomic counter, so a class of pairs of very different cost
+// (lengths 100 bp .. 10 kbp, distances up to 2048) keeps every wave busy until
+// the queue drains; the host orders each class longest first.  The wavefront
+// stack is per wave, reused from pair to pair: row s at offset s*s, 2s+1
+// entries, 16-bit offsets when every text is shorter than 32767 bytes (the
+// offset of a reachable diagonal never exceeds the text length), else 32-bit.
+template <typename Off> struct OffNull;
+template <> struct OffNull<int16_t> { static constexpr int16_t v = INT16_MIN; };
+template <> struct OffNull<int32_t> { static constexpr int32_t v = kWfaNeg; };
+
+template <typename Off>
+__device__ __forceinline__ int ws_get(const Off* ws, int s, int k)
 {
-    extern __shared__ int lds[];
-    const int slot = blockIdx.x;
-    const int pid = L.first_pair + slot;
-    const int lane = threadIdx.x;
+    if (k < -s || k > s) return kWfaNeg;
+    const int v = ws[int64_t(s) * s + k + s];
+    return v == OffNull<Off>::v ? kWfaNeg : v;
+}
+
+template <bool TRACE, typename Off>
+__device__ void wfa_pair_q(const WfaQueueLaunch& L, int pid, int lane, int* lds, Off* ws)
+{
     const int m = L.plen[pid];
     const int n = L.tlen[pid];
-    const int smax = L.smax[pid];
+    const int smax = L.smax[pid] < L.smax_cls ? L.smax[pid] : L.smax_cls;
     const uint8_t* P = L.pat + L.poff[pid];
     const uint8_t* T = L.txt + L.toff[pid];
-
-    // Degenerate pair: the score is the other length whatever smax is, and the
-    // CIGAR is all I (empty pattern) or all D (empty text) (wfa_kernel.hip:480-502).
+    const bool trace = TRACE && (L.want == nullptr || L.want[pid] != 0);
     if (m <= 0 || n <= 0) {
         const int mm = m > 0 ? m : 0, nn = n > 0 ? n : 0;
         if (lane == 0) L.score[pid] = (m <= 0) ? nn : mm;
-        if (TRACE) {
+        if (trace) {
             uint8_t* cg = L.cigar + L.cig_off[pid];
             const uint8_t c = (m <= 0) ? uint8_t('I') : uint8_t('D');
             for (int q = lane; q < mm + nn; q += 64) cg[q] = c;
             if (lane == 0) L.cig_len[pid] = mm + nn;
+        } else if (TRACE && L.cig_len && lane == 0) {
+            L.cig_len[pid] = 0;
         }
         return;
     }
-
-    const int S = L.smax_max;
+    const int S = L.smax_cls;
     const int W = 2 * S + 3;
     int* A = lds;
     int* B = lds + W;
     for (int q = lane; q < 2 * W; q += 64) lds[q] = kWfaNeg;
-    int32_t* ws = TRACE ? L.ws + int64_t(slot) * L.ws_stride : nullptr;
-
     const int o0 = match_run(P, T, m < n ? m : n);
     __syncthreads();
     if (lane == 0) {
         A[S + 1] = o0;
-        if (TRACE) ws[0] = o0;
+        if (trace) ws[0] = Off(o0);
     }
     __syncthreads();
-
     const int ak = n - m;
     int score = -1;
     if (ak == 0 && o0 >= n) {
@@ -120,7 +132,7 @@ __global__ __launch_bounds__(64) void wfa_kernel(WfaLaunch L)
                         mx += match_run(P + v, T + mx, lim);
                     }
                     cur[k + S + 1] = mx;
-                    if (TRACE) ws[int64_t(s) * s + k + s] = mx;
+                    if (trace) ws[int64_t(s) * s + k + s] = reachable(mx) ? Off(mx) : OffNull<Off>::v;
                 }
             }
             __syncthreads();
@@ -133,17 +145,14 @@ __global__ __launch_bounds__(64) void wfa_kernel(WfaLaunch L)
     }
     if (lane == 0) L.score[pid] = score;
     if (!TRACE) return;
-    if (score < 0) {
-        if (lane == 0) L.cig_len[pid] = 0;
+    if (!trace || score < 0) {
+        if (lane == 0 && L.cig_len) L.cig_len[pid] = 0;
+        __syncthreads();
         return;
     }
     __syncthreads();
-
-    // ---- traceback, uniform across the wave; CIGAR right-aligned in its slot.
     uint8_t* cg = L.cigar + L.cig_off[pid];
     int pos = m + n;
-    // The walk emits at most m + n ops (each consumes pattern or text); the
-    // clamp keeps a slot from ever being overrun.
     auto emit = [&](uint8_t c, int cnt) {
         if (cnt > pos) cnt = pos;
         if (cnt <= 0) return;
@@ -153,11 +162,8 @@ __global__ __launch_bounds__(64) void wfa_kernel(WfaLaunch L)
     if (score == 0) {
         emit('M', n);
     } else {
-        auto Wv = [&](int s, int k) -> int {
-            return (k < -s || k > s) ? kWfaNeg : ws[int64_t(s) * s + k + s];
-        };
         int sc = score, k = ak;
-        int offset = Wv(score, k);
+        int offset = ws_get(ws, score, k);
         int h = offset, v = offset - k;
         emit('D', v - m);
         emit('I', h - n);
@@ -165,9 +171,9 @@ __global__ __launch_bounds__(64) void wfa_kernel(WfaLaunch L)
         if (h > n) h = n;
         while (v > 0 && h > 0 && sc > 0) {
             const int sp = sc - 1;
-            const int misms = Wv(sp, k) + 1;
-            const int ins = Wv(sp, k - 1) + 1;
-            const int del = Wv(sp, k + 1);
+            const int misms = ws_get(ws, sp, k) + 1;
+            const int ins = ws_get(ws, sp, k - 1) + 1;
+            const int del = ws_get(ws, sp, k + 1);
             int best = kWfaNeg, op = 0;
             if (reachable(misms) && misms >= best) { best = misms; op = 1; }
             if (reachable(ins) && ins > best) { best = ins; op = 2; }
@@ -192,17 +198,75 @@ __global__ __launch_bounds__(64) void wfa_kernel(WfaLaunch L)
         emit('I', h);
     }
     if (lane == 0) L.cig_len[pid] = m + n - pos;
+    __syncthreads();
+}
+
+__global__ void zero_i32_kernel(int32_t* p, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+template <bool TRACE, typename Off>
+__global__ __launch_bounds__(64) void wfa_queue_kernel(WfaQueueLaunch L)
+{
+    extern __shared__ int lds[];
+    const int lane = threadIdx.x;
+    Off* ws = TRACE ? static_cast<Off*>(L.ws) + int64_t(blockIdx.x) * (int64_t(L.smax_cls) + 1) * (int64_t(L.smax_cls) + 1)
+                    : nullptr;
+    for (;;) {
+        // Claim without a branch on the lane id: every lane takes part in one
+        // atomic adding (lane == 0), and lane 0's result is read into a scalar
+        // register.  (With `if (lane == 0) q = atomicAdd(..)` and a shuffle,
+        // the compiler may thread the loop-invariant branch into the loop and
+        // leave lanes 1..63 looping on a stale claim while lane 0 is masked off.)
+        const int mine = atomicAdd(L.next, lane == 0 ? 1 : 0);
+        const int q = __builtin_amdgcn_readfirstlane(mine);
+        if (q >= L.n) break;                      // every wave ends here once the queue is empty
+        const int pid = L.order ? L.order[q] : q;
+        wfa_pair_q<TRACE, Off>(L, pid, lane, lds, ws);
+    }
 }
 
 }  // namespace
 
-hipError_t wfa_launch(const WfaLaunch& L, bool trace, hipStream_t st)
+}  // namespace ga
+
+namespace ga {
+
+int wfa_queue_waves(int smax_cls, int n)
 {
-    if (L.n <= 0) return hipSuccess;
-    const size_t lds = size_t(2) * size_t(2 * L.smax_max + 3) * sizeof(int);
-    const dim3 grid(unsigned(L.n)), block(64);
-    if (trace) hipLaunchKernelGGL(wfa_kernel<true>, grid, block, lds, st, L);
-    else       hipLaunchKernelGGL(wfa_kernel<false>, grid, block, lds, st, L);
+    // LDS per wave: two wavefronts of 2*smax+3 ints; at most 16 waves per CU.
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t p{};
+        if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) cus = p.multiProcessorCount;
+    }
+    const size_t lds = size_t(2) * size_t(2 * smax_cls + 3) * sizeof(int);
+    int per_cu = int((160 * 1024) / (lds > 0 ? lds : 1));
+    if (per_cu > 16) per_cu = 16;
+    if (per_cu < 1) per_cu = 1;
+    const int waves = cus * per_cu;
+    return n < waves ? n : waves;
+}
+
+size_t wfa_queue_ws_bytes(int smax_cls, int n_waves, bool wide)
+{
+    return size_t(n_waves) * size_t(smax_cls + 1) * size_t(smax_cls + 1) * (wide ? 4u : 2u);
+}
+
+hipError_t wfa_queue_launch(const WfaQueueLaunch& L, hipStream_t st)
+{
+    if (L.n <= 0 || L.n_waves <= 0) return hipSuccess;
+    const size_t lds = size_t(2) * size_t(2 * L.smax_cls + 3) * sizeof(int);
+    // the claim counter is zeroed by a kernel, not a memset node, so this
+    // submission can be captured into a graph and replayed
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, st, L.next, 1);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const dim3 grid(unsigned(L.n_waves)), block(64);
+    if (!L.cigar)     hipLaunchKernelGGL((wfa_queue_kernel<false, int16_t>), grid, block, lds, st, L);
+    else if (L.wide)  hipLaunchKernelGGL((wfa_queue_kernel<true, int32_t>), grid, block, lds, st, L);
+    else              hipLaunchKernelGGL((wfa_queue_kernel<true, int16_t>), grid, block, lds, st, L);
     return hipGetLastError();
 }
 

@@ -37,9 +37,15 @@ def _check(rc: int, err: C.c_char_p, what: str):
 
 
 class WfaDeviceBatch:
-    """WFA over a PairSet (a = pattern, b = text) with per-pair smax."""
+    """WFA over a PairSet (a = pattern, b = text) with per-pair smax.
 
-    def __init__(self, ps: PairSet, trace: bool = True, device: str = "cuda:0"):
+    want: optional per-pair CIGAR flags (0 = score only for that pair; needs
+    trace).  classes: hand the host copies of smax and the lengths to the
+    library so it splits the batch into smax size classes, longest pair first.
+    """
+
+    def __init__(self, ps: PairSet, trace: bool = True, device: str = "cuda:0",
+                 want: Optional[np.ndarray] = None, classes: bool = True):
         dev = torch.device(device)
         self.n = ps.n
         self.trace = trace
@@ -61,12 +67,24 @@ class WfaDeviceBatch:
             self.cig_len = torch.empty(self.n, dtype=torch.int32, device=dev)
         self.smax_max = int(ps.smax.max()) if self.n else 0
         self.max_len = int(max(ps.a_len.max(), ps.b_len.max())) if self.n else 0
+        self.want_np = None
+        if want is not None:
+            assert trace and len(want) == self.n
+            self.want_np = np.ascontiguousarray(want, dtype=np.uint8)
+            self.want = torch.from_numpy(self.want_np).to(dev)
+        # host copies stay referenced by self for as long as the batch lives
+        self.h_smax = np.ascontiguousarray(ps.smax, dtype=np.int32)
+        self.h_plen = np.ascontiguousarray(ps.a_len, dtype=np.int32)
+        self.h_tlen = np.ascontiguousarray(ps.b_len, dtype=np.int32)
+        hp = (lambda a: a.ctypes.data) if classes else (lambda a: None)
         self.b = N.WfaDevBatch(
             self.pat.data_ptr(), self.poff.data_ptr(), self.plen.data_ptr(),
             self.txt.data_ptr(), self.toff.data_ptr(), self.tlen.data_ptr(),
             self.smax.data_ptr(), self.n, self.smax_max, self.max_len, self.score.data_ptr(),
             self.cigar.data_ptr() if trace else None, self.cig_off.data_ptr() if trace else None,
-            self.cig_len.data_ptr() if trace else None)
+            self.cig_len.data_ptr() if trace else None,
+            self.want.data_ptr() if want is not None else None,
+            hp(self.h_smax), hp(self.h_plen), hp(self.h_tlen))
 
     def run(self, stream: Optional[torch.cuda.Stream] = None):
         err = C.c_char_p()
@@ -85,6 +103,8 @@ class WfaDeviceBatch:
         for i in range(self.n):
             end = int(self.slot_off_np[i] + self.slot[i])
             L = int(ln[i]) if sc[i] >= 0 else 0
+            if self.want_np is not None and not self.want_np[i]:
+                L = 0
             cig.append(raw[end - L: end])
         return sc, cig
 

@@ -75,34 +75,51 @@ def _offsets(lens: np.ndarray) -> np.ndarray:
 
 
 def random_bases(rng: np.random.Generator, n: int) -> np.ndarray:
-    return ACGT[rng.integers(0, 4, size=n)]
+    return ACGT[rng.integers(0, 4, size=n, dtype=np.uint8)]
 
 
-def mutate(rng: np.random.Generator, seq: np.ndarray, lens: np.ndarray, rate: np.ndarray):
+_CODE = np.zeros(256, dtype=np.uint8)
+_CODE[ACGT] = np.arange(4, dtype=np.uint8)
+
+
+def mutate(rng: np.random.Generator, seq: np.ndarray, lens: np.ndarray, rate: np.ndarray,
+           chunk: int = 1 << 24):
     """Apply per-base edits (sub:ins:del = 2:1:1 at total probability rate[pair])
-    to every sequence of a packed array.  Returns (bytes, offsets, lengths)."""
+    to every sequence of a packed array.  Returns (bytes, offsets, lengths).
+    Works through the pairs in groups of about `chunk` bases."""
     lens = lens.astype(np.int64)
-    total = int(lens.sum())
-    pair = np.repeat(np.arange(len(lens)), lens)
-    r = rng.random(total)
-    p = rate[pair]
-    sub = r < p * 0.5
-    ins = (r >= p * 0.5) & (r < p * 0.75)
-    dele = (r >= p * 0.75) & (r < p)
-    base = seq.copy()
-    # substitution: one of the three other bases
-    codes = np.searchsorted(ACGT, base)
-    shift = rng.integers(1, 4, size=total)
-    base[sub] = ACGT[(codes[sub] + shift[sub]) % 4]
-    count = 1 + ins.astype(np.int64) - dele.astype(np.int64)
-    out_len = np.bincount(pair, weights=count, minlength=len(lens)).astype(np.int64)
-    src = np.repeat(np.arange(total), count)
-    out = base[src]
-    # the second copy of an inserted base becomes a uniform random base
-    first = np.ones(len(src), dtype=bool)
-    first[1:] = src[1:] != src[:-1]
-    dup = ~first
-    out[dup] = random_bases(rng, int(dup.sum()))
+    off = _offsets(lens)
+    rate = np.asarray(rate, dtype=np.float32)
+    n = len(lens)
+    pieces, out_len = [], np.zeros(n, dtype=np.int64)
+    ends = np.cumsum(lens)
+    i = 0
+    while i < n:
+        j = int(np.searchsorted(ends, (ends[i - 1] if i else 0) + chunk, side="right"))
+        j = max(j, i + 1)
+        lo, hi = int(off[i]), int(off[j - 1] + lens[j - 1])
+        total = hi - lo
+        pair = np.repeat(np.arange(j - i), lens[i:j])
+        r = rng.random(total, dtype=np.float32)
+        p = rate[i:j][pair]
+        sub = r < p * 0.5
+        ins = (r >= p * 0.5) & (r < p * 0.75)
+        dele = (r >= p * 0.75) & (r < p)
+        base = seq[lo:hi].copy()
+        # substitution: one of the three other bases
+        shift = rng.integers(1, 4, size=total, dtype=np.uint8)
+        base[sub] = ACGT[(_CODE[base[sub]] + shift[sub]) & 3]
+        count = 1 + ins.astype(np.int64) - dele.astype(np.int64)
+        out_len[i:j] = np.bincount(pair, weights=count, minlength=j - i).astype(np.int64)
+        src = np.repeat(np.arange(total), count)
+        out = base[src]
+        # the second copy of an inserted base becomes a uniform random base
+        dup = np.zeros(len(src), dtype=bool)
+        dup[1:] = src[1:] == src[:-1]
+        out[dup] = random_bases(rng, int(dup.sum()))
+        pieces.append(out)
+        i = j
+    out = np.concatenate(pieces) if pieces else np.zeros(0, dtype=np.uint8)
     return out, _offsets(out_len), out_len.astype(np.int32)
 
 

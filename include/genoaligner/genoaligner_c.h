@@ -150,6 +150,15 @@ typedef struct ga_wfa_dev_batch {
     uint8_t*       cigar;           /* NULL = score only */
     const int64_t* cigar_off;
     int32_t*       cigar_len;       /* out */
+    /* Optional scheduling metadata (may all be NULL):
+     *   want_cigar    device, per pair: 0 = score only for that pair
+     *   host_smax,    host copies of smax, pattern_len and text_len: with them
+     *   host_pattern_len,  the batch is split into smax size classes, each run
+     *   host_text_len      longest pair first, classes concurrently. */
+    const uint8_t* want_cigar;
+    const int32_t* host_smax;
+    const int32_t* host_pattern_len;
+    const int32_t* host_text_len;
 } ga_wfa_dev_batch;
 
 int ga_wfa_device(const ga_wfa_dev_batch* b, void* stream, const char** error);

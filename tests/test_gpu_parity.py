@@ -243,3 +243,39 @@ def test_sw_packed_overflow_rerun(gpu, oracle):
     bs = ga.align_sw_batch([_sreq(ga, t, p, sc, False) for t, p in pairs])
     for got, (t, p) in zip(bs.results, pairs):
         assert (got.score, got.end_i, got.end_j) == oracle.sw_score_lane(t, p, sc)
+
+
+def test_wfa_device_classes_want(gpu, oracle):
+    """Mixed lengths and smax across size classes, 70% CIGAR / 30% score-only
+    per pair, scheduled by class (longest first) and as one class."""
+    from genoaligner_amd import workloads as W
+    from genoaligner_amd.device import WfaDeviceBatch
+    ps = W.wfa_mixed(n=600, lo=20, hi=2500, seed=99)
+    want = (np.random.default_rng(5).random(ps.n) < 0.7).astype(np.uint8)
+    exp = [oracle.wfa(ps.a_seq(i), ps.b_seq(i), int(ps.smax[i])) for i in range(ps.n)]
+    for classes in (True, False):
+        dev = WfaDeviceBatch(ps, trace=True, want=want, classes=classes)
+        dev.run()
+        sc, cig = dev.results()
+        for i in range(ps.n):
+            e_sc, e_cg = exp[i]
+            assert int(sc[i]) == e_sc, (classes, i)
+            assert cig[i].decode() == (e_cg if want[i] else ""), (classes, i)
+    dev = WfaDeviceBatch(ps, trace=False)
+    dev.run()
+    sc, _ = dev.results()
+    assert [int(x) for x in sc] == [e[0] for e in exp]
+
+
+def test_wfa_long_text_wide_stack(gpu, oracle):
+    """Texts of 32767 bytes or more keep 32-bit wavefront stacks."""
+    import genoaligner_amd as ga
+    rng = random.Random(11)
+    p = "".join(rng.choice("ACGT") for _ in range(33000))
+    t = _mutate(rng, p, 0.002)
+    reqs = [_areq(ga, p, t, 200), _areq(ga, p[:500], t[:505], 200), _areq(ga, t, p, 200, False)]
+    b = ga.align_batch_ext(reqs)
+    assert b.ok()
+    for r, q in zip(b.results, reqs):
+        sc, cg = oracle.wfa(q.pattern, q.text, 200)
+        assert (r.score, r.cigar) == (sc, cg if q.with_cigar else "")
