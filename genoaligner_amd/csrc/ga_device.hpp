@@ -16,11 +16,13 @@ constexpr int kSeqPad = 128;
 // (reference: include/genoaligner/backend/wfa_kernel.hip:103,110).
 constexpr int kWfaNeg = INT32_MIN / 2;
 
-// Persistent WFA over one size class: n_waves waves claim pairs order[0..n)
-// from an atomic counter (zeroed by the launcher) and align them one at a time;
-// want[pid] selects score + CIGAR (else score only).  Each wave owns a
-// wavefront stack of (smax_cls+1)^2 offsets, 16-bit unless `wide` (needed
-// when a text is 32767 bytes or longer).
+// Persistent WFA over one size class: n_groups workgroups of `waves`
+// wavefronts claim pairs order[0..n) from an atomic counter (zeroed by the
+// launcher) and align them one at a time; want[pid] selects score + CIGAR (else
+// score only).  Each workgroup owns a wavefront stack of (smax_cls+1)^2
+// offsets, 16-bit unless `wide` (needed when a text is 32767 bytes or longer).
+// `staged`: both sequences of a pair are copied into LDS (sized by max_plen,
+// max_tlen) and extended there.
 struct WfaQueueLaunch {
     const uint8_t* pat; const int64_t* poff; const int32_t* plen;
     const uint8_t* txt; const int64_t* toff; const int32_t* tlen;
@@ -30,15 +32,19 @@ struct WfaQueueLaunch {
     int32_t n;
     int32_t* next;           // claim counter, 0 at launch
     int32_t smax_cls;        // >= smax of every pair in the class (LDS + stack sizing)
-    int32_t n_waves;
+    int32_t max_plen, max_tlen;
+    int32_t waves;           // wavefronts per workgroup: 1, 2, 4 or 8
+    int32_t n_groups;
     int32_t* score;
     uint8_t* cigar; const int64_t* cig_off; int32_t* cig_len;
-    void* ws;                // n_waves * (smax_cls+1)^2 entries of int16 (int32 if wide)
+    void* ws;                // n_groups * (smax_cls+1)^2 entries of int16 (int32 if wide)
     bool wide;
+    bool staged;
 };
 
-int wfa_queue_waves(int smax_cls, int n);     // waves to launch for a class
-size_t wfa_queue_ws_bytes(int smax_cls, int n_waves, bool wide);
+size_t wfa_lds_bytes(const WfaQueueLaunch& L);     // dynamic LDS of one workgroup
+int wfa_queue_groups(const WfaQueueLaunch& L);     // resident workgroups (<= n)
+size_t wfa_queue_ws_bytes(int smax_cls, int n_groups, bool wide);
 hipError_t wfa_queue_launch(const WfaQueueLaunch& L, hipStream_t st);
 
 struct SwLaunch {

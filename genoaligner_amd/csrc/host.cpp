@@ -259,38 +259,58 @@ constexpr int64_t kWfaStackBudget = int64_t(24) << 30;   // bytes of stacks per 
 struct WfaPlan {
     std::vector<int32_t> order;                 // pair ids, class by class
     std::vector<int> bound;                     // per class: smax bound
+    std::vector<int> max_plen, max_tlen;        // per class: longest sequences
     std::vector<int64_t> first, count;          // per class: slice of `order`
 };
 
-// Classes by smax; inside a class the longest pairs are claimed first.
+// Classes by smax; inside a class the longest pairs are claimed first (a
+// counting sort on (pattern_len + text_len) / 32, so planning stays O(n)).
 WfaPlan plan_wfa(int64_t n, const int32_t* smax, const int32_t* plen, const int32_t* tlen)
 {
+    constexpr int kKeys = 1 << 14;
     WfaPlan P;
-    std::vector<std::vector<int32_t>> cls(kWfaNumClasses);
-    std::vector<int> cmax(kWfaNumClasses, 0);
+    std::vector<int> cls(static_cast<size_t>(n));
+    std::vector<int64_t> hist(size_t(kWfaNumClasses) * kKeys, 0);
+    std::vector<int> cmax(kWfaNumClasses, 0), pmax(kWfaNumClasses, 0), tmax(kWfaNumClasses, 0);
+    std::vector<int64_t> ccount(kWfaNumClasses, 0);
+    auto key = [&](int64_t i) {
+        const int64_t l = (int64_t(std::max(plen[i], 0)) + std::max(tlen[i], 0)) >> 5;
+        return int(kKeys - 1 - std::min<int64_t>(l, kKeys - 1));      // longest first
+    };
     for (int64_t i = 0; i < n; ++i) {
         int c = 0;
         while (c < kWfaNumClasses - 1 && smax[i] > kWfaClassBound[c]) ++c;
-        cls[size_t(c)].push_back(int32_t(i));
+        cls[size_t(i)] = c;
         cmax[size_t(c)] = std::max(cmax[size_t(c)], int(smax[i]));
+        pmax[size_t(c)] = std::max(pmax[size_t(c)], int(plen[i]));
+        tmax[size_t(c)] = std::max(tmax[size_t(c)], int(tlen[i]));
+        ++ccount[size_t(c)];
+        ++hist[size_t(c) * kKeys + size_t(key(i))];
     }
+    int64_t at = 0;
     for (int c = 0; c < kWfaNumClasses; ++c) {
-        auto& v = cls[size_t(c)];
-        if (v.empty()) continue;
-        std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) {
-            return int64_t(plen[a]) + tlen[a] > int64_t(plen[b]) + tlen[b];
-        });
+        if (ccount[size_t(c)] == 0) continue;
         P.bound.push_back(std::max(cmax[size_t(c)], 0));
-        P.first.push_back(int64_t(P.order.size()));
-        P.count.push_back(int64_t(v.size()));
-        P.order.insert(P.order.end(), v.begin(), v.end());
+        P.max_plen.push_back(pmax[size_t(c)]);
+        P.max_tlen.push_back(tmax[size_t(c)]);
+        P.first.push_back(at);
+        P.count.push_back(ccount[size_t(c)]);
+        for (int k = 0; k < kKeys; ++k) {                       // exclusive scan -> slots
+            const int64_t h = hist[size_t(c) * kKeys + size_t(k)];
+            hist[size_t(c) * kKeys + size_t(k)] = at;
+            at += h;
+        }
     }
+    P.order.resize(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i)
+        P.order[size_t(hist[size_t(cls[size_t(i)]) * kKeys + size_t(key(i))]++)] = int32_t(i);
     return P;
 }
 
-// Enqueue every class of `plan` (pair data and outputs in `base`) with the
-// classes running concurrently on forked streams joined back into `st`.
-// Device buffers are stream-ordered allocations released after use.
+// Wavefronts per workgroup for a class: wide wavefronts split over more waves.
+int wfa_class_waves(int bound) { return bound <= 64 ? 1 : bound <= 256 ? 2 : bound <= 1024 ? 4 : 8; }
+constexpr size_t kWfaMaxStagedLds = 64 * 1024;
+
 const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, bool trace,
                         hipStream_t st, KernelTimer* timer)
 {
@@ -323,12 +343,17 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
         L.n = int32_t(plan.count[c]);
         L.next = d_next + c;
         L.smax_cls = plan.bound[c];
-        int waves = ga::wfa_queue_waves(L.smax_cls, L.n);
-        const int64_t per_wave = int64_t(ga::wfa_queue_ws_bytes(L.smax_cls, 1, L.wide));
-        if (trace) waves = int(std::max<int64_t>(1, std::min<int64_t>(waves, kWfaStackBudget / per_wave)));
-        L.n_waves = waves;
+        L.max_plen = plan.max_plen[c];
+        L.max_tlen = plan.max_tlen[c];
+        L.waves = wfa_class_waves(L.smax_cls);
+        L.staged = true;
+        if (ga::wfa_lds_bytes(L) > kWfaMaxStagedLds) L.staged = false;
+        int groups = ga::wfa_queue_groups(L);
+        const int64_t per_group = int64_t(ga::wfa_queue_ws_bytes(L.smax_cls, 1, L.wide));
+        if (trace) groups = int(std::max<int64_t>(1, std::min<int64_t>(groups, kWfaStackBudget / per_group)));
+        L.n_groups = groups;
         void* ws = nullptr;
-        if (trace && hipMallocAsync(&ws, size_t(per_wave) * size_t(waves), cs) != hipSuccess) {
+        if (trace && hipMallocAsync(&ws, size_t(per_group) * size_t(groups), cs) != hipSuccess) {
             err = "hipMalloc(wavefront stacks)";
             break;
         }
@@ -909,6 +934,8 @@ int ga_wfa_device(const ga_wfa_dev_batch* b, void* stream, const char** error)
         plan.order.resize(size_t(b->n_pairs));
         for (int64_t i = 0; i < b->n_pairs; ++i) plan.order[size_t(i)] = int32_t(i);
         plan.bound.push_back(b->smax_max);
+        plan.max_plen.push_back(b->max_len);
+        plan.max_tlen.push_back(b->max_len);
         plan.first.push_back(0);
         plan.count.push_back(b->n_pairs);
     }

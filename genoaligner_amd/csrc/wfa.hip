@@ -27,8 +27,6 @@
 namespace ga {
 namespace {
 
-__device__ __forceinline__ bool reachable(int off) { return off > kWfaNeg + 1024; }
-
 // 8 bytes starting at any byte address.  Reads the two aligned words that
 // contain them, so up to 15 bytes past `p` must be readable (sequence buffers
 // carry kSeqPad bytes of padding on both sides).
@@ -55,176 +53,288 @@ __device__ __forceinline__ int match_run(const uint8_t* P, const uint8_t* T, int
 }
 
 // ---------------------------------------------------------------------------
-// Persistent, queue-fed kernel.  One wave aligns one pair at a time and claims
-// the next from an atomic counter, so a class of pairs of very different cost
-// (lengths 100 bp .. 10 kbp, distances up to 2048) keeps every wave busy until
-// the queue drains; the host orders each class longest first.  The wavefront
-// stack is per wave, reused from pair to pair: row s at offset s*s, 2s+1
-// entries, 16-bit offsets when every text is shorter than 32767 bytes (the
-// offset of a reachable diagonal never exceeds the text length), else 32-bit.
+// Persistent, queue-fed kernel.  A workgroup of WAVES wavefronts aligns one
+// pair at a time and claims the next from an atomic counter, so a class of
+// pairs of very different cost (lengths 100 bp .. 10 kbp, distances up to
+// 2048) keeps every CU busy until the queue drains; the host orders each class
+// longest first.  Per pair the workgroup
+//   * stages both sequences into LDS (STAGED; the extension then never touches
+//     global memory), else extends from the packed global buffers;
+//   * keeps the two live wavefronts in LDS as Off (16-bit when every text is
+//     shorter than 32767 bytes), thread t owning diagonals -s + t, -s + t +
+//     64*WAVES, ...;
+//   * stores every wavefront to the workgroup's HBM stack (trace pairs only):
+//     row s at offset s*s, 2s+1 entries, reused from pair to pair;
+//   * runs the traceback on wave 0 (uniform state, 64-wide CIGAR stores).
+// An offset is reachable iff it is >= 0: real offsets are, and everything
+// derived from the unreachable sentinel (INT16_MIN or INT32_MIN/2, plus at
+// most one) is negative; this is the reference's `> NEG + 1024` test on the
+// values the recurrence can produce.
 template <typename Off> struct OffNull;
-template <> struct OffNull<int16_t> { static constexpr int16_t v = INT16_MIN; };
-template <> struct OffNull<int32_t> { static constexpr int32_t v = kWfaNeg; };
+template <> struct OffNull<int16_t> { static constexpr int v = INT16_MIN; };
+template <> struct OffNull<int32_t> { static constexpr int v = kWfaNeg; };
 
 template <typename Off>
 __device__ __forceinline__ int ws_get(const Off* ws, int s, int k)
 {
-    if (k < -s || k > s) return kWfaNeg;
-    const int v = ws[int64_t(s) * s + k + s];
-    return v == OffNull<Off>::v ? kWfaNeg : v;
+    if (k < -s || k > s) return OffNull<Off>::v;
+    return ws[int64_t(s) * s + k + s];
 }
 
-template <bool TRACE, typename Off>
-__device__ void wfa_pair_q(const WfaQueueLaunch& L, int pid, int lane, int* lds, Off* ws)
+// 4 bytes of an LDS byte array starting at any index: two aligned 32-bit
+// reads and one byte funnel shift (v_alignbyte_b32).
+__device__ __forceinline__ uint32_t lds_load4(const uint32_t* w, int i)
 {
-    const int m = L.plen[pid];
-    const int n = L.tlen[pid];
-    const int smax = L.smax[pid] < L.smax_cls ? L.smax[pid] : L.smax_cls;
-    const uint8_t* P = L.pat + L.poff[pid];
-    const uint8_t* T = L.txt + L.toff[pid];
-    const bool trace = TRACE && (L.want == nullptr || L.want[pid] != 0);
-    if (m <= 0 || n <= 0) {
-        const int mm = m > 0 ? m : 0, nn = n > 0 ? n : 0;
-        if (lane == 0) L.score[pid] = (m <= 0) ? nn : mm;
-        if (trace) {
-            uint8_t* cg = L.cigar + L.cig_off[pid];
-            const uint8_t c = (m <= 0) ? uint8_t('I') : uint8_t('D');
-            for (int q = lane; q < mm + nn; q += 64) cg[q] = c;
-            if (lane == 0) L.cig_len[pid] = mm + nn;
-        } else if (TRACE && L.cig_len && lane == 0) {
-            L.cig_len[pid] = 0;
-        }
-        return;
-    }
+    const uint32_t lo = w[i >> 2];
+    const uint32_t hi = w[(i >> 2) + 1];
+    return __builtin_amdgcn_alignbyte(hi, lo, uint32_t(i) & 3u);
+}
+
+// Run of equal bytes P[v..] == T[h..], capped at lim (lim >= 0; P[v], T[h]
+// must lie inside the staged buffers even when lim == 0).  One loop exit, so
+// lanes leave the loop without extra exec-mask bookkeeping.
+__device__ __forceinline__ int lds_match_run(const uint32_t* P, int v, const uint32_t* T, int h, int lim)
+{
+    int run = -4;
+    uint32_t x;
+    do {
+        run += 4;
+        x = lds_load4(P, v + run) ^ lds_load4(T, h + run);
+    } while (x == 0 && run + 4 < lim);
+    run += x ? int(__builtin_ctz(x) >> 3) : 4;
+    return run < lim ? run : lim;
+}
+
+// LDS layout of a workgroup: [claim slots: 4 ints][wavefront A][wavefront B]
+// [traceback window][pattern][text], each region 16-byte aligned.  After the
+// forward pass the pattern + text region (>= m + n bytes) holds the CIGAR.
+__host__ __device__ inline int wfa_wf_bytes(int smax_cls, int off_bytes)
+{
+    return ((2 * smax_cls + 3) * off_bytes + 15) & ~15;
+}
+__host__ __device__ inline int wfa_seq_bytes(int len) { return ((len > 0 ? len : 0) + 31) & ~15; }
+
+// Traceback window: kTbRows rows of the stack below the current score, at
+// diagonals k0 - kTbRows .. k0 + kTbRows, loaded with one batch of coalesced
+// reads and walked out of LDS for up to kTbRows steps.
+constexpr int kTbRows = 16;
+constexpr int kTbCols = 2 * kTbRows + 1;
+__host__ __device__ inline int wfa_win_bytes(int off_bytes) { return (kTbRows * kTbCols * off_bytes + 15) & ~15; }
+
+// Eight waves per SIMD: <= 64 VGPRs and <= 96 SGPRs, so four 8-wave
+// workgroups (or 32 single waves) fit a CU when LDS allows.
+template <bool TRACE, typename Off, bool STAGED, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void wfa_kernel(WfaQueueLaunch L)
+{
+    extern __shared__ __align__(16) uint8_t smem[];
+    constexpr int NT = 64 * WAVES;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int S = L.smax_cls;
     const int W = 2 * S + 3;
-    int* A = lds;
-    int* B = lds + W;
-    for (int q = lane; q < 2 * W; q += 64) lds[q] = kWfaNeg;
-    const int o0 = match_run(P, T, m < n ? m : n);
-    __syncthreads();
-    if (lane == 0) {
-        A[S + 1] = o0;
-        if (trace) ws[0] = Off(o0);
-    }
-    __syncthreads();
-    const int ak = n - m;
-    int score = -1;
-    if (ak == 0 && o0 >= n) {
-        score = 0;
-    } else {
-        int* prev = A;
-        int* cur = B;
-        for (int s = 1; s <= smax; ++s) {
-            for (int base = -s; base <= s; base += 64) {
-                const int k = base + lane;
-                if (k <= s) {
+    const int wfb = wfa_wf_bytes(S, int(sizeof(Off)));
+    int* claim = reinterpret_cast<int*>(smem);
+    Off* A = reinterpret_cast<Off*>(smem + 16);
+    Off* B = reinterpret_cast<Off*>(smem + 16 + wfb);
+    Off* win = reinterpret_cast<Off*>(smem + 16 + 2 * wfb);
+    uint8_t* seq = smem + 16 + 2 * wfb + wfa_win_bytes(int(sizeof(Off)));
+    uint32_t* sP = reinterpret_cast<uint32_t*>(seq);
+    uint32_t* sT = reinterpret_cast<uint32_t*>(seq + wfa_seq_bytes(L.max_plen));
+    Off* ws = TRACE ? static_cast<Off*>(L.ws) + int64_t(blockIdx.x) * (int64_t(S) + 1) * (int64_t(S) + 1)
+                    : nullptr;
+    const int G = int(gridDim.x);
+    for (int it = 0;; ++it) {
+        // The first round is static (workgroup g takes pair g); later pairs are
+        // claimed from the counter, which then starts at G.  No atomics at all
+        // when the class has no more pairs than workgroups.
+        // The claim has no branch on the lane id: wave 0 takes part in one
+        // atomic adding (lane == 0) and lane 0's result is read into a scalar
+        // register.  (With `if (lane == 0) q = atomicAdd(..)` and a shuffle,
+        // the compiler may thread the loop-invariant branch into the loop and
+        // leave lanes 1..63 looping on a stale claim while lane 0 is masked off.)
+        int q;
+        if (it == 0) {
+            q = int(blockIdx.x);
+        } else if (L.n <= G) {
+            break;
+        } else if (WAVES == 1) {
+            q = G + __builtin_amdgcn_readfirstlane(atomicAdd(L.next, lane == 0 ? 1 : 0));
+        } else {
+            if (wave == 0) claim[it & 1] = __builtin_amdgcn_readfirstlane(atomicAdd(L.next, lane == 0 ? 1 : 0));
+            __syncthreads();
+            q = G + __builtin_amdgcn_readfirstlane(claim[it & 1]);
+        }
+        if (q >= L.n) break;                      // every wave ends here once the queue is empty
+        const int pid = L.order ? L.order[q] : q;
+
+        const int m = L.plen[pid];
+        const int n = L.tlen[pid];
+        const int smax = L.smax[pid] < S ? L.smax[pid] : S;
+        const uint8_t* P = L.pat + L.poff[pid];
+        const uint8_t* T = L.txt + L.toff[pid];
+        const bool trace = TRACE && (L.want == nullptr || L.want[pid] != 0);
+        if (m <= 0 || n <= 0) {
+            const int mm = m > 0 ? m : 0, nn = n > 0 ? n : 0;
+            if (tid == 0) L.score[pid] = (m <= 0) ? nn : mm;
+            if (trace) {
+                uint8_t* cg = L.cigar + L.cig_off[pid];
+                const uint8_t c = (m <= 0) ? uint8_t('I') : uint8_t('D');
+                for (int x = tid; x < mm + nn; x += NT) cg[x] = c;
+                if (tid == 0) L.cig_len[pid] = mm + nn;
+            } else if (TRACE && L.cig_len && tid == 0) {
+                L.cig_len[pid] = 0;
+            }
+            continue;
+        }
+        for (int x = tid; x < W; x += NT) {
+            A[x] = Off(OffNull<Off>::v);
+            B[x] = Off(OffNull<Off>::v);
+        }
+        if (STAGED) {
+            // copy 8 bytes per thread per step; the global buffers are readable
+            // past each sequence, and the words past the end are never compared
+            uint64_t* dP = reinterpret_cast<uint64_t*>(sP);
+            uint64_t* dT = reinterpret_cast<uint64_t*>(sT);
+            const int wp = (m + 15) >> 3, wt = (n + 15) >> 3;
+            for (int x = tid; x < wp; x += NT) dP[x] = load8(P + 8 * x);
+            for (int x = tid; x < wt; x += NT) dT[x] = load8(T + 8 * x);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            const int o0 = STAGED ? lds_match_run(sP, 0, sT, 0, m < n ? m : n) : match_run(P, T, m < n ? m : n);
+            A[S + 1] = Off(o0);
+            if (trace) ws[0] = Off(o0);
+        }
+        __syncthreads();
+        const int ak = n - m;
+        int score = -1;
+        if (ak == 0 && int(A[S + 1]) >= n) {
+            score = 0;
+        } else {
+            Off* prev = A;
+            Off* cur = B;
+            for (int s = 1; s <= smax; ++s) {
+                for (int k = -s + tid; k <= s; k += NT) {
                     const int ins = prev[k + S];
                     const int mis = prev[k + S + 1];
                     const int del = prev[k + S + 2];
                     const int mid = ins > mis ? ins : mis;
                     int mx = del > mid + 1 ? del : mid + 1;
-                    if (!reachable(mx) || mx > n || mx - k > m) {
-                        mx = kWfaNeg;
-                    } else {
-                        const int v = mx - k;
+                    const int v = mx - k;
+                    const bool ok = mx >= 0 && mx <= n && v <= m;
+                    if (STAGED) {
+                        // invalid diagonals probe (0, 0) with lim 0 and are dropped
+                        const int lim = ok ? ((m - v) < (n - mx) ? (m - v) : (n - mx)) : 0;
+                        const int run = lds_match_run(sP, ok ? v : 0, sT, ok ? mx : 0, lim);
+                        mx = ok ? mx + run : OffNull<Off>::v;
+                    } else if (ok) {
                         const int lim = (m - v) < (n - mx) ? (m - v) : (n - mx);
                         mx += match_run(P + v, T + mx, lim);
+                    } else {
+                        mx = OffNull<Off>::v;
                     }
-                    cur[k + S + 1] = mx;
-                    if (trace) ws[int64_t(s) * s + k + s] = reachable(mx) ? Off(mx) : OffNull<Off>::v;
+                    cur[k + S + 1] = Off(mx);
+                    if (trace) ws[int64_t(s) * s + k + s] = Off(mx);
                 }
+                __syncthreads();
+                if (ak >= -s && ak <= s) {
+                    const int off = cur[ak + S + 1];
+                    if (off >= n) { score = s; break; }
+                }
+                Off* t = prev; prev = cur; cur = t;
             }
-            __syncthreads();
-            if (ak >= -s && ak <= s) {
-                const int off = cur[ak + S + 1];
-                if (reachable(off) && off >= n) { score = s; break; }
-            }
-            int* t = prev; prev = cur; cur = t;
         }
-    }
-    if (lane == 0) L.score[pid] = score;
-    if (!TRACE) return;
-    if (!trace || score < 0) {
-        if (lane == 0 && L.cig_len) L.cig_len[pid] = 0;
+        if (tid == 0) L.score[pid] = score;
+        if (TRACE && wave == 0) {
+            if (!trace || score < 0) {
+                if (lane == 0 && L.cig_len) L.cig_len[pid] = 0;
+            } else {
+                uint8_t* const gcg = L.cigar + L.cig_off[pid];
+                uint8_t* const cg = STAGED ? seq : gcg;    // CIGAR built in LDS when staged
+                int pos = m + n;
+                auto emit = [&](uint8_t c, int cnt) {
+                    if (cnt > pos) cnt = pos;
+                    if (cnt <= 0) return;
+                    for (int x = lane; x < cnt; x += 64) cg[pos - 1 - x] = c;
+                    pos -= cnt;
+                };
+                if (score == 0) {
+                    emit('M', n);
+                } else {
+                    int sc = score, k = ak;
+                    int offset = ws_get(ws, score, k);
+                    int h = offset, v = offset - k;
+                    emit('D', v - m);
+                    emit('I', h - n);
+                    if (v > m) v = m;
+                    if (h > n) h = n;
+                    int top = 0, k0 = 0, d = kTbRows;
+                    while (v > 0 && h > 0 && sc > 0) {
+                        if (d == kTbRows) {
+                            // rows sc-1 .. sc-kTbRows, diagonals k-kTbRows .. k+kTbRows
+                            top = sc;
+                            k0 = k;
+                            d = 0;
+                            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                            __builtin_amdgcn_wave_barrier();
+#pragma unroll 3
+                            for (int e = lane; e < kTbRows * kTbCols; e += 64) {
+                                const int r = top - 1 - e / kTbCols;
+                                const int c = k0 - kTbRows + e % kTbCols;
+                                win[e] = (r >= 0 && c >= -r && c <= r) ? ws[int64_t(r) * r + c + r]
+                                                                        : Off(OffNull<Off>::v);
+                            }
+                            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                            __builtin_amdgcn_wave_barrier();
+                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        }
+                        const Off* wr = win + d * kTbCols + (k - k0 + kTbRows);
+                        const int misms = int(wr[0]) + 1;
+                        const int ins = int(wr[-1]) + 1;
+                        const int del = int(wr[1]);
+                        int best = kWfaNeg, op = 0;
+                        if (misms >= 0 && misms >= best) { best = misms; op = 1; }
+                        if (ins >= 0 && ins > best) { best = ins; op = 2; }
+                        if (del >= 0 && del > best) { best = del; op = 3; }
+                        if (op == 0) break;
+                        emit('M', offset - best);
+                        offset = best;
+                        if (op == 1)      { emit('X', 1); --offset; }
+                        else if (op == 2) { emit('I', 1); --k; --offset; }
+                        else              { emit('D', 1); ++k; }
+                        v = offset - k;
+                        h = offset;
+                        --sc;
+                        ++d;
+                    }
+                    if (v > 0 && h > 0) {
+                        const int nm = v < h ? v : h;
+                        emit('M', nm);
+                        v -= nm;
+                        h -= nm;
+                    }
+                    emit('D', v);
+                    emit('I', h);
+                }
+                if (STAGED) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    for (int x = pos + lane; x < m + n; x += 64) gcg[x] = seq[x];
+                }
+                if (lane == 0) L.cig_len[pid] = m + n - pos;
+            }
+        }
+        // the next pair rewrites the LDS wavefronts, sequences and the stack
         __syncthreads();
-        return;
     }
-    __syncthreads();
-    uint8_t* cg = L.cigar + L.cig_off[pid];
-    int pos = m + n;
-    auto emit = [&](uint8_t c, int cnt) {
-        if (cnt > pos) cnt = pos;
-        if (cnt <= 0) return;
-        for (int q = lane; q < cnt; q += 64) cg[pos - 1 - q] = c;
-        pos -= cnt;
-    };
-    if (score == 0) {
-        emit('M', n);
-    } else {
-        int sc = score, k = ak;
-        int offset = ws_get(ws, score, k);
-        int h = offset, v = offset - k;
-        emit('D', v - m);
-        emit('I', h - n);
-        if (v > m) v = m;
-        if (h > n) h = n;
-        while (v > 0 && h > 0 && sc > 0) {
-            const int sp = sc - 1;
-            const int misms = ws_get(ws, sp, k) + 1;
-            const int ins = ws_get(ws, sp, k - 1) + 1;
-            const int del = ws_get(ws, sp, k + 1);
-            int best = kWfaNeg, op = 0;
-            if (reachable(misms) && misms >= best) { best = misms; op = 1; }
-            if (reachable(ins) && ins > best) { best = ins; op = 2; }
-            if (reachable(del) && del > best) { best = del; op = 3; }
-            if (op == 0) break;
-            emit('M', offset - best);
-            offset = best;
-            if (op == 1)      { emit('X', 1); --offset; }
-            else if (op == 2) { emit('I', 1); --k; --offset; }
-            else              { emit('D', 1); ++k; }
-            v = offset - k;
-            h = offset;
-            --sc;
-        }
-        if (v > 0 && h > 0) {
-            const int nm = v < h ? v : h;
-            emit('M', nm);
-            v -= nm;
-            h -= nm;
-        }
-        emit('D', v);
-        emit('I', h);
-    }
-    if (lane == 0) L.cig_len[pid] = m + n - pos;
-    __syncthreads();
 }
 
 __global__ void zero_i32_kernel(int32_t* p, int n)
 {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
-}
-
-template <bool TRACE, typename Off>
-__global__ __launch_bounds__(64) void wfa_queue_kernel(WfaQueueLaunch L)
-{
-    extern __shared__ int lds[];
-    const int lane = threadIdx.x;
-    Off* ws = TRACE ? static_cast<Off*>(L.ws) + int64_t(blockIdx.x) * (int64_t(L.smax_cls) + 1) * (int64_t(L.smax_cls) + 1)
-                    : nullptr;
-    for (;;) {
-        // Claim without a branch on the lane id: every lane takes part in one
-        // atomic adding (lane == 0), and lane 0's result is read into a scalar
-        // register.  (With `if (lane == 0) q = atomicAdd(..)` and a shuffle,
-        // the compiler may thread the loop-invariant branch into the loop and
-        // leave lanes 1..63 looping on a stale claim while lane 0 is masked off.)
-        const int mine = atomicAdd(L.next, lane == 0 ? 1 : 0);
-        const int q = __builtin_amdgcn_readfirstlane(mine);
-        if (q >= L.n) break;                      // every wave ends here once the queue is empty
-        const int pid = L.order ? L.order[q] : q;
-        wfa_pair_q<TRACE, Off>(L, pid, lane, lds, ws);
-    }
 }
 
 }  // namespace
@@ -233,41 +343,100 @@ __global__ __launch_bounds__(64) void wfa_queue_kernel(WfaQueueLaunch L)
 
 namespace ga {
 
-int wfa_queue_waves(int smax_cls, int n)
+namespace {
+
+int device_cus()
 {
-    // LDS per wave: two wavefronts of 2*smax+3 ints; at most 16 waves per CU.
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
+    static int cus[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cus[dev] == 0) {
         hipDeviceProp_t p{};
-        if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) cus = p.multiProcessorCount;
+        cus[dev] = (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0)
+                       ? p.multiProcessorCount : 256;
     }
-    const size_t lds = size_t(2) * size_t(2 * smax_cls + 3) * sizeof(int);
-    int per_cu = int((160 * 1024) / (lds > 0 ? lds : 1));
-    if (per_cu > 16) per_cu = 16;
-    if (per_cu < 1) per_cu = 1;
-    const int waves = cus * per_cu;
-    return n < waves ? n : waves;
+    return cus[dev];
 }
 
-size_t wfa_queue_ws_bytes(int smax_cls, int n_waves, bool wide)
+template <bool TRACE, typename Off, bool STAGED, int WAVES>
+hipError_t launch_one(const WfaQueueLaunch& L, size_t lds, hipStream_t st)
 {
-    return size_t(n_waves) * size_t(smax_cls + 1) * size_t(smax_cls + 1) * (wide ? 4u : 2u);
+    auto k = wfa_kernel<TRACE, Off, STAGED, WAVES>;
+    hipLaunchKernelGGL(k, dim3(unsigned(L.n_groups)), dim3(64 * WAVES), lds, st, L);
+    return hipGetLastError();
+}
+
+template <bool TRACE, typename Off, bool STAGED, int WAVES>
+int resident_groups(size_t lds)
+{
+    int nb = 0;
+    auto k = wfa_kernel<TRACE, Off, STAGED, WAVES>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, 64 * WAVES, lds) != hipSuccess || nb < 1) nb = 1;
+    return nb;
+}
+
+template <bool TRACE, typename Off, bool STAGED>
+hipError_t dispatch_waves(const WfaQueueLaunch& L, size_t lds, hipStream_t st, int* groups)
+{
+    switch (L.waves) {
+    case 1: if (groups) { *groups = resident_groups<TRACE, Off, STAGED, 1>(lds); return hipSuccess; }
+            return launch_one<TRACE, Off, STAGED, 1>(L, lds, st);
+    case 2: if (groups) { *groups = resident_groups<TRACE, Off, STAGED, 2>(lds); return hipSuccess; }
+            return launch_one<TRACE, Off, STAGED, 2>(L, lds, st);
+    case 4: if (groups) { *groups = resident_groups<TRACE, Off, STAGED, 4>(lds); return hipSuccess; }
+            return launch_one<TRACE, Off, STAGED, 4>(L, lds, st);
+    default: if (groups) { *groups = resident_groups<TRACE, Off, STAGED, 8>(lds); return hipSuccess; }
+            return launch_one<TRACE, Off, STAGED, 8>(L, lds, st);
+    }
+}
+
+// groups != null: report resident workgroups per CU instead of launching
+hipError_t dispatch(const WfaQueueLaunch& L, size_t lds, hipStream_t st, int* groups)
+{
+    const bool tr = L.cigar != nullptr;
+    if (L.wide) {
+        if (tr) return L.staged ? dispatch_waves<true, int32_t, true>(L, lds, st, groups)
+                                : dispatch_waves<true, int32_t, false>(L, lds, st, groups);
+        return L.staged ? dispatch_waves<false, int32_t, true>(L, lds, st, groups)
+                        : dispatch_waves<false, int32_t, false>(L, lds, st, groups);
+    }
+    if (tr) return L.staged ? dispatch_waves<true, int16_t, true>(L, lds, st, groups)
+                            : dispatch_waves<true, int16_t, false>(L, lds, st, groups);
+    return L.staged ? dispatch_waves<false, int16_t, true>(L, lds, st, groups)
+                    : dispatch_waves<false, int16_t, false>(L, lds, st, groups);
+}
+
+}  // namespace
+
+size_t wfa_lds_bytes(const WfaQueueLaunch& L)
+{
+    size_t b = 16 + 2 * size_t(wfa_wf_bytes(L.smax_cls, L.wide ? 4 : 2)) + size_t(wfa_win_bytes(L.wide ? 4 : 2));
+    if (L.staged) b += size_t(wfa_seq_bytes(L.max_plen)) + size_t(wfa_seq_bytes(L.max_tlen));
+    return b;
+}
+
+int wfa_queue_groups(const WfaQueueLaunch& L)
+{
+    int per_cu = 1;
+    (void)dispatch(L, wfa_lds_bytes(L), nullptr, &per_cu);
+    const int64_t g = int64_t(device_cus()) * per_cu;
+    return int(g < L.n ? g : L.n);
+}
+
+size_t wfa_queue_ws_bytes(int smax_cls, int n_groups, bool wide)
+{
+    return size_t(n_groups) * size_t(smax_cls + 1) * size_t(smax_cls + 1) * (wide ? 4u : 2u);
 }
 
 hipError_t wfa_queue_launch(const WfaQueueLaunch& L, hipStream_t st)
 {
-    if (L.n <= 0 || L.n_waves <= 0) return hipSuccess;
-    const size_t lds = size_t(2) * size_t(2 * L.smax_cls + 3) * sizeof(int);
+    if (L.n <= 0 || L.n_groups <= 0) return hipSuccess;
     // the claim counter is zeroed by a kernel, not a memset node, so this
     // submission can be captured into a graph and replayed
     hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, st, L.next, 1);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const dim3 grid(unsigned(L.n_waves)), block(64);
-    if (!L.cigar)     hipLaunchKernelGGL((wfa_queue_kernel<false, int16_t>), grid, block, lds, st, L);
-    else if (L.wide)  hipLaunchKernelGGL((wfa_queue_kernel<true, int32_t>), grid, block, lds, st, L);
-    else              hipLaunchKernelGGL((wfa_queue_kernel<true, int16_t>), grid, block, lds, st, L);
-    return hipGetLastError();
+    return dispatch(L, wfa_lds_bytes(L), st, nullptr);
 }
 
 }  // namespace ga
