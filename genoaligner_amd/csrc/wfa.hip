@@ -81,38 +81,70 @@ __device__ __forceinline__ int ws_get(const Off* ws, int s, int k)
     return ws[int64_t(s) * s + k + s];
 }
 
-// 4 bytes of an LDS byte array starting at any index: two aligned 32-bit
-// reads and one byte funnel shift (v_alignbyte_b32).
-__device__ __forceinline__ uint32_t lds_load4(const uint32_t* w, int i)
+// 2-bit packed sequences in LDS: base i is bits 2(i mod 16) .. +1 of word
+// i / 16, code (byte >> 1) & 3 (A 0, C 1, T 2, G 3).  Used only for pairs whose
+// bytes are all A, C, G or T, where code equality is byte equality.
+__device__ __forceinline__ uint32_t base_code(uint32_t b) { return (b >> 1) & 3u; }
+__device__ __forceinline__ bool is_acgt(uint32_t b) { return b == 'A' || b == 'C' || b == 'G' || b == 'T'; }
+
+// Bases i .. i+15 as 32 bits: two aligned reads and one bit funnel shift
+// (v_alignbit_b32).
+__device__ __forceinline__ uint32_t lds_load16(const uint32_t* w, int i)
 {
-    const uint32_t lo = w[i >> 2];
-    const uint32_t hi = w[(i >> 2) + 1];
-    return __builtin_amdgcn_alignbyte(hi, lo, uint32_t(i) & 3u);
+    const uint32_t lo = w[i >> 4];
+    const uint32_t hi = w[(i >> 4) + 1];
+    return __builtin_amdgcn_alignbit(hi, lo, (uint32_t(i) & 15u) * 2u);
 }
 
-// Run of equal bytes P[v..] == T[h..], capped at lim (lim >= 0; P[v], T[h]
-// must lie inside the staged buffers even when lim == 0).  One loop exit, so
+// Run of equal bases P[v..] == T[h..], capped at lim (lim >= 0; bases v and h
+// must lie inside the staged words even when lim == 0).  One loop exit, so
 // lanes leave the loop without extra exec-mask bookkeeping.
 __device__ __forceinline__ int lds_match_run(const uint32_t* P, int v, const uint32_t* T, int h, int lim)
 {
-    int run = -4;
+    int run = -16;
     uint32_t x;
     do {
-        run += 4;
-        x = lds_load4(P, v + run) ^ lds_load4(T, h + run);
-    } while (x == 0 && run + 4 < lim);
-    run += x ? int(__builtin_ctz(x) >> 3) : 4;
+        run += 16;
+        x = lds_load16(P, v + run) ^ lds_load16(T, h + run);
+    } while (x == 0 && run + 16 < lim);
+    run += x ? int(__builtin_ctz(x) >> 1) : 16;
     return run < lim ? run : lim;
 }
 
+// Stage one sequence as 2-bit words (16 bases per thread per step, two spare
+// words past the end); returns false (for this thread) on any non-ACGT byte.
+template <int NT>
+__device__ __forceinline__ bool stage2(uint32_t* dst, const uint8_t* src, int len, int tid)
+{
+    bool ok = true;
+    const int words = ((len + 15) >> 4) + 2;
+    for (int x = tid; x < words; x += NT) {
+        uint32_t w = 0;
+        const int base = 16 * x;
+        if (base < len) {
+            const uint64_t a = load8(src + base);
+            const uint64_t b = load8(src + base + 8);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const uint32_t c = uint32_t(((j < 8 ? a : b) >> (8 * (j & 7))) & 0xffu);
+                if (base + j < len) {
+                    ok = ok && is_acgt(c);
+                    w |= base_code(c) << (2 * j);
+                }
+            }
+        }
+        dst[x] = w;
+    }
+    return ok;
+}
+
 // LDS layout of a workgroup: [claim slots: 4 ints][wavefront A][wavefront B]
-// [traceback window][pattern][text], each region 16-byte aligned.  After the
-// forward pass the pattern + text region (>= m + n bytes) holds the CIGAR.
+// [traceback window][pattern, 2-bit][text, 2-bit], each region 16-byte aligned.
 __host__ __device__ inline int wfa_wf_bytes(int smax_cls, int off_bytes)
 {
     return ((2 * smax_cls + 3) * off_bytes + 15) & ~15;
 }
-__host__ __device__ inline int wfa_seq_bytes(int len) { return ((len > 0 ? len : 0) + 31) & ~15; }
+__host__ __device__ inline int wfa_seq_bytes(int len) { return ((((len > 0 ? len : 0) + 15) / 16 + 2) * 4 + 15) & ~15; }
 
 // Traceback window: kTbRows rows of the stack below the current score, at
 // diagonals k0 - kTbRows .. k0 + kTbRows, loaded with one batch of coalesced
@@ -120,6 +152,54 @@ __host__ __device__ inline int wfa_seq_bytes(int len) { return ((len > 0 ? len :
 constexpr int kTbRows = 16;
 constexpr int kTbCols = 2 * kTbRows + 1;
 __host__ __device__ inline int wfa_win_bytes(int off_bytes) { return (kTbRows * kTbCols * off_bytes + 15) & ~15; }
+
+// Forward pass over one pair: wavefronts s = 1 .. smax until W[s][n-m] >= n.
+// Returns the score or -1.  PACKED: extend over the 2-bit LDS copies, else over
+// the byte sequences in global memory.
+template <bool TRACE, typename Off, bool PACKED, int NT>
+__device__ __forceinline__ int wfa_forward(const uint32_t* sP, const uint32_t* sT, const uint8_t* P,
+                                           const uint8_t* T, int m, int n, int smax, int S,
+                                           Off* A, Off* B, Off* ws, bool trace, int tid)
+{
+    if (tid == 0) {
+        const int o0 = PACKED ? lds_match_run(sP, 0, sT, 0, m < n ? m : n) : match_run(P, T, m < n ? m : n);
+        A[S + 1] = Off(o0);
+        if (trace) ws[0] = Off(o0);
+    }
+    __syncthreads();
+    const int ak = n - m;
+    if (ak == 0 && int(A[S + 1]) >= n) return 0;
+    Off* prev = A;
+    Off* cur = B;
+    for (int s = 1; s <= smax; ++s) {
+        for (int k = -s + tid; k <= s; k += NT) {
+            const int ins = prev[k + S];
+            const int mis = prev[k + S + 1];
+            const int del = prev[k + S + 2];
+            const int mid = ins > mis ? ins : mis;
+            int mx = del > mid + 1 ? del : mid + 1;
+            const int v = mx - k;
+            const bool ok = mx >= 0 && mx <= n && v <= m;
+            if (PACKED) {
+                // invalid diagonals probe (0, 0) with lim 0 and are dropped
+                const int lim = ok ? ((m - v) < (n - mx) ? (m - v) : (n - mx)) : 0;
+                const int run = lds_match_run(sP, ok ? v : 0, sT, ok ? mx : 0, lim);
+                mx = ok ? mx + run : OffNull<Off>::v;
+            } else if (ok) {
+                const int lim = (m - v) < (n - mx) ? (m - v) : (n - mx);
+                mx += match_run(P + v, T + mx, lim);
+            } else {
+                mx = OffNull<Off>::v;
+            }
+            cur[k + S + 1] = Off(mx);
+            if (TRACE && trace) ws[int64_t(s) * s + k + s] = Off(mx);
+        }
+        __syncthreads();
+        if (ak >= -s && ak <= s && int(cur[ak + S + 1]) >= n) return s;
+        Off* t = prev; prev = cur; cur = t;
+    }
+    return -1;
+}
 
 // Eight waves per SIMD: <= 64 VGPRs and <= 96 SGPRs, so four 8-wave
 // workgroups (or 32 single waves) fit a CU when LDS allows.
@@ -192,67 +272,23 @@ void wfa_kernel(WfaQueueLaunch L)
             A[x] = Off(OffNull<Off>::v);
             B[x] = Off(OffNull<Off>::v);
         }
+        // STAGED: 2-bit copies in LDS unless a byte other than A/C/G/T occurs
+        bool packed = false;
         if (STAGED) {
-            // copy 8 bytes per thread per step; the global buffers are readable
-            // past each sequence, and the words past the end are never compared
-            uint64_t* dP = reinterpret_cast<uint64_t*>(sP);
-            uint64_t* dT = reinterpret_cast<uint64_t*>(sT);
-            const int wp = (m + 15) >> 3, wt = (n + 15) >> 3;
-            for (int x = tid; x < wp; x += NT) dP[x] = load8(P + 8 * x);
-            for (int x = tid; x < wt; x += NT) dT[x] = load8(T + 8 * x);
-        }
-        __syncthreads();
-        if (tid == 0) {
-            const int o0 = STAGED ? lds_match_run(sP, 0, sT, 0, m < n ? m : n) : match_run(P, T, m < n ? m : n);
-            A[S + 1] = Off(o0);
-            if (trace) ws[0] = Off(o0);
-        }
-        __syncthreads();
-        const int ak = n - m;
-        int score = -1;
-        if (ak == 0 && int(A[S + 1]) >= n) {
-            score = 0;
+            const bool ok = stage2<NT>(sP, P, m, tid) & stage2<NT>(sT, T, n, tid);
+            packed = __syncthreads_and(ok);
         } else {
-            Off* prev = A;
-            Off* cur = B;
-            for (int s = 1; s <= smax; ++s) {
-                for (int k = -s + tid; k <= s; k += NT) {
-                    const int ins = prev[k + S];
-                    const int mis = prev[k + S + 1];
-                    const int del = prev[k + S + 2];
-                    const int mid = ins > mis ? ins : mis;
-                    int mx = del > mid + 1 ? del : mid + 1;
-                    const int v = mx - k;
-                    const bool ok = mx >= 0 && mx <= n && v <= m;
-                    if (STAGED) {
-                        // invalid diagonals probe (0, 0) with lim 0 and are dropped
-                        const int lim = ok ? ((m - v) < (n - mx) ? (m - v) : (n - mx)) : 0;
-                        const int run = lds_match_run(sP, ok ? v : 0, sT, ok ? mx : 0, lim);
-                        mx = ok ? mx + run : OffNull<Off>::v;
-                    } else if (ok) {
-                        const int lim = (m - v) < (n - mx) ? (m - v) : (n - mx);
-                        mx += match_run(P + v, T + mx, lim);
-                    } else {
-                        mx = OffNull<Off>::v;
-                    }
-                    cur[k + S + 1] = Off(mx);
-                    if (trace) ws[int64_t(s) * s + k + s] = Off(mx);
-                }
-                __syncthreads();
-                if (ak >= -s && ak <= s) {
-                    const int off = cur[ak + S + 1];
-                    if (off >= n) { score = s; break; }
-                }
-                Off* t = prev; prev = cur; cur = t;
-            }
+            __syncthreads();
         }
+        const int ak = n - m;
+        const int score = packed ? wfa_forward<TRACE, Off, true, NT>(sP, sT, P, T, m, n, smax, S, A, B, ws, trace, tid)
+                                 : wfa_forward<TRACE, Off, false, NT>(sP, sT, P, T, m, n, smax, S, A, B, ws, trace, tid);
         if (tid == 0) L.score[pid] = score;
         if (TRACE && wave == 0) {
             if (!trace || score < 0) {
                 if (lane == 0 && L.cig_len) L.cig_len[pid] = 0;
             } else {
-                uint8_t* const gcg = L.cigar + L.cig_off[pid];
-                uint8_t* const cg = STAGED ? seq : gcg;    // CIGAR built in LDS when staged
+                uint8_t* const cg = L.cigar + L.cig_off[pid];
                 int pos = m + n;
                 auto emit = [&](uint8_t c, int cnt) {
                     if (cnt > pos) cnt = pos;
@@ -317,12 +353,6 @@ void wfa_kernel(WfaQueueLaunch L)
                     }
                     emit('D', v);
                     emit('I', h);
-                }
-                if (STAGED) {
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    for (int x = pos + lane; x < m + n; x += 64) gcg[x] = seq[x];
                 }
                 if (lane == 0) L.cig_len[pid] = m + n - pos;
             }

@@ -279,3 +279,28 @@ def test_wfa_long_text_wide_stack(gpu, oracle):
     for r, q in zip(b.results, reqs):
         sc, cg = oracle.wfa(q.pattern, q.text, 200)
         assert (r.score, r.cigar) == (sc, cg if q.with_cigar else "")
+
+
+def test_wfa_non_acgt_bytes_across_classes(gpu, oracle):
+    """Pairs holding N, lowercase or other bytes are compared byte for byte
+    (N matches only N) in every size class; all-ACGT neighbours are unaffected."""
+    import genoaligner_amd as ga
+    rng = random.Random(23)
+    reqs, exp = [], []
+    for i in range(160):
+        L = rng.choice([40, 300, 1200, 4000])
+        p = "".join(rng.choice("ACGT") for _ in range(L))
+        t = _mutate(rng, p, rng.uniform(0, 0.1))
+        if i % 3 == 0:
+            j = rng.randrange(len(t)) if t else 0
+            t = t[:j] + rng.choice("NnacgtX") + t[j + 1:]
+        if i % 5 == 0:
+            j = rng.randrange(len(p))
+            p = p[:j] + "N" + p[j + 1:]
+        smax = min(int(0.2 * L) + 1, 2048)
+        reqs.append(_areq(ga, p, t, smax, i % 4 != 0))
+        sc, cg = oracle.wfa(p, t, smax)
+        exp.append((sc, cg if i % 4 != 0 else ""))
+    b = ga.align_batch_ext(reqs)
+    assert b.ok()
+    assert [(r.score, r.cigar) for r in b.results] == exp
