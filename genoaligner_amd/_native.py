@@ -25,6 +25,8 @@ EXPORTS = (
     "ga_backend_name", "ga_device_name", "ga_device_available",
     "ga_cigar_arena_bytes_wfa", "ga_cigar_arena_bytes_sw",
     "ga_wfa_device", "ga_sw_device", "ga_set_kernel_timing", "ga_last_kernel_ms",
+    "ga_sketch_device", "ga_mlp_forward_device", "ga_mlp_backward_device", "ga_mlp_sgd_device",
+    "ga_mlp_route_device", "ga_route_bucket_bound", "ga_wfa_routed_device",
 )
 
 
@@ -65,6 +67,10 @@ class WfaDevBatch(C.Structure):
                 ("cigar_off", C.c_void_p), ("cigar_len", C.c_void_p),
                 ("want_cigar", C.c_void_p), ("host_smax", C.c_void_p),
                 ("host_pattern_len", C.c_void_p), ("host_text_len", C.c_void_p)]
+
+
+class RouteStats(C.Structure):
+    _fields_ = [("shortcut", C.c_int64), ("routed", C.c_int64), ("plain", C.c_int64), ("rerun", C.c_int64)]
 
 
 class SwDevBatch(C.Structure):
@@ -110,5 +116,17 @@ def lib() -> C.CDLL:
         L.ga_set_kernel_timing.restype = None
         L.ga_last_kernel_ms.argtypes = []
         L.ga_last_kernel_ms.restype = C.c_float
+        vp, i64, f32 = C.c_void_p, C.c_int64, C.c_float
+        L.ga_sketch_device.argtypes = [C.POINTER(WfaDevBatch), vp, vp, vp, cpp]
+        L.ga_mlp_forward_device.argtypes = [vp, vp, i64, vp, vp, vp, vp, cpp]
+        L.ga_mlp_backward_device.argtypes = [vp, vp, vp, vp, vp, vp, i64, vp, vp, vp, vp, cpp]
+        L.ga_mlp_sgd_device.argtypes = [vp, vp, vp, f32, f32, vp, cpp]
+        L.ga_mlp_route_device.argtypes = [vp, i64, f32, vp, vp, cpp]
+        L.ga_wfa_routed_device.argtypes = [C.POINTER(WfaDevBatch), vp, f32, C.POINTER(RouteStats), vp, cpp]
+        for f in (L.ga_sketch_device, L.ga_mlp_forward_device, L.ga_mlp_backward_device,
+                  L.ga_mlp_sgd_device, L.ga_mlp_route_device, L.ga_wfa_routed_device):
+            f.restype = C.c_int
+        L.ga_route_bucket_bound.argtypes = [C.c_int32]
+        L.ga_route_bucket_bound.restype = C.c_int32
         _lib = L
     return _lib

@@ -47,6 +47,39 @@ int wfa_queue_groups(const WfaQueueLaunch& L);     // resident workgroups (<= n)
 size_t wfa_queue_ws_bytes(int smax_cls, int n_groups, bool wide);
 hipError_t wfa_queue_launch(const WfaQueueLaunch& L, hipStream_t st);
 
+// ---------------------------------------------------------------- router
+constexpr int kSketchDim = 64;
+constexpr int kHidden = 128;
+constexpr int kBuckets = 16;
+// parameter offsets (floats) of the MLP 64 -> 128 -> 128 -> 16; W row-major [in][out]
+struct MlpLayout {
+    static constexpr int w1 = 0;
+    static constexpr int b1 = w1 + kSketchDim * kHidden;
+    static constexpr int w2 = b1 + kHidden;
+    static constexpr int b2 = w2 + kHidden * kHidden;
+    static constexpr int w3 = b2 + kHidden;
+    static constexpr int b3 = w3 + kHidden * kBuckets;
+    static constexpr int total = b3 + kBuckets;
+};
+constexpr int kMlpParams = MlpLayout::total;     // 26,896
+
+struct SketchLaunch {
+    const uint8_t* pat; const int64_t* poff; const int32_t* plen;
+    const uint8_t* txt; const int64_t* toff; const int32_t* tlen;
+    int64_t n;
+    float* feat;             // n * kSketchDim
+    int32_t* lower_bound;    // n
+};
+hipError_t sketch_launch(const SketchLaunch& L, hipStream_t st);
+hipError_t mlp_forward(const float* params, const float* X, int64_t N, float* H1, float* H2, float* logits,
+                       hipStream_t st);
+hipError_t mlp_backward(const float* params, const float* X, const float* H1, const float* H2, const float* logits,
+                        const int32_t* label, int64_t N, float* grads, float* loss, float* dL, float* dH2,
+                        float* dH1, hipStream_t st);
+hipError_t mlp_sgd(float* params, const float* grads, float* mom, float lr, float mu, hipStream_t st);
+hipError_t mlp_route(const float* logits, int64_t N, float q, int32_t* bucket, hipStream_t st);
+hipError_t mark_unresolved_launch(const int32_t* ids, int64_t k, int32_t* score, int32_t* cig_len, hipStream_t st);
+
 struct SwLaunch {
     const uint8_t* txt; const int64_t* toff; const int32_t* tlen;   // rows
     const uint8_t* pat; const int64_t* poff; const int32_t* plen;   // columns

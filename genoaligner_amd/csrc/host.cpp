@@ -265,11 +265,15 @@ struct WfaPlan {
 
 // Classes by smax; inside a class the longest pairs are claimed first (a
 // counting sort on (pattern_len + text_len) / 32, so planning stays O(n)).
-WfaPlan plan_wfa(int64_t n, const int32_t* smax, const int32_t* plen, const int32_t* tlen)
+// `ids` (optional) restricts the plan to those pairs.
+WfaPlan plan_wfa(int64_t n, const int32_t* smax, const int32_t* plen, const int32_t* tlen,
+                 const std::vector<int32_t>* ids = nullptr)
 {
     constexpr int kKeys = 1 << 14;
     WfaPlan P;
-    std::vector<int> cls(static_cast<size_t>(n));
+    const int64_t cnt = ids ? int64_t(ids->size()) : n;
+    auto pair_at = [&](int64_t x) { return ids ? int64_t((*ids)[size_t(x)]) : x; };
+    std::vector<int> cls(static_cast<size_t>(cnt));
     std::vector<int64_t> hist(size_t(kWfaNumClasses) * kKeys, 0);
     std::vector<int> cmax(kWfaNumClasses, 0), pmax(kWfaNumClasses, 0), tmax(kWfaNumClasses, 0);
     std::vector<int64_t> ccount(kWfaNumClasses, 0);
@@ -277,10 +281,11 @@ WfaPlan plan_wfa(int64_t n, const int32_t* smax, const int32_t* plen, const int3
         const int64_t l = (int64_t(std::max(plen[i], 0)) + std::max(tlen[i], 0)) >> 5;
         return int(kKeys - 1 - std::min<int64_t>(l, kKeys - 1));      // longest first
     };
-    for (int64_t i = 0; i < n; ++i) {
+    for (int64_t x = 0; x < cnt; ++x) {
+        const int64_t i = pair_at(x);
         int c = 0;
         while (c < kWfaNumClasses - 1 && smax[i] > kWfaClassBound[c]) ++c;
-        cls[size_t(i)] = c;
+        cls[size_t(x)] = c;
         cmax[size_t(c)] = std::max(cmax[size_t(c)], int(smax[i]));
         pmax[size_t(c)] = std::max(pmax[size_t(c)], int(plen[i]));
         tmax[size_t(c)] = std::max(tmax[size_t(c)], int(tlen[i]));
@@ -301,9 +306,11 @@ WfaPlan plan_wfa(int64_t n, const int32_t* smax, const int32_t* plen, const int3
             at += h;
         }
     }
-    P.order.resize(static_cast<size_t>(n));
-    for (int64_t i = 0; i < n; ++i)
-        P.order[size_t(hist[size_t(cls[size_t(i)]) * kKeys + size_t(key(i))]++)] = int32_t(i);
+    P.order.resize(static_cast<size_t>(cnt));
+    for (int64_t x = 0; x < cnt; ++x) {
+        const int64_t i = pair_at(x);
+        P.order[size_t(hist[size_t(cls[size_t(x)]) * kKeys + size_t(key(i))]++)] = int32_t(i);
+    }
     return P;
 }
 
@@ -323,22 +330,12 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
         return "hipMalloc(WFA queue)";
     if (hipMemcpyAsync(d_order, plan.order.data(), plan.order.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
         return "hipMemcpy(WFA queue)";
+    // set-up first (launch shapes, stacks), so the timed region holds only GPU work
     const char* err = nullptr;
-    if (timer) timer->begin(st);
-    hipEvent_t fork = nullptr;
-    std::vector<hipStream_t> side;
-    std::vector<hipEvent_t> joins;
-    if (nc > 1 && (hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess ||
-                   hipEventRecord(fork, st) != hipSuccess))
-        err = "hipEventRecord";
+    std::vector<ga::WfaQueueLaunch> Ls(nc);
     for (size_t c = 0; c < nc && !err; ++c) {
-        hipStream_t cs = st;
-        if (c > 0) {
-            cs = acquire_stream();
-            if (!cs || hipStreamWaitEvent(cs, fork, 0) != hipSuccess) { err = "hipStreamWaitEvent"; break; }
-            side.push_back(cs);
-        }
-        ga::WfaQueueLaunch L = base;
+        ga::WfaQueueLaunch& L = Ls[c];
+        L = base;
         L.order = d_order + plan.first[c];
         L.n = int32_t(plan.count[c]);
         L.next = d_next + c;
@@ -352,14 +349,25 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
         const int64_t per_group = int64_t(ga::wfa_queue_ws_bytes(L.smax_cls, 1, L.wide));
         if (trace) groups = int(std::max<int64_t>(1, std::min<int64_t>(groups, kWfaStackBudget / per_group)));
         L.n_groups = groups;
-        void* ws = nullptr;
-        if (trace && hipMallocAsync(&ws, size_t(per_group) * size_t(groups), cs) != hipSuccess) {
+        L.ws = nullptr;
+        if (trace && hipMallocAsync(&L.ws, size_t(per_group) * size_t(groups), st) != hipSuccess)
             err = "hipMalloc(wavefront stacks)";
-            break;
+    }
+    if (timer && !err) timer->begin(st);
+    hipEvent_t fork = nullptr;
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> joins;
+    if (!err && nc > 1 && (hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess ||
+                           hipEventRecord(fork, st) != hipSuccess))
+        err = "hipEventRecord";
+    for (size_t c = 0; c < nc && !err; ++c) {
+        hipStream_t cs = st;
+        if (c > 0) {
+            cs = acquire_stream();
+            if (!cs || hipStreamWaitEvent(cs, fork, 0) != hipSuccess) { err = "hipStreamWaitEvent"; break; }
+            side.push_back(cs);
         }
-        L.ws = ws;
-        if (ga::wfa_queue_launch(L, cs) != hipSuccess) err = "WFA kernel launch";
-        if (ws) (void)hipFreeAsync(ws, cs);
+        if (ga::wfa_queue_launch(Ls[c], cs) != hipSuccess) err = "WFA kernel launch";
         if (c > 0) {
             hipEvent_t j = nullptr;
             if (hipEventCreateWithFlags(&j, hipEventDisableTiming) == hipSuccess &&
@@ -369,10 +377,12 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
                 err = "hipEventRecord";
         }
     }
-    if (timer) timer->end(st);
+    if (timer && !err) timer->end(st);
     for (hipEvent_t j : joins) (void)hipEventDestroy(j);
     if (fork) (void)hipEventDestroy(fork);
     for (hipStream_t cs : side) release_stream(cs);
+    for (auto& L : Ls)
+        if (L.ws) (void)hipFreeAsync(L.ws, st);
     (void)hipFreeAsync(d_order, st);
     (void)hipFreeAsync(d_next, st);
     return err;
@@ -771,6 +781,18 @@ void reset_sw(ga_sw_res* res, int64_t n)
     for (int64_t i = 0; i < n; ++i) res[i] = ga_sw_res{-1, 0, 0, 0, -1, -1, -1, -1, 0, 0, 0};
 }
 
+// Upper distance bound of each router bucket; the last bucket means "above
+// the pair's smax" (unresolved).
+constexpr int32_t kBucketBound[16] = {0, 1, 2, 4, 8, 16, 32, 64, 128, 256, 384, 512, 768, 1024, 2048, INT32_MAX};
+
+const char* check_dev_batch(const ga_wfa_dev_batch* b)
+{
+    if (!b) return "null batch";
+    if (b->smax_max < 0 || b->smax_max > GA_MAX_SMAX_EXT) return "smax_max out of range [0, 4000]";
+    if (b->n_pairs > INT32_MAX) return "too many pairs";
+    return nullptr;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1056,6 +1078,179 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)
     }
     release();
     return rc;
+}
+
+// ------------------------------------------------------------------ router
+int ga_sketch_device(const ga_wfa_dev_batch* b, float* feat, int32_t* lower_bound, void* stream,
+                     const char** error)
+{
+    if (error) *error = nullptr;
+    if (const char* e = check_dev_batch(b)) { if (error) *error = e; return GA_INVALID_ARGUMENT; }
+    if (b->n_pairs <= 0) return GA_OK;
+    if (!feat || !lower_bound) { if (error) *error = "null output"; return GA_INVALID_ARGUMENT; }
+    ga::SketchLaunch L{b->pattern, b->pattern_off, b->pattern_len, b->text, b->text_off, b->text_len,
+                       b->n_pairs, feat, lower_bound};
+    if (ga::sketch_launch(L, static_cast<hipStream_t>(stream)) != hipSuccess) {
+        if (error) *error = "sketch kernel launch";
+        return GA_DEVICE_ERROR;
+    }
+    return GA_OK;
+}
+
+int ga_mlp_forward_device(const float* params, const float* feat, int64_t n, float* h1, float* h2,
+                          float* logits, void* stream, const char** error)
+{
+    if (error) *error = nullptr;
+    if (n < 0 || !params || (n > 0 && (!feat || !h1 || !h2 || !logits))) {
+        if (error) *error = "invalid MLP arguments";
+        return GA_INVALID_ARGUMENT;
+    }
+    if (ga::mlp_forward(params, feat, n, h1, h2, logits, static_cast<hipStream_t>(stream)) != hipSuccess) {
+        if (error) *error = "MLP forward launch";
+        return GA_DEVICE_ERROR;
+    }
+    return GA_OK;
+}
+
+int ga_mlp_backward_device(const float* params, const float* feat, const float* h1, const float* h2,
+                           const float* logits, const int32_t* label, int64_t n, float* grads, float* loss,
+                           float* work, void* stream, const char** error)
+{
+    if (error) *error = nullptr;
+    if (n <= 0 || !params || !feat || !h1 || !h2 || !logits || !label || !grads || !loss || !work) {
+        if (error) *error = "invalid MLP arguments";
+        return GA_INVALID_ARGUMENT;
+    }
+    float* dL = work;
+    float* dH2 = dL + n * ga::kBuckets;
+    float* dH1 = dH2 + n * ga::kHidden;
+    if (ga::mlp_backward(params, feat, h1, h2, logits, label, n, grads, loss, dL, dH2, dH1,
+                         static_cast<hipStream_t>(stream)) != hipSuccess) {
+        if (error) *error = "MLP backward launch";
+        return GA_DEVICE_ERROR;
+    }
+    return GA_OK;
+}
+
+int ga_mlp_sgd_device(float* params, const float* grads, float* momentum, float lr, float mu, void* stream,
+                      const char** error)
+{
+    if (error) *error = nullptr;
+    if (!params || !grads || !momentum) { if (error) *error = "null MLP buffer"; return GA_INVALID_ARGUMENT; }
+    if (ga::mlp_sgd(params, grads, momentum, lr, mu, static_cast<hipStream_t>(stream)) != hipSuccess) {
+        if (error) *error = "SGD launch";
+        return GA_DEVICE_ERROR;
+    }
+    return GA_OK;
+}
+
+int ga_mlp_route_device(const float* logits, int64_t n, float q, int32_t* bucket, void* stream,
+                        const char** error)
+{
+    if (error) *error = nullptr;
+    if (n < 0 || (n > 0 && (!logits || !bucket))) { if (error) *error = "invalid route arguments"; return GA_INVALID_ARGUMENT; }
+    if (ga::mlp_route(logits, n, q, bucket, static_cast<hipStream_t>(stream)) != hipSuccess) {
+        if (error) *error = "route launch";
+        return GA_DEVICE_ERROR;
+    }
+    return GA_OK;
+}
+
+int32_t ga_route_bucket_bound(int32_t bucket)
+{
+    return (bucket >= 0 && bucket < ga::kBuckets) ? kBucketBound[bucket] : -1;
+}
+
+int ga_wfa_routed_device(const ga_wfa_dev_batch* b, const float* params, float q, ga_route_stats* stats,
+                         void* stream, const char** error)
+{
+    if (error) *error = nullptr;
+    if (stats) *stats = ga_route_stats{};
+    if (const char* e = check_dev_batch(b)) { if (error) *error = e; return GA_INVALID_ARGUMENT; }
+    if (!params) { if (error) *error = "null MLP parameters"; return GA_INVALID_ARGUMENT; }
+    if (!b->host_smax || !b->host_pattern_len || !b->host_text_len) {
+        if (error) *error = "routing needs the host copies of smax and the lengths";
+        return GA_INVALID_ARGUMENT;
+    }
+    const int64_t n = b->n_pairs;
+    if (n <= 0) return GA_OK;
+    std::call_once(g_pool_once, init_pool);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    DevArena D;
+    D.st = st;
+    float* feat = D.alloc<float>(size_t(n) * ga::kSketchDim);
+    int32_t* lb = D.alloc<int32_t>(size_t(n));
+    float* h1 = D.alloc<float>(size_t(n) * ga::kHidden);
+    float* h2 = D.alloc<float>(size_t(n) * ga::kHidden);
+    float* logits = D.alloc<float>(size_t(n) * ga::kBuckets);
+    int32_t* bucket = D.alloc<int32_t>(size_t(n));
+    const char* err = D.ok ? nullptr : "hipMalloc(router buffers)";
+    ga::SketchLaunch S{b->pattern, b->pattern_off, b->pattern_len, b->text, b->text_off, b->text_len, n, feat, lb};
+    if (!err && (ga::sketch_launch(S, st) != hipSuccess || ga::mlp_forward(params, feat, n, h1, h2, logits, st) != hipSuccess ||
+                 ga::mlp_route(logits, n, q, bucket, st) != hipSuccess))
+        err = "router kernel launch";
+    std::vector<int32_t> hb(static_cast<size_t>(n)), hlb(static_cast<size_t>(n));
+    if (!err && (hipMemcpyAsync(hb.data(), bucket, size_t(n) * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                 hipMemcpyAsync(hlb.data(), lb, size_t(n) * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                 hipStreamSynchronize(st) != hipSuccess))
+        err = "hipMemcpy(router decisions)";
+    // first pass: each pair under the bound of its predicted bucket; pairs whose
+    // exact lower bound exceeds their smax are unresolved without an alignment
+    std::vector<int32_t> sm1(static_cast<size_t>(n)), shortcut, first;
+    ga_route_stats rs{};
+    if (!err) {
+        for (int64_t i = 0; i < n; ++i) {
+            const int32_t smax = b->host_smax[i];
+            // (a pair with an empty side scores the other length whatever its smax)
+            const bool degenerate = b->host_pattern_len[i] <= 0 || b->host_text_len[i] <= 0;
+            if (!degenerate && hlb[size_t(i)] > smax) { shortcut.push_back(int32_t(i)); sm1[size_t(i)] = 0; continue; }
+            sm1[size_t(i)] = std::min(smax, kBucketBound[std::clamp(hb[size_t(i)], 0, ga::kBuckets - 1)]);
+            sm1[size_t(i)] = std::max(sm1[size_t(i)], std::min(smax, hlb[size_t(i)]));   // never below the bound
+            first.push_back(int32_t(i));
+            if (sm1[size_t(i)] < smax) ++rs.routed; else ++rs.plain;
+        }
+        rs.shortcut = int64_t(shortcut.size());
+    }
+    int32_t* d_sm1 = err ? nullptr : D.upload(sm1);
+    int32_t* d_short = (err || shortcut.empty()) ? nullptr : D.upload(shortcut);
+    if (!err && !D.ok) err = "hipMalloc/hipMemcpy (routing)";
+    if (!err && d_short && ga::mark_unresolved_launch(d_short, int64_t(shortcut.size()), b->score,
+                                                      b->cigar ? b->cigar_len : nullptr, st) != hipSuccess)
+        err = "mark kernel launch";
+    ga::WfaQueueLaunch L{};
+    L.pat = b->pattern; L.poff = b->pattern_off; L.plen = b->pattern_len;
+    L.txt = b->text; L.toff = b->text_off; L.tlen = b->text_len;
+    L.want = b->want_cigar;
+    L.score = b->score; L.cigar = b->cigar; L.cig_off = b->cigar_off; L.cig_len = b->cigar_len;
+    L.wide = b->max_len > 32766;
+    if (!err) {
+        L.smax = d_sm1;
+        const WfaPlan p1 = plan_wfa(n, sm1.data(), b->host_pattern_len, b->host_text_len, &first);
+        err = enqueue_wfa(L, p1, b->cigar != nullptr, st, nullptr);
+    }
+    // second pass: pairs not resolved under a bound below their own smax
+    std::vector<int32_t> sc(static_cast<size_t>(n));
+    if (!err && (hipMemcpyAsync(sc.data(), b->score, size_t(n) * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                 hipStreamSynchronize(st) != hipSuccess))
+        err = "hipMemcpy(first-pass scores)";
+    if (!err) {
+        std::vector<int32_t> again;
+        for (int32_t i : first)
+            if (sc[size_t(i)] < 0 && sm1[size_t(i)] < b->host_smax[i]) again.push_back(i);
+        rs.rerun = int64_t(again.size());
+        if (!again.empty()) {
+            L.smax = b->smax;
+            const WfaPlan p2 = plan_wfa(n, b->host_smax, b->host_pattern_len, b->host_text_len, &again);
+            err = enqueue_wfa(L, p2, b->cigar != nullptr, st, nullptr);
+        }
+    }
+    D.release();
+    if (stats) *stats = rs;
+    if (err) {
+        if (error) *error = err;
+        return GA_DEVICE_ERROR;
+    }
+    return GA_OK;
 }
 
 }  // extern "C"

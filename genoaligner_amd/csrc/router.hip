@@ -1,0 +1,281 @@
+// router.hip -- gfx950 kernels of the learned router: the 4-mer/length sketch
+// of a pair, and a small FP32 MLP (64 -> 128 -> 128 -> 16, ReLU) with its
+// forward pass, backward pass, SGD-with-momentum update and the routing rule.
+//
+// Nothing here changes an alignment result: the router only picks the smax a
+// pair is first tried with (and so its kernel size class); a pair that is not
+// resolved under that bound is re-run with its own smax by the host layer.
+// The one exact decision taken here is the q-gram lower bound: every edit
+// changes at most q q-grams of one sequence, so ||Q4(a) - Q4(b)||_1 <= 8 ED
+// (and |m - n| <= ED); a pair whose bound exceeds its smax is unresolved.
+//
+// The MLP is a tiny dense contraction (26,896 parameters); it runs on FP32
+// VALU with LDS-staged input rows.  Accumulation order is fixed (input index
+// ascending, separate multiply and add) so a CPU restatement reproduces it.
+#include "ga_device.hpp"
+
+namespace ga {
+namespace {
+
+__device__ __forceinline__ int code4(uint32_t b)
+{
+    // A 0, C 1, G 2, T 3; anything else -1
+    return b == 'A' ? 0 : b == 'C' ? 1 : b == 'G' ? 2 : b == 'T' ? 3 : -1;
+}
+
+// 4-mer counts of one sequence into an LDS histogram (windows holding a byte
+// other than A/C/G/T are skipped).
+__device__ void count4(const uint8_t* s, int len, int* hist, int lane)
+{
+    for (int i = lane; i + 3 < len; i += 64) {
+        const int a = code4(s[i]), b = code4(s[i + 1]), c = code4(s[i + 2]), d = code4(s[i + 3]);
+        if ((a | b | c | d) >= 0) atomicAdd(&hist[(a << 6) | (b << 4) | (c << 2) | d], 1);
+    }
+}
+
+__device__ __forceinline__ int wave_sum(int v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// One wave per pair: features (kSketchDim floats) and the exact lower bound
+// max(|m - n|, ceil(L1 / 8)) on the edit distance.
+__global__ __launch_bounds__(64) void sketch_kernel(SketchLaunch L)
+{
+    __shared__ int hp[256];
+    __shared__ int ht[256];
+    const int lane = threadIdx.x;
+    const int64_t pid = blockIdx.x;
+    if (pid >= L.n) return;
+    for (int b = lane; b < 256; b += 64) { hp[b] = 0; ht[b] = 0; }
+    __syncthreads();
+    const int m = L.plen[pid], n = L.tlen[pid];
+    count4(L.pat + L.poff[pid], m, hp, lane);
+    count4(L.txt + L.toff[pid], n, ht, lane);
+    __syncthreads();
+    int part = 0;
+    for (int b = lane; b < 256; b += 64) part += abs(hp[b] - ht[b]);
+    const int l1 = wave_sum(part);
+    int grp = 0;                                    // group j = bins b with b % 60 == j
+    if (lane < 60)
+        for (int b = lane; b < 256; b += 60) grp += abs(hp[b] - ht[b]);
+    const int mm = m > 0 ? m : 0, nn = n > 0 ? n : 0;
+    const int mx = (mm > nn ? mm : nn) > 0 ? (mm > nn ? mm : nn) : 1;
+    const int dl = mm > nn ? mm - nn : nn - mm;
+    const float fm = float(mx);
+    float* f = L.feat + pid * kSketchDim;
+    if (lane == 0) {
+        f[0] = float(mm) / 10000.0f;
+        f[1] = float(nn) / 10000.0f;
+        f[2] = float(dl) / fm;
+        f[3] = float(l1) / float(2 * mx);
+        const int qb = (l1 + 7) / 8;
+        L.lower_bound[pid] = dl > qb ? dl : qb;
+    }
+    if (lane < 60) f[4 + lane] = float(grp * 30) / fm;
+}
+
+// ---------------------------------------------------------------- MLP
+// Y[r, o] = act(b[o] + sum_i X[r, i] W[i, o]); a workgroup of 256 threads
+// covers kRows rows, staged in LDS; W (at most 64 KB) is read through the
+// caches, 64 consecutive outputs per wave.
+constexpr int kRows = 32;
+
+template <bool RELU>
+__global__ __launch_bounds__(256) void dense_fwd(const float* X, const float* W, const float* bias,
+                                                float* Y, int64_t N, int I, int O)
+{
+    extern __shared__ float sm[];
+    float* sX = sm;                  // kRows * I
+    const int t = threadIdx.x;
+    const int64_t r0 = int64_t(blockIdx.x) * kRows;
+    for (int x = t; x < kRows * I; x += 256) {
+        const int64_t r = r0 + x / I;
+        sX[x] = r < N ? X[r * I + x % I] : 0.0f;
+    }
+    __syncthreads();
+    for (int x = t; x < kRows * O; x += 256) {
+        const int rr = x / O, o = x % O;
+        const int64_t r = r0 + rr;
+        if (r >= N) continue;
+        float acc = 0.0f;
+        for (int i = 0; i < I; ++i) acc += sX[rr * I + i] * W[i * O + o];
+        acc += bias[o];
+        Y[r * O + o] = RELU ? (acc > 0.0f ? acc : 0.0f) : acc;
+    }
+}
+
+// dX[r, i] = (sum_o dY[r, o] W[i, o]) * (A[r, i] > 0), A the layer input
+// after its ReLU.
+__global__ __launch_bounds__(256) void dense_bwd_data(const float* dY, const float* W, const float* A,
+                                                     float* dX, int64_t N, int I, int O)
+{
+    extern __shared__ float sm[];
+    float* sD = sm;                  // kRows * O
+    const int t = threadIdx.x;
+    const int64_t r0 = int64_t(blockIdx.x) * kRows;
+    for (int x = t; x < kRows * O; x += 256) {
+        const int64_t r = r0 + x / O;
+        sD[x] = r < N ? dY[r * O + x % O] : 0.0f;
+    }
+    __syncthreads();
+    for (int x = t; x < kRows * I; x += 256) {
+        const int rr = x / I, i = x % I;
+        const int64_t r = r0 + rr;
+        if (r >= N) continue;
+        float acc = 0.0f;
+        for (int o = 0; o < O; ++o) acc += sD[rr * O + o] * W[i * O + o];
+        dX[r * I + i] = A[r * I + i] > 0.0f ? acc : 0.0f;
+    }
+}
+
+// dW[i, o] = sum_r X[r, i] dY[r, o]; db[o] = sum_r dY[r, o] (r ascending).
+// One thread per weight; the threads of row i == I compute the bias.
+__global__ __launch_bounds__(256) void dense_bwd_weight(const float* X, const float* dY, float* dW,
+                                                       float* db, int64_t N, int I, int O)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= (I + 1) * O) return;
+    const int i = x / O, o = x % O;
+    float acc = 0.0f;
+    if (i < I) {
+        for (int64_t r = 0; r < N; ++r) acc += X[r * I + i] * dY[r * O + o];
+        dW[i * O + o] = acc;
+    } else {
+        for (int64_t r = 0; r < N; ++r) acc += dY[r * O + o];
+        db[o] = acc;
+    }
+}
+
+// Softmax cross-entropy over kBuckets logits: loss[r] = -log p[label], and
+// dL[r, c] = (p[c] - [c == label]) / N (mean over the batch).
+__global__ __launch_bounds__(256) void softmax_xent(const float* logits, const int32_t* label, float* dL,
+                                                   float* loss, int64_t N)
+{
+    const int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (r >= N) return;
+    const float* z = logits + r * kBuckets;
+    float mx = z[0];
+    for (int c = 1; c < kBuckets; ++c) mx = z[c] > mx ? z[c] : mx;
+    float e[kBuckets];
+    float sum = 0.0f;
+    for (int c = 0; c < kBuckets; ++c) { e[c] = expf(z[c] - mx); sum += e[c]; }
+    const int y = label[r];
+    const float inv_n = 1.0f / float(N);
+    for (int c = 0; c < kBuckets; ++c) dL[r * kBuckets + c] = (e[c] / sum - (c == y ? 1.0f : 0.0f)) * inv_n;
+    loss[r] = logf(sum) - (z[y] - mx);
+}
+
+__global__ __launch_bounds__(256) void sgd_momentum(float* p, const float* g, float* v, int n, float lr, float mu)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= n) return;
+    const float vv = mu * v[x] + g[x];
+    v[x] = vv;
+    p[x] = p[x] - lr * vv;
+}
+
+// Routing rule: the smallest bucket whose cumulative softmax probability
+// reaches q (q = 0: the arg-max bucket).
+__global__ __launch_bounds__(256) void route_buckets(const float* logits, int64_t N, float q, int32_t* bucket)
+{
+    const int64_t r = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (r >= N) return;
+    const float* z = logits + r * kBuckets;
+    float mx = z[0];
+    int am = 0;
+    for (int c = 1; c < kBuckets; ++c)
+        if (z[c] > mx) { mx = z[c]; am = c; }
+    if (q <= 0.0f) { bucket[r] = am; return; }
+    float e[kBuckets];
+    float sum = 0.0f;
+    for (int c = 0; c < kBuckets; ++c) { e[c] = expf(z[c] - mx); sum += e[c]; }
+    float cum = 0.0f;
+    int b = kBuckets - 1;
+    for (int c = 0; c < kBuckets; ++c) {
+        cum += e[c];
+        if (cum >= q * sum) { b = c; break; }
+    }
+    bucket[r] = b;
+}
+
+// score = -1, no CIGAR, for pairs decided without an alignment
+__global__ __launch_bounds__(256) void mark_unresolved(const int32_t* ids, int64_t k, int32_t* score, int32_t* cig_len)
+{
+    const int64_t x = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (x >= k) return;
+    const int32_t pid = ids[x];
+    score[pid] = -1;
+    if (cig_len) cig_len[pid] = 0;
+}
+
+inline unsigned blocks(int64_t n, int per) { return unsigned((n + per - 1) / per); }
+
+}  // namespace
+
+hipError_t sketch_launch(const SketchLaunch& L, hipStream_t st)
+{
+    if (L.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sketch_kernel, dim3(unsigned(L.n)), dim3(64), 0, st, L);
+    return hipGetLastError();
+}
+
+hipError_t mlp_forward(const float* params, const float* X, int64_t N, float* H1, float* H2, float* logits,
+                       hipStream_t st)
+{
+    if (N <= 0) return hipSuccess;
+    const MlpLayout P;
+    const unsigned g = blocks(N, kRows);
+    hipLaunchKernelGGL((dense_fwd<true>), dim3(g), dim3(256), kRows * kSketchDim * 4, st,
+                       X, params + P.w1, params + P.b1, H1, N, kSketchDim, kHidden);
+    hipLaunchKernelGGL((dense_fwd<true>), dim3(g), dim3(256), kRows * kHidden * 4, st,
+                       H1, params + P.w2, params + P.b2, H2, N, kHidden, kHidden);
+    hipLaunchKernelGGL((dense_fwd<false>), dim3(g), dim3(256), kRows * kHidden * 4, st,
+                       H2, params + P.w3, params + P.b3, logits, N, kHidden, kBuckets);
+    return hipGetLastError();
+}
+
+hipError_t mlp_backward(const float* params, const float* X, const float* H1, const float* H2, const float* logits,
+                        const int32_t* label, int64_t N, float* grads, float* loss, float* dL, float* dH2,
+                        float* dH1, hipStream_t st)
+{
+    if (N <= 0) return hipSuccess;
+    const MlpLayout P;
+    const unsigned g = blocks(N, kRows);
+    hipLaunchKernelGGL(softmax_xent, dim3(blocks(N, 256)), dim3(256), 0, st, logits, label, dL, loss, N);
+    hipLaunchKernelGGL(dense_bwd_weight, dim3(blocks((kHidden + 1) * kBuckets, 256)), dim3(256), 0, st,
+                       H2, dL, grads + P.w3, grads + P.b3, N, kHidden, kBuckets);
+    hipLaunchKernelGGL(dense_bwd_data, dim3(g), dim3(256), kRows * kBuckets * 4, st,
+                       dL, params + P.w3, H2, dH2, N, kHidden, kBuckets);
+    hipLaunchKernelGGL(dense_bwd_weight, dim3(blocks((kHidden + 1) * kHidden, 256)), dim3(256), 0, st,
+                       H1, dH2, grads + P.w2, grads + P.b2, N, kHidden, kHidden);
+    hipLaunchKernelGGL(dense_bwd_data, dim3(g), dim3(256), kRows * kHidden * 4, st,
+                       dH2, params + P.w2, H1, dH1, N, kHidden, kHidden);
+    hipLaunchKernelGGL(dense_bwd_weight, dim3(blocks((kSketchDim + 1) * kHidden, 256)), dim3(256), 0, st,
+                       X, dH1, grads + P.w1, grads + P.b1, N, kSketchDim, kHidden);
+    return hipGetLastError();
+}
+
+hipError_t mlp_sgd(float* params, const float* grads, float* mom, float lr, float mu, hipStream_t st)
+{
+    hipLaunchKernelGGL(sgd_momentum, dim3(blocks(kMlpParams, 256)), dim3(256), 0, st, params, grads, mom,
+                       int(kMlpParams), lr, mu);
+    return hipGetLastError();
+}
+
+hipError_t mlp_route(const float* logits, int64_t N, float q, int32_t* bucket, hipStream_t st)
+{
+    if (N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(route_buckets, dim3(blocks(N, 256)), dim3(256), 0, st, logits, N, q, bucket);
+    return hipGetLastError();
+}
+
+hipError_t mark_unresolved_launch(const int32_t* ids, int64_t k, int32_t* score, int32_t* cig_len, hipStream_t st)
+{
+    if (k <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mark_unresolved, dim3(blocks(k, 256)), dim3(256), 0, st, ids, k, score, cig_len);
+    return hipGetLastError();
+}
+
+}  // namespace ga

@@ -24,6 +24,9 @@
 // cooperatively, 64 bytes per store instruction.
 #include "ga_device.hpp"
 
+#include <mutex>
+#include <vector>
+
 namespace ga {
 namespace {
 
@@ -447,8 +450,26 @@ size_t wfa_lds_bytes(const WfaQueueLaunch& L)
 
 int wfa_queue_groups(const WfaQueueLaunch& L)
 {
-    int per_cu = 1;
-    (void)dispatch(L, wfa_lds_bytes(L), nullptr, &per_cu);
+    // resident workgroups per CU, cached per (device, variant, LDS bytes)
+    struct Entry { int dev, variant; size_t lds; int per_cu; };
+    static std::mutex mu;
+    static std::vector<Entry> cache;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const int variant = (L.cigar ? 1 : 0) | (L.wide ? 2 : 0) | (L.staged ? 4 : 0) | (L.waves << 3);
+    const size_t lds = wfa_lds_bytes(L);
+    int per_cu = 0;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (const Entry& e : cache)
+            if (e.dev == dev && e.variant == variant && e.lds == lds) { per_cu = e.per_cu; break; }
+    }
+    if (per_cu == 0) {
+        per_cu = 1;
+        (void)dispatch(L, lds, nullptr, &per_cu);
+        std::lock_guard<std::mutex> lk(mu);
+        cache.push_back({dev, variant, lds, per_cu});
+    }
     const int64_t g = int64_t(device_cus()) * per_cu;
     return int(g < L.n ? g : L.n);
 }

@@ -134,6 +134,45 @@ def wfa_plain(n: int = 10_000, length: int = 150, rate: float = 0.02, smax: int 
                          "smax": smax, "seed": seed})
 
 
+def wfa_routing(n: int = 200_000, lo: int = 100, hi: int = 10_000, smax_frac: float = 0.2,
+                smax_cap: int = 2048, seed: int = 20261205) -> PairSet:
+    """Pairs for the learned router: lengths log-uniform lo..hi; 80% of pairs
+    with divergence U[0, 0.03], 15% U[0.03, 0.15] (edits 2:1:1), 5% unrelated
+    (the text is an independent random sequence whose length is drawn from the
+    same distribution); smax = min(smax_frac * pattern length, smax_cap)."""
+    rng = np.random.default_rng(seed)
+    pl = np.exp(rng.uniform(np.log(lo), np.log(hi), size=n)).astype(np.int32)
+    kind = rng.choice(3, size=n, p=[0.80, 0.15, 0.05])
+    rate = np.where(kind == 0, rng.uniform(0.0, 0.03, size=n), rng.uniform(0.03, 0.15, size=n))
+    pat = random_bases(rng, int(pl.astype(np.int64).sum()))
+    txt, toff, tl = mutate(rng, pat, pl, rate)
+    unrel = np.flatnonzero(kind == 2)
+    if len(unrel):
+        ul = np.exp(rng.uniform(np.log(lo), np.log(hi), size=len(unrel))).astype(np.int32)
+        ub = random_bases(rng, int(ul.astype(np.int64).sum()))
+        # rebuild the text array with the unrelated texts swapped in
+        lens = tl.astype(np.int64).copy()
+        lens[unrel] = ul
+        new_off = _offsets(lens)
+        out = np.empty(int(lens.sum()), dtype=np.uint8)
+        keep = np.ones(n, dtype=bool)
+        keep[unrel] = False
+        ki = np.flatnonzero(keep)
+        kl = lens[ki]
+        within = np.arange(int(kl.sum()), dtype=np.int64) - np.repeat(_offsets(kl), kl)
+        out[np.repeat(new_off[ki], kl) + within] = txt[np.repeat(toff[ki], kl) + within]
+        ull = ul.astype(np.int64)
+        within = np.arange(int(ull.sum()), dtype=np.int64) - np.repeat(_offsets(ull), ull)
+        out[np.repeat(new_off[unrel], ull) + within] = ub
+        txt, toff, tl = out, new_off, lens.astype(np.int32)
+        rate = rate.copy()
+        rate[unrel] = np.nan
+    smax = np.minimum((smax_frac * pl).astype(np.int32), smax_cap).astype(np.int32)
+    return PairSet(pat, _offsets(pl), pl, txt, toff, tl, smax,
+                   meta={"workload": "wfa_routing", "n": n, "lo": lo, "hi": hi, "seed": seed,
+                         "kind": kind, "rate": rate})
+
+
 def sw_embedded(n: int = 50_000, qlen: int = 256, tlen: int = 1024, rate: float = 0.10,
                 seed: int = 20261203) -> PairSet:
     rng = np.random.default_rng(seed)

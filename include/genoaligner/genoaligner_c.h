@@ -163,6 +163,51 @@ typedef struct ga_wfa_dev_batch {
 
 int ga_wfa_device(const ga_wfa_dev_batch* b, void* stream, const char** error);
 
+/* ---- learned router (EXTENSION: no reference counterpart) ------------------
+ * Sketch of a pair: GA_SKETCH_DIM floats of 4-mer / length features and the
+ * exact lower bound max(|m - n|, ceil(||Q4(P) - Q4(T)||_1 / 8)) on its edit
+ * distance.  The MLP (GA_SKETCH_DIM -> 128 -> 128 -> GA_ROUTE_BUCKETS, ReLU,
+ * FP32) keeps its GA_MLP_PARAMS parameters in one device array:
+ * W1[64][128], b1[128], W2[128][128], b2[128], W3[128][16], b3[16]. */
+#define GA_SKETCH_DIM 64
+#define GA_MLP_HIDDEN 128
+#define GA_ROUTE_BUCKETS 16
+#define GA_MLP_PARAMS 26896
+
+int ga_sketch_device(const ga_wfa_dev_batch* b, float* feat, int32_t* lower_bound, void* stream,
+                     const char** error);
+/* h1, h2: n x 128 activations kept for the backward pass; logits n x 16 */
+int ga_mlp_forward_device(const float* params, const float* feat, int64_t n, float* h1, float* h2,
+                          float* logits, void* stream, const char** error);
+/* mean softmax cross-entropy over the n samples: grads (GA_MLP_PARAMS) and the
+ * per-sample loss; work holds n * (16 + 128 + 128) floats */
+int ga_mlp_backward_device(const float* params, const float* feat, const float* h1, const float* h2,
+                           const float* logits, const int32_t* label, int64_t n, float* grads, float* loss,
+                           float* work, void* stream, const char** error);
+/* momentum <- mu * momentum + grads; params <- params - lr * momentum */
+int ga_mlp_sgd_device(float* params, const float* grads, float* momentum, float lr, float mu, void* stream,
+                      const char** error);
+/* bucket = smallest bucket whose cumulative softmax probability reaches q
+ * (q <= 0: arg-max) */
+int ga_mlp_route_device(const float* logits, int64_t n, float q, int32_t* bucket, void* stream,
+                        const char** error);
+/* distance bound of a bucket (the last bucket: above smax, INT32_MAX) */
+int32_t ga_route_bucket_bound(int32_t bucket);
+
+typedef struct ga_route_stats {
+    int64_t shortcut;   /* unresolved by the exact lower bound, not aligned */
+    int64_t routed;     /* first tried under a bound below their smax */
+    int64_t plain;      /* first tried under their own smax */
+    int64_t rerun;      /* routed pairs not resolved under the bound: re-run with their smax */
+} ga_route_stats;
+
+/* ga_wfa_device with routing: same outputs pair for pair (the router only
+ * chooses the first bound and size class; every pair not resolved under a
+ * routed bound is re-run with its own smax).  Needs host_smax,
+ * host_pattern_len and host_text_len.  Synchronises `stream` twice. */
+int ga_wfa_routed_device(const ga_wfa_dev_batch* b, const float* params, float q, ga_route_stats* stats,
+                         void* stream, const char** error);
+
 typedef struct ga_sw_dev_batch {
     const uint8_t* text;    const int64_t* text_off;    const int32_t* text_len;
     const uint8_t* pattern; const int64_t* pattern_off; const int32_t* pattern_len;

@@ -68,6 +68,12 @@ def lib():
                                           vp, vp, vp, vp, C.c_int]
         L.ga_oracle_sw_batch.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int64, vp, C.c_int, C.c_int,
                                          C.c_int, vp, vp, vp, vp, C.c_int]
+        L.ga_oracle_sketch.argtypes = [cp, C.c_int, cp, C.c_int, vp, vp]
+        L.ga_oracle_sketch.restype = None
+        L.ga_oracle_mlp_forward.argtypes = [vp, vp, C.c_int64, vp, vp, vp]
+        L.ga_oracle_mlp_forward.restype = None
+        L.ga_oracle_mlp_backward.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int64, vp, vp]
+        L.ga_oracle_mlp_backward.restype = None
         _lib = L
     return _lib
 
@@ -405,3 +411,45 @@ def ref_sw_score(pairs, scoring, wsize: int = 64):
                           poff.ctypes.data, pl.ctypes.data, len(pairs), *scoring, wsize,
                           out.ctypes.data)
     return [tuple(int(v) for v in out[3 * i: 3 * i + 3]) for i in range(len(pairs))]
+
+
+# ------------------------------------------------------------------ router
+# (no reference counterpart: restatements of this build's own definitions,
+# pinned in tests/test_router_oracle.py)
+SKETCH_DIM, MLP_HIDDEN, ROUTE_BUCKETS, MLP_PARAMS = 64, 128, 16, 26896
+
+
+def sketch(pattern, text):
+    """(features float32[64], lower bound) of one pair."""
+    p, t = _b(pattern), _b(text)
+    f = np.zeros(SKETCH_DIM, dtype=np.float32)
+    lb = C.c_int32()
+    lib().ga_oracle_sketch(p, len(p), t, len(t), f.ctypes.data, C.byref(lb))
+    return f, lb.value
+
+
+def mlp_forward(params: np.ndarray, x: np.ndarray):
+    """(h1, h2, logits) for rows x (float32 [n, 64])."""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n = x.shape[0]
+    h1 = np.zeros((n, MLP_HIDDEN), np.float32)
+    h2 = np.zeros((n, MLP_HIDDEN), np.float32)
+    lg = np.zeros((n, ROUTE_BUCKETS), np.float32)
+    lib().ga_oracle_mlp_forward(params.ctypes.data, x.ctypes.data, n, h1.ctypes.data, h2.ctypes.data,
+                                lg.ctypes.data)
+    return h1, h2, lg
+
+
+def mlp_backward(params: np.ndarray, x: np.ndarray, label: np.ndarray):
+    """(grads float32[26896], per-sample loss) of the mean cross-entropy."""
+    h1, h2, lg = mlp_forward(params, x)
+    label = np.ascontiguousarray(label, dtype=np.int32)
+    n = x.shape[0]
+    g = np.zeros(MLP_PARAMS, np.float32)
+    loss = np.zeros(n, np.float32)
+    lib().ga_oracle_mlp_backward(np.ascontiguousarray(params, dtype=np.float32).ctypes.data,
+                                 np.ascontiguousarray(x, dtype=np.float32).ctypes.data, h1.ctypes.data,
+                                 h2.ctypes.data, lg.ctypes.data, label.ctypes.data, n, g.ctypes.data,
+                                 loss.ctypes.data)
+    return g, loss
