@@ -178,6 +178,11 @@ def main():
     ap.add_argument("--no-wfa", action="store_true")
     ap.add_argument("--no-wfa-mixed", action="store_true")
     ap.add_argument("--wfa-mixed-pairs", type=int, default=200_000)
+    ap.add_argument("--no-routing", action="store_true")
+    ap.add_argument("--routing-pairs", type=int, default=200_000)
+    ap.add_argument("--router-weights", default=None,
+                    help="router parameters (.npy); default: the trained file shipped in the package")
+    ap.add_argument("--route-q", type=float, default=0.9)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, to scale hardware byte counters)")
@@ -315,6 +320,47 @@ def main():
             log(f"wfa_mixed cpu baseline: {mixed_line['cpu_baseline']}")
         del wmd
 
+    # ---------------- WFA routing: plain per-class path vs learned router ----------------
+    routing_line = None
+    if not args.no_routing:
+        from genoaligner_amd import router as R
+        t0 = time.perf_counter()
+        wr = W.wfa_routing(n=args.routing_pairs)
+        rwant = (np.random.default_rng(6).random(wr.n) < 0.7).astype(np.uint8)
+        params = torch.from_numpy(R.load(args.router_weights or R.WEIGHTS)).cuda()
+        plain = WfaDeviceBatch(wr, trace=True, want=rwant)
+        routed = WfaDeviceBatch(wr, trace=True, want=rwant)
+        log(f"wfa_routing workload ready ({time.perf_counter() - t0:.1f} s)")
+        pwall, pstep, _ = time_steps(plain.run, args.steps, args.warmup, lib)
+        stats = []
+        rwall, rstep, _ = time_steps(lambda: stats.append(R.run_routed(routed, params, q=args.route_q)),
+                                     args.steps, args.warmup, lib)
+        psc, pcg = plain.results()
+        rsc, rcg = routed.results()
+        diff = int(sum(1 for i in range(wr.n) if psc[i] != rsc[i] or pcg[i] != rcg[i]))
+        rsample = rng.choice(wr.n, size=min(200, wr.n), replace=False)
+        obad = 0
+        for i in rsample:
+            osc, ocg = O.wfa(wr.a_seq(i), wr.b_seq(i), int(wr.smax[i]))
+            if int(rsc[i]) != osc or rcg[i].decode() != (ocg if rwant[i] else ""):
+                obad += 1
+        st = stats[-1]
+        routing_line = {
+            "workload": f"wfa_routing {wr.n} pairs, lengths log-uniform 100..10000 bp, divergence 80% "
+                        "U[0,0.03] / 15% U[0.03,0.15] / 5% unrelated, smax min(0.2 len, 2048), "
+                        "70% CIGAR / 30% score-only",
+            "plain_pairs_per_s": round(wr.n / (pwall / args.steps), 1),
+            "routed_pairs_per_s": round(wr.n / (rwall / args.steps), 1),
+            "routed_over_plain": round(pwall / rwall, 3),
+            "plain_ms": round(pwall * 1000 / args.steps, 3), "routed_ms": round(rwall * 1000 / args.steps, 3),
+            "routed_p50_ms": pctl(rstep, 50), "routed_p99_ms": pctl(rstep, 99),
+            "routed_vs_plain_differences": diff, "oracle_mismatches": obad, "oracle_checked": int(len(rsample)),
+            "shortcut": st.shortcut, "routed": st.routed, "plain": st.plain, "rerun": st.rerun,
+            "miss_rate": round(st.miss_rate, 4), "route_q": args.route_q,
+        }
+        log(f"wfa_routing: {routing_line}")
+        del plain, routed
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
@@ -346,6 +392,7 @@ def main():
         "sw_score_only": sw_score_line,
         "wfa": wfa_line,
         "wfa_mixed": mixed_line,
+        "wfa_routing": routing_line,
     }
     print(json.dumps(line), flush=True)
 
