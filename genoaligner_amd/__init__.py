@@ -8,5 +8,5 @@ go through ``genoaligner_amd.device``.
 from .api import (  # noqa: F401
     AlignRequest, AlignResult, BatchResult, SWAlignResult, SWBatchResult, SWRequest,
     SWScoring, Status, SW_MAX_TRACE_CELLS, align, align_batch, align_batch_ext, align_sw,
-    align_sw_batch, backend_name, device_available, device_name,
+    align_sw_batch, align_sw_matrix_batch, backend_name, blosum62, device_available, device_name,
 )

@@ -27,6 +27,7 @@ EXPORTS = (
     "ga_wfa_device", "ga_sw_device", "ga_set_kernel_timing", "ga_last_kernel_ms",
     "ga_sketch_device", "ga_mlp_forward_device", "ga_mlp_backward_device", "ga_mlp_sgd_device",
     "ga_mlp_route_device", "ga_route_bucket_bound", "ga_wfa_routed_device",
+    "ga_align_sw_matrix_batch", "ga_sw_matrix_blosum62",
 )
 
 
@@ -67,6 +68,11 @@ class WfaDevBatch(C.Structure):
                 ("cigar_off", C.c_void_p), ("cigar_len", C.c_void_p),
                 ("want_cigar", C.c_void_p), ("host_smax", C.c_void_p),
                 ("host_pattern_len", C.c_void_p), ("host_text_len", C.c_void_p)]
+
+
+class SwMatrix(C.Structure):
+    _fields_ = [("alphabet", C.c_char * 32), ("score", (C.c_int8 * 32) * 32), ("unknown", C.c_int32),
+                ("gap_open", C.c_int32), ("gap_extend", C.c_int32)]
 
 
 class RouteStats(C.Structure):
@@ -126,6 +132,11 @@ def lib() -> C.CDLL:
         for f in (L.ga_sketch_device, L.ga_mlp_forward_device, L.ga_mlp_backward_device,
                   L.ga_mlp_sgd_device, L.ga_mlp_route_device, L.ga_wfa_routed_device):
             f.restype = C.c_int
+        L.ga_align_sw_matrix_batch.argtypes = [C.POINTER(SWReq), C.c_int64, C.POINTER(SwMatrix),
+                                               C.POINTER(SWRes), C.c_void_p, C.c_int64, i32p, i32p, cpp]
+        L.ga_align_sw_matrix_batch.restype = C.c_int
+        L.ga_sw_matrix_blosum62.argtypes = [C.POINTER(SwMatrix), C.c_int32, C.c_int32]
+        L.ga_sw_matrix_blosum62.restype = None
         L.ga_route_bucket_bound.argtypes = [C.c_int32]
         L.ga_route_bucket_bound.restype = C.c_int32
         _lib = L

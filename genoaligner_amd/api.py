@@ -173,8 +173,7 @@ def _scoring_c(s) -> N.SWScoringC:
     return N.SWScoringC(*[int(x) for x in s])
 
 
-def align_sw_batch(reqs: Sequence[SWRequest]) -> SWBatchResult:
-    """genoaligner::align_sw_batch (api.hpp:312)."""
+def _sw_call(reqs: Sequence[SWRequest], matrix=None) -> SWBatchResult:
     n = len(reqs)
     out = SWBatchResult(results=[SWAlignResult() for _ in range(n)])
     if n == 0:
@@ -190,7 +189,11 @@ def align_sw_batch(reqs: Sequence[SWRequest]) -> SWBatchResult:
     need = L.ga_cigar_arena_bytes_sw(arr, n)
     arena = C.create_string_buffer(max(need, 1))
     rc_ok, rc_un, err = C.c_int32(), C.c_int32(), C.c_char_p()
-    rc = L.ga_align_sw_batch(arr, n, res, arena, need, C.byref(rc_ok), C.byref(rc_un), C.byref(err))
+    if matrix is None:
+        rc = L.ga_align_sw_batch(arr, n, res, arena, need, C.byref(rc_ok), C.byref(rc_un), C.byref(err))
+    else:
+        rc = L.ga_align_sw_matrix_batch(arr, n, C.byref(matrix), res, arena, need, C.byref(rc_ok),
+                                        C.byref(rc_un), C.byref(err))
     out.status = Status(rc)
     out.error = err.value.decode() if err.value else None
     if rc != N.GA_OK:
@@ -206,6 +209,25 @@ def align_sw_batch(reqs: Sequence[SWRequest]) -> SWBatchResult:
         o.too_large = bool(r.too_large)
     out.resolved_count, out.unresolved_count = rc_ok.value, rc_un.value
     return out
+
+
+def align_sw_batch(reqs: Sequence[SWRequest]) -> SWBatchResult:
+    """genoaligner::align_sw_batch (api.hpp:312)."""
+    return _sw_call(reqs)
+
+
+def blosum62(gap_open: int = 12, gap_extend: int = 1) -> "N.SwMatrix":
+    """BLOSUM62 scheme (ARNDCQEGHILKMFPSTWYV, unknown residues -1).  A gap of
+    length L costs gap_open + (L - 1) * gap_extend."""
+    m = N.SwMatrix()
+    N.lib().ga_sw_matrix_blosum62(C.byref(m), int(gap_open), int(gap_extend))
+    return m
+
+
+def align_sw_matrix_batch(reqs: Sequence[SWRequest], matrix) -> SWBatchResult:
+    """Smith-Waterman under a substitution matrix (an extension: the same
+    rules as align_sw_batch, with the requests' scoring replaced by `matrix`)."""
+    return _sw_call(reqs, matrix)
 
 
 def align_sw(req: SWRequest) -> SWAlignResult:

@@ -103,6 +103,10 @@ struct SwLaunch {
     const uint8_t* rerun;
     const int32_t* pair_a; const int32_t* pair_b;
     int32_t n_total;
+    // Substitution-matrix scoring (null: match / mismatch): residue codes
+    // 0..31 of the text and pattern, laid out like txt / pat, and the 32 x 32
+    // score table sub[text code * 32 + pattern code].
+    const uint8_t* tcode; const uint8_t* pcode; const int32_t* sub;
 };
 
 // Marks, in res[0] of a pair, that the packed 16-bit kernel left it to the int32 kernel.

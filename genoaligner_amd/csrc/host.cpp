@@ -246,12 +246,9 @@ void sw_validate(const char* cg, int len, const char* text, int text_len, int st
 // ---------------------------------------------------------------------------
 // WFA batch on the device
 // ---------------------------------------------------------------------------
-// Workspace budget for the trace variant's wavefront stacks; batches whose
-// stacks exceed it run in consecutive chunks that reuse one buffer.
-constexpr int64_t kWfaWsBudget = int64_t(4) << 30;
-
 // Size classes of the persistent WFA kernel, by a pair's smax: each class is
-// one launch whose LDS wavefronts and per-wave stacks are sized for its bound.
+// one launch whose LDS wavefronts and per-workgroup stacks are sized for its
+// bound; the stacks of one class launch stay under kWfaStackBudget.
 constexpr int kWfaClassBound[] = {64, 256, 1024, GA_MAX_SMAX_EXT};
 constexpr int kWfaNumClasses = 4;
 constexpr int64_t kWfaStackBudget = int64_t(24) << 30;   // bytes of stacks per class launch
@@ -543,6 +540,61 @@ int wfa_common(const ga_align_req* reqs, int64_t n, ga_align_res* res, char* are
 // ---------------------------------------------------------------------------
 constexpr int64_t kSwDirBudget = int64_t(16) << 30;     // direction nibbles per chunk
 constexpr int kLdsBndMaxRows = 8192;                     // 2 ints per row in LDS
+constexpr int kLdsBndMaxRowsMatrix = 7168;               // leaves 8 KB for the score profile
+
+// Substitution-matrix scheme on the host: residue code of every byte (the
+// alphabet's letters 0 .. k-1, any other byte kSwUnknownCode) and the 32 x 32
+// table over codes (rows: text residue, columns: pattern residue).
+constexpr int kSwUnknownCode = 31;
+struct SwMatrixHost {
+    uint8_t code[256];
+    int32_t sub[32 * 32];
+    int32_t gap_open, gap_extend;
+    int score(uint8_t t, uint8_t p) const { return sub[code[t] * 32 + code[p]]; }
+};
+
+// rescore_ok / wellformed_ok of a CIGAR under a substitution matrix: the same
+// walk as sw_validate (affine runs: go + (L-1) ge), M/X scored by the table and
+// still required on equal / unequal bytes.
+void sw_validate_matrix(const char* cg, int len, const char* text, int text_len, int start_i, int end_i,
+                        const char* pattern, int pattern_len, int start_j, int end_j,
+                        const SwMatrixHost& mt, int score, int32_t* rescore_ok, int32_t* wellformed_ok)
+{
+    int i = start_i, j = start_j, s = 0, run = 0;
+    bool wf = true;
+    auto close_gap = [&] {
+        if (run != 0) {
+            const int L = run > 0 ? run : -run;
+            s -= mt.gap_open + (L - 1) * mt.gap_extend;
+            run = 0;
+        }
+    };
+    for (int c = 0; c < len; ++c) {
+        const bool in = i <= end_i && j <= end_j && i < text_len && j < pattern_len;
+        switch (cg[c]) {
+            case 'M':
+            case 'X':
+                close_gap();
+                if (!in || ((text[i] == pattern[j]) != (cg[c] == 'M'))) wf = false;
+                if (in) s += mt.score(uint8_t(text[i]), uint8_t(pattern[j]));
+                ++i; ++j; break;
+            case 'I':
+                if (run < 0) close_gap();
+                if (i > end_i || i >= text_len) wf = false;
+                ++i; ++run; break;
+            case 'D':
+                if (run > 0) close_gap();
+                if (j > end_j || j >= pattern_len) wf = false;
+                ++j; --run; break;
+            default: wf = false; break;
+        }
+    }
+    close_gap();
+    if (i != end_i + 1 || j != end_j + 1) wf = false;
+    *wellformed_ok = wf;
+    *rescore_ok = wf && (s == score);
+}
+
 
 const char* sw_scheme_error(const ga_sw_scoring& s, bool with_cigar)
 {
@@ -583,7 +635,8 @@ hipError_t enqueue_sw16_chunk(const ga::Sw16Launch& P16, bool trace, hipStream_t
 // ga::SwLaunch).  walk[i] selects pairs whose span + CIGAR are produced.
 const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>& skip,
                    bool trace, const std::vector<uint8_t>& walk, char* arena,
-                   std::vector<int32_t>& res8, std::vector<int64_t>& slot_off)
+                   std::vector<int32_t>& res8, std::vector<int64_t>& slot_off,
+                   const SwMatrixHost* mat = nullptr)
 {
     std::call_once(g_pool_once, init_pool);
     // Order on the device: pairs for the int32 kernel first (texts longer than
@@ -599,7 +652,7 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
         slot_off[size_t(i)] = slot;
         slot += std::max(r.text_len, 0) + std::max(r.pattern_len, 0);
         const bool sk = skip[size_t(i)] != 0;
-        if (!sk && r.text_len > 0 && r.pattern_len > 0 && r.text_len <= ga::kSw16MaxRows)
+        if (!mat && !sk && r.text_len > 0 && r.pattern_len > 0 && r.text_len <= ga::kSw16MaxRows)
             packed.push_back(i);
         else
             order.push_back(i);
@@ -630,6 +683,14 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
         if (k < n32) max_m32 = std::max(max_m32, tl[size_t(k)]);
     }
     T.end(); P.end();
+    // residue codes, laid out like the bytes (the padding maps to code 0)
+    std::vector<uint8_t> tcode, pcode;
+    if (mat) {
+        tcode.resize(T.bytes.size());
+        pcode.resize(P.bytes.size());
+        for (size_t x = 0; x < tcode.size(); ++x) tcode[x] = mat->code[T.bytes[x]];
+        for (size_t x = 0; x < pcode.size(); ++x) pcode[x] = mat->code[P.bytes[x]];
+    }
 
     hipStream_t st = acquire_stream();
     if (!st) return "hipStreamCreate";
@@ -638,6 +699,11 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
     const char* err = nullptr;
     auto* d_txt = D.upload(T.bytes);
     auto* d_pat = D.upload(P.bytes);
+    const uint8_t* d_tcode = mat ? D.upload(tcode) : nullptr;
+    const uint8_t* d_pcode = mat ? D.upload(pcode) : nullptr;
+    std::vector<int32_t> subv;
+    if (mat) subv.assign(mat->sub, mat->sub + 32 * 32);
+    const int32_t* d_sub = mat ? D.upload(subv) : nullptr;
     auto* d_toff = D.upload(T.off);
     auto* d_poff = D.upload(P.off);
     auto* d_tl = D.upload(tl);
@@ -679,7 +745,7 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
         } else {
             chunks.emplace_back(0, n32);
         }
-        const bool lds_bnd = max_m32 <= kLdsBndMaxRows;
+        const bool lds_bnd = max_m32 <= (mat ? kLdsBndMaxRowsMatrix : kLdsBndMaxRows);
         int32_t* d_bnd = nullptr;
         const int64_t bnd_chunk = int64_t(1) << 16;
         if (!lds_bnd) d_bnd = D.alloc<int32_t>(size_t(std::min<int64_t>(n32, bnd_chunk)) * 2 * size_t(max_m32));
@@ -696,6 +762,8 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
                 L.res = d_res; L.dirs = d_dirs; L.dir_off = d_diroff; L.walk = d_walk;
                 L.cigar = d_cig; L.cig_off = d_slot; L.bnd = d_bnd; L.first_pair = int32_t(a);
                 L.n_total = int32_t(n);
+                L.tcode = d_tcode; L.pcode = d_pcode; L.sub = d_sub;
+                if (mat) { L.go = mat->gap_open; L.ge = mat->gap_extend; }
                 if (ga::sw_launch(L, trace, st) != hipSuccess) err = "SW kernel launch";
                 a += len;
             }
@@ -825,9 +893,9 @@ int ga_align_batch_ext(const ga_align_req* reqs, int64_t n, ga_align_res* res, c
     return wfa_common(reqs, n, res, arena, arena_bytes, resolved_count, unresolved_count, error, true);
 }
 
-int ga_align_sw_batch(const ga_sw_req* reqs, int64_t n, ga_sw_res* res, char* arena,
-                      int64_t arena_bytes, int32_t* resolved_count, int32_t* unresolved_count,
-                      const char** error)
+static int sw_common(const ga_sw_req* reqs, int64_t n, ga_sw_res* res, char* arena, int64_t arena_bytes,
+                     int32_t* resolved_count, int32_t* unresolved_count, const char** error,
+                     const SwMatrixHost* mat)
 {
     if (error) *error = nullptr;
     if (resolved_count) *resolved_count = 0;
@@ -846,6 +914,12 @@ int ga_align_sw_batch(const ga_sw_req* reqs, int64_t n, ga_sw_res* res, char* ar
         if ((r.text_len > 0 && r.text == nullptr) || (r.pattern_len > 0 && r.pattern == nullptr))
             return refuse("null pointer with non-zero length");
         if (r.text_len < 0 || r.pattern_len < 0) return refuse("negative length");
+        if (mat) {
+            if (r.with_cigar && mat->gap_extend > mat->gap_open)
+                return refuse("SW matrix scoring: gap_extend > gap_open with with_cigar cannot re-score "
+                              "to its own score; score-only requests may use it");
+            continue;                             // the matrix scheme replaces r.scoring
+        }
         if (const char* e = sw_scheme_error(r.scoring, r.with_cigar != 0)) return refuse(e);
         if (r.scoring.match != s0.match || r.scoring.mismatch != s0.mismatch ||
             r.scoring.gap_open != s0.gap_open || r.scoring.gap_extend != s0.gap_extend)
@@ -873,7 +947,7 @@ int ga_align_sw_batch(const ga_sw_req* reqs, int64_t n, ga_sw_res* res, char* ar
     }
     std::vector<int32_t> r8;
     std::vector<int64_t> slot_off;
-    const char* err = run_sw(reqs, n, skip, any_cigar, walk, arena, r8, slot_off);
+    const char* err = run_sw(reqs, n, skip, any_cigar, walk, arena, r8, slot_off, mat);
     if (err) {
         if (error) *error = err;
         reset_sw(res, n);
@@ -902,9 +976,14 @@ int ga_align_sw_batch(const ga_sw_req* reqs, int64_t n, ga_sw_res* res, char* ar
             const int slot_len = std::max(r.text_len, 0) + std::max(r.pattern_len, 0);
             o.cigar_off = slot_off[size_t(i)] + slot_len - len;
             o.cigar_len = len;
-            sw_validate(arena + o.cigar_off, len, r.text, r.text_len, o.start_i, o.end_i,
-                        r.pattern, r.pattern_len, o.start_j, o.end_j, r.scoring, o.score,
-                        &o.rescore_ok, &o.wellformed_ok);
+            if (mat)
+                sw_validate_matrix(arena + o.cigar_off, len, r.text, r.text_len, o.start_i, o.end_i,
+                                   r.pattern, r.pattern_len, o.start_j, o.end_j, *mat, o.score,
+                                   &o.rescore_ok, &o.wellformed_ok);
+            else
+                sw_validate(arena + o.cigar_off, len, r.text, r.text_len, o.start_i, o.end_i,
+                            r.pattern, r.pattern_len, o.start_j, o.end_j, r.scoring, o.score,
+                            &o.rescore_ok, &o.wellformed_ok);
         }
     });
     int32_t rc = 0;
@@ -912,6 +991,80 @@ int ga_align_sw_batch(const ga_sw_req* reqs, int64_t n, ga_sw_res* res, char* ar
     if (resolved_count) *resolved_count = rc;
     if (unresolved_count) *unresolved_count = int32_t(n) - rc;
     return GA_OK;
+}
+
+int ga_align_sw_batch(const ga_sw_req* reqs, int64_t n, ga_sw_res* res, char* arena,
+                      int64_t arena_bytes, int32_t* resolved_count, int32_t* unresolved_count,
+                      const char** error)
+{
+    return sw_common(reqs, n, res, arena, arena_bytes, resolved_count, unresolved_count, error, nullptr);
+}
+
+int ga_align_sw_matrix_batch(const ga_sw_req* reqs, int64_t n, const ga_sw_matrix* matrix, ga_sw_res* res,
+                             char* arena, int64_t arena_bytes, int32_t* resolved_count,
+                             int32_t* unresolved_count, const char** error)
+{
+    if (error) *error = nullptr;
+    if (n > 0 && res) reset_sw(res, n);
+    if (!matrix) { if (error) *error = "null substitution matrix"; return GA_INVALID_ARGUMENT; }
+    SwMatrixHost mt{};
+    const int k = int(strnlen(matrix->alphabet, sizeof(matrix->alphabet)));
+    if (k < 1 || k > 31) { if (error) *error = "SW matrix: alphabet must hold 1..31 letters"; return GA_INVALID_ARGUMENT; }
+    if (matrix->gap_open < 0 || matrix->gap_extend < 0) {
+        if (error) *error = "SW matrix: gap costs must be >= 0";
+        return GA_INVALID_ARGUMENT;
+    }
+    std::memset(mt.code, kSwUnknownCode, sizeof(mt.code));
+    for (int a = 0; a < k; ++a) {
+        const uint8_t c = uint8_t(matrix->alphabet[a]);
+        if (mt.code[c] != kSwUnknownCode) { if (error) *error = "SW matrix: repeated alphabet letter"; return GA_INVALID_ARGUMENT; }
+        mt.code[c] = uint8_t(a);
+    }
+    for (int a = 0; a < 32; ++a)
+        for (int b = 0; b < 32; ++b)
+            mt.sub[a * 32 + b] = (a < k && b < k) ? int32_t(matrix->score[a][b]) : matrix->unknown;
+    mt.gap_open = matrix->gap_open;
+    mt.gap_extend = matrix->gap_extend;
+    return sw_common(reqs, n, res, arena, arena_bytes, resolved_count, unresolved_count, error, &mt);
+}
+
+void ga_sw_matrix_blosum62(ga_sw_matrix* out, int32_t gap_open, int32_t gap_extend)
+{
+    // BLOSUM62 (Henikoff & Henikoff 1992) as the reference holds it
+    // (src/msa/msa_ref.cpp:116-137), rows and columns in the order below; the
+    // reference's row M, column H reads -1 where the published matrix has -2,
+    // and that value is kept so results match the reference's table.
+    static const char kOrder[] = "ARNDCQEGHILKMFPSTWYV";
+    static const int8_t kB62[20][20] = {
+        { 4,-1,-2,-2, 0,-1,-1, 0,-2,-1,-1,-1,-1,-2,-1, 1, 0,-3,-2, 0},
+        {-1, 5, 0,-2,-3, 1, 0,-2, 0,-3,-2, 2,-1,-3,-2,-1,-1,-3,-2,-3},
+        {-2, 0, 6, 1,-3, 0, 0, 0, 1,-3,-3, 0,-2,-3,-2, 1, 0,-4,-2,-3},
+        {-2,-2, 1, 6,-3, 0, 2,-1,-1,-3,-4,-1,-3,-3,-1, 0,-1,-4,-3,-3},
+        { 0,-3,-3,-3, 9,-3,-4,-3,-3,-1,-1,-3,-1,-2,-3,-1,-1,-2,-2,-1},
+        {-1, 1, 0, 0,-3, 5, 2,-2, 0,-3,-2, 1, 0,-3,-1, 0,-1,-2,-1,-2},
+        {-1, 0, 0, 2,-4, 2, 5,-2, 0,-3,-3, 1,-2,-3,-1, 0,-1,-3,-2,-2},
+        { 0,-2, 0,-1,-3,-2,-2, 6,-2,-4,-4,-2,-3,-3,-2, 0,-2,-2,-3,-3},
+        {-2, 0, 1,-1,-3, 0, 0,-2, 8,-3,-3,-1,-2,-1,-2,-1,-2,-2, 2,-3},
+        {-1,-3,-3,-3,-1,-3,-3,-4,-3, 4, 2,-3, 1, 0,-3,-2,-1,-3,-1, 3},
+        {-1,-2,-3,-4,-1,-2,-3,-4,-3, 2, 4,-2, 2, 0,-3,-2,-1,-2,-1, 1},
+        {-1, 2, 0,-1,-3, 1, 1,-2,-1,-3,-2, 5,-1,-3,-1, 0,-1,-3,-2,-2},
+        {-1,-1,-2,-3,-1, 0,-2,-3,-1, 1, 2,-1, 5, 0,-2,-1,-1,-1,-1, 1},
+        {-2,-3,-3,-3,-2,-3,-3,-3,-1, 0, 0,-3, 0, 6,-4,-2,-2, 1, 3,-1},
+        {-1,-2,-2,-1,-3,-1,-1,-2,-2,-3,-3,-1,-2,-4, 7,-1,-1,-4,-3,-2},
+        { 1,-1, 1, 0,-1, 0, 0, 0,-1,-2,-2, 0,-1,-2,-1, 4, 1,-3,-2,-2},
+        { 0,-1, 0,-1,-1,-1,-1,-2,-2,-1,-1,-1,-1,-2,-1, 1, 5,-2,-2, 0},
+        {-3,-3,-4,-4,-2,-2,-3,-2,-2,-3,-2,-3,-1, 1,-4,-3,-2,11, 2,-3},
+        {-2,-2,-2,-3,-2,-1,-2,-3, 2,-1,-1,-2,-1, 3,-3,-2,-2, 2, 7,-1},
+        { 0,-3,-3,-3,-1,-2,-2,-3,-3, 3, 1,-2, 1,-1,-2,-2, 0,-3,-1, 4},
+    };
+    if (!out) return;
+    std::memset(out, 0, sizeof(*out));
+    std::memcpy(out->alphabet, kOrder, 20);
+    for (int a = 0; a < 20; ++a)
+        for (int b = 0; b < 20; ++b) out->score[a][b] = kB62[a][b];
+    out->unknown = -1;
+    out->gap_open = gap_open;
+    out->gap_extend = gap_extend;
 }
 
 void ga_set_kernel_timing(int enabled) { t_timer.enabled = enabled != 0; }

@@ -64,7 +64,9 @@ __device__ __forceinline__ uint64_t load8(const uint8_t* p)
     return sh ? ((lo >> sh) | (hi << (64u - sh))) : lo;
 }
 
-template <bool TRACE, bool LDS_BND>
+constexpr int kSwProfileInts = 32 * 64;
+
+template <bool TRACE, bool LDS_BND, bool MATRIX>
 __global__ __launch_bounds__(64) void sw_kernel(SwLaunch L)
 {
     extern __shared__ int lds[];
@@ -88,7 +90,10 @@ __global__ __launch_bounds__(64) void sw_kernel(SwLaunch L)
     }
     const uint8_t* T = L.txt + L.toff[pid];
     const uint8_t* P = L.pat + L.poff[pid];
-    int* bH = LDS_BND ? lds : L.bnd + int64_t(slot) * 2 * L.max_m;
+    int* bH = LDS_BND ? lds + (MATRIX ? kSwProfileInts : 0) : L.bnd + int64_t(slot) * 2 * L.max_m;
+    // MATRIX: each lane's column of the score table for its pattern residue,
+    // prof[code * 64 + lane] (one LDS read per cell, only by the owning lane)
+    int* prof = lds;
     int* bE = bH + L.max_m;
     for (int q = lane; q < m; q += 64) { bH[q] = 0; bE[q] = 0; }
     __syncthreads();
@@ -103,15 +108,18 @@ __global__ __launch_bounds__(64) void sw_kernel(SwLaunch L)
     for (int s = 0; s < nstrips; ++s) {
         const int j = (s << 6) + lane;               // 0-based column of this lane
         const bool jvalid = j < n;
-        const uint8_t pc = jvalid ? P[j] : uint8_t(0);
+        const uint8_t pc = jvalid ? (MATRIX ? L.pcode[L.poff[pid] + j] : P[j]) : uint8_t(0);
+        if (MATRIX)
+            for (int c = 0; c < 32; ++c) prof[c * 64 + lane] = L.sub[c * 32 + pc];
         int h = 0, e = 0, f = 0;                     // this lane's last row
         int hl_prev = 0;                             // H[i-1][j-1]
         int bHv = 0, bIv = -1;
         // text bytes for rows t - lane .. t - lane + 7, prefetched a chunk ahead
-        uint64_t tcur = load8(T - lane);
+        const uint8_t* TS = MATRIX ? L.tcode + L.toff[pid] : T;   // bytes or residue codes
+        uint64_t tcur = load8(TS - lane);
         for (int c = 0; c < nchunks; ++c) {
             const int t0 = c << 3;
-            const uint64_t tnext = load8(T + t0 + 8 - lane);
+            const uint64_t tnext = load8(TS + t0 + 8 - lane);
             uint32_t acc = 0;
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
@@ -134,7 +142,7 @@ __global__ __launch_bounds__(64) void sw_kernel(SwLaunch L)
                     { const int e2 = el - ge; if (e2 > ee) { ee = e2; code |= 4u; } }
                     int ff = h - go;
                     { const int f2 = f - ge; if (f2 > ff) { ff = f2; code |= 8u; } }
-                    int hv = diag + (tc == pc ? sm : sx);
+                    int hv = diag + (MATRIX ? prof[(tc & 31) * 64 + lane] : (tc == pc ? sm : sx));
                     uint32_t hsrc = 1u;
                     if (ee > hv) { hv = ee; hsrc = 2u; }
                     if (ff > hv) { hv = ff; hsrc = 3u; }
@@ -252,25 +260,35 @@ hipError_t sw_gather(const int32_t* res8, int64_t n, bool trace, int32_t* out5, 
     return hipGetLastError();
 }
 
+namespace {
+template <bool TRACE, bool MATRIX>
+void launch_forward(const SwLaunch& L, bool in_lds, size_t lds, hipStream_t st)
+{
+    const dim3 grid(unsigned(L.n)), block(64);
+    if (in_lds) hipLaunchKernelGGL((sw_kernel<TRACE, true, MATRIX>), grid, block, lds, st, L);
+    else        hipLaunchKernelGGL((sw_kernel<TRACE, false, MATRIX>), grid, block, lds, st, L);
+}
+}  // namespace
+
 hipError_t sw_launch(const SwLaunch& L, bool trace, hipStream_t st, LaunchHooks* hooks)
 {
     if (L.n <= 0) return hipSuccess;
+    const bool matrix = L.sub != nullptr;
     const size_t bnd_bytes = size_t(2) * size_t(L.max_m) * sizeof(int);
     const bool in_lds = L.bnd == nullptr;
-    const size_t lds = in_lds ? bnd_bytes : 0;
-    const dim3 grid(unsigned(L.n)), block(64);
+    const size_t lds = (in_lds ? bnd_bytes : 0) + (matrix ? size_t(kSwProfileInts) * sizeof(int) : 0);
     if (hooks) hooks->before(st);
     if (trace) {
-        if (in_lds) hipLaunchKernelGGL((sw_kernel<true, true>), grid, block, lds, st, L);
-        else        hipLaunchKernelGGL((sw_kernel<true, false>), grid, block, lds, st, L);
+        if (matrix) launch_forward<true, true>(L, in_lds, lds, st);
+        else        launch_forward<true, false>(L, in_lds, lds, st);
         if (hooks) hooks->after(st);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        const dim3 wgrid(unsigned((L.n + 63) / 64));
+        const dim3 wgrid(unsigned((L.n + 63) / 64)), block(64);
         hipLaunchKernelGGL(sw_walk_kernel, wgrid, block, 0, st, L);
     } else {
-        if (in_lds) hipLaunchKernelGGL((sw_kernel<false, true>), grid, block, lds, st, L);
-        else        hipLaunchKernelGGL((sw_kernel<false, false>), grid, block, lds, st, L);
+        if (matrix) launch_forward<false, true>(L, in_lds, lds, st);
+        else        launch_forward<false, false>(L, in_lds, lds, st);
         if (hooks) hooks->after(st);
     }
     return hipGetLastError();

@@ -173,6 +173,28 @@ def wfa_routing(n: int = 200_000, lo: int = 100, hi: int = 10_000, smax_frac: fl
                          "kind": kind, "rate": rate})
 
 
+# Marginal residue frequencies of the BLOSUM62 target distribution, in the
+# matrix's ARNDCQEGHILKMFPSTWYV order.
+PROTEIN = np.frombuffer(b"ARNDCQEGHILKMFPSTWYV", dtype=np.uint8)
+BLOSUM62_FREQ = np.array([0.074, 0.052, 0.045, 0.054, 0.025, 0.034, 0.054, 0.074, 0.026, 0.068,
+                          0.099, 0.058, 0.025, 0.047, 0.039, 0.057, 0.051, 0.013, 0.032, 0.073])
+BLOSUM62_FREQ = BLOSUM62_FREQ / BLOSUM62_FREQ.sum()
+
+
+def protein_pairs(n: int = 2000, qlo: int = 100, qhi: int = 1000, tlo: int = 100, thi: int = 2000,
+                  seed: int = 20261207) -> PairSet:
+    """Protein SW pairs: a = target (text, rows) of length U[tlo, thi], b = query
+    (pattern, columns) of length U[qlo, qhi]; residues i.i.d. at the BLOSUM62
+    background frequencies."""
+    rng = np.random.default_rng(seed)
+    tl = rng.integers(tlo, thi + 1, size=n).astype(np.int32)
+    ql = rng.integers(qlo, qhi + 1, size=n).astype(np.int32)
+    t = PROTEIN[rng.choice(20, size=int(tl.astype(np.int64).sum()), p=BLOSUM62_FREQ)]
+    q = PROTEIN[rng.choice(20, size=int(ql.astype(np.int64).sum()), p=BLOSUM62_FREQ)]
+    return PairSet(t, _offsets(tl), tl, q, _offsets(ql), ql,
+                   meta={"workload": "protein_pairs", "n": n, "seed": seed})
+
+
 def sw_embedded(n: int = 50_000, qlen: int = 256, tlen: int = 1024, rate: float = 0.10,
                 seed: int = 20261203) -> PairSet:
     rng = np.random.default_rng(seed)

@@ -127,6 +127,32 @@ int         ga_device_available(void);
 
 /* Bytes of CIGAR arena a batch needs: sum over pairs of pattern_len + text_len
  * (negative lengths count as 0). */
+/* ---- substitution-matrix Smith-Waterman (EXTENSION: no reference counterpart;
+ * the reference's SW takes match/mismatch only) ------------------------------
+ * Same local affine alignment, tie rules, CIGAR, too_large and score-only end
+ * rules as ga_align_sw_batch, with the substitution score of text residue a
+ * against pattern residue b read from score[code(a)][code(b)], code = index of
+ * the byte in `alphabet`; a pair involving a byte outside the alphabet scores
+ * `unknown`.  M / X in the CIGAR still mean equal / unequal bytes.  A gap of
+ * length L costs gap_open + (L - 1) * gap_extend (so BLAST's 11/1 is
+ * gap_open 12, gap_extend 1 here).  The requests' `scoring` fields are
+ * ignored.  rescore_ok re-scores the CIGAR with the matrix. */
+typedef struct ga_sw_matrix {
+    char    alphabet[32];      /* 1..31 distinct letters, NUL-terminated if shorter */
+    int8_t  score[32][32];
+    int32_t unknown;
+    int32_t gap_open, gap_extend;
+} ga_sw_matrix;
+
+int ga_align_sw_matrix_batch(const ga_sw_req* reqs, int64_t n, const ga_sw_matrix* matrix, ga_sw_res* res,
+                             char* arena, int64_t arena_bytes, int32_t* resolved_count,
+                             int32_t* unresolved_count, const char** error);
+
+/* BLOSUM62 over ARNDCQEGHILKMFPSTWYV as the reference's table holds it
+ * (src/msa/msa_ref.cpp:116-137; its row M, column H is -1, the published
+ * matrix has -2), unknown residues -1, with the given gaps. */
+void ga_sw_matrix_blosum62(ga_sw_matrix* out, int32_t gap_open, int32_t gap_extend);
+
 int64_t ga_cigar_arena_bytes_wfa(const ga_align_req* reqs, int64_t n);
 int64_t ga_cigar_arena_bytes_sw(const ga_sw_req* reqs, int64_t n);
 

@@ -148,6 +148,16 @@ def mm_table(match: int, mismatch: int) -> np.ndarray:
     return tab
 
 
+def matrix_table(alphabet: str, score, unknown: int) -> np.ndarray:
+    """256x256 table (text byte, pattern byte) of a substitution matrix over
+    `alphabet`; any pair with a byte outside it scores `unknown`."""
+    tab = np.full((256, 256), int(unknown), dtype=np.int32)
+    idx = [ord(c) for c in alphabet]
+    sc = np.asarray(score, dtype=np.int32)[: len(idx), : len(idx)]
+    tab[np.ix_(idx, idx)] = sc
+    return tab.reshape(-1)
+
+
 def _table(scoring) -> np.ndarray:
     if isinstance(scoring, np.ndarray):
         return np.ascontiguousarray(scoring, dtype=np.int32).reshape(-1)

@@ -7,6 +7,8 @@ import os
 import re
 import subprocess
 
+import numpy as np
+
 import pytest
 
 from conftest import ROOT
@@ -139,3 +141,47 @@ def test_refusals_match_reference():
             assert st == 1 and ga.align_batch(reqs).status == ga.Status.invalid_argument
         else:
             assert st == 0
+
+
+def test_blosum62_table():
+    """The library's BLOSUM62 equals the reference's table
+    (src/msa/msa_ref.cpp:116-137): the published matrix except row M, column H,
+    which the reference holds as -1 (published: -2), so the table is symmetric
+    everywhere else."""
+    import genoaligner_amd as ga
+    m = ga.blosum62(11, 1)
+    alpha = m.alphabet.decode()
+    assert alpha == "ARNDCQEGHILKMFPSTWYV"
+    S = np.array([[m.score[a][b] for b in range(20)] for a in range(20)])
+    ix = {c: i for i, c in enumerate(alpha)}
+    asym = {(alpha[i], alpha[j]) for i in range(20) for j in range(20) if S[i, j] != S[j, i]}
+    assert asym == {("M", "H"), ("H", "M")}
+    assert (S[ix["M"], ix["H"]], S[ix["H"], ix["M"]]) == (-1, -2)
+    diag = dict(zip(alpha, np.diag(S)))
+    assert diag == {"A": 4, "R": 5, "N": 6, "D": 6, "C": 9, "Q": 5, "E": 5, "G": 6, "H": 8, "I": 4,
+                    "L": 4, "K": 5, "M": 5, "F": 6, "P": 7, "S": 4, "T": 5, "W": 11, "Y": 7, "V": 4}
+    for a, b, v in (("W", "C", -2), ("Y", "F", 3), ("I", "V", 3), ("D", "E", 2), ("W", "P", -4), ("K", "R", 2)):
+        assert S[ix[a], ix[b]] == v
+    assert (m.unknown, m.gap_open, m.gap_extend) == (-1, 11, 1)
+
+
+@pytest.mark.parametrize("mutate,msg", [
+    (lambda m: setattr(m, "alphabet", b""), "alphabet"),
+    (lambda m: setattr(m, "alphabet", b"ARNA"), "repeated"),
+    (lambda m: setattr(m, "gap_open", -1), "gap costs"),
+])
+def test_matrix_refusals(mutate, msg):
+    import genoaligner_amd as ga
+    m = ga.blosum62(11, 1)
+    mutate(m)
+    r = ga.SWRequest(text="ARND", text_len=4, pattern="ARN", pattern_len=3, with_cigar=True)
+    b = ga.align_sw_matrix_batch([r], m)
+    assert b.status == ga.Status.invalid_argument and msg in b.error
+
+
+def test_matrix_gap_rule_refused_with_cigar_only():
+    import genoaligner_amd as ga
+    m = ga.blosum62(1, 3)
+    r = ga.SWRequest(text="ARND", text_len=4, pattern="ARN", pattern_len=3, with_cigar=True)
+    b = ga.align_sw_matrix_batch([r], m)
+    assert b.status == ga.Status.invalid_argument and "gap_extend > gap_open" in b.error

@@ -304,3 +304,47 @@ def test_wfa_non_acgt_bytes_across_classes(gpu, oracle):
     b = ga.align_batch_ext(reqs)
     assert b.ok()
     assert [(r.score, r.cigar) for r in b.results] == exp
+
+
+# ------------------------------------------------------------------ protein SW
+def _protein(rng, n, alphabet="ARNDCQEGHILKMFPSTWYV"):
+    return "".join(rng.choice(alphabet) for _ in range(n))
+
+
+def test_protein_sw_blosum62_vs_oracle(gpu, oracle):
+    """BLOSUM62 with gap open 11 / extend 1: score, span and CIGAR (trace
+    batches) and the lane-rule end (score-only batches) equal the oracle's
+    table-driven Gotoh; residues outside the alphabet score -1."""
+    import genoaligner_amd as ga
+    m = ga.blosum62(11, 1)
+    alpha = m.alphabet.decode()
+    S = [[m.score[a][b] for b in range(20)] for a in range(20)]
+    tab = oracle.matrix_table(alpha, S, -1)
+    rng = random.Random(41)
+    reqs, pairs = [], []
+    for k in range(160):
+        t = _protein(rng, rng.randint(0, 600))
+        if rng.random() < 0.6 and t:
+            a = rng.randint(0, len(t) - 1)
+            p = _mutate(rng, t[a: a + rng.randint(1, 300)], 0.25, alpha)
+        else:
+            p = _protein(rng, rng.randint(0, 300))
+        if k % 9 == 0 and p:
+            p = p[:1] + "X" + p[2:]
+        if k % 13 == 0 and t:
+            t = "b" + t[1:]
+        pairs.append((t, p))
+        reqs.append(ga.SWRequest(text=t, text_len=len(t), pattern=p, pattern_len=len(p), with_cigar=True))
+    b = ga.align_sw_matrix_batch(reqs, m)
+    assert b.ok(), b.error
+    for (t, p), r in zip(pairs, b.results):
+        sc, si, sj, ei, ej, cg = oracle.sw_trace(t, p, tab, 11, 1)
+        assert (r.score, r.start_i, r.start_j, r.end_i, r.end_j, r.cigar) == (sc, si, sj, ei, ej, cg), (t, p)
+        if sc > 0:
+            assert r.rescore_ok and r.wellformed_ok
+    for r in reqs:
+        r.with_cigar = False
+    bs = ga.align_sw_matrix_batch(reqs, m)
+    assert bs.ok()
+    for (t, p), r in zip(pairs, bs.results):
+        assert (r.score, r.end_i, r.end_j) == oracle.sw_score_lane(t, p, tab, 11, 1), (t, p)

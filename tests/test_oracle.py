@@ -17,6 +17,8 @@ Pins per oracle function:
 import functools
 import random
 
+import numpy as np
+
 import pytest
 
 from conftest import golden_sw_score, golden_sw_trace, golden_wfa
@@ -253,3 +255,20 @@ def test_oracle_equals_reference_random(oracle):
         t = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 60)))
         p = _mutate(rng, t[rng.randint(0, len(t)):], 0.15)
         assert oracle.sw_trace(t, p, sc) == oracle.ref_sw_trace(t, p, sc)
+
+
+def test_matrix_table_reduces_to_match_mismatch(oracle):
+    """A 4-letter ACGT matrix with 2 / -3 (unknown -3) gives the oracle exactly
+    the match/mismatch table on every byte pair that matters: both sequences
+    over ACGT plus other bytes (unequal bytes score the mismatch either way,
+    equal non-ACGT bytes are the one difference and are left out)."""
+    import random
+    mm = oracle.mm_table(2, -3).reshape(256, 256)
+    mt = oracle.matrix_table("ACGT", [[2 if a == b else -3 for b in range(4)] for a in range(4)], -3).reshape(256, 256)
+    idx = [ord(c) for c in "ACGT"]
+    assert (mm[np.ix_(idx, idx)] == mt[np.ix_(idx, idx)]).all()
+    rng = random.Random(4)
+    for _ in range(40):
+        t = "".join(rng.choice("ACGT") for _ in range(rng.randint(1, 60)))
+        p = "".join(rng.choice("ACGT") for _ in range(rng.randint(1, 40)))
+        assert oracle.sw_trace(t, p, (2, -3, 5, 2)) == oracle.sw_trace(t, p, mt, 5, 2)
