@@ -183,6 +183,10 @@ def main():
     ap.add_argument("--router-weights", default=None,
                     help="router parameters (.npy); default: the trained file shipped in the package")
     ap.add_argument("--route-q", type=float, default=0.9)
+    ap.add_argument("--no-protein", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="requests per second (Poisson)")
+    ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, to scale hardware byte counters)")
@@ -361,6 +365,66 @@ def main():
         log(f"wfa_routing: {routing_line}")
         del plain, routed
 
+    # ---------------- protein SW (BLOSUM62, gap open 11 / extend 1), host entry point ----------------
+    protein_line = None
+    if not args.no_protein:
+        import genoaligner_amd as ga
+        pp = W.protein_pairs(n=2000)
+        m62 = ga.blosum62(11, 1)
+        preqs = [ga.SWRequest(text=pp.a_seq(i), text_len=int(pp.a_len[i]), pattern=pp.b_seq(i),
+                              pattern_len=int(pp.b_len[i]), with_cigar=True) for i in range(pp.n)]
+        pcells = int((pp.a_len.astype(np.int64) * pp.b_len).sum())
+        ga.align_sw_matrix_batch(preqs[:64], m62)
+        t0 = time.perf_counter()
+        pb = ga.align_sw_matrix_batch(preqs, m62)
+        pdt = time.perf_counter() - t0
+        tab = O.matrix_table(m62.alphabet.decode(), [[m62.score[a][b] for b in range(20)] for a in range(20)], -1)
+        pbad = 0
+        for i in rng.choice(pp.n, size=20, replace=False):
+            r = pb.results[i]
+            e = O.sw_trace(pp.a_seq(i), pp.b_seq(i), tab, 11, 1)
+            if (r.score, r.start_i, r.start_j, r.end_i, r.end_j, r.cigar) != e:
+                pbad += 1
+        protein_line = {
+            "workload": "protein_pairs 2000, target 100..2000 aa x query 100..1000 aa, BLOSUM62 background "
+                        "frequencies, BLOSUM62 gap open 11 / extend 1, score + span + CIGAR, host entry point "
+                        "(includes host packing, PCIe copies and CIGAR validation)",
+            "gcups": round(pcells / pdt / 1e9, 3), "ms": round(pdt * 1e3, 2), "cells": pcells,
+            "mismatches": pbad, "checked": 20,
+        }
+        log(f"protein: {protein_line}")
+
+    # ---------------- serving: Poisson arrivals, continuous batching ----------------
+    serve_line = None
+    if not args.no_serve:
+        import genoaligner_amd as ga
+        from genoaligner_amd.serve import Server, poisson_load
+        srng = np.random.default_rng(5)
+        sreqs = [W.serving_request(srng, ga) for _ in range(args.serve_requests)]
+        server = Server(budget_bytes=64 << 30, slots=4)
+        t0 = time.perf_counter()
+        tickets = poisson_load(server, lambda k: sreqs[k], args.serve_rate, len(sreqs), seed=9)
+        for t in tickets:
+            t.wait()
+        wall = time.perf_counter() - t0
+        server.close()
+        lat = [t.latency_ms for t in tickets]
+        errors = sum(1 for t in tickets if t.error)
+        by = {"wfa": [t for t in tickets if t.kind == "wfa"], "sw": [t for t in tickets if t.kind == "sw"]}
+        serve_line = {
+            "workload": f"{len(sreqs)} requests, Poisson {args.serve_rate}/s, 1..512 pairs each; 60% WFA DNA "
+                        "100 bp..5 kbp at 0..10% divergence, 40% protein SW (query 100..1000 aa, target "
+                        "100..2000 aa, BLOSUM62 11/1); 70% of requests with CIGAR; 64 GB budget, 4 slots",
+            "p50_ms": pctl(lat, 50), "p99_ms": pctl(lat, 99),
+            "wfa_p50_ms": pctl([t.latency_ms for t in by["wfa"]], 50),
+            "sw_p50_ms": pctl([t.latency_ms for t in by["sw"]], 50),
+            "pairs": int(sum(t.pairs() for t in tickets)), "wall_s": round(wall, 3),
+            "pairs_per_s": round(sum(t.pairs() for t in tickets) / wall, 1),
+            "batches": len(server.batches), "peak_reserved_gb": round(server.peak_reserved / 2**30, 3),
+            "errors": errors,
+        }
+        log(f"serving: {serve_line}")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
@@ -393,6 +457,8 @@ def main():
         "wfa": wfa_line,
         "wfa_mixed": mixed_line,
         "wfa_routing": routing_line,
+        "sw_protein": protein_line,
+        "serving": serve_line,
     }
     print(json.dumps(line), flush=True)
 

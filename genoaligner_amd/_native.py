@@ -27,7 +27,7 @@ EXPORTS = (
     "ga_wfa_device", "ga_sw_device", "ga_set_kernel_timing", "ga_last_kernel_ms",
     "ga_sketch_device", "ga_mlp_forward_device", "ga_mlp_backward_device", "ga_mlp_sgd_device",
     "ga_mlp_route_device", "ga_route_bucket_bound", "ga_wfa_routed_device",
-    "ga_align_sw_matrix_batch", "ga_sw_matrix_blosum62",
+    "ga_align_sw_matrix_batch", "ga_sw_matrix_blosum62", "ga_set_thread_workspace_budget",
 )
 
 
@@ -137,6 +137,8 @@ def lib() -> C.CDLL:
         L.ga_align_sw_matrix_batch.restype = C.c_int
         L.ga_sw_matrix_blosum62.argtypes = [C.POINTER(SwMatrix), C.c_int32, C.c_int32]
         L.ga_sw_matrix_blosum62.restype = None
+        L.ga_set_thread_workspace_budget.argtypes = [C.c_int64]
+        L.ga_set_thread_workspace_budget.restype = None
         L.ga_route_bucket_bound.argtypes = [C.c_int32]
         L.ga_route_bucket_bound.restype = C.c_int32
         _lib = L

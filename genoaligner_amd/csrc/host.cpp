@@ -252,6 +252,13 @@ void sw_validate(const char* cg, int len, const char* text, int text_len, int st
 constexpr int kWfaClassBound[] = {64, 256, 1024, GA_MAX_SMAX_EXT};
 constexpr int kWfaNumClasses = 4;
 constexpr int64_t kWfaStackBudget = int64_t(24) << 30;   // bytes of stacks per class launch
+// Per-thread cap on the workspace of one batch call (0: the defaults above);
+// a serving loop gives each of its concurrent slots a share of its budget.
+thread_local int64_t t_workspace_budget = 0;
+int64_t wfa_stack_budget()
+{
+    return t_workspace_budget > 0 ? std::max<int64_t>(t_workspace_budget / 4, 1 << 20) : kWfaStackBudget;
+}
 
 struct WfaPlan {
     std::vector<int32_t> order;                 // pair ids, class by class
@@ -344,7 +351,7 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
         if (ga::wfa_lds_bytes(L) > kWfaMaxStagedLds) L.staged = false;
         int groups = ga::wfa_queue_groups(L);
         const int64_t per_group = int64_t(ga::wfa_queue_ws_bytes(L.smax_cls, 1, L.wide));
-        if (trace) groups = int(std::max<int64_t>(1, std::min<int64_t>(groups, kWfaStackBudget / per_group)));
+        if (trace) groups = int(std::max<int64_t>(1, std::min<int64_t>(groups, wfa_stack_budget() / per_group)));
         L.n_groups = groups;
         L.ws = nullptr;
         if (trace && hipMallocAsync(&L.ws, size_t(per_group) * size_t(groups), st) != hipSuccess)
@@ -538,7 +545,11 @@ int wfa_common(const ga_align_req* reqs, int64_t n, ga_align_res* res, char* are
 // ---------------------------------------------------------------------------
 // SW batch on the device
 // ---------------------------------------------------------------------------
-constexpr int64_t kSwDirBudget = int64_t(16) << 30;     // direction nibbles per chunk
+constexpr int64_t kSwDirBudgetDefault = int64_t(16) << 30;   // direction nibbles per chunk
+int64_t sw_dir_budget()
+{
+    return t_workspace_budget > 0 ? std::max<int64_t>(t_workspace_budget / 2, 1 << 20) : kSwDirBudgetDefault;
+}
 constexpr int kLdsBndMaxRows = 8192;                     // 2 ints per row in LDS
 constexpr int kLdsBndMaxRowsMatrix = 7168;               // leaves 8 KB for the score profile
 
@@ -730,7 +741,7 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
             std::vector<int64_t> local(static_cast<size_t>(n32));
             for (int64_t k = 0; k < n32; ++k) {
                 const int64_t w = ga::sw_dir_words(tl[size_t(k)], pl[size_t(k)]);
-                if (k > a && (words + w) * 4 > kSwDirBudget) {
+                if (k > a && (words + w) * 4 > sw_dir_budget()) {
                     chunks.emplace_back(a, k);
                     max_words = std::max(max_words, words);
                     a = k; words = 0;
@@ -787,7 +798,7 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
                 }
                 const int64_t nm = std::max(mm, wm), nn = std::max(mn, wn);
                 const int64_t stride = trace ? ga::sw16_dir_words_per_wave(int(nm), int(nn)) : 0;
-                if (w > w0 && stride * (w - w0 + 1) * 4 > kSwDirBudget) {
+                if (w > w0 && stride * (w - w0 + 1) * 4 > sw_dir_budget()) {
                     wchunks.emplace_back(w0, w);
                     max_stride = std::max(max_stride, trace ? ga::sw16_dir_words_per_wave(int(mm), int(mn)) * (w - w0) : 0);
                     w0 = w; mm = wm; mn = wn;
@@ -1159,7 +1170,7 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)
         const int64_t n_waves = (b->n_pairs + 1) / 2;
         const int64_t stride = trace ? ga::sw16_dir_words_per_wave(b->max_text_len, b->max_pattern_len) : 0;
         int64_t cw = n_waves;
-        if (trace && stride > 0) cw = std::max<int64_t>(1, std::min<int64_t>(cw, kSwDirBudget / (stride * 4)));
+        if (trace && stride > 0) cw = std::max<int64_t>(1, std::min<int64_t>(cw, sw_dir_budget() / (stride * 4)));
         uint8_t* d_rerun = nullptr;
         uint32_t* d_dirs = nullptr;
         int32_t* d_wave_m = nullptr;
@@ -1199,7 +1210,7 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)
     const int64_t per = ga::sw_dir_words(b->max_text_len, b->max_pattern_len);
     const bool lds_bnd = b->max_text_len <= kLdsBndMaxRows;
     int64_t chunk = b->n_pairs;
-    if (trace) chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, kSwDirBudget / std::max<int64_t>(per * 4, 1)));
+    if (trace) chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, sw_dir_budget() / std::max<int64_t>(per * 4, 1)));
     if (!lds_bnd) chunk = std::min<int64_t>(chunk, int64_t(1) << 16);
     uint32_t* d_dirs = nullptr;
     int32_t* d_bnd = nullptr;
@@ -1405,5 +1416,7 @@ int ga_wfa_routed_device(const ga_wfa_dev_batch* b, const float* params, float q
     }
     return GA_OK;
 }
+
+void ga_set_thread_workspace_budget(int64_t bytes) { t_workspace_budget = bytes > 0 ? bytes : 0; }
 
 }  // extern "C"

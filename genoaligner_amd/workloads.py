@@ -225,3 +225,33 @@ def wfa_mixed(n: int = 200_000, lo: int = 100, hi: int = 10_000, dmax: float = 0
     return PairSet(pat, _offsets(pl), pl, txt, toff, tl, smax,
                    meta={"workload": "wfa_mixed", "n": n, "lo": lo, "hi": hi, "dmax": dmax,
                          "seed": seed, "rate": rate})
+
+
+# ---------------------------------------------------------------- serving
+def serving_request(rng: np.random.Generator, api_mod, max_pairs: int = 512, wfa_frac: float = 0.6,
+                    wfa_len=(100, 5000), wfa_div: float = 0.10, sw_q=(100, 1000), sw_t=(100, 2000),
+                    cigar_frac: float = 0.7):
+    """One request of the serving mix: (kind, items, with_cigar).  WFA: DNA
+    pairs, lengths log-uniform in wfa_len, divergence U[0, wfa_div] (2:1:1),
+    smax = min(0.2 len, 2048).  SW: protein pairs (query / target lengths
+    uniform in sw_q / sw_t, BLOSUM62 background frequencies)."""
+    k = int(rng.integers(1, max_pairs + 1))
+    cig = bool(rng.random() < cigar_frac)
+    if rng.random() < wfa_frac:
+        pl = np.exp(rng.uniform(np.log(wfa_len[0]), np.log(wfa_len[1]), size=k)).astype(np.int32)
+        pat = random_bases(rng, int(pl.astype(np.int64).sum()))
+        txt, toff, tl = mutate(rng, pat, pl, rng.uniform(0.0, wfa_div, size=k))
+        po = _offsets(pl)
+        items = []
+        for i in range(k):
+            p = pat[po[i]: po[i] + pl[i]].tobytes()
+            t = txt[toff[i]: toff[i] + tl[i]].tobytes()
+            items.append(api_mod.AlignRequest(pattern=p, pattern_len=len(p), text=t, text_len=len(t),
+                                              smax=int(min(int(0.2 * len(p)), 2048)), with_cigar=cig))
+        return "wfa", items, cig
+    ps = protein_pairs(n=k, qlo=sw_q[0], qhi=sw_q[1], tlo=sw_t[0], thi=sw_t[1],
+                       seed=int(rng.integers(1 << 62)))
+    items = [api_mod.SWRequest(text=ps.a_seq(i), text_len=int(ps.a_len[i]),
+                               pattern=ps.b_seq(i), pattern_len=int(ps.b_len[i]), with_cigar=cig)
+             for i in range(k)]
+    return "sw", items, cig

@@ -258,6 +258,12 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error);
 void  ga_set_kernel_timing(int enabled);
 float ga_last_kernel_ms(void);
 
+/* Caps the device workspace (wavefront stacks, direction tables) that batch
+ * calls made from the calling thread hold at once; 0 restores the defaults.
+ * Larger batches then run in more chunks.  (EXTENSION: for serving loops that
+ * run several batches concurrently under one memory budget.) */
+void ga_set_thread_workspace_budget(int64_t bytes);
+
 #ifdef __cplusplus
 }
 #endif
