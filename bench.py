@@ -235,12 +235,20 @@ def main():
     sw_bad = check_sw(sw, swd, sample, O)
     log(f"sw: {sw_bad} mismatches in {len(sample)} checked pairs")
 
+    # HBM bytes per launch of the SW trace kernel, measured with rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes scaled by a copy of known size
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "sw16_trace_sw_embedded_50k_hbm_traffic.md")
+    if os.path.exists(tpath):
+        import re
+        mt = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(tpath).read())
+        traffic = float(mt.group(1)) if mt else None
     roof = None
     if kern_ms:
         achieved = SW_OPS_PER_CELL * cells / (kern_ms * 1e-3) / 1e12
         roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS,
                 "unit": "TOP/s", "frac": round(achieved / PEAK_INT16_TOPS, 4),
-                "traffic": None, "kernel": "sw_kernel<trace> (forward)",
+                "traffic": traffic, "kernel": "sw16_kernel<true> (packed forward + fused walk)",
                 "kernel_ms": round(kern_ms, 3)}
 
     # score-only on the same pairs: score + end cell (the lane-rule end)
