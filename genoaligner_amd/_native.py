@@ -28,6 +28,7 @@ EXPORTS = (
     "ga_sketch_device", "ga_mlp_forward_device", "ga_mlp_backward_device", "ga_mlp_sgd_device",
     "ga_mlp_route_device", "ga_route_bucket_bound", "ga_wfa_routed_device",
     "ga_align_sw_matrix_batch", "ga_sw_matrix_blosum62", "ga_set_thread_workspace_budget",
+    "ga_wfa_graph_create", "ga_wfa_graph_launch", "ga_wfa_graph_destroy",
 )
 
 
@@ -137,6 +138,12 @@ def lib() -> C.CDLL:
         L.ga_align_sw_matrix_batch.restype = C.c_int
         L.ga_sw_matrix_blosum62.argtypes = [C.POINTER(SwMatrix), C.c_int32, C.c_int32]
         L.ga_sw_matrix_blosum62.restype = None
+        L.ga_wfa_graph_create.argtypes = [C.POINTER(WfaDevBatch), C.POINTER(C.c_void_p), cpp]
+        L.ga_wfa_graph_create.restype = C.c_int
+        L.ga_wfa_graph_launch.argtypes = [C.c_void_p, C.c_void_p, cpp]
+        L.ga_wfa_graph_launch.restype = C.c_int
+        L.ga_wfa_graph_destroy.argtypes = [C.c_void_p]
+        L.ga_wfa_graph_destroy.restype = None
         L.ga_set_thread_workspace_budget.argtypes = [C.c_int64]
         L.ga_set_thread_workspace_budget.restype = None
         L.ga_route_bucket_bound.argtypes = [C.c_int32]

@@ -40,7 +40,22 @@ struct WfaQueueLaunch {
     void* ws;                // n_groups * (smax_cls+1)^2 entries of int16 (int32 if wide)
     bool wide;
     bool staged;
+    const int32_t* dev_class; // device-planned: {first, count} of this class (then n, order ignored / offset)
 };
+
+// Size classes by smax (device planning uses the same bounds as the host plan).
+constexpr int kWfaClasses = 4;
+constexpr int kWfaClassBound0 = 64, kWfaClassBound1 = 256, kWfaClassBound2 = 1024;
+constexpr int kPlanKeys = 1024;              // length bins of 32 bytes per class
+
+struct WfaDevPlan {
+    const int32_t* smax; const int32_t* plen; const int32_t* tlen;
+    int32_t n;
+    int32_t* hist;           // kWfaClasses * kPlanKeys
+    int32_t* cls;            // 2 * kWfaClasses: {first, count} per class
+    int32_t* order;          // n
+};
+hipError_t wfa_plan_launch(const WfaDevPlan& P, hipStream_t st);
 
 size_t wfa_lds_bytes(const WfaQueueLaunch& L);     // dynamic LDS of one workgroup
 int wfa_queue_groups(const WfaQueueLaunch& L);     // resident workgroups (<= n)

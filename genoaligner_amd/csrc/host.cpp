@@ -17,6 +17,7 @@
 #include <atomic>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -249,7 +250,7 @@ void sw_validate(const char* cg, int len, const char* text, int text_len, int st
 // Size classes of the persistent WFA kernel, by a pair's smax: each class is
 // one launch whose LDS wavefronts and per-workgroup stacks are sized for its
 // bound; the stacks of one class launch stay under kWfaStackBudget.
-constexpr int kWfaClassBound[] = {64, 256, 1024, GA_MAX_SMAX_EXT};
+constexpr int kWfaClassBound[] = {ga::kWfaClassBound0, ga::kWfaClassBound1, ga::kWfaClassBound2, GA_MAX_SMAX_EXT};
 constexpr int kWfaNumClasses = 4;
 constexpr int64_t kWfaStackBudget = int64_t(24) << 30;   // bytes of stacks per class launch
 // Per-thread cap on the workspace of one batch call (0: the defaults above);
@@ -390,6 +391,71 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
     (void)hipFreeAsync(d_order, st);
     (void)hipFreeAsync(d_next, st);
     return err;
+}
+
+// Launch shapes and stacks of the four size classes of a device-planned batch
+// (class sizes unknown on the host: every class that can hold a pair is
+// launched over a full grid, and a class whose plan count is 0 exits at once).
+struct WfaDevClasses {
+    std::vector<ga::WfaQueueLaunch> L;
+    int32_t* order = nullptr;
+    int32_t* next = nullptr;
+    ga::WfaDevPlan plan{};
+};
+
+const char* setup_wfa_dev_classes(const ga::WfaQueueLaunch& base, int64_t n, int smax_max, int max_len,
+                                  bool trace, WfaDevClasses& C, const std::function<void*(size_t)>& alloc)
+{
+    C.order = static_cast<int32_t*>(alloc(size_t(n) * 4 + 64));
+    C.next = static_cast<int32_t*>(alloc(64));
+    C.plan.hist = static_cast<int32_t*>(alloc(size_t(ga::kWfaClasses) * ga::kPlanKeys * 4));
+    C.plan.cls = static_cast<int32_t*>(alloc(64));
+    if (!C.order || !C.next || !C.plan.hist || !C.plan.cls) return "hipMalloc(WFA plan)";
+    C.plan.smax = base.smax; C.plan.plen = base.plen; C.plan.tlen = base.tlen;
+    C.plan.n = int32_t(n);
+    C.plan.order = C.order;
+    for (int c = 0; c < ga::kWfaClasses; ++c) {
+        if (c > 0 && smax_max <= kWfaClassBound[c - 1]) break;        // no pair can fall in class c
+        ga::WfaQueueLaunch L = base;
+        L.smax_cls = c < ga::kWfaClasses - 1 ? std::min(kWfaClassBound[c], smax_max) : smax_max;
+        L.max_plen = max_len;
+        L.max_tlen = max_len;
+        L.waves = wfa_class_waves(L.smax_cls);
+        L.staged = true;
+        if (ga::wfa_lds_bytes(L) > kWfaMaxStagedLds) L.staged = false;
+        L.n = int32_t(n);
+        L.order = C.order;
+        L.next = C.next + c;
+        L.dev_class = C.plan.cls + 2 * c;
+        int groups = ga::wfa_queue_groups(L);
+        const int64_t per_group = int64_t(ga::wfa_queue_ws_bytes(L.smax_cls, 1, L.wide));
+        if (trace) groups = int(std::max<int64_t>(1, std::min<int64_t>(groups, wfa_stack_budget() / per_group)));
+        L.n_groups = groups;
+        L.ws = trace ? alloc(size_t(per_group) * size_t(groups)) : nullptr;
+        if (trace && !L.ws) return "hipMalloc(wavefront stacks)";
+        C.L.push_back(L);
+    }
+    return nullptr;
+}
+
+// Plan on the device, then the class launches on forked streams joined back
+// into `st`; `side` holds one stream per class beyond the first (so the same
+// streams can serve a graph capture).  No host synchronisation.
+const char* enqueue_wfa_dev(const WfaDevClasses& C, hipStream_t st, const std::vector<hipStream_t>& side,
+                            std::vector<hipEvent_t>& ev)
+{
+    if (ga::wfa_plan_launch(C.plan, st) != hipSuccess) return "WFA plan launch";
+    const size_t nc = C.L.size();
+    // ev[0] forks, ev[1..] join
+    if (nc > 1 && hipEventRecord(ev[0], st) != hipSuccess) return "hipEventRecord";
+    for (size_t c = 0; c < nc; ++c) {
+        hipStream_t cs = c == 0 ? st : side[c - 1];
+        if (c > 0 && hipStreamWaitEvent(cs, ev[0], 0) != hipSuccess) return "hipStreamWaitEvent";
+        if (ga::wfa_queue_launch(C.L[c], cs) != hipSuccess) return "WFA kernel launch";
+        if (c > 0 && (hipEventRecord(ev[c], cs) != hipSuccess || hipStreamWaitEvent(st, ev[c], 0) != hipSuccess))
+            return "hipEventRecord";
+    }
+    return nullptr;
 }
 
 struct WfaOut {
@@ -1104,27 +1170,10 @@ const char* ga_device_name(void)
 int ga_wfa_device(const ga_wfa_dev_batch* b, void* stream, const char** error)
 {
     if (error) *error = nullptr;
-    if (!b) { if (error) *error = "null batch"; return GA_INVALID_ARGUMENT; }
+    if (const char* e = check_dev_batch(b)) { if (error) *error = e; return GA_INVALID_ARGUMENT; }
     if (b->n_pairs <= 0) return GA_OK;
-    if (b->smax_max < 0 || b->smax_max > GA_MAX_SMAX_EXT) {
-        if (error) *error = "smax_max out of range [0, 4000]";
-        return GA_INVALID_ARGUMENT;
-    }
-    if (b->n_pairs > INT32_MAX) { if (error) *error = "too many pairs"; return GA_INVALID_ARGUMENT; }
     std::call_once(g_pool_once, init_pool);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    WfaPlan plan;
-    if (b->host_smax && b->host_pattern_len && b->host_text_len) {
-        plan = plan_wfa(b->n_pairs, b->host_smax, b->host_pattern_len, b->host_text_len);
-    } else {
-        plan.order.resize(size_t(b->n_pairs));
-        for (int64_t i = 0; i < b->n_pairs; ++i) plan.order[size_t(i)] = int32_t(i);
-        plan.bound.push_back(b->smax_max);
-        plan.max_plen.push_back(b->max_len);
-        plan.max_tlen.push_back(b->max_len);
-        plan.first.push_back(0);
-        plan.count.push_back(b->n_pairs);
-    }
     ga::WfaQueueLaunch L{};
     L.pat = b->pattern; L.poff = b->pattern_off; L.plen = b->pattern_len;
     L.txt = b->text; L.toff = b->text_off; L.tlen = b->text_len;
@@ -1132,9 +1181,129 @@ int ga_wfa_device(const ga_wfa_dev_batch* b, void* stream, const char** error)
     L.score = b->score; L.cigar = b->cigar; L.cig_off = b->cigar_off; L.cig_len = b->cigar_len;
     L.wide = b->max_len > 32766;
     t_timer.reset();
-    const char* err = enqueue_wfa(L, plan, b->cigar != nullptr, st, &t_timer);
+    const char* err = nullptr;
+    if (b->host_smax && b->host_pattern_len && b->host_text_len) {
+        // host plan: exact class maxima, sizes known before launch
+        const WfaPlan plan = plan_wfa(b->n_pairs, b->host_smax, b->host_pattern_len, b->host_text_len);
+        err = enqueue_wfa(L, plan, b->cigar != nullptr, st, &t_timer);
+    } else {
+        // device plan: classes and claim order computed on the GPU
+        std::vector<void*> held;
+        auto alloc = [&](size_t bytes) -> void* {
+            void* p = nullptr;
+            if (hipMallocAsync(&p, bytes, st) != hipSuccess) return nullptr;
+            held.push_back(p);
+            return p;
+        };
+        WfaDevClasses C;
+        err = setup_wfa_dev_classes(L, b->n_pairs, b->smax_max, b->max_len, b->cigar != nullptr, C, alloc);
+        std::vector<hipStream_t> side;
+        std::vector<hipEvent_t> ev(ga::kWfaClasses, nullptr);
+        for (size_t c = 1; !err && c < C.L.size(); ++c) {
+            hipStream_t cs = acquire_stream();
+            if (!cs) err = "hipStreamCreate";
+            else side.push_back(cs);
+        }
+        for (auto& e : ev)
+            if (!err && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) err = "hipEventCreate";
+        if (!err) {
+            t_timer.begin(st);
+            err = enqueue_wfa_dev(C, st, side, ev);
+            t_timer.end(st);
+        }
+        for (void* p : held) (void)hipFreeAsync(p, st);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (hipStream_t cs : side) release_stream(cs);
+    }
     if (err) {
         if (error) *error = err;
+        return GA_DEVICE_ERROR;
+    }
+    return GA_OK;
+}
+
+// ------------------------------------------------------------------ graphs
+struct ga_wfa_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap = nullptr;
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> ev;
+    std::vector<void*> bufs;
+};
+
+void ga_wfa_graph_destroy(ga_wfa_graph* g)
+{
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    for (auto e : g->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto s : g->side)
+        if (s) (void)hipStreamDestroy(s);
+    if (g->cap) (void)hipStreamDestroy(g->cap);
+    for (void* p : g->bufs) (void)hipFree(p);
+    delete g;
+}
+
+int ga_wfa_graph_create(const ga_wfa_dev_batch* b, ga_wfa_graph** out, const char** error)
+{
+    if (error) *error = nullptr;
+    if (!out) { if (error) *error = "null output"; return GA_INVALID_ARGUMENT; }
+    *out = nullptr;
+    if (const char* e = check_dev_batch(b)) { if (error) *error = e; return GA_INVALID_ARGUMENT; }
+    if (b->n_pairs <= 0) { if (error) *error = "empty batch"; return GA_INVALID_ARGUMENT; }
+    auto* g = new ga_wfa_graph;
+    auto alloc = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        g->bufs.push_back(p);
+        return p;
+    };
+    ga::WfaQueueLaunch L{};
+    L.pat = b->pattern; L.poff = b->pattern_off; L.plen = b->pattern_len;
+    L.txt = b->text; L.toff = b->text_off; L.tlen = b->text_len;
+    L.smax = b->smax; L.want = b->want_cigar;
+    L.score = b->score; L.cigar = b->cigar; L.cig_off = b->cigar_off; L.cig_len = b->cigar_len;
+    L.wide = b->max_len > 32766;
+    WfaDevClasses C;
+    const char* err = setup_wfa_dev_classes(L, b->n_pairs, b->smax_max, b->max_len, b->cigar != nullptr, C, alloc);
+    if (!err && hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking) != hipSuccess) err = "hipStreamCreate";
+    for (size_t c = 1; !err && c < C.L.size(); ++c) {
+        hipStream_t cs = nullptr;
+        if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) err = "hipStreamCreate";
+        else g->side.push_back(cs);
+    }
+    g->ev.assign(ga::kWfaClasses, nullptr);
+    for (auto& e : g->ev)
+        if (!err && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) err = "hipEventCreate";
+    if (!err) {
+        if (hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            err = "hipStreamBeginCapture";
+        } else {
+            err = enqueue_wfa_dev(C, g->cap, g->side, g->ev);
+            const hipError_t ce = hipStreamEndCapture(g->cap, &g->graph);
+            if (!err && ce != hipSuccess) err = "hipStreamEndCapture";
+        }
+    }
+    if (!err && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+        err = "hipGraphInstantiate";
+    if (err) {
+        ga_wfa_graph_destroy(g);
+        if (error) *error = err;
+        return GA_DEVICE_ERROR;
+    }
+    *out = g;
+    return GA_OK;
+}
+
+int ga_wfa_graph_launch(ga_wfa_graph* g, void* stream, const char** error)
+{
+    if (error) *error = nullptr;
+    if (!g || !g->exec) { if (error) *error = "null graph"; return GA_INVALID_ARGUMENT; }
+    if (hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)) != hipSuccess) {
+        if (error) *error = "hipGraphLaunch";
         return GA_DEVICE_ERROR;
     }
     return GA_OK;

@@ -228,6 +228,14 @@ void wfa_kernel(WfaQueueLaunch L)
     Off* ws = TRACE ? static_cast<Off*>(L.ws) + int64_t(blockIdx.x) * (int64_t(S) + 1) * (int64_t(S) + 1)
                     : nullptr;
     const int G = int(gridDim.x);
+    // class size and the start of its slice of the claim order: from the launch,
+    // or (device-planned batches) from the plan the device wrote
+    int n_cls = L.n;
+    const int32_t* order = L.order;
+    if (L.dev_class) {
+        n_cls = __builtin_amdgcn_readfirstlane(L.dev_class[1]);
+        order = L.order + __builtin_amdgcn_readfirstlane(L.dev_class[0]);
+    }
     for (int it = 0;; ++it) {
         // The first round is static (workgroup g takes pair g); later pairs are
         // claimed from the counter, which then starts at G.  No atomics at all
@@ -240,7 +248,7 @@ void wfa_kernel(WfaQueueLaunch L)
         int q;
         if (it == 0) {
             q = int(blockIdx.x);
-        } else if (L.n <= G) {
+        } else if (n_cls <= G) {
             break;
         } else if (WAVES == 1) {
             q = G + __builtin_amdgcn_readfirstlane(atomicAdd(L.next, lane == 0 ? 1 : 0));
@@ -249,8 +257,8 @@ void wfa_kernel(WfaQueueLaunch L)
             __syncthreads();
             q = G + __builtin_amdgcn_readfirstlane(claim[it & 1]);
         }
-        if (q >= L.n) break;                      // every wave ends here once the queue is empty
-        const int pid = L.order ? L.order[q] : q;
+        if (q >= n_cls) break;                    // every wave ends here once the queue is empty
+        const int pid = order ? order[q] : q;
 
         const int m = L.plen[pid];
         const int n = L.tlen[pid];
@@ -367,7 +375,69 @@ void wfa_kernel(WfaQueueLaunch L)
 
 __global__ void zero_i32_kernel(int32_t* p, int n)
 {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += int(gridDim.x * blockDim.x)) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Device-side planning: size class by smax, claim order by length (longest
+// first) as a counting sort over kPlanKeys length bins of 32 bytes per class:
+// histogram, one-workgroup exclusive scan, scatter.  Order inside a bin is
+// whatever the atomics give; results do not depend on it.
+__device__ __forceinline__ int plan_class(int smax)
+{
+    return smax <= kWfaClassBound0 ? 0 : smax <= kWfaClassBound1 ? 1 : smax <= kWfaClassBound2 ? 2 : 3;
+}
+__device__ __forceinline__ int plan_key(int m, int n)
+{
+    const int l = ((m > 0 ? m : 0) + (n > 0 ? n : 0)) >> 5;
+    return kPlanKeys - 1 - (l < kPlanKeys - 1 ? l : kPlanKeys - 1);
+}
+
+__global__ __launch_bounds__(256) void plan_hist_kernel(WfaDevPlan P)
+{
+    const int i = int(blockIdx.x * 256 + threadIdx.x);
+    if (i >= P.n) return;
+    atomicAdd(&P.hist[plan_class(P.smax[i]) * kPlanKeys + plan_key(P.plen[i], P.tlen[i])], 1);
+}
+
+// hist (kWfaClasses * kPlanKeys counts) -> exclusive offsets; cls[2c] = first
+// pair of class c in the order, cls[2c+1] = its size
+__global__ __launch_bounds__(1024) void plan_scan_kernel(WfaDevPlan P)
+{
+    __shared__ int part[1024];
+    constexpr int kPer = kWfaClasses * kPlanKeys / 1024;
+    const int t = threadIdx.x;
+    int v[kPer];
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) { v[q] = P.hist[t * kPer + q]; sum += v[q]; }
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {             // inclusive Hillis-Steele scan
+        const int add = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    int run = part[t] - sum;                                // exclusive prefix of this thread
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) { P.hist[t * kPer + q] = run; run += v[q]; }
+    // class c spans bins [c*K, (c+1)*K): its first offset is that of bin c*K
+    __syncthreads();
+    if (t < kWfaClasses) {
+        const int first = P.hist[t * kPlanKeys];
+        const int end = t + 1 < kWfaClasses ? P.hist[(t + 1) * kPlanKeys] : part[1023];
+        P.cls[2 * t] = first;
+        P.cls[2 * t + 1] = end - first;
+    }
+}
+
+__global__ __launch_bounds__(256) void plan_scatter_kernel(WfaDevPlan P)
+{
+    const int i = int(blockIdx.x * 256 + threadIdx.x);
+    if (i >= P.n) return;
+    const int pos = atomicAdd(&P.hist[plan_class(P.smax[i]) * kPlanKeys + plan_key(P.plen[i], P.tlen[i])], 1);
+    P.order[pos] = i;
 }
 
 }  // namespace
@@ -488,6 +558,17 @@ hipError_t wfa_queue_launch(const WfaQueueLaunch& L, hipStream_t st)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return dispatch(L, wfa_lds_bytes(L), st, nullptr);
+}
+
+hipError_t wfa_plan_launch(const WfaDevPlan& P, hipStream_t st)
+{
+    if (P.n <= 0) return hipSuccess;
+    const unsigned nb = unsigned((P.n + 255) / 256);
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(16), dim3(256), 0, st, P.hist, kWfaClasses * kPlanKeys);
+    hipLaunchKernelGGL(plan_hist_kernel, dim3(nb), dim3(256), 0, st, P);
+    hipLaunchKernelGGL(plan_scan_kernel, dim3(1), dim3(1024), 0, st, P);
+    hipLaunchKernelGGL(plan_scatter_kernel, dim3(nb), dim3(256), 0, st, P);
+    return hipGetLastError();
 }
 
 }  // namespace ga

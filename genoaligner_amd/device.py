@@ -91,6 +91,11 @@ class WfaDeviceBatch:
         _check(N.lib().ga_wfa_device(C.byref(self.b), C.c_void_p(_stream_handle(stream)),
                                      C.byref(err)), err, "ga_wfa_device")
 
+    def graph(self) -> "WfaGraph":
+        """Capture the device-planned submission of this batch into a HIP graph
+        (its buffers are baked in: refill them in place and replay)."""
+        return WfaGraph(self)
+
     def results(self):
         """(scores, cigars) on the host; cigars[i] is bytes ('' when none)."""
         torch.cuda.synchronize()
@@ -107,6 +112,32 @@ class WfaDeviceBatch:
                 L = 0
             cig.append(raw[end - L: end])
         return sc, cig
+
+
+class WfaGraph:
+    def __init__(self, batch: WfaDeviceBatch):
+        self.batch = batch                      # keeps the buffers alive
+        h = C.c_void_p()
+        err = C.c_char_p()
+        _check(N.lib().ga_wfa_graph_create(C.byref(batch.b), C.byref(h), C.byref(err)), err,
+               "ga_wfa_graph_create")
+        self.h = h
+
+    def launch(self, stream: Optional[torch.cuda.Stream] = None):
+        err = C.c_char_p()
+        _check(N.lib().ga_wfa_graph_launch(self.h, C.c_void_p(_stream_handle(stream)), C.byref(err)), err,
+               "ga_wfa_graph_launch")
+
+    def close(self):
+        if self.h:
+            N.lib().ga_wfa_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class SwDeviceBatch:

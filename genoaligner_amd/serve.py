@@ -87,7 +87,7 @@ class LibraryBackend:
 
 class Server:
     def __init__(self, budget_bytes: int = 64 << 30, slots: int = 4, max_batch_pairs: int = 16384,
-                 backend=None, clock: Callable[[], float] = time.perf_counter):
+                 backend=None, clock: Callable[[], float] = time.perf_counter, start: bool = True):
         self.budget = int(budget_bytes)
         self.slots = int(slots)
         self.workspace = self.budget // (2 * self.slots)     # per running batch
@@ -102,8 +102,16 @@ class Server:
         self.batches: List[dict] = []                          # one record per batch run
         self._threads = [threading.Thread(target=self._slot, name=f"ga-slot-{i}", daemon=True)
                          for i in range(self.slots)]
-        for t in self._threads:
-            t.start()
+        self._started = False
+        if start:
+            self.start()
+
+    def start(self):
+        """Start the slots (a server created with start=False only queues)."""
+        if not self._started:
+            self._started = True
+            for t in self._threads:
+                t.start()
 
     # ------------------------------------------------------------ client side
     def submit(self, kind: str, items: Sequence, with_cigar: bool = True) -> Request:
@@ -124,7 +132,7 @@ class Server:
         with self._cv:
             self._closed = True
             self._cv.notify_all()
-        if wait:
+        if wait and self._started:
             for t in self._threads:
                 t.join()
 

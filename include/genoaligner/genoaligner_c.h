@@ -189,6 +189,21 @@ typedef struct ga_wfa_dev_batch {
 
 int ga_wfa_device(const ga_wfa_dev_batch* b, void* stream, const char** error);
 
+/* Without the host copies, ga_wfa_device plans on the device: a histogram /
+ * scan / scatter over (size class, length) gives each class its slice of a
+ * longest-first claim order, and every class that can hold a pair is launched
+ * over a full persistent grid, with no host synchronisation.
+ *
+ * Graph submission (EXTENSION): ga_wfa_graph_create captures that whole
+ * device-planned submission of batch `b` (its buffers, sizes and bounds are
+ * baked in) into a HIP graph; ga_wfa_graph_launch replays it on a stream, so a
+ * caller that refills the same buffers resubmits with one call.  Workspace is
+ * held by the graph until ga_wfa_graph_destroy. */
+typedef struct ga_wfa_graph ga_wfa_graph;
+int  ga_wfa_graph_create(const ga_wfa_dev_batch* b, ga_wfa_graph** out, const char** error);
+int  ga_wfa_graph_launch(ga_wfa_graph* g, void* stream, const char** error);
+void ga_wfa_graph_destroy(ga_wfa_graph* g);
+
 /* ---- learned router (EXTENSION: no reference counterpart) ------------------
  * Sketch of a pair: GA_SKETCH_DIM floats of 4-mer / length features and the
  * exact lower bound max(|m - n|, ceil(||Q4(P) - Q4(T)||_1 / 8)) on its edit

@@ -348,3 +348,35 @@ def test_protein_sw_blosum62_vs_oracle(gpu, oracle):
     assert bs.ok()
     for (t, p), r in zip(pairs, bs.results):
         assert (r.score, r.end_i, r.end_j) == oracle.sw_score_lane(t, p, tab, 11, 1), (t, p)
+
+
+def test_wfa_graph_replays_match_eager(gpu, oracle):
+    """The device-planned submission captured into a HIP graph: every replay
+    (outputs reset between replays, eager work of the program in between)
+    equals the eager run, which equals the oracle."""
+    import torch
+    from genoaligner_amd import workloads as W
+    from genoaligner_amd.device import WfaDeviceBatch
+    ps = W.wfa_mixed(n=900, lo=30, hi=3000, seed=404)
+    want = (np.random.default_rng(9).random(ps.n) < 0.7).astype(np.uint8)
+    eager = WfaDeviceBatch(ps, trace=True, want=want, classes=False)
+    eager.run()
+    esc, ecg = eager.results()
+    for i in range(0, ps.n, 20):
+        sc, cg = oracle.wfa(ps.a_seq(i), ps.b_seq(i), int(ps.smax[i]))
+        assert (int(esc[i]), ecg[i].decode()) == (sc, cg if want[i] else "")
+    gb = WfaDeviceBatch(ps, trace=True, want=want, classes=False)
+    g = gb.graph()
+    other = WfaDeviceBatch(W.wfa_plain(n=500, seed=3), trace=True)
+    for rep in range(4):
+        gb.score.fill_(-7)
+        gb.cig_len.fill_(-7)
+        gb.cigar.fill_(0)
+        torch.cuda.synchronize()
+        g.launch()
+        other.run()                                   # the program's own eager work in between
+        torch.cuda.synchronize()
+        sc, cg = gb.results()
+        assert [int(x) for x in sc] == [int(x) for x in esc], rep
+        assert cg == ecg, rep
+    g.close()

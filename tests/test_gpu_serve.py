@@ -26,11 +26,12 @@ def test_results_alone_equal_results_in_the_mix(gpu, oracle):
     reqs = [W.serving_request(rng, ga, max_pairs=48, wfa_len=(100, 1500), sw_q=(50, 300), sw_t=(50, 500))
             for _ in range(30)]
     m = ga.blosum62(11, 1)
-    s = Server(budget_bytes=4 << 30, slots=3)
-    tickets = []
-    for kind, items, cig in reqs:
+    s = Server(budget_bytes=4 << 30, slots=3, start=False)
+    tickets = [s.submit(kind, items, with_cigar=cig) for kind, items, cig in reqs[:12]]
+    s.start()                                         # 12 queued requests batch together
+    for kind, items, cig in reqs[12:]:                # the rest arrive while slots are busy
         tickets.append(s.submit(kind, items, with_cigar=cig))
-        time.sleep(float(rng.exponential(0.003)))
+        time.sleep(float(rng.exponential(0.002)))
     for t in tickets:
         assert t.wait(120)
     s.close()

@@ -139,3 +139,14 @@ def test_oversized_request_is_refused():
     r = s.submit("wfa", [_wfa("huge", 10_000)])
     assert r.wait(1) and r.error == "request larger than the memory budget"
     s.close()
+
+
+def test_paused_server_batches_the_queue():
+    fb = FakeBackend(delay=0.01)
+    s = Server(budget_bytes=1 << 30, slots=2, backend=fb, start=False)
+    reqs = [s.submit("wfa", [_wfa(f"q{k}")]) for k in range(8)]
+    s.start()
+    for r in reqs:
+        assert r.wait(5)
+    s.close()
+    assert len(fb.calls) <= 2 and sum(len(t) for _, t in fb.calls) == 8
