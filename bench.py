@@ -284,10 +284,21 @@ def main():
         in_bytes = int(((wf.a_len.astype(np.int64) + 3) // 4 + (wf.b_len.astype(np.int64) + 3) // 4).sum())
         sc, cg = wfd.results()
         out_bytes = 5 * wf.n + int(sum(len(c) for c in cg))
+        # the same batch resubmitted as a captured HIP graph (device-planned)
+        gb = WfaDeviceBatch(wf, trace=True, classes=False)
+        graph = gb.graph()
+        gwall, gstep, _ = time_steps(graph.launch, args.steps, args.warmup, lib)
+        gsc, gcg = gb.results()
+        gdiff = int(sum(1 for i in range(wf.n) if int(gsc[i]) != int(sc[i]) or gcg[i] != cg[i]))
+        graph.close()
+        del gb
         wfa_line = {
             "workload": "wfa_plain 10000 x 150 bp, 2% edits (2:1:1), smax 64, score + CIGAR",
             "pairs_per_s": round(wf.n / (wwall / args.steps), 1),
             "ms_per_step": round(wwall * 1000.0 / args.steps, 3),
+            "graph_pairs_per_s": round(wf.n / (gwall / args.steps), 1),
+            "graph_ms_per_step": round(gwall * 1000.0 / args.steps, 3),
+            "graph_vs_eager_differences": gdiff,
             "p50_ms": pctl(wstep, 50), "p99_ms": pctl(wstep, 99),
             "kernel_ms": round(wk, 4) if wk else None,
             "hbm_roofline": {"bound": "hbm", "algorithmic_bytes": in_bytes + out_bytes,
@@ -307,6 +318,14 @@ def main():
         want = (np.random.default_rng(4).random(wm.n) < 0.7).astype(np.uint8)
         wmd = WfaDeviceBatch(wm, trace=True, want=want)
         log(f"wfa_mixed workload ready ({time.perf_counter() - t0:.1f} s)")
+        # the autotuner picks waves per workgroup for each size class on a
+        # sample of the batch, then the whole batch runs with that choice
+        from genoaligner_amd.device import wfa_autotune, wfa_set_tuning
+        tidx = np.random.default_rng(8).choice(wm.n, min(20000, wm.n), replace=False)
+        tsample = WfaDeviceBatch(wm.subset(tidx), trace=True, want=want[tidx])
+        tuned_waves, tuned_ms = wfa_autotune(tsample, reps=2)
+        del tsample
+        log(f"wfa_mixed autotune: waves per class {tuned_waves}, ms {tuned_ms}")
         mwall, mstep, mkern = time_steps(wmd.run, args.steps, args.warmup, lib)
         msample = rng.choice(wm.n, size=min(300, wm.n), replace=False)
         sc, cg = wmd.results()
@@ -324,6 +343,7 @@ def main():
             "p50_ms": pctl(mstep, 50), "p99_ms": pctl(mstep, 99),
             "kernel_ms": round(mk, 3) if mk else None,
             "resolved": int((sc >= 0).sum()), "mismatches": mbad, "checked": int(len(msample)),
+            "autotuned_waves_per_class": tuned_waves,
         }
         log(f"wfa_mixed: {mixed_line}")
         if not args.no_cpu_baseline:
@@ -331,6 +351,7 @@ def main():
                                                           min(os.cpu_count() or 1, 16), O)
             log(f"wfa_mixed cpu baseline: {mixed_line['cpu_baseline']}")
         del wmd
+        wfa_set_tuning(None)
 
     # ---------------- WFA routing: plain per-class path vs learned router ----------------
     routing_line = None

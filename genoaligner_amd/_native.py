@@ -29,6 +29,7 @@ EXPORTS = (
     "ga_mlp_route_device", "ga_route_bucket_bound", "ga_wfa_routed_device",
     "ga_align_sw_matrix_batch", "ga_sw_matrix_blosum62", "ga_set_thread_workspace_budget",
     "ga_wfa_graph_create", "ga_wfa_graph_launch", "ga_wfa_graph_destroy",
+    "ga_wfa_autotune", "ga_wfa_set_tuning", "ga_wfa_get_tuning",
 )
 
 
@@ -74,6 +75,10 @@ class WfaDevBatch(C.Structure):
 class SwMatrix(C.Structure):
     _fields_ = [("alphabet", C.c_char * 32), ("score", (C.c_int8 * 32) * 32), ("unknown", C.c_int32),
                 ("gap_open", C.c_int32), ("gap_extend", C.c_int32)]
+
+
+class WfaTuning(C.Structure):
+    _fields_ = [("waves", C.c_int32 * 4), ("class_ms", C.c_float * 4)]
 
 
 class RouteStats(C.Structure):
@@ -138,6 +143,12 @@ def lib() -> C.CDLL:
         L.ga_align_sw_matrix_batch.restype = C.c_int
         L.ga_sw_matrix_blosum62.argtypes = [C.POINTER(SwMatrix), C.c_int32, C.c_int32]
         L.ga_sw_matrix_blosum62.restype = None
+        L.ga_wfa_autotune.argtypes = [C.POINTER(WfaDevBatch), C.c_int, C.POINTER(WfaTuning), C.c_void_p, cpp]
+        L.ga_wfa_autotune.restype = C.c_int
+        L.ga_wfa_set_tuning.argtypes = [C.POINTER(WfaTuning)]
+        L.ga_wfa_set_tuning.restype = None
+        L.ga_wfa_get_tuning.argtypes = [C.POINTER(WfaTuning)]
+        L.ga_wfa_get_tuning.restype = None
         L.ga_wfa_graph_create.argtypes = [C.POINTER(WfaDevBatch), C.POINTER(C.c_void_p), cpp]
         L.ga_wfa_graph_create.restype = C.c_int
         L.ga_wfa_graph_launch.argtypes = [C.c_void_p, C.c_void_p, cpp]

@@ -54,7 +54,9 @@ struct WfaDevPlan {
     int32_t* hist;           // kWfaClasses * kPlanKeys
     int32_t* cls;            // 2 * kWfaClasses: {first, count} per class
     int32_t* order;          // n
+    int32_t* next;           // kWfaClasses claim counters, zeroed by the plan
 };
+constexpr int kPlanSmallMax = 65536;         // single-workgroup plan up to this many pairs
 hipError_t wfa_plan_launch(const WfaDevPlan& P, hipStream_t st);
 
 size_t wfa_lds_bytes(const WfaQueueLaunch& L);     // dynamic LDS of one workgroup

@@ -320,7 +320,21 @@ WfaPlan plan_wfa(int64_t n, const int32_t* smax, const int32_t* plen, const int3
 }
 
 // Wavefronts per workgroup for a class: wide wavefronts split over more waves.
-int wfa_class_waves(int bound) { return bound <= 64 ? 1 : bound <= 256 ? 2 : bound <= 1024 ? 4 : 8; }
+// Wavefronts per workgroup of each size class: the autotuner's choice when
+// one is set (process-wide), else the default (wide wavefronts over more waves).
+std::atomic<int> g_tuned_waves[4] = {{0}, {0}, {0}, {0}};
+int wfa_class_of(int bound)
+{
+    int c = 0;
+    while (c < 3 && bound > kWfaClassBound[c]) ++c;
+    return c;
+}
+int wfa_class_waves(int bound)
+{
+    const int t = g_tuned_waves[wfa_class_of(bound)].load(std::memory_order_relaxed);
+    if (t == 1 || t == 2 || t == 4 || t == 8) return t;
+    return bound <= 64 ? 1 : bound <= 256 ? 2 : bound <= 1024 ? 4 : 8;
+}
 constexpr size_t kWfaMaxStagedLds = 64 * 1024;
 
 const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, bool trace,
@@ -414,6 +428,7 @@ const char* setup_wfa_dev_classes(const ga::WfaQueueLaunch& base, int64_t n, int
     C.plan.smax = base.smax; C.plan.plen = base.plen; C.plan.tlen = base.tlen;
     C.plan.n = int32_t(n);
     C.plan.order = C.order;
+    C.plan.next = C.next;
     for (int c = 0; c < ga::kWfaClasses; ++c) {
         if (c > 0 && smax_max <= kWfaClassBound[c - 1]) break;        // no pair can fall in class c
         ga::WfaQueueLaunch L = base;
@@ -1587,5 +1602,102 @@ int ga_wfa_routed_device(const ga_wfa_dev_batch* b, const float* params, float q
 }
 
 void ga_set_thread_workspace_budget(int64_t bytes) { t_workspace_budget = bytes > 0 ? bytes : 0; }
+
+// ------------------------------------------------------------------ autotuner
+void ga_wfa_set_tuning(const ga_wfa_tuning* t)
+{
+    for (int c = 0; c < 4; ++c) g_tuned_waves[c].store(t ? t->waves[c] : 0, std::memory_order_relaxed);
+}
+
+void ga_wfa_get_tuning(ga_wfa_tuning* t)
+{
+    if (!t) return;
+    for (int c = 0; c < 4; ++c) t->waves[c] = wfa_class_waves(c == 0 ? 64 : c == 1 ? 256 : c == 2 ? 1024 : 2048);
+    for (int c = 0; c < 4; ++c) t->class_ms[c] = -1.0f;
+}
+
+int ga_wfa_autotune(const ga_wfa_dev_batch* b, int reps, ga_wfa_tuning* out, void* stream, const char** error)
+{
+    if (error) *error = nullptr;
+    if (const char* e = check_dev_batch(b)) { if (error) *error = e; return GA_INVALID_ARGUMENT; }
+    if (!out) { if (error) *error = "null output"; return GA_INVALID_ARGUMENT; }
+    if (!b->host_smax || !b->host_pattern_len || !b->host_text_len) {
+        if (error) *error = "autotuning needs the host copies of smax and the lengths";
+        return GA_INVALID_ARGUMENT;
+    }
+    if (b->n_pairs <= 0) { if (error) *error = "empty batch"; return GA_INVALID_ARGUMENT; }
+    std::call_once(g_pool_once, init_pool);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    reps = std::max(1, std::min(reps, 16));
+    const WfaPlan plan = plan_wfa(b->n_pairs, b->host_smax, b->host_pattern_len, b->host_text_len);
+    ga_wfa_tuning t{};
+    ga_wfa_get_tuning(&t);
+    ga::WfaQueueLaunch base{};
+    base.pat = b->pattern; base.poff = b->pattern_off; base.plen = b->pattern_len;
+    base.txt = b->text; base.toff = b->text_off; base.tlen = b->text_len;
+    base.smax = b->smax; base.want = b->want_cigar;
+    base.score = b->score; base.cigar = b->cigar; base.cig_off = b->cigar_off; base.cig_len = b->cigar_len;
+    base.wide = b->max_len > 32766;
+    const char* err = nullptr;
+    int32_t* d_order = nullptr;
+    int32_t* d_next = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void**>(&d_order), plan.order.size() * 4 + 64, st) != hipSuccess ||
+        hipMallocAsync(reinterpret_cast<void**>(&d_next), 64, st) != hipSuccess ||
+        hipMemcpyAsync(d_order, plan.order.data(), plan.order.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        err = "autotune set-up";
+    // each class on its own: time every candidate, keep the fastest
+    for (size_t c = 0; c < plan.bound.size() && !err; ++c) {
+        const int cls = wfa_class_of(plan.bound[c]);
+        float best = -1.0f;
+        int best_w = t.waves[cls];
+        for (int w : {1, 2, 4, 8}) {
+            ga::WfaQueueLaunch L = base;
+            L.order = d_order + plan.first[c];
+            L.n = int32_t(plan.count[c]);
+            L.next = d_next;
+            L.smax_cls = plan.bound[c];
+            L.max_plen = plan.max_plen[c];
+            L.max_tlen = plan.max_tlen[c];
+            L.waves = w;
+            L.staged = ga::wfa_lds_bytes(L) <= kWfaMaxStagedLds;
+            int groups = ga::wfa_queue_groups(L);
+            const int64_t per_group = int64_t(ga::wfa_queue_ws_bytes(L.smax_cls, 1, L.wide));
+            const bool trace = b->cigar != nullptr;
+            if (trace) groups = int(std::max<int64_t>(1, std::min<int64_t>(groups, wfa_stack_budget() / per_group)));
+            L.n_groups = groups;
+            L.ws = nullptr;
+            if (trace && hipMallocAsync(&L.ws, size_t(per_group) * size_t(groups), st) != hipSuccess) {
+                err = "hipMalloc(wavefront stacks)";
+                break;
+            }
+            float tot = 0.0f;
+            for (int r = 0; r <= reps && !err; ++r) {         // first run warms up
+                if (hipEventRecord(e0, st) != hipSuccess || ga::wfa_queue_launch(L, st) != hipSuccess ||
+                    hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                    err = "autotune launch";
+                    break;
+                }
+                float ms = 0.0f;
+                if (r > 0 && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) tot += ms;
+            }
+            if (L.ws) (void)hipFreeAsync(L.ws, st);
+            if (err) break;
+            const float avg = tot / float(reps);
+            if (best < 0.0f || avg < best) { best = avg; best_w = w; }
+        }
+        t.waves[cls] = best_w;
+        t.class_ms[cls] = best;
+    }
+    if (d_order) (void)hipFreeAsync(d_order, st);
+    if (d_next) (void)hipFreeAsync(d_next, st);
+    (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (err) { if (error) *error = err; return GA_DEVICE_ERROR; }
+    *out = t;
+    return GA_OK;
+}
 
 }  // extern "C"

@@ -432,6 +432,50 @@ __global__ __launch_bounds__(1024) void plan_scan_kernel(WfaDevPlan P)
     }
 }
 
+// The whole plan in one workgroup, histogram in LDS (batches up to
+// kPlanSmallMax pairs): one launch instead of four.  Also zeroes the classes'
+// claim counters.
+__global__ __launch_bounds__(1024) void plan_small_kernel(WfaDevPlan P)
+{
+    __shared__ int hist[kWfaClasses * kPlanKeys];
+    __shared__ int part[1024];
+    constexpr int kPer = kWfaClasses * kPlanKeys / 1024;
+    const int t = threadIdx.x;
+    for (int b = t; b < kWfaClasses * kPlanKeys; b += 1024) hist[b] = 0;
+    if (t < kWfaClasses) P.next[t] = 0;
+    __syncthreads();
+    for (int i = t; i < P.n; i += 1024)
+        atomicAdd(&hist[plan_class(P.smax[i]) * kPlanKeys + plan_key(P.plen[i], P.tlen[i])], 1);
+    __syncthreads();
+    int v[kPer];
+    int sum = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) { v[q] = hist[t * kPer + q]; sum += v[q]; }
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int add = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) { hist[t * kPer + q] = run; run += v[q]; }
+    __syncthreads();
+    if (t < kWfaClasses) {
+        const int first = hist[t * kPlanKeys];
+        const int end = t + 1 < kWfaClasses ? hist[(t + 1) * kPlanKeys] : part[1023];
+        P.cls[2 * t] = first;
+        P.cls[2 * t + 1] = end - first;
+    }
+    __syncthreads();
+    for (int i = t; i < P.n; i += 1024) {
+        const int pos = atomicAdd(&hist[plan_class(P.smax[i]) * kPlanKeys + plan_key(P.plen[i], P.tlen[i])], 1);
+        P.order[pos] = i;
+    }
+}
+
 __global__ __launch_bounds__(256) void plan_scatter_kernel(WfaDevPlan P)
 {
     const int i = int(blockIdx.x * 256 + threadIdx.x);
@@ -553,17 +597,25 @@ hipError_t wfa_queue_launch(const WfaQueueLaunch& L, hipStream_t st)
 {
     if (L.n <= 0 || L.n_groups <= 0) return hipSuccess;
     // the claim counter is zeroed by a kernel, not a memset node, so this
-    // submission can be captured into a graph and replayed
-    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, st, L.next, 1);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    // submission can be captured into a graph and replayed (device-planned
+    // classes: the plan kernels zero it)
+    if (!L.dev_class) {
+        hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, st, L.next, 1);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     return dispatch(L, wfa_lds_bytes(L), st, nullptr);
 }
 
 hipError_t wfa_plan_launch(const WfaDevPlan& P, hipStream_t st)
 {
     if (P.n <= 0) return hipSuccess;
+    if (P.n <= kPlanSmallMax) {
+        hipLaunchKernelGGL(plan_small_kernel, dim3(1), dim3(1024), 0, st, P);
+        return hipGetLastError();
+    }
     const unsigned nb = unsigned((P.n + 255) / 256);
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, st, P.next, kWfaClasses);
     hipLaunchKernelGGL(zero_i32_kernel, dim3(16), dim3(256), 0, st, P.hist, kWfaClasses * kPlanKeys);
     hipLaunchKernelGGL(plan_hist_kernel, dim3(nb), dim3(256), 0, st, P);
     hipLaunchKernelGGL(plan_scan_kernel, dim3(1), dim3(1024), 0, st, P);

@@ -114,6 +114,36 @@ class WfaDeviceBatch:
         return sc, cig
 
 
+def wfa_autotune(batch: "WfaDeviceBatch", reps: int = 2, apply: bool = True):
+    """Time waves-per-workgroup candidates per WFA size class on `batch` (needs
+    classes=True) and, with apply, make the fastest current.  Returns
+    (waves per class, ms per class)."""
+    t = N.WfaTuning()
+    err = C.c_char_p()
+    _check(N.lib().ga_wfa_autotune(C.byref(batch.b), int(reps), C.byref(t), C.c_void_p(_stream_handle(None)),
+                                   C.byref(err)), err, "ga_wfa_autotune")
+    if apply:
+        N.lib().ga_wfa_set_tuning(C.byref(t))
+    return list(t.waves), list(t.class_ms)
+
+
+def wfa_set_tuning(waves=None):
+    """Waves per workgroup for the four size classes (None: defaults)."""
+    if waves is None:
+        N.lib().ga_wfa_set_tuning(None)
+        return
+    t = N.WfaTuning()
+    for c in range(4):
+        t.waves[c] = int(waves[c])
+    N.lib().ga_wfa_set_tuning(C.byref(t))
+
+
+def wfa_get_tuning():
+    t = N.WfaTuning()
+    N.lib().ga_wfa_get_tuning(C.byref(t))
+    return list(t.waves)
+
+
 class WfaGraph:
     def __init__(self, batch: WfaDeviceBatch):
         self.batch = batch                      # keeps the buffers alive

@@ -199,6 +199,20 @@ int ga_wfa_device(const ga_wfa_dev_batch* b, void* stream, const char** error);
  * baked in) into a HIP graph; ga_wfa_graph_launch replays it on a stream, so a
  * caller that refills the same buffers resubmits with one call.  Workspace is
  * held by the graph until ga_wfa_graph_destroy. */
+/* Autotuner (EXTENSION): wavefronts per workgroup (1, 2, 4 or 8) for each of
+ * the four WFA size classes (smax <= 64, <= 256, <= 1024, above).
+ * ga_wfa_autotune times every candidate on the pairs of batch `b` (which needs
+ * the host copies; its outputs are written, and equal for every candidate)
+ * and returns the fastest per class with its time; ga_wfa_set_tuning makes a
+ * tuning current for the process (NULL restores the defaults). */
+typedef struct ga_wfa_tuning {
+    int32_t waves[4];
+    float   class_ms[4];     /* autotune: time of the chosen candidate, -1 when the class was empty */
+} ga_wfa_tuning;
+int  ga_wfa_autotune(const ga_wfa_dev_batch* b, int reps, ga_wfa_tuning* out, void* stream, const char** error);
+void ga_wfa_set_tuning(const ga_wfa_tuning* t);
+void ga_wfa_get_tuning(ga_wfa_tuning* t);
+
 typedef struct ga_wfa_graph ga_wfa_graph;
 int  ga_wfa_graph_create(const ga_wfa_dev_batch* b, ga_wfa_graph** out, const char** error);
 int  ga_wfa_graph_launch(ga_wfa_graph* g, void* stream, const char** error);

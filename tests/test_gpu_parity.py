@@ -380,3 +380,58 @@ def test_wfa_graph_replays_match_eager(gpu, oracle):
         assert [int(x) for x in sc] == [int(x) for x in esc], rep
         assert cg == ecg, rep
     g.close()
+
+
+def test_wfa_outputs_invariant_across_tunings(gpu):
+    """Every waves-per-workgroup choice of every size class (and the
+    autotuner's pick) gives the default configuration's results exactly."""
+    from genoaligner_amd import workloads as W
+    from genoaligner_amd.device import WfaDeviceBatch, wfa_autotune, wfa_get_tuning, wfa_set_tuning
+    ps = W.wfa_mixed(n=700, lo=40, hi=4000, seed=515)
+    want = (np.random.default_rng(3).random(ps.n) < 0.7).astype(np.uint8)
+    wfa_set_tuning(None)
+    ref = WfaDeviceBatch(ps, trace=True, want=want)
+    ref.run()
+    rsc, rcg = ref.results()
+    try:
+        for w in (1, 2, 4, 8):
+            wfa_set_tuning([w] * 4)
+            assert wfa_get_tuning() == [w] * 4
+            for classes in (True, False):
+                b = WfaDeviceBatch(ps, trace=True, want=want, classes=classes)
+                b.run()
+                sc, cg = b.results()
+                assert [int(x) for x in sc] == [int(x) for x in rsc], (w, classes)
+                assert cg == rcg, (w, classes)
+        wfa_set_tuning(None)
+        tb = WfaDeviceBatch(ps, trace=True, want=want)
+        waves, ms = wfa_autotune(tb, reps=1)
+        assert all(w in (1, 2, 4, 8) for w in waves)
+        assert wfa_get_tuning() == waves
+        b = WfaDeviceBatch(ps, trace=True, want=want)
+        b.run()
+        sc, cg = b.results()
+        assert [int(x) for x in sc] == [int(x) for x in rsc] and cg == rcg
+        sc_t, cg_t = tb.results()                 # the tuner's own runs wrote the same results
+        assert [int(x) for x in sc_t] == [int(x) for x in rsc] and cg_t == rcg
+    finally:
+        wfa_set_tuning(None)
+
+
+def test_wfa_device_plan_large_batch(gpu, oracle):
+    """Above the single-workgroup plan size the device plan runs as four
+    kernels; its results equal the host-planned run and the oracle."""
+    from genoaligner_amd import workloads as W
+    from genoaligner_amd.device import WfaDeviceBatch
+    ps = W.wfa_mixed(n=70000, lo=20, hi=700, seed=616)
+    host = WfaDeviceBatch(ps, trace=True, classes=True)
+    host.run()
+    dev = WfaDeviceBatch(ps, trace=True, classes=False)
+    dev.run()
+    hs, hc = host.results()
+    ds, dc = dev.results()
+    assert [int(x) for x in hs] == [int(x) for x in ds]
+    assert hc == dc
+    for i in range(0, ps.n, 997):
+        sc, cg = oracle.wfa(ps.a_seq(i), ps.b_seq(i), int(ps.smax[i]))
+        assert (int(ds[i]), dc[i].decode()) == (sc, cg)
