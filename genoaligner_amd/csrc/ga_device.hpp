@@ -95,7 +95,12 @@ hipError_t mlp_backward(const float* params, const float* X, const float* H1, co
                         float* dH1, hipStream_t st);
 hipError_t mlp_sgd(float* params, const float* grads, float* mom, float lr, float mu, hipStream_t st);
 hipError_t mlp_route(const float* logits, int64_t N, float q, int32_t* bucket, hipStream_t st);
-hipError_t mark_unresolved_launch(const int32_t* ids, int64_t k, int32_t* score, int32_t* cig_len, hipStream_t st);
+hipError_t route_first_pass_launch(const int32_t* bucket, const int32_t* lb, const int32_t* smax, const int32_t* plen,
+                                   const int32_t* tlen, int n, int32_t* sm1, int32_t* score, int32_t* cig_len,
+                                   unsigned long long* stats, hipStream_t st);
+hipError_t route_second_pass_launch(const int32_t* sm1, const int32_t* smax, const int32_t* score, int n, int32_t* sm2,
+                                    unsigned long long* stats, hipStream_t st);
+hipError_t zero_u64_launch(unsigned long long* p, int n, hipStream_t st);
 
 struct SwLaunch {
     const uint8_t* txt; const int64_t* toff; const int32_t* tlen;   // rows

@@ -1516,87 +1516,85 @@ int ga_wfa_routed_device(const ga_wfa_dev_batch* b, const float* params, float q
     if (stats) *stats = ga_route_stats{};
     if (const char* e = check_dev_batch(b)) { if (error) *error = e; return GA_INVALID_ARGUMENT; }
     if (!params) { if (error) *error = "null MLP parameters"; return GA_INVALID_ARGUMENT; }
-    if (!b->host_smax || !b->host_pattern_len || !b->host_text_len) {
-        if (error) *error = "routing needs the host copies of smax and the lengths";
-        return GA_INVALID_ARGUMENT;
-    }
     const int64_t n = b->n_pairs;
     if (n <= 0) return GA_OK;
     std::call_once(g_pool_once, init_pool);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    DevArena D;
-    D.st = st;
-    float* feat = D.alloc<float>(size_t(n) * ga::kSketchDim);
-    int32_t* lb = D.alloc<int32_t>(size_t(n));
-    float* h1 = D.alloc<float>(size_t(n) * ga::kHidden);
-    float* h2 = D.alloc<float>(size_t(n) * ga::kHidden);
-    float* logits = D.alloc<float>(size_t(n) * ga::kBuckets);
-    int32_t* bucket = D.alloc<int32_t>(size_t(n));
-    const char* err = D.ok ? nullptr : "hipMalloc(router buffers)";
+    std::vector<void*> held;
+    auto alloc = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMallocAsync(&p, bytes, st) != hipSuccess) return nullptr;
+        held.push_back(p);
+        return p;
+    };
+    auto release = [&](size_t from) {
+        for (size_t k = from; k < held.size(); ++k) (void)hipFreeAsync(held[k], st);
+        held.resize(from);
+    };
+    const char* err = nullptr;
+    float* feat = static_cast<float*>(alloc(size_t(n) * ga::kSketchDim * 4));
+    int32_t* lb = static_cast<int32_t*>(alloc(size_t(n) * 4));
+    float* h1 = static_cast<float*>(alloc(size_t(n) * ga::kHidden * 4));
+    float* h2 = static_cast<float*>(alloc(size_t(n) * ga::kHidden * 4));
+    float* logits = static_cast<float*>(alloc(size_t(n) * ga::kBuckets * 4));
+    int32_t* bucket = static_cast<int32_t*>(alloc(size_t(n) * 4));
+    int32_t* sm1 = static_cast<int32_t*>(alloc(size_t(n) * 4));
+    int32_t* sm2 = static_cast<int32_t*>(alloc(size_t(n) * 4));
+    auto* dstats = static_cast<unsigned long long*>(alloc(64));
+    if (!feat || !lb || !h1 || !h2 || !logits || !bucket || !sm1 || !sm2 || !dstats) err = "hipMalloc(router buffers)";
     ga::SketchLaunch S{b->pattern, b->pattern_off, b->pattern_len, b->text, b->text_off, b->text_len, n, feat, lb};
-    if (!err && (ga::sketch_launch(S, st) != hipSuccess || ga::mlp_forward(params, feat, n, h1, h2, logits, st) != hipSuccess ||
-                 ga::mlp_route(logits, n, q, bucket, st) != hipSuccess))
+    if (!err && (ga::zero_u64_launch(dstats, 4, st) != hipSuccess || ga::sketch_launch(S, st) != hipSuccess ||
+                 ga::mlp_forward(params, feat, n, h1, h2, logits, st) != hipSuccess ||
+                 ga::mlp_route(logits, n, q, bucket, st) != hipSuccess ||
+                 ga::route_first_pass_launch(bucket, lb, b->smax, b->pattern_len, b->text_len, int(n), sm1, b->score,
+                                             b->cigar ? b->cigar_len : nullptr, dstats, st) != hipSuccess))
         err = "router kernel launch";
-    std::vector<int32_t> hb(static_cast<size_t>(n)), hlb(static_cast<size_t>(n));
-    if (!err && (hipMemcpyAsync(hb.data(), bucket, size_t(n) * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                 hipMemcpyAsync(hlb.data(), lb, size_t(n) * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                 hipStreamSynchronize(st) != hipSuccess))
-        err = "hipMemcpy(router decisions)";
-    // first pass: each pair under the bound of its predicted bucket; pairs whose
-    // exact lower bound exceeds their smax are unresolved without an alignment
-    std::vector<int32_t> sm1(static_cast<size_t>(n)), shortcut, first;
-    ga_route_stats rs{};
-    if (!err) {
-        for (int64_t i = 0; i < n; ++i) {
-            const int32_t smax = b->host_smax[i];
-            // (a pair with an empty side scores the other length whatever its smax)
-            const bool degenerate = b->host_pattern_len[i] <= 0 || b->host_text_len[i] <= 0;
-            if (!degenerate && hlb[size_t(i)] > smax) { shortcut.push_back(int32_t(i)); sm1[size_t(i)] = 0; continue; }
-            sm1[size_t(i)] = std::min(smax, kBucketBound[std::clamp(hb[size_t(i)], 0, ga::kBuckets - 1)]);
-            sm1[size_t(i)] = std::max(sm1[size_t(i)], std::min(smax, hlb[size_t(i)]));   // never below the bound
-            first.push_back(int32_t(i));
-            if (sm1[size_t(i)] < smax) ++rs.routed; else ++rs.plain;
-        }
-        rs.shortcut = int64_t(shortcut.size());
-    }
-    int32_t* d_sm1 = err ? nullptr : D.upload(sm1);
-    int32_t* d_short = (err || shortcut.empty()) ? nullptr : D.upload(shortcut);
-    if (!err && !D.ok) err = "hipMalloc/hipMemcpy (routing)";
-    if (!err && d_short && ga::mark_unresolved_launch(d_short, int64_t(shortcut.size()), b->score,
-                                                      b->cigar ? b->cigar_len : nullptr, st) != hipSuccess)
-        err = "mark kernel launch";
     ga::WfaQueueLaunch L{};
     L.pat = b->pattern; L.poff = b->pattern_off; L.plen = b->pattern_len;
     L.txt = b->text; L.toff = b->text_off; L.tlen = b->text_len;
     L.want = b->want_cigar;
     L.score = b->score; L.cigar = b->cigar; L.cig_off = b->cigar_off; L.cig_len = b->cigar_len;
     L.wide = b->max_len > 32766;
-    if (!err) {
-        L.smax = d_sm1;
-        const WfaPlan p1 = plan_wfa(n, sm1.data(), b->host_pattern_len, b->host_text_len, &first);
-        err = enqueue_wfa(L, p1, b->cigar != nullptr, st, nullptr);
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> ev(ga::kWfaClasses, nullptr);
+    for (int c = 1; !err && c < ga::kWfaClasses; ++c) {
+        hipStream_t cs = acquire_stream();
+        if (!cs) err = "hipStreamCreate";
+        else side.push_back(cs);
     }
-    // second pass: pairs not resolved under a bound below their own smax
-    std::vector<int32_t> sc(static_cast<size_t>(n));
-    if (!err && (hipMemcpyAsync(sc.data(), b->score, size_t(n) * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                 hipStreamSynchronize(st) != hipSuccess))
-        err = "hipMemcpy(first-pass scores)";
-    if (!err) {
-        std::vector<int32_t> again;
-        for (int32_t i : first)
-            if (sc[size_t(i)] < 0 && sm1[size_t(i)] < b->host_smax[i]) again.push_back(i);
-        rs.rerun = int64_t(again.size());
-        if (!again.empty()) {
-            L.smax = b->smax;
-            const WfaPlan p2 = plan_wfa(n, b->host_smax, b->host_pattern_len, b->host_text_len, &again);
-            err = enqueue_wfa(L, p2, b->cigar != nullptr, st, nullptr);
+    for (auto& e : ev)
+        if (!err && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) err = "hipEventCreate";
+    // two device-planned passes, no host synchronisation: the routed bounds,
+    // then the pairs not resolved under them with their own smax
+    for (int pass = 0; pass < 2 && !err; ++pass) {
+        if (pass == 1 && ga::route_second_pass_launch(sm1, b->smax, b->score, int(n), sm2, dstats, st) != hipSuccess) {
+            err = "router kernel launch";
+            break;
         }
+        const size_t mark = held.size();
+        L.smax = pass == 0 ? sm1 : sm2;
+        WfaDevClasses C;
+        err = setup_wfa_dev_classes(L, n, b->smax_max, b->max_len, b->cigar != nullptr, C, alloc);
+        if (!err) err = enqueue_wfa_dev(C, st, side, ev);
+        release(mark);                                    // stream-ordered: reused by the next pass
     }
-    D.release();
-    if (stats) *stats = rs;
+    unsigned long long hs[4] = {0, 0, 0, 0};
+    if (!err && stats && (hipMemcpyAsync(hs, dstats, sizeof(hs), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                          hipStreamSynchronize(st) != hipSuccess))
+        err = "hipMemcpy(router stats)";
+    release(0);
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t cs : side) release_stream(cs);
     if (err) {
         if (error) *error = err;
         return GA_DEVICE_ERROR;
+    }
+    if (stats) {
+        stats->shortcut = int64_t(hs[0]);
+        stats->routed = int64_t(hs[1]);
+        stats->plain = int64_t(hs[2]);
+        stats->rerun = int64_t(hs[3]);
     }
     return GA_OK;
 }

@@ -23,13 +23,43 @@ __device__ __forceinline__ int code4(uint32_t b)
     return b == 'A' ? 0 : b == 'C' ? 1 : b == 'G' ? 2 : b == 'T' ? 3 : -1;
 }
 
+// 8 bytes starting at any byte address (two aligned words; sequences carry
+// 128 readable pad bytes on both sides).
+__device__ __forceinline__ uint64_t load8u(const uint8_t* p)
+{
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    const uint64_t* w = reinterpret_cast<const uint64_t*>(a & ~uintptr_t(7));
+    const unsigned sh = unsigned(a & 7u) * 8u;
+    const uint64_t lo = w[0];
+    const uint64_t hi = w[1];
+    return sh ? ((lo >> sh) | (hi << (64u - sh))) : lo;
+}
+
 // 4-mer counts of one sequence into an LDS histogram (windows holding a byte
-// other than A/C/G/T are skipped).
+// other than A/C/G/T are skipped).  Lane l rolls a 4-mer code over windows
+// [l*chunk, (l+1)*chunk), reading each base once, 8 bytes per load.
 __device__ void count4(const uint8_t* s, int len, int* hist, int lane)
 {
-    for (int i = lane; i + 3 < len; i += 64) {
-        const int a = code4(s[i]), b = code4(s[i + 1]), c = code4(s[i + 2]), d = code4(s[i + 3]);
-        if ((a | b | c | d) >= 0) atomicAdd(&hist[(a << 6) | (b << 4) | (c << 2) | d], 1);
+    const int nwin = len - 3;
+    if (nwin <= 0) return;
+    const int chunk = (nwin + 63) / 64;
+    const int w0 = lane * chunk;
+    const int w1 = w0 + chunk < nwin ? w0 + chunk : nwin;
+    if (w0 >= w1) return;
+    // bases w0 .. w1 + 2 feed windows w0 .. w1 - 1
+    int code = 0, valid = 0;                 // rolling 8-bit code, trailing run of ACGT bases
+    for (int b = w0; b < w1 + 3; b += 8) {
+        const uint64_t x = load8u(s + b);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int pos = b + q;
+            if (pos < w1 + 3) {
+                const int c = code4(uint32_t((x >> (8 * q)) & 0xffu));
+                code = ((code << 2) | (c & 3)) & 0xff;
+                valid = c >= 0 ? valid + 1 : 0;
+                if (pos >= w0 + 3 && valid >= 4) atomicAdd(&hist[code], 1);
+            }
+        }
     }
 }
 
@@ -200,14 +230,65 @@ __global__ __launch_bounds__(256) void route_buckets(const float* logits, int64_
     bucket[r] = b;
 }
 
-// score = -1, no CIGAR, for pairs decided without an alignment
-__global__ __launch_bounds__(256) void mark_unresolved(const int32_t* ids, int64_t k, int32_t* score, int32_t* cig_len)
+__global__ void zero_u64(unsigned long long* p, int n)
 {
-    const int64_t x = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (x >= k) return;
-    const int32_t pid = ids[x];
-    score[pid] = -1;
-    if (cig_len) cig_len[pid] = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
+__constant__ int32_t kBound[kBuckets] = {0, 1, 2, 4, 8, 16, 32, 64, 128, 256, 384, 512, 768, 1024, 2048, 0x7fffffff};
+
+// Adds the number of lanes with `pred` to *ctr: one atomic per wave.
+__device__ __forceinline__ void wave_count(bool pred, unsigned long long* ctr)
+{
+    const unsigned long long m = __ballot(pred);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(ctr, (unsigned long long)__popcll(m));
+}
+
+// First pass of a routed batch: each pair's bound = its predicted bucket's
+// bound (never below the exact lower bound, never above its own smax); a pair
+// whose exact lower bound exceeds its smax is unresolved at once (score -1, no
+// CIGAR) and left out (sm1 = -1).  stats[0..2] += shortcut, routed, plain.
+__global__ __launch_bounds__(256) void route_first_pass(const int32_t* bucket, const int32_t* lb, const int32_t* smax,
+                                                       const int32_t* plen, const int32_t* tlen, int n, int32_t* sm1,
+                                                       int32_t* score, int32_t* cig_len, unsigned long long* stats)
+{
+    const int i = int(blockIdx.x * 256 + threadIdx.x);
+    int kind = -1;                                          // 0 shortcut, 1 routed, 2 plain
+    if (i < n) {
+        const int s = smax[i];
+        const bool degenerate = plen[i] <= 0 || tlen[i] <= 0;   // scores the other length whatever its smax
+        if (!degenerate && lb[i] > s) {
+            sm1[i] = -1;
+            score[i] = -1;
+            if (cig_len) cig_len[i] = 0;
+            kind = 0;
+        } else {
+            int b = bucket[i];
+            b = b < 0 ? 0 : (b >= kBuckets ? kBuckets - 1 : b);
+            int v = kBound[b] < s ? kBound[b] : s;
+            const int l = lb[i] < s ? lb[i] : s;
+            v = v > l ? v : l;
+            sm1[i] = v;
+            kind = v < s ? 1 : 2;
+        }
+    }
+    wave_count(kind == 0, &stats[0]);
+    wave_count(kind == 1, &stats[1]);
+    wave_count(kind == 2, &stats[2]);
+}
+
+// Second pass: the routed pairs not resolved under their first bound, with
+// their own smax; everything else is left out.  stats[3] += re-runs.
+__global__ __launch_bounds__(256) void route_second_pass(const int32_t* sm1, const int32_t* smax, const int32_t* score,
+                                                        int n, int32_t* sm2, unsigned long long* stats)
+{
+    const int i = int(blockIdx.x * 256 + threadIdx.x);
+    bool again = false;
+    if (i < n) {
+        again = sm1[i] >= 0 && sm1[i] < smax[i] && score[i] < 0;
+        sm2[i] = again ? smax[i] : -1;
+    }
+    wave_count(again, &stats[3]);
 }
 
 inline unsigned blocks(int64_t n, int per) { return unsigned((n + per - 1) / per); }
@@ -271,10 +352,27 @@ hipError_t mlp_route(const float* logits, int64_t N, float q, int32_t* bucket, h
     return hipGetLastError();
 }
 
-hipError_t mark_unresolved_launch(const int32_t* ids, int64_t k, int32_t* score, int32_t* cig_len, hipStream_t st)
+hipError_t route_first_pass_launch(const int32_t* bucket, const int32_t* lb, const int32_t* smax, const int32_t* plen,
+                                   const int32_t* tlen, int n, int32_t* sm1, int32_t* score, int32_t* cig_len,
+                                   unsigned long long* stats, hipStream_t st)
 {
-    if (k <= 0) return hipSuccess;
-    hipLaunchKernelGGL(mark_unresolved, dim3(blocks(k, 256)), dim3(256), 0, st, ids, k, score, cig_len);
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(route_first_pass, dim3(blocks(n, 256)), dim3(256), 0, st, bucket, lb, smax, plen, tlen, n,
+                       sm1, score, cig_len, stats);
+    return hipGetLastError();
+}
+
+hipError_t route_second_pass_launch(const int32_t* sm1, const int32_t* smax, const int32_t* score, int n, int32_t* sm2,
+                                    unsigned long long* stats, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(route_second_pass, dim3(blocks(n, 256)), dim3(256), 0, st, sm1, smax, score, n, sm2, stats);
+    return hipGetLastError();
+}
+
+hipError_t zero_u64_launch(unsigned long long* p, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, st, p, n);
     return hipGetLastError();
 }
 

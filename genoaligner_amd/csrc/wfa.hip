@@ -279,6 +279,15 @@ void wfa_kernel(WfaQueueLaunch L)
             }
             continue;
         }
+        // the end diagonal n - m is first reached at score |n - m|: a pair whose
+        // length difference exceeds its bound is unresolved without a pass
+        if ((n > m ? n - m : m - n) > smax) {
+            if (tid == 0) {
+                L.score[pid] = -1;
+                if (TRACE && L.cig_len) L.cig_len[pid] = 0;
+            }
+            continue;
+        }
         for (int x = tid; x < W; x += NT) {
             A[x] = Off(OffNull<Off>::v);
             B[x] = Off(OffNull<Off>::v);
@@ -382,7 +391,8 @@ __global__ void zero_i32_kernel(int32_t* p, int n)
 // Device-side planning: size class by smax, claim order by length (longest
 // first) as a counting sort over kPlanKeys length bins of 32 bytes per class:
 // histogram, one-workgroup exclusive scan, scatter.  Order inside a bin is
-// whatever the atomics give; results do not depend on it.
+// whatever the atomics give; results do not depend on it.  A pair whose smax
+// is negative is left out of the plan (the router's passes use this).
 __device__ __forceinline__ int plan_class(int smax)
 {
     return smax <= kWfaClassBound0 ? 0 : smax <= kWfaClassBound1 ? 1 : smax <= kWfaClassBound2 ? 2 : 3;
@@ -396,7 +406,7 @@ __device__ __forceinline__ int plan_key(int m, int n)
 __global__ __launch_bounds__(256) void plan_hist_kernel(WfaDevPlan P)
 {
     const int i = int(blockIdx.x * 256 + threadIdx.x);
-    if (i >= P.n) return;
+    if (i >= P.n || P.smax[i] < 0) return;             // smax < 0: not in this plan
     atomicAdd(&P.hist[plan_class(P.smax[i]) * kPlanKeys + plan_key(P.plen[i], P.tlen[i])], 1);
 }
 
@@ -445,7 +455,7 @@ __global__ __launch_bounds__(1024) void plan_small_kernel(WfaDevPlan P)
     if (t < kWfaClasses) P.next[t] = 0;
     __syncthreads();
     for (int i = t; i < P.n; i += 1024)
-        atomicAdd(&hist[plan_class(P.smax[i]) * kPlanKeys + plan_key(P.plen[i], P.tlen[i])], 1);
+        if (P.smax[i] >= 0) atomicAdd(&hist[plan_class(P.smax[i]) * kPlanKeys + plan_key(P.plen[i], P.tlen[i])], 1);
     __syncthreads();
     int v[kPer];
     int sum = 0;
@@ -471,6 +481,7 @@ __global__ __launch_bounds__(1024) void plan_small_kernel(WfaDevPlan P)
     }
     __syncthreads();
     for (int i = t; i < P.n; i += 1024) {
+        if (P.smax[i] < 0) continue;
         const int pos = atomicAdd(&hist[plan_class(P.smax[i]) * kPlanKeys + plan_key(P.plen[i], P.tlen[i])], 1);
         P.order[pos] = i;
     }
@@ -479,7 +490,7 @@ __global__ __launch_bounds__(1024) void plan_small_kernel(WfaDevPlan P)
 __global__ __launch_bounds__(256) void plan_scatter_kernel(WfaDevPlan P)
 {
     const int i = int(blockIdx.x * 256 + threadIdx.x);
-    if (i >= P.n) return;
+    if (i >= P.n || P.smax[i] < 0) return;
     const int pos = atomicAdd(&P.hist[plan_class(P.smax[i]) * kPlanKeys + plan_key(P.plen[i], P.tlen[i])], 1);
     P.order[pos] = i;
 }

@@ -258,8 +258,9 @@ typedef struct ga_route_stats {
 
 /* ga_wfa_device with routing: same outputs pair for pair (the router only
  * chooses the first bound and size class; every pair not resolved under a
- * routed bound is re-run with its own smax).  Needs host_smax,
- * host_pattern_len and host_text_len.  Synchronises `stream` twice. */
+ * routed bound is re-run with its own smax).  Sketch, network, both passes and
+ * their plans run on the device with no host synchronisation; with `stats`
+ * non-NULL the call waits for the stream once at the end to report them. */
 int ga_wfa_routed_device(const ga_wfa_dev_batch* b, const float* params, float q, ga_route_stats* stats,
                          void* stream, const char** error);
 
