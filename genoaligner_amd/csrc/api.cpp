@@ -1,6 +1,8 @@
 // api.cpp -- the reference's C++ surface (include/genoaligner/api.hpp) over
 // the C ABI.  Marshalling only: every rule lives behind ga_align_batch /
 // ga_align_sw_batch.
+#include <algorithm>
+#include <cstring>
 #include "../../include/genoaligner/api.hpp"
 #include "../../include/genoaligner/genoaligner_c.h"
 
@@ -57,7 +59,10 @@ AlignResult align(const AlignRequest& req)
     return b.results.empty() ? AlignResult{} : b.results[0];
 }
 
-SWBatchResult align_sw_batch(const std::vector<SWRequest>& reqs)
+namespace {
+// One SW batch through the C ABI, with (matrix != null) or without a
+// substitution matrix.
+SWBatchResult sw_call(const std::vector<SWRequest>& reqs, const ga_sw_matrix* matrix)
 {
     SWBatchResult out;
     out.results.resize(reqs.size());
@@ -75,8 +80,10 @@ SWBatchResult align_sw_batch(const std::vector<SWRequest>& reqs)
     std::string arena(size_t(need > 0 ? need : 1), '\0');
     int32_t rc_ok = 0, rc_un = 0;
     const char* err = nullptr;
-    const int rc = ga_align_sw_batch(cr.data(), int64_t(cr.size()), res.data(), &arena[0], need,
-                                     &rc_ok, &rc_un, &err);
+    const int rc = matrix ? ga_align_sw_matrix_batch(cr.data(), int64_t(cr.size()), matrix, res.data(), &arena[0],
+                                                     need, &rc_ok, &rc_un, &err)
+                          : ga_align_sw_batch(cr.data(), int64_t(cr.size()), res.data(), &arena[0], need,
+                                              &rc_ok, &rc_un, &err);
     out.status = status_of<SWBatchResult::Status>(rc);
     out.error = err;
     if (rc != GA_OK) return out;
@@ -95,6 +102,47 @@ SWBatchResult align_sw_batch(const std::vector<SWRequest>& reqs)
     out.resolved_count = rc_ok;
     out.unresolved_count = rc_un;
     return out;
+}
+}  // namespace
+
+SWBatchResult align_sw_batch(const std::vector<SWRequest>& reqs) { return sw_call(reqs, nullptr); }
+
+SWMatrixScoring blosum62(int gap_open, int gap_extend)
+{
+    ga_sw_matrix m;
+    ga_sw_matrix_blosum62(&m, gap_open, gap_extend);
+    SWMatrixScoring s;
+    s.alphabet.assign(m.alphabet, strnlen(m.alphabet, sizeof(m.alphabet)));
+    const size_t k = s.alphabet.size();
+    s.score.assign(k, std::vector<int>(k, 0));
+    for (size_t a = 0; a < k; ++a)
+        for (size_t b = 0; b < k; ++b) s.score[a][b] = m.score[a][b];
+    s.unknown = m.unknown;
+    s.gap_open = m.gap_open;
+    s.gap_extend = m.gap_extend;
+    return s;
+}
+
+SWBatchResult align_sw_matrix_batch(const std::vector<SWRequest>& reqs, const SWMatrixScoring& scheme)
+{
+    if (scheme.alphabet.size() > 31) {              // the C struct holds at most 31 letters
+        SWBatchResult out;
+        out.results.resize(reqs.size());
+        out.status = SWBatchResult::Status::invalid_argument;
+        out.error = "SW matrix: alphabet must hold 1..31 letters";
+        return out;
+    }
+    ga_sw_matrix m;
+    std::memset(&m, 0, sizeof(m));
+    const size_t k = std::min(scheme.alphabet.size(), sizeof(m.alphabet));
+    std::memcpy(m.alphabet, scheme.alphabet.data(), k);
+    for (size_t a = 0; a < k && a < scheme.score.size(); ++a)
+        for (size_t b = 0; b < k && b < scheme.score[a].size(); ++b)
+            m.score[a][b] = int8_t(scheme.score[a][b]);
+    m.unknown = scheme.unknown;
+    m.gap_open = scheme.gap_open;
+    m.gap_extend = scheme.gap_extend;
+    return sw_call(reqs, &m);
 }
 
 SWAlignResult align_sw(const SWRequest& req)

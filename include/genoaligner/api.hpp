@@ -19,7 +19,8 @@
 // Deliberate differences, all strictly more capable:
 //   * no shared-memory size refusals (the reference refuses with_cigar at large
 //     smax and score-only SW at long patterns on shared-memory grounds);
-//   * align_batch_ext: per-request smax up to 4000 and per-request with_cigar.
+//   * align_batch_ext: per-request smax up to 4000 and per-request with_cigar;
+//   * align_sw_matrix_batch: substitution-matrix (protein) Smith-Waterman.
 #ifndef GENOALIGNER_API_HPP
 #define GENOALIGNER_API_HPP
 
@@ -105,6 +106,21 @@ struct SWBatchResult {
 };
 SWBatchResult align_sw_batch(const std::vector<SWRequest>& reqs);
 SWAlignResult align_sw(const SWRequest& req);
+
+// Extension: Smith-Waterman under a substitution matrix (protein).  Same rules
+// as align_sw_batch; the requests' `scoring` is replaced by the scheme below:
+// score[a][b] for text residue alphabet[a] against pattern residue alphabet[b],
+// `unknown` for any pair with a byte outside the alphabet, and a gap of length
+// L costing gap_open + (L - 1) * gap_extend.
+struct SWMatrixScoring {
+    std::string alphabet;                    // 1..31 distinct letters
+    std::vector<std::vector<int>> score;     // alphabet.size() x alphabet.size(), int8 range
+    int unknown    = -1;
+    int gap_open   = 11;
+    int gap_extend = 1;
+};
+SWMatrixScoring blosum62(int gap_open = 11, int gap_extend = 1);
+SWBatchResult align_sw_matrix_batch(const std::vector<SWRequest>& reqs, const SWMatrixScoring& scheme);
 
 const char* backend_name();
 const char* device_name();

@@ -185,3 +185,36 @@ def test_matrix_gap_rule_refused_with_cigar_only():
     r = ga.SWRequest(text="ARND", text_len=4, pattern="ARN", pattern_len=3, with_cigar=True)
     b = ga.align_sw_matrix_batch([r], m)
     assert b.status == ga.Status.invalid_argument and "gap_extend > gap_open" in b.error
+
+
+def test_cpp_surface_compiles_links_and_refuses(tmp_path):
+    """A C++ caller of include/genoaligner/api.hpp builds against the library
+    and sees the reference's refusals (no device work is needed for them)."""
+    from genoaligner_amd import _native as N
+    src = tmp_path / "caller.cpp"
+    src.write_text(r'''
+#include "genoaligner/api.hpp"
+#include <cstdio>
+int main() {
+    genoaligner::AlignRequest r;
+    r.pattern = "ACGT"; r.pattern_len = 4; r.text = "ACGA"; r.text_len = 4; r.smax = 600;
+    auto b = genoaligner::align_batch({r});
+    std::printf("wfa %d %s\n", int(b.status), b.error ? b.error : "-");
+    auto m = genoaligner::blosum62(11, 1);
+    std::printf("b62 %s %d %d %d\n", m.alphabet.c_str(), m.score[17][17], m.score[12][8], m.gap_open);
+    m.alphabet = "";
+    genoaligner::SWRequest q;
+    q.text = "ARND"; q.text_len = 4; q.pattern = "ARN"; q.pattern_len = 3;
+    auto s = genoaligner::align_sw_matrix_batch({q}, m);
+    std::printf("sw %d %s\n", int(s.status), s.error ? s.error : "-");
+    return 0;
+}
+''')
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(N.LIB_PATH)
+    subprocess.check_call(["g++", "-std=c++17", str(src), "-I", os.path.join(ROOT, "include"), "-L", libdir,
+                           "-lgenoaligner_amd", f"-Wl,-rpath,{libdir}", "-o", str(exe)])
+    out = subprocess.check_output([str(exe)], text=True).splitlines()
+    assert out[0] == "wfa 1 smax out of range [0, 511]"
+    assert out[1] == "b62 ARNDCQEGHILKMFPSTWYV 11 -1 11"
+    assert out[2].startswith("sw 1 SW matrix: alphabet")
