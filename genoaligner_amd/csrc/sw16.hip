@@ -330,24 +330,23 @@ void sw16_kernel(Sw16Launch L)
         S.bestH = 0; S.bestT = 0xffffffffu;
         const bool full = (s << 6) + 64 <= nmin;  // every column of the strip exists in both pairs
 
-        uint32_t sv[4], svn[4], svnn[4];
-        uint2 bv[4], bvn[4];
-        // Lane 0 reads the boundary rows (rows past the text repeat row M's
-        // spare entries, values that only reach drained cells); the other lanes
-        // read the zero rows.
+        // Selectors and boundary rows of group t0 are read two groups ahead,
+        // into one of two register sets that alternate between even and odd
+        // groups, so no register copy waits for a load in flight.  Lane 0
+        // reads the boundary rows (rows past the text repeat row M's spare
+        // entries, values that only reach drained cells); the other lanes read
+        // the zero rows.  Guard rows keep every read inside its buffer.
         const uint32_t* sp = sel - lane;
         const int brow_max = M + 4;                // rows M .. M+7 are spare
+        auto brow = [&](int t) { return lane == 0 ? min(t, brow_max) : zrow; };
+        uint32_t svA[4], svB[4];
+        uint2 bvA[4], bvB[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            sv[u] = sp[u]; svn[u] = sp[4 + u];
-            bv[u] = bnd[lane == 0 ? u : zrow + u];
+            svA[u] = sp[u]; svB[u] = sp[4 + u];
+            bvA[u] = bnd[brow(0) + u]; bvB[u] = bnd[brow(4) + u];
         }
-        for (int t0 = 0; t0 < nsteps; t0 += 4) {
-            // prefetch: selectors two groups ahead (HBM/L2), boundary one group
-            // ahead (LDS); guard rows keep every read inside its buffer
-            const int brow = lane == 0 ? min(t0 + 4, brow_max) : zrow;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) { svnn[u] = sp[t0 + 8 + u]; bvn[u] = bnd[brow + u]; }
+        auto group = [&](int t0, const uint32_t (&sv)[4], const uint2 (&bv)[4]) {
             uint32_t acc = 0, hout[4], eout[4];
             if (t0 < t_main && full) {
                 if (!gk) strip_group<TRACE, false, false, false>(S, t0, lane, sv, bv, b2, gk2, ge2, mAp, mBp, acc, hout, eout);
@@ -356,15 +355,25 @@ void sw16_kernel(Sw16Launch L)
                 strip_group<TRACE, true, true, true>(S, t0, lane, sv, bv, b2, gk2, ge2, mAp, mBp, acc, hout, eout);
             }
             if (TRACE) dirs[(int64_t(s) * nq + (t0 >> 2)) * 64 + lane] = acc;
+            // lane 63 hands rows t0-63 .. t0-60 to the next strip; rows before
+            // the first land in spare row M+7, rows past the text in the spare
+            // rows M .. M+6 (lane 0 has read each row before it is written)
             if (lane == 63) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const int r = t0 + u - 63;
-                    if (r >= 0 && r < M) bnd[r] = make_uint2(hout[u], eout[u]);
+                    const unsigned r = unsigned(t0 + u - 63);
+                    bnd[min(r, unsigned(M + 7))] = make_uint2(hout[u], eout[u]);
                 }
             }
+        };
+        for (int t0 = 0; t0 < nsteps; t0 += 8) {
+            group(t0, svA, bvA);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) { sv[u] = svn[u]; svn[u] = svnn[u]; bv[u] = bvn[u]; }
+            for (int u = 0; u < 4; ++u) { svA[u] = sp[t0 + 8 + u]; bvA[u] = bnd[brow(t0 + 8) + u]; }
+            if (t0 + 4 >= nsteps) break;
+            group(t0 + 4, svB, bvB);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) { svB[u] = sp[t0 + 12 + u]; bvB[u] = bnd[brow(t0 + 12) + u]; }
         }
         // fold this strip's first maxima (per half) into the running records
 #pragma unroll
