@@ -330,21 +330,21 @@ void sw16_kernel(Sw16Launch L)
         S.bestH = 0; S.bestT = 0xffffffffu;
         const bool full = (s << 6) + 64 <= nmin;  // every column of the strip exists in both pairs
 
-        // Selectors and boundary rows of group t0 are read two groups ahead,
-        // into one of two register sets that alternate between even and odd
-        // groups, so no register copy waits for a load in flight.  Lane 0
-        // reads the boundary rows (rows past the text repeat row M's spare
-        // entries, values that only reach drained cells); the other lanes read
-        // the zero rows.  Guard rows keep every read inside its buffer.
+        // Selectors and boundary rows of group t0 are read three groups ahead,
+        // into one of three register sets that rotate with the group index, so
+        // no register copy waits for a load in flight.  Lane 0 reads the
+        // boundary rows (rows past the text repeat row M's spare entries,
+        // values that only reach drained cells); the other lanes read the zero
+        // rows.  Guard rows keep every read inside its buffer.
         const uint32_t* sp = sel - lane;
         const int brow_max = M + 4;                // rows M .. M+7 are spare
         auto brow = [&](int t) { return lane == 0 ? min(t, brow_max) : zrow; };
-        uint32_t svA[4], svB[4];
-        uint2 bvA[4], bvB[4];
+        uint32_t svA[4], svB[4], svC[4];
+        uint2 bvA[4], bvB[4], bvC[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            svA[u] = sp[u]; svB[u] = sp[4 + u];
-            bvA[u] = bnd[brow(0) + u]; bvB[u] = bnd[brow(4) + u];
+            svA[u] = sp[u]; svB[u] = sp[4 + u]; svC[u] = sp[8 + u];
+            bvA[u] = bnd[brow(0) + u]; bvB[u] = bnd[brow(4) + u]; bvC[u] = bnd[brow(8) + u];
         }
         auto group = [&](int t0, const uint32_t (&sv)[4], const uint2 (&bv)[4]) {
             uint32_t acc = 0, hout[4], eout[4];
@@ -366,14 +366,18 @@ void sw16_kernel(Sw16Launch L)
                 }
             }
         };
-        for (int t0 = 0; t0 < nsteps; t0 += 8) {
+        for (int t0 = 0; t0 < nsteps; t0 += 12) {
             group(t0, svA, bvA);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) { svA[u] = sp[t0 + 8 + u]; bvA[u] = bnd[brow(t0 + 8) + u]; }
+            for (int u = 0; u < 4; ++u) { svA[u] = sp[t0 + 12 + u]; bvA[u] = bnd[brow(t0 + 12) + u]; }
             if (t0 + 4 >= nsteps) break;
             group(t0 + 4, svB, bvB);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) { svB[u] = sp[t0 + 12 + u]; bvB[u] = bnd[brow(t0 + 12) + u]; }
+            for (int u = 0; u < 4; ++u) { svB[u] = sp[t0 + 16 + u]; bvB[u] = bnd[brow(t0 + 16) + u]; }
+            if (t0 + 8 >= nsteps) break;
+            group(t0 + 8, svC, bvC);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) { svC[u] = sp[t0 + 20 + u]; bvC[u] = bnd[brow(t0 + 20) + u]; }
         }
         // fold this strip's first maxima (per half) into the running records
 #pragma unroll
