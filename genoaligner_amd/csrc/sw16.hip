@@ -185,6 +185,14 @@ __global__ __launch_bounds__(64) void sw16_walk_kernel(Sw16Launch L)
     res[7] = len;
 }
 
+// The score bias b = max(gap_open, -mismatch, 0) exceeds gap_open, so H needs
+// the term b - go (same test as the kernel's bias).
+inline bool sw16_gk(const Sw16Launch& L)
+{
+    const int m0 = L.mismatch < 0 ? -L.mismatch : 0;
+    return m0 > L.go;
+}
+
 // Per-lane state of one strip.
 struct Strip {
     uint32_t H, Eh, Fh, diag;      // last row's H, Eh, Fh; H of the left column one row up
@@ -228,11 +236,15 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
         }
         Hb[u] = hb;
         if (TRACE) {
-            // nd: A2 > A1 (both carry the same bias, so the comparison is exact)
-            const uint32_t P1 = __builtin_amdgcn_perm(pdiff(En, Fn), pdiff(A1, A2), 0x07050301u);
-            const uint32_t P2 = __builtin_amdgcn_perm(pdiff(S.H, xf), pdiff(Hl, xe), 0x07050301u);
-            const uint32_t Z = (P1 & 0x80808080u) | ((P2 >> 4) & 0x08080808u);
-            acc = (u == 0) ? Z : ((acc >> 1) | Z);
+            // nd: A2 > A1 (both carry the same bias, so the comparison is exact).
+            // v_perm selectors 8..11 replicate the sign bit of a half into a
+            // whole byte (8: S1 low half, 9: S1 high, 10: S0 low, 11: S0 high),
+            // so each perm yields 0xff/0x00 byte masks for two flags of both
+            // pairs; step u keeps bit 4+u (nd, fe) and bit u (ee, fx) of them.
+            const uint32_t P1 = __builtin_amdgcn_perm(pdiff(En, Fn), pdiff(A1, A2), 0x0b0a0908u);
+            const uint32_t P2 = __builtin_amdgcn_perm(pdiff(S.H, xf), pdiff(Hl, xe), 0x0b0a0908u);
+            const uint32_t Z = P1 & (0x10101010u << u);
+            acc = (P2 & (0x01010101u << u)) | (u == 0 ? Z : (Z | acc));
         }
         hout[u] = Hn;
         eout[u] = En;
@@ -257,10 +269,18 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
 }
 
 // At most 96 VGPRs (5 waves per SIMD) for the trace variant.
-template <bool TRACE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TRACE ? 5 : 6)))
+template <bool B> struct BoolC { static constexpr bool value = B; };
+constexpr BoolC<true> kTrue{};
+constexpr BoolC<false> kFalse{};
+
+// GK: gap_open < -mismatch, where H needs the extra bias term (sw16_gk).
+// Prefetch depth in groups: 3 for score-only; the trace variant keeps 2 to stay
+// within 96 VGPRs without spilling in the strip loop.
+template <bool TRACE, bool GK>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
 void sw16_kernel(Sw16Launch L)
 {
+    constexpr int kDepth = TRACE ? 2 : 3;
     extern __shared__ __align__(16) uint32_t lds16[];
     const int w = blockIdx.x;
     const int lane = threadIdx.x;
@@ -286,7 +306,10 @@ void sw16_kernel(Sw16Launch L)
     const int m0 = L.mismatch < 0 ? -L.mismatch : 0;
     const int bias = go > m0 ? go : m0;            // u = s + bias >= 0
     const uint32_t b2 = uint32_t(bias) * 0x10001u, gk2 = uint32_t(bias - go) * 0x10001u;
-    const bool gk = bias != go;
+    // lanes other than 0 read the boundary from the zero rows
+    const int zrow = L.max_m + 8;
+    const uint32_t lmask = lane == 0 ? 0xffffffffu : 0u;
+    const uint32_t zaddr = lane == 0 ? 0u : uint32_t(zrow) * 8u;
 
     bool badA = false, badB = false;
     for (int i = lane - 64; i < M + 80; i += 64) {
@@ -300,7 +323,6 @@ void sw16_kernel(Sw16Launch L)
         sel[i] = sv;
     }
     for (int i = lane; i < M + 8; i += 64) bnd[i] = make_uint2(0u, go2);
-    const int zrow = L.max_m + 8;
     if (lane < 8) bnd[zrow + lane] = make_uint2(0u, 0u);
     for (int j = lane; j < N; j += 64) {
         if (j < nA && mAp > 0) badA |= dna_code(PA[j]) < 0;
@@ -330,54 +352,64 @@ void sw16_kernel(Sw16Launch L)
         S.bestH = 0; S.bestT = 0xffffffffu;
         const bool full = (s << 6) + 64 <= nmin;  // every column of the strip exists in both pairs
 
-        // Selectors and boundary rows of group t0 are read three groups ahead,
-        // into one of three register sets that rotate with the group index, so
+        // Selectors and boundary rows of group t0 are read kDepth groups ahead,
+        // into one of kDepth register sets that rotate with the group index, so
         // no register copy waits for a load in flight.  Lane 0 reads the
         // boundary rows (rows past the text repeat row M's spare entries,
         // values that only reach drained cells); the other lanes read the zero
         // rows.  Guard rows keep every read inside its buffer.
         const uint32_t* sp = sel - lane;
         const int brow_max = M + 4;                // rows M .. M+7 are spare
-        auto brow = [&](int t) { return lane == 0 ? min(t, brow_max) : zrow; };
-        uint32_t svA[4], svB[4], svC[4];
-        uint2 bvA[4], bvB[4], bvC[4];
+        auto ldg = [&](int t, uint32_t (&sv)[4], uint2 (&bv)[4]) {
+            const uint32_t ba = ((uint32_t(min(t, brow_max)) * 8u) & lmask) | zaddr;
+            const uint2* b = reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(bnd) + ba);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            svA[u] = sp[u]; svB[u] = sp[4 + u]; svC[u] = sp[8 + u];
-            bvA[u] = bnd[brow(0) + u]; bvB[u] = bnd[brow(4) + u]; bvC[u] = bnd[brow(8) + u];
-        }
-        auto group = [&](int t0, const uint32_t (&sv)[4], const uint2 (&bv)[4]) {
+            for (int u = 0; u < 4; ++u) { sv[u] = sp[t + u]; bv[u] = b[u]; }
+        };
+        uint32_t sv[kDepth][4];
+        uint2 bv[kDepth][4];
+#pragma unroll
+        for (int k = 0; k < kDepth; ++k) ldg(4 * k, sv[k], bv[k]);
+        // Groups before t_fast need neither row nor column masks.
+        const int t_fast = full ? t_main : 0;
+        auto group = [&](auto fast, int t0, const uint32_t (&sv)[4], const uint2 (&bv)[4]) {
             uint32_t acc = 0, hout[4], eout[4];
-            if (t0 < t_main && full) {
-                if (!gk) strip_group<TRACE, false, false, false>(S, t0, lane, sv, bv, b2, gk2, ge2, mAp, mBp, acc, hout, eout);
-                else     strip_group<TRACE, false, false, true>(S, t0, lane, sv, bv, b2, gk2, ge2, mAp, mBp, acc, hout, eout);
-            } else {
+            if (decltype(fast)::value || t0 < t_fast)
+                strip_group<TRACE, false, false, GK>(S, t0, lane, sv, bv, b2, gk2, ge2, mAp, mBp, acc, hout, eout);
+            else
                 strip_group<TRACE, true, true, true>(S, t0, lane, sv, bv, b2, gk2, ge2, mAp, mBp, acc, hout, eout);
-            }
             if (TRACE) dirs[(int64_t(s) * nq + (t0 >> 2)) * 64 + lane] = acc;
             // lane 63 hands rows t0-63 .. t0-60 to the next strip; rows before
             // the first land in spare row M+7, rows past the text in the spare
-            // rows M .. M+6 (lane 0 has read each row before it is written)
+            // rows M .. M+2 (lane 0 has read each row before it is written)
             if (lane == 63) {
+                if (t0 >= 64) {
+                    uint2* b = bnd + (t0 - 63);
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const unsigned r = unsigned(t0 + u - 63);
-                    bnd[min(r, unsigned(M + 7))] = make_uint2(hout[u], eout[u]);
+                    for (int u = 0; u < 4; ++u) b[u] = make_uint2(hout[u], eout[u]);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        bnd[min(unsigned(t0 + u - 63), unsigned(M + 7))] = make_uint2(hout[u], eout[u]);
                 }
             }
         };
-        for (int t0 = 0; t0 < nsteps; t0 += 12) {
-            group(t0, svA, bvA);
+        constexpr int kSpan = 4 * kDepth;
+        int t0 = 0;
+        for (; t0 + kSpan <= t_fast; t0 += kSpan) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) { svA[u] = sp[t0 + 12 + u]; bvA[u] = bnd[brow(t0 + 12) + u]; }
-            if (t0 + 4 >= nsteps) break;
-            group(t0 + 4, svB, bvB);
+            for (int k = 0; k < kDepth; ++k) {
+                group(kTrue, t0 + 4 * k, sv[k], bv[k]);
+                ldg(t0 + 4 * k + kSpan, sv[k], bv[k]);
+            }
+        }
+        for (; t0 < nsteps; t0 += kSpan) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) { svB[u] = sp[t0 + 16 + u]; bvB[u] = bnd[brow(t0 + 16) + u]; }
-            if (t0 + 8 >= nsteps) break;
-            group(t0 + 8, svC, bvC);
-#pragma unroll
-            for (int u = 0; u < 4; ++u) { svC[u] = sp[t0 + 20 + u]; bvC[u] = bnd[brow(t0 + 20) + u]; }
+            for (int k = 0; k < kDepth; ++k) {
+                if (t0 + 4 * k >= nsteps) break;
+                group(kFalse, t0 + 4 * k, sv[k], bv[k]);
+                ldg(t0 + 4 * k + kSpan, sv[k], bv[k]);
+            }
         }
         // fold this strip's first maxima (per half) into the running records
 #pragma unroll
@@ -482,8 +514,13 @@ hipError_t sw16_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHo
     const size_t lds = sw16_lds_bytes(L.max_m, trace ? L.max_n : 0);
     const dim3 grid(unsigned(L.n_waves)), block(64);
     if (hooks) hooks->before(st);
-    if (trace) hipLaunchKernelGGL(sw16_kernel<true>, grid, block, lds, st, L);
-    else       hipLaunchKernelGGL(sw16_kernel<false>, grid, block, lds, st, L);
+    if (sw16_gk(L)) {
+        if (trace) hipLaunchKernelGGL((sw16_kernel<true, true>), grid, block, lds, st, L);
+        else       hipLaunchKernelGGL((sw16_kernel<false, true>), grid, block, lds, st, L);
+    } else {
+        if (trace) hipLaunchKernelGGL((sw16_kernel<true, false>), grid, block, lds, st, L);
+        else       hipLaunchKernelGGL((sw16_kernel<false, false>), grid, block, lds, st, L);
+    }
     if (hooks) hooks->after(st);
     return hipGetLastError();
 }
