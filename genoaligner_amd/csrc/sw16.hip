@@ -66,10 +66,6 @@ __device__ __forceinline__ uint32_t pdiff(uint32_t a, uint32_t b)   // a - b per
 {
     return __builtin_bit_cast(uint32_t, __builtin_bit_cast(v2i16, a) - __builtin_bit_cast(v2i16, b));
 }
-__device__ __forceinline__ uint32_t psign_mask(uint32_t d)          // 0xffff where half < 0
-{
-    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(v2i16, d) >> short(15));
-}
 // lane l <- lane l-1 (DPP wave_shr:1, lane 0 gets 0) plus `add`, which is the
 // boundary value on lane 0 and 0 on every other lane; folds into one
 // v_add_u32_dpp.
@@ -260,7 +256,9 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
         uint32_t b = S.bestH;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const uint32_t up = psign_mask(pdiff(b, Hb[u]));
+            // 0xffff per half where Hb[u] > b: v_perm selectors 8 and 9
+            // replicate the sign bits of the low and high half of b - Hb[u]
+            const uint32_t up = __builtin_amdgcn_perm(0u, pdiff(b, Hb[u]), 0x09090808u);
             b = pmax(b, Hb[u]);
             S.bestT = (up & (uint32_t(t0 + u) * 0x10001u)) | (~up & S.bestT);
         }
@@ -370,6 +368,11 @@ void sw16_kernel(Sw16Launch L)
         uint2 bv[kDepth][4];
 #pragma unroll
         for (int k = 0; k < kDepth; ++k) ldg(4 * k, sv[k], bv[k]);
+        // Wait here for the first set: the loop head then waits only for what
+        // the loop itself issued (with the stores of the trace variant between
+        // the loads, the wait count at the head would otherwise be set by this
+        // prologue and hold up every iteration until the last store is done).
+        asm volatile("" ::"v"(sv[0][0]));
         // Groups before t_fast need neither row nor column masks.
         const int t_fast = full ? t_main : 0;
         auto group = [&](auto fast, int t0, const uint32_t (&sv)[4], const uint2 (&bv)[4]) {
