@@ -148,16 +148,19 @@ struct Sw16Launch {
     uint8_t* rerun;         // out: 1 = needs the int32 kernel
     uint32_t* dirs; int64_t dir_stride;             // words per wave (trace)
     uint32_t* selbuf;       // per-wave text selector rows (scratch), sw16_sel_stride dwords per wave
-    int32_t* wave_m;        // out: per wave, rows streamed (trace walk)
     const uint8_t* walk;    // per pair: emit span + CIGAR; null = all
     uint8_t* cigar; const int64_t* cig_off;
 };
 
 // Largest text length the packed kernel takes (LDS: 8 bytes per text row).
 constexpr int kSw16MaxRows = 4096;
+// Largest pattern length the packed kernel takes in a CIGAR batch (its walk
+// stages both pairs' sequences and CIGARs, 4 bytes per row and column, in LDS:
+// with kSw16MaxRows that stays under 64 KB per wave).
+constexpr int kSw16MaxCols = 8192;
 
 // Selector scratch per wave: 64 guard rows before and 80 after max_m rows
-// (reads reach 63 rows before row 0 and 74 rows past the last row).
+// (reads reach 63 rows before row 0 and 77 rows past the last row).
 __host__ __device__ inline int64_t sw16_sel_stride(int max_m) { return (int64_t(max_m) + 144 + 3) & ~int64_t(3); }
 
 inline int64_t sw16_dir_words_per_wave(int max_m, int max_n)
@@ -192,7 +195,6 @@ hipError_t sw_launch(const SwLaunch& L, bool trace, hipStream_t st, LaunchHooks*
 // Packed kernel over L.n_waves waves (two pairs each), then -- separately --
 // its walk over 2 * n_waves slots.
 hipError_t sw16_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHooks* hooks = nullptr);
-hipError_t sw16_walk_launch(const Sw16Launch& L, hipStream_t st);
 
 // Reduce the 8-int per-pair records to the device-resident result layout:
 // out5 = score, start_i, start_j, end_i, end_j (trace: row-major end and span;

@@ -744,7 +744,8 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
         slot_off[size_t(i)] = slot;
         slot += std::max(r.text_len, 0) + std::max(r.pattern_len, 0);
         const bool sk = skip[size_t(i)] != 0;
-        if (!mat && !sk && r.text_len > 0 && r.pattern_len > 0 && r.text_len <= ga::kSw16MaxRows)
+        if (!mat && !sk && r.text_len > 0 && r.pattern_len > 0 && r.text_len <= ga::kSw16MaxRows &&
+            (!trace || r.pattern_len <= ga::kSw16MaxCols))
             packed.push_back(i);
         else
             order.push_back(i);
@@ -891,7 +892,6 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
             max_stride = std::max(max_stride, trace ? ga::sw16_dir_words_per_wave(int(mm), int(mn)) * (n_waves - w0) : 0);
         }
         uint32_t* d_dirs16 = trace ? D.alloc<uint32_t>(size_t(max_stride)) : nullptr;
-        int32_t* d_wave_m = D.alloc<int32_t>(size_t(n_waves));
         int32_t pk_max_m = 0;
         for (int64_t k = n32; k < n; ++k) pk_max_m = std::max(pk_max_m, tl[size_t(k)]);
         int64_t max_cw = 0;
@@ -914,7 +914,7 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
             L.res = d_res; L.rerun = d_rerun;
             L.dirs = d_dirs16; L.dir_stride = trace ? ga::sw16_dir_words_per_wave(cm, cn) : 0;
             L.selbuf = d_sel;
-            L.wave_m = d_wave_m; L.walk = d_walk; L.cigar = d_cig; L.cig_off = d_slot;
+            L.walk = d_walk; L.cigar = d_cig; L.cig_off = d_slot;
             if (enqueue_sw16_chunk(L, trace, st, nullptr) != hipSuccess) err = "SW kernel launch";
         }
     }
@@ -1349,7 +1349,7 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)
     };
     auto release = [&] { for (void* p : owned) (void)hipFreeAsync(p, st); };
     int32_t* d_res = nullptr;
-    if (b->max_text_len <= ga::kSw16MaxRows) {
+    if (b->max_text_len <= ga::kSw16MaxRows && (!trace || b->max_pattern_len <= ga::kSw16MaxCols)) {
         // packed path: pairs (2w, 2w+1) share a wave
         const int64_t n_waves = (b->n_pairs + 1) / 2;
         const int64_t stride = trace ? ga::sw16_dir_words_per_wave(b->max_text_len, b->max_pattern_len) : 0;
@@ -1357,10 +1357,8 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)
         if (trace && stride > 0) cw = std::max<int64_t>(1, std::min<int64_t>(cw, sw_dir_budget() / (stride * 4)));
         uint8_t* d_rerun = nullptr;
         uint32_t* d_dirs = nullptr;
-        int32_t* d_wave_m = nullptr;
         bool ok = dalloc(reinterpret_cast<void**>(&d_res), size_t(b->n_pairs) * 32) &&
-                  dalloc(reinterpret_cast<void**>(&d_rerun), size_t(b->n_pairs)) &&
-                  dalloc(reinterpret_cast<void**>(&d_wave_m), size_t(cw) * 4);
+                  dalloc(reinterpret_cast<void**>(&d_rerun), size_t(b->n_pairs));
         if (ok && trace) ok = dalloc(reinterpret_cast<void**>(&d_dirs), size_t(cw * stride) * 4);
         uint32_t* d_sel = nullptr;
         if (ok) ok = dalloc(reinterpret_cast<void**>(&d_sel), size_t(cw * ga::sw16_sel_stride(b->max_text_len)) * 4);
@@ -1377,7 +1375,7 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)
             L.match = b->scoring.match; L.mismatch = b->scoring.mismatch;
             L.go = b->scoring.gap_open; L.ge = b->scoring.gap_extend;
             L.res = d_res; L.rerun = d_rerun; L.dirs = d_dirs; L.dir_stride = stride; L.selbuf = d_sel;
-            L.wave_m = d_wave_m; L.walk = nullptr; L.cigar = b->cigar; L.cig_off = b->cigar_off;
+            L.walk = nullptr; L.cigar = b->cigar; L.cig_off = b->cigar_off;
             if (enqueue_sw16_chunk(L, trace, st, &t_timer_hooks) != hipSuccess) {
                 if (error) *error = "SW kernel launch";
                 rc = GA_DEVICE_ERROR;

@@ -92,93 +92,101 @@ __device__ __forceinline__ uint32_t u_table(uint8_t q, int match, int mismatch, 
 
 constexpr uint32_t kSelNone = 0x0c0c0c0cu;   // v_perm selector: both halves u = 0
 
+// Flag-word window of a walk (walk16): kWinC columns ending at the anchor
+// column (one per lane of the walk's half-wave), and per column the kWinQ
+// words that hold the kWinR rows ending at the anchor row.
+constexpr int kWinR = 32, kWinC = 32, kWinQ = kWinR / 4 + 1;
+constexpr size_t kWinBytes = size_t(2) * kWinC * kWinQ * 4;   // both walks of a wave
+
+// Offset of the walk windows behind the trace staging area (2 CIGAR slots and
+// both pairs' sequences: at most 4 * (max_m + max_n) bytes).
+__host__ __device__ inline size_t sw16_win_offset(int max_m, int max_n)
+{
+    return (size_t(4) * size_t(max_m + max_n) + 15) & ~size_t(15);
+}
+
 // LDS layout: the boundary column, max_m + 8 rows of uint2, followed by 8 zero
-// rows that lanes 1..63 read in place of the boundary.
+// rows that lanes 1..63 read in place of the boundary.  Once the strips are
+// done (trace variant) the same bytes hold the staging area and the windows.
 __host__ __device__ inline size_t sw16_lds_bytes(int max_m, int max_n)
 {
     const size_t bnd = size_t(max_m + 16) * 8;
-    const size_t stage = size_t(4) * size_t(max_m + max_n) + 16;   // trace: 2 CIGAR slots + 2 pairs
+    const size_t stage = max_n > 0 ? sw16_win_offset(max_m, max_n) + kWinBytes : 0;
     return bnd > stage ? bnd : stage;
-}
-
-// The 4 flags of cell (i, j) of half h, in bits 0..3 = nd, fe, ee, fx.
-__device__ __forceinline__ uint32_t flags16(const uint32_t* dirs, int nq, int i, int j, int h)
-{
-    const int j0 = j - 1, s = j0 >> 6, l = j0 & 63, t = (i - 1) + l;
-    const uint32_t w = dirs[(int64_t(s) * nq + (t >> 2)) * 64 + l];
-    const int u = t & 3;
-    const uint32_t lo = w >> (8 * h), hi = w >> (8 * (2 + h));   // bytes h (nd, ee) and 2+h (fe, fx)
-    return ((lo >> (4 + u)) & 1u) | (((hi >> (4 + u)) & 1u) << 1) |
-           (((lo >> u) & 1u) << 2) | (((hi >> u) & 1u) << 3);
 }
 
 // The reference's three-state walk (sw_kernel_impl.hip:198-242) driven by the 4
 // flags and the carried cell value, from the 1-based end cell (i, j) with H =
-// score.  T, P and cg may live in LDS or global memory.  Writes the CIGAR
-// right-aligned into cg[0 .. m+n) and returns its length; *si, *sj receive the
-// 0-based start.  Flag words along the current diagonal are loaded four steps
-// ahead, since most moves of a local alignment are diagonal.
+// score; writes the CIGAR right-aligned into cg[0 .. m+n) and returns its
+// length; *si, *sj receive the 0-based start.  Both walks of a wave run at once
+// (pair A on lane 0, pair B on lane 32; `walker` marks those lanes when their
+// pair is walked).  The flag words come from LDS windows: when a walk leaves its
+// window, its half-wave loads the kWinC x kWinR cells up and left of the
+// current cell (lane c of the half loads column j - c, coalesced across the
+// half), so a walk waits for one round of loads per window instead of one
+// every few steps.  Every lane of the wave calls it; lanes other than the
+// walkers return 0.  Flag word layout: cell (i, j) of half h sits in word
+// (s * nq + (t >> 2)) * 64 + l, with s, l the strip and lane of column j - 1
+// and t = i - 1 + l; with b = 8h + (t & 3), bit b + 4 is nd, b + 20 fe, b ee
+// and b + 16 fx.
 template <class CP, class SP>
-__device__ int walk16(const uint32_t* dirs, int nq, int h, SP T, SP P, int m, int n, int i, int j,
-                      int score, int match, int mismatch, int go, int ge, CP cg, int* si, int* sj)
+__device__ int walk16(const uint32_t* dirs, int nq, uint32_t* win, bool walker, int lane, SP T, SP P,
+                      int m, int n, int i, int j, int score, int match, int mismatch, int go, int ge,
+                      CP cg, int* si, int* sj)
 {
-    // fq[k] = flags of (i-k, j-k): a window along the current diagonal, so a
-    // diagonal run waits for one load every 4 steps
-    auto fl = [&](int a, int b) -> uint32_t { return (a > 0 && b > 0) ? flags16(dirs, nq, a, b, h) : 0u; };
-    uint32_t fq0 = fl(i, j), fq1 = fl(i - 1, j - 1), fq2 = fl(i - 2, j - 2), fq3 = fl(i - 3, j - 3);
+    const int hw = lane >> 5, c = lane & 31;
+    uint32_t* w = win + hw * (kWinC * kWinQ);
     int pos = m + n, state = 0, v = score;
-    while (i > 0 && j > 0 && pos > 0) {
-        if (state == 0 && v == 0) break;
-        const uint32_t f = fq0;
-        if (state == 0 && !(f & 1u)) {
-            const bool eq = T[i - 1] == P[j - 1];
-            cg[--pos] = eq ? 'M' : 'X';
-            v -= eq ? match : mismatch;
-            --i; --j;
-            fq0 = fq1; fq1 = fq2; fq2 = fq3; fq3 = fl(i - 3, j - 3);
-            continue;
+    bool active = walker;
+    bool have = false;
+    int iw = 0, jw = 0;                            // window anchor (walker lanes)
+    while (__any(active)) {
+        if (active && (!(i > 0 && j > 0 && pos > 0) || (state == 0 && v == 0))) active = false;
+        const bool need = active && !(have && iw - i < kWinR && jw - j < kWinC);
+        const int need_h = __shfl(int(need), hw * 32);
+        const int ih = __shfl(i, hw * 32), jh = __shfl(j, hw * 32);
+        if (need_h) {
+            const int jj = jh - c;                 // 1-based column of this lane
+            if (jj >= 1) {
+                const int l = (jj - 1) & 63, s = (jj - 1) >> 6;
+                const int q0 = (ih - kWinR + l) >> 2;   // floor
+#pragma unroll
+                for (int k = 0; k < kWinQ; ++k) {
+                    const int q = q0 + k;
+                    w[c * kWinQ + k] = (q >= 0 && q < nq) ? dirs[(int64_t(s) * nq + q) * 64 + l] : 0u;
+                }
+            }
         }
-        if (state == 1 || (state == 0 && !(f & 2u))) {
-            cg[--pos] = 'D';
-            const bool ext = f & 4u;
-            state = ext ? 1 : 0;
-            v += ext ? ge : go;
-            --j;
-        } else {
-            cg[--pos] = 'I';
-            const bool ext = f & 8u;
-            state = ext ? 2 : 0;
-            v += ext ? ge : go;
-            --i;
+        if (need) { iw = i; jw = j; have = true; }
+        if (!active) continue;
+        // at most kWinR + kWinC - 1 steps stay inside the window
+        while (i > 0 && j > 0 && pos > 0 && !(state == 0 && v == 0) && iw - i < kWinR && jw - j < kWinC) {
+            const int l = (j - 1) & 63, t = i - 1 + l;
+            const uint32_t word = w[(jw - j) * kWinQ + ((t >> 2) - ((iw - kWinR + l) >> 2))];
+            const int b = 8 * hw + (t & 3);
+            if (state == 0 && !((word >> (b + 4)) & 1u)) {
+                const bool eq = T[i - 1] == P[j - 1];
+                cg[--pos] = eq ? 'M' : 'X';
+                v -= eq ? match : mismatch;
+                --i; --j;
+            } else if (state == 1 || (state == 0 && !((word >> (b + 20)) & 1u))) {
+                cg[--pos] = 'D';
+                const bool ext = (word >> b) & 1u;
+                state = ext ? 1 : 0;
+                v += ext ? ge : go;
+                --j;
+            } else {
+                cg[--pos] = 'I';
+                const bool ext = (word >> (b + 16)) & 1u;
+                state = ext ? 2 : 0;
+                v += ext ? ge : go;
+                --i;
+            }
         }
-        fq0 = fl(i, j); fq1 = fl(i - 1, j - 1); fq2 = fl(i - 2, j - 2); fq3 = fl(i - 3, j - 3);
     }
     *si = i;
     *sj = j;
-    return m + n - pos;
-}
-
-// Stand-alone walk over 2 * n_waves slots (kept for launches that separate it).
-__global__ __launch_bounds__(64) void sw16_walk_kernel(Sw16Launch L)
-{
-    const int q = int(blockIdx.x) * 64 + int(threadIdx.x);
-    if (q >= 2 * L.n_waves) return;
-    const int w = q >> 1, h = q & 1;
-    int pid = L.pair_a ? (h ? L.pair_b[w] : L.pair_a[w]) : L.first_pair + q;
-    if (pid >= L.n_total || pid < 0) return;
-    if (L.rerun[pid]) return;
-    int32_t* res = L.res + int64_t(pid) * 8;
-    const int score = res[0];
-    if (score <= 0 || (L.walk && !L.walk[pid])) return;
-    const int Mw = L.wave_m[w];
-    const int nq = ((Mw + 63 + 3) & ~3) >> 2;
-    int si, sj;
-    const int len = walk16(L.dirs + int64_t(w) * L.dir_stride, nq, h, L.txt + L.toff[pid],
-                           L.pat + L.poff[pid], L.tlen[pid], L.plen[pid], res[1] + 1, res[2] + 1,
-                           score, L.match, L.mismatch, L.go, L.ge, L.cigar + L.cig_off[pid], &si, &sj);
-    res[5] = si;
-    res[6] = sj;
-    res[7] = len;
+    return walker ? m + n - pos : 0;
 }
 
 // The score bias b = max(gap_open, -mismatch, 0) exceeds gap_open, so H needs
@@ -330,7 +338,6 @@ void sw16_kernel(Sw16Launch L)
     badB = __any(badB);
     __syncthreads();
 
-    if (TRACE && lane == 0) L.wave_m[w] = M;
     const int nstrips = (N + 63) >> 6;
     const int nsteps = (M + 63 + 3) & ~3;
     const int nq = nsteps >> 2;
@@ -481,21 +488,20 @@ void sw16_kernel(Sw16Launch L)
     for (int q = lane; q < mB; q += 64) sTB[q] = TB[q];
     for (int q = lane; q < nB; q += 64) sPB[q] = PB[q];
     __syncthreads();
-    int len_h = 0;
-    if ((lane & 31) == 0) {
-        const int h = lane >> 5;
-        const int pid = h == 0 ? ia : ib;
-        if (pid >= 0 && !redo[h] && endS[h] > 0 && (!L.walk || L.walk[pid])) {
-            const int m = h == 0 ? mA : mB, n = h == 0 ? nA : nB;
-            int si, sj;
-            len_h = walk16(dirs, nq, h, h == 0 ? sTA : sTB, h == 0 ? sPA : sPB, m, n,
-                           endI[h] + 1, endJ[h] + 1, endS[h], L.match, L.mismatch, go, ge,
-                           h == 0 ? cgA : cgB, &si, &sj);
-            int32_t* res = L.res + int64_t(pid) * 8;
-            res[5] = si;
-            res[6] = sj;
-            res[7] = len_h;
-        }
+    uint32_t* win = reinterpret_cast<uint32_t*>(st8 + sw16_win_offset(L.max_m, L.max_n));
+    const int hw = lane >> 5;
+    const int pidw = hw == 0 ? ia : ib;
+    const bool walker = (lane & 31) == 0 && pidw >= 0 && !redo[hw] && endS[hw] > 0 &&
+                        (!L.walk || L.walk[pidw]);
+    int si = -1, sj = -1;
+    const int len_h = walk16(dirs, nq, win, walker, lane, hw == 0 ? sTA : sTB, hw == 0 ? sPA : sPB,
+                             hw == 0 ? mA : mB, hw == 0 ? nA : nB, endI[hw] + 1, endJ[hw] + 1,
+                             endS[hw], L.match, L.mismatch, go, ge, hw == 0 ? cgA : cgB, &si, &sj);
+    if (walker) {
+        int32_t* res = L.res + int64_t(pidw) * 8;
+        res[5] = si;
+        res[6] = sj;
+        res[7] = len_h;
     }
     __syncthreads();
     const int lenA = __shfl(len_h, 0), lenB = __shfl(len_h, 32);
@@ -525,13 +531,6 @@ hipError_t sw16_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHo
         else       hipLaunchKernelGGL((sw16_kernel<false, false>), grid, block, lds, st, L);
     }
     if (hooks) hooks->after(st);
-    return hipGetLastError();
-}
-
-hipError_t sw16_walk_launch(const Sw16Launch& L, hipStream_t st)
-{
-    if (L.n_waves <= 0) return hipSuccess;
-    hipLaunchKernelGGL(sw16_walk_kernel, dim3(unsigned((2 * L.n_waves + 63) / 64)), dim3(64), 0, st, L);
     return hipGetLastError();
 }
 
