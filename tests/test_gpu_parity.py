@@ -193,6 +193,23 @@ def test_sw_long_text_global_boundary(gpu, oracle):
                (exp.score, exp.cigar, exp.start_i, exp.end_i, exp.start_j, exp.end_j)
 
 
+def test_sw_long_pattern_cigar(gpu, oracle):
+    """Patterns longer than the packed kernel stages in LDS (8192 columns with
+    CIGAR) take the int32 kernel; shorter ones in the same batch stay packed."""
+    import genoaligner_amd as ga
+    rng = random.Random(5)
+    p = "".join(rng.choice("ACGT") for _ in range(9000))
+    t = _mutate(rng, p[3000:3300], 0.1)
+    reqs = [_sreq(ga, t, p, (2, -3, 3, 1)), _sreq(ga, t, p[2900:3400], (2, -3, 3, 1)),
+            _sreq(ga, t[:80], p[:8192], (2, -3, 3, 1))]
+    b = ga.align_sw_batch(reqs)
+    want = oracle.align_sw_batch(reqs)
+    assert b.ok(), b.error
+    for got, exp in zip(b.results, want.results):
+        assert (got.score, got.cigar, got.start_i, got.end_i, got.start_j, got.end_j) == \
+               (exp.score, exp.cigar, exp.start_i, exp.end_i, exp.start_j, exp.end_j)
+
+
 def test_sw_too_large(gpu):
     import genoaligner_amd as ga
     big = "A" * 8192
