@@ -74,6 +74,15 @@ __device__ __forceinline__ uint32_t shr1(uint32_t add, uint32_t v)
     return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x138, 0xf, 0xf, true)) + add;
 }
 
+// (m & a) | (~m & b) as one v_bfi_b32 (written out, since the compiler turns
+// the C form with constant masks into separate and/or instructions)
+__device__ __forceinline__ uint32_t bsel(uint32_t m, uint32_t a, uint32_t b)
+{
+    uint32_t d;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(d) : "s"(m), "v"(a), "v"(b));
+    return d;
+}
+
 __device__ __forceinline__ int dna_code(uint8_t c)
 {
     return c == 'A' ? 0 : c == 'C' ? 1 : c == 'G' ? 2 : c == 'T' ? 3 : -1;
@@ -215,7 +224,7 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
                                             uint32_t ge2, int mAp, int mBp, uint32_t& acc,
                                             uint32_t (&hout)[4], uint32_t (&eout)[4])
 {
-    uint32_t Hb[4];
+    uint32_t Hb[4], Q[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         const uint32_t Hl = shr1(bv[u].x, S.H);
@@ -244,11 +253,10 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
             // v_perm selectors 8..11 replicate the sign bit of a half into a
             // whole byte (8: S1 low half, 9: S1 high, 10: S0 low, 11: S0 high),
             // so each perm yields 0xff/0x00 byte masks for two flags of both
-            // pairs; step u keeps bit 4+u (nd, fe) and bit u (ee, fx) of them.
+            // pairs; Q[u] keeps nd/fe in the high and ee/fx in the low nibble.
             const uint32_t P1 = __builtin_amdgcn_perm(pdiff(En, Fn), pdiff(A1, A2), 0x0b0a0908u);
             const uint32_t P2 = __builtin_amdgcn_perm(pdiff(S.H, xf), pdiff(Hl, xe), 0x0b0a0908u);
-            const uint32_t Z = P1 & (0x10101010u << u);
-            acc = (P2 & (0x01010101u << u)) | (u == 0 ? Z : (Z | acc));
+            Q[u] = bsel(0xf0f0f0f0u, P1, P2);
         }
         hout[u] = Hn;
         eout[u] = En;
@@ -257,6 +265,10 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
         S.Eh = En;
         S.Fh = Fn;
     }
+    // Row u of the group goes to bit 4+u (nd, fe) and bit u (ee, fx) of each
+    // byte: bits alternate between rows 0/1 and 2/3, then pairs of bits
+    // between the two.
+    if (TRACE) acc = bsel(0x33333333u, bsel(0x55555555u, Q[0], Q[1]), bsel(0x55555555u, Q[2], Q[3]));
     // First maxima: one max per step; the exact per-step update runs only for a
     // group in which some lane's maximum grew.
     const uint32_t nb = pmax(pmax(S.bestH, Hb[0]), pmax(pmax(Hb[1], Hb[2]), Hb[3]));
