@@ -292,13 +292,12 @@ constexpr BoolC<true> kTrue{};
 constexpr BoolC<false> kFalse{};
 
 // GK: gap_open < -mismatch, where H needs the extra bias term (sw16_gk).
-// Prefetch depth in groups: 3 for score-only; the trace variant keeps 2 to stay
-// within 96 VGPRs without spilling in the strip loop.
+// Prefetch depth in groups (register sets of selectors and boundary rows).
 template <bool TRACE, bool GK>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
 void sw16_kernel(Sw16Launch L)
 {
-    constexpr int kDepth = TRACE ? 2 : 3;
+    constexpr int kDepth = 3;
     extern __shared__ __align__(16) uint32_t lds16[];
     const int w = blockIdx.x;
     const int lane = threadIdx.x;
@@ -387,11 +386,12 @@ void sw16_kernel(Sw16Launch L)
         uint2 bv[kDepth][4];
 #pragma unroll
         for (int k = 0; k < kDepth; ++k) ldg(4 * k, sv[k], bv[k]);
-        // Wait here for the first set: the loop head then waits only for what
-        // the loop itself issued (with the stores of the trace variant between
-        // the loads, the wait count at the head would otherwise be set by this
-        // prologue and hold up every iteration until the last store is done).
-        asm volatile("" ::"v"(sv[0][0]));
+        // Wait here for these sets: each group then waits only for what the
+        // loop itself issued (with the stores of the trace variant between the
+        // loads, the wait counts in the loop would otherwise be set by this
+        // prologue and hold up every iteration until an older store is done).
+#pragma unroll
+        for (int k = 0; k < kDepth; ++k) asm volatile("" ::"v"(sv[k][0]));
         // Groups before t_fast need neither row nor column masks.
         const int t_fast = full ? t_main : 0;
         auto group = [&](auto fast, int t0, const uint32_t (&sv)[4], const uint2 (&bv)[4]) {
