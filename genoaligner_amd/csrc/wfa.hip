@@ -90,26 +90,27 @@ __device__ __forceinline__ int ws_get(const Off* ws, int s, int k)
 __device__ __forceinline__ uint32_t base_code(uint32_t b) { return (b >> 1) & 3u; }
 __device__ __forceinline__ bool is_acgt(uint32_t b) { return b == 'A' || b == 'C' || b == 'G' || b == 'T'; }
 
-// Bases i .. i+15 as 32 bits: two aligned reads and one bit funnel shift
-// (v_alignbit_b32).
-__device__ __forceinline__ uint32_t lds_load16(const uint32_t* w, int i)
-{
-    const uint32_t lo = w[i >> 4];
-    const uint32_t hi = w[(i >> 4) + 1];
-    return __builtin_amdgcn_alignbit(hi, lo, (uint32_t(i) & 15u) * 2u);
-}
 
 // Run of equal bases P[v..] == T[h..], capped at lim (lim >= 0; bases v and h
-// must lie inside the staged words even when lim == 0).  One loop exit, so
-// lanes leave the loop without extra exec-mask bookkeeping.
+// must lie inside the staged words even when lim == 0).  Bases i .. i+15 of a
+// staged sequence are two aligned words joined by one bit funnel shift
+// (v_alignbit_b32).
 __device__ __forceinline__ int lds_match_run(const uint32_t* P, int v, const uint32_t* T, int h, int lim)
 {
-    int run = -16;
+    // word pointers and bit shifts fixed for the run: one step is two word
+    // pairs, two funnel shifts and a compare
+    const uint32_t* p = P + (v >> 4);
+    const uint32_t* t = T + (h >> 4);
+    const uint32_t sp = (uint32_t(v) & 15u) * 2u, st = (uint32_t(h) & 15u) * 2u;
+    int run = 0;
     uint32_t x;
-    do {
+    for (;;) {
+        x = __builtin_amdgcn_alignbit(p[1], p[0], sp) ^ __builtin_amdgcn_alignbit(t[1], t[0], st);
+        if (x != 0 || run + 16 >= lim) break;
         run += 16;
-        x = lds_load16(P, v + run) ^ lds_load16(T, h + run);
-    } while (x == 0 && run + 16 < lim);
+        ++p;
+        ++t;
+    }
     run += x ? int(__builtin_ctz(x) >> 1) : 16;
     return run < lim ? run : lim;
 }
