@@ -66,12 +66,12 @@ __device__ __forceinline__ uint32_t pdiff(uint32_t a, uint32_t b)   // a - b per
 {
     return __builtin_bit_cast(uint32_t, __builtin_bit_cast(v2i16, a) - __builtin_bit_cast(v2i16, b));
 }
-// lane l <- lane l-1 (DPP wave_shr:1, lane 0 gets 0) plus `add`, which is the
-// boundary value on lane 0 and 0 on every other lane; folds into one
-// v_add_u32_dpp.
-__device__ __forceinline__ uint32_t shr1(uint32_t add, uint32_t v)
+// lane l <- lane l-1 (DPP wave_shr:1); lane 0, which has no source lane, keeps
+// `old` (bound_ctrl off), the boundary value: one v_mov_b32_dpp into the
+// register the boundary row was read into.
+__device__ __forceinline__ uint32_t shr1(uint32_t old, uint32_t v)
 {
-    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x138, 0xf, 0xf, true)) + add;
+    return uint32_t(__builtin_amdgcn_update_dpp(int(old), int(v), 0x138, 0xf, 0xf, false));
 }
 
 // (m & a) | (~m & b) as one v_bfi_b32 (written out, since the compiler turns
@@ -114,12 +114,21 @@ __host__ __device__ inline size_t sw16_win_offset(int max_m, int max_n)
     return (size_t(4) * size_t(max_m + max_n) + 15) & ~size_t(15);
 }
 
-// LDS layout: the boundary column, max_m + 8 rows of uint2, followed by 8 zero
-// rows that lanes 1..63 read in place of the boundary.  Once the strips are
-// done (trace variant) the same bytes hold the staging area and the windows.
+// Rows of the LDS boundary column: the text rows rounded up to whole groups of
+// 4 (at least one group).  8 bytes per row, so a 1024-row text takes 8 KB and
+// 20 waves fit a CU's 160 KB.
+__host__ __device__ inline int sw16_bnd_rows(int m)
+{
+    const int r = (m + 3) & ~3;
+    return r > 4 ? r : 4;
+}
+
+// LDS layout: the boundary column (H and Eh of both pairs per text row).  Once
+// the strips are done (trace variant) the same bytes hold the staging area and
+// the windows.
 __host__ __device__ inline size_t sw16_lds_bytes(int max_m, int max_n)
 {
-    const size_t bnd = size_t(max_m + 16) * 8;
+    const size_t bnd = size_t(sw16_bnd_rows(max_m)) * 8;
     const size_t stage = max_n > 0 ? sw16_win_offset(max_m, max_n) + kWinBytes : 0;
     return bnd > stage ? bnd : stage;
 }
@@ -323,10 +332,7 @@ void sw16_kernel(Sw16Launch L)
     const int m0 = L.mismatch < 0 ? -L.mismatch : 0;
     const int bias = go > m0 ? go : m0;            // u = s + bias >= 0
     const uint32_t b2 = uint32_t(bias) * 0x10001u, gk2 = uint32_t(bias - go) * 0x10001u;
-    // lanes other than 0 read the boundary from the zero rows
-    const int zrow = L.max_m + 8;
-    const uint32_t lmask = lane == 0 ? 0xffffffffu : 0u;
-    const uint32_t zaddr = lane == 0 ? 0u : uint32_t(zrow) * 8u;
+    const int Mr = sw16_bnd_rows(M);             // boundary rows in use (<= the LDS rows)
 
     bool badA = false, badB = false;
     for (int i = lane - 64; i < M + 80; i += 64) {
@@ -339,8 +345,7 @@ void sw16_kernel(Sw16Launch L)
         }
         sel[i] = sv;
     }
-    for (int i = lane; i < M + 8; i += 64) bnd[i] = make_uint2(0u, go2);
-    if (lane < 8) bnd[zrow + lane] = make_uint2(0u, 0u);
+    for (int i = lane; i < Mr; i += 64) bnd[i] = make_uint2(0u, go2);
     for (int j = lane; j < N; j += 64) {
         if (j < nA && mAp > 0) badA |= dna_code(PA[j]) < 0;
         if (j < nB && mBp > 0) badB |= dna_code(PB[j]) < 0;
@@ -370,15 +375,14 @@ void sw16_kernel(Sw16Launch L)
 
         // Selectors and boundary rows of group t0 are read kDepth groups ahead,
         // into one of kDepth register sets that rotate with the group index, so
-        // no register copy waits for a load in flight.  Lane 0 reads the
-        // boundary rows (rows past the text repeat row M's spare entries,
-        // values that only reach drained cells); the other lanes read the zero
-        // rows.  Guard rows keep every read inside its buffer.
+        // no register copy waits for a load in flight.  Every lane reads the
+        // group's boundary rows (one LDS address, a broadcast); only lane 0
+        // keeps them (shr1).  Groups past the last one repeat it: those rows
+        // only reach drained cells.  Guard rows keep every selector read inside
+        // its buffer.
         const uint32_t* sp = sel - lane;
-        const int brow_max = M + 4;                // rows M .. M+7 are spare
         auto ldg = [&](int t, uint32_t (&sv)[4], uint2 (&bv)[4]) {
-            const uint32_t ba = ((uint32_t(min(t, brow_max)) * 8u) & lmask) | zaddr;
-            const uint2* b = reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(bnd) + ba);
+            const uint2* b = bnd + min(t, Mr - 4);
 #pragma unroll
             for (int u = 0; u < 4; ++u) { sv[u] = sp[t + u]; bv[u] = b[u]; }
         };
@@ -401,18 +405,20 @@ void sw16_kernel(Sw16Launch L)
             else
                 strip_group<TRACE, true, true, true>(S, t0, lane, sv, bv, b2, gk2, ge2, mAp, mBp, acc, hout, eout);
             if (TRACE) dirs[(int64_t(s) * nq + (t0 >> 2)) * 64 + lane] = acc;
-            // lane 63 hands rows t0-63 .. t0-60 to the next strip; rows before
-            // the first land in spare row M+7, rows past the text in the spare
-            // rows M .. M+2 (lane 0 has read each row before it is written)
+            // lane 63 hands rows t0-63 .. t0-60 to the next strip (lane 0 has
+            // read each row before it is written); rows outside 0 .. Mr-1 are
+            // dropped, and rows M .. Mr-1 only reach drained cells
             if (lane == 63) {
-                if (t0 >= 64) {
+                if (t0 >= 64 && t0 - 60 < Mr) {
                     uint2* b = bnd + (t0 - 63);
 #pragma unroll
                     for (int u = 0; u < 4; ++u) b[u] = make_uint2(hout[u], eout[u]);
                 } else {
 #pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        bnd[min(unsigned(t0 + u - 63), unsigned(M + 7))] = make_uint2(hout[u], eout[u]);
+                    for (int u = 0; u < 4; ++u) {
+                        const int r = t0 + u - 63;
+                        if (r >= 0 && r < Mr) bnd[r] = make_uint2(hout[u], eout[u]);
+                    }
                 }
             }
         };
