@@ -42,6 +42,28 @@ SW_SCHEME = (2, -3, 3, 1)
 SW_OPS_PER_CELL = 12            # E: 2 sub + 1 max; F: 2 sub + 1 max; H: add + 3 max; best: 1 max
 PEAK_INT16_TOPS = 78.6          # gfx950 packed-int16 VALU: 256 CU x 2.4 GHz x 64 lanes x 2
 PEAK_HBM_GBS = 8000.0
+PEAK_INT32_TOPS = 39.3          # gfx950 int32 VALU: 256 CU x 2.4 GHz x 64 lanes
+# WFA, per diagonal update: max(ins, mis), +1, max with del, v = mx - k, two
+# bound checks, and the first 16-base compare (xor + test)
+WFA_OPS_PER_UPDATE = 8
+
+
+def wfa_valu_roofline(ps, scores, kernel_ms):
+    """Diagonal updates of a WFA batch, sum over pairs of (w + 1)^2 with w the
+    score (resolved), smax (unresolved after a full pass) or nothing (length
+    difference above smax), against the int32 VALU peak."""
+    a = ps.a_len.astype(np.int64)
+    b = ps.b_len.astype(np.int64)
+    sm = ps.smax.astype(np.int64)
+    sc = np.asarray(scores, dtype=np.int64)
+    w = np.where(sc >= 0, sc, np.where(np.abs(a - b) <= sm, sm, -1))
+    upd = int(((w + 1) ** 2).sum())
+    if not kernel_ms:
+        return {"bound": "valu", "diagonal_updates": upd}
+    tops = WFA_OPS_PER_UPDATE * upd / (kernel_ms * 1e-3) / 1e12
+    return {"bound": "valu", "diagonal_updates": upd, "ops_per_update": WFA_OPS_PER_UPDATE,
+            "achieved_TOPs": round(tops, 3), "peak_TOPs": PEAK_INT32_TOPS,
+            "frac": round(tops / PEAK_INT32_TOPS, 4)}
 
 
 def log(*a):
@@ -304,6 +326,7 @@ def main():
             "hbm_roofline": {"bound": "hbm", "algorithmic_bytes": in_bytes + out_bytes,
                              "achieved_GBs": round((in_bytes + out_bytes) / (wk * 1e-3) / 1e9, 2) if wk else None,
                              "peak_GBs": PEAK_HBM_GBS},
+            "valu_roofline": wfa_valu_roofline(wf, sc, wk),
             "mismatches": wbad, "checked": int(len(wsample)),
             "tcups_ref_style": round(float((wf.a_len.astype(np.int64) * wf.b_len).sum())
                                      / (wwall / args.steps) / 1e12, 4),
@@ -342,6 +365,7 @@ def main():
             "ms_per_step": round(mwall * 1000.0 / args.steps, 3),
             "p50_ms": pctl(mstep, 50), "p99_ms": pctl(mstep, 99),
             "kernel_ms": round(mk, 3) if mk else None,
+            "valu_roofline": wfa_valu_roofline(wm, sc, mk),
             "resolved": int((sc >= 0).sum()), "mismatches": mbad, "checked": int(len(msample)),
             "autotuned_waves_per_class": tuned_waves,
         }
