@@ -66,6 +66,89 @@ __device__ __forceinline__ uint64_t load8(const uint8_t* p)
 
 constexpr int kSwProfileInts = 32 * 64;
 
+// One lane's running state in a strip.
+struct SwLane {
+    int h, e, f;        // this lane's last row
+    int hl_prev;        // H[i-1][j-1]
+    int bHv, bIv;       // first strict maximum of the lane's column in this strip
+};
+
+// Eight steps t0 .. t0+7 of a strip (rows t0 - lane .. t0 - lane + 7).  Lane 0
+// takes its left column from bHr/bEr (wave-uniform LDS or global reads), lane
+// 63 hands its column to the next strip through bHw/bEw.  Returns the 8
+// direction nibbles of the lane (TRACE).
+template <bool TRACE, bool MATRIX>
+__device__ __forceinline__ uint32_t sw_chunk(SwLane& S, int t0, int lane, int m, bool jvalid, uint8_t pc,
+                                             uint64_t tcur, const int* bHr, const int* bEr, int* bHw,
+                                             int* bEw, const int* prof, int go, int ge, int sm, int sx)
+{
+    uint32_t acc = 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int i = t0 + u - lane;         // 0-based row of this step
+        // Left neighbour's H and E, one step old, by a whole-wave shift
+        // right (DPP wave_shr:1); lane 0 receives the strip boundary
+        // (row t0+u), read at a wave-uniform address.
+        const int i0 = t0 + u;
+        const int bh = i0 < m ? bHr[i0] : 0;
+        const int be = i0 < m ? bEr[i0] : 0;
+        const int hl = __builtin_amdgcn_update_dpp(bh, S.h, 0x138, 0xf, 0xf, false);
+        const int el = __builtin_amdgcn_update_dpp(be, S.e, 0x138, 0xf, 0xf, false);
+        const int diag = S.hl_prev;
+        S.hl_prev = hl;
+        const bool active = (i >= 0) && (i < m) && jvalid;
+        if (active) {
+            const uint8_t tc = uint8_t(tcur >> (8 * u));
+            int ee = hl - go;
+            uint32_t code = 0;
+            { const int e2 = el - ge; if (e2 > ee) { ee = e2; code |= 4u; } }
+            int ff = S.h - go;
+            { const int f2 = S.f - ge; if (f2 > ff) { ff = f2; code |= 8u; } }
+            int hv = diag + (MATRIX ? prof[(tc & 31) * 64 + lane] : (tc == pc ? sm : sx));
+            uint32_t hsrc = 1u;
+            if (ee > hv) { hv = ee; hsrc = 2u; }
+            if (ff > hv) { hv = ff; hsrc = 3u; }
+            if (hv <= 0) { hv = 0; hsrc = 0u; }
+            S.h = hv; S.e = ee; S.f = ff;
+            if (hv > S.bHv) { S.bHv = hv; S.bIv = i; }
+            if (TRACE) acc |= (code | hsrc) << (4 * u);
+            if (lane == 63) { bHw[i] = hv; bEw[i] = ee; }
+        }
+    }
+    return acc;
+}
+
+// A strip's first maximum folded into the lane's record (strips in order:
+// a later strip wins only with a greater value or the same value on an
+// earlier row).
+__device__ __forceinline__ void sw_fold(const SwLane& S, int j, int& gH, int& gI, int& gJ)
+{
+    if (S.bHv > gH || (S.bHv == gH && S.bHv > 0 && S.bIv < gI)) { gH = S.bHv; gI = S.bIv; gJ = j; }
+}
+
+// Both end cells from the per-lane records, and the result row of the pair.
+__device__ __forceinline__ void sw_finish(int gH, int gI, int gJ, int lane, int32_t* res)
+{
+    // Score-only end: the reference's xor-butterfly argmax at width 64.
+    int lH = gH, lI = gI, lJ = gJ;
+    // Trace end: the row-major first cell among lanes holding the maximum.
+    int rH = gH, rI = gI, rJ = gJ;
+    for (int off = 32; off > 0; off >>= 1) {
+        const int ob = __shfl_xor(lH, off), oi = __shfl_xor(lI, off), oj = __shfl_xor(lJ, off);
+        if (ob > lH) { lH = ob; lI = oi; lJ = oj; }
+        const int rb = __shfl_xor(rH, off), ri = __shfl_xor(rI, off), rj = __shfl_xor(rJ, off);
+        if (rb > rH || (rb == rH && rb > 0 && (ri < rI || (ri == rI && rj < rJ)))) {
+            rH = rb; rI = ri; rJ = rj;
+        }
+    }
+    if (lane == 0) {
+        res[0] = lH;          // lH == rH: both are the maximum
+        res[1] = rI; res[2] = rJ;
+        res[3] = lI; res[4] = lJ;
+        res[5] = -1; res[6] = -1; res[7] = 0;
+    }
+}
+
 template <bool TRACE, bool LDS_BND, bool MATRIX>
 __global__ __launch_bounds__(64) void sw_kernel(SwLaunch L)
 {
@@ -104,6 +187,8 @@ __global__ __launch_bounds__(64) void sw_kernel(SwLaunch L)
     uint32_t* dirs = nullptr;
     if (TRACE) dirs = L.dirs + dir_base(L, pid, slot);
     int gH = 0, gI = -1, gJ = -1;
+    // text bytes (or residue codes) for rows t - lane .. t - lane + 7
+    const uint8_t* TS = MATRIX ? L.tcode + L.toff[pid] : T;
 
     for (int s = 0; s < nstrips; ++s) {
         const int j = (s << 6) + lane;               // 0-based column of this lane
@@ -111,73 +196,125 @@ __global__ __launch_bounds__(64) void sw_kernel(SwLaunch L)
         const uint8_t pc = jvalid ? (MATRIX ? L.pcode[L.poff[pid] + j] : P[j]) : uint8_t(0);
         if (MATRIX)
             for (int c = 0; c < 32; ++c) prof[c * 64 + lane] = L.sub[c * 32 + pc];
-        int h = 0, e = 0, f = 0;                     // this lane's last row
-        int hl_prev = 0;                             // H[i-1][j-1]
-        int bHv = 0, bIv = -1;
-        // text bytes for rows t - lane .. t - lane + 7, prefetched a chunk ahead
-        const uint8_t* TS = MATRIX ? L.tcode + L.toff[pid] : T;   // bytes or residue codes
-        uint64_t tcur = load8(TS - lane);
+        SwLane S{0, 0, 0, 0, 0, -1};
+        uint64_t tcur = load8(TS - lane);            // prefetched a chunk ahead
         for (int c = 0; c < nchunks; ++c) {
             const int t0 = c << 3;
             const uint64_t tnext = load8(TS + t0 + 8 - lane);
-            uint32_t acc = 0;
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int i = t0 + u - lane;         // 0-based row of this step
-                // Left neighbour's H and E, one step old, by a whole-wave shift
-                // right (DPP wave_shr:1); lane 0 receives the strip boundary
-                // (row t0+u), read at a wave-uniform LDS address.
-                const int i0 = t0 + u;
-                const int bh = i0 < m ? bH[i0] : 0;
-                const int be = i0 < m ? bE[i0] : 0;
-                const int hl = __builtin_amdgcn_update_dpp(bh, h, 0x138, 0xf, 0xf, false);
-                const int el = __builtin_amdgcn_update_dpp(be, e, 0x138, 0xf, 0xf, false);
-                const int diag = hl_prev;
-                hl_prev = hl;
-                const bool active = (i >= 0) && (i < m) && jvalid;
-                if (active) {
-                    const uint8_t tc = uint8_t(tcur >> (8 * u));
-                    int ee = hl - go;
-                    uint32_t code = 0;
-                    { const int e2 = el - ge; if (e2 > ee) { ee = e2; code |= 4u; } }
-                    int ff = h - go;
-                    { const int f2 = f - ge; if (f2 > ff) { ff = f2; code |= 8u; } }
-                    int hv = diag + (MATRIX ? prof[(tc & 31) * 64 + lane] : (tc == pc ? sm : sx));
-                    uint32_t hsrc = 1u;
-                    if (ee > hv) { hv = ee; hsrc = 2u; }
-                    if (ff > hv) { hv = ff; hsrc = 3u; }
-                    if (hv <= 0) { hv = 0; hsrc = 0u; }
-                    h = hv; e = ee; f = ff;
-                    if (hv > bHv) { bHv = hv; bIv = i; }
-                    if (TRACE) acc |= (code | hsrc) << (4 * u);
-                    if (lane == 63) { bH[i] = hv; bE[i] = ee; }
-                }
-            }
+            const uint32_t acc = sw_chunk<TRACE, MATRIX>(S, t0, lane, m, jvalid, pc, tcur, bH, bE, bH, bE,
+                                                         prof, go, ge, sm, sx);
             if (TRACE) dirs[(int64_t(s) * nchunks + c) * 64 + lane] = acc;
             tcur = tnext;
         }
-        if (bHv > gH || (bHv == gH && bHv > 0 && bIv < gI)) { gH = bHv; gI = bIv; gJ = j; }
+        sw_fold(S, j, gH, gI, gJ);
         __syncthreads();
     }
+    sw_finish(gH, gI, gJ, lane, res);
+}
 
-    // Score-only end: the reference's xor-butterfly argmax at width 64.
-    int lH = gH, lI = gI, lJ = gJ;
-    // Trace end: the row-major first cell among lanes holding the maximum.
-    int rH = gH, rI = gI, rJ = gJ;
-    for (int off = 32; off > 0; off >>= 1) {
-        const int ob = __shfl_xor(lH, off), oi = __shfl_xor(lI, off), oj = __shfl_xor(lJ, off);
-        if (ob > lH) { lH = ob; lI = oi; lJ = oj; }
-        const int rb = __shfl_xor(rH, off), ri = __shfl_xor(rI, off), rj = __shfl_xor(rJ, off);
-        if (rb > rH || (rb == rH && rb > 0 && (ri < rI || (ri == rI && rj < rJ)))) {
-            rH = rb; rI = ri; rJ = rj;
+// Substitution-matrix pairs with several waves per pair.  The strips of one
+// pair run as a pipeline over the kMwWaves waves of a workgroup: wave w takes
+// strips w, w + kMwWaves, ..., strip s starting kMwLag 8-row chunks after
+// strip s - 1 (lane 63 of strip s - 1 hands row r over at step r + 63, lane 0
+// of strip s needs it at step r) and no earlier than its wave is free; one
+// barrier per chunk keeps the waves in step.  Strip s reads the boundary
+// column buffer (s + 1) & 1 and writes buffer s & 1: strip s + 2 overwrites a
+// row of that buffer only after strip s + 1 has read it.  Each strip computes
+// exactly what the one-wave kernel computes for it, so results, direction
+// words and end cells are the same; the per-lane first-maximum records of the
+// waves are merged by the same order (value, then row, then column).
+constexpr int kMwWaves = 4;
+constexpr int kMwLag = 9;           // > 63 / 8 + 1 chunks
+constexpr size_t kSwMwMaxLds = 96 * 1024;
+
+__host__ __device__ inline size_t sw_mw_lds_bytes(int max_m)
+{
+    return size_t(kMwWaves) * kSwProfileInts * 4 + size_t(4) * size_t(max_m) * 4 + size_t(kMwWaves) * 64 * 12;
+}
+
+template <bool TRACE>
+__global__ __launch_bounds__(64 * kMwWaves) void sw_matrix_mw_kernel(SwLaunch L)
+{
+    extern __shared__ int lds[];
+    const int slot = blockIdx.x;
+    const int pid = L.first_pair + slot;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int m = L.tlen[pid];
+    const int n = L.plen[pid];
+    int32_t* res = L.res + int64_t(pid) * 8;
+    if (m <= 0 || n <= 0) {                         // the whole workgroup leaves
+        if (tid == 0) {
+            res[0] = 0;
+            for (int q = 1; q < 7; ++q) res[q] = -1;
+            res[7] = 0;
         }
+        return;
     }
-    if (lane == 0) {
-        res[0] = lH;          // lH == rH: both are the maximum
-        res[1] = rI; res[2] = rJ;
-        res[3] = lI; res[4] = lJ;
-        res[5] = -1; res[6] = -1; res[7] = 0;
+    int* prof = lds + w * kSwProfileInts;
+    int* bnd = lds + kMwWaves * kSwProfileInts;     // [buffer 0, 1][H, E][max_m]
+    int* rec = bnd + 4 * L.max_m;                   // [wave][lane][H, I, J]
+    const int mm = L.max_m;
+    for (int q = tid; q < m; q += 64 * kMwWaves) { bnd[2 * mm + q] = 0; bnd[3 * mm + q] = 0; }
+    __syncthreads();
+
+    const int go = L.go, ge = L.ge;
+    const int nstrips = (n + 63) >> 6;
+    const int nchunks = (m + 63 + 7) >> 3;
+    const int period = nchunks > kMwWaves * kMwLag ? nchunks : kMwWaves * kMwLag;
+    const int last = nstrips - 1;
+    const int total = (last / kMwWaves) * period + (last % kMwWaves) * kMwLag + nchunks;
+    uint32_t* dirs = nullptr;
+    if (TRACE) dirs = L.dirs + dir_base(L, pid, slot);
+    const uint8_t* TS = L.tcode + L.toff[pid];
+    int gH = 0, gI = -1, gJ = -1;
+    SwLane S{0, 0, 0, 0, 0, -1};
+    int j = 0;
+    bool jvalid = false;
+    uint64_t tcur = 0;
+    for (int G = 0; G < total; ++G) {
+        // this wave's strip and chunk at pipeline step G (wave-uniform)
+        int s = -1, c = 0;
+        if (G >= w * kMwLag) {
+            const int r = (G - w * kMwLag) / period;
+            const int cc = G - w * kMwLag - r * period;
+            if (cc < nchunks && r * kMwWaves + w < nstrips) { s = r * kMwWaves + w; c = cc; }
+        }
+        if (s >= 0) {
+            if (c == 0) {
+                j = (s << 6) + lane;
+                jvalid = j < n;
+                const uint8_t pc = jvalid ? L.pcode[L.poff[pid] + j] : uint8_t(0);
+                for (int k = 0; k < 32; ++k) prof[k * 64 + lane] = L.sub[k * 32 + pc];
+                S = SwLane{0, 0, 0, 0, 0, -1};
+                tcur = load8(TS - lane);
+            }
+            const int t0 = c << 3;
+            const uint64_t tnext = load8(TS + t0 + 8 - lane);
+            const int* bHr = bnd + ((s + 1) & 1) * 2 * mm;
+            int* bHw = bnd + (s & 1) * 2 * mm;
+            const uint32_t acc = sw_chunk<TRACE, true>(S, t0, lane, m, jvalid, 0, tcur, bHr, bHr + mm, bHw,
+                                                       bHw + mm, prof, go, ge, 0, 0);
+            if (TRACE) dirs[(int64_t(s) * nchunks + c) * 64 + lane] = acc;
+            tcur = tnext;
+            if (c == nchunks - 1) sw_fold(S, j, gH, gI, gJ);
+        }
+        __syncthreads();
     }
+    // merge the waves' per-lane records (each is the first maximum over its
+    // own strips; the order value, row, column makes the merge exact)
+    rec[(w * 64 + lane) * 3 + 0] = gH;
+    rec[(w * 64 + lane) * 3 + 1] = gI;
+    rec[(w * 64 + lane) * 3 + 2] = gJ;
+    __syncthreads();
+    if (w != 0) return;
+    for (int v = 1; v < kMwWaves; ++v) {
+        const int h = rec[(v * 64 + lane) * 3 + 0];
+        const int i = rec[(v * 64 + lane) * 3 + 1];
+        const int jj = rec[(v * 64 + lane) * 3 + 2];
+        if (h > gH || (h == gH && h > 0 && (i < gI || (i == gI && jj < gJ)))) { gH = h; gI = i; gJ = jj; }
+    }
+    sw_finish(gH, gI, gJ, lane, res);
 }
 
 // One thread per pair: the reference's walk over the nibble table, writing the
@@ -277,7 +414,22 @@ hipError_t sw_launch(const SwLaunch& L, bool trace, hipStream_t st, LaunchHooks*
     const size_t bnd_bytes = size_t(2) * size_t(L.max_m) * sizeof(int);
     const bool in_lds = L.bnd == nullptr;
     const size_t lds = (in_lds ? bnd_bytes : 0) + (matrix ? size_t(kSwProfileInts) * sizeof(int) : 0);
+    // substitution-matrix batches (no re-run slots) take the multi-wave
+    // pipeline while both boundary buffers fit the LDS next to the profiles
+    const size_t mw_lds = sw_mw_lds_bytes(L.max_m);
+    const bool mw = matrix && !L.rerun && mw_lds <= kSwMwMaxLds;
     if (hooks) hooks->before(st);
+    if (mw) {
+        const dim3 grid(unsigned(L.n)), block(64 * kMwWaves);
+        if (trace) hipLaunchKernelGGL(sw_matrix_mw_kernel<true>, grid, block, mw_lds, st, L);
+        else       hipLaunchKernelGGL(sw_matrix_mw_kernel<false>, grid, block, mw_lds, st, L);
+        if (hooks) hooks->after(st);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess || !trace) return e;
+        const dim3 wgrid(unsigned((L.n + 63) / 64)), wblock(64);
+        hipLaunchKernelGGL(sw_walk_kernel, wgrid, wblock, 0, st, L);
+        return hipGetLastError();
+    }
     if (trace) {
         if (matrix) launch_forward<true, true>(L, in_lds, lds, st);
         else        launch_forward<true, false>(L, in_lds, lds, st);

@@ -369,6 +369,39 @@ def test_protein_sw_blosum62_vs_oracle(gpu, oracle):
         assert (r.score, r.end_i, r.end_j) == oracle.sw_score_lane(t, p, tab, 11, 1), (t, p)
 
 
+def test_protein_sw_long_queries(gpu, oracle):
+    """Queries of 10..16 strips (several rounds of the multi-wave pipeline) over
+    short texts (pipeline period set by the wave lag) and long ones (set by the
+    text length): trace and score-only results equal the oracle's."""
+    import genoaligner_amd as ga
+    m = ga.blosum62(11, 1)
+    alpha = m.alphabet.decode()
+    S = [[m.score[a][b] for b in range(20)] for a in range(20)]
+    tab = oracle.matrix_table(alpha, S, -1)
+    rng = random.Random(43)
+    pairs = []
+    for k in range(24):
+        p = _protein(rng, rng.randint(600, 1000))
+        if k % 2:
+            a = rng.randint(0, len(p) - 200)
+            t = _protein(rng, rng.randint(0, 400)) + _mutate(rng, p[a: a + 200], 0.2, alpha) + \
+                _protein(rng, rng.randint(0, 800))
+        else:
+            t = _protein(rng, rng.randint(20, 120))
+        pairs.append((t, p))
+    for cigar in (True, False):
+        reqs = [ga.SWRequest(text=t, text_len=len(t), pattern=p, pattern_len=len(p), with_cigar=cigar)
+                for t, p in pairs]
+        b = ga.align_sw_matrix_batch(reqs, m)
+        assert b.ok(), b.error
+        for (t, p), r in zip(pairs, b.results):
+            if cigar:
+                assert (r.score, r.start_i, r.start_j, r.end_i, r.end_j, r.cigar) == \
+                       oracle.sw_trace(t, p, tab, 11, 1), (len(t), len(p))
+            else:
+                assert (r.score, r.end_i, r.end_j) == oracle.sw_score_lane(t, p, tab, 11, 1), (len(t), len(p))
+
+
 def test_wfa_graph_replays_match_eager(gpu, oracle):
     """The device-planned submission captured into a HIP graph: every replay
     (outputs reset between replays, eager work of the program in between)
