@@ -115,12 +115,15 @@ def check_wfa(ps, dev, sample, oracle):
     return bad
 
 
-def cpu_baseline_sw(ps, seconds, threads, oracle):
+def cpu_baseline_sw(ps, seconds, threads, oracle, tab=None, gaps=None,
+                    label="256 x 1024, score + span + CIGAR"):
     """The oracle's SW trace path on `threads` host threads over a sample of the
-    same workload, sized to run about `seconds`."""
-    import ctypes as C
+    same workload, sized to run about `seconds` (score table and gap costs of
+    the DNA scheme unless given)."""
     L = oracle.lib()
-    tab = oracle.mm_table(SW_SCHEME[0], SW_SCHEME[1])
+    if tab is None:
+        tab = oracle.mm_table(SW_SCHEME[0], SW_SCHEME[1])
+    go, ge = gaps if gaps else (SW_SCHEME[2], SW_SCHEME[3])
 
     def run(n):
         sub = ps.subset(np.arange(n))
@@ -132,7 +135,7 @@ def cpu_baseline_sw(ps, seconds, threads, oracle):
         t0 = time.perf_counter()
         rc = L.ga_oracle_sw_batch(sub.a.ctypes.data, sub.a_off.ctypes.data, sub.a_len.ctypes.data,
                                   sub.b.ctypes.data, sub.b_off.ctypes.data, sub.b_len.ctypes.data,
-                                  n, tab.ctypes.data, SW_SCHEME[2], SW_SCHEME[3], 1,
+                                  n, tab.ctypes.data, go, ge, 1,
                                   out.ctypes.data, cig.ctypes.data, coff.ctypes.data,
                                   clen.ctypes.data, threads)
         assert rc == 0
@@ -148,7 +151,7 @@ def cpu_baseline_sw(ps, seconds, threads, oracle):
     dt, cells = run(n)
     return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads,
             "kind": "port",
-            "sample": f"{n} of the {ps.n} pairs (256 x 1024, score + span + CIGAR), "
+            "sample": f"{n} of the {ps.n} pairs ({label}), "
                       f"{dt:.1f} s on {threads} threads of the benchmark host"}
 
 
@@ -332,6 +335,10 @@ def main():
                                      / (wwall / args.steps) / 1e12, 4),
         }
         log(f"wfa: {wfa_line['pairs_per_s']:.0f} pairs/s, {wbad} mismatches")
+        if not args.no_cpu_baseline:
+            wfa_line["cpu_baseline"] = cpu_baseline_wfa(wf, np.ones(wf.n, dtype=np.uint8), args.cpu_seconds,
+                                                        min(os.cpu_count() or 1, 16), O)
+            log(f"wfa cpu baseline: {wfa_line['cpu_baseline']}")
 
     # ---------------- WFA, mixed lengths / distances ----------------
     mixed_line = None
@@ -469,6 +476,11 @@ def main():
             "mismatches": pbad, "checked": 20,
         }
         log(f"protein: {protein_line}")
+        if not args.no_cpu_baseline:
+            protein_line["cpu_baseline"] = cpu_baseline_sw(
+                pp, args.cpu_seconds, min(os.cpu_count() or 1, 16), O, tab=tab, gaps=(11, 1),
+                label="protein, BLOSUM62 11/1, score + span + CIGAR")
+            log(f"protein cpu baseline: {protein_line['cpu_baseline']}")
 
     # ---------------- serving: Poisson arrivals, continuous batching ----------------
     serve_line = None
