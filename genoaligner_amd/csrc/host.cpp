@@ -344,11 +344,19 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
     if (nc == 0) return nullptr;
     int32_t* d_order = nullptr;
     int32_t* d_next = nullptr;
+    auto drop_queue = [&] {
+        if (d_order) (void)hipFreeAsync(d_order, st);
+        if (d_next) (void)hipFreeAsync(d_next, st);
+    };
     if (hipMallocAsync(reinterpret_cast<void**>(&d_order), plan.order.size() * 4 + 64, st) != hipSuccess ||
-        hipMallocAsync(reinterpret_cast<void**>(&d_next), nc * 4 + 64, st) != hipSuccess)
+        hipMallocAsync(reinterpret_cast<void**>(&d_next), nc * 4 + 64, st) != hipSuccess) {
+        drop_queue();
         return "hipMalloc(WFA queue)";
-    if (hipMemcpyAsync(d_order, plan.order.data(), plan.order.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+    }
+    if (hipMemcpyAsync(d_order, plan.order.data(), plan.order.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess) {
+        drop_queue();
         return "hipMemcpy(WFA queue)";
+    }
     // set-up first (launch shapes, stacks), so the timed region holds only GPU work
     const char* err = nullptr;
     std::vector<ga::WfaQueueLaunch> Ls(nc);
@@ -402,8 +410,7 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
     for (hipStream_t cs : side) release_stream(cs);
     for (auto& L : Ls)
         if (L.ws) (void)hipFreeAsync(L.ws, st);
-    (void)hipFreeAsync(d_order, st);
-    (void)hipFreeAsync(d_next, st);
+    drop_queue();
     return err;
 }
 
