@@ -15,6 +15,8 @@ import enum
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
 
+import numpy as np
+
 from . import _native as N
 
 SW_MAX_TRACE_CELLS = N.GA_SW_MAX_TRACE_CELLS
@@ -118,35 +120,99 @@ def _buf(b: Optional[bytes], keep: list) -> Optional[int]:
     return C.addressof(cb)
 
 
+def _np_dtype(cstruct) -> np.dtype:
+    """numpy dtype with the field offsets and size of a ctypes struct (nested
+    structs of int32 become int32 sub-arrays)."""
+    names, formats, offsets = [], [], []
+    for name, ctype in cstruct._fields_:
+        off = getattr(cstruct, name).offset
+        if ctype is C.c_void_p:
+            fmt = np.uint64
+        elif ctype is C.c_int32:
+            fmt = np.int32
+        elif ctype is C.c_int64:
+            fmt = np.int64
+        elif isinstance(ctype, type) and issubclass(ctype, C.Structure):
+            fmt = (np.int32, C.sizeof(ctype) // 4)
+        else:
+            raise TypeError(f"no numpy layout for field {name}")
+        names.append(name)
+        formats.append(fmt)
+        offsets.append(off)
+    return np.dtype({"names": names, "formats": formats, "offsets": offsets, "itemsize": C.sizeof(cstruct)})
+
+
+_ALIGN_REQ = _np_dtype(N.AlignReq)
+_ALIGN_RES = _np_dtype(N.AlignRes)
+_SW_REQ = _np_dtype(N.SWReq)
+_SW_RES = _np_dtype(N.SWRes)
+
+
+def _seq_table(seqs, lens, keep: list) -> np.ndarray:
+    """Addresses (uint64, 0 for None) of the sequences, copied back to back into
+    one buffer, each followed by zero bytes up to max(len, declared len) + 1."""
+    bs = [_bytes(x) for x in seqs]
+    parts, addr, off = [], np.zeros(len(bs), dtype=np.uint64), 0
+    offs = []
+    for b, ln in zip(bs, lens):
+        if b is None:
+            offs.append(-1)
+            continue
+        pad = max(int(ln) - len(b), 0) + 1
+        parts.append(b)
+        parts.append(bytes(pad))
+        offs.append(off)
+        off += len(b) + pad
+    blob = b"".join(parts)
+    keep.append(blob)
+    base = C.cast(C.c_char_p(blob), C.c_void_p).value or 0
+    o = np.asarray(offs, dtype=np.int64)
+    addr[o >= 0] = np.uint64(base) + o[o >= 0].astype(np.uint64)
+    return addr
+
+
+def _ptr(a: np.ndarray, cstruct):
+    return a.ctypes.data_as(C.POINTER(cstruct))
+
+
 def _wfa(reqs: Sequence[AlignRequest], ext: bool) -> BatchResult:
     n = len(reqs)
-    out = BatchResult(results=[AlignResult() for _ in range(n)])
+    out = BatchResult(results=[])
     if n == 0:
         return out
     keep: list = []
-    arr = (N.AlignReq * n)()
-    for i, r in enumerate(reqs):
-        arr[i] = N.AlignReq(_buf(_bytes(r.pattern), keep), int(r.pattern_len),
-                            _buf(_bytes(r.text), keep), int(r.text_len), int(r.smax),
-                            1 if r.with_cigar else 0)
-    res = (N.AlignRes * n)()
+    plen = [int(r.pattern_len) for r in reqs]
+    tlen = [int(r.text_len) for r in reqs]
+    arr = np.zeros(n, dtype=_ALIGN_REQ)
+    arr["pattern"] = _seq_table([r.pattern for r in reqs], plen, keep)
+    arr["pattern_len"] = plen
+    arr["text"] = _seq_table([r.text for r in reqs], tlen, keep)
+    arr["text_len"] = tlen
+    arr["smax"] = [int(r.smax) for r in reqs]
+    arr["with_cigar"] = [1 if r.with_cigar else 0 for r in reqs]
+    res = np.zeros(n, dtype=_ALIGN_RES)
     L = N.lib()
-    need = L.ga_cigar_arena_bytes_wfa(arr, n)
+    pa = _ptr(arr, N.AlignReq)
+    need = L.ga_cigar_arena_bytes_wfa(pa, n)
     arena = C.create_string_buffer(max(need, 1))
     rc_ok, rc_un, err = C.c_int32(), C.c_int32(), C.c_char_p()
     fn = L.ga_align_batch_ext if ext else L.ga_align_batch
-    rc = fn(arr, n, res, arena, need, C.byref(rc_ok), C.byref(rc_un), C.byref(err))
+    rc = fn(pa, n, _ptr(res, N.AlignRes), arena, need, C.byref(rc_ok), C.byref(rc_un), C.byref(err))
     out.status = Status(rc)
     out.error = err.value.decode() if err.value else None
     if rc != N.GA_OK:
+        out.results = [AlignResult() for _ in range(n)]
         return out
     raw = arena.raw
-    for i in range(n):
-        r, o = res[i], out.results[i]
-        o.score, o.resolved = r.score, bool(r.resolved)
-        if r.cigar_len > 0:
-            o.cigar = raw[r.cigar_off: r.cigar_off + r.cigar_len].decode("latin-1")
-        o.rescore_ok, o.wellformed_ok = bool(r.rescore_ok), bool(r.wellformed_ok)
+    sc = res["score"].tolist()
+    rv = res["resolved"].tolist()
+    co = res["cigar_off"].tolist()
+    cl = res["cigar_len"].tolist()
+    rs = res["rescore_ok"].tolist()
+    wf = res["wellformed_ok"].tolist()
+    out.results = [AlignResult(sc[i], bool(rv[i]),
+                               raw[co[i]: co[i] + cl[i]].decode("latin-1") if cl[i] > 0 else "",
+                               bool(rs[i]), bool(wf[i])) for i in range(n)]
     out.resolved_count, out.unresolved_count = rc_ok.value, rc_un.value
     return out
 
@@ -175,38 +241,47 @@ def _scoring_c(s) -> N.SWScoringC:
 
 def _sw_call(reqs: Sequence[SWRequest], matrix=None) -> SWBatchResult:
     n = len(reqs)
-    out = SWBatchResult(results=[SWAlignResult() for _ in range(n)])
+    out = SWBatchResult(results=[])
     if n == 0:
         return out
     keep: list = []
-    arr = (N.SWReq * n)()
-    for i, r in enumerate(reqs):
-        arr[i] = N.SWReq(_buf(_bytes(r.text), keep), int(r.text_len),
-                         _buf(_bytes(r.pattern), keep), int(r.pattern_len),
-                         _scoring_c(r.scoring), 1 if r.with_cigar else 0)
-    res = (N.SWRes * n)()
+    tlen = [int(r.text_len) for r in reqs]
+    plen = [int(r.pattern_len) for r in reqs]
+    arr = np.zeros(n, dtype=_SW_REQ)
+    arr["text"] = _seq_table([r.text for r in reqs], tlen, keep)
+    arr["text_len"] = tlen
+    arr["pattern"] = _seq_table([r.pattern for r in reqs], plen, keep)
+    arr["pattern_len"] = plen
+    sc_of = [r.scoring for r in reqs]
+    arr["scoring"] = [(s.match, s.mismatch, s.gap_open, s.gap_extend) if isinstance(s, SWScoring)
+                      else tuple(int(x) for x in s) for s in sc_of]
+    arr["with_cigar"] = [1 if r.with_cigar else 0 for r in reqs]
+    res = np.zeros(n, dtype=_SW_RES)
     L = N.lib()
-    need = L.ga_cigar_arena_bytes_sw(arr, n)
+    pa = _ptr(arr, N.SWReq)
+    pr = _ptr(res, N.SWRes)
+    need = L.ga_cigar_arena_bytes_sw(pa, n)
     arena = C.create_string_buffer(max(need, 1))
     rc_ok, rc_un, err = C.c_int32(), C.c_int32(), C.c_char_p()
     if matrix is None:
-        rc = L.ga_align_sw_batch(arr, n, res, arena, need, C.byref(rc_ok), C.byref(rc_un), C.byref(err))
+        rc = L.ga_align_sw_batch(pa, n, pr, arena, need, C.byref(rc_ok), C.byref(rc_un), C.byref(err))
     else:
-        rc = L.ga_align_sw_matrix_batch(arr, n, C.byref(matrix), res, arena, need, C.byref(rc_ok),
+        rc = L.ga_align_sw_matrix_batch(pa, n, C.byref(matrix), pr, arena, need, C.byref(rc_ok),
                                         C.byref(rc_un), C.byref(err))
     out.status = Status(rc)
     out.error = err.value.decode() if err.value else None
     if rc != N.GA_OK:
+        out.results = [SWAlignResult() for _ in range(n)]
         return out
     raw = arena.raw
-    for i in range(n):
-        r, o = res[i], out.results[i]
-        o.score, o.resolved = r.score, bool(r.resolved)
-        if r.cigar_len > 0:
-            o.cigar = raw[r.cigar_off: r.cigar_off + r.cigar_len].decode("latin-1")
-        o.start_i, o.start_j, o.end_i, o.end_j = r.start_i, r.start_j, r.end_i, r.end_j
-        o.rescore_ok, o.wellformed_ok = bool(r.rescore_ok), bool(r.wellformed_ok)
-        o.too_large = bool(r.too_large)
+    f = {k: res[k].tolist() for k in ("score", "resolved", "cigar_off", "cigar_len", "start_i", "start_j",
+                                       "end_i", "end_j", "rescore_ok", "wellformed_ok", "too_large")}
+    out.results = [SWAlignResult(
+        f["score"][i], bool(f["resolved"][i]),
+        raw[f["cigar_off"][i]: f["cigar_off"][i] + f["cigar_len"][i]].decode("latin-1") if f["cigar_len"][i] > 0
+        else "",
+        f["start_i"][i], f["start_j"][i], f["end_i"][i], f["end_j"][i], bool(f["rescore_ok"][i]),
+        bool(f["wellformed_ok"][i]), bool(f["too_large"][i])) for i in range(n)]
     out.resolved_count, out.unresolved_count = rc_ok.value, rc_un.value
     return out
 
