@@ -97,19 +97,21 @@ __device__ __forceinline__ bool is_acgt(uint32_t b) { return b == 'A' || b == 'C
 // (v_alignbit_b32).
 __device__ __forceinline__ int lds_match_run(const uint32_t* P, int v, const uint32_t* T, int h, int lim)
 {
-    // word pointers and bit shifts fixed for the run: one step is two word
-    // pairs, two funnel shifts and a compare
+    // word pointers and bit shifts fixed for the run; the first probe is
+    // straight-line code (most runs end in it), the loop only continues runs
     const uint32_t* p = P + (v >> 4);
     const uint32_t* t = T + (h >> 4);
     const uint32_t sp = (uint32_t(v) & 15u) * 2u, st = (uint32_t(h) & 15u) * 2u;
+    uint32_t x = __builtin_amdgcn_alignbit(p[1], p[0], sp) ^ __builtin_amdgcn_alignbit(t[1], t[0], st);
     int run = 0;
-    uint32_t x;
-    for (;;) {
-        x = __builtin_amdgcn_alignbit(p[1], p[0], sp) ^ __builtin_amdgcn_alignbit(t[1], t[0], st);
-        if (x != 0 || run + 16 >= lim) break;
-        run += 16;
-        ++p;
-        ++t;
+    if (x == 0 && 16 < lim) {
+        for (;;) {
+            run += 16;
+            ++p;
+            ++t;
+            x = __builtin_amdgcn_alignbit(p[1], p[0], sp) ^ __builtin_amdgcn_alignbit(t[1], t[0], st);
+            if (x != 0 || run + 16 >= lim) break;
+        }
     }
     run += x ? int(__builtin_ctz(x) >> 1) : 16;
     return run < lim ? run : lim;
@@ -176,6 +178,7 @@ __device__ __forceinline__ int wfa_forward(const uint32_t* sP, const uint32_t* s
     Off* prev = A;
     Off* cur = B;
     for (int s = 1; s <= smax; ++s) {
+        Off* wrow = TRACE ? ws + int64_t(s) * s : nullptr;   // row s of the stack (wave-uniform)
         for (int k = -s + tid; k <= s; k += NT) {
             const int ins = prev[k + S];
             const int mis = prev[k + S + 1];
@@ -196,7 +199,10 @@ __device__ __forceinline__ int wfa_forward(const uint32_t* sP, const uint32_t* s
                 mx = OffNull<Off>::v;
             }
             cur[k + S + 1] = Off(mx);
-            if (TRACE && trace) ws[int64_t(s) * s + k + s] = Off(mx);
+            // (a 32-bit byte offset from the wave-uniform row base: one
+            // scalar-base store, no per-lane 64-bit address)
+            if (TRACE && trace)
+                *reinterpret_cast<Off*>(reinterpret_cast<char*>(wrow) + uint32_t(k + s) * uint32_t(sizeof(Off))) = Off(mx);
         }
         __syncthreads();
         if (ak >= -s && ak <= s && int(cur[ak + S + 1]) >= n) return s;
