@@ -218,3 +218,32 @@ int main() {
     assert out[0] == "wfa 1 smax out of range [0, 511]"
     assert out[1] == "b62 ARNDCQEGHILKMFPSTWYV 11 -1 11"
     assert out[2].startswith("sw 1 SW matrix: alphabet")
+
+
+def test_request_marshalling():
+    """The Python layer's request arrays: the C structs' layout, each sequence
+    at its address followed by zero bytes up to its declared length, null for
+    None, str / bytes / bytearray alike."""
+    import ctypes as C
+    import numpy as np
+    from genoaligner_amd import api as A, _native as N
+    for cs, dt in ((N.AlignReq, A._ALIGN_REQ), (N.AlignRes, A._ALIGN_RES),
+                   (N.SWReq, A._SW_REQ), (N.SWRes, A._SW_RES)):
+        assert dt.itemsize == C.sizeof(cs)
+        for name, _ in cs._fields_:
+            assert dt.fields[name][1] == getattr(cs, name).offset, (cs.__name__, name)
+    keep = []
+    seqs = ["ACGT", None, b"GG", bytearray(b"T"), ""]
+    lens = [4, 0, 5, 1, 0]
+    addr = A._seq_table(seqs, lens, keep)
+    assert addr[1] == 0 and all(addr[i] != 0 for i in (0, 2, 3, 4))
+    assert C.string_at(int(addr[0]), 5) == b"ACGT\0"
+    assert C.string_at(int(addr[2]), 6) == b"GG\0\0\0\0"
+    assert C.string_at(int(addr[3]), 2) == b"T\0"
+    assert C.string_at(int(addr[4]), 1) == b"\0"
+    arr = np.zeros(2, dtype=A._SW_REQ)
+    arr["scoring"] = [(2, -3, 5, 2), (1, -1, 2, 1)]
+    arr["with_cigar"] = [1, 0]
+    p = C.cast(A._ptr(arr, N.SWReq), C.POINTER(N.SWReq))
+    assert (p[1].scoring.match, p[1].scoring.mismatch, p[1].scoring.gap_open, p[1].scoring.gap_extend,
+            p[1].with_cigar) == (1, -1, 2, 1, 0)
