@@ -243,26 +243,33 @@ void wfa_kernel(WfaQueueLaunch L)
         n_cls = __builtin_amdgcn_readfirstlane(L.dev_class[1]);
         order = L.order + __builtin_amdgcn_readfirstlane(L.dev_class[0]);
     }
+    // Static rounds first: in round r workgroup g takes pair r*G + g; the
+    // counter then starts at rounds*G.  Claims on the one counter serialize
+    // (≈ 12 ns each on this device), which caps the smallest class (smax <= 64,
+    // short pairs of similar cost): there a class with at most 8 pairs per
+    // workgroup is scheduled statically throughout and a larger one keeps only
+    // its last two rounds' worth of pairs for the counter.  The larger classes
+    // take one static round: their pairs' costs differ by orders of magnitude
+    // and a claim costs little next to a pair, so dynamic claims balance them.
+    const int rounds = S > 64 ? 1 : n_cls <= 8 * G ? (n_cls + G - 1) / G : n_cls / G - 2;
     for (int it = 0;; ++it) {
-        // The first round is static (workgroup g takes pair g); later pairs are
-        // claimed from the counter, which then starts at G.  No atomics at all
-        // when the class has no more pairs than workgroups.
-        // The claim has no branch on the lane id: wave 0 takes part in one
-        // atomic adding (lane == 0) and lane 0's result is read into a scalar
-        // register.  (With `if (lane == 0) q = atomicAdd(..)` and a shuffle,
-        // the compiler may thread the loop-invariant branch into the loop and
-        // leave lanes 1..63 looping on a stale claim while lane 0 is masked off.)
+        // The dynamic claim has no branch on the lane id: wave 0 takes part in
+        // one atomic adding (lane == 0) and lane 0's result is read into a
+        // scalar register.  (With `if (lane == 0) q = atomicAdd(..)` and a
+        // shuffle, the compiler may thread the loop-invariant branch into the
+        // loop and leave lanes 1..63 looping on a stale claim while lane 0 is
+        // masked off.)
         int q;
-        if (it == 0) {
-            q = int(blockIdx.x);
-        } else if (n_cls <= G) {
+        if (it < rounds) {
+            q = int(blockIdx.x) + it * G;
+        } else if (n_cls <= rounds * G) {
             break;
         } else if (WAVES == 1) {
-            q = G + __builtin_amdgcn_readfirstlane(atomicAdd(L.next, lane == 0 ? 1 : 0));
+            q = rounds * G + __builtin_amdgcn_readfirstlane(atomicAdd(L.next, lane == 0 ? 1 : 0));
         } else {
             if (wave == 0) claim[it & 1] = __builtin_amdgcn_readfirstlane(atomicAdd(L.next, lane == 0 ? 1 : 0));
             __syncthreads();
-            q = G + __builtin_amdgcn_readfirstlane(claim[it & 1]);
+            q = rounds * G + __builtin_amdgcn_readfirstlane(claim[it & 1]);
         }
         if (q >= n_cls) break;                    // every wave ends here once the queue is empty
         const int pid = order ? order[q] : q;
