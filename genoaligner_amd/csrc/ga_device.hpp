@@ -87,6 +87,8 @@ struct SketchLaunch {
     float* feat;             // n * kSketchDim
     int32_t* lower_bound;    // n
 };
+// Compute units of the current device (cached per device).
+int device_cus();
 hipError_t sketch_launch(const SketchLaunch& L, hipStream_t st);
 hipError_t mlp_forward(const float* params, const float* X, int64_t N, float* H1, float* H2, float* logits,
                        hipStream_t st);

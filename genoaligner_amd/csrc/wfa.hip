@@ -24,6 +24,8 @@
 // cooperatively, 64 bytes per store instruction.
 #include "ga_device.hpp"
 
+#include <atomic>
+
 #include <mutex>
 #include <vector>
 
@@ -515,20 +517,24 @@ __global__ __launch_bounds__(256) void plan_scatter_kernel(WfaDevPlan P)
 
 namespace ga {
 
-namespace {
-
 int device_cus()
 {
-    static int cus[64] = {};
+    // per device, looked up once (threads racing on the first lookup store the
+    // same value)
+    static std::atomic<int> cus[64] = {};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    if (cus[dev] == 0) {
+    int c = cus[dev].load(std::memory_order_relaxed);
+    if (c == 0) {
         hipDeviceProp_t p{};
-        cus[dev] = (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0)
-                       ? p.multiProcessorCount : 256;
+        c = (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0) ? p.multiProcessorCount
+                                                                                        : 256;
+        cus[dev].store(c, std::memory_order_relaxed);
     }
-    return cus[dev];
+    return c;
 }
+
+namespace {
 
 template <bool TRACE, typename Off, bool STAGED, int WAVES>
 hipError_t launch_one(const WfaQueueLaunch& L, size_t lds, hipStream_t st)
