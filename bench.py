@@ -348,13 +348,12 @@ def main():
         want = (np.random.default_rng(4).random(wm.n) < 0.7).astype(np.uint8)
         wmd = WfaDeviceBatch(wm, trace=True, want=want)
         log(f"wfa_mixed workload ready ({time.perf_counter() - t0:.1f} s)")
-        # the autotuner picks waves per workgroup for each size class on a
-        # sample of the batch, then the whole batch runs with that choice
+        # the autotuner picks waves per workgroup for each size class on the
+        # batch itself (a sample has too few pairs per workgroup in the large
+        # classes to rank the choices as the whole batch does), then the batch
+        # runs with that choice
         from genoaligner_amd.device import wfa_autotune, wfa_set_tuning
-        tidx = np.random.default_rng(8).choice(wm.n, min(20000, wm.n), replace=False)
-        tsample = WfaDeviceBatch(wm.subset(tidx), trace=True, want=want[tidx])
-        tuned_waves, tuned_ms = wfa_autotune(tsample, reps=2)
-        del tsample
+        tuned_waves, tuned_ms = wfa_autotune(wmd, reps=1)
         log(f"wfa_mixed autotune: waves per class {tuned_waves}, ms {tuned_ms}")
         mwall, mstep, mkern = time_steps(wmd.run, args.steps, args.warmup, lib)
         msample = rng.choice(wm.n, size=min(300, wm.n), replace=False)

@@ -329,11 +329,20 @@ int wfa_class_of(int bound)
     while (c < 3 && bound > kWfaClassBound[c]) ++c;
     return c;
 }
-int wfa_class_waves(int bound)
+// Waves per workgroup of a size class: the tuned value, else by class.  The
+// largest class takes 4 waves when it holds many pairs (throughput: more pairs
+// in flight per CU; measured 30 ms against 38 ms with 8 on the 200k mixed
+// batch) and 8 when it holds few or an unknown number, `count` < 0
+// (device-planned batches: the router's passes leave few pairs there), where
+// the batch's tail is a handful of long pairs and latency decides.
+int wfa_class_waves(int bound, int64_t count = -1)
 {
     const int t = g_tuned_waves[wfa_class_of(bound)].load(std::memory_order_relaxed);
     if (t == 1 || t == 2 || t == 4 || t == 8) return t;
-    return bound <= 64 ? 1 : bound <= 256 ? 2 : bound <= 1024 ? 4 : 8;
+    if (bound <= 64) return 1;
+    if (bound <= 256) return 2;
+    if (bound <= 1024) return 4;
+    return count >= int64_t(ga::device_cus()) * 24 ? 4 : 8;
 }
 constexpr size_t kWfaMaxStagedLds = 64 * 1024;
 
@@ -369,7 +378,7 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
         L.smax_cls = plan.bound[c];
         L.max_plen = plan.max_plen[c];
         L.max_tlen = plan.max_tlen[c];
-        L.waves = wfa_class_waves(L.smax_cls);
+        L.waves = wfa_class_waves(L.smax_cls, plan.count[c]);
         L.staged = true;
         if (ga::wfa_lds_bytes(L) > kWfaMaxStagedLds) L.staged = false;
         int groups = ga::wfa_queue_groups(L);
