@@ -247,13 +247,15 @@ void wfa_kernel(WfaQueueLaunch L)
     }
     // Static rounds first: in round r workgroup g takes pair r*G + g; the
     // counter then starts at rounds*G.  Claims on the one counter serialize
-    // (≈ 12 ns each on this device), which caps the smallest class (smax <= 64,
-    // short pairs of similar cost): there a class with at most 8 pairs per
-    // workgroup is scheduled statically throughout and a larger one keeps only
-    // its last two rounds' worth of pairs for the counter.  The larger classes
-    // take one static round: their pairs' costs differ by orders of magnitude
-    // and a claim costs little next to a pair, so dynamic claims balance them.
-    const int rounds = S > 64 ? 1 : n_cls <= 8 * G ? (n_cls + G - 1) / G : n_cls / G - 2;
+    // (≈ 12 ns each on this device), which caps a launch of short pairs of the
+    // smallest class (smax <= 64, sequences <= 1024): there a class with at
+    // most 8 pairs per workgroup is scheduled statically throughout and a
+    // larger one keeps only its last two rounds' worth of pairs for the
+    // counter.  Other launches take one static round: their pairs' costs differ
+    // by orders of magnitude (lengths up to the launch maximum) and a claim
+    // costs little next to a pair, so dynamic claims balance them.
+    const bool short_pairs = S <= 64 && L.max_plen <= 1024 && L.max_tlen <= 1024;
+    const int rounds = !short_pairs ? 1 : n_cls <= 8 * G ? (n_cls + G - 1) / G : n_cls / G - 2;
     for (int it = 0;; ++it) {
         // The dynamic claim has no branch on the lane id: wave 0 takes part in
         // one atomic adding (lane == 0) and lane 0's result is read into a
