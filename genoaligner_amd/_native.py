@@ -28,6 +28,7 @@ EXPORTS = (
     "ga_sketch_device", "ga_mlp_forward_device", "ga_mlp_backward_device", "ga_mlp_sgd_device",
     "ga_mlp_route_device", "ga_route_bucket_bound", "ga_wfa_routed_device",
     "ga_align_sw_matrix_batch", "ga_sw_matrix_blosum62", "ga_set_thread_workspace_budget",
+    "ga_trim_workspace",
     "ga_wfa_graph_create", "ga_wfa_graph_launch", "ga_wfa_graph_destroy",
     "ga_wfa_autotune", "ga_wfa_set_tuning", "ga_wfa_get_tuning",
 )
@@ -157,6 +158,8 @@ def lib() -> C.CDLL:
         L.ga_wfa_graph_destroy.restype = None
         L.ga_set_thread_workspace_budget.argtypes = [C.c_int64]
         L.ga_set_thread_workspace_budget.restype = None
+        L.ga_trim_workspace.argtypes = []
+        L.ga_trim_workspace.restype = None
         L.ga_route_bucket_bound.argtypes = [C.c_int32]
         L.ga_route_bucket_bound.restype = C.c_int32
         _lib = L

@@ -71,6 +71,168 @@ void release_stream(hipStream_t s)
     g_streams.push_back(s);
 }
 
+// ---------------------------------------------------------------------------
+// Workspace blocks.  Large per-call workspaces (wavefront stacks and direction
+// tables, up to tens of GB) come from a process-wide cache of device blocks,
+// not from the stream-ordered pool: allocating and freeing blocks of that size
+// through the pool took 0.3-2 s now and then (the runtime maps and unmaps
+// their memory) against 15 ms of kernels per call.  A block handed back
+// carries an event recorded on the returning stream after its last use, and
+// the next user's stream waits on that event before touching the block.
+// Small requests, and every request on a capturing stream, stay stream-ordered
+// allocations (a graph records those as its own allocation nodes).  Idle
+// blocks of a device are capped at 3/8 of its memory (the oldest go first);
+// ga_trim_workspace() frees them all.
+// ---------------------------------------------------------------------------
+constexpr size_t kWsCacheMin = size_t(32) << 20;
+
+struct WsBlock {
+    void* p;
+    size_t bytes;
+    int dev;
+    hipEvent_t done;
+};
+std::mutex g_ws_mu;
+std::vector<WsBlock> g_ws_idle;   // handed back, oldest first
+std::vector<WsBlock> g_ws_busy;   // handed out
+
+// sizes rounded up to a granularity of 1/16..1/32 of the request, so requests
+// that differ a little reuse one block
+size_t ws_round(size_t b)
+{
+    size_t g = size_t(2) << 20;
+    while (g * 16 < b) g <<= 1;
+    return (b + g - 1) / g * g;
+}
+
+// Free idle blocks: those of `dev` (every device if dev < 0) until the idle
+// bytes of that device are at most `keep`.
+void ws_trim(int dev, size_t keep)
+{
+    std::vector<WsBlock> drop;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        std::vector<size_t> idle_bytes;
+        for (const WsBlock& b : g_ws_idle) {
+            if (size_t(b.dev) >= idle_bytes.size()) idle_bytes.resize(size_t(b.dev) + 1, 0);
+            idle_bytes[size_t(b.dev)] += b.bytes;
+        }
+        std::vector<WsBlock> kept;
+        for (const WsBlock& b : g_ws_idle) {
+            size_t& held = idle_bytes[size_t(b.dev)];
+            if ((dev < 0 || b.dev == dev) && held > keep) {
+                held -= b.bytes;
+                drop.push_back(b);
+            } else {
+                kept.push_back(b);
+            }
+        }
+        g_ws_idle.swap(kept);
+    }
+    if (drop.empty()) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (const WsBlock& b : drop) {
+        if (b.dev != cur) (void)hipSetDevice(b.dev);
+        (void)hipEventSynchronize(b.done);
+        (void)hipEventDestroy(b.done);
+        (void)hipFree(b.p);
+        if (b.dev != cur) (void)hipSetDevice(cur);
+    }
+}
+
+size_t ws_idle_cap(int dev)
+{
+    static std::atomic<size_t> cap[64];
+    if (dev < 0 || dev >= 64) return 0;
+    size_t c = cap[dev].load(std::memory_order_relaxed);
+    if (c == 0) {
+        size_t free_b = 0, total = 0;
+        if (hipMemGetInfo(&free_b, &total) != hipSuccess) return 0;
+        c = total / 8 * 3;
+        cap[dev].store(c, std::memory_order_relaxed);
+    }
+    return c;
+}
+
+// Drop-in for hipMallocAsync(p, bytes, st) on the workspace paths.
+hipError_t ws_alloc(void** p, size_t bytes, hipStream_t st)
+{
+    *p = nullptr;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (bytes < kWsCacheMin || hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+        return hipMallocAsync(p, bytes, st);
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    {
+        // best fit among idle blocks of this device, at most twice the request
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        size_t best = g_ws_idle.size();
+        for (size_t i = 0; i < g_ws_idle.size(); ++i) {
+            const WsBlock& b = g_ws_idle[i];
+            if (b.dev == dev && b.bytes >= bytes && b.bytes / 2 <= bytes &&
+                (best == g_ws_idle.size() || b.bytes < g_ws_idle[best].bytes))
+                best = i;
+        }
+        if (best < g_ws_idle.size()) {
+            const WsBlock b = g_ws_idle[best];
+            e = hipStreamWaitEvent(st, b.done, 0);
+            if (e != hipSuccess) return e;
+            g_ws_idle.erase(g_ws_idle.begin() + std::ptrdiff_t(best));
+            g_ws_busy.push_back(b);
+            *p = b.p;
+            return hipSuccess;
+        }
+    }
+    WsBlock b{nullptr, ws_round(bytes), dev, nullptr};
+    e = hipMalloc(&b.p, b.bytes);
+    if (e != hipSuccess) {
+        // out of memory: give the idle blocks of this device back and retry once
+        (void)hipGetLastError();
+        ws_trim(dev, 0);
+        e = hipMalloc(&b.p, b.bytes);
+        if (e != hipSuccess) { (void)hipGetLastError(); return e; }
+    }
+    e = hipEventCreateWithFlags(&b.done, hipEventDisableTiming);
+    if (e != hipSuccess) { (void)hipFree(b.p); return e; }
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    g_ws_busy.push_back(b);
+    *p = b.p;
+    return hipSuccess;
+}
+
+// Drop-in for hipFreeAsync(p, st): a cached block goes back to the idle list
+// once an event marks the end of its use on `st`.
+hipError_t ws_free(void* p, hipStream_t st)
+{
+    if (!p) return hipSuccess;
+    WsBlock b{};
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        auto it = std::find_if(g_ws_busy.begin(), g_ws_busy.end(), [p](const WsBlock& x) { return x.p == p; });
+        if (it == g_ws_busy.end()) return hipFreeAsync(p, st);
+        b = *it;
+        g_ws_busy.erase(it);
+    }
+    hipError_t e = hipEventRecord(b.done, st);
+    if (e != hipSuccess) {
+        // no event to order the next use by: wait for the stream here
+        (void)hipStreamSynchronize(st);
+        (void)hipEventDestroy(b.done);
+        if (hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess) {
+            (void)hipFree(b.p);
+            return e;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        g_ws_idle.push_back(b);
+    }
+    ws_trim(b.dev, ws_idle_cap(b.dev));
+    return e;
+}
+
 // Stream-ordered device allocations freed together at the end of a call.
 struct DevArena {
     hipStream_t st = nullptr;
@@ -80,7 +242,7 @@ struct DevArena {
     {
         void* p = nullptr;
         const size_t bytes = std::max<size_t>(count * sizeof(T), 64);
-        if (hipMallocAsync(&p, bytes, st) != hipSuccess) { ok = false; return nullptr; }
+        if (ws_alloc(&p, bytes, st) != hipSuccess) { ok = false; return nullptr; }
         ptrs.push_back(p);
         return static_cast<T*>(p);
     }
@@ -94,7 +256,7 @@ struct DevArena {
     }
     void release()
     {
-        for (void* p : ptrs) (void)hipFreeAsync(p, st);
+        for (void* p : ptrs) (void)ws_free(p, st);
         ptrs.clear();
     }
 };
@@ -252,7 +414,12 @@ void sw_validate(const char* cg, int len, const char* text, int text_len, int st
 // bound; the stacks of one class launch stay under kWfaStackBudget.
 constexpr int kWfaClassBound[] = {ga::kWfaClassBound0, ga::kWfaClassBound1, ga::kWfaClassBound2, GA_MAX_SMAX_EXT};
 constexpr int kWfaNumClasses = 4;
-constexpr int64_t kWfaStackBudget = int64_t(24) << 30;   // bytes of stacks per class launch
+// 12 GiB: ~1,460 groups of the largest class (8.4 MB stacks each).  The 200k
+// mixed batch runs as fast as with 24 GiB and slows below 8 GiB (30.7 ms at 8
+// to 24, 34.7 ms at 6, 47 ms at 4); batches whose largest class holds few
+// pairs gain from the lower cap (the routing batch 15.6 -> 14.9 ms), since
+// the other classes, on their own streams, get more of the chip.
+constexpr int64_t kWfaStackBudget = int64_t(12) << 30;   // bytes of stacks per class launch
 // Per-thread cap on the workspace of one batch call (0: the defaults above);
 // a serving loop gives each of its concurrent slots a share of its budget.
 thread_local int64_t t_workspace_budget = 0;
@@ -354,11 +521,11 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
     int32_t* d_order = nullptr;
     int32_t* d_next = nullptr;
     auto drop_queue = [&] {
-        if (d_order) (void)hipFreeAsync(d_order, st);
-        if (d_next) (void)hipFreeAsync(d_next, st);
+        if (d_order) (void)ws_free(d_order, st);
+        if (d_next) (void)ws_free(d_next, st);
     };
-    if (hipMallocAsync(reinterpret_cast<void**>(&d_order), plan.order.size() * 4 + 64, st) != hipSuccess ||
-        hipMallocAsync(reinterpret_cast<void**>(&d_next), nc * 4 + 64, st) != hipSuccess) {
+    if (ws_alloc(reinterpret_cast<void**>(&d_order), plan.order.size() * 4 + 64, st) != hipSuccess ||
+        ws_alloc(reinterpret_cast<void**>(&d_next), nc * 4 + 64, st) != hipSuccess) {
         drop_queue();
         return "hipMalloc(WFA queue)";
     }
@@ -386,7 +553,7 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
         if (trace) groups = int(std::max<int64_t>(1, std::min<int64_t>(groups, wfa_stack_budget() / per_group)));
         L.n_groups = groups;
         L.ws = nullptr;
-        if (trace && hipMallocAsync(&L.ws, size_t(per_group) * size_t(groups), st) != hipSuccess)
+        if (trace && ws_alloc(&L.ws, size_t(per_group) * size_t(groups), st) != hipSuccess)
             err = "hipMalloc(wavefront stacks)";
     }
     if (timer && !err) timer->begin(st);
@@ -418,7 +585,7 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
     if (fork) (void)hipEventDestroy(fork);
     for (hipStream_t cs : side) release_stream(cs);
     for (auto& L : Ls)
-        if (L.ws) (void)hipFreeAsync(L.ws, st);
+        if (L.ws) (void)ws_free(L.ws, st);
     drop_queue();
     return err;
 }
@@ -1222,7 +1389,7 @@ int ga_wfa_device(const ga_wfa_dev_batch* b, void* stream, const char** error)
         std::vector<void*> held;
         auto alloc = [&](size_t bytes) -> void* {
             void* p = nullptr;
-            if (hipMallocAsync(&p, bytes, st) != hipSuccess) return nullptr;
+            if (ws_alloc(&p, bytes, st) != hipSuccess) return nullptr;
             held.push_back(p);
             return p;
         };
@@ -1242,7 +1409,7 @@ int ga_wfa_device(const ga_wfa_dev_batch* b, void* stream, const char** error)
             err = enqueue_wfa_dev(C, st, side, ev);
             t_timer.end(st);
         }
-        for (void* p : held) (void)hipFreeAsync(p, st);
+        for (void* p : held) (void)ws_free(p, st);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (hipStream_t cs : side) release_stream(cs);
@@ -1359,11 +1526,11 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)
     hipStream_t st = static_cast<hipStream_t>(stream);
     std::vector<void*> owned;
     auto dalloc = [&](void** p, size_t bytes) {
-        if (hipMallocAsync(p, std::max<size_t>(bytes, 64), st) != hipSuccess) return false;
+        if (ws_alloc(p, std::max<size_t>(bytes, 64), st) != hipSuccess) return false;
         owned.push_back(*p);
         return true;
     };
-    auto release = [&] { for (void* p : owned) (void)hipFreeAsync(p, st); };
+    auto release = [&] { for (void* p : owned) (void)ws_free(p, st); };
     int32_t* d_res = nullptr;
     if (b->max_text_len <= ga::kSw16MaxRows && (!trace || b->max_pattern_len <= ga::kSw16MaxCols)) {
         // packed path: pairs (2w, 2w+1) share a wave
@@ -1537,12 +1704,12 @@ int ga_wfa_routed_device(const ga_wfa_dev_batch* b, const float* params, float q
     std::vector<void*> held;
     auto alloc = [&](size_t bytes) -> void* {
         void* p = nullptr;
-        if (hipMallocAsync(&p, bytes, st) != hipSuccess) return nullptr;
+        if (ws_alloc(&p, bytes, st) != hipSuccess) return nullptr;
         held.push_back(p);
         return p;
     };
     auto release = [&](size_t from) {
-        for (size_t k = from; k < held.size(); ++k) (void)hipFreeAsync(held[k], st);
+        for (size_t k = from; k < held.size(); ++k) (void)ws_free(held[k], st);
         held.resize(from);
     };
     const char* err = nullptr;
@@ -1615,6 +1782,8 @@ int ga_wfa_routed_device(const ga_wfa_dev_batch* b, const float* params, float q
 
 void ga_set_thread_workspace_budget(int64_t bytes) { t_workspace_budget = bytes > 0 ? bytes : 0; }
 
+void ga_trim_workspace(void) { ws_trim(-1, 0); }
+
 // ------------------------------------------------------------------ autotuner
 void ga_wfa_set_tuning(const ga_wfa_tuning* t)
 {
@@ -1654,8 +1823,8 @@ int ga_wfa_autotune(const ga_wfa_dev_batch* b, int reps, ga_wfa_tuning* out, voi
     int32_t* d_order = nullptr;
     int32_t* d_next = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void**>(&d_order), plan.order.size() * 4 + 64, st) != hipSuccess ||
-        hipMallocAsync(reinterpret_cast<void**>(&d_next), 64, st) != hipSuccess ||
+    if (ws_alloc(reinterpret_cast<void**>(&d_order), plan.order.size() * 4 + 64, st) != hipSuccess ||
+        ws_alloc(reinterpret_cast<void**>(&d_next), 64, st) != hipSuccess ||
         hipMemcpyAsync(d_order, plan.order.data(), plan.order.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         err = "autotune set-up";
@@ -1680,7 +1849,7 @@ int ga_wfa_autotune(const ga_wfa_dev_batch* b, int reps, ga_wfa_tuning* out, voi
             if (trace) groups = int(std::max<int64_t>(1, std::min<int64_t>(groups, wfa_stack_budget() / per_group)));
             L.n_groups = groups;
             L.ws = nullptr;
-            if (trace && hipMallocAsync(&L.ws, size_t(per_group) * size_t(groups), st) != hipSuccess) {
+            if (trace && ws_alloc(&L.ws, size_t(per_group) * size_t(groups), st) != hipSuccess) {
                 err = "hipMalloc(wavefront stacks)";
                 break;
             }
@@ -1694,7 +1863,7 @@ int ga_wfa_autotune(const ga_wfa_dev_batch* b, int reps, ga_wfa_tuning* out, voi
                 float ms = 0.0f;
                 if (r > 0 && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) tot += ms;
             }
-            if (L.ws) (void)hipFreeAsync(L.ws, st);
+            if (L.ws) (void)ws_free(L.ws, st);
             if (err) break;
             const float avg = tot / float(reps);
             if (best < 0.0f || avg < best) { best = avg; best_w = w; }
@@ -1702,8 +1871,8 @@ int ga_wfa_autotune(const ga_wfa_dev_batch* b, int reps, ga_wfa_tuning* out, voi
         t.waves[cls] = best_w;
         t.class_ms[cls] = best;
     }
-    if (d_order) (void)hipFreeAsync(d_order, st);
-    if (d_next) (void)hipFreeAsync(d_next, st);
+    if (d_order) (void)ws_free(d_order, st);
+    if (d_next) (void)ws_free(d_next, st);
     (void)hipStreamSynchronize(st);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);

@@ -75,6 +75,11 @@ class LibraryBackend:
         from . import _native as N
         N.lib().ga_set_thread_workspace_budget(int(nbytes))
 
+    def release(self):
+        """Free the idle workspace blocks the library keeps for reuse."""
+        from . import _native as N
+        N.lib().ga_trim_workspace()
+
     def run(self, key, items):
         if key[0] == "wfa":
             b = api.align_batch_ext(items)
@@ -135,6 +140,9 @@ class Server:
         if wait and self._started:
             for t in self._threads:
                 t.join()
+            release = getattr(self.backend, "release", None)
+            if release is not None:
+                release()
 
     # ------------------------------------------------------------ slots
     def _take(self):

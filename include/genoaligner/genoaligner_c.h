@@ -294,6 +294,13 @@ float ga_last_kernel_ms(void);
  * run several batches concurrently under one memory budget.) */
 void ga_set_thread_workspace_budget(int64_t bytes);
 
+/* Frees the idle device blocks that batch calls keep for reuse (large
+ * workspaces: wavefront stacks, direction tables).  Calls still running keep
+ * theirs.  Idle blocks are otherwise capped at 3/8 of each device's memory.
+ * (EXTENSION: no counterpart in the reference, whose workspaces are host
+ * memory.) */
+void ga_trim_workspace(void);
+
 #ifdef __cplusplus
 }
 #endif

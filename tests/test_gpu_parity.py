@@ -487,3 +487,32 @@ def test_wfa_device_plan_large_batch(gpu, oracle):
     for i in range(0, ps.n, 997):
         sc, cg = oracle.wfa(ps.a_seq(i), ps.b_seq(i), int(ps.smax[i]))
         assert (int(ds[i]), dc[i].decode()) == (sc, cg)
+
+
+def test_workspace_blocks_reused_across_streams(gpu):
+    """Large workspaces (wavefront stacks) are cached device blocks handed from
+    one call to the next; two batches, each on a stream of its own and enqueued
+    in turn with no host wait, take each other's blocks and still give the
+    results of calls run one at a time."""
+    import torch
+    from genoaligner_amd import _native as N
+    from genoaligner_amd import workloads as W
+    from genoaligner_amd.device import WfaDeviceBatch
+    batches = [WfaDeviceBatch(W.wfa_mixed(n=3000, lo=40, hi=3000, seed=s), trace=True, classes=c)
+               for s, c in ((717, True), (718, False))]
+    refs = []
+    for b in batches:
+        b.run()
+        torch.cuda.synchronize()
+        refs.append(b.results())
+    N.lib().ga_trim_workspace()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for _ in range(4):
+        for b, st in zip(batches, streams):
+            b.run(st)
+    torch.cuda.synchronize()
+    for b, (rs, rc) in zip(batches, refs):
+        sc, cg = b.results()
+        assert [int(x) for x in sc] == [int(x) for x in rs]
+        assert cg == rc
+    N.lib().ga_trim_workspace()
