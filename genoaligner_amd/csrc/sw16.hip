@@ -83,6 +83,20 @@ __device__ __forceinline__ uint32_t bsel(uint32_t m, uint32_t a, uint32_t b)
     return d;
 }
 
+// Packed u16 maximum over the wave, as a uniform value: row shifts leave each
+// row's maximum in its lane 15, the row broadcasts 15 and 31 carry them up to
+// lane 63 (lanes without a source read 0, the identity).
+__device__ __forceinline__ uint32_t wave_pmax(uint32_t v)
+{
+    v = pmax(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x111, 0xf, 0xf, true)));
+    v = pmax(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x112, 0xf, 0xf, true)));
+    v = pmax(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x114, 0xf, 0xf, true)));
+    v = pmax(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x118, 0xf, 0xf, true)));
+    v = pmax(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x142, 0xa, 0xf, false)));
+    v = pmax(v, uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x143, 0xc, 0xf, false)));
+    return uint32_t(__builtin_amdgcn_readlane(int(v), 63));
+}
+
 __device__ __forceinline__ int dna_code(uint8_t c)
 {
     return c == 'A' ? 0 : c == 'C' ? 1 : c == 'G' ? 2 : c == 'T' ? 3 : -1;
@@ -224,14 +238,19 @@ struct Strip {
 };
 
 // Four steps t0 .. t0+3 of a strip.  sel4/bnd4 hold this group's selectors and
-// boundary rows (prefetched by the caller).  ROWS: rows past a pair's end are
-// masked out of the maximum (drain tail only).  COLS: columns past a pair's
-// end are masked (partial strips only).
-template <bool TRACE, bool ROWS, bool COLS, bool GK>
+// boundary rows (prefetched by the caller).  COLS: columns past a pair's end
+// are masked out of the maxima (partial strips only).
+//
+// Rows past a pair's end (the drain, and the shorter pair of a wave) need no
+// mask: their selectors give u = 0, i.e. s = -bias <= -1, and gap costs are at
+// least 1, so each such cell's H, E and F are below a value of an earlier
+// cell: by induction H <= max(0, R - 1), R being the pair's largest real H.
+// They never reach the pair's maximum, and real cells never depend on them.
+template <bool TRACE, bool COLS, bool GK>
 __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const uint32_t (&sv)[4],
                                             const uint2 (&bv)[4], uint32_t b2, uint32_t gk2,
-                                            uint32_t ge2, int mAp, int mBp, uint32_t& acc,
-                                            uint32_t (&hout)[4], uint32_t (&eout)[4])
+                                            uint32_t ge2, uint32_t& acc, uint32_t (&hout)[4],
+                                            uint32_t (&eout)[4], uint32_t& thrm1)
 {
     uint32_t Hb[4], Q[4];
 #pragma unroll
@@ -252,10 +271,6 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
         const uint32_t Hn = psubs(pmax(A1, A2), b2);
         uint32_t hb = Hn;
         if (COLS) hb &= S.colmask;
-        if (ROWS) {
-            const int i = t0 + u - lane;
-            hb &= (i < mAp ? 0xffffu : 0u) | (i < mBp ? 0xffff0000u : 0u);
-        }
         Hb[u] = hb;
         if (TRACE) {
             // nd: A2 > A1 (both carry the same bias, so the comparison is exact).
@@ -278,11 +293,17 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
     // byte: bits alternate between rows 0/1 and 2/3, then pairs of bits
     // between the two.
     if (TRACE) acc = bsel(0x33333333u, bsel(0x55555555u, Q[0], Q[1]), bsel(0x55555555u, Q[2], Q[3]));
-    // First maxima: one max per step; the exact per-step update runs only for a
-    // group in which some lane's maximum grew.
-    const uint32_t nb = pmax(pmax(S.bestH, Hb[0]), pmax(pmax(Hb[1], Hb[2]), Hb[3]));
-    if (__any(nb != S.bestH)) {
-        uint32_t b = S.bestH;
+    // First maxima.  Only a cell at or above the pair's best so far can be the
+    // end cell, so a lane's column record needs the exact per-step update only
+    // when its H exceeds max(record, thr - 1), thr being the best over the wave
+    // (per half, uniform; thrm1 = thr - 1).  A column maximum below that bound
+    // is never the pair's maximum: such records are raised to the bound with a
+    // stale step, and lose the final comparison.  Without the bound nearly
+    // every group of a strip had some column reaching a new maximum.
+    const uint32_t cmpv = pmax(S.bestH, thrm1);
+    const uint32_t nb = pmax(pmax(cmpv, Hb[0]), pmax(pmax(Hb[1], Hb[2]), Hb[3]));
+    if (__any(nb != cmpv)) {
+        uint32_t b = cmpv;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             // 0xffff per half where Hb[u] > b: v_perm selectors 8 and 9
@@ -291,8 +312,9 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
             b = pmax(b, Hb[u]);
             S.bestT = (up & (uint32_t(t0 + u) * 0x10001u)) | (~up & S.bestT);
         }
+        S.bestH = b;
+        thrm1 = psubs(wave_pmax(b), 0x00010001u);
     }
-    S.bestH = nb;
 }
 
 // At most 96 VGPRs (5 waves per SIMD) for the trace variant.
@@ -357,11 +379,10 @@ void sw16_kernel(Sw16Launch L)
     const int nstrips = (N + 63) >> 6;
     const int nsteps = (M + 63 + 3) & ~3;
     const int nq = nsteps >> 2;
-    const int mmin = mAp < mBp ? mAp : mBp;
-    const int t_main = mmin & ~3;                 // groups [0, t_main) need no row mask
     const int nmin = nA < nB ? nA : nB;
     uint32_t* dirs = TRACE ? L.dirs + int64_t(w) * L.dir_stride : nullptr;
     int gH[2] = {0, 0}, gI[2] = {-1, -1}, gJ[2] = {-1, -1};
+    uint32_t thrm1 = 0;                           // per half: the pair's best so far - 1 (uniform)
 
     for (int s = 0; s < nstrips; ++s) {
         const int j = (s << 6) + lane;
@@ -396,14 +417,14 @@ void sw16_kernel(Sw16Launch L)
         // prologue and hold up every iteration until an older store is done).
 #pragma unroll
         for (int k = 0; k < kDepth; ++k) asm volatile("" ::"v"(sv[k][0]));
-        // Groups before t_fast need neither row nor column masks.
-        const int t_fast = full ? t_main : 0;
+        // A strip whose columns all exist in both pairs needs no mask at all.
+        const int t_fast = full ? nsteps : 0;
         auto group = [&](auto fast, int t0, const uint32_t (&sv)[4], const uint2 (&bv)[4]) {
             uint32_t acc = 0, hout[4], eout[4];
             if (decltype(fast)::value || t0 < t_fast)
-                strip_group<TRACE, false, false, GK>(S, t0, lane, sv, bv, b2, gk2, ge2, mAp, mBp, acc, hout, eout);
+                strip_group<TRACE, false, GK>(S, t0, lane, sv, bv, b2, gk2, ge2, acc, hout, eout, thrm1);
             else
-                strip_group<TRACE, true, true, true>(S, t0, lane, sv, bv, b2, gk2, ge2, mAp, mBp, acc, hout, eout);
+                strip_group<TRACE, true, true>(S, t0, lane, sv, bv, b2, gk2, ge2, acc, hout, eout, thrm1);
             if (TRACE) dirs[(int64_t(s) * nq + (t0 >> 2)) * 64 + lane] = acc;
             // lane 63 hands rows t0-63 .. t0-60 to the next strip (lane 0 has
             // read each row before it is written); rows outside 0 .. Mr-1 are
