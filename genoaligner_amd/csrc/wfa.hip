@@ -105,18 +105,20 @@ __device__ __forceinline__ int lds_match_run(const uint32_t* P, int v, const uin
     const uint32_t* t = T + (h >> 4);
     const uint32_t sp = (uint32_t(v) & 15u) * 2u, st = (uint32_t(h) & 15u) * 2u;
     uint32_t x = __builtin_amdgcn_alignbit(p[1], p[0], sp) ^ __builtin_amdgcn_alignbit(t[1], t[0], st);
-    int run = 0;
+    // equal bases before the first differing one: ctz / 2, or 16 for x == 0
+    uint32_t run = uint32_t(__builtin_ctzg(x, 32)) >> 1;
     if (x == 0 && 16 < lim) {
+        uint32_t r = 16;
         for (;;) {
-            run += 16;
             ++p;
             ++t;
             x = __builtin_amdgcn_alignbit(p[1], p[0], sp) ^ __builtin_amdgcn_alignbit(t[1], t[0], st);
-            if (x != 0 || run + 16 >= lim) break;
+            if (x != 0 || r + 16 >= uint32_t(lim)) break;
+            r += 16;
         }
+        run = r + (uint32_t(__builtin_ctzg(x, 32)) >> 1);
     }
-    run += x ? int(__builtin_ctz(x) >> 1) : 16;
-    return run < lim ? run : lim;
+    return int(run < uint32_t(lim) ? run : uint32_t(lim));
 }
 
 // Stage one sequence as 2-bit words (16 bases per thread per step, two spare
@@ -146,8 +148,10 @@ __device__ __forceinline__ bool stage2(uint32_t* dst, const uint8_t* src, int le
     return ok;
 }
 
-// LDS layout of a workgroup: [claim slots: 4 ints][wavefront A][wavefront B]
-// [traceback window][pattern, 2-bit][text, 2-bit], each region 16-byte aligned.
+// LDS layout of a workgroup: [claim slots: 4 ints][pattern, 2-bit][text, 2-bit]
+// [wavefront A][wavefront B][traceback window], each region 16-byte aligned
+// (the sequences first: the pattern's base then fits the offset field of the
+// extension's LDS reads, and no add per diagonal is spent on it).
 __host__ __device__ inline int wfa_wf_bytes(int smax_cls, int off_bytes)
 {
     return ((2 * smax_cls + 3) * off_bytes + 15) & ~15;
@@ -181,10 +185,12 @@ __device__ __forceinline__ int wfa_forward(const uint32_t* sP, const uint32_t* s
     Off* cur = B;
     for (int s = 1; s <= smax; ++s) {
         Off* wrow = TRACE ? ws + int64_t(s) * s : nullptr;   // row s of the stack (wave-uniform)
+        const Off* pk = prev + S + 1;                          // pk[k] = W[s-1][k]
+        Off* ck = cur + S + 1;
         for (int k = -s + tid; k <= s; k += NT) {
-            const int ins = prev[k + S];
-            const int mis = prev[k + S + 1];
-            const int del = prev[k + S + 2];
+            const int ins = pk[k - 1];
+            const int mis = pk[k];
+            const int del = pk[k + 1];
             const int mid = ins > mis ? ins : mis;
             int mx = del > mid + 1 ? del : mid + 1;
             const int v = mx - k;
@@ -200,7 +206,7 @@ __device__ __forceinline__ int wfa_forward(const uint32_t* sP, const uint32_t* s
             } else {
                 mx = OffNull<Off>::v;
             }
-            cur[k + S + 1] = Off(mx);
+            ck[k] = Off(mx);
             // (a 32-bit byte offset from the wave-uniform row base: one
             // scalar-base store, no per-lane 64-bit address)
             if (TRACE && trace)
@@ -228,12 +234,14 @@ void wfa_kernel(WfaQueueLaunch L)
     const int W = 2 * S + 3;
     const int wfb = wfa_wf_bytes(S, int(sizeof(Off)));
     int* claim = reinterpret_cast<int*>(smem);
-    Off* A = reinterpret_cast<Off*>(smem + 16);
-    Off* B = reinterpret_cast<Off*>(smem + 16 + wfb);
-    Off* win = reinterpret_cast<Off*>(smem + 16 + 2 * wfb);
-    uint8_t* seq = smem + 16 + 2 * wfb + wfa_win_bytes(int(sizeof(Off)));
-    uint32_t* sP = reinterpret_cast<uint32_t*>(seq);
-    uint32_t* sT = reinterpret_cast<uint32_t*>(seq + wfa_seq_bytes(L.max_plen));
+    const int pbytes = STAGED ? wfa_seq_bytes(L.max_plen) : 0;
+    const int tbytes = STAGED ? wfa_seq_bytes(L.max_tlen) : 0;
+    uint32_t* sP = reinterpret_cast<uint32_t*>(smem + 16);
+    uint32_t* sT = reinterpret_cast<uint32_t*>(smem + 16 + pbytes);
+    uint8_t* wfs = smem + 16 + pbytes + tbytes;
+    Off* A = reinterpret_cast<Off*>(wfs);
+    Off* B = reinterpret_cast<Off*>(wfs + wfb);
+    Off* win = reinterpret_cast<Off*>(wfs + 2 * wfb);
     Off* ws = TRACE ? static_cast<Off*>(L.ws) + int64_t(blockIdx.x) * (int64_t(S) + 1) * (int64_t(S) + 1)
                     : nullptr;
     const int G = int(gridDim.x);
