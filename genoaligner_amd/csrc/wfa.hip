@@ -187,7 +187,22 @@ __device__ __forceinline__ int wfa_forward(const uint32_t* sP, const uint32_t* s
         Off* wrow = TRACE ? ws + int64_t(s) * s : nullptr;   // row s of the stack (wave-uniform)
         const Off* pk = prev + S + 1;                          // pk[k] = W[s-1][k]
         Off* ck = cur + S + 1;
-        for (int k = -s + tid; k <= s; k += NT) {
+        // Budget pruning: every step changes the diagonal by at most one, so
+        // from score s only diagonals with |k - ak| <= smax - s can still
+        // reach the end diagonal ak by smax; the others are not computed.  A
+        // kept diagonal reads only kept diagonals of the previous score (its
+        // neighbours are within smax - s + 1 of ak), so kept offsets, the end
+        // test and the backtrace (whose candidates are kept too) are exactly
+        // those of the full wavefront.  A pair that runs to smax computes
+        // about half the cells.  The next score reads slots lo - 1 .. hi + 1
+        // of this one's buffer only where they were written now or never
+        // written since the pair began (the range first grows with s, then
+        // shrinks by one per side once the budget binds), so no stale value
+        // of score s - 2 is read.
+        const int bud = smax - s;
+        const int lo = -s > ak - bud ? -s : ak - bud;
+        const int hi = s < ak + bud ? s : ak + bud;
+        for (int k = lo + tid; k <= hi; k += NT) {
             const int ins = pk[k - 1];
             const int mis = pk[k];
             const int del = pk[k + 1];

@@ -108,6 +108,33 @@ def test_wfa_ext_per_pair_smax(gpu, oracle):
     assert [(r.score, r.cigar) for r in b.results] == exp
 
 
+def test_wfa_smax_at_the_distance(gpu, oracle):
+    """Pairs whose bound is the exact distance, one more and one less (the
+    last unresolved), with unequal lengths: the wavefront's budget pruning
+    (diagonals that can no longer reach the end diagonal within smax are
+    skipped) must leave every score and CIGAR as the full computation's."""
+    import genoaligner_amd as ga
+    rng = random.Random(11)
+    reqs, exp = [], []
+    for _ in range(120):
+        L = rng.randint(40, 1500)
+        p = "".join(rng.choice("ACGT") for _ in range(L))
+        t = _mutate(rng, p, rng.uniform(0.02, 0.3))
+        if rng.random() < 0.3:                    # shift the end diagonal
+            t = t + "".join(rng.choice("ACGT") for _ in range(rng.randint(1, 40)))
+        d, _ = oracle.wfa(p, t, 4000)
+        assert d >= 0
+        for smax in (d - 1, d, d + 1):
+            if smax < 1 or smax > 4000:
+                continue
+            reqs.append(_areq(ga, p, t, smax, True))
+            exp.append(oracle.wfa(p, t, smax))
+    b = ga.align_batch_ext(reqs)
+    assert b.ok()
+    got = [(r.score, r.cigar) for r in b.results]
+    assert got == [(sc, cg if sc >= 0 else "") for sc, cg in exp]
+
+
 def test_wfa_device_resident_plain(gpu, oracle):
     from genoaligner_amd import workloads as W
     from genoaligner_amd.device import WfaDeviceBatch
