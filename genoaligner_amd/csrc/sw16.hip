@@ -232,7 +232,7 @@ inline bool sw16_gk(const Sw16Launch& L)
 // Per-lane state of one strip.
 struct Strip {
     uint32_t H, Eh, Fh, diag;      // last row's H, Eh, Fh; H of the left column one row up
-    uint32_t bestH, bestT;         // per half: first maximum of this column and its step
+    uint32_t bestH, bestT;         // per half: first maximum of this column (at least thr - 1) and its step
     uint32_t tabA, tabB;           // u tables of this lane's pattern bytes
     uint32_t colmask;              // 0xffff per half whose column exists
 };
@@ -296,14 +296,14 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
     // First maxima.  Only a cell at or above the pair's best so far can be the
     // end cell, so a lane's column record needs the exact per-step update only
     // when its H exceeds max(record, thr - 1), thr being the best over the wave
-    // (per half, uniform; thrm1 = thr - 1).  A column maximum below that bound
-    // is never the pair's maximum: such records are raised to the bound with a
-    // stale step, and lose the final comparison.  Without the bound nearly
-    // every group of a strip had some column reaching a new maximum.
-    const uint32_t cmpv = pmax(S.bestH, thrm1);
-    const uint32_t nb = pmax(pmax(cmpv, Hb[0]), pmax(pmax(Hb[1], Hb[2]), Hb[3]));
-    if (__any(nb != cmpv)) {
-        uint32_t b = cmpv;
+    // (per half, uniform; thrm1 = thr - 1).  The record is kept at that bound
+    // (raised to it with a stale step where the column's own maximum is lower:
+    // such a record is below the pair's maximum and loses the final
+    // comparison), so one comparison with it suffices.  Without the bound
+    // nearly every group of a strip had some column reaching a new maximum.
+    const uint32_t nb = pmax(pmax(S.bestH, Hb[0]), pmax(pmax(Hb[1], Hb[2]), Hb[3]));
+    if (__any(nb != S.bestH)) {
+        uint32_t b = S.bestH;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             // 0xffff per half where Hb[u] > b: v_perm selectors 8 and 9
@@ -312,8 +312,8 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
             b = pmax(b, Hb[u]);
             S.bestT = (up & (uint32_t(t0 + u) * 0x10001u)) | (~up & S.bestT);
         }
-        S.bestH = b;
         thrm1 = psubs(wave_pmax(b), 0x00010001u);
+        S.bestH = pmax(b, thrm1);
     }
 }
 
@@ -391,7 +391,7 @@ void sw16_kernel(Sw16Launch L)
         S.tabB = u_table(j < nB ? PB[j] : uint8_t(0), L.match, L.mismatch, bias);
         S.colmask = (j < nA ? 0xffffu : 0u) | (j < nB ? 0xffff0000u : 0u);
         S.H = 0; S.Eh = go2; S.Fh = go2; S.diag = 0;
-        S.bestH = 0; S.bestT = 0xffffffffu;
+        S.bestH = thrm1; S.bestT = 0xffffffffu;     // record at the bound (see strip_group)
         const bool full = (s << 6) + 64 <= nmin;  // every column of the strip exists in both pairs
 
         // Selectors and boundary rows of group t0 are read kDepth groups ahead,
