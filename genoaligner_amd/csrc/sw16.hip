@@ -159,22 +159,44 @@ __host__ __device__ inline size_t sw16_lds_bytes(int max_m, int max_n)
 // every few steps.  Every lane of the wave calls it; lanes other than the
 // walkers return 0.  Flag word layout: cell (i, j) of half h sits in word
 // (s * nq + (t >> 2)) * 64 + l, with s, l the strip and lane of column j - 1
-// and t = i - 1 + l; with b = 8h + (t & 3), bit b + 4 is nd, b + 20 fe, b ee
-// and b + 16 fx.
-template <class CP, class SP>
+// and t = i - 1 + l; with r = t & 3 and b = 8h + r,
+//   !HMOD: bit b + 4 is nd, b + 20 fe, b ee and b + 16 fx;
+//   HMOD:  bit b is nd, b + 16 fe, and bits 16h + 4 + 2(r & 1) + 8(r >> 1) and
+//          the next hold H mod 4.  A gap's open-or-extend choice then follows
+//          from the cell it came from: in E at (i, j) with value v, the gap
+//          opened iff H(i, j-1) = v + go, and otherwise (extension, strictly
+//          better) v + ge <= H(i, j-1) < v + go, so H mod 4 decides it when
+//          go - ge <= 3 (F likewise with H(i-1, j)).  The walk reads that cell's
+//          word too, so the window keeps a margin of one row and one column
+//          (a window's words start at row iw - kWinR + 1 or up to 3 rows
+//          lower, depending on the word alignment).
+template <bool HMOD, class CP, class SP>
 __device__ int walk16(const uint32_t* dirs, int nq, uint32_t* win, bool walker, int lane, SP T, SP P,
                       int m, int n, int i, int j, int score, int match, int mismatch, int go, int ge,
                       CP cg, int* si, int* sj)
 {
+    // rows and columns a step may use: with HMOD a gap step also reads the
+    // cell above or to the left, which must be inside the window too
+    constexpr int kRows = HMOD ? kWinR - 1 : kWinR;
+    constexpr int kCols = HMOD ? kWinC - 1 : kWinC;
     const int hw = lane >> 5, c = lane & 31;
     uint32_t* w = win + hw * (kWinC * kWinQ);
     int pos = m + n, state = 0, v = score;
     bool active = walker;
     bool have = false;
     int iw = 0, jw = 0;                            // window anchor (walker lanes)
+    // the window word of cell (ii, jj) (inside the window)
+    auto word_at = [&](int ii, int jj) {
+        const int l = (jj - 1) & 63, t = ii - 1 + l;
+        return w[(jw - jj) * kWinQ + ((t >> 2) - ((iw - kWinR + l) >> 2))];
+    };
+    auto hmod_at = [&](int ii, int jj) {
+        const int l = (jj - 1) & 63, r = (ii - 1 + l) & 3;
+        return int((word_at(ii, jj) >> (16 * hw + 4 + 2 * (r & 1) + 8 * (r >> 1))) & 3u);
+    };
     while (__any(active)) {
         if (active && (!(i > 0 && j > 0 && pos > 0) || (state == 0 && v == 0))) active = false;
-        const bool need = active && !(have && iw - i < kWinR && jw - j < kWinC);
+        const bool need = active && !(have && iw - i < kRows && jw - j < kCols);
         const int need_h = __shfl(int(need), hw * 32);
         const int ih = __shfl(i, hw * 32), jh = __shfl(j, hw * 32);
         if (need_h) {
@@ -191,25 +213,25 @@ __device__ int walk16(const uint32_t* dirs, int nq, uint32_t* win, bool walker, 
         }
         if (need) { iw = i; jw = j; have = true; }
         if (!active) continue;
-        // at most kWinR + kWinC - 1 steps stay inside the window
-        while (i > 0 && j > 0 && pos > 0 && !(state == 0 && v == 0) && iw - i < kWinR && jw - j < kWinC) {
-            const int l = (j - 1) & 63, t = i - 1 + l;
-            const uint32_t word = w[(jw - j) * kWinQ + ((t >> 2) - ((iw - kWinR + l) >> 2))];
-            const int b = 8 * hw + (t & 3);
-            if (state == 0 && !((word >> (b + 4)) & 1u)) {
+        // at most kRows + kCols - 1 steps stay inside the window
+        while (i > 0 && j > 0 && pos > 0 && !(state == 0 && v == 0) && iw - i < kRows && jw - j < kCols) {
+            const uint32_t word = word_at(i, j);
+            const int b = 8 * hw + ((((j - 1) & 63) + i - 1) & 3);
+            const int bnd = HMOD ? b : b + 4, bfe = HMOD ? b + 16 : b + 20;
+            if (state == 0 && !((word >> bnd) & 1u)) {
                 const bool eq = T[i - 1] == P[j - 1];
                 cg[--pos] = eq ? 'M' : 'X';
                 v -= eq ? match : mismatch;
                 --i; --j;
-            } else if (state == 1 || (state == 0 && !((word >> (b + 20)) & 1u))) {
+            } else if (state == 1 || (state == 0 && !((word >> bfe) & 1u))) {
                 cg[--pos] = 'D';
-                const bool ext = (word >> b) & 1u;
+                const bool ext = HMOD ? ((hmod_at(i, j - 1) - v - go) & 3) != 0 : ((word >> b) & 1u) != 0;
                 state = ext ? 1 : 0;
                 v += ext ? ge : go;
                 --j;
             } else {
                 cg[--pos] = 'I';
-                const bool ext = (word >> (b + 16)) & 1u;
+                const bool ext = HMOD ? ((hmod_at(i - 1, j) - v - go) & 3) != 0 : ((word >> (b + 16)) & 1u) != 0;
                 state = ext ? 2 : 0;
                 v += ext ? ge : go;
                 --i;
@@ -229,6 +251,9 @@ inline bool sw16_gk(const Sw16Launch& L)
     return m0 > L.go;
 }
 
+// H mod 4 decides a gap's open-or-extend step when 0 <= go - ge <= 3.
+inline bool sw16_hmod(const Sw16Launch& L) { return L.go >= L.ge && L.go - L.ge <= 3; }
+
 // Per-lane state of one strip.
 struct Strip {
     uint32_t H, Eh, Fh, diag;      // last row's H, Eh, Fh; H of the left column one row up
@@ -246,7 +271,7 @@ struct Strip {
 // least 1, so each such cell's H, E and F are below a value of an earlier
 // cell: by induction H <= max(0, R - 1), R being the pair's largest real H.
 // They never reach the pair's maximum, and real cells never depend on them.
-template <bool TRACE, bool COLS, bool GK>
+template <bool TRACE, bool HMOD, bool COLS, bool GK>
 __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const uint32_t (&sv)[4],
                                             const uint2 (&bv)[4], uint32_t b2, uint32_t gk2,
                                             uint32_t ge2, uint32_t& acc, uint32_t (&hout)[4],
@@ -279,8 +304,12 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
             // so each perm yields 0xff/0x00 byte masks for two flags of both
             // pairs; Q[u] keeps nd/fe in the high and ee/fx in the low nibble.
             const uint32_t P1 = __builtin_amdgcn_perm(pdiff(En, Fn), pdiff(A1, A2), 0x0b0a0908u);
-            const uint32_t P2 = __builtin_amdgcn_perm(pdiff(S.H, xf), pdiff(Hl, xe), 0x0b0a0908u);
-            Q[u] = bsel(0xf0f0f0f0u, P1, P2);
+            if (HMOD) {
+                Q[u] = P1;
+            } else {
+                const uint32_t P2 = __builtin_amdgcn_perm(pdiff(S.H, xf), pdiff(Hl, xe), 0x0b0a0908u);
+                Q[u] = bsel(0xf0f0f0f0u, P1, P2);
+            }
         }
         hout[u] = Hn;
         eout[u] = En;
@@ -291,8 +320,19 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, const ui
     }
     // Row u of the group goes to bit 4+u (nd, fe) and bit u (ee, fx) of each
     // byte: bits alternate between rows 0/1 and 2/3, then pairs of bits
-    // between the two.
-    if (TRACE) acc = bsel(0x33333333u, bsel(0x55555555u, Q[0], Q[1]), bsel(0x55555555u, Q[2], Q[3]));
+    // between the two.  HMOD: the flags give bits u and 4+u copies of nd, fe;
+    // the high nibbles then take H mod 4 of each row (one shift and one
+    // bit-select per row, where the two gap flags took two differences, a
+    // perm and a select).
+    if (TRACE) {
+        acc = bsel(0x33333333u, bsel(0x55555555u, Q[0], Q[1]), bsel(0x55555555u, Q[2], Q[3]));
+        if (HMOD) {
+            acc = bsel(0x00030003u << 4, hout[0] << 4, acc);
+            acc = bsel(0x00030003u << 6, hout[1] << 6, acc);
+            acc = bsel(0x00030003u << 12, hout[2] << 12, acc);
+            acc = bsel(0x00030003u << 14, hout[3] << 14, acc);
+        }
+    }
     // First maxima.  Only a cell at or above the pair's best so far can be the
     // end cell, so a lane's column record needs the exact per-step update only
     // when its H exceeds max(record, thr - 1), thr being the best over the wave
@@ -323,8 +363,9 @@ constexpr BoolC<true> kTrue{};
 constexpr BoolC<false> kFalse{};
 
 // GK: gap_open < -mismatch, where H needs the extra bias term (sw16_gk).
-// Prefetch depth in groups (register sets of selectors and boundary rows).
-template <bool TRACE, bool GK>
+// HMOD (trace only): store H mod 4 in place of the two gap flags, for
+// gap_open - gap_extend <= 3 (sw16_hmod; the walk explains why that suffices).
+template <bool TRACE, bool GK, bool HMOD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
 void sw16_kernel(Sw16Launch L)
 {
@@ -422,9 +463,9 @@ void sw16_kernel(Sw16Launch L)
         auto group = [&](auto fast, int t0, const uint32_t (&sv)[4], const uint2 (&bv)[4]) {
             uint32_t acc = 0, hout[4], eout[4];
             if (decltype(fast)::value || t0 < t_fast)
-                strip_group<TRACE, false, GK>(S, t0, lane, sv, bv, b2, gk2, ge2, acc, hout, eout, thrm1);
+                strip_group<TRACE, HMOD, false, GK>(S, t0, lane, sv, bv, b2, gk2, ge2, acc, hout, eout, thrm1);
             else
-                strip_group<TRACE, true, true>(S, t0, lane, sv, bv, b2, gk2, ge2, acc, hout, eout, thrm1);
+                strip_group<TRACE, HMOD, true, true>(S, t0, lane, sv, bv, b2, gk2, ge2, acc, hout, eout, thrm1);
             if (TRACE) dirs[(int64_t(s) * nq + (t0 >> 2)) * 64 + lane] = acc;
             // lane 63 hands rows t0-63 .. t0-60 to the next strip (lane 0 has
             // read each row before it is written); rows outside 0 .. Mr-1 are
@@ -533,7 +574,7 @@ void sw16_kernel(Sw16Launch L)
     const bool walker = (lane & 31) == 0 && pidw >= 0 && !redo[hw] && endS[hw] > 0 &&
                         (!L.walk || L.walk[pidw]);
     int si = -1, sj = -1;
-    const int len_h = walk16(dirs, nq, win, walker, lane, hw == 0 ? sTA : sTB, hw == 0 ? sPA : sPB,
+    const int len_h = walk16<HMOD>(dirs, nq, win, walker, lane, hw == 0 ? sTA : sTB, hw == 0 ? sPA : sPB,
                              hw == 0 ? mA : mB, hw == 0 ? nA : nB, endI[hw] + 1, endJ[hw] + 1,
                              endS[hw], L.match, L.mismatch, go, ge, hw == 0 ? cgA : cgB, &si, &sj);
     if (walker) {
@@ -562,12 +603,15 @@ hipError_t sw16_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHo
     const size_t lds = sw16_lds_bytes(L.max_m, trace ? L.max_n : 0);
     const dim3 grid(unsigned(L.n_waves)), block(64);
     if (hooks) hooks->before(st);
+    const bool hm = sw16_hmod(L);
     if (sw16_gk(L)) {
-        if (trace) hipLaunchKernelGGL((sw16_kernel<true, true>), grid, block, lds, st, L);
-        else       hipLaunchKernelGGL((sw16_kernel<false, true>), grid, block, lds, st, L);
+        if (trace && hm) hipLaunchKernelGGL((sw16_kernel<true, true, true>), grid, block, lds, st, L);
+        else if (trace)  hipLaunchKernelGGL((sw16_kernel<true, true, false>), grid, block, lds, st, L);
+        else             hipLaunchKernelGGL((sw16_kernel<false, true, false>), grid, block, lds, st, L);
     } else {
-        if (trace) hipLaunchKernelGGL((sw16_kernel<true, false>), grid, block, lds, st, L);
-        else       hipLaunchKernelGGL((sw16_kernel<false, false>), grid, block, lds, st, L);
+        if (trace && hm) hipLaunchKernelGGL((sw16_kernel<true, false, true>), grid, block, lds, st, L);
+        else if (trace)  hipLaunchKernelGGL((sw16_kernel<true, false, false>), grid, block, lds, st, L);
+        else             hipLaunchKernelGGL((sw16_kernel<false, false, false>), grid, block, lds, st, L);
     }
     if (hooks) hooks->after(st);
     return hipGetLastError();

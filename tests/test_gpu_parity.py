@@ -206,6 +206,36 @@ def test_sw_random_vs_oracle(gpu, oracle):
                     exp.end_j, exp.rescore_ok, exp.wellformed_ok, exp.too_large), (sc, t, p)
 
 
+def test_sw_long_gaps_both_trace_layouts(gpu, oracle):
+    """Pairs with inserted and deleted blocks of 2..40 bases, under schemes
+    with gap_open - gap_extend <= 3 (the packed trace stores H mod 4 and the
+    walk derives each gap step from it) and above 3 (it stores the two gap
+    flags), with and without the gk bias term."""
+    import genoaligner_amd as ga
+    rng = random.Random(47)
+    for sc in ((2, -3, 3, 1), (2, -3, 5, 2), (1, -2, 4, 1), (2, -3, 6, 1), (3, -4, 9, 2), (2, -8, 6, 1)):
+        reqs, pairs = [], []
+        for _ in range(120):
+            t = "".join(rng.choice("ACGT") for _ in range(rng.randint(50, 900)))
+            a = rng.randint(0, len(t) // 2)
+            q = list(t[a: a + rng.randint(20, 300)])
+            for _ in range(rng.randint(0, 4)):       # gap blocks in the query copy
+                at = rng.randint(0, len(q))
+                if rng.random() < 0.5:
+                    q[at:at] = [rng.choice("ACGT") for _ in range(rng.randint(2, 40))]
+                else:
+                    del q[at: at + rng.randint(2, 40)]
+            p = _mutate(rng, "".join(q), 0.05)
+            pairs.append((t, p))
+            reqs.append(_sreq(ga, t, p, sc, True))
+        b = ga.align_sw_batch(reqs)
+        want = oracle.align_sw_batch(reqs)
+        assert b.ok()
+        for got, exp, (t, p) in zip(b.results, want.results, pairs):
+            assert (got.score, got.cigar, got.start_i, got.start_j, got.end_i, got.end_j) == \
+                   (exp.score, exp.cigar, exp.start_i, exp.start_j, exp.end_i, exp.end_j), (sc, t, p)
+
+
 def test_sw_long_text_global_boundary(gpu, oracle):
     """Texts longer than the LDS boundary buffer take the HBM boundary path."""
     import genoaligner_amd as ga
