@@ -18,7 +18,7 @@ def _word_index(iw, jw, ii, jj):
 
 def _outside(hmod, rows, cols):
     bad = 0
-    for iw, jw in itertools.product(range(2, 70), range(2, 140)):
+    for iw, jw in itertools.product(range(30, 42), range(2, 134)):   # all row alignments, all lanes
         for di, dj in itertools.product(range(rows), range(cols)):
             i, j = iw - di, jw - dj
             if i < 1 or j < 1:
