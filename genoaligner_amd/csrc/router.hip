@@ -17,12 +17,6 @@
 namespace ga {
 namespace {
 
-__device__ __forceinline__ int code4(uint32_t b)
-{
-    // A 0, C 1, G 2, T 3; anything else -1
-    return b == 'A' ? 0 : b == 'C' ? 1 : b == 'G' ? 2 : b == 'T' ? 3 : -1;
-}
-
 // 8 bytes starting at any byte address (two aligned words; sequences carry
 // 128 readable pad bytes on both sides).
 __device__ __forceinline__ uint64_t load8u(const uint8_t* p)
@@ -37,7 +31,10 @@ __device__ __forceinline__ uint64_t load8u(const uint8_t* p)
 
 // 4-mer counts of one sequence into an LDS histogram (windows holding a byte
 // other than A/C/G/T are skipped).  Lane l rolls a 4-mer code over windows
-// [l*chunk, (l+1)*chunk), reading each base once, 8 bytes per load.
+// [l*chunk, (l+1)*chunk), reading each base once, 8 bytes per load.  Codes
+// come four bytes at a time: ((b >> 1) ^ (b >> 2)) & 3 is A 0, C 1, G 2, T 3
+// (bits 1..3 of the ASCII letters), and a byte is one of them iff the code's
+// letter (a byte select from "ACGT") equals it.
 __device__ void count4(const uint8_t* s, int len, int* hist, int lane)
 {
     const int nwin = len - 3;
@@ -47,17 +44,23 @@ __device__ void count4(const uint8_t* s, int len, int* hist, int lane)
     const int w1 = w0 + chunk < nwin ? w0 + chunk : nwin;
     if (w0 >= w1) return;
     // bases w0 .. w1 + 2 feed windows w0 .. w1 - 1
-    int code = 0, valid = 0;                 // rolling 8-bit code, trailing run of ACGT bases
+    uint32_t code = 0;                       // rolling 8-bit code
+    int valid = 0;                           // trailing run of A/C/G/T bases
     for (int b = w0; b < w1 + 3; b += 8) {
         const uint64_t x = load8u(s + b);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int pos = b + q;
-            if (pos < w1 + 3) {
-                const int c = code4(uint32_t((x >> (8 * q)) & 0xffu));
-                code = ((code << 2) | (c & 3)) & 0xff;
-                valid = c >= 0 ? valid + 1 : 0;
-                if (pos >= w0 + 3 && valid >= 4) atomicAdd(&hist[code], 1);
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t w = uint32_t(x >> (32 * h));
+            const uint32_t c = ((w >> 1) ^ (w >> 2)) & 0x03030303u;
+            const uint32_t bad = __builtin_amdgcn_perm(0u, 0x54474341u, c) ^ w;   // "ACGT"
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int pos = b + 4 * h + q;
+                if (pos < w1 + 3) {
+                    code = ((code << 2) | ((c >> (8 * q)) & 3u)) & 0xffu;
+                    valid = ((bad >> (8 * q)) & 0xffu) ? 0 : valid + 1;
+                    if (pos >= w0 + 3 && valid >= 4) atomicAdd(&hist[code], 1);
+                }
             }
         }
     }
@@ -69,21 +72,33 @@ __device__ __forceinline__ int wave_sum(int v)
     return v;
 }
 
-// One wave per pair: features (kSketchDim floats) and the exact lower bound
+// One wave per pair, four pairs per workgroup (each wave with its own LDS
+// histograms; a launch of one-wave workgroups was bound by workgroup dispatch
+// at ~5 ns each): features (kSketchDim floats) and the exact lower bound
 // max(|m - n|, ceil(L1 / 8)) on the edit distance.
-__global__ __launch_bounds__(64) void sketch_kernel(SketchLaunch L)
+constexpr int kSketchWaves = 4;
+__global__ __launch_bounds__(64 * kSketchWaves) void sketch_kernel(SketchLaunch L)
 {
-    __shared__ int hp[256];
-    __shared__ int ht[256];
-    const int lane = threadIdx.x;
-    const int64_t pid = blockIdx.x;
-    if (pid >= L.n) return;
+    __shared__ int hist[kSketchWaves][2][256];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t pid = int64_t(blockIdx.x) * kSketchWaves + wave;
+    if (pid >= L.n) return;                      // whole waves only; no workgroup barrier follows
+    int* hp = hist[wave][0];
+    int* ht = hist[wave][1];
+    // the histograms are this wave's own: a wave-scope fence and barrier order
+    // the clearing before the counting and the counting before the reads
+    auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
     for (int b = lane; b < 256; b += 64) { hp[b] = 0; ht[b] = 0; }
-    __syncthreads();
+    wave_sync();
     const int m = L.plen[pid], n = L.tlen[pid];
     count4(L.pat + L.poff[pid], m, hp, lane);
     count4(L.txt + L.toff[pid], n, ht, lane);
-    __syncthreads();
+    wave_sync();
     int part = 0;
     for (int b = lane; b < 256; b += 64) part += abs(hp[b] - ht[b]);
     const int l1 = wave_sum(part);
@@ -298,7 +313,8 @@ inline unsigned blocks(int64_t n, int per) { return unsigned((n + per - 1) / per
 hipError_t sketch_launch(const SketchLaunch& L, hipStream_t st)
 {
     if (L.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(sketch_kernel, dim3(unsigned(L.n)), dim3(64), 0, st, L);
+    hipLaunchKernelGGL(sketch_kernel, dim3(unsigned((L.n + kSketchWaves - 1) / kSketchWaves)),
+                       dim3(64 * kSketchWaves), 0, st, L);
     return hipGetLastError();
 }
 

@@ -29,7 +29,8 @@ def test_sketch_kernel_matches_oracle(gpu, oracle):
     rng = random.Random(12)
     pairs = [("", ""), ("ACG", ""), ("", "ACGTA")]
     for _ in range(200):
-        p = "".join(rng.choice("ACGTNa") if rng.random() < 0.05 else rng.choice("ACGT")
+        # bytes other than A/C/G/T, some sharing their low bits (a c g t Q E @ S)
+        p = "".join(rng.choice("ACGTNacgtQE@S") if rng.random() < 0.05 else rng.choice("ACGT")
                     for _ in range(rng.randint(0, 2500)))
         t = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 2500)))
         pairs.append((p, t))
