@@ -140,14 +140,26 @@ __global__ __launch_bounds__(256) void dense_fwd(const float* X, const float* W,
         sX[x] = r < N ? X[r * I + x % I] : 0.0f;
     }
     __syncthreads();
-    for (int x = t; x < kRows * O; x += 256) {
-        const int rr = x / O, o = x % O;
-        const int64_t r = r0 + rr;
-        if (r >= N) continue;
-        float acc = 0.0f;
-        for (int i = 0; i < I; ++i) acc += sX[rr * I + i] * W[i * O + o];
-        acc += bias[o];
-        Y[r * O + o] = RELU ? (acc > 0.0f ? acc : 0.0f) : acc;
+    // a thread takes 4 rows of one output: each weight read serves 4 rows
+    // (the weights come through L2; 64 KB does not fit L1), and each row's sum
+    // runs over i ascending, multiply then add, as a one-row loop would
+    constexpr int kRt = 4;
+    for (int x = t; x < (kRows / kRt) * O; x += 256) {
+        const int rg = x / O, o = x % O;
+        const float* xr = sX + rg * kRt * I;
+        float acc[kRt] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < I; ++i) {
+            const float w = W[i * O + o];
+#pragma unroll
+            for (int q = 0; q < kRt; ++q) acc[q] += xr[q * I + i] * w;
+        }
+#pragma unroll
+        for (int q = 0; q < kRt; ++q) {
+            const int64_t r = r0 + rg * kRt + q;
+            if (r >= N) break;
+            const float v = acc[q] + bias[o];
+            Y[r * O + o] = RELU ? (v > 0.0f ? v : 0.0f) : v;
+        }
     }
 }
 
