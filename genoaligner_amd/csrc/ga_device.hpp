@@ -131,6 +131,7 @@ struct SwLaunch {
     // 0..31 of the text and pattern, laid out like txt / pat, and the 32 x 32
     // score table sub[text code * 32 + pattern code].
     const uint8_t* tcode; const uint8_t* pcode; const int32_t* sub;
+    bool sub8;              // every sub entry fits int8 (8-bit LDS profiles)
 };
 
 // Marks, in res[0] of a pair, that the packed 16-bit kernel left it to the int32 kernel.

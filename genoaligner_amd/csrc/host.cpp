@@ -16,8 +16,11 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdio>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -266,36 +269,125 @@ struct DevArena {
 struct Packed {
     std::vector<uint8_t> bytes;
     std::vector<int64_t> off;
-    void reserve(size_t n, size_t total) { off.reserve(n); bytes.reserve(total + 2 * kSeqPad); }
-    void begin() { bytes.assign(kSeqPad, 0); }
-    void add(const char* p, int len)
-    {
-        off.push_back(int64_t(bytes.size()));
-        if (len > 0) bytes.insert(bytes.end(), p, p + len);
-    }
-    void end() { bytes.insert(bytes.end(), kSeqPad, 0); }
+    // All n sequences at once (get(k) -> {bytes, length}): offsets by a prefix
+    // sum, then the copies on the host worker pool; codes (optional) receives
+    // map[byte] for every byte of the buffer, padding included.
+    template <class Get> void build(int64_t n, Get get, const uint8_t* map = nullptr,
+                                    std::vector<uint8_t>* codes = nullptr);
 };
 
-// Run fn(i) for i in [0, n) on up to 16 host threads.
-template <class F> void parallel_for(int64_t n, F&& fn)
-{
-    const int64_t grain = 2048;
-    unsigned hw = std::thread::hardware_concurrency();
-    int nt = int(std::min<int64_t>((n + grain - 1) / grain, std::min<unsigned>(hw ? hw : 1, 16u)));
-    if (nt <= 1) { for (int64_t i = 0; i < n; ++i) fn(i); return; }
-    std::atomic<int64_t> next{0};
-    auto work = [&] {
-        for (;;) {
-            const int64_t i0 = next.fetch_add(grain);
-            if (i0 >= n) break;
-            const int64_t i1 = std::min(n, i0 + grain);
-            for (int64_t i = i0; i < i1; ++i) fn(i);
+// A process-wide pool of host worker threads (up to 15, plus the caller):
+// jobs queue in order, every idle worker takes chunks of the front job, and a
+// caller works on its own job until all chunks are taken, then waits for the
+// chunks still running.  Several callers (the serving loop's slots) may submit
+// at once.  The pool is never torn down (its threads end with the process),
+// so no static destructor joins threads at exit.
+class WorkPool {
+public:
+    static WorkPool& get()
+    {
+        static WorkPool* pool = new WorkPool();
+        return *pool;
+    }
+    int workers() const { return int(threads_.size()); }
+    void run(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& fn)
+    {
+        auto job = std::make_shared<Job>();
+        job->fn = &fn;
+        job->n = n;
+        job->grain = grain;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            queue_.push_back(job);
         }
+        cv_.notify_all();
+        work(*job);
+        while (job->done.load(std::memory_order_acquire) < n) std::this_thread::yield();
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto it = queue_.begin(); it != queue_.end(); ++it)
+            if (*it == job) { queue_.erase(it); break; }
+    }
+
+private:
+    struct Job {
+        const std::function<void(int64_t, int64_t)>* fn = nullptr;
+        int64_t n = 0, grain = 1;
+        std::atomic<int64_t> next{0}, done{0};
     };
-    std::vector<std::thread> th;
-    for (int t = 1; t < nt; ++t) th.emplace_back(work);
-    work();
-    for (auto& t : th) t.join();
+    WorkPool()
+    {
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int nt = int(std::min<unsigned>(hw ? hw : 1, 16u)) - 1;
+        for (int t = 0; t < nt; ++t) threads_.emplace_back([this] { loop(); });
+    }
+    static void work(Job& j)
+    {
+        for (;;) {
+            const int64_t i0 = j.next.fetch_add(j.grain);
+            if (i0 >= j.n) break;
+            const int64_t i1 = std::min(j.n, i0 + j.grain);
+            (*j.fn)(i0, i1);
+            j.done.fetch_add(i1 - i0, std::memory_order_release);
+        }
+    }
+    void loop()
+    {
+        for (;;) {
+            std::shared_ptr<Job> job;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return !queue_.empty(); });
+                job = queue_.front();
+                if (job->next.load() >= job->n) {   // every chunk taken: drop it, look again
+                    queue_.pop_front();
+                    continue;
+                }
+            }
+            work(*job);
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::shared_ptr<Job>> queue_;
+    std::vector<std::thread> threads_;
+};
+
+// Run fn(i) for i in [0, n) on the pool, `grain` indices per chunk.
+template <class F> void parallel_for(int64_t n, F&& fn, int64_t grain = 2048)
+{
+    if (n <= 0) return;
+    if (n <= grain || WorkPool::get().workers() == 0) {
+        for (int64_t i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    const std::function<void(int64_t, int64_t)> chunk = [&](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) fn(i);
+    };
+    WorkPool::get().run(n, grain, chunk);
+}
+
+template <class Get>
+void Packed::build(int64_t n, Get get, const uint8_t* map, std::vector<uint8_t>* codes)
+{
+    off.resize(size_t(n));
+    int64_t pos = kSeqPad;
+    for (int64_t k = 0; k < n; ++k) {
+        off[size_t(k)] = pos;
+        const int len = get(k).second;
+        pos += len > 0 ? len : 0;
+    }
+    bytes.assign(size_t(pos + kSeqPad), 0);
+    if (codes) codes->assign(bytes.size(), map[0]);
+    parallel_for(n, [&](int64_t k) {
+        const auto sq = get(k);
+        if (sq.second <= 0) return;
+        uint8_t* dst = bytes.data() + off[size_t(k)];
+        std::memcpy(dst, sq.first, size_t(sq.second));
+        if (codes) {
+            uint8_t* c = codes->data() + off[size_t(k)];
+            for (int x = 0; x < sq.second; ++x) c[x] = map[dst[x]];
+        }
+    }, 16);
 }
 
 // ---------------------------------------------------------------------------
@@ -667,8 +759,6 @@ const char* run_wfa(const ga_align_req* reqs, int64_t n, const std::vector<int32
                     const std::vector<uint8_t>& trace, char* arena, WfaOut& out)
 {
     std::call_once(g_pool_once, init_pool);
-    Packed P, T;
-    P.begin(); T.begin();
     out.slot_off.resize(static_cast<size_t>(n));
     int64_t slot = 0;
     std::vector<int32_t> plen(static_cast<size_t>(n)), tlen(static_cast<size_t>(n));
@@ -679,13 +769,13 @@ const char* run_wfa(const ga_align_req* reqs, int64_t n, const std::vector<int32
         max_text = std::max(max_text, r.text_len);
         plen[size_t(i)] = r.pattern_len;
         tlen[size_t(i)] = r.text_len;
-        P.add(r.pattern, r.pattern_len);
-        T.add(r.text, r.text_len);
         out.slot_off[size_t(i)] = slot;
         slot += std::max(r.pattern_len, 0) + std::max(r.text_len, 0);
         any_trace |= trace[size_t(i)] != 0;
     }
-    P.end(); T.end();
+    Packed P, T;
+    P.build(n, [&](int64_t i) { return std::make_pair(reqs[i].pattern, reqs[i].pattern_len); });
+    T.build(n, [&](int64_t i) { return std::make_pair(reqs[i].text, reqs[i].text_len); });
     const WfaPlan plan = plan_wfa(n, smax.data(), plen.data(), tlen.data());
 
     hipStream_t st = acquire_stream();
@@ -826,6 +916,12 @@ struct SwMatrixHost {
     int32_t sub[32 * 32];
     int32_t gap_open, gap_extend;
     int score(uint8_t t, uint8_t p) const { return sub[code[t] * 32 + code[p]]; }
+    bool fits8() const
+    {
+        for (int x = 0; x < 32 * 32; ++x)
+            if (sub[x] < -128 || sub[x] > 127) return false;
+        return true;
+    }
 };
 
 // rescore_ok / wellformed_ok of a CIGAR under a substitution matrix: the same
@@ -940,8 +1036,6 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
     });
     order.insert(order.end(), packed.begin(), packed.end());
 
-    Packed T, P;
-    T.begin(); P.begin();
     std::vector<int32_t> tl(static_cast<size_t>(n)), pl(static_cast<size_t>(n));
     std::vector<int64_t> cig_off(static_cast<size_t>(n));
     std::vector<uint8_t> walk_p(static_cast<size_t>(n));
@@ -952,21 +1046,18 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
         const bool sk = skip[size_t(i)] != 0;
         tl[size_t(k)] = sk ? 0 : r.text_len;
         pl[size_t(k)] = sk ? 0 : r.pattern_len;
-        T.add(r.text, sk ? 0 : r.text_len);
-        P.add(r.pattern, sk ? 0 : r.pattern_len);
         cig_off[size_t(k)] = slot_off[size_t(i)];
         walk_p[size_t(k)] = walk[size_t(i)];
         if (k < n32) max_m32 = std::max(max_m32, tl[size_t(k)]);
     }
-    T.end(); P.end();
-    // residue codes, laid out like the bytes (the padding maps to code 0)
+    // the sequences in launch order, and (matrix scoring) their residue codes,
+    // laid out like the bytes (the padding maps to the code of byte 0)
+    Packed T, P;
     std::vector<uint8_t> tcode, pcode;
-    if (mat) {
-        tcode.resize(T.bytes.size());
-        pcode.resize(P.bytes.size());
-        for (size_t x = 0; x < tcode.size(); ++x) tcode[x] = mat->code[T.bytes[x]];
-        for (size_t x = 0; x < pcode.size(); ++x) pcode[x] = mat->code[P.bytes[x]];
-    }
+    T.build(n, [&](int64_t k) { return std::make_pair(reqs[order[size_t(k)]].text, tl[size_t(k)]); },
+            mat ? mat->code : nullptr, mat ? &tcode : nullptr);
+    P.build(n, [&](int64_t k) { return std::make_pair(reqs[order[size_t(k)]].pattern, pl[size_t(k)]); },
+            mat ? mat->code : nullptr, mat ? &pcode : nullptr);
 
     hipStream_t st = acquire_stream();
     if (!st) return "hipStreamCreate";
@@ -1039,7 +1130,7 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
                 L.cigar = d_cig; L.cig_off = d_slot; L.bnd = d_bnd; L.first_pair = int32_t(a);
                 L.n_total = int32_t(n);
                 L.tcode = d_tcode; L.pcode = d_pcode; L.sub = d_sub;
-                if (mat) { L.go = mat->gap_open; L.ge = mat->gap_extend; }
+                if (mat) { L.go = mat->gap_open; L.ge = mat->gap_extend; L.sub8 = mat->fits8(); }
                 if (ga::sw_launch(L, trace, st) != hipSuccess) err = "SW kernel launch";
                 a += len;
             }

@@ -32,6 +32,8 @@
 // lanes holding the maximum.
 #include "ga_device.hpp"
 
+#include <type_traits>
+
 namespace ga {
 namespace {
 
@@ -77,10 +79,10 @@ struct SwLane {
 // takes its left column from bHr/bEr (wave-uniform LDS or global reads), lane
 // 63 hands its column to the next strip through bHw/bEw.  Returns the 8
 // direction nibbles of the lane (TRACE).
-template <bool TRACE, bool MATRIX>
+template <bool TRACE, bool MATRIX, class PT = int>
 __device__ __forceinline__ uint32_t sw_chunk(SwLane& S, int t0, int lane, int m, bool jvalid, uint8_t pc,
                                              uint64_t tcur, const int* bHr, const int* bEr, int* bHw,
-                                             int* bEw, const int* prof, int go, int ge, int sm, int sx)
+                                             int* bEw, const PT* prof, int go, int ge, int sm, int sx)
 {
     uint32_t acc = 0;
 #pragma unroll
@@ -104,7 +106,7 @@ __device__ __forceinline__ uint32_t sw_chunk(SwLane& S, int t0, int lane, int m,
             { const int e2 = el - ge; if (e2 > ee) { ee = e2; code |= 4u; } }
             int ff = S.h - go;
             { const int f2 = S.f - ge; if (f2 > ff) { ff = f2; code |= 8u; } }
-            int hv = diag + (MATRIX ? prof[(tc & 31) * 64 + lane] : (tc == pc ? sm : sx));
+            int hv = diag + (MATRIX ? int(prof[(tc & 31) * 64 + lane]) : (tc == pc ? sm : sx));
             uint32_t hsrc = 1u;
             if (ee > hv) { hv = ee; hsrc = 2u; }
             if (ff > hv) { hv = ff; hsrc = 3u; }
@@ -227,14 +229,19 @@ constexpr int kMwWaves = 4;
 constexpr int kMwLag = 9;           // > 63 / 8 + 1 chunks
 constexpr size_t kSwMwMaxLds = 96 * 1024;
 
-__host__ __device__ inline size_t sw_mw_lds_bytes(int max_m)
+// P8: the profiles hold the scores as int8 (a table whose entries all fit),
+// a quarter of the LDS, so more workgroups fit a CU (BLOSUM62 with 2,000-row
+// texts: 43 KB instead of 67 KB, three workgroups instead of two).
+__host__ __device__ inline size_t sw_mw_lds_bytes(int max_m, bool p8)
 {
-    return size_t(kMwWaves) * kSwProfileInts * 4 + size_t(4) * size_t(max_m) * 4 + size_t(kMwWaves) * 64 * 12;
+    return size_t(kMwWaves) * kSwProfileInts * (p8 ? 1 : 4) + size_t(4) * size_t(max_m) * 4 +
+           size_t(kMwWaves) * 64 * 12;
 }
 
-template <bool TRACE>
+template <bool TRACE, bool P8>
 __global__ __launch_bounds__(64 * kMwWaves) void sw_matrix_mw_kernel(SwLaunch L)
 {
+    using PT = typename std::conditional<P8, int8_t, int>::type;
     extern __shared__ int lds[];
     const int slot = blockIdx.x;
     const int pid = L.first_pair + slot;
@@ -251,8 +258,8 @@ __global__ __launch_bounds__(64 * kMwWaves) void sw_matrix_mw_kernel(SwLaunch L)
         }
         return;
     }
-    int* prof = lds + w * kSwProfileInts;
-    int* bnd = lds + kMwWaves * kSwProfileInts;     // [buffer 0, 1][H, E][max_m]
+    PT* prof = reinterpret_cast<PT*>(lds) + w * kSwProfileInts;
+    int* bnd = reinterpret_cast<int*>(reinterpret_cast<PT*>(lds) + kMwWaves * kSwProfileInts);   // [buffer 0, 1][H, E][max_m]
     int* rec = bnd + 4 * L.max_m;                   // [wave][lane][H, I, J]
     const int mm = L.max_m;
     for (int q = tid; q < m; q += 64 * kMwWaves) { bnd[2 * mm + q] = 0; bnd[3 * mm + q] = 0; }
@@ -285,7 +292,7 @@ __global__ __launch_bounds__(64 * kMwWaves) void sw_matrix_mw_kernel(SwLaunch L)
                 j = (s << 6) + lane;
                 jvalid = j < n;
                 const uint8_t pc = jvalid ? L.pcode[L.poff[pid] + j] : uint8_t(0);
-                for (int k = 0; k < 32; ++k) prof[k * 64 + lane] = L.sub[k * 32 + pc];
+                for (int k = 0; k < 32; ++k) prof[k * 64 + lane] = PT(L.sub[k * 32 + pc]);
                 S = SwLane{0, 0, 0, 0, 0, -1};
                 tcur = load8(TS - lane);
             }
@@ -416,13 +423,18 @@ hipError_t sw_launch(const SwLaunch& L, bool trace, hipStream_t st, LaunchHooks*
     const size_t lds = (in_lds ? bnd_bytes : 0) + (matrix ? size_t(kSwProfileInts) * sizeof(int) : 0);
     // substitution-matrix batches (no re-run slots) take the multi-wave
     // pipeline while both boundary buffers fit the LDS next to the profiles
-    const size_t mw_lds = sw_mw_lds_bytes(L.max_m);
+    const size_t mw_lds = sw_mw_lds_bytes(L.max_m, L.sub8);
     const bool mw = matrix && !L.rerun && mw_lds <= kSwMwMaxLds;
     if (hooks) hooks->before(st);
     if (mw) {
         const dim3 grid(unsigned(L.n)), block(64 * kMwWaves);
-        if (trace) hipLaunchKernelGGL(sw_matrix_mw_kernel<true>, grid, block, mw_lds, st, L);
-        else       hipLaunchKernelGGL(sw_matrix_mw_kernel<false>, grid, block, mw_lds, st, L);
+        if (L.sub8) {
+            if (trace) hipLaunchKernelGGL((sw_matrix_mw_kernel<true, true>), grid, block, mw_lds, st, L);
+            else       hipLaunchKernelGGL((sw_matrix_mw_kernel<false, true>), grid, block, mw_lds, st, L);
+        } else {
+            if (trace) hipLaunchKernelGGL((sw_matrix_mw_kernel<true, false>), grid, block, mw_lds, st, L);
+            else       hipLaunchKernelGGL((sw_matrix_mw_kernel<false, false>), grid, block, mw_lds, st, L);
+        }
         if (hooks) hooks->after(st);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess || !trace) return e;
