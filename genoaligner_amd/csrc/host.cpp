@@ -276,12 +276,16 @@ struct Packed {
                                     std::vector<uint8_t>* codes = nullptr);
 };
 
-// A process-wide pool of host worker threads (up to 15, plus the caller):
-// jobs queue in order, every idle worker takes chunks of the front job, and a
-// caller works on its own job until all chunks are taken, then waits for the
-// chunks still running.  Several callers (the serving loop's slots) may submit
-// at once.  The pool is never torn down (its threads end with the process),
-// so no static destructor joins threads at exit.
+// A process-wide pool of host worker threads (half the host threads, at most
+// 8 with the caller), shared by all callers: jobs queue in order, woken
+// workers take chunks of the front job, and a caller works on its own job
+// until all chunks are taken, then sleeps until the chunks still running are
+// done.  Half, and sleeping rather than spinning, because the batch calls run
+// beside the serving loop's threads and the GPU runtime's: with a worker per
+// CPU and spinning callers, a caller could wait a whole scheduler time slice
+// for a preempted worker's chunk (serving p99 went from 5 to 20 ms).  The pool
+// is never torn down (its threads end with the process), so no static
+// destructor joins threads at exit.
 class WorkPool {
 public:
     static WorkPool& get()
@@ -300,9 +304,14 @@ public:
             std::lock_guard<std::mutex> lk(mu_);
             queue_.push_back(job);
         }
-        cv_.notify_all();
+        const int64_t chunks = (n + grain - 1) / grain;
+        const int wake = int(std::min<int64_t>(chunks - 1, int64_t(threads_.size())));
+        for (int k = 0; k < wake; ++k) cv_.notify_one();
         work(*job);
-        while (job->done.load(std::memory_order_acquire) < n) std::this_thread::yield();
+        {
+            std::unique_lock<std::mutex> lk(job->m);
+            job->cv.wait(lk, [&] { return job->done.load() >= n; });
+        }
         std::lock_guard<std::mutex> lk(mu_);
         for (auto it = queue_.begin(); it != queue_.end(); ++it)
             if (*it == job) { queue_.erase(it); break; }
@@ -313,11 +322,13 @@ private:
         const std::function<void(int64_t, int64_t)>* fn = nullptr;
         int64_t n = 0, grain = 1;
         std::atomic<int64_t> next{0}, done{0};
+        std::mutex m;
+        std::condition_variable cv;
     };
     WorkPool()
     {
         const unsigned hw = std::thread::hardware_concurrency();
-        const int nt = int(std::min<unsigned>(hw ? hw : 1, 16u)) - 1;
+        const int nt = int(std::min<unsigned>(hw ? hw : 1, 16u)) / 2 - 1;
         for (int t = 0; t < nt; ++t) threads_.emplace_back([this] { loop(); });
     }
     static void work(Job& j)
@@ -327,7 +338,10 @@ private:
             if (i0 >= j.n) break;
             const int64_t i1 = std::min(j.n, i0 + j.grain);
             (*j.fn)(i0, i1);
-            j.done.fetch_add(i1 - i0, std::memory_order_release);
+            if (j.done.fetch_add(i1 - i0) + (i1 - i0) >= j.n) {
+                std::lock_guard<std::mutex> lk(j.m);    // the last chunk wakes the caller
+                j.cv.notify_all();
+            }
         }
     }
     void loop()
