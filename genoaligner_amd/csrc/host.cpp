@@ -283,9 +283,8 @@ struct Packed {
 // done.  Half, and sleeping rather than spinning, because the batch calls run
 // beside the serving loop's threads and the GPU runtime's: with a worker per
 // CPU and spinning callers, a caller could wait a whole scheduler time slice
-// for a preempted worker's chunk (serving p99 went from 5 to 20 ms).  The pool
-// is never torn down (its threads end with the process), so no static
-// destructor joins threads at exit.
+// for a preempted worker's chunk.  The pool is never torn down (its threads
+// end with the process), so no static destructor joins threads at exit.
 class WorkPool {
 public:
     static WorkPool& get()
