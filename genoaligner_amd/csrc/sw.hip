@@ -78,13 +78,16 @@ struct SwLane {
 // Eight steps t0 .. t0+7 of a strip (rows t0 - lane .. t0 - lane + 7).  Lane 0
 // takes its left column from bHr/bEr (wave-uniform LDS or global reads), lane
 // 63 hands its column to the next strip through bHw/bEw.  Returns the 8
-// direction nibbles of the lane (TRACE).
-template <bool TRACE, bool MATRIX, class PT = int>
+// direction nibbles of the lane (TRACE).  FULL: every lane's 8 rows exist and
+// every lane's column exists (t0 >= 63, t0 + 8 <= m, a full strip), so no
+// row needs the activity test, and lane 63 hands its 8 rows over after them.
+template <bool TRACE, bool MATRIX, class PT = int, bool FULL = false>
 __device__ __forceinline__ uint32_t sw_chunk(SwLane& S, int t0, int lane, int m, bool jvalid, uint8_t pc,
                                              uint64_t tcur, const int* bHr, const int* bEr, int* bHw,
                                              int* bEw, const PT* prof, int go, int ge, int sm, int sx)
 {
     uint32_t acc = 0;
+    int hvs[8], ees[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
         const int i = t0 + u - lane;         // 0-based row of this step
@@ -98,7 +101,7 @@ __device__ __forceinline__ uint32_t sw_chunk(SwLane& S, int t0, int lane, int m,
         const int el = __builtin_amdgcn_update_dpp(be, S.e, 0x138, 0xf, 0xf, false);
         const int diag = S.hl_prev;
         S.hl_prev = hl;
-        const bool active = (i >= 0) && (i < m) && jvalid;
+        const bool active = FULL || ((i >= 0) && (i < m) && jvalid);
         if (active) {
             const uint8_t tc = uint8_t(tcur >> (8 * u));
             int ee = hl - go;
@@ -114,8 +117,13 @@ __device__ __forceinline__ uint32_t sw_chunk(SwLane& S, int t0, int lane, int m,
             S.h = hv; S.e = ee; S.f = ff;
             if (hv > S.bHv) { S.bHv = hv; S.bIv = i; }
             if (TRACE) acc |= (code | hsrc) << (4 * u);
-            if (lane == 63) { bHw[i] = hv; bEw[i] = ee; }
+            if (FULL) { hvs[u] = hv; ees[u] = ee; }
+            else if (lane == 63) { bHw[i] = hv; bEw[i] = ee; }
         }
+    }
+    if (FULL && lane == 63) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) { bHw[t0 + u - 63] = hvs[u]; bEw[t0 + u - 63] = ees[u]; }
     }
     return acc;
 }
@@ -195,6 +203,7 @@ __global__ __launch_bounds__(64) void sw_kernel(SwLaunch L)
     for (int s = 0; s < nstrips; ++s) {
         const int j = (s << 6) + lane;               // 0-based column of this lane
         const bool jvalid = j < n;
+        const bool full_strip = (s << 6) + 64 <= n;
         const uint8_t pc = jvalid ? (MATRIX ? L.pcode[L.poff[pid] + j] : P[j]) : uint8_t(0);
         if (MATRIX)
             for (int c = 0; c < 32; ++c) prof[c * 64 + lane] = L.sub[c * 32 + pc];
@@ -203,8 +212,11 @@ __global__ __launch_bounds__(64) void sw_kernel(SwLaunch L)
         for (int c = 0; c < nchunks; ++c) {
             const int t0 = c << 3;
             const uint64_t tnext = load8(TS + t0 + 8 - lane);
-            const uint32_t acc = sw_chunk<TRACE, MATRIX>(S, t0, lane, m, jvalid, pc, tcur, bH, bE, bH, bE,
-                                                         prof, go, ge, sm, sx);
+            const uint32_t acc = (t0 >= 63 && t0 + 8 <= m && full_strip)
+                ? sw_chunk<TRACE, MATRIX, int, true>(S, t0, lane, m, jvalid, pc, tcur, bH, bE, bH, bE,
+                                                     prof, go, ge, sm, sx)
+                : sw_chunk<TRACE, MATRIX, int, false>(S, t0, lane, m, jvalid, pc, tcur, bH, bE, bH, bE,
+                                                      prof, go, ge, sm, sx);
             if (TRACE) dirs[(int64_t(s) * nchunks + c) * 64 + lane] = acc;
             tcur = tnext;
         }
@@ -300,8 +312,11 @@ __global__ __launch_bounds__(64 * kMwWaves) void sw_matrix_mw_kernel(SwLaunch L)
             const uint64_t tnext = load8(TS + t0 + 8 - lane);
             const int* bHr = bnd + ((s + 1) & 1) * 2 * mm;
             int* bHw = bnd + (s & 1) * 2 * mm;
-            const uint32_t acc = sw_chunk<TRACE, true>(S, t0, lane, m, jvalid, 0, tcur, bHr, bHr + mm, bHw,
-                                                       bHw + mm, prof, go, ge, 0, 0);
+            const uint32_t acc = (t0 >= 63 && t0 + 8 <= m && (s << 6) + 64 <= n)
+                ? sw_chunk<TRACE, true, PT, true>(S, t0, lane, m, jvalid, 0, tcur, bHr, bHr + mm, bHw,
+                                                  bHw + mm, prof, go, ge, 0, 0)
+                : sw_chunk<TRACE, true, PT, false>(S, t0, lane, m, jvalid, 0, tcur, bHr, bHr + mm, bHw,
+                                                   bHw + mm, prof, go, ge, 0, 0);
             if (TRACE) dirs[(int64_t(s) * nchunks + c) * 64 + lane] = acc;
             tcur = tnext;
             if (c == nchunks - 1) sw_fold(S, j, gH, gI, gJ);
