@@ -31,7 +31,14 @@ EXPORTS = (
     "ga_trim_workspace",
     "ga_wfa_graph_create", "ga_wfa_graph_launch", "ga_wfa_graph_destroy",
     "ga_wfa_autotune", "ga_wfa_set_tuning", "ga_wfa_get_tuning",
+    "ga_fasta_parse", "ga_fasta_read_file", "ga_fasta_count", "ga_fasta_record", "ga_fasta_free",
+    "ga_fasta_device",
 )
+
+
+class FastaDevOut(C.Structure):
+    _fields_ = [("seq", C.c_void_p), ("seq_off", C.c_void_p), ("seq_len", C.c_void_p),
+                ("acgt", C.c_void_p), ("pack2", C.c_void_p), ("max_records", C.c_int64)]
 
 
 class AlignReq(C.Structure):
@@ -160,6 +167,21 @@ def lib() -> C.CDLL:
         L.ga_set_thread_workspace_budget.restype = None
         L.ga_trim_workspace.argtypes = []
         L.ga_trim_workspace.restype = None
+        L.ga_fasta_parse.argtypes = [C.c_char_p, C.c_int64, C.POINTER(C.c_void_p), cpp]
+        L.ga_fasta_parse.restype = C.c_int
+        L.ga_fasta_read_file.argtypes = [C.c_char_p, C.POINTER(C.c_void_p), cpp]
+        L.ga_fasta_read_file.restype = C.c_int
+        L.ga_fasta_count.argtypes = [C.c_void_p]
+        L.ga_fasta_count.restype = C.c_int64
+        i64p = C.POINTER(C.c_int64)
+        L.ga_fasta_record.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p), i64p,
+                                      C.POINTER(C.c_void_p), i64p, C.POINTER(C.c_void_p), i64p]
+        L.ga_fasta_record.restype = C.c_int
+        L.ga_fasta_free.argtypes = [C.c_void_p]
+        L.ga_fasta_free.restype = None
+        L.ga_fasta_device.argtypes = [C.c_void_p, C.c_int64, C.POINTER(FastaDevOut), i64p, i64p,
+                                      C.c_void_p, cpp]
+        L.ga_fasta_device.restype = C.c_int
         L.ga_route_bucket_bound.argtypes = [C.c_int32]
         L.ga_route_bucket_bound.restype = C.c_int32
         _lib = L

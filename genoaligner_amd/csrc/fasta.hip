@@ -1,0 +1,509 @@
+// fasta.hip -- FASTA ingest on the device: a whole file in HBM becomes its
+// records' sequences back to back (the byte layout the batch kernels read),
+// with the host reader's rules (fasta.cpp; reference include/genoaligner/io/
+// fasta.hpp:50-175).
+//
+// Every byte is classified from its line: a line starts at byte 0, after a
+// \n, and after a \r not followed by \n; a line whose first byte is '>' is a
+// header; \n and \r are never sequence.  A byte is kept iff its line starts
+// at or after the first header and is not a header, and the byte is not a
+// line break.  So a byte's fate depends only on the line start in force and
+// whether that line is a header: the state carried from tile to tile is one
+// number, (line start << 1) | header, combined by max.
+//
+// A tile is 256 threads x 32 bytes (8 KiB).  Two passes over the file's
+// tiles and one small scan between them:
+//   A  per tile: the last line state, the header count, the first header, the
+//      error candidates (a header with an empty id; a line before the first
+//      header starting with a byte other than '>', space, tab or a line
+//      break), and the kept bytes under either entry state: bytes before the
+//      tile's first line start (kept iff the line in force is a sequence
+//      line), bytes after it (decided inside the tile), and bytes after the
+//      tile's first header (for the tile holding the file's first header);
+//   S  one workgroup: the file's first header and error candidates, the
+//      line-state carry into each tile, headers and kept bytes before each
+//      tile, the record and byte totals, and whether the file is well formed
+//      (the kernels below do nothing otherwise);
+//   E  per tile: scatter of the kept bytes, each record's start, and a clear
+//      of the record's A/C/G/T flag at any other kept byte;
+//   F  per record: length and the empty-record check;
+//   G  (optional) the 2-bit codes of the whole sequence stream.
+// Each thread reads its 32 bytes with two 128-bit loads (when the file is
+// 16-byte aligned) plus the byte before and the byte after them.  The host
+// waits once, at the end; a malformed file is only flagged on the device and
+// the host reader words the message.
+#include "../../include/genoaligner/genoaligner_c.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+bool ga_fasta_host_message(const char* data, int64_t len, std::string& err);
+
+namespace {
+
+constexpr int kFaThreads = 256;
+constexpr int kFaWaves = kFaThreads / 64;
+constexpr int kFaBytes = 32;                          // per thread
+constexpr int64_t kFaTile = int64_t(kFaThreads) * kFaBytes;
+constexpr int kScanThreads = 1024;
+
+struct FaInfo {
+    int64_t first_hdr;      // first header line start (len if none)
+    int64_t n_records;
+    int64_t total;          // kept bytes
+    int64_t bad_pre;        // first offending line start (len if none)
+    int64_t bad_id;         // first header with an empty id (len if none)
+    int32_t bad_rec;        // 1: some record has no sequence byte; 2: one is longer than INT32_MAX
+    int32_t ok;             // well formed so far, and n_records <= max_records
+};
+
+// Per-tile values of pass A (ntiles each); S rewrites three of them.
+struct FaTiles {
+    int64_t* state;         // last (line start << 1 | header) in the tile, -1 if none -> carry into the tile
+    int64_t* nhdr;          // headers -> headers before the tile
+    int64_t* first_gt;      // first header in the tile (len if none)
+    int64_t* bad_pre;
+    int64_t* bad_id;
+    int64_t* kpre;          // non-break bytes before the tile's first line start
+    int64_t* kloc;          // kept bytes after it -> kept bytes before the tile
+    int64_t* kfh;           // kept bytes after the tile's first header
+};
+
+__device__ __forceinline__ bool fa_eol(uint8_t c) { return c == '\n' || c == '\r'; }
+__device__ __forceinline__ bool fa_blank(uint8_t c) { return c == ' ' || c == '\t' || fa_eol(c); }
+__device__ __forceinline__ bool fa_base(uint8_t c) { return c == 'A' || c == 'C' || c == 'G' || c == 'T'; }
+
+// One thread's 32 bytes, and its neighbours: prev = the byte before (a line
+// break at the file start, so byte 0 starts a line), next = the byte after
+// ('\n' past the end).  Bytes past the end read 0 and are never used.
+struct FaBytes {
+    uint8_t b[kFaBytes];
+    uint8_t prev, next;
+};
+template <bool AL>
+__device__ __forceinline__ FaBytes fa_load(const uint8_t* d, int64_t len, int64_t base)
+{
+    FaBytes r;
+    if (AL && base + kFaBytes <= len) {
+        const uint4 v0 = *reinterpret_cast<const uint4*>(d + base);
+        const uint4 v1 = *reinterpret_cast<const uint4*>(d + base + 16);
+        const uint32_t w[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+        for (int q = 0; q < kFaBytes; ++q) r.b[q] = uint8_t(w[q >> 2] >> (8 * (q & 3)));
+    } else {
+#pragma unroll
+        for (int q = 0; q < kFaBytes; ++q) r.b[q] = base + q < len ? d[base + q] : uint8_t(0);
+    }
+    r.prev = base > 0 && base - 1 < len ? d[base - 1] : uint8_t('\n');
+    r.next = base + kFaBytes < len ? d[base + kFaBytes] : uint8_t('\n');
+    return r;
+}
+// Does byte q of the thread start a line?
+__device__ __forceinline__ bool fa_starts(const FaBytes& r, int q)
+{
+    const uint8_t p = q ? r.b[q - 1] : r.prev;
+    return p == '\n' || (p == '\r' && r.b[q] != '\n');
+}
+
+struct OpSum { __device__ int64_t operator()(int64_t a, int64_t b) const { return a + b; } };
+struct OpMax { __device__ int64_t operator()(int64_t a, int64_t b) const { return a > b ? a : b; } };
+struct OpMin { __device__ int64_t operator()(int64_t a, int64_t b) const { return a < b ? a : b; } };
+
+template <class Op>
+__device__ __forceinline__ int64_t wave_reduce(int64_t v, Op op)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
+    return v;
+}
+template <class Op>
+__device__ __forceinline__ int64_t wave_inscan(int64_t v, Op op)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int64_t x = __shfl_up(v, off, 64);
+        if (lane >= off) v = op(v, x);
+    }
+    return v;
+}
+
+// Exclusive scan over the workgroup (at most 16 waves); *all gets the
+// reduction.  wtmp: 16 entries of LDS owned by this call site.
+template <class Op>
+__device__ int64_t block_exscan(int64_t v, int64_t id, Op op, int64_t* wtmp, int64_t* all)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int64_t inc = wave_inscan(v, op);
+    if (lane == 63) wtmp[w] = inc;
+    __syncthreads();
+    int64_t pre = id, tot = id;
+    for (int k = 0; k < nw; ++k) {
+        if (k < w) pre = op(pre, wtmp[k]);
+        tot = op(tot, wtmp[k]);
+    }
+    int64_t ex = __shfl_up(inc, 1, 64);
+    if (lane == 0) ex = id;
+    if (all) *all = tot;
+    __syncthreads();
+    return op(pre, ex);
+}
+template <class Op>
+__device__ int64_t block_reduce(int64_t v, int64_t id, Op op, int64_t* wtmp)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    v = wave_reduce(v, op);
+    if (lane == 0) wtmp[w] = v;
+    __syncthreads();
+    int64_t tot = id;
+    for (int k = 0; k < nw; ++k) tot = op(tot, wtmp[k]);
+    __syncthreads();
+    return tot;
+}
+
+// A: per-tile values.
+template <bool AL>
+__global__ __launch_bounds__(kFaThreads) void fa_tiles_a(const uint8_t* d, int64_t len, FaTiles T)
+{
+    __shared__ int64_t wt[2][16];
+    __shared__ int64_t part[6][kFaWaves];
+    const int64_t base = int64_t(blockIdx.x) * kFaTile + int64_t(threadIdx.x) * kFaBytes;
+    const FaBytes r = fa_load<AL>(d, len, base);
+    int64_t last = -1, first_gt = len, pre = len, bid = len;
+    int nh = 0, kpre = 0, kloc = 0, kfh = 0;
+    bool seen_ls = false, seen_gt = false, hdr = false;
+#pragma unroll
+    for (int q = 0; q < kFaBytes; ++q) {
+        const int64_t i = base + q;
+        if (i >= len) break;
+        const uint8_t c = r.b[q];
+        if (fa_starts(r, q)) {
+            seen_ls = true;
+            hdr = c == '>';
+            last = 2 * i + (hdr ? 1 : 0);
+            if (hdr) {
+                ++nh;
+                if (!seen_gt) {
+                    seen_gt = true;
+                    first_gt = i;
+                }
+                const uint8_t nx = i + 1 >= len ? uint8_t('\n') : q + 1 < kFaBytes ? r.b[q + 1] : r.next;
+                if (fa_blank(nx) && bid == len) bid = i;
+            } else if (!fa_blank(c) && pre == len) {
+                pre = i;
+            }
+        }
+        if (!fa_eol(c)) {
+            if (!seen_ls) ++kpre;
+            else if (!hdr) {
+                ++kloc;
+                kfh += seen_gt ? 1 : 0;
+            }
+        }
+    }
+    // the line in force at this thread's first byte, from the tile's earlier threads (-1: none)
+    int64_t tile_last;
+    const int64_t e = block_exscan(last, int64_t(-1), OpMax(), wt[0], &tile_last);
+    const int64_t g = block_reduce(first_gt, len, OpMin(), wt[1]);
+    const bool e_seq = e >= 0 && !(e & 1);
+    const int64_t c_pre = e < 0 ? kpre : 0;
+    const int64_t c_loc = kloc + (e_seq ? kpre : 0);
+    const int64_t c_fh = (g < len && first_gt == g) ? int64_t(kfh) : (g < base ? c_loc : 0);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t r_nh = wave_reduce(int64_t(nh), OpSum());
+    const int64_t r_pre = wave_reduce(pre, OpMin());
+    const int64_t r_bid = wave_reduce(bid, OpMin());
+    const int64_t r_cpre = wave_reduce(c_pre, OpSum());
+    const int64_t r_cloc = wave_reduce(c_loc, OpSum());
+    const int64_t r_cfh = wave_reduce(c_fh, OpSum());
+    if (lane == 0) {
+        part[0][w] = r_nh;
+        part[1][w] = r_pre;
+        part[2][w] = r_bid;
+        part[3][w] = r_cpre;
+        part[4][w] = r_cloc;
+        part[5][w] = r_cfh;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t s_nh = 0, s_pre = len, s_bid = len, s_cpre = 0, s_cloc = 0, s_cfh = 0;
+        for (int k = 0; k < kFaWaves; ++k) {
+            s_nh += part[0][k];
+            s_pre = part[1][k] < s_pre ? part[1][k] : s_pre;
+            s_bid = part[2][k] < s_bid ? part[2][k] : s_bid;
+            s_cpre += part[3][k];
+            s_cloc += part[4][k];
+            s_cfh += part[5][k];
+        }
+        const int64_t t = blockIdx.x;
+        T.state[t] = tile_last;
+        T.nhdr[t] = s_nh;
+        T.first_gt[t] = g;
+        T.bad_pre[t] = s_pre;
+        T.bad_id[t] = s_bid;
+        T.kpre[t] = s_cpre;
+        T.kloc[t] = s_cloc;
+        T.kfh[t] = s_cfh;
+    }
+}
+
+// S: one workgroup over the tile values, each thread a contiguous range of
+// tiles.  Leaves T.state = line-state carry into each tile, T.nhdr = headers
+// before it, T.kloc = kept bytes before it, and fills *info.
+__global__ __launch_bounds__(kScanThreads) void fa_scan(FaTiles T, int64_t n, int64_t len, int64_t max_records,
+                                                        FaInfo* info)
+{
+    __shared__ int64_t wt[6][16];
+    const int64_t per = (n + kScanThreads - 1) / kScanThreads;
+    const int64_t lo = int64_t(threadIdx.x) * per;
+    const int64_t hi = lo + per < n ? lo + per : n;
+    int64_t fh = len, bp = len, bi = len, st = -1, nh = 0;
+    for (int64_t k = lo; k < hi; ++k) {
+        fh = T.first_gt[k] < fh ? T.first_gt[k] : fh;
+        bp = T.bad_pre[k] < bp ? T.bad_pre[k] : bp;
+        bi = T.bad_id[k] < bi ? T.bad_id[k] : bi;
+        st = T.state[k] > st ? T.state[k] : st;
+        nh += T.nhdr[k];
+    }
+    const int64_t first_hdr = block_reduce(fh, len, OpMin(), wt[0]);
+    const int64_t bad_pre = block_reduce(bp, len, OpMin(), wt[1]);
+    const int64_t bad_id = block_reduce(bi, len, OpMin(), wt[2]);
+    int64_t n_rec;
+    int64_t cst = block_exscan(st, int64_t(-1), OpMax(), wt[3], nullptr);
+    int64_t chd = block_exscan(nh, int64_t(0), OpSum(), wt[4], &n_rec);
+    const int64_t ft = first_hdr < len ? first_hdr / kFaTile : n;
+    // the carries, and each tile's kept bytes under its entry state
+    int64_t kept = 0;
+    for (int64_t k = lo; k < hi; ++k) {
+        const int64_t s = T.state[k], h = T.nhdr[k];
+        T.state[k] = cst;
+        T.nhdr[k] = chd;
+        const bool seq_entry = cst >= 0 && !(cst & 1);
+        const int64_t kk = k < ft ? 0 : k == ft ? T.kfh[k] : T.kloc[k] + (seq_entry ? T.kpre[k] : 0);
+        T.kloc[k] = kk;
+        kept += kk;
+        cst = s > cst ? s : cst;
+        chd += h;
+    }
+    int64_t total;
+    int64_t ck = block_exscan(kept, int64_t(0), OpSum(), wt[5], &total);
+    for (int64_t k = lo; k < hi; ++k) {
+        const int64_t kk = T.kloc[k];
+        T.kloc[k] = ck;
+        ck += kk;
+    }
+    if (threadIdx.x == 0) {
+        info->first_hdr = first_hdr;
+        info->n_records = n_rec;
+        info->total = total;
+        info->bad_pre = bad_pre;
+        info->bad_id = bad_id;
+        info->bad_rec = 0;
+        info->ok = n_rec > 0 && bad_pre >= first_hdr && bad_id >= len && n_rec <= max_records ? 1 : 0;
+    }
+}
+
+__global__ __launch_bounds__(kFaThreads) void fa_flags_init(uint8_t* acgt, int64_t cap, const FaInfo* info)
+{
+    const int64_t k = int64_t(blockIdx.x) * kFaThreads + threadIdx.x;
+    if (info->ok && k < info->n_records && k < cap) acgt[k] = 1;
+}
+
+// E: scatter, record starts, and the A/C/G/T flags (acgt[] holds 1 for every
+// record on entry; a kept byte other than A, C, G, T clears its record's).
+template <bool AL>
+__global__ __launch_bounds__(kFaThreads) void fa_tiles_e(const uint8_t* d, int64_t len, FaTiles T,
+                                                        const FaInfo* info, uint8_t* seq, int64_t* seq_off,
+                                                        uint8_t* acgt, int64_t max_records)
+{
+    __shared__ int64_t wt[2][16];
+    if (!info->ok) return;                            // the same for the whole workgroup
+    const int64_t first_hdr = info->first_hdr;
+    const int64_t base = int64_t(blockIdx.x) * kFaTile + int64_t(threadIdx.x) * kFaBytes;
+    const FaBytes r = fa_load<AL>(d, len, base);
+    int64_t last = -1;
+    uint32_t hdr = 0;
+#pragma unroll
+    for (int q = 0; q < kFaBytes; ++q) {
+        const int64_t i = base + q;
+        if (i < len && fa_starts(r, q)) {
+            const bool h = r.b[q] == '>';
+            last = 2 * i + (h ? 1 : 0);
+            hdr |= h ? 1u << q : 0u;
+        }
+    }
+    const int64_t ex = block_exscan(last, int64_t(-1), OpMax(), wt[0], nullptr);
+    const int64_t carry = T.state[blockIdx.x];
+    int64_t state = ex > carry ? ex : carry;
+    uint32_t keep = 0;
+#pragma unroll
+    for (int q = 0; q < kFaBytes; ++q) {
+        const int64_t i = base + q;
+        if (i >= len) break;
+        const uint8_t c = r.b[q];
+        if (fa_starts(r, q)) state = 2 * i + (c == '>' ? 1 : 0);
+        if (state >= 0 && (state >> 1) >= first_hdr && !(state & 1) && !fa_eol(c)) keep |= 1u << q;
+    }
+    // kept bytes and headers before this thread in one scan: (kept << 32) | headers
+    const int64_t packed = (int64_t(__builtin_popcount(keep)) << 32) | int64_t(__builtin_popcount(hdr));
+    const int64_t pex = block_exscan(packed, int64_t(0), OpSum(), wt[1], nullptr);
+    int64_t pos = T.kloc[blockIdx.x] + (pex >> 32);
+    int64_t rec = T.nhdr[blockIdx.x] + (pex & 0xffffffff);
+    int64_t cleared = -1;
+#pragma unroll
+    for (int q = 0; q < kFaBytes; ++q) {
+        if (hdr & (1u << q)) {
+            if (rec < max_records) seq_off[rec] = pos;     // record rec starts at the next kept byte
+            ++rec;
+        }
+        if (keep & (1u << q)) {
+            const uint8_t c = r.b[q];
+            seq[pos++] = c;
+            // a kept byte follows the first header, so rec >= 1
+            if (!fa_base(c) && rec - 1 != cleared && rec - 1 < max_records) {
+                acgt[rec - 1] = 0;
+                cleared = rec - 1;
+            }
+        }
+    }
+}
+
+// F: one thread per record: length and the empty-record check.
+__global__ __launch_bounds__(kFaThreads) void fa_records(const int64_t* seq_off, int64_t cap, int32_t* seq_len,
+                                                        uint8_t* acgt, FaInfo* info)
+{
+    const int64_t k = int64_t(blockIdx.x) * kFaThreads + threadIdx.x;
+    if (!info->ok) return;
+    const int64_t n = info->n_records;
+    if (k >= n || k >= cap) return;
+    const int64_t o = seq_off[k];
+    const int64_t e = k + 1 < n ? seq_off[k + 1] : info->total;
+    seq_len[k] = e - o > INT32_MAX ? INT32_MAX : int32_t(e - o);
+    if (e == o) {
+        acgt[k] = 0;
+        info->bad_rec = 1;
+    } else if (e - o > INT32_MAX) {
+        info->bad_rec = 2;
+    }
+}
+
+// G: 2-bit codes of the sequence stream, 16 bases to a 32-bit word, base p at
+// bits 2(p mod 16) of word p/16: A 0, C 1, G 2, T 3 (((c >> 1) ^ (c >> 2)) & 3
+// of the ASCII byte), any other byte 0 -- its record's acgt flag is 0.
+template <bool AL>
+__global__ __launch_bounds__(kFaThreads) void fa_pack2(const uint8_t* seq, const FaInfo* info, uint32_t* pack)
+{
+    if (!info->ok) return;
+    const int64_t total = info->total;
+    const int64_t w = int64_t(blockIdx.x) * kFaThreads + threadIdx.x;
+    const int64_t p0 = w * 16;
+    if (p0 >= total) return;
+    uint8_t b[16];
+    if (AL && p0 + 16 <= total) {
+        const uint4 v = *reinterpret_cast<const uint4*>(seq + p0);
+        const uint32_t x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int q = 0; q < 16; ++q) b[q] = uint8_t(x[q >> 2] >> (8 * (q & 3)));
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) b[q] = p0 + q < total ? seq[p0 + q] : uint8_t(0);
+    }
+    uint32_t word = 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const uint32_t c = b[q];
+        word |= (fa_base(b[q]) ? ((c >> 1) ^ (c >> 2)) & 3u : 0u) << (2 * q);
+    }
+    pack[w] = word;
+}
+
+}  // namespace
+
+extern "C" int ga_fasta_device(const uint8_t* data, int64_t len, const ga_fasta_dev_out* out, int64_t* n_records,
+                               int64_t* total, void* stream, const char** error)
+{
+    static thread_local std::string msg;
+    if (error) *error = nullptr;
+    if (n_records) *n_records = 0;
+    if (total) *total = 0;
+    if (!out || !out->seq || !out->seq_off || !out->seq_len || !out->acgt || (!data && len > 0) || len < 0 ||
+        out->max_records < 0) {
+        if (error) *error = "null argument";
+        return GA_INVALID_ARGUMENT;
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    auto fail_dev = [&](const char* what) {
+        msg = what;
+        if (error) *error = msg.c_str();
+        return GA_DEVICE_ERROR;
+    };
+    const int64_t ntiles = len > 0 ? (len + kFaTile - 1) / kFaTile : 0;
+    // scratch: 8 per-tile arrays and the info block, one allocation
+    const int64_t nt = ntiles > 0 ? ntiles : 1;
+    void* scratch = nullptr;
+    if (hipMallocAsync(&scratch, size_t(8 * nt) * sizeof(int64_t) + sizeof(FaInfo), st) != hipSuccess)
+        return fail_dev("hipMalloc(FASTA scratch)");
+    int64_t* a = static_cast<int64_t*>(scratch);
+    const FaTiles T{a, a + nt, a + 2 * nt, a + 3 * nt, a + 4 * nt, a + 5 * nt, a + 6 * nt, a + 7 * nt};
+    FaInfo* info = reinterpret_cast<FaInfo*>(a + 8 * nt);
+    FaInfo h{};
+    hipError_t e = hipSuccess;
+    const bool al = (reinterpret_cast<uintptr_t>(data) & 15) == 0;
+    const bool sal = (reinterpret_cast<uintptr_t>(out->seq) & 15) == 0;
+    if (ntiles > 0) {
+        const dim3 g{unsigned(ntiles), 1, 1}, b{unsigned(kFaThreads), 1, 1};
+        if (al) hipLaunchKernelGGL(fa_tiles_a<true>, g, b, 0, st, data, len, T);
+        else    hipLaunchKernelGGL(fa_tiles_a<false>, g, b, 0, st, data, len, T);
+        hipLaunchKernelGGL(fa_scan, dim3(1), dim3(kScanThreads), 0, st, T, ntiles, len, out->max_records, info);
+        const int64_t cap = out->max_records < len ? out->max_records : len;   // records <= bytes
+        if (cap > 0) {
+            const dim3 rg{unsigned((cap + kFaThreads - 1) / kFaThreads), 1, 1};
+            hipLaunchKernelGGL(fa_flags_init, rg, b, 0, st, out->acgt, cap, info);
+            if (al)
+                hipLaunchKernelGGL(fa_tiles_e<true>, g, b, 0, st, data, len, T, info, out->seq, out->seq_off,
+                                   out->acgt, out->max_records);
+            else
+                hipLaunchKernelGGL(fa_tiles_e<false>, g, b, 0, st, data, len, T, info, out->seq, out->seq_off,
+                                   out->acgt, out->max_records);
+            hipLaunchKernelGGL(fa_records, rg, b, 0, st, out->seq_off, cap, out->seq_len, out->acgt, info);
+            if (out->pack2) {
+                const int64_t words = (len + 15) / 16;                           // total <= len
+                const dim3 pg{unsigned((words + kFaThreads - 1) / kFaThreads), 1, 1};
+                if (sal) hipLaunchKernelGGL(fa_pack2<true>, pg, b, 0, st, out->seq, info, out->pack2);
+                else     hipLaunchKernelGGL(fa_pack2<false>, pg, b, 0, st, out->seq, info, out->pack2);
+            }
+        }
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(&h, info, sizeof(FaInfo), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    (void)hipFreeAsync(scratch, st);
+    if (e != hipSuccess) return fail_dev("FASTA kernels");
+    const bool malformed = ntiles == 0 || h.n_records == 0 || h.bad_pre < h.first_hdr || h.bad_id < len ||
+                           (h.ok && h.bad_rec != 0);
+    if (malformed) {
+        // word the error as the host reader does, on a copy of the bytes
+        std::vector<char> host(static_cast<size_t>(len));
+        if (len > 0 && (hipMemcpy(host.data(), data, size_t(len), hipMemcpyDeviceToHost) != hipSuccess))
+            return fail_dev("hipMemcpy(FASTA bytes)");
+        std::string err;
+        if (ga_fasta_host_message(host.data(), len, err))
+            err = h.bad_rec == 2 ? "record longer than 2147483647 bytes" : "malformed FASTA";
+        msg = err;
+        if (error) *error = msg.c_str();
+        return GA_INVALID_ARGUMENT;
+    }
+    if (h.n_records > out->max_records) {
+        // *n_records reports the count needed; nothing was written
+        if (n_records) *n_records = h.n_records;
+        msg = "more records than max_records";
+        if (error) *error = msg.c_str();
+        return GA_INVALID_ARGUMENT;
+    }
+    if (n_records) *n_records = h.n_records;
+    if (total) *total = h.total;
+    return GA_OK;
+}

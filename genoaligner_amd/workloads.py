@@ -255,3 +255,34 @@ def serving_request(rng: np.random.Generator, api_mod, max_pairs: int = 512, wfa
                                pattern=ps.b_seq(i), pattern_len=int(ps.b_len[i]), with_cigar=cig)
              for i in range(k)]
     return "sw", items, cig
+
+
+def fasta_genome(total: int = 512 << 20, seed: int = 0, width: int = 60, rec_lo: int = 10_000,
+                 rec_hi: int = 2_000_000) -> np.ndarray:
+    """A synthetic multi-record FASTA file of about ``total`` bytes (uint8):
+    records of uniform length in [rec_lo, rec_hi), ``width``-byte lines, uniform
+    ACGT with the variations real assemblies carry: every third record 2 % N,
+    every fifth a soft-masked (lowercase) first quarter, every seventh CRLF
+    line ends, headers with a description."""
+    rng = np.random.default_rng(seed)
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    parts, size, k = [], 0, 0
+    while size < total:
+        ln = int(rng.integers(rec_lo, rec_hi))
+        s = acgt[rng.integers(0, 4, ln, dtype=np.uint8)]
+        if k % 3 == 1:
+            s[rng.integers(0, ln, ln // 50)] = ord("N")
+        if k % 5 == 2:
+            s[: ln // 4] += 32
+        eol = np.frombuffer(b"\r\n" if k % 7 == 3 else b"\n", np.uint8)
+        nl = (ln + width - 1) // width
+        pad = nl * width - ln
+        body = np.concatenate([s, np.zeros(pad, np.uint8)]).reshape(nl, width)
+        body = np.concatenate([body, np.tile(eol, (nl, 1))], axis=1).ravel()
+        if pad:
+            body = np.delete(body, np.s_[len(body) - len(eol) - pad:len(body) - len(eol)])
+        hdr = np.frombuffer(f">chr{k} synthetic record {k} length={ln}\n".encode(), np.uint8)
+        parts += [hdr, body]
+        size += len(hdr) + len(body)
+        k += 1
+    return np.concatenate(parts)

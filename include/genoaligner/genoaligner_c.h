@@ -301,6 +301,50 @@ void ga_set_thread_workspace_budget(int64_t bytes);
  * memory.) */
 void ga_trim_workspace(void);
 
+/* ---- FASTA (reference: include/genoaligner/io/fasta.hpp:50-175) ------------
+ * ga_fasta_parse / ga_fasta_read_file replace io::read_fasta /
+ * io::read_fasta_file (fasta.hpp:47-53) with the same rules: raw bytes kept
+ * (case, N, IUPAC codes), lines joined, \n, \r\n and a lone \r end a line,
+ * blank lines skipped, the first header token is the id and the rest the
+ * description.  A malformed file returns GA_INVALID_ARGUMENT and *error holds
+ * the reference's message, e.g. "record has a header but no sequence (line 3)"
+ * (valid until the next FASTA call on the thread).  The records stay valid
+ * until ga_fasta_free. */
+typedef struct ga_fasta ga_fasta;
+int     ga_fasta_parse(const char* data, int64_t len, ga_fasta** out, const char** error);
+int     ga_fasta_read_file(const char* path, ga_fasta** out, const char** error);
+int64_t ga_fasta_count(const ga_fasta* f);
+int     ga_fasta_record(const ga_fasta* f, int64_t k, const char** id, int64_t* id_len, const char** desc,
+                        int64_t* desc_len, const char** seq, int64_t* seq_len);
+void    ga_fasta_free(ga_fasta* f);
+
+/* FASTA ingest on the device (EXTENSION: no reference counterpart, whose reader
+ * runs on the host only).  `data` holds a whole FASTA file
+ * in device memory; its records' sequences are written back to back into
+ * `seq` (same rules and bytes as ga_fasta_parse), record k at seq_off[k] with
+ * seq_len[k] bytes, acgt[k] = 1 iff every byte is A, C, G or T (the 2-bit
+ * packable records; the others keep their literal bytes).  When pack2 is not
+ * NULL it receives the 2-bit codes of all of seq, 16 bases to a word, byte p of
+ * seq at bits 2*(p%16) of pack2[p/16], A 0, C 1, G 2, T 3 (any other byte 0;
+ * its record has acgt 0).  Buffers: seq >= len bytes, pack2 >= (len+15)/16
+ * words, the per-record arrays >= max_records entries (the number of '>' bytes
+ * bounds the records).  *n_records and *total (bytes in seq) return on the
+ * host; the call waits for the stream.  A malformed file returns
+ * GA_INVALID_ARGUMENT with ga_fasta_parse's message.  More than max_records
+ * records returns GA_INVALID_ARGUMENT with the count needed in *n_records
+ * (nothing written; call again with larger arrays); a record longer than
+ * INT32_MAX bytes also returns GA_INVALID_ARGUMENT. */
+typedef struct ga_fasta_dev_out {
+    uint8_t*  seq;
+    int64_t*  seq_off;
+    int32_t*  seq_len;
+    uint8_t*  acgt;
+    uint32_t* pack2;      /* optional */
+    int64_t   max_records;
+} ga_fasta_dev_out;
+int ga_fasta_device(const uint8_t* data, int64_t len, const ga_fasta_dev_out* out, int64_t* n_records,
+                    int64_t* total, void* stream, const char** error);
+
 #ifdef __cplusplus
 }
 #endif

@@ -100,6 +100,12 @@ def ref():
         R.ga_ref_align_sw_batch.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int,
                                             vp, vp, vp, vp]
         R.ga_ref_align_batch_status.argtypes = [vp, vp, vp, vp, vp, C.c_int]
+        R.ga_ref_fasta_parse.argtypes = [cp, C.c_longlong, C.POINTER(C.c_longlong), vp, C.c_int]
+        R.ga_ref_fasta_parse.restype = C.c_longlong
+        R.ga_ref_fasta_blob.argtypes = [vp]
+        R.ga_ref_fasta_blob.restype = None
+        R.ga_ref_fasta_file.argtypes = [cp, vp, C.c_int]
+        R.ga_ref_fasta_file.restype = C.c_longlong
         _ref = R
     return _ref
 
@@ -463,3 +469,116 @@ def mlp_backward(params: np.ndarray, x: np.ndarray, label: np.ndarray):
                                  h2.ctypes.data, lg.ctypes.data, label.ctypes.data, n, g.ctypes.data,
                                  loss.ctypes.data)
     return g, loss
+
+
+# ------------------------------------------------------------------ FASTA
+def fasta_parse(data: bytes):
+    """(records [(id, description, sequence)], None) or (None, error) for an
+    in-memory FASTA file: a restatement of io::read_fasta
+    (include/genoaligner/io/fasta.hpp:59-157), a byte scan with the same states
+    (between records / inside a record), line counting and messages."""
+    data = _b(data)
+    total, i, line = len(data), 0, 1
+    nl, cr, sp, tab, gt = 10, 13, 32, 9, 62
+
+    nxt = {b"\n": -1, b"\r": -1}      # next position of each line-break byte seen so far
+
+    def eol(j):                       # first \n or \r at or after j (total if none)
+        for ch in nxt:
+            if nxt[ch] < j <= total:
+                f = data.find(ch, j)
+                nxt[ch] = total if f < 0 else f
+        return min(nxt.values())
+
+    def consume_eol(j, ln):           # fasta.hpp:86-98
+        e = eol(j)
+        if e == total:
+            return total, ln
+        if data[e] == cr and e + 1 < total and data[e + 1] == nl:
+            return e + 2, ln + 1
+        return e + 1, ln + 1
+
+    recs, in_record, have_seq, record_line = [], False, False, 0
+    while i < total:
+        c = data[i]
+        if not in_record:             # fasta.hpp:105-137
+            if c in (nl, cr, sp, tab):
+                i, line = consume_eol(i, line)
+                continue
+            if c != gt:
+                return None, "expected '>' at start of a record, found '%s' (line %d)" % (
+                    chr(c), line)
+            e = eol(i + 1)
+            header = data[i + 1:e].rstrip(b" \t")
+            i = e
+            cut = [k for k, ch in enumerate(header) if ch in (sp, tab)]
+            rid = header if not cut else header[:cut[0]]
+            desc = b"" if not cut else header[cut[0]:].lstrip(b" \t")
+            if not rid:
+                return None, "empty identifier in header (line %d)" % line
+            recs.append([rid, desc, []])
+            in_record, have_seq, record_line = True, False, line
+            i, line = consume_eol(i, line)
+            continue
+        if c == gt:                   # fasta.hpp:141-146
+            if not have_seq:
+                return None, "record has a header but no sequence (line %d)" % record_line
+            in_record = False
+            continue
+        if c in (nl, cr):
+            i, line = consume_eol(i, line)
+            continue
+        e = eol(i)                    # fasta.hpp:150-153
+        recs[-1][2].append(data[i:e])
+        i = e
+        have_seq = True
+    if in_record and not have_seq:
+        return None, "record has a header but no sequence (line %d)" % record_line
+    if not recs:
+        return None, "no records found (line %d)" % line
+    return [(r[0], r[1], b"".join(r[2])) for r in recs], None
+
+
+def ref_fasta(data: bytes):
+    """The reference's own io::read_fasta (compiled into oracle/_ref), same
+    return convention as fasta_parse."""
+    data = _b(data)
+    R = ref()
+    blen = C.c_longlong()
+    err = C.create_string_buffer(512)
+    n = R.ga_ref_fasta_parse(data, len(data), C.byref(blen), err, 512)
+    if n == -1:
+        return None, err.value.decode("latin-1")
+    assert n >= 0, "ga_ref_fasta_parse: tmpfile failed"
+    blob = C.create_string_buffer(max(blen.value, 1))
+    R.ga_ref_fasta_blob(blob)
+    raw = blob.raw[:blen.value]
+    out, o = [], 0
+    for _ in range(n):
+        li, ld, ls = np.frombuffer(raw, dtype=np.int64, count=3, offset=o)
+        o += 24
+        f = []
+        for ln in (li, ld, ls):
+            f.append(raw[o:o + int(ln)])
+            o += int(ln)
+        out.append(tuple(f))
+    return out, None
+
+
+def ref_fasta_file(path: str):
+    """(record count, None) or (None, error) from io::read_fasta_file."""
+    err = C.create_string_buffer(512)
+    n = ref().ga_ref_fasta_file(os.fsencode(path), err, 512)
+    return (None, err.value.decode("latin-1")) if n < 0 else (int(n), None)
+
+
+def fasta_pack2(seq: bytes) -> np.ndarray:
+    """uint32 2-bit codes of a byte stream, 16 bases a word, base p at bits
+    2(p mod 16) of word p/16: A 0, C 1, G 2, T 3, any other byte 0."""
+    a = np.frombuffer(_b(seq), dtype=np.uint8)
+    code = np.zeros(len(a), dtype=np.uint32)
+    for ch, v in ((b"C", 1), (b"G", 2), (b"T", 3)):
+        code[a == ch[0]] = v
+    pad = (-len(a)) % 16
+    code = np.concatenate([code, np.zeros(pad, np.uint32)]).reshape(-1, 16)
+    return (code << (2 * np.arange(16, dtype=np.uint32))).sum(axis=1, dtype=np.uint64).astype(np.uint32)

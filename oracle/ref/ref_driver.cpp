@@ -23,7 +23,9 @@
 // headers -- are compiled exactly once, together with the entry points below.
 #include "src/api/api.cpp"
 #include "src/reference/edit_distance_cpu.hpp"
+#include "include/genoaligner/io/fasta.hpp"
 
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -209,6 +211,54 @@ int ga_ref_align_batch_status(const int* pat_len, const int* text_len,
     BatchResult b = align_batch(reqs);
     return b.status == BatchResult::Status::ok ? 0
          : b.status == BatchResult::Status::invalid_argument ? 1 : 2;
+}
+
+// io::read_fasta over an in-memory file (written to a tmpfile, as a caller's
+// file would be read).  Returns the record count, or -1 with the reader's
+// message in err.  The records are kept and copied out by ga_ref_fasta_blob as
+// [int64 id_len, desc_len, seq_len, id bytes, desc bytes, seq bytes] per record.
+static std::string g_fasta_blob;
+
+long long ga_ref_fasta_parse(const char* data, long long len, long long* blob_len, char* err, int err_cap)
+{
+    g_fasta_blob.clear();
+    *blob_len = 0;
+    FILE* f = std::tmpfile();
+    if (!f) return -2;
+    if (len > 0 && std::fwrite(data, 1, size_t(len), f) != size_t(len)) { std::fclose(f); return -2; }
+    std::rewind(f);
+    std::vector<io::FastaRecord> recs;
+    std::string e;
+    const bool ok = io::read_fasta(f, &recs, &e);
+    std::fclose(f);
+    if (!ok) {
+        std::snprintf(err, size_t(err_cap), "%s", e.c_str());
+        return -1;
+    }
+    for (const io::FastaRecord& r : recs) {
+        const long long l[3] = {(long long)r.id.size(), (long long)r.description.size(),
+                                (long long)r.sequence.size()};
+        g_fasta_blob.append(reinterpret_cast<const char*>(l), sizeof(l));
+        g_fasta_blob += r.id;
+        g_fasta_blob += r.description;
+        g_fasta_blob += r.sequence;
+    }
+    *blob_len = (long long)g_fasta_blob.size();
+    return (long long)recs.size();
+}
+
+void ga_ref_fasta_blob(char* out) { std::memcpy(out, g_fasta_blob.data(), g_fasta_blob.size()); }
+
+// io::read_fasta_file: returns the record count or -1 with the message in err.
+long long ga_ref_fasta_file(const char* path, char* err, int err_cap)
+{
+    std::vector<io::FastaRecord> recs;
+    std::string e;
+    if (!io::read_fasta_file(path, &recs, &e)) {
+        std::snprintf(err, size_t(err_cap), "%s", e.c_str());
+        return -1;
+    }
+    return (long long)recs.size();
 }
 
 }  // extern "C"
