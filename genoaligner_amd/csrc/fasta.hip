@@ -11,7 +11,7 @@
 // whether that line is a header: the state carried from tile to tile is one
 // number, (line start << 1) | header, combined by max.
 //
-// A tile is 256 threads x 32 bytes (8 KiB).  Two passes over the file's
+// A tile is 512 threads x 32 bytes (16 KiB).  Two passes over the file's
 // tiles and one small scan between them:
 //   A  per tile: the last line state, the header count, the first header, the
 //      error candidates (a header with an empty id; a line before the first
@@ -44,7 +44,7 @@ bool ga_fasta_host_message(const char* data, int64_t len, std::string& err);
 
 namespace {
 
-constexpr int kFaThreads = 256;
+constexpr int kFaThreads = 512;
 constexpr int kFaWaves = kFaThreads / 64;
 constexpr int kFaBytes = 32;                          // per thread
 constexpr int64_t kFaTile = int64_t(kFaThreads) * kFaBytes;
@@ -72,8 +72,6 @@ struct FaTiles {
     int64_t* kfh;           // kept bytes after the tile's first header
 };
 
-__device__ __forceinline__ bool fa_eol(uint8_t c) { return c == '\n' || c == '\r'; }
-__device__ __forceinline__ bool fa_blank(uint8_t c) { return c == ' ' || c == '\t' || fa_eol(c); }
 __device__ __forceinline__ bool fa_base(uint8_t c) { return c == 'A' || c == 'C' || c == 'G' || c == 'T'; }
 
 // One thread's 32 bytes, and its neighbours: prev = the byte before (a line
@@ -101,11 +99,77 @@ __device__ __forceinline__ FaBytes fa_load(const uint8_t* d, int64_t len, int64_
     r.next = base + kFaBytes < len ? d[base + kFaBytes] : uint8_t('\n');
     return r;
 }
-// Does byte q of the thread start a line?
-__device__ __forceinline__ bool fa_starts(const FaBytes& r, int q)
+// Bit k of the result: byte k of x equals the byte in c4 (c * 0x01010101).
+// Exact per byte (no borrow between bytes): the high bit of each byte of
+// ((t & 0x7f..) + 0x7f..) | t is set iff that byte of t is nonzero.
+__device__ __forceinline__ uint32_t fa_eq4(uint32_t x, uint32_t c4)
 {
-    const uint8_t p = q ? r.b[q - 1] : r.prev;
-    return p == '\n' || (p == '\r' && r.b[q] != '\n');
+    const uint32_t t = x ^ c4;
+    const uint32_t z = ~(((t & 0x7f7f7f7fu) + 0x7f7f7f7fu) | t) & 0x80808080u;
+    return ((z >> 7) * 0x00204081u) >> 21 & 0xfu;     // bits 0, 8, 16, 24 -> 0..3
+}
+__device__ __forceinline__ uint32_t fa_mask(const uint32_t (&w)[kFaBytes / 4], uint8_t c)
+{
+    const uint32_t c4 = uint32_t(c) * 0x01010101u;
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < kFaBytes / 4; ++k) m |= fa_eq4(w[k], c4) << (4 * k);
+    return m;
+}
+__device__ __forceinline__ uint32_t fa_from(int q) { return q >= 32 ? 0u : ~0u << q; }     // bits q..31
+__device__ __forceinline__ uint32_t fa_below(int q) { return q >= 32 ? ~0u : (1u << q) - 1u; }  // bits 0..q-1
+
+__device__ __forceinline__ uint32_t fa_mask_bytes(const FaBytes& r, uint8_t c)
+{
+    uint32_t w[kFaBytes / 4];
+#pragma unroll
+    for (int k = 0; k < kFaBytes / 4; ++k)
+        w[k] = uint32_t(r.b[4 * k]) | uint32_t(r.b[4 * k + 1]) << 8 | uint32_t(r.b[4 * k + 2]) << 16 |
+               uint32_t(r.b[4 * k + 3]) << 24;
+    return fa_mask(w, c);
+}
+
+// The classes of one thread's 32 bytes as bit masks (bit q = byte base + q).
+struct FaMasks {
+    uint32_t valid;         // inside the file
+    uint32_t eol;           // \n or \r
+    uint32_t ls;            // starts a line
+    uint32_t hs;            // starts a header line
+    uint32_t hl;            // inside a header line that starts in these bytes
+    uint32_t other;         // not ' ', '\t', '\n', '\r' or '>'
+    uint32_t blank;         // ' ', '\t', '\n' or '\r'
+};
+template <bool AL>
+__device__ __forceinline__ FaMasks fa_masks(const uint8_t* d, int64_t len, int64_t base, FaBytes& r)
+{
+    r = fa_load<AL>(d, len, base);
+    uint32_t w[kFaBytes / 4];
+#pragma unroll
+    for (int k = 0; k < kFaBytes / 4; ++k)
+        w[k] = uint32_t(r.b[4 * k]) | uint32_t(r.b[4 * k + 1]) << 8 | uint32_t(r.b[4 * k + 2]) << 16 |
+               uint32_t(r.b[4 * k + 3]) << 24;
+    FaMasks m;
+    const int64_t nv = len - base;
+    m.valid = nv >= kFaBytes ? ~0u : nv <= 0 ? 0u : fa_below(int(nv));
+    const uint32_t nl = fa_mask(w, '\n'), cr = fa_mask(w, '\r'), gt = fa_mask(w, '>');
+    const uint32_t sp = fa_mask(w, ' '), tab = fa_mask(w, '\t');
+    m.eol = (nl | cr) & m.valid;
+    // byte q starts a line iff byte q-1 is \n, or is \r and byte q is not \n
+    const uint32_t pnl = nl << 1 | (r.prev == '\n' ? 1u : 0u);
+    const uint32_t pcr = cr << 1 | (r.prev == '\r' ? 1u : 0u);
+    m.ls = (pnl | (pcr & ~nl)) & m.valid;
+    m.hs = m.ls & gt;
+    m.other = ~(nl | cr | gt | sp | tab) & m.valid;
+    m.blank = (nl | cr | sp | tab) & m.valid;
+    m.hl = 0;
+    for (uint32_t rem = m.hs; rem;) {                 // each header line runs to the next line start
+        const int s = __builtin_ctz(rem);
+        rem &= rem - 1;
+        const uint32_t after = m.ls & fa_from(s + 1);
+        const int e = after ? __builtin_ctz(after) : 32;
+        m.hl |= fa_from(s) & fa_below(e);
+    }
+    return m;
 }
 
 struct OpSum { __device__ int64_t operator()(int64_t a, int64_t b) const { return a + b; } };
@@ -171,39 +235,28 @@ __global__ __launch_bounds__(kFaThreads) void fa_tiles_a(const uint8_t* d, int64
     __shared__ int64_t wt[2][16];
     __shared__ int64_t part[6][kFaWaves];
     const int64_t base = int64_t(blockIdx.x) * kFaTile + int64_t(threadIdx.x) * kFaBytes;
-    const FaBytes r = fa_load<AL>(d, len, base);
-    int64_t last = -1, first_gt = len, pre = len, bid = len;
-    int nh = 0, kpre = 0, kloc = 0, kfh = 0;
-    bool seen_ls = false, seen_gt = false, hdr = false;
-#pragma unroll
-    for (int q = 0; q < kFaBytes; ++q) {
-        const int64_t i = base + q;
-        if (i >= len) break;
-        const uint8_t c = r.b[q];
-        if (fa_starts(r, q)) {
-            seen_ls = true;
-            hdr = c == '>';
-            last = 2 * i + (hdr ? 1 : 0);
-            if (hdr) {
-                ++nh;
-                if (!seen_gt) {
-                    seen_gt = true;
-                    first_gt = i;
-                }
-                const uint8_t nx = i + 1 >= len ? uint8_t('\n') : q + 1 < kFaBytes ? r.b[q + 1] : r.next;
-                if (fa_blank(nx) && bid == len) bid = i;
-            } else if (!fa_blank(c) && pre == len) {
-                pre = i;
-            }
-        }
-        if (!fa_eol(c)) {
-            if (!seen_ls) ++kpre;
-            else if (!hdr) {
-                ++kloc;
-                kfh += seen_gt ? 1 : 0;
-            }
-        }
+    FaBytes r;
+    const FaMasks m = fa_masks<AL>(d, len, base, r);
+    const int fls = m.ls ? __builtin_ctz(m.ls) : 32;
+    const int fhs = m.hs ? __builtin_ctz(m.hs) : 32;
+    const uint32_t seqb = m.valid & ~m.eol;           // bytes that are sequence if their line is
+    const int kpre = __builtin_popcount(seqb & fa_below(fls));
+    const int kloc = __builtin_popcount(seqb & fa_from(fls) & ~m.hl);
+    const int kfh = __builtin_popcount(seqb & fa_from(fhs) & ~m.hl);
+    int64_t last = -1;
+    if (m.ls) {
+        const int q = 31 - __builtin_clz(m.ls);
+        last = 2 * (base + q) + ((m.hs >> q) & 1);
     }
+    const int64_t first_gt = m.hs ? base + fhs : len;
+    const uint32_t bp = m.ls & m.other;               // a line that must not come before the first header
+    const int64_t pre = bp ? base + __builtin_ctz(bp) : len;
+    // a header whose next byte is a blank, a line break or the end of the file has an empty id
+    const bool next_blank = base + kFaBytes >= len || r.next == ' ' || r.next == '\t' || r.next == '\n' ||
+                            r.next == '\r';
+    const uint32_t blank_next = ((m.blank | ~m.valid) >> 1) | (next_blank ? 0x80000000u : 0u);
+    const uint32_t bi_m = m.hs & blank_next;
+    const int64_t bid = bi_m ? base + __builtin_ctz(bi_m) : len;
     // the line in force at this thread's first byte, from the tile's earlier threads (-1: none)
     int64_t tile_last;
     const int64_t e = block_exscan(last, int64_t(-1), OpMax(), wt[0], &tile_last);
@@ -213,7 +266,7 @@ __global__ __launch_bounds__(kFaThreads) void fa_tiles_a(const uint8_t* d, int64
     const int64_t c_loc = kloc + (e_seq ? kpre : 0);
     const int64_t c_fh = (g < len && first_gt == g) ? int64_t(kfh) : (g < base ? c_loc : 0);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t r_nh = wave_reduce(int64_t(nh), OpSum());
+    const int64_t r_nh = wave_reduce(int64_t(__builtin_popcount(m.hs)), OpSum());
     const int64_t r_pre = wave_reduce(pre, OpMin());
     const int64_t r_bid = wave_reduce(bid, OpMin());
     const int64_t r_cpre = wave_reduce(c_pre, OpSum());
@@ -314,61 +367,77 @@ __global__ __launch_bounds__(kFaThreads) void fa_flags_init(uint8_t* acgt, int64
 
 // E: scatter, record starts, and the A/C/G/T flags (acgt[] holds 1 for every
 // record on entry; a kept byte other than A, C, G, T clears its record's).
+// The tile's kept bytes are gathered in LDS at (output address mod 16) + their
+// offset in the tile, so that 16-byte-aligned output runs are 16-byte-aligned
+// in LDS, then written with 128-bit stores (bytes at the two ends).
 template <bool AL>
 __global__ __launch_bounds__(kFaThreads) void fa_tiles_e(const uint8_t* d, int64_t len, FaTiles T,
                                                         const FaInfo* info, uint8_t* seq, int64_t* seq_off,
                                                         uint8_t* acgt, int64_t max_records)
 {
     __shared__ int64_t wt[2][16];
+    __shared__ __align__(16) uint8_t stage[kFaTile + 16];
     if (!info->ok) return;                            // the same for the whole workgroup
     const int64_t first_hdr = info->first_hdr;
     const int64_t base = int64_t(blockIdx.x) * kFaTile + int64_t(threadIdx.x) * kFaBytes;
-    const FaBytes r = fa_load<AL>(d, len, base);
+    FaBytes r;
+    const FaMasks m = fa_masks<AL>(d, len, base, r);
     int64_t last = -1;
-    uint32_t hdr = 0;
-#pragma unroll
-    for (int q = 0; q < kFaBytes; ++q) {
-        const int64_t i = base + q;
-        if (i < len && fa_starts(r, q)) {
-            const bool h = r.b[q] == '>';
-            last = 2 * i + (h ? 1 : 0);
-            hdr |= h ? 1u << q : 0u;
-        }
+    if (m.ls) {
+        const int q = 31 - __builtin_clz(m.ls);
+        last = 2 * (base + q) + ((m.hs >> q) & 1);
     }
     const int64_t ex = block_exscan(last, int64_t(-1), OpMax(), wt[0], nullptr);
     const int64_t carry = T.state[blockIdx.x];
-    int64_t state = ex > carry ? ex : carry;
-    uint32_t keep = 0;
-#pragma unroll
-    for (int q = 0; q < kFaBytes; ++q) {
-        const int64_t i = base + q;
-        if (i >= len) break;
-        const uint8_t c = r.b[q];
-        if (fa_starts(r, q)) state = 2 * i + (c == '>' ? 1 : 0);
-        if (state >= 0 && (state >> 1) >= first_hdr && !(state & 1) && !fa_eol(c)) keep |= 1u << q;
-    }
+    const int64_t entry = ex > carry ? ex : carry;    // the line in force at this thread's first byte
+    const int fls = m.ls ? __builtin_ctz(m.ls) : 32;
+    // bytes before the first line start here: kept iff the entry line is a sequence line after the first header
+    const bool entry_keep = entry >= 0 && !(entry & 1) && (entry >> 1) >= first_hdr;
+    // bytes of lines starting here: kept iff the line is not a header and starts at or after the first header
+    const int64_t cut = first_hdr - base;
+    const uint32_t after_first = cut <= 0 ? ~0u : cut >= 32 ? 0u : fa_from(int(cut));
+    const uint32_t keep = m.valid & ~m.eol & ~m.hl &
+                          ((entry_keep ? fa_below(fls) : 0u) | (fa_from(fls) & after_first));
     // kept bytes and headers before this thread in one scan: (kept << 32) | headers
-    const int64_t packed = (int64_t(__builtin_popcount(keep)) << 32) | int64_t(__builtin_popcount(hdr));
-    const int64_t pex = block_exscan(packed, int64_t(0), OpSum(), wt[1], nullptr);
-    int64_t pos = T.kloc[blockIdx.x] + (pex >> 32);
+    const int64_t packed = (int64_t(__builtin_popcount(keep)) << 32) | int64_t(__builtin_popcount(m.hs));
+    int64_t tile_sum;
+    const int64_t pex = block_exscan(packed, int64_t(0), OpSum(), wt[1], &tile_sum);
+    const int64_t ob = T.kloc[blockIdx.x];           // output offset of the tile's first kept byte
+    const int sh = int(ob & 15);
+    int lp = int(pex >> 32);                          // offset in the tile of this thread's first kept byte
     int64_t rec = T.nhdr[blockIdx.x] + (pex & 0xffffffff);
-    int64_t cleared = -1;
-#pragma unroll
-    for (int q = 0; q < kFaBytes; ++q) {
-        if (hdr & (1u << q)) {
-            if (rec < max_records) seq_off[rec] = pos;     // record rec starts at the next kept byte
-            ++rec;
-        }
-        if (keep & (1u << q)) {
-            const uint8_t c = r.b[q];
-            seq[pos++] = c;
-            // a kept byte follows the first header, so rec >= 1
-            if (!fa_base(c) && rec - 1 != cleared && rec - 1 < max_records) {
-                acgt[rec - 1] = 0;
-                cleared = rec - 1;
-            }
-        }
+    // record starts, and the A/C/G/T flag per record segment of these bytes
+    const uint32_t a_ = fa_mask_bytes(r, 'A') | fa_mask_bytes(r, 'C') | fa_mask_bytes(r, 'G') |
+                        fa_mask_bytes(r, 'T');
+    const uint32_t odd = keep & ~a_;                  // kept bytes that are not A, C, G or T
+    uint32_t seg = fa_below(m.hs ? __builtin_ctz(m.hs) : 32);
+    if ((odd & seg) && rec >= 1 && rec - 1 < max_records) acgt[rec - 1] = 0;
+    for (uint32_t rem = m.hs; rem;) {
+        const int s = __builtin_ctz(rem);
+        rem &= rem - 1;
+        if (rec < max_records) seq_off[rec] = ob + lp + __builtin_popcount(keep & fa_below(s));
+        seg = fa_from(s) & fa_below(rem ? __builtin_ctz(rem) : 32);
+        if ((odd & seg) && rec < max_records) acgt[rec] = 0;
+        ++rec;
     }
+#pragma unroll
+    for (int q = 0; q < kFaBytes; ++q)
+        if (keep & (1u << q)) stage[sh + lp++] = r.b[q];
+    __syncthreads();
+    const int nk = int(tile_sum >> 32);
+    if (nk == 0) return;
+    // output [ob, ob + nk): bytes up to the first 16-byte boundary, 128-bit words, bytes after the last
+    const int64_t a0 = (ob + 15) & ~int64_t(15), a1 = (ob + nk) & ~int64_t(15);
+    if (a0 >= a1) {
+        for (int j = threadIdx.x; j < nk; j += kFaThreads) seq[ob + j] = stage[sh + j];
+        return;
+    }
+    const int head = int(a0 - ob), words = int((a1 - a0) >> 4), tail = int(ob + nk - a1);
+    for (int j = threadIdx.x; j < head; j += kFaThreads) seq[ob + j] = stage[sh + j];
+    for (int j = threadIdx.x; j < words; j += kFaThreads)
+        *reinterpret_cast<uint4*>(seq + a0 + 16 * int64_t(j)) =
+            *reinterpret_cast<const uint4*>(stage + sh + head + 16 * j);
+    for (int j = threadIdx.x; j < tail; j += kFaThreads) seq[a1 + j] = stage[sh + head + 16 * words + j];
 }
 
 // F: one thread per record: length and the empty-record check.
