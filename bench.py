@@ -212,6 +212,9 @@ def main():
     ap.add_argument("--no-protein", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-fasta", action="store_true")
+    ap.add_argument("--no-gene-qc", action="store_true")
+    ap.add_argument("--gene-qc-records", type=int, default=20000)
+    ap.add_argument("--gene-qc-cpu-records", type=int, default=300)
     ap.add_argument("--fasta-mib", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=100.0, help="requests per second (Poisson)")
     ap.add_argument("--serve-requests", type=int, default=200)
@@ -542,6 +545,53 @@ def main():
             }
         log(f"fasta: {fasta_line}")
 
+    # ---------------- gene QC task: one bait, many records ----------------
+    qc_line = None
+    if not args.no_gene_qc:
+        import io
+        from genoaligner_amd import gene_qc as G
+        seqs, refs = W.gene_family(args.gene_qc_records, codons=380, seed=21)
+        qdir = tempfile.mkdtemp()
+        sp, rp = os.path.join(qdir, "seqs.fa"), os.path.join(qdir, "ref.fa")
+        open(sp, "wb").write(seqs)
+        open(rp, "wb").write(refs)
+        G.run(sp, rp, code=2, emit=os.path.join(qdir, "warm.tsv"), out=io.StringIO(), err=io.StringIO())
+        qc_ms = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            G.run(sp, rp, code=2, emit=os.path.join(qdir, "got.tsv"), out=io.StringIO(), err=io.StringIO())
+            qc_ms.append((time.perf_counter() - t0) * 1e3)
+        med = float(np.median(qc_ms))
+        got_tsv = open(os.path.join(qdir, "got.tsv"), encoding="latin-1").read()
+        n_rec = args.gene_qc_records
+        cells = sum(int(ln.split("\t")[1]) for ln in got_tsv.splitlines()[1:]) * 1141
+        qc_line = {
+            "workload": f"gene_family {n_rec} records vs a 1,141 bp bait (CYTB-sized, code 2, 3 % "
+                        "substitutions, frameshifts, stops, fragments, flanks), FASTA files to TSV",
+            "value": round(n_rec / (med * 1e-3), 1), "unit": "records/s", "ms_per_run": round(med, 2),
+            "gcups": round(cells / (med * 1e-3) / 1e9, 1),   # record length x bait length, whole run
+        }
+        if os.path.exists(O.REF_GENE_QC):
+            ssub, _ = W.gene_family(args.gene_qc_cpu_records, codons=380, seed=21)
+            ssp = os.path.join(qdir, "sample.fa")
+            open(ssp, "wb").write(ssub)
+            # checked on the sample: the reference tool on the CPU takes ~13 ms a record
+            G.run(ssp, rp, code=2, emit=os.path.join(qdir, "got_s.tsv"), out=io.StringIO(), err=io.StringIO())
+            t0 = time.perf_counter()
+            rc, _, _, ref_tsv = O.ref_gene_qc(ssp, rp, ("--code", "2"), emit=os.path.join(qdir, "ref_s.tsv"))
+            el = time.perf_counter() - t0
+            qc_line["matches_reference_tool"] = bool(
+                rc == 0 and ref_tsv == open(os.path.join(qdir, "got_s.tsv"), encoding="latin-1").read())
+            qc_line["checked_records"] = args.gene_qc_cpu_records
+            if not args.no_cpu_baseline:
+                qc_line["cpu_baseline"] = {
+                    "value": round(args.gene_qc_cpu_records / el, 1), "unit": "records/s", "cores": 1,
+                    "kind": "reference",
+                    "sample": f"the reference gene_qc (tools/gene_qc.cpp on its CPU shim) over the first "
+                              f"{args.gene_qc_cpu_records} records of the same family",
+                }
+        log(f"gene_qc: {qc_line}")
+
     # ---------------- serving: Poisson arrivals, continuous batching ----------------
     serve_line = None
     if not args.no_serve:
@@ -615,6 +665,7 @@ def main():
         "sw_protein": protein_line,
         "serving": serve_line,
         "fasta_ingest": fasta_line,
+        "gene_qc": qc_line,
     }
     print(json.dumps(line), flush=True)
 

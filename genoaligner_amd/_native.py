@@ -32,8 +32,13 @@ EXPORTS = (
     "ga_wfa_graph_create", "ga_wfa_graph_launch", "ga_wfa_graph_destroy",
     "ga_wfa_autotune", "ga_wfa_set_tuning", "ga_wfa_get_tuning",
     "ga_fasta_parse", "ga_fasta_read_file", "ga_fasta_count", "ga_fasta_record", "ga_fasta_free",
-    "ga_fasta_device",
+    "ga_fasta_device", "ga_gene_qc_walk",
 )
+
+
+class QcRow(C.Structure):
+    _fields_ = [("cov_text", C.c_double), ("fs_runs", C.c_int32), ("fs_net", C.c_int32),
+                ("stops", C.c_int32), ("stop_end", C.c_int32)]
 
 
 class FastaDevOut(C.Structure):
@@ -182,6 +187,9 @@ def lib() -> C.CDLL:
         L.ga_fasta_device.argtypes = [C.c_void_p, C.c_int64, C.POINTER(FastaDevOut), i64p, i64p,
                                       C.c_void_p, cpp]
         L.ga_fasta_device.restype = C.c_int
+        L.ga_gene_qc_walk.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_int32, C.c_int32, C.c_void_p]
+        L.ga_gene_qc_walk.restype = C.c_int
         L.ga_route_bucket_bound.argtypes = [C.c_int32]
         L.ga_route_bucket_bound.restype = C.c_int32
         _lib = L

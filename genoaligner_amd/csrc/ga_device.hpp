@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include <functional>
+
 namespace ga {
 
 // Packed sequence buffers carry this many readable bytes before their first
@@ -204,5 +206,9 @@ hipError_t sw16_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHo
 // score-only: the wave64 lane-rule end), cig_len = CIGAR length (may be null).
 hipError_t sw_gather(const int32_t* res8, int64_t n, bool trace, int32_t* out5,
                      int32_t* cig_len, hipStream_t st);
+
+// Run chunk(k0, k1) over [0, n) on the host work pool, about `grain` indices a
+// chunk; returns when every chunk has run.
+void host_parallel_for(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& chunk);
 
 }  // namespace ga

@@ -1242,6 +1242,16 @@ const char* check_dev_batch(const ga_wfa_dev_batch* b)
 
 }  // namespace
 
+void ga::host_parallel_for(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& chunk)
+{
+    if (n <= 0) return;
+    if (n <= grain || WorkPool::get().workers() == 0) {
+        chunk(0, n);
+        return;
+    }
+    WorkPool::get().run(n, grain, chunk);
+}
+
 extern "C" {
 
 int64_t ga_cigar_arena_bytes_wfa(const ga_align_req* reqs, int64_t n)

@@ -286,3 +286,68 @@ def fasta_genome(total: int = 512 << 20, seed: int = 0, width: int = 60, rec_lo:
         size += len(hdr) + len(body)
         k += 1
     return np.concatenate(parts)
+
+
+_STOPS_ANY = {b"TAA", b"TAG", b"TGA", b"AGA", b"AGG"}
+
+
+def gene_family(n: int = 5000, codons: int = 380, seed: int = 0):
+    """A synthetic protein-coding gene family in FASTA form: (records file
+    bytes, bait file bytes).  The bait is ATG + random sense codons (no stop
+    under code 1 or 2) + TAA, like a mitochondrial CYTB (1,140 bp at 380
+    codons).  The bait file holds a stop-riddled fragment, the bait itself and
+    the bait behind one extra base (the same stop-free frame at offset 1, one
+    base longer).  Records: 55 % the bait at 3 % substitutions, 10 % plus a 1-
+    or 2-base indel (frameshift), 5 % plus an in-frame 3-base deletion, 10 %
+    with a sense codon turned into TAG, 8 % fragments of 30-70 %, 5 % inside
+    long random flanks, 3 % unrelated, 2 % soft-masked (lowercase), 2 % with N
+    runs."""
+    rng = np.random.default_rng(seed)
+    bases = np.frombuffer(b"ACGT", np.uint8)
+    sense = [bytes([a, b, c]) for a in b"ACGT" for b in b"ACGT" for c in b"ACGT"]
+    sense = [s for s in sense if s not in _STOPS_ANY]
+    bait = b"ATG" + b"".join(sense[k] for k in rng.integers(0, len(sense), codons - 2)) + b"TAA"
+
+    def rand(k):
+        return bases[rng.integers(0, 4, k)].tobytes()
+
+    def subst(s, rate):
+        a = np.frombuffer(s, np.uint8).copy()
+        hit = rng.random(len(a)) < rate
+        a[hit] = bases[rng.integers(0, 4, int(hit.sum()))]
+        return a.tobytes()
+
+    recs = []
+    for k in range(n):
+        u = rng.random()
+        s = subst(bait, 0.03)
+        if u < 0.55:
+            pass
+        elif u < 0.65:
+            p = int(rng.integers(30, len(s) - 30))
+            s = s[:p] + rand(int(rng.integers(1, 3))) + s[p:] if rng.random() < 0.5 else \
+                s[:p] + s[p + int(rng.integers(1, 3)):]
+        elif u < 0.70:
+            p = 3 * int(rng.integers(10, codons - 10))
+            s = s[:p] + s[p + 3:]
+        elif u < 0.80:
+            p = 3 * int(rng.integers(5, codons - 5))
+            s = s[:p] + b"TAG" + s[p + 3:]
+        elif u < 0.88:
+            ln = int(len(s) * rng.uniform(0.3, 0.7))
+            p = int(rng.integers(0, len(s) - ln))
+            s = s[p:p + ln]
+        elif u < 0.93:
+            s = rand(int(rng.integers(800, 1600))) + s + rand(int(rng.integers(800, 1600)))
+        elif u < 0.96:
+            s = rand(len(bait))
+        elif u < 0.98:
+            s = s.lower()
+        else:
+            p = int(rng.integers(0, len(s) - 40))
+            s = s[:p] + b"N" * int(rng.integers(5, 40)) + s[p + 40:]
+        recs.append(b">acc%05d synthetic gene record\n" % k + b"\n".join(s[i:i + 70] for i in range(0, len(s), 70)))
+    frag = b"TAAA" * 60 + rand(200)          # period 4: a TAA in every frame
+    ref = (b">frag stop-riddled fragment\n" + frag + b"\n>bait0 clean\n" + bait + b"\n>bait1 shifted\nC" +
+           bait + b"\n")
+    return b"\n".join(recs) + b"\n", ref

@@ -345,6 +345,27 @@ typedef struct ga_fasta_dev_out {
 int ga_fasta_device(const uint8_t* data, int64_t len, const ga_fasta_dev_out* out, int64_t* n_records,
                     int64_t* total, void* stream, const char** error);
 
+/* ---- gene QC walk (reference: tools/gene_qc.cpp:146-217) ------------------
+ * The per-record analysis of the reference's gene_qc tool, over the results
+ * and CIGAR arena of a ga_align_sw_batch call whose requests align record k
+ * (text) against the bait (pattern): text coverage of the span, indel runs
+ * with length % 3 != 0, the net indel length mod 3 (-1, 0, +1), in-frame
+ * stops of the bait's codons in `frame` (code 1 standard, 2 vertebrate
+ * mitochondrial, any other TAA/TAG only), the span's terminal stop reported
+ * apart.  Records with
+ * resolved == 0 or score <= 0 get an all-zero row.  text/text_off/text_len
+ * are the records' bytes as given to the batch.  Runs on the host. */
+typedef struct ga_qc_row {
+    double  cov_text;
+    int32_t fs_runs;
+    int32_t fs_net;
+    int32_t stops;
+    int32_t stop_end;
+} ga_qc_row;
+int ga_gene_qc_walk(const ga_sw_res* res, int64_t n, const uint8_t* cigar_arena, const uint8_t* text,
+                    const int64_t* text_off, const int32_t* text_len, int32_t frame, int32_t code,
+                    ga_qc_row* rows);
+
 #ifdef __cplusplus
 }
 #endif

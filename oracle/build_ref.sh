@@ -1,6 +1,7 @@
 #!/usr/bin/env bash
-# Build oracle/_ref/libga_ref.so: the reference's own pairwise kernels and API,
-# compiled for the CPU with the reference's own HIP CPU shim, from the sources
+# Build oracle/_ref/libga_ref.so (the reference's own pairwise kernels and API)
+# and oracle/_ref/gene_qc (its gene QC tool on that API), compiled for the CPU
+# with the reference's own HIP CPU shim, from the sources
 # where they lie under the reference checkout.  TEST INFRASTRUCTURE ONLY.
 #
 #   oracle/build_ref.sh [REFERENCE_ROOT]     (default /root/reference)
@@ -23,3 +24,11 @@ g++ -O2 -std=c++17 -pthread -fPIC -shared -ffp-contract=off \
     -o "$OUT/libga_ref.so.tmp"
 mv "$OUT/libga_ref.so.tmp" "$OUT/libga_ref.so"
 echo "build_ref: wrote $OUT/libga_ref.so"
+# the reference's gene QC tool (tools/gene_qc.cpp) over the same API build
+g++ -O2 -std=c++17 -pthread -ffp-contract=off \
+    -DGENOALIGNER_HIP_SHIM \
+    -I "$REF/tests/parity/hip_cpu_shim" -I "$REF" -I "$REF/include" \
+    "$REF/tools/gene_qc.cpp" "$HERE/ref/ref_driver.cpp" \
+    -o "$OUT/gene_qc.tmp"
+mv "$OUT/gene_qc.tmp" "$OUT/gene_qc"
+echo "build_ref: wrote $OUT/gene_qc"

@@ -582,3 +582,152 @@ def fasta_pack2(seq: bytes) -> np.ndarray:
     pad = (-len(a)) % 16
     code = np.concatenate([code, np.zeros(pad, np.uint32)]).reshape(-1, 16)
     return (code << (2 * np.arange(16, dtype=np.uint32))).sum(axis=1, dtype=np.uint64).astype(np.uint32)
+
+
+# ------------------------------------------------------------------ gene QC
+REF_GENE_QC = os.path.join(HERE, "_ref", "gene_qc")
+
+
+def _is_stop(a: int, b: int, c: int, code: int) -> bool:
+    """tools/gene_qc.cpp:53-61."""
+    t = bytes((a, b, c))
+    if t in (b"TAA", b"TAG"):
+        return True
+    if code == 1 and t == b"TGA":
+        return True
+    return code == 2 and t in (b"AGA", b"AGG")
+
+
+def gene_qc_walk(cigar: bytes, text: bytes, score: int, resolved: bool, start_i: int, start_j: int,
+                 end_i: int, frame: int, code: int):
+    """(cov_text, fs_runs, fs_net, stops, stop_end) of one record: a
+    restatement of tools/gene_qc.cpp:146-217."""
+    cigar, text = _b(cigar), _b(text)
+    if not resolved or score <= 0:
+        return 0.0, 0, 0, 0, 0
+    cov = (end_i - start_i + 1) / len(text)
+    i, j = start_i, start_j
+    codon, fill, i0, last_codon_j, stop_at, stops = [0, 0, 0], -1, -1, -1, -1, 0
+    for op in cigar:
+        if op in b"MX":
+            pos = (j - frame) % 3
+            if pos == 0:
+                fill, i0 = 0, i
+            if fill == pos:
+                codon[fill] = text[i]
+                fill += 1
+            else:
+                fill = -1
+            if fill == 3:
+                if i == i0 + 2:
+                    last_codon_j = j - 2
+                    if _is_stop(codon[0], codon[1], codon[2], code):
+                        stops += 1
+                        stop_at = j - 2
+                fill = -1
+            i += 1
+            j += 1
+        elif op == ord("I"):
+            i += 1
+        elif op == ord("D"):
+            fill = -1
+            j += 1
+    runs, ins, dele = 0, 0, 0
+    p = 0
+    while p < len(cigar):
+        o = cigar[p]
+        if o not in b"ID":
+            p += 1
+            continue
+        q = p
+        while q < len(cigar) and cigar[q] == o:
+            q += 1
+        runs += (q - p) % 3 != 0
+        if o == ord("I"):
+            ins += q - p
+        else:
+            dele += q - p
+        p = q
+    net = (ins - dele) % 3
+    if net > 1:
+        net -= 3
+    stop_end = 0
+    if stops > 0 and stop_at == last_codon_j:
+        stop_end, stops = 1, stops - 1
+    return cov, runs, net, stops, stop_end
+
+
+def gene_qc_bait(seqs, code: int):
+    """(record, frame, stops) of the bait choice, tools/gene_qc.cpp:111-131."""
+    best, best_stops, frame, best_atg = 0, 1 << 30, 0, False
+    for r, s in enumerate(seqs):
+        s = _b(s)
+        for f in range(3):
+            st, i = 0, f
+            while i + 2 < len(s):
+                st += _is_stop(s[i], s[i + 1], s[i + 2], code)
+                i += 3
+            atg = f + 2 < len(s) and s[f:f + 3] == b"ATG"
+            if (st < best_stops or (st == best_stops and atg and not best_atg) or
+                    (st == best_stops and atg == best_atg and len(s) > len(_b(seqs[best])))):
+                best_stops, best, frame, best_atg = st, r, f, atg
+    return best, frame, best_stops
+
+
+def ref_gene_qc(seqs_path: str, ref_path: str, args=(), emit: Optional[str] = None):
+    """Run the reference's gene_qc (oracle/_ref/gene_qc): (exit status, stdout,
+    stderr, emitted TSV or None)."""
+    cmd = [REF_GENE_QC, seqs_path, "--ref", ref_path, *args] + (["--emit", emit] if emit else [])
+    p = subprocess.run(cmd, capture_output=True, timeout=600)
+    tsv = open(emit, "rb").read().decode("latin-1") if emit and os.path.exists(emit) else None
+    return p.returncode, p.stdout.decode("latin-1"), p.stderr.decode("latin-1"), tsv
+
+
+def gene_qc_tsv(seqs: bytes, refs: bytes, code: int = 1, scoring=(1, -1, 2, 1), min_cov: float = 0.5,
+                min_score: int = 0):
+    """The reference gene_qc tool's TSV (tools/gene_qc.cpp:175-244) from the
+    oracle's FASTA reader, SW batch and walk: (tsv text, summary line)."""
+    recs, err = fasta_parse(seqs)
+    rrecs, err2 = fasta_parse(refs)
+    assert err is None and err2 is None, (err, err2)
+    best, frame, _ = gene_qc_bait([r[2] for r in rrecs], code)
+    bait = rrecs[best][2]
+
+    @dataclass
+    class _Req:
+        text: bytes
+        text_len: int
+        pattern: bytes
+        pattern_len: int
+        scoring: tuple
+        with_cigar: bool = True
+    br = align_sw_batch([_Req(r[2], len(r[2]), bait, len(bait), tuple(scoring)) for r in recs])
+    assert br.status == "ok", br.error
+    lines = ["acc\tlen\tscore\tsi\tsj\tei\tej\tcov_text\tfs_runs\tfs_net\tstops\tstop_end\tflag"]
+    cnt = dict(ok=0, na=0, lc=0, fs=0, st=0, bad=0)
+    for rec, r in zip(recs, br.results):
+        cov, runs, net, stops, stop_end = gene_qc_walk(r.cigar, rec[2], r.score, r.resolved, r.start_i,
+                                                       r.start_j, r.end_i, frame, code)
+        if not r.resolved or r.score <= min_score:
+            flag = "NO_ALIGN"
+        else:
+            flag = "LOW_COV" if cov < min_cov else ""
+            if net != 0:
+                flag += ("+" if flag else "") + "FS=%d" % net
+            if stops > 0:
+                flag += ("+" if flag else "") + "STOP=%d" % stops
+            flag = flag or "OK"
+        lines.append("%s\t%d\t%d\t%d\t%d\t%d\t%d\t%.3f\t%d\t%d\t%d\t%d\t%s" % (
+            rec[0].decode("latin-1"), len(rec[2]), r.score, r.start_i, r.start_j, r.end_i, r.end_j, cov, runs,
+            net, stops, stop_end, flag))
+        if flag == "OK":
+            cnt["ok"] += 1
+        else:
+            cnt["na"] += "NO_ALIGN" in flag
+            cnt["lc"] += "LOW_COV" in flag
+            cnt["fs"] += "FS=" in flag
+            cnt["st"] += "STOP=" in flag
+        cnt["bad"] += r.resolved and r.score > 0 and (not r.rescore_ok or not r.wellformed_ok)
+    summary = "summary: %d seqs | OK=%d NO_ALIGN=%d LOW_COV=%d FS=%d STOP=%d | invalid=%d" % (
+        len(recs), cnt["ok"], cnt["na"], cnt["lc"], cnt["fs"], cnt["st"], cnt["bad"])
+    return "\n".join(lines) + "\n", summary

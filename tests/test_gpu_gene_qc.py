@@ -1,0 +1,56 @@
+"""The gene QC driver on the GPU (SW + CIGAR through ga_align_sw_batch, the
+native walk): its TSV and summary against the reference tool itself
+(oracle/_ref/gene_qc, built from tools/gene_qc.cpp) where it travelled with the
+tree, else against the oracle's pipeline (pinned to the reference tool in
+tests/test_gene_qc.py).  Byte-exact."""
+import io
+import os
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _want(oracle, sp, rp, seqs, refs, code, args, tmp_path, kw):
+    if os.path.exists(oracle.REF_GENE_QC):
+        rc, out, err, tsv = oracle.ref_gene_qc(str(sp), str(rp), ("--code", str(code)) + args,
+                                               emit=str(tmp_path / "ref.tsv"))
+        assert rc == 0
+        return tsv, [ln for ln in err.splitlines() if ln.startswith("summary:")][0]
+    return oracle.gene_qc_tsv(seqs, refs, code=code, **kw)
+
+
+@pytest.mark.parametrize("n,codons,code,args,kw,batch", [
+    (300, 380, 2, (), {}, 0),                                   # CYTB-sized, vertebrate mitochondrial code
+    (200, 200, 1, ("--min-cov", "0.9"), {"min_cov": 0.9}, 0),
+    (120, 150, 2, ("--scoring", "2,-3,5,2"), {"scoring": (2, -3, 5, 2)}, 37),   # chunked
+])
+def test_driver_matches_reference_tool(gpu, oracle, tmp_path, n, codons, code, args, kw, batch):
+    from genoaligner_amd import gene_qc as G
+    from genoaligner_amd import workloads as W
+    seqs, refs = W.gene_family(n, codons=codons, seed=n + code)
+    sp, rp = tmp_path / "s.fa", tmp_path / "r.fa"
+    sp.write_bytes(seqs)
+    rp.write_bytes(refs)
+    want_tsv, want_summary = _want(oracle, sp, rp, seqs, refs, code, args, tmp_path, kw)
+    out, err = io.StringIO(), io.StringIO()
+    emit = str(tmp_path / "got.tsv")
+    rc = G.run(str(sp), str(rp), code=code, emit=emit, out=out, err=err, batch_size=batch,
+               **{k: v for k, v in kw.items()})
+    assert rc == 0
+    assert open(emit, encoding="latin-1").read() == want_tsv
+    assert want_summary in err.getvalue()
+    assert out.getvalue().startswith("gene_qc: %d records vs ref bait1 (%d bp, frame 1, bait_stops 1, code %d)"
+                                     % (n, 3 * codons + 1, code))
+
+
+def test_driver_cli_to_stdout(gpu, oracle, tmp_path, capsys):
+    from genoaligner_amd import gene_qc as G
+    from genoaligner_amd import workloads as W
+    seqs, refs = W.gene_family(40, codons=100, seed=2)
+    sp, rp = tmp_path / "s.fa", tmp_path / "r.fa"
+    sp.write_bytes(seqs)
+    rp.write_bytes(refs)
+    assert G.main(["gene_qc", str(sp), "--ref", str(rp), "--code", "2"]) == 0
+    got = capsys.readouterr().out.split("\n", 1)[1]
+    assert got == oracle.gene_qc_tsv(seqs, refs, code=2)[0]
