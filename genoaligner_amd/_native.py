@@ -32,7 +32,7 @@ EXPORTS = (
     "ga_wfa_graph_create", "ga_wfa_graph_launch", "ga_wfa_graph_destroy",
     "ga_wfa_autotune", "ga_wfa_set_tuning", "ga_wfa_get_tuning",
     "ga_fasta_parse", "ga_fasta_read_file", "ga_fasta_count", "ga_fasta_record", "ga_fasta_free",
-    "ga_fasta_device", "ga_gene_qc_walk",
+    "ga_fasta_sizes", "ga_fasta_export", "ga_fasta_device", "ga_gene_qc_walk",
 )
 
 
@@ -182,6 +182,10 @@ def lib() -> C.CDLL:
         L.ga_fasta_record.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p), i64p,
                                       C.POINTER(C.c_void_p), i64p, C.POINTER(C.c_void_p), i64p]
         L.ga_fasta_record.restype = C.c_int
+        L.ga_fasta_sizes.argtypes = [C.c_void_p, i64p, i64p, i64p]
+        L.ga_fasta_sizes.restype = None
+        L.ga_fasta_export.argtypes = [C.c_void_p] + [C.c_void_p] * 6
+        L.ga_fasta_export.restype = C.c_int
         L.ga_fasta_free.argtypes = [C.c_void_p]
         L.ga_fasta_free.restype = None
         L.ga_fasta_device.argtypes = [C.c_void_p, C.c_int64, C.POINTER(FastaDevOut), i64p, i64p,

@@ -27,6 +27,16 @@ namespace {
 
 thread_local std::string t_fasta_error;   // *error of the FASTA calls on this thread
 
+// First \n or \r at or after i (total if none): memchr for each, the \r
+// search bounded by the \n found.
+int64_t line_end(const char* data, int64_t i, int64_t total)
+{
+    const void* nl = std::memchr(data + i, '\n', size_t(total - i));
+    const int64_t e = nl ? static_cast<const char*>(nl) - data : total;
+    const void* cr = std::memchr(data + i, '\r', size_t(e - i));
+    return cr ? static_cast<const char*>(cr) - data : e;
+}
+
 bool parse_fasta(const char* data, int64_t total, ga_fasta& F, std::string& err)
 {
     int64_t i = 0;
@@ -36,17 +46,11 @@ bool parse_fasta(const char* data, int64_t total, ga_fasta& F, std::string& err)
         return false;
     };
     auto consume_eol = [&] {
-        while (i < total) {
-            const char c = data[i];
-            if (c == '\n') { ++i; ++line; return; }
-            if (c == '\r') {
-                ++i;
-                if (i < total && data[i] == '\n') ++i;
-                ++line;
-                return;
-            }
-            ++i;
-        }
+        i = line_end(data, i, total);
+        if (i >= total) return;
+        if (data[i] == '\r' && i + 1 < total && data[i + 1] == '\n') ++i;
+        ++i;
+        ++line;
     };
     bool in_record = false, have_seq = false;
     long record_line = 0;
@@ -57,7 +61,7 @@ bool parse_fasta(const char* data, int64_t total, ga_fasta& F, std::string& err)
             if (c != '>') return fail("expected '>' at start of a record, found '" + std::string(1, c) + "'", line);
             ++i;
             const int64_t h0 = i;
-            while (i < total && data[i] != '\n' && data[i] != '\r') ++i;
+            i = line_end(data, i, total);
             int64_t h1 = i;
             while (h1 > h0 && (data[h1 - 1] == ' ' || data[h1 - 1] == '\t')) --h1;
             int64_t sp = h0;
@@ -81,7 +85,7 @@ bool parse_fasta(const char* data, int64_t total, ga_fasta& F, std::string& err)
         }
         if (c == '\n' || c == '\r') { consume_eol(); continue; }
         const int64_t s0 = i;
-        while (i < total && data[i] != '\n' && data[i] != '\r') ++i;
+        i = line_end(data, i, total);
         if (i > s0) {
             F.seq.back().append(data + s0, size_t(i - s0));
             have_seq = true;
@@ -161,6 +165,43 @@ int ga_fasta_record(const ga_fasta* f, int64_t k, const char** id, int64_t* id_l
     if (desc_len) *desc_len = int64_t(f->desc[q].size());
     if (seq) *seq = f->seq[q].data();
     if (seq_len) *seq_len = int64_t(f->seq[q].size());
+    return GA_OK;
+}
+
+void ga_fasta_sizes(const ga_fasta* f, int64_t* id_bytes, int64_t* desc_bytes, int64_t* seq_bytes)
+{
+    int64_t a = 0, b = 0, c = 0;
+    if (f)
+        for (size_t k = 0; k < f->id.size(); ++k) {
+            a += int64_t(f->id[k].size());
+            b += int64_t(f->desc[k].size());
+            c += int64_t(f->seq[k].size());
+        }
+    if (id_bytes) *id_bytes = a;
+    if (desc_bytes) *desc_bytes = b;
+    if (seq_bytes) *seq_bytes = c;
+}
+
+int ga_fasta_export(const ga_fasta* f, char* ids, int64_t* id_off, char* descs, int64_t* desc_off, char* seqs,
+                    int64_t* seq_off)
+{
+    if (!f) return GA_INVALID_ARGUMENT;
+    int64_t a = 0, b = 0, c = 0;
+    const size_t n = f->id.size();
+    for (size_t k = 0; k < n; ++k) {
+        if (id_off) id_off[k] = a;
+        if (desc_off) desc_off[k] = b;
+        if (seq_off) seq_off[k] = c;
+        if (ids) std::memcpy(ids + a, f->id[k].data(), f->id[k].size());
+        if (descs) std::memcpy(descs + b, f->desc[k].data(), f->desc[k].size());
+        if (seqs) std::memcpy(seqs + c, f->seq[k].data(), f->seq[k].size());
+        a += int64_t(f->id[k].size());
+        b += int64_t(f->desc[k].size());
+        c += int64_t(f->seq[k].size());
+    }
+    if (id_off) id_off[n] = a;
+    if (desc_off) desc_off[n] = b;
+    if (seq_off) seq_off[n] = c;
     return GA_OK;
 }
 

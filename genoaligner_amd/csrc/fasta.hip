@@ -20,10 +20,11 @@
 //      tile's first line start (kept iff the line in force is a sequence
 //      line), bytes after it (decided inside the tile), and bytes after the
 //      tile's first header (for the tile holding the file's first header);
-//   S  one workgroup: the file's first header and error candidates, the
-//      line-state carry into each tile, headers and kept bytes before each
-//      tile, the record and byte totals, and whether the file is well formed
-//      (the kernels below do nothing otherwise);
+//   S  three small launches over groups of 1024 tiles: the file's first
+//      header and error candidates, the line-state carry into each tile,
+//      headers and kept bytes before each tile, the record and byte totals,
+//      and whether the file is well formed (the kernels below do nothing
+//      otherwise);
 //   E  per tile: scatter of the kept bytes, each record's start, and a clear
 //      of the record's A/C/G/T flag at any other kept byte;
 //   F  per record: length and the empty-record check;
@@ -60,7 +61,7 @@ struct FaInfo {
     int32_t ok;             // well formed so far, and n_records <= max_records
 };
 
-// Per-tile values of pass A (ntiles each); S rewrites three of them.
+// Per-tile values of pass A (ntiles each); the scan rewrites three of them.
 struct FaTiles {
     int64_t* state;         // last (line start << 1 | header) in the tile, -1 if none -> carry into the tile
     int64_t* nhdr;          // headers -> headers before the tile
@@ -303,59 +304,116 @@ __global__ __launch_bounds__(kFaThreads) void fa_tiles_a(const uint8_t* d, int64
     }
 }
 
-// S: one workgroup over the tile values, each thread a contiguous range of
-// tiles.  Leaves T.state = line-state carry into each tile, T.nhdr = headers
-// before it, T.kloc = kept bytes before it, and fills *info.
-__global__ __launch_bounds__(kScanThreads) void fa_scan(FaTiles T, int64_t n, int64_t len, int64_t max_records,
-                                                        FaInfo* info)
+// S, in three launches over groups of kScanThreads tiles (one thread a tile):
+//   S1  per group: first header, error candidates, last line state, headers;
+//   S2  per group: the carries into the group from the earlier groups'
+//       aggregates, then per tile the line-state carry (T.state), headers
+//       before it (T.nhdr) and its kept bytes under its entry state (into
+//       T.kloc); the group's kept sum; group 0 also fills *info but for the
+//       totals;
+//   S3  per group: kept bytes before each tile (T.kloc); group 0 fills the
+//       byte total and the well-formed flag.
+// A group reads at most all groups' aggregates (a few hundred for a GB file).
+struct FaGroups {
+    int64_t* first_gt;
+    int64_t* bad_pre;
+    int64_t* bad_id;
+    int64_t* state;
+    int64_t* nhdr;
+    int64_t* kept;
+};
+
+__global__ __launch_bounds__(kScanThreads) void fa_scan_up(FaTiles T, int64_t n, int64_t len, FaGroups G)
 {
-    __shared__ int64_t wt[6][16];
-    const int64_t per = (n + kScanThreads - 1) / kScanThreads;
-    const int64_t lo = int64_t(threadIdx.x) * per;
-    const int64_t hi = lo + per < n ? lo + per : n;
-    int64_t fh = len, bp = len, bi = len, st = -1, nh = 0;
-    for (int64_t k = lo; k < hi; ++k) {
-        fh = T.first_gt[k] < fh ? T.first_gt[k] : fh;
-        bp = T.bad_pre[k] < bp ? T.bad_pre[k] : bp;
-        bi = T.bad_id[k] < bi ? T.bad_id[k] : bi;
-        st = T.state[k] > st ? T.state[k] : st;
-        nh += T.nhdr[k];
+    __shared__ int64_t wt[5][16];
+    const int64_t k = int64_t(blockIdx.x) * kScanThreads + threadIdx.x;
+    const bool in = k < n;
+    const int64_t fg = block_reduce(in ? T.first_gt[k] : len, len, OpMin(), wt[0]);
+    const int64_t bp = block_reduce(in ? T.bad_pre[k] : len, len, OpMin(), wt[1]);
+    const int64_t bi = block_reduce(in ? T.bad_id[k] : len, len, OpMin(), wt[2]);
+    const int64_t st = block_reduce(in ? T.state[k] : int64_t(-1), int64_t(-1), OpMax(), wt[3]);
+    const int64_t nh = block_reduce(in ? T.nhdr[k] : int64_t(0), int64_t(0), OpSum(), wt[4]);
+    if (threadIdx.x == 0) {
+        G.first_gt[blockIdx.x] = fg;
+        G.bad_pre[blockIdx.x] = bp;
+        G.bad_id[blockIdx.x] = bi;
+        G.state[blockIdx.x] = st;
+        G.nhdr[blockIdx.x] = nh;
     }
-    const int64_t first_hdr = block_reduce(fh, len, OpMin(), wt[0]);
+}
+
+__global__ __launch_bounds__(kScanThreads) void fa_scan_mid(FaTiles T, int64_t n, int64_t len, FaGroups G,
+                                                            int64_t ngroups, FaInfo* info)
+{
+    __shared__ int64_t wt[8][16];
+    const int64_t gi = blockIdx.x;
+    // aggregates of all groups (first header, errors, records) and of the earlier ones (carries)
+    int64_t fg = len, bp = len, bi = len, nrec = 0, cst = -1, chd = 0;
+    for (int64_t q = threadIdx.x; q < ngroups; q += kScanThreads) {
+        fg = G.first_gt[q] < fg ? G.first_gt[q] : fg;
+        bp = G.bad_pre[q] < bp ? G.bad_pre[q] : bp;
+        bi = G.bad_id[q] < bi ? G.bad_id[q] : bi;
+        nrec += G.nhdr[q];
+        if (q < gi) {
+            cst = G.state[q] > cst ? G.state[q] : cst;
+            chd += G.nhdr[q];
+        }
+    }
+    const int64_t first_hdr = block_reduce(fg, len, OpMin(), wt[0]);
     const int64_t bad_pre = block_reduce(bp, len, OpMin(), wt[1]);
     const int64_t bad_id = block_reduce(bi, len, OpMin(), wt[2]);
-    int64_t n_rec;
-    int64_t cst = block_exscan(st, int64_t(-1), OpMax(), wt[3], nullptr);
-    int64_t chd = block_exscan(nh, int64_t(0), OpSum(), wt[4], &n_rec);
+    const int64_t n_rec = block_reduce(nrec, int64_t(0), OpSum(), wt[3]);
+    const int64_t g_st = block_reduce(cst, int64_t(-1), OpMax(), wt[4]);
+    const int64_t g_hd = block_reduce(chd, int64_t(0), OpSum(), wt[5]);
+    const int64_t k = gi * kScanThreads + threadIdx.x;
+    const bool in = k < n;
+    const int64_t s = in ? T.state[k] : int64_t(-1), h = in ? T.nhdr[k] : int64_t(0);
+    int64_t e_st = block_exscan(s, int64_t(-1), OpMax(), wt[6], nullptr);
+    e_st = e_st > g_st ? e_st : g_st;
+    const int64_t e_hd = g_hd + block_exscan(h, int64_t(0), OpSum(), wt[7], nullptr);
     const int64_t ft = first_hdr < len ? first_hdr / kFaTile : n;
-    // the carries, and each tile's kept bytes under its entry state
-    int64_t kept = 0;
-    for (int64_t k = lo; k < hi; ++k) {
-        const int64_t s = T.state[k], h = T.nhdr[k];
-        T.state[k] = cst;
-        T.nhdr[k] = chd;
-        const bool seq_entry = cst >= 0 && !(cst & 1);
-        const int64_t kk = k < ft ? 0 : k == ft ? T.kfh[k] : T.kloc[k] + (seq_entry ? T.kpre[k] : 0);
+    int64_t kk = 0;
+    if (in) {
+        const bool seq_entry = e_st >= 0 && !(e_st & 1);
+        kk = k < ft ? 0 : k == ft ? T.kfh[k] : T.kloc[k] + (seq_entry ? T.kpre[k] : 0);
+        T.state[k] = e_st;
+        T.nhdr[k] = e_hd;
         T.kloc[k] = kk;
-        kept += kk;
-        cst = s > cst ? s : cst;
-        chd += h;
     }
-    int64_t total;
-    int64_t ck = block_exscan(kept, int64_t(0), OpSum(), wt[5], &total);
-    for (int64_t k = lo; k < hi; ++k) {
-        const int64_t kk = T.kloc[k];
-        T.kloc[k] = ck;
-        ck += kk;
-    }
+    const int64_t ks = block_reduce(kk, int64_t(0), OpSum(), wt[0]);
     if (threadIdx.x == 0) {
-        info->first_hdr = first_hdr;
-        info->n_records = n_rec;
-        info->total = total;
-        info->bad_pre = bad_pre;
-        info->bad_id = bad_id;
-        info->bad_rec = 0;
-        info->ok = n_rec > 0 && bad_pre >= first_hdr && bad_id >= len && n_rec <= max_records ? 1 : 0;
+        G.kept[gi] = ks;
+        if (gi == 0) {
+            info->first_hdr = first_hdr;
+            info->n_records = n_rec;
+            info->bad_pre = bad_pre;
+            info->bad_id = bad_id;
+            info->bad_rec = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void fa_scan_down(FaTiles T, int64_t n, int64_t len, FaGroups G,
+                                                             int64_t ngroups, int64_t max_records, FaInfo* info)
+{
+    __shared__ int64_t wt[3][16];
+    const int64_t gi = blockIdx.x;
+    int64_t before = 0, all = 0;
+    for (int64_t q = threadIdx.x; q < ngroups; q += kScanThreads) {
+        all += G.kept[q];
+        before += q < gi ? G.kept[q] : 0;
+    }
+    before = block_reduce(before, int64_t(0), OpSum(), wt[0]);
+    all = block_reduce(all, int64_t(0), OpSum(), wt[1]);
+    const int64_t k = gi * kScanThreads + threadIdx.x;
+    const bool in = k < n;
+    const int64_t ex = before + block_exscan(in ? T.kloc[k] : int64_t(0), int64_t(0), OpSum(), wt[2], nullptr);
+    if (in) T.kloc[k] = ex;
+    if (gi == 0 && threadIdx.x == 0) {
+        const int64_t n_rec = info->n_records;
+        info->total = all;
+        info->ok = n_rec > 0 && info->bad_pre >= info->first_hdr && info->bad_id >= len && n_rec <= max_records
+                       ? 1 : 0;
     }
 }
 
@@ -510,14 +568,18 @@ extern "C" int ga_fasta_device(const uint8_t* data, int64_t len, const ga_fasta_
         return GA_DEVICE_ERROR;
     };
     const int64_t ntiles = len > 0 ? (len + kFaTile - 1) / kFaTile : 0;
-    // scratch: 8 per-tile arrays and the info block, one allocation
+    // scratch: 8 per-tile arrays, 6 per-group arrays and the info block, one allocation
     const int64_t nt = ntiles > 0 ? ntiles : 1;
+    const int64_t ngroups = (nt + kScanThreads - 1) / kScanThreads;
     void* scratch = nullptr;
-    if (hipMallocAsync(&scratch, size_t(8 * nt) * sizeof(int64_t) + sizeof(FaInfo), st) != hipSuccess)
+    if (hipMallocAsync(&scratch, size_t(8 * nt + 6 * ngroups) * sizeof(int64_t) + sizeof(FaInfo), st) !=
+        hipSuccess)
         return fail_dev("hipMalloc(FASTA scratch)");
     int64_t* a = static_cast<int64_t*>(scratch);
     const FaTiles T{a, a + nt, a + 2 * nt, a + 3 * nt, a + 4 * nt, a + 5 * nt, a + 6 * nt, a + 7 * nt};
-    FaInfo* info = reinterpret_cast<FaInfo*>(a + 8 * nt);
+    int64_t* gp = a + 8 * nt;
+    const FaGroups G{gp, gp + ngroups, gp + 2 * ngroups, gp + 3 * ngroups, gp + 4 * ngroups, gp + 5 * ngroups};
+    FaInfo* info = reinterpret_cast<FaInfo*>(gp + 6 * ngroups);
     FaInfo h{};
     hipError_t e = hipSuccess;
     const bool al = (reinterpret_cast<uintptr_t>(data) & 15) == 0;
@@ -526,7 +588,10 @@ extern "C" int ga_fasta_device(const uint8_t* data, int64_t len, const ga_fasta_
         const dim3 g{unsigned(ntiles), 1, 1}, b{unsigned(kFaThreads), 1, 1};
         if (al) hipLaunchKernelGGL(fa_tiles_a<true>, g, b, 0, st, data, len, T);
         else    hipLaunchKernelGGL(fa_tiles_a<false>, g, b, 0, st, data, len, T);
-        hipLaunchKernelGGL(fa_scan, dim3(1), dim3(kScanThreads), 0, st, T, ntiles, len, out->max_records, info);
+        const dim3 gg{unsigned(ngroups), 1, 1}, sb{unsigned(kScanThreads), 1, 1};
+        hipLaunchKernelGGL(fa_scan_up, gg, sb, 0, st, T, ntiles, len, G);
+        hipLaunchKernelGGL(fa_scan_mid, gg, sb, 0, st, T, ntiles, len, G, ngroups, info);
+        hipLaunchKernelGGL(fa_scan_down, gg, sb, 0, st, T, ntiles, len, G, ngroups, out->max_records, info);
         const int64_t cap = out->max_records < len ? out->max_records : len;   // records <= bytes
         if (cap > 0) {
             const dim3 rg{unsigned((cap + kFaThreads - 1) / kFaThreads), 1, 1};

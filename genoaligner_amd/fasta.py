@@ -34,16 +34,20 @@ class FastaRecord:
 
 
 def _records(h: C.c_void_p) -> List[FastaRecord]:
+    """Every record, through one bulk copy (ga_fasta_sizes / ga_fasta_export)."""
+    import numpy as np
     L = N.lib()
-    out = []
-    p = [C.c_void_p() for _ in range(3)]
-    n = [C.c_int64() for _ in range(3)]
-    for k in range(L.ga_fasta_count(h)):
-        L.ga_fasta_record(h, k, C.byref(p[0]), C.byref(n[0]), C.byref(p[1]), C.byref(n[1]),
-                          C.byref(p[2]), C.byref(n[2]))
-        out.append(FastaRecord(*(C.string_at(p[j].value, n[j].value) if n[j].value else b""
-                                 for j in range(3))))
-    return out
+    n = L.ga_fasta_count(h)
+    sz = [C.c_int64() for _ in range(3)]
+    L.ga_fasta_sizes(h, C.byref(sz[0]), C.byref(sz[1]), C.byref(sz[2]))
+    bufs = [np.empty(max(x.value, 1), np.uint8) for x in sz]
+    offs = [np.empty(n + 1, np.int64) for _ in range(3)]
+    L.ga_fasta_export(h, bufs[0].ctypes.data, offs[0].ctypes.data, bufs[1].ctypes.data, offs[1].ctypes.data,
+                      bufs[2].ctypes.data, offs[2].ctypes.data)
+    raw = [b.tobytes() for b in bufs]
+    o = [x.tolist() for x in offs]
+    return [FastaRecord(raw[0][o[0][k]:o[0][k + 1]], raw[1][o[1][k]:o[1][k + 1]], raw[2][o[2][k]:o[2][k + 1]])
+            for k in range(n)]
 
 
 def _take(rc: int, h: C.c_void_p, err: C.c_char_p) -> List[FastaRecord]:

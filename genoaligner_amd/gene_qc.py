@@ -148,14 +148,16 @@ def run(seqs_path: str, ref_path: str, code: int = 1, scoring=(1, -1, 2, 1), min
         return 1
     lines = ["acc\tlen\tscore\tsi\tsj\tei\tej\tcov_text\tfs_runs\tfs_net\tstops\tstop_end\tflag"]
     n_ok = n_noalign = n_lowcov = n_fs = n_stop = n_bad = 0
+    R = {f: res[f].tolist() for f in ("score", "resolved", "start_i", "start_j", "end_i", "end_j",
+                                       "rescore_ok", "wellformed_ok")}
+    Q = {f: rows[f].tolist() for f in ("cov_text", "fs_runs", "fs_net", "stops", "stop_end")}
     for k, rec in enumerate(recs):
-        r, q = res[k], rows[k]
-        score, resolved = int(r["score"]), bool(r["resolved"])
-        flag = flag_of(score, resolved, float(q["cov_text"]), int(q["fs_net"]), int(q["stops"]), min_cov,
-                       min_score)
+        score, resolved = R["score"][k], bool(R["resolved"][k])
+        flag = flag_of(score, resolved, Q["cov_text"][k], Q["fs_net"][k], Q["stops"][k], min_cov, min_score)
         lines.append("%s\t%d\t%d\t%d\t%d\t%d\t%d\t%.3f\t%d\t%d\t%d\t%d\t%s" % (
-            rec.id.decode("latin-1"), len(rec.sequence), score, r["start_i"], r["start_j"], r["end_i"],
-            r["end_j"], q["cov_text"], q["fs_runs"], q["fs_net"], q["stops"], q["stop_end"], flag))
+            rec.id.decode("latin-1"), len(rec.sequence), score, R["start_i"][k], R["start_j"][k], R["end_i"][k],
+            R["end_j"][k], Q["cov_text"][k], Q["fs_runs"][k], Q["fs_net"][k], Q["stops"][k], Q["stop_end"][k],
+            flag))
         if flag == "OK":
             n_ok += 1
         else:
@@ -163,7 +165,7 @@ def run(seqs_path: str, ref_path: str, code: int = 1, scoring=(1, -1, 2, 1), min
             n_lowcov += "LOW_COV" in flag
             n_fs += "FS=" in flag
             n_stop += "STOP=" in flag
-        if resolved and score > 0 and (not r["rescore_ok"] or not r["wellformed_ok"]):
+        if resolved and score > 0 and (not R["rescore_ok"][k] or not R["wellformed_ok"][k]):
             n_bad += 1
     dst.write("\n".join(lines) + "\n")
     if emit:

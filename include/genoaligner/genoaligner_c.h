@@ -317,6 +317,13 @@ int64_t ga_fasta_count(const ga_fasta* f);
 int     ga_fasta_record(const ga_fasta* f, int64_t k, const char** id, int64_t* id_len, const char** desc,
                         int64_t* desc_len, const char** seq, int64_t* seq_len);
 void    ga_fasta_free(ga_fasta* f);
+/* All records at once (EXTENSION: bulk access for bindings): ga_fasta_sizes
+ * gives the total id, description and sequence bytes; ga_fasta_export writes
+ * them back to back, record k at id_off[k] .. id_off[k+1] (and likewise), the
+ * offset arrays holding count+1 entries.  Any pointer may be NULL. */
+void    ga_fasta_sizes(const ga_fasta* f, int64_t* id_bytes, int64_t* desc_bytes, int64_t* seq_bytes);
+int     ga_fasta_export(const ga_fasta* f, char* ids, int64_t* id_off, char* descs, int64_t* desc_off,
+                        char* seqs, int64_t* seq_off);
 
 /* FASTA ingest on the device (EXTENSION: no reference counterpart, whose reader
  * runs on the host only).  `data` holds a whole FASTA file
