@@ -514,13 +514,20 @@ def main():
                  and int(d.seq_len.sum().item()) == d.total)
         med = float(np.median(fa_ms))
         moved = len(fa) + d.total + (d.total + 15) // 16 * 4 + 13 * d.n   # file in, bytes + codes + per-record out
+        fa_traffic = None     # HBM bytes per call from the FETCH_SIZE / WRITE_SIZE passes, per input byte
+        fpath = os.path.join(ROOT, "profiles", "fasta_ingest_genome_256mib_hbm_traffic.md")
+        if os.path.exists(fpath):
+            import re
+            mt = re.search(r"traffic_bytes_per_input_byte:\s*([0-9.eE+]+)", open(fpath).read())
+            fa_traffic = round(float(mt.group(1)) * len(fa)) if mt else None
         fasta_line = {
             "workload": f"fasta_genome {len(fa) / 2**20:.0f} MiB, {d.n} records of 10 kbp..2 Mbp, 60-byte "
                         "lines, N runs, soft-masked runs, CRLF records; file resident in HBM",
             "value": round(len(fa) / (med * 1e-3) / 1e9, 2), "unit": "GB/s of FASTA input",
             "ms_per_call": round(med, 3), "records": d.n, "seq_bytes": d.total, "checked": bool(fa_ok),
             "roofline": {"bound": "hbm", "achieved": round(moved / (med * 1e-3) / 1e9, 1), "peak": 8000.0,
-                         "unit": "GB/s", "frac": round(moved / (med * 1e-3) / 8e12, 4), "traffic": None},
+                         "unit": "GB/s", "frac": round(moved / (med * 1e-3) / 8e12, 4), "traffic": fa_traffic,
+                         "algorithmic_bytes": moved},
         }
         del fa_dev, d
         if not args.no_cpu_baseline:
