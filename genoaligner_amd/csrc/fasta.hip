@@ -25,10 +25,11 @@
 //      headers and kept bytes before each tile, the record and byte totals,
 //      and whether the file is well formed (the kernels below do nothing
 //      otherwise);
-//   E  per tile: scatter of the kept bytes, each record's start, and a clear
-//      of the record's A/C/G/T flag at any other kept byte;
+//   E  per tile: scatter of the kept bytes, each record's start, a clear of
+//      the record's A/C/G/T flag at any other kept byte, and (optional) the
+//      2-bit codes of the 16-byte output words inside the tile's output;
 //   F  per record: length and the empty-record check;
-//   G  (optional) the 2-bit codes of the whole sequence stream.
+//   G  (optional) the 2-bit words cut by tile seams.
 // Each thread reads its 32 bytes with two 128-bit loads (when the file is
 // 16-byte aligned) plus the byte before and the byte after them.  The host
 // waits once, at the end; a malformed file is only flagged on the device and
@@ -109,6 +110,18 @@ __device__ __forceinline__ uint32_t fa_eq4(uint32_t x, uint32_t c4)
     const uint32_t z = ~(((t & 0x7f7f7f7fu) + 0x7f7f7f7fu) | t) & 0x80808080u;
     return ((z >> 7) * 0x00204081u) >> 21 & 0xfu;     // bits 0, 8, 16, 24 -> 0..3
 }
+// 2-bit codes of the 4 bytes of w as 8 bits, byte k at bits 2k: A 0, C 1,
+// G 2, T 3 (((c >> 1) ^ (c >> 2)) & 3 of the ASCII byte), any other byte 0.
+__device__ __forceinline__ uint32_t fa_code8(uint32_t w)
+{
+    const uint32_t m4 = fa_eq4(w, 0x41414141u) | fa_eq4(w, 0x43434343u) | fa_eq4(w, 0x47474747u) |
+                        fa_eq4(w, 0x54545454u);
+    const uint32_t keep = ((m4 * 0x00204081u) & 0x01010101u) * 0xffu;    // bit k -> byte k
+    uint32_t c = ((w >> 1) ^ (w >> 2)) & 0x03030303u & keep;
+    c = (c | (c >> 6)) & 0x000f000fu;
+    return (c | (c >> 12)) & 0xffu;
+}
+
 __device__ __forceinline__ uint32_t fa_mask(const uint32_t (&w)[kFaBytes / 4], uint8_t c)
 {
     const uint32_t c4 = uint32_t(c) * 0x01010101u;
@@ -431,7 +444,7 @@ __global__ __launch_bounds__(kFaThreads) void fa_flags_init(uint8_t* acgt, int64
 template <bool AL>
 __global__ __launch_bounds__(kFaThreads) void fa_tiles_e(const uint8_t* d, int64_t len, FaTiles T,
                                                         const FaInfo* info, uint8_t* seq, int64_t* seq_off,
-                                                        uint8_t* acgt, int64_t max_records)
+                                                        uint8_t* acgt, uint32_t* pack, int64_t max_records)
 {
     __shared__ int64_t wt[2][16];
     __shared__ __align__(16) uint8_t stage[kFaTile + 16];
@@ -492,9 +505,12 @@ __global__ __launch_bounds__(kFaThreads) void fa_tiles_e(const uint8_t* d, int64
     }
     const int head = int(a0 - ob), words = int((a1 - a0) >> 4), tail = int(ob + nk - a1);
     for (int j = threadIdx.x; j < head; j += kFaThreads) seq[ob + j] = stage[sh + j];
-    for (int j = threadIdx.x; j < words; j += kFaThreads)
-        *reinterpret_cast<uint4*>(seq + a0 + 16 * int64_t(j)) =
-            *reinterpret_cast<const uint4*>(stage + sh + head + 16 * j);
+    for (int j = threadIdx.x; j < words; j += kFaThreads) {
+        const uint4 v = *reinterpret_cast<const uint4*>(stage + sh + head + 16 * j);
+        *reinterpret_cast<uint4*>(seq + a0 + 16 * int64_t(j)) = v;
+        if (pack)   // the 2-bit word of these 16 output bytes (words cut by a tile seam: fa_pack2_edges)
+            pack[(a0 >> 4) + j] = fa_code8(v.x) | fa_code8(v.y) << 8 | fa_code8(v.z) << 16 | fa_code8(v.w) << 24;
+    }
     for (int j = threadIdx.x; j < tail; j += kFaThreads) seq[a1 + j] = stage[sh + head + 16 * words + j];
 }
 
@@ -517,34 +533,31 @@ __global__ __launch_bounds__(kFaThreads) void fa_records(const int64_t* seq_off,
     }
 }
 
-// G: 2-bit codes of the sequence stream, 16 bases to a 32-bit word, base p at
-// bits 2(p mod 16) of word p/16: A 0, C 1, G 2, T 3 (((c >> 1) ^ (c >> 2)) & 3
-// of the ASCII byte), any other byte 0 -- its record's acgt flag is 0.
-template <bool AL>
-__global__ __launch_bounds__(kFaThreads) void fa_pack2(const uint8_t* seq, const FaInfo* info, uint32_t* pack)
+// G: the 2-bit words (16 bases to a 32-bit word, base p at bits 2(p mod 16)
+// of word p/16) that pass E could not write whole: the word holding a tile's
+// first output byte and the one holding its last, when a tile seam falls
+// inside them.  One thread a tile, from the finished sequence stream; tiles
+// sharing a word write the same value.
+__global__ __launch_bounds__(kFaThreads) void fa_pack2_edges(const uint8_t* seq, FaTiles T, int64_t n,
+                                                            const FaInfo* info, uint32_t* pack)
 {
     if (!info->ok) return;
+    const int64_t t = int64_t(blockIdx.x) * kFaThreads + threadIdx.x;
+    if (t >= n) return;
     const int64_t total = info->total;
-    const int64_t w = int64_t(blockIdx.x) * kFaThreads + threadIdx.x;
-    const int64_t p0 = w * 16;
-    if (p0 >= total) return;
-    uint8_t b[16];
-    if (AL && p0 + 16 <= total) {
-        const uint4 v = *reinterpret_cast<const uint4*>(seq + p0);
-        const uint32_t x[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int q = 0; q < 16; ++q) b[q] = uint8_t(x[q >> 2] >> (8 * (q & 3)));
-    } else {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) b[q] = p0 + q < total ? seq[p0 + q] : uint8_t(0);
+    const int64_t ob = T.kloc[t], oe = t + 1 < n ? T.kloc[t + 1] : total;
+    if (oe <= ob) return;
+    const int64_t ends[2] = {(ob & 15) ? ob >> 4 : -1, (oe & 15) ? (oe - 1) >> 4 : -1};
+    for (int e = 0; e < 2; ++e) {
+        const int64_t w = ends[e];
+        if (w < 0) continue;
+        uint32_t word = 0;
+        for (int q = 0; q < 16 && 16 * w + q < total; ++q) {
+            const uint32_t c = seq[16 * w + q];
+            word |= (fa_base(uint8_t(c)) ? ((c >> 1) ^ (c >> 2)) & 3u : 0u) << (2 * q);
+        }
+        pack[w] = word;
     }
-    uint32_t word = 0;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const uint32_t c = b[q];
-        word |= (fa_base(b[q]) ? ((c >> 1) ^ (c >> 2)) & 3u : 0u) << (2 * q);
-    }
-    pack[w] = word;
 }
 
 }  // namespace
@@ -583,7 +596,6 @@ extern "C" int ga_fasta_device(const uint8_t* data, int64_t len, const ga_fasta_
     FaInfo h{};
     hipError_t e = hipSuccess;
     const bool al = (reinterpret_cast<uintptr_t>(data) & 15) == 0;
-    const bool sal = (reinterpret_cast<uintptr_t>(out->seq) & 15) == 0;
     if (ntiles > 0) {
         const dim3 g{unsigned(ntiles), 1, 1}, b{unsigned(kFaThreads), 1, 1};
         if (al) hipLaunchKernelGGL(fa_tiles_a<true>, g, b, 0, st, data, len, T);
@@ -598,16 +610,14 @@ extern "C" int ga_fasta_device(const uint8_t* data, int64_t len, const ga_fasta_
             hipLaunchKernelGGL(fa_flags_init, rg, b, 0, st, out->acgt, cap, info);
             if (al)
                 hipLaunchKernelGGL(fa_tiles_e<true>, g, b, 0, st, data, len, T, info, out->seq, out->seq_off,
-                                   out->acgt, out->max_records);
+                                   out->acgt, out->pack2, out->max_records);
             else
                 hipLaunchKernelGGL(fa_tiles_e<false>, g, b, 0, st, data, len, T, info, out->seq, out->seq_off,
-                                   out->acgt, out->max_records);
+                                   out->acgt, out->pack2, out->max_records);
             hipLaunchKernelGGL(fa_records, rg, b, 0, st, out->seq_off, cap, out->seq_len, out->acgt, info);
             if (out->pack2) {
-                const int64_t words = (len + 15) / 16;                           // total <= len
-                const dim3 pg{unsigned((words + kFaThreads - 1) / kFaThreads), 1, 1};
-                if (sal) hipLaunchKernelGGL(fa_pack2<true>, pg, b, 0, st, out->seq, info, out->pack2);
-                else     hipLaunchKernelGGL(fa_pack2<false>, pg, b, 0, st, out->seq, info, out->pack2);
+                const dim3 pg{unsigned((ntiles + kFaThreads - 1) / kFaThreads), 1, 1};
+                hipLaunchKernelGGL(fa_pack2_edges, pg, b, 0, st, out->seq, T, ntiles, info, out->pack2);
             }
         }
         e = hipGetLastError();
