@@ -186,24 +186,24 @@ __device__ __forceinline__ FaMasks fa_masks(const uint8_t* d, int64_t len, int64
     return m;
 }
 
-struct OpSum { __device__ int64_t operator()(int64_t a, int64_t b) const { return a + b; } };
-struct OpMax { __device__ int64_t operator()(int64_t a, int64_t b) const { return a > b ? a : b; } };
-struct OpMin { __device__ int64_t operator()(int64_t a, int64_t b) const { return a < b ? a : b; } };
+struct OpSum { template <class V> __device__ V operator()(V a, V b) const { return a + b; } };
+struct OpMax { template <class V> __device__ V operator()(V a, V b) const { return a > b ? a : b; } };
+struct OpMin { template <class V> __device__ V operator()(V a, V b) const { return a < b ? a : b; } };
 
-template <class Op>
-__device__ __forceinline__ int64_t wave_reduce(int64_t v, Op op)
+template <class V, class Op>
+__device__ __forceinline__ V wave_reduce(V v, Op op)
 {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
+    for (int off = 32; off > 0; off >>= 1) v = op(v, V(__shfl_xor(v, off, 64)));
     return v;
 }
-template <class Op>
-__device__ __forceinline__ int64_t wave_inscan(int64_t v, Op op)
+template <class V, class Op>
+__device__ __forceinline__ V wave_inscan(V v, Op op)
 {
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        const int64_t x = __shfl_up(v, off, 64);
+        const V x = __shfl_up(v, off, 64);
         if (lane >= off) v = op(v, x);
     }
     return v;
@@ -211,44 +211,49 @@ __device__ __forceinline__ int64_t wave_inscan(int64_t v, Op op)
 
 // Exclusive scan over the workgroup (at most 16 waves); *all gets the
 // reduction.  wtmp: 16 entries of LDS owned by this call site.
-template <class Op>
-__device__ int64_t block_exscan(int64_t v, int64_t id, Op op, int64_t* wtmp, int64_t* all)
+template <class V> struct FaSame { using type = V; };   // keeps `all` out of deduction (nullptr)
+template <class V, class Op>
+__device__ V block_exscan(V v, V id, Op op, V* wtmp, typename FaSame<V>::type* all)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int64_t inc = wave_inscan(v, op);
+    const V inc = wave_inscan(v, op);
     if (lane == 63) wtmp[w] = inc;
     __syncthreads();
-    int64_t pre = id, tot = id;
+    V pre = id, tot = id;
     for (int k = 0; k < nw; ++k) {
         if (k < w) pre = op(pre, wtmp[k]);
         tot = op(tot, wtmp[k]);
     }
-    int64_t ex = __shfl_up(inc, 1, 64);
+    V ex = __shfl_up(inc, 1, 64);
     if (lane == 0) ex = id;
     if (all) *all = tot;
     __syncthreads();
     return op(pre, ex);
 }
-template <class Op>
-__device__ int64_t block_reduce(int64_t v, int64_t id, Op op, int64_t* wtmp)
+template <class V, class Op>
+__device__ V block_reduce(V v, V id, Op op, V* wtmp)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     v = wave_reduce(v, op);
     if (lane == 0) wtmp[w] = v;
     __syncthreads();
-    int64_t tot = id;
+    V tot = id;
     for (int k = 0; k < nw; ++k) tot = op(tot, wtmp[k]);
     __syncthreads();
     return tot;
 }
 
-// A: per-tile values.
+// A: per-tile values.  Positions inside the thread block are tile-relative
+// (< 16 KiB), so every reduction runs on 32-bit values, and the four counts
+// travel as two 16-bit pairs (each at most 16,384).
 template <bool AL>
 __global__ __launch_bounds__(kFaThreads) void fa_tiles_a(const uint8_t* d, int64_t len, FaTiles T)
 {
-    __shared__ int64_t wt[2][16];
-    __shared__ int64_t part[6][kFaWaves];
-    const int64_t base = int64_t(blockIdx.x) * kFaTile + int64_t(threadIdx.x) * kFaBytes;
+    __shared__ int32_t wt[2][16];
+    __shared__ int32_t part[4][kFaWaves];
+    const int64_t tile0 = int64_t(blockIdx.x) * kFaTile;
+    const int rel = int(threadIdx.x) * kFaBytes;      // this thread's first byte, from the tile start
+    const int64_t base = tile0 + rel;
     FaBytes r;
     const FaMasks m = fa_masks<AL>(d, len, base, r);
     const int fls = m.ls ? __builtin_ctz(m.ls) : 32;
@@ -257,60 +262,57 @@ __global__ __launch_bounds__(kFaThreads) void fa_tiles_a(const uint8_t* d, int64
     const int kpre = __builtin_popcount(seqb & fa_below(fls));
     const int kloc = __builtin_popcount(seqb & fa_from(fls) & ~m.hl);
     const int kfh = __builtin_popcount(seqb & fa_from(fhs) & ~m.hl);
-    int64_t last = -1;
+    int32_t last = -1;                                // (tile-relative line start << 1) | header
     if (m.ls) {
         const int q = 31 - __builtin_clz(m.ls);
-        last = 2 * (base + q) + ((m.hs >> q) & 1);
+        last = 2 * (rel + q) + int((m.hs >> q) & 1);
     }
-    const int64_t first_gt = m.hs ? base + fhs : len;
+    const int32_t first_gt = m.hs ? rel + fhs : int32_t(kFaTile);
     const uint32_t bp = m.ls & m.other;               // a line that must not come before the first header
-    const int64_t pre = bp ? base + __builtin_ctz(bp) : len;
+    const int32_t pre = bp ? rel + __builtin_ctz(bp) : int32_t(kFaTile);
     // a header whose next byte is a blank, a line break or the end of the file has an empty id
     const bool next_blank = base + kFaBytes >= len || r.next == ' ' || r.next == '\t' || r.next == '\n' ||
                             r.next == '\r';
     const uint32_t blank_next = ((m.blank | ~m.valid) >> 1) | (next_blank ? 0x80000000u : 0u);
     const uint32_t bi_m = m.hs & blank_next;
-    const int64_t bid = bi_m ? base + __builtin_ctz(bi_m) : len;
+    const int32_t bid = bi_m ? rel + __builtin_ctz(bi_m) : int32_t(kFaTile);
     // the line in force at this thread's first byte, from the tile's earlier threads (-1: none)
-    int64_t tile_last;
-    const int64_t e = block_exscan(last, int64_t(-1), OpMax(), wt[0], &tile_last);
-    const int64_t g = block_reduce(first_gt, len, OpMin(), wt[1]);
+    int32_t tile_last;
+    const int32_t e = block_exscan(last, int32_t(-1), OpMax(), wt[0], &tile_last);
+    const int32_t g = block_reduce(first_gt, int32_t(kFaTile), OpMin(), wt[1]);
     const bool e_seq = e >= 0 && !(e & 1);
-    const int64_t c_pre = e < 0 ? kpre : 0;
-    const int64_t c_loc = kloc + (e_seq ? kpre : 0);
-    const int64_t c_fh = (g < len && first_gt == g) ? int64_t(kfh) : (g < base ? c_loc : 0);
+    const int32_t c_pre = e < 0 ? kpre : 0;
+    const int32_t c_loc = kloc + (e_seq ? kpre : 0);
+    const int32_t c_fh = (g < kFaTile && first_gt == g) ? kfh : (g < rel ? c_loc : 0);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t r_nh = wave_reduce(int64_t(__builtin_popcount(m.hs)), OpSum());
-    const int64_t r_pre = wave_reduce(pre, OpMin());
-    const int64_t r_bid = wave_reduce(bid, OpMin());
-    const int64_t r_cpre = wave_reduce(c_pre, OpSum());
-    const int64_t r_cloc = wave_reduce(c_loc, OpSum());
-    const int64_t r_cfh = wave_reduce(c_fh, OpSum());
+    const int32_t r_a = wave_reduce(int32_t(__builtin_popcount(m.hs)) | c_fh << 16, OpSum());
+    const int32_t r_b = wave_reduce(c_pre | c_loc << 16, OpSum());
+    const int32_t r_pre = wave_reduce(pre, OpMin());
+    const int32_t r_bid = wave_reduce(bid, OpMin());
     if (lane == 0) {
-        part[0][w] = r_nh;
-        part[1][w] = r_pre;
-        part[2][w] = r_bid;
-        part[3][w] = r_cpre;
-        part[4][w] = r_cloc;
-        part[5][w] = r_cfh;
+        part[0][w] = r_a;
+        part[1][w] = r_b;
+        part[2][w] = r_pre;
+        part[3][w] = r_bid;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        int64_t s_nh = 0, s_pre = len, s_bid = len, s_cpre = 0, s_cloc = 0, s_cfh = 0;
+        int64_t s_nh = 0, s_cfh = 0, s_cpre = 0, s_cloc = 0;
+        int32_t s_pre = kFaTile, s_bid = kFaTile;
         for (int k = 0; k < kFaWaves; ++k) {
-            s_nh += part[0][k];
-            s_pre = part[1][k] < s_pre ? part[1][k] : s_pre;
-            s_bid = part[2][k] < s_bid ? part[2][k] : s_bid;
-            s_cpre += part[3][k];
-            s_cloc += part[4][k];
-            s_cfh += part[5][k];
+            s_nh += part[0][k] & 0xffff;
+            s_cfh += part[0][k] >> 16;
+            s_cpre += part[1][k] & 0xffff;
+            s_cloc += part[1][k] >> 16;
+            s_pre = part[2][k] < s_pre ? part[2][k] : s_pre;
+            s_bid = part[3][k] < s_bid ? part[3][k] : s_bid;
         }
         const int64_t t = blockIdx.x;
-        T.state[t] = tile_last;
+        T.state[t] = tile_last < 0 ? int64_t(-1) : 2 * tile0 + tile_last;
         T.nhdr[t] = s_nh;
-        T.first_gt[t] = g;
-        T.bad_pre[t] = s_pre;
-        T.bad_id[t] = s_bid;
+        T.first_gt[t] = g < kFaTile ? tile0 + g : len;
+        T.bad_pre[t] = s_pre < kFaTile ? tile0 + s_pre : len;
+        T.bad_id[t] = s_bid < kFaTile ? tile0 + s_bid : len;
         T.kpre[t] = s_cpre;
         T.kloc[t] = s_cloc;
         T.kfh[t] = s_cfh;
@@ -446,21 +448,24 @@ __global__ __launch_bounds__(kFaThreads) void fa_tiles_e(const uint8_t* d, int64
                                                         const FaInfo* info, uint8_t* seq, int64_t* seq_off,
                                                         uint8_t* acgt, uint32_t* pack, int64_t max_records)
 {
-    __shared__ int64_t wt[2][16];
+    __shared__ int32_t wt[2][16];
     __shared__ __align__(16) uint8_t stage[kFaTile + 16];
     if (!info->ok) return;                            // the same for the whole workgroup
     const int64_t first_hdr = info->first_hdr;
-    const int64_t base = int64_t(blockIdx.x) * kFaTile + int64_t(threadIdx.x) * kFaBytes;
+    const int64_t tile0 = int64_t(blockIdx.x) * kFaTile;
+    const int rel = int(threadIdx.x) * kFaBytes;
+    const int64_t base = tile0 + rel;
     FaBytes r;
     const FaMasks m = fa_masks<AL>(d, len, base, r);
-    int64_t last = -1;
+    int32_t last = -1;                                // (tile-relative line start << 1) | header
     if (m.ls) {
         const int q = 31 - __builtin_clz(m.ls);
-        last = 2 * (base + q) + ((m.hs >> q) & 1);
+        last = 2 * (rel + q) + int((m.hs >> q) & 1);
     }
-    const int64_t ex = block_exscan(last, int64_t(-1), OpMax(), wt[0], nullptr);
+    const int32_t ex = block_exscan(last, int32_t(-1), OpMax(), wt[0], nullptr);
     const int64_t carry = T.state[blockIdx.x];
-    const int64_t entry = ex > carry ? ex : carry;    // the line in force at this thread's first byte
+    // the line in force at this thread's first byte
+    const int64_t entry = ex >= 0 ? 2 * tile0 + ex : carry;
     const int fls = m.ls ? __builtin_ctz(m.ls) : 32;
     // bytes before the first line start here: kept iff the entry line is a sequence line after the first header
     const bool entry_keep = entry >= 0 && !(entry & 1) && (entry >> 1) >= first_hdr;
@@ -469,14 +474,14 @@ __global__ __launch_bounds__(kFaThreads) void fa_tiles_e(const uint8_t* d, int64
     const uint32_t after_first = cut <= 0 ? ~0u : cut >= 32 ? 0u : fa_from(int(cut));
     const uint32_t keep = m.valid & ~m.eol & ~m.hl &
                           ((entry_keep ? fa_below(fls) : 0u) | (fa_from(fls) & after_first));
-    // kept bytes and headers before this thread in one scan: (kept << 32) | headers
-    const int64_t packed = (int64_t(__builtin_popcount(keep)) << 32) | int64_t(__builtin_popcount(m.hs));
-    int64_t tile_sum;
-    const int64_t pex = block_exscan(packed, int64_t(0), OpSum(), wt[1], &tile_sum);
+    // kept bytes and headers before this thread in one scan: headers | kept << 16 (each <= 16,384)
+    const int32_t packed = int32_t(__builtin_popcount(m.hs)) | int32_t(__builtin_popcount(keep)) << 16;
+    int32_t tile_sum;
+    const int32_t pex = block_exscan(packed, int32_t(0), OpSum(), wt[1], &tile_sum);
     const int64_t ob = T.kloc[blockIdx.x];           // output offset of the tile's first kept byte
     const int sh = int(ob & 15);
-    int lp = int(pex >> 32);                          // offset in the tile of this thread's first kept byte
-    int64_t rec = T.nhdr[blockIdx.x] + (pex & 0xffffffff);
+    int lp = pex >> 16;                               // offset in the tile of this thread's first kept byte
+    int64_t rec = T.nhdr[blockIdx.x] + (pex & 0xffff);
     // record starts, and the A/C/G/T flag per record segment of these bytes
     const uint32_t a_ = fa_mask_bytes(r, 'A') | fa_mask_bytes(r, 'C') | fa_mask_bytes(r, 'G') |
                         fa_mask_bytes(r, 'T');
@@ -495,7 +500,7 @@ __global__ __launch_bounds__(kFaThreads) void fa_tiles_e(const uint8_t* d, int64
     for (int q = 0; q < kFaBytes; ++q)
         if (keep & (1u << q)) stage[sh + lp++] = r.b[q];
     __syncthreads();
-    const int nk = int(tile_sum >> 32);
+    const int nk = tile_sum >> 16;
     if (nk == 0) return;
     // output [ob, ob + nk): bytes up to the first 16-byte boundary, 128-bit words, bytes after the last
     const int64_t a0 = (ob + 15) & ~int64_t(15), a1 = (ob + nk) & ~int64_t(15);
