@@ -248,3 +248,56 @@ def test_request_marshalling():
     p = C.cast(A._ptr(arr, N.SWReq), C.POINTER(N.SWReq))
     assert (p[1].scoring.match, p[1].scoring.mismatch, p[1].scoring.gap_open, p[1].scoring.gap_extend,
             p[1].with_cigar) == (1, -1, 2, 1, 0)
+
+
+def test_cpp_fasta_surface(tmp_path):
+    """A C++ caller of include/genoaligner/io/fasta.hpp (the reference's
+    io::read_fasta / read_fasta_file surface) gets the reference's records and
+    messages (tests/io/test_fasta.cpp:43-166 cases); host only."""
+    from genoaligner_amd import _native as N
+    src = tmp_path / "fa.cpp"
+    src.write_text(r'''
+#include "genoaligner/io/fasta.hpp"
+#include <cstdio>
+using namespace genoaligner::io;
+static void show(const char* text) {
+    FILE* f = std::tmpfile();
+    std::fputs(text, f);
+    std::rewind(f);
+    std::vector<FastaRecord> r;
+    std::string err;
+    const bool ok = read_fasta(f, &r, &err);
+    std::fclose(f);
+    std::printf("%d %zu", int(ok), r.size());
+    for (auto& x : r) std::printf(" [%s|%s|%s]", x.id.c_str(), x.description.c_str(), x.sequence.c_str());
+    std::printf(" %s\n", ok ? "-" : err.c_str());
+}
+int main(int argc, char** argv) {
+    show(">seq1 first record\nACGTACGTAC\nGTACGTACGT\nACGT\n>seq2 second\nTTTT\n");
+    show(">a\r\nACGT\r\nACGT\r\n>b\r\nTT\r\n");
+    show(">empty\n>next\nACGT\n");
+    show("");
+    std::vector<FastaRecord> r;
+    std::string err;
+    const bool miss = read_fasta_file(argv[1], &r, &err);
+    std::printf("%d %s\n", int(miss), err.c_str());
+    const bool got = read_fasta_file(argv[2], &r, &err);
+    std::printf("%d %zu %s\n", int(got), r.size(), r.empty() ? "" : r[0].id.c_str());
+    return 0;
+}
+''')
+    exe = tmp_path / "fa"
+    libdir = os.path.dirname(N.LIB_PATH)
+    subprocess.check_call(["g++", "-std=c++17", str(src), "-I", os.path.join(ROOT, "include"), "-L", libdir,
+                           "-lgenoaligner_amd", f"-Wl,-rpath,{libdir}", "-o", str(exe)])
+    good = tmp_path / "x.fa"
+    good.write_bytes(b">x\nACGT\n")
+    missing = str(tmp_path / "definitely_not_here_9f3a.fa")
+    out = subprocess.check_output([str(exe), missing, str(good)], text=True).splitlines()
+    assert out[0] == "1 2 [seq1|first record|ACGTACGTACGTACGTACGTACGT] [seq2|second|TTTT] -"
+    assert out[1] == "1 2 [a||ACGTACGT] [b||TT] -"
+    assert out[2] == "0 0 record has a header but no sequence (line 1)"
+    assert out[3] == "0 0 no records found (line 1)"
+    assert out[4] == "0 cannot open '%s'" % missing
+    assert out[5] == "1 1 x"
+
