@@ -223,9 +223,9 @@ def main():
                          "known traffic, to scale hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path "
-              f"(--gpus {args.gpus} is not supported; pairs are independent, see DESIGN.md "
-              f"for how the batch would be sharded)", file=sys.stderr)
+        print(f"bench.py measures one GPU: there is no multi-GPU path "
+              f"(--gpus {args.gpus} is not supported; pairs are independent, so a batch would be "
+              f"split by cost into one contiguous range per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import torch

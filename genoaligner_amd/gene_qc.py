@@ -1,4 +1,4 @@
-"""Gene QC task driver (reference: tools/gene_qc.cpp): every record of a gene
+"""Gene QC tool (reference: tools/gene_qc.cpp): every record of a gene
 FASTA aligned to the gene's bait by Smith-Waterman with CIGAR on the GPU (one
 pattern, many texts), then each CIGAR walked for coverage, frameshifts and
 in-frame stops (``ga_gene_qc_walk``, native, on the host's worker threads).

@@ -3,7 +3,7 @@ walk on hand-worked CIGARs; the oracle's whole pipeline (FASTA reader, bait
 choice, SW batch, walk, flags) against the reference tool itself (built from
 its sources into oracle/_ref by oracle/build_ref.sh) on generated gene
 families; the library's native walk (ga_gene_qc_walk, a host call) against the
-oracle's; and the driver's command-line and FASTA errors."""
+oracle's; and the tool's command-line and FASTA errors."""
 import ctypes as C
 import io
 
@@ -104,7 +104,7 @@ def test_library_walk_matches_oracle(oracle):
         assert got == want
 
 
-def test_driver_argument_and_fasta_errors(tmp_path, capsys):
+def test_tool_argument_and_fasta_errors(tmp_path, capsys):
     from genoaligner_amd import gene_qc as G
     assert G.main(["gene_qc", "a.fa"]) == 2
     assert G.main(["gene_qc", "a.fa", "--code", "2"]) == 2

@@ -1,4 +1,4 @@
-"""The gene QC driver on the GPU (SW + CIGAR through ga_align_sw_batch, the
+"""The gene QC tool on the GPU (SW + CIGAR through ga_align_sw_batch, the
 native walk): its TSV and summary against the reference tool itself
 (oracle/_ref/gene_qc, built from tools/gene_qc.cpp) where it travelled with the
 tree, else against the oracle's pipeline (pinned to the reference tool in
@@ -25,7 +25,7 @@ def _want(oracle, sp, rp, seqs, refs, code, args, tmp_path, kw):
     (200, 200, 1, ("--min-cov", "0.9"), {"min_cov": 0.9}, 0),
     (120, 150, 2, ("--scoring", "2,-3,5,2"), {"scoring": (2, -3, 5, 2)}, 37),   # chunked
 ])
-def test_driver_matches_reference_tool(gpu, oracle, tmp_path, n, codons, code, args, kw, batch):
+def test_tool_matches_reference_tool(gpu, oracle, tmp_path, n, codons, code, args, kw, batch):
     from genoaligner_amd import gene_qc as G
     from genoaligner_amd import workloads as W
     seqs, refs = W.gene_family(n, codons=codons, seed=n + code)
@@ -44,7 +44,7 @@ def test_driver_matches_reference_tool(gpu, oracle, tmp_path, n, codons, code, a
                                      % (n, 3 * codons + 1, code))
 
 
-def test_driver_cli_to_stdout(gpu, oracle, tmp_path, capsys):
+def test_tool_cli_to_stdout(gpu, oracle, tmp_path, capsys):
     from genoaligner_amd import gene_qc as G
     from genoaligner_amd import workloads as W
     seqs, refs = W.gene_family(40, codons=100, seed=2)
