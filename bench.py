@@ -571,12 +571,12 @@ def main():
         med = float(np.median(qc_ms))
         got_tsv = open(os.path.join(qdir, "got.tsv"), encoding="latin-1").read()
         n_rec = args.gene_qc_records
-        cells = sum(int(ln.split("\t")[1]) for ln in got_tsv.splitlines()[1:]) * 1141
+        qc_cells = sum(int(ln.split("\t")[1]) for ln in got_tsv.splitlines()[1:]) * 1141
         qc_line = {
             "workload": f"gene_family {n_rec} records vs a 1,141 bp bait (CYTB-sized, code 2, 3 % "
                         "substitutions, frameshifts, stops, fragments, flanks), FASTA files to TSV",
             "value": round(n_rec / (med * 1e-3), 1), "unit": "records/s", "ms_per_run": round(med, 2),
-            "gcups": round(cells / (med * 1e-3) / 1e9, 1),   # record length x bait length, whole run
+            "gcups": round(qc_cells / (med * 1e-3) / 1e9, 1),   # record length x bait length, whole run
         }
         if os.path.exists(O.REF_GENE_QC):
             ssub, _ = W.gene_family(args.gene_qc_cpu_records, codons=380, seed=21)

@@ -121,8 +121,16 @@ class Server:
     # ------------------------------------------------------------ client side
     def submit(self, kind: str, items: Sequence, with_cigar: bool = True) -> Request:
         req = Request(kind=kind, items=list(items), with_cigar=with_cigar, arrival=self.clock())
-        if req.device_bytes() + self.workspace > self.budget:
-            req.error = "request larger than the memory budget"
+        refuse = None
+        if kind == "sw" and any(bool(it.with_cigar) != bool(with_cigar) for it in req.items):
+            # SW batches are keyed by the flag, and a score-only pair's reported
+            # end cell depends on whether its batch traces: a request whose
+            # items disagree with its flag could land in a mixed batch
+            refuse = "SW request items must all have with_cigar equal to the request's flag"
+        elif req.device_bytes() + self.workspace > self.budget:
+            refuse = "request larger than the memory budget"
+        if refuse:
+            req.error = refuse
             req.done_at = self.clock()
             req._done.set()
             return req

@@ -64,19 +64,29 @@ def test_device_ingest_random_small(gpu, oracle):
     assert 40 < bad < 360
 
 
+# The device ingest cuts a file into tiles of 512 threads x 32 bytes; the
+# line state, header count and kept-byte count cross each seam between them.
+FASTA_TILE = 512 * 32
+
+
 def test_device_ingest_tile_seams(gpu, oracle):
-    """Files of several 4 KiB tiles with line breaks, headers and \\r\\n pairs
-    placed across the tile seams."""
+    """Files of two to six 16 KiB tiles with a \\r\\n pair, a header and
+    sequence bytes planted at a tile seam (one byte before, at, or after it),
+    plus random line breaks, headers and \\r\\n pairs elsewhere."""
     rng = random.Random(3)
-    for it in range(40):
+    for it in range(48):
         parts = [b">r0 x\n"]
-        while sum(map(len, parts)) < rng.randint(9000, 40000):
+        target = rng.randint(2 * FASTA_TILE + 100, 6 * FASTA_TILE)
+        while sum(map(len, parts)) < target:
             parts.append(rng.choice([b"ACGT" * rng.randint(1, 40), b"\n", b"\r\n", b"\r", b"acgn",
                                      b"\n>h%d d\n" % it, b"\r\n>q\r\n"]))
         data = b"".join(parts) + b"\nA"
-        seam = rng.choice([4095, 4096, 8191, 8192])
-        if len(data) > seam + 2:
-            data = data[:seam - 1] + b"\r\n>s\nGG" + data[seam - 1:]
+        k = rng.randint(1, len(data) // FASTA_TILE - 1)
+        seam = k * FASTA_TILE + (-1, 0, 1)[it % 3]
+        # the planted bytes start one before the seam, so \\r, \\n, '>' and
+        # the first sequence byte each land on it in turn as `seam` moves
+        for cut in (seam - 1, seam + 7):
+            data = data[:cut] + b"\r\n>s\nGG" + data[cut:]
         _check(oracle, data, shift=(0, 1, 7, 16)[it % 4])
 
 

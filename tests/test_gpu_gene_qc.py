@@ -1,38 +1,31 @@
 """The gene QC tool on the GPU (SW + CIGAR through ga_align_sw_batch, the
-native walk): its TSV and summary against the reference tool itself
-(oracle/_ref/gene_qc, built from tools/gene_qc.cpp) where it travelled with the
-tree, else against the oracle's pipeline (pinned to the reference tool in
-tests/test_gene_qc.py).  Byte-exact."""
+native walk): its TSV and summary against the oracle's pipeline, which
+tests/test_gene_qc.py pins byte for byte to the reference tool itself
+(tools/gene_qc.cpp built from its sources) on the CPU.  Byte-exact."""
 import io
-import os
 
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-def _want(oracle, sp, rp, seqs, refs, code, args, tmp_path, kw):
-    if os.path.exists(oracle.REF_GENE_QC):
-        rc, out, err, tsv = oracle.ref_gene_qc(str(sp), str(rp), ("--code", str(code)) + args,
-                                               emit=str(tmp_path / "ref.tsv"))
-        assert rc == 0
-        return tsv, [ln for ln in err.splitlines() if ln.startswith("summary:")][0]
+def _want(oracle, seqs, refs, code, kw):
     return oracle.gene_qc_tsv(seqs, refs, code=code, **kw)
 
 
-@pytest.mark.parametrize("n,codons,code,args,kw,batch", [
-    (300, 380, 2, (), {}, 0),                                   # CYTB-sized, vertebrate mitochondrial code
-    (200, 200, 1, ("--min-cov", "0.9"), {"min_cov": 0.9}, 0),
-    (120, 150, 2, ("--scoring", "2,-3,5,2"), {"scoring": (2, -3, 5, 2)}, 37),   # chunked
+@pytest.mark.parametrize("n,codons,code,kw,batch", [
+    (300, 380, 2, {}, 0),                                   # CYTB-sized, vertebrate mitochondrial code
+    (200, 200, 1, {"min_cov": 0.9}, 0),
+    (120, 150, 2, {"scoring": (2, -3, 5, 2)}, 37),          # chunked
 ])
-def test_tool_matches_reference_tool(gpu, oracle, tmp_path, n, codons, code, args, kw, batch):
+def test_tool_matches_oracle_pipeline(gpu, oracle, tmp_path, n, codons, code, kw, batch):
     from genoaligner_amd import gene_qc as G
     from genoaligner_amd import workloads as W
     seqs, refs = W.gene_family(n, codons=codons, seed=n + code)
     sp, rp = tmp_path / "s.fa", tmp_path / "r.fa"
     sp.write_bytes(seqs)
     rp.write_bytes(refs)
-    want_tsv, want_summary = _want(oracle, sp, rp, seqs, refs, code, args, tmp_path, kw)
+    want_tsv, want_summary = _want(oracle, seqs, refs, code, kw)
     out, err = io.StringIO(), io.StringIO()
     emit = str(tmp_path / "got.tsv")
     rc = G.run(str(sp), str(rp), code=code, emit=emit, out=out, err=err, batch_size=batch,

@@ -150,3 +150,16 @@ def test_paused_server_batches_the_queue():
         assert r.wait(5)
     s.close()
     assert len(fb.calls) <= 2 and sum(len(t) for _, t in fb.calls) == 8
+
+
+def test_sw_request_with_disagreeing_item_flags_is_refused():
+    """SW batches are keyed by the request's CIGAR flag; an item whose own flag
+    differs would make a mixed batch, so such a request is refused."""
+    fb = FakeBackend(delay=0.01)
+    s = Server(budget_bytes=1 << 30, slots=1, backend=fb)
+    r = s.submit("sw", [_sw("a", cigar=True), _sw("b", cigar=False)], with_cigar=True)
+    ok = s.submit("sw", [_sw("c", cigar=False)], with_cigar=False)
+    assert r.wait(1) and r.error.startswith("SW request items must all have with_cigar")
+    assert ok.wait(5) and ok.error is None
+    s.close()
+    assert [tags for _, tags in fb.calls] == [["c"]]
