@@ -7,6 +7,8 @@
 
 #include <functional>
 
+struct ga_wfa_dev_batch;
+
 namespace ga {
 
 // Packed sequence buffers carry this many readable bytes before their first
@@ -206,6 +208,13 @@ hipError_t sw16_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHo
 // score-only: the wave64 lane-rule end), cig_len = CIGAR length (may be null).
 hipError_t sw_gather(const int32_t* res8, int64_t n, bool trace, int32_t* out5,
                      int32_t* cig_len, hipStream_t st);
+
+// ga_wfa_device with device planning and every scratch buffer carved from
+// one device region (plan_bytes for the claim order and plan, stack_bytes of
+// wavefront stacks reused by each size class in turn); the classes run one
+// after another on `st`.  Allocates nothing.  Returns an error string or null.
+const char* wfa_device_in_region(const ga_wfa_dev_batch* b, hipStream_t st, uint8_t* region, size_t plan_bytes,
+                                 size_t stack_bytes);
 
 // Run chunk(k0, k1) over [0, n) on the host work pool, about `grain` indices a
 // chunk; returns when every chunk has run.

@@ -705,9 +705,16 @@ struct WfaDevClasses {
     ga::WfaDevPlan plan{};
 };
 
+// `stack_cap`: bytes of wavefront stacks one class launch may hold (its
+// workgroup count is capped to fit); `stack_alloc` (default: `alloc`) gives
+// each class its stacks.
 const char* setup_wfa_dev_classes(const ga::WfaQueueLaunch& base, int64_t n, int smax_max, int max_len,
-                                  bool trace, WfaDevClasses& C, const std::function<void*(size_t)>& alloc)
+                                  bool trace, WfaDevClasses& C, const std::function<void*(size_t)>& alloc,
+                                  int64_t stack_cap = -1,
+                                  const std::function<void*(size_t)>* stack_alloc = nullptr)
 {
+    if (stack_cap < 0) stack_cap = wfa_stack_budget();
+    if (!stack_alloc) stack_alloc = &alloc;
     C.order = static_cast<int32_t*>(alloc(size_t(n) * 4 + 64));
     C.next = static_cast<int32_t*>(alloc(64));
     C.plan.hist = static_cast<int32_t*>(alloc(size_t(ga::kWfaClasses) * ga::kPlanKeys * 4));
@@ -732,9 +739,9 @@ const char* setup_wfa_dev_classes(const ga::WfaQueueLaunch& base, int64_t n, int
         L.dev_class = C.plan.cls + 2 * c;
         int groups = ga::wfa_queue_groups(L);
         const int64_t per_group = int64_t(ga::wfa_queue_ws_bytes(L.smax_cls, 1, L.wide));
-        if (trace) groups = int(std::max<int64_t>(1, std::min<int64_t>(groups, wfa_stack_budget() / per_group)));
+        if (trace) groups = int(std::max<int64_t>(1, std::min<int64_t>(groups, stack_cap / per_group)));
         L.n_groups = groups;
-        L.ws = trace ? alloc(size_t(per_group) * size_t(groups)) : nullptr;
+        L.ws = trace ? (*stack_alloc)(size_t(per_group) * size_t(groups)) : nullptr;
         if (trace && !L.ws) return "hipMalloc(wavefront stacks)";
         C.L.push_back(L);
     }
@@ -1241,6 +1248,43 @@ const char* check_dev_batch(const ga_wfa_dev_batch* b)
 }
 
 }  // namespace
+
+// A device-planned WFA batch whose every scratch buffer is carved from one
+// caller-given device region of plan_bytes + stack_bytes (no allocation):
+// the claim order, plan histogram, class slices and claim counters from the
+// first plan_bytes, and the wavefront stacks of each class from the rest,
+// which every class reuses in turn -- the classes run one after another on
+// `st` (no forked streams, no events).  The caller has checked the batch and
+// that it fits.
+const char* ga::wfa_device_in_region(const ga_wfa_dev_batch* b, hipStream_t st, uint8_t* region,
+                                     size_t plan_bytes, size_t stack_bytes)
+{
+    ga::WfaQueueLaunch L{};
+    L.pat = b->pattern; L.poff = b->pattern_off; L.plen = b->pattern_len;
+    L.txt = b->text; L.toff = b->text_off; L.tlen = b->text_len;
+    L.smax = b->smax; L.want = b->want_cigar;
+    L.score = b->score; L.cigar = b->cigar; L.cig_off = b->cigar_off; L.cig_len = b->cigar_len;
+    L.wide = b->max_len > 32766;
+    size_t used = 0;
+    auto carve = [&](size_t bytes) -> void* {
+        const size_t at = (used + 255) & ~size_t(255);
+        if (at + bytes > plan_bytes) return nullptr;
+        used = at + bytes;
+        return region + at;
+    };
+    std::function<void*(size_t)> alloc = carve;
+    std::function<void*(size_t)> stacks = [&](size_t bytes) -> void* {
+        return bytes <= stack_bytes ? region + plan_bytes : nullptr;
+    };
+    WfaDevClasses C;
+    if (const char* e = setup_wfa_dev_classes(L, b->n_pairs, b->smax_max, b->max_len, b->cigar != nullptr, C,
+                                              alloc, int64_t(stack_bytes), &stacks))
+        return e;
+    if (ga::wfa_plan_launch(C.plan, st) != hipSuccess) return "WFA plan launch";
+    for (const auto& cl : C.L)
+        if (ga::wfa_queue_launch(cl, st) != hipSuccess) return "WFA kernel launch";
+    return nullptr;
+}
 
 void ga::host_parallel_for(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& chunk)
 {

@@ -189,6 +189,24 @@ typedef struct ga_wfa_dev_batch {
 
 int ga_wfa_device(const ga_wfa_dev_batch* b, void* stream, const char** error);
 
+/* EXTENSION with no counterpart in the reference, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): ga_wfa_device with device planning (the
+ * host copies in `b` are ignored) that calls no hipMalloc / hipFree and no
+ * allocating library routine.  Every scratch buffer (claim order, plan,
+ * claim counters, wavefront stacks) lives in one static device array of
+ * GA_WFA_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; the size
+ * classes run one after another on `stream`, each with as many workgroups as
+ * its stacks fit in the array.  Same scores and CIGARs as ga_wfa_device.
+ * Refused with GA_INVALID_ARGUMENT: more than GA_WFA_NOALLOC_MAX_PAIRS pairs;
+ * with CIGAR, a (smax_max + 1)^2 stack of 16-bit offsets (32-bit when max_len
+ * >= 32767) larger than the array's stack part (any smax_max <= 4000 with
+ * 16-bit offsets; up to 3962 with 32-bit ones).  Calls on one GPU share its
+ * array: calls that overlap on one GPU race. */
+#define GA_WFA_NOALLOC_SCRATCH_BYTES (64 << 20)
+#define GA_WFA_NOALLOC_MAX_PAIRS (1 << 20)
+int ga_wfa_device_noalloc(const ga_wfa_dev_batch* b, void* stream, const char** error);
+
 /* Without the host copies, ga_wfa_device plans on the device: a histogram /
  * scan / scatter over (size class, length) gives each class its slice of a
  * longest-first claim order, and every class that can hold a pair is launched
