@@ -116,56 +116,121 @@ def check_wfa(ps, dev, sample, oracle):
     return bad
 
 
-def cpu_baseline_sw(ps, seconds, threads, oracle, tab=None, gaps=None,
-                    label="256 x 1024, score + span + CIGAR"):
-    """The oracle's SW trace path on `threads` host threads over a sample of the
-    same workload, sized to run about `seconds` (score table and gap costs of
-    the DNA scheme unless given)."""
+def host_info():
+    """The benchmark host's CPU: model, logical CPUs, physical cores, the CPUs
+    this process may run on, and the threads the CPU baselines use (this
+    process's CPU share, at most 16: the share one GPU's host slice carries)."""
+    model, phys = "?", set()
+    try:
+        cur = {}
+        for line in open("/proc/cpuinfo"):
+            if ":" not in line:
+                if "core id" in cur:
+                    phys.add((cur.get("physical id", "0"), cur["core id"]))
+                cur = {}
+                continue
+            k, v = (x.strip() for x in line.split(":", 1))
+            cur[k] = v
+            if k == "model name":
+                model = v
+        if "core id" in cur:
+            phys.add((cur.get("physical id", "0"), cur["core id"]))
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "physical_cores": len(phys) or None,
+            "allowed_cpus": allowed, "threads": max(1, min(allowed, 16))}
+
+
+def cpu_timed(prepare, n_total, work_of, unit, label, threads, seed, target_s=1.2, single_s=0.6):
+    """The oracle timed as the reference CPU path is timed: a sample sized to
+    about `target_s` per repetition on `threads` host threads, one warm-up,
+    then the median (and min / max) of 5 repetitions; the same on one thread
+    with a sample sized to about `single_s`.  prepare(idx) -> go(threads) ->
+    seconds for the pairs idx (a random sample; a repetition passes over it
+    several times when the whole workload runs in less than the target);
+    work_of(idx) -> units of work (pairs, or cells / 1e9)."""
+    perm = np.random.default_rng(seed).permutation(n_total)
+    cache = {}
+
+    def timed(k, th):
+        m = min(k, n_total)
+        if m not in cache:
+            cache.clear()
+            idx = np.sort(perm[:m])
+            cache[m] = (idx, prepare(idx))
+        idx, go = cache[m]
+        loops = -(-k // m)
+        return sum(go(th) for _ in range(loops)), loops * work_of(idx), loops * m
+
+    def reps(th, target):
+        k = max(th * 4, 16)
+        dt, _, _ = timed(k, th)
+        while dt < 0.25:
+            k = int(k * max(2.0, 0.3 / max(dt, 1e-4)))
+            dt, _, _ = timed(k, th)
+        k = max(1, int(k * target / dt))
+        timed(k, th)                                   # warm-up
+        runs = sorted(timed(k, th)[:2] for _ in range(5))
+        ts = [t for t, _ in runs]
+        w = runs[0][1]
+        return w / ts[2], w / ts[-1], w / ts[0], k, ts[2]
+
+    v, lo, hi, n, med = reps(threads, target_s)
+    v1, lo1, hi1, n1, med1 = reps(1, single_s)
+    return {"value": round(v, 4), "unit": unit, "cores": threads, "kind": "port",
+            "sample": f"{n} pairs of {label} (a random sample, passed over several times if the workload is "
+                      f"smaller); median of 5 repetitions ({med:.2f} s each) after one warm-up, on {threads} "
+                      "host threads",
+            "min": round(lo, 4), "max": round(hi, 4),
+            "single_core": {"value": round(v1, 4), "min": round(lo1, 4), "max": round(hi1, 4),
+                            "sample": f"{n1} pairs, median of 5 ({med1:.2f} s each) after one warm-up"},
+            "host": host_info()}
+
+
+def cpu_baseline_sw(ps, threads, oracle, tab=None, gaps=None, label="256 x 1024, score + span + CIGAR"):
+    """The oracle's SW trace path (score, span, CIGAR) over a random sample of
+    the same pairs (score table and gap costs of the DNA scheme unless given)."""
     L = oracle.lib()
     if tab is None:
         tab = oracle.mm_table(SW_SCHEME[0], SW_SCHEME[1])
-    go, ge = gaps if gaps else (SW_SCHEME[2], SW_SCHEME[3])
+    go_, ge_ = gaps if gaps else (SW_SCHEME[2], SW_SCHEME[3])
 
-    def run(n):
-        sub = ps.subset(np.arange(n))
+    def prepare(idx):
+        sub = ps.subset(idx)
+        n = len(idx)
         out = np.zeros(5 * n, dtype=np.int32)
         cig = np.zeros(int((sub.a_len.astype(np.int64) + sub.b_len).sum()) + 1, dtype=np.uint8)
         coff = np.zeros(n, dtype=np.int64)
         coff[1:] = np.cumsum(sub.a_len.astype(np.int64) + sub.b_len)[:-1]
         clen = np.zeros(n, dtype=np.int32)
-        t0 = time.perf_counter()
-        rc = L.ga_oracle_sw_batch(sub.a.ctypes.data, sub.a_off.ctypes.data, sub.a_len.ctypes.data,
-                                  sub.b.ctypes.data, sub.b_off.ctypes.data, sub.b_len.ctypes.data,
-                                  n, tab.ctypes.data, go, ge, 1,
-                                  out.ctypes.data, cig.ctypes.data, coff.ctypes.data,
-                                  clen.ctypes.data, threads)
-        assert rc == 0
-        return time.perf_counter() - t0, int((sub.a_len.astype(np.int64) * sub.b_len).sum())
 
-    # calibrate on growing samples until one takes ~1 s, then size the timed run
-    probe = max(threads * 4, 32)
-    dt, _ = run(probe)
-    while dt < 1.0 and probe < ps.n:
-        probe = int(min(ps.n, probe * max(2.0, 1.2 / max(dt, 1e-3))))
-        dt, _ = run(probe)
-    n = int(min(ps.n, max(probe, probe * seconds / max(dt, 1e-3))))
-    dt, cells = run(n)
-    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": threads,
-            "kind": "port",
-            "sample": f"{n} of the {ps.n} pairs ({label}), "
-                      f"{dt:.1f} s on {threads} threads of the benchmark host"}
+        def go(th):
+            t0 = time.perf_counter()
+            rc = L.ga_oracle_sw_batch(sub.a.ctypes.data, sub.a_off.ctypes.data, sub.a_len.ctypes.data,
+                                      sub.b.ctypes.data, sub.b_off.ctypes.data, sub.b_len.ctypes.data,
+                                      n, tab.ctypes.data, go_, ge_, 1,
+                                      out.ctypes.data, cig.ctypes.data, coff.ctypes.data,
+                                      clen.ctypes.data, th)
+            assert rc == 0
+            return time.perf_counter() - t0
+        return go
+
+    cells = lambda idx: float((ps.a_len[idx].astype(np.int64) * ps.b_len[idx]).sum()) / 1e9
+    return cpu_timed(prepare, ps.n, cells, "GCUPS", label, threads, seed=2)
 
 
-def cpu_baseline_wfa(ps, want, seconds, threads, oracle):
-    """The oracle's WFA batch (score, CIGAR where wanted) on `threads` host
-    threads over a random sample of the pairs, sized to run about `seconds`."""
+def cpu_baseline_wfa(ps, want, threads, oracle, label):
+    """The oracle's WFA batch (score; CIGAR where wanted) over a random sample
+    of the pairs."""
     L = oracle.lib()
-    rng = np.random.default_rng(3)
-    perm = rng.permutation(ps.n)
 
-    def run(n):
-        idx = np.sort(perm[:n])
+    def prepare(idx):
         sub = ps.subset(idx)
+        n = len(idx)
         w = np.ascontiguousarray(want[idx], dtype=np.uint8)
         slot = sub.a_len.astype(np.int64) + sub.b_len
         cig = np.zeros(int(slot.sum()) + 1, dtype=np.uint8)
@@ -174,24 +239,47 @@ def cpu_baseline_wfa(ps, want, seconds, threads, oracle):
         sc = np.zeros(n, dtype=np.int32)
         clen = np.zeros(n, dtype=np.int32)
         smax = np.ascontiguousarray(sub.smax, dtype=np.int32)
-        t0 = time.perf_counter()
-        rc = L.ga_oracle_wfa_batch(sub.a.ctypes.data, sub.a_off.ctypes.data, sub.a_len.ctypes.data,
-                                   sub.b.ctypes.data, sub.b_off.ctypes.data, sub.b_len.ctypes.data,
-                                   smax.ctypes.data, w.ctypes.data, n, sc.ctypes.data,
-                                   cig.ctypes.data, coff.ctypes.data, clen.ctypes.data, threads)
-        assert rc == 0
-        return time.perf_counter() - t0
 
-    probe = max(threads * 8, 64)
-    dt = run(probe)
-    while dt < 1.0 and probe < ps.n:
-        probe = int(min(ps.n, probe * max(2.0, 1.2 / max(dt, 1e-3))))
-        dt = run(probe)
-    n = int(min(ps.n, max(probe, probe * seconds / max(dt, 1e-3))))
-    dt = run(n)
-    return {"value": round(n / dt, 1), "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": f"{n} pairs drawn at random from the {ps.n}, {dt:.1f} s on "
-                      f"{threads} threads of the benchmark host"}
+        def go(th):
+            t0 = time.perf_counter()
+            rc = L.ga_oracle_wfa_batch(sub.a.ctypes.data, sub.a_off.ctypes.data, sub.a_len.ctypes.data,
+                                       sub.b.ctypes.data, sub.b_off.ctypes.data, sub.b_len.ctypes.data,
+                                       smax.ctypes.data, w.ctypes.data, n, sc.ctypes.data,
+                                       cig.ctypes.data, coff.ctypes.data, clen.ctypes.data, th)
+            assert rc == 0
+            return time.perf_counter() - t0
+        return go
+
+    return cpu_timed(prepare, ps.n, lambda idx: float(len(idx)), "pairs/s", label, threads, seed=3)
+
+
+SECTIONS = ("sw", "sw_host", "wfa", "mixed", "routing", "protein", "fasta", "gene_qc", "serving")
+
+
+def mean_pos(xs):
+    xs = [x for x in xs if x and x > 0]
+    return float(np.mean(xs)) if xs else None
+
+
+def _pack_seqs(seqs):
+    lens = np.array([len(x) for x in seqs], dtype=np.int32)
+    off = np.zeros(len(seqs), dtype=np.int64)
+    if len(seqs) > 1:
+        off[1:] = np.cumsum(lens.astype(np.int64))[:-1]
+    return np.frombuffer(b"".join(seqs), dtype=np.uint8).copy(), off, lens
+
+
+def wfa_requests_pairset(items):
+    """A PairSet (a = pattern, b = text, per-pair smax) of WFA requests."""
+    from genoaligner_amd.workloads import PairSet
+    return PairSet(*_pack_seqs([bytes(r.pattern) for r in items]), *_pack_seqs([bytes(r.text) for r in items]),
+                   np.array([r.smax for r in items], dtype=np.int32))
+
+
+def sw_requests_pairset(items):
+    """A PairSet (a = text, b = pattern) of SW requests."""
+    from genoaligner_amd.workloads import PairSet
+    return PairSet(*_pack_seqs([bytes(r.text) for r in items]), *_pack_seqs([bytes(r.pattern) for r in items]))
 
 
 def main():
@@ -199,7 +287,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--only", default=None,
+                    help="comma-separated sections to run (" + ", ".join(SECTIONS) + "); default: all")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-wfa", action="store_true")
     ap.add_argument("--no-wfa-mixed", action="store_true")
@@ -216,7 +305,7 @@ def main():
     ap.add_argument("--gene-qc-records", type=int, default=20000)
     ap.add_argument("--gene-qc-cpu-records", type=int, default=300)
     ap.add_argument("--fasta-mib", type=int, default=512)
-    ap.add_argument("--serve-rate", type=float, default=100.0, help="requests per second (Poisson)")
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="requests per second (Poisson), lowest load")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
@@ -227,6 +316,17 @@ def main():
               f"(--gpus {args.gpus} is not supported; pairs are independent, so a batch would be "
               f"split by cost into one contiguous range per GPU)", file=sys.stderr)
         sys.exit(2)
+    only = set(args.only.split(",")) if args.only else set(SECTIONS)
+    bad = only - set(SECTIONS)
+    if bad:
+        print(f"unknown sections {sorted(bad)}; known: {', '.join(SECTIONS)}", file=sys.stderr)
+        sys.exit(2)
+    for flag, sec in ((args.no_wfa, "wfa"), (args.no_wfa_mixed, "mixed"), (args.no_routing, "routing"),
+                      (args.no_protein, "protein"), (args.no_serve, "serving"), (args.no_fasta, "fasta"),
+                      (args.no_gene_qc, "gene_qc")):
+        if flag:
+            only.discard(sec)
+    cpu_on = not args.no_cpu_baseline
 
     import torch
     from genoaligner_amd import _native as N
@@ -240,6 +340,8 @@ def main():
     props = torch.cuda.get_device_properties(0)
     dev_name = f"{torch.cuda.get_device_name(0)} ({getattr(props, 'gcnArchName', '?')})"
     log(f"device: {dev_name}")
+    threads = host_info()["threads"]
+    rng = np.random.default_rng(1)
 
     if args.copy_probe_mib > 0:
         nbytes = args.copy_probe_mib << 20
@@ -251,67 +353,122 @@ def main():
         del src, dst
 
     # ---------------- SW headline ----------------
-    t0 = time.perf_counter()
-    sw = W.sw_embedded(n=50_000, qlen=256, tlen=1024, rate=0.10)
-    cells = int((sw.a_len.astype(np.int64) * sw.b_len).sum())
-    swd = SwDeviceBatch(sw, scoring=SW_SCHEME, trace=True)
-    log(f"sw workload ready ({time.perf_counter() - t0:.1f} s), {cells / 1e9:.2f} Gcells/step")
-    wall, step_ms, kern = time_steps(swd.run, args.steps, args.warmup, lib)
-    ms_per_step = wall * 1000.0 / args.steps
-    gcups = cells / (wall / args.steps) / 1e9
-    kern_ms = float(np.mean([k for k in kern if k > 0])) if any(k > 0 for k in kern) else None
-    log(f"sw: {ms_per_step:.2f} ms/step, {gcups:.1f} GCUPS, forward kernel {kern_ms} ms")
-    rng = np.random.default_rng(1)
-    sample = rng.choice(sw.n, size=min(300, sw.n), replace=False)
-    sw_bad = check_sw(sw, swd, sample, O)
-    log(f"sw: {sw_bad} mismatches in {len(sample)} checked pairs")
+    sw = None
+    gcups = ms_per_step = step_ms = roof = sw_score_line = cpu = sw_host_line = None
+    sw_bad, sample, cells = None, [], None
+    if only & {"sw", "sw_host"}:
+        t0 = time.perf_counter()
+        sw = W.sw_embedded(n=50_000, qlen=256, tlen=1024, rate=0.10)
+        cells = int((sw.a_len.astype(np.int64) * sw.b_len).sum())
+        log(f"sw workload ready ({time.perf_counter() - t0:.1f} s), {cells / 1e9:.2f} Gcells/step")
+        sample = rng.choice(sw.n, size=min(300, sw.n), replace=False)
+    if "sw" in only:
+        swd = SwDeviceBatch(sw, scoring=SW_SCHEME, trace=True)
+        wall, step_ms, kern = time_steps(swd.run, args.steps, args.warmup, lib)
+        ms_per_step = wall * 1000.0 / args.steps
+        gcups = cells / (wall / args.steps) / 1e9
+        kern_ms = mean_pos(kern)
+        log(f"sw: {ms_per_step:.2f} ms/step, {gcups:.1f} GCUPS, forward kernel {kern_ms} ms")
+        sw_bad = check_sw(sw, swd, sample, O)
+        log(f"sw: {sw_bad} mismatches in {len(sample)} checked pairs")
+        del swd
 
-    # HBM bytes per launch of the SW trace kernel, measured with rocprofv3
-    # FETCH_SIZE / WRITE_SIZE passes scaled by a copy of known size
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "sw16_trace_sw_embedded_50k_hbm_traffic.md")
-    if os.path.exists(tpath):
-        import re
-        mt = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(tpath).read())
-        traffic = float(mt.group(1)) if mt else None
-    roof = None
-    if kern_ms:
-        achieved = SW_OPS_PER_CELL * cells / (kern_ms * 1e-3) / 1e12
-        roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS,
-                "unit": "TOP/s", "frac": round(achieved / PEAK_INT16_TOPS, 4),
-                "traffic": traffic, "kernel": "sw16_kernel<true, false, true> (packed forward + fused walk)",
-                "kernel_ms": round(kern_ms, 3)}
+        # HBM bytes per launch of the SW trace kernel, measured with rocprofv3
+        # FETCH_SIZE / WRITE_SIZE passes scaled by a copy of known size
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "sw16_trace_sw_embedded_50k_hbm_traffic.md")
+        if os.path.exists(tpath):
+            import re
+            mt = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(tpath).read())
+            traffic = float(mt.group(1)) if mt else None
+        if kern_ms:
+            achieved = SW_OPS_PER_CELL * cells / (kern_ms * 1e-3) / 1e12
+            roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS,
+                    "unit": "TOP/s", "frac": round(achieved / PEAK_INT16_TOPS, 4),
+                    "traffic": traffic, "kernel": "sw16_kernel<true, false, true> (packed forward + fused walk)",
+                    "kernel_ms": round(kern_ms, 3)}
 
-    # score-only on the same pairs: score + end cell (the lane-rule end)
-    swd_s = SwDeviceBatch(sw, scoring=SW_SCHEME, trace=False)
-    swall, sstep, skern = time_steps(swd_s.run, args.steps, args.warmup, lib)
-    sk = float(np.mean([k for k in skern if k > 0])) if any(k > 0 for k in skern) else None
-    out_s, _ = swd_s.results()
-    s_bad = 0
-    for i in sample[:100]:
-        s_, ei, ej = O.sw_score_lane(sw.a_seq(i), sw.b_seq(i), SW_SCHEME)
-        if tuple(int(x) for x in out_s[i]) != (s_, -1, -1, ei, ej):
-            s_bad += 1
-    sw_score_line = {
-        "workload": "same pairs, score + end cell only",
-        "gcups": round(cells / (swall / args.steps) / 1e9, 3),
-        "ms_per_step": round(swall * 1000.0 / args.steps, 3),
-        "kernel_ms": round(sk, 4) if sk else None,
-        "roofline_frac": round(SW_OPS_PER_CELL * cells / (sk * 1e-3) / 1e12 / PEAK_INT16_TOPS, 4) if sk else None,
-        "mismatches": s_bad, "checked": 100,
-    }
-    log(f"sw score-only: {sw_score_line}")
-    del swd_s
+        # score-only on the same pairs: score + end cell (the lane-rule end)
+        swd_s = SwDeviceBatch(sw, scoring=SW_SCHEME, trace=False)
+        swall, sstep, skern = time_steps(swd_s.run, args.steps, args.warmup, lib)
+        sk = mean_pos(skern)
+        out_s, _ = swd_s.results()
+        s_bad = 0
+        for i in sample[:100]:
+            s_, ei, ej = O.sw_score_lane(sw.a_seq(i), sw.b_seq(i), SW_SCHEME)
+            if tuple(int(x) for x in out_s[i]) != (s_, -1, -1, ei, ej):
+                s_bad += 1
+        sw_score_line = {
+            "workload": "same pairs, score + end cell only",
+            "gcups": round(cells / (swall / args.steps) / 1e9, 3),
+            "ms_per_step": round(swall * 1000.0 / args.steps, 3),
+            "kernel_ms": round(sk, 4) if sk else None,
+            "roofline_frac": round(SW_OPS_PER_CELL * cells / (sk * 1e-3) / 1e12 / PEAK_INT16_TOPS, 4) if sk else None,
+            "mismatches": s_bad, "checked": 100,
+        }
+        log(f"sw score-only: {sw_score_line}")
+        del swd_s
+        if cpu_on:
+            cpu = cpu_baseline_sw(sw, threads, O)
+            log(f"cpu baseline: {cpu}")
 
-    # ---------------- WFA secondary ----------------
+    # ---------------- SW headline pairs through the drop-in host entry point ----------------
+    if "sw_host" in only:
+        import ctypes as C
+        # the C entry point as a C++ caller of align_sw_batch uses it: host
+        # sequences, request array and arena built before the clock
+        hreqs = (N.SWReq * sw.n)()
+        ab, bb = sw.a.ctypes.data, sw.b.ctypes.data
+        sc_c = N.SWScoringC(*SW_SCHEME)
+        for i in range(sw.n):
+            hreqs[i] = N.SWReq(ab + int(sw.a_off[i]), int(sw.a_len[i]), bb + int(sw.b_off[i]), int(sw.b_len[i]),
+                               sc_c, 1)
+        hres = (N.SWRes * sw.n)()
+        hneed = lib.ga_cigar_arena_bytes_sw(hreqs, sw.n)
+        harena = C.create_string_buffer(max(hneed, 1))
+        hok, hun, herr = C.c_int32(), C.c_int32(), C.c_char_p()
+        hts, hk = [], []
+        for rep in range(1 + 5):
+            t0 = time.perf_counter()
+            hrc = lib.ga_align_sw_batch(hreqs, sw.n, hres, harena, hneed, C.byref(hok), C.byref(hun),
+                                        C.byref(herr))
+            dt = time.perf_counter() - t0
+            if rep > 0:
+                hts.append(dt)
+                hk.append(lib.ga_last_kernel_ms())
+        assert hrc == N.GA_OK, herr.value
+        raw = harena.raw
+        hbad = 0
+        for i in sample[:100]:
+            r = hres[i]
+            cg = raw[r.cigar_off: r.cigar_off + r.cigar_len].decode() if r.cigar_len > 0 else ""
+            if (r.score, r.start_i, r.start_j, r.end_i, r.end_j, cg) != O.sw_trace(sw.a_seq(i), sw.b_seq(i),
+                                                                                       SW_SCHEME):
+                hbad += 1
+            if not (r.rescore_ok and r.wellformed_ok):
+                hbad += 1
+        hmed = float(np.median(hts))
+        sw_host_line = {
+            "workload": "the headline pairs through ga_align_sw_batch (the drop-in entry point of "
+                        "align_sw_batch): host sequences in, host results, CIGARs and validators out; "
+                        "packing, PCIe copies both ways and the CIGAR checks included; median of 5 calls "
+                        "after one untimed call",
+            "gcups": round(cells / hmed / 1e9, 3), "ms": round(hmed * 1e3, 2),
+            "p99_ms": round(max(hts) * 1e3, 2), "kernel_ms": round(mean_pos(hk), 3) if mean_pos(hk) else None,
+            "mismatches": hbad, "checked": 100,
+        }
+        log(f"sw host entry: {sw_host_line}")
+        del hreqs, hres, harena
+
+    # ---------------- WFA, 10k x 150 bp ----------------
     wfa_line = None
-    if not args.no_wfa:
+    if "wfa" in only:
         wf = W.wfa_plain(n=10_000, length=150, rate=0.02, smax=64)
         wfd = WfaDeviceBatch(wf, trace=True)
         wwall, wstep, wkern = time_steps(wfd.run, args.steps, args.warmup, lib)
         wsample = rng.choice(wf.n, size=min(1000, wf.n), replace=False)
         wbad = check_wfa(wf, wfd, wsample, O)
-        wk = float(np.mean([k for k in wkern if k > 0])) if any(k > 0 for k in wkern) else None
+        wk = mean_pos(wkern)
         in_bytes = int(((wf.a_len.astype(np.int64) + 3) // 4 + (wf.b_len.astype(np.int64) + 3) // 4).sum())
         sc, cg = wfd.results()
         out_bytes = 5 * wf.n + int(sum(len(c) for c in cg))
@@ -331,24 +488,27 @@ def main():
             "graph_ms_per_step": round(gwall * 1000.0 / args.steps, 3),
             "graph_vs_eager_differences": gdiff,
             "p50_ms": pctl(wstep, 50), "p99_ms": pctl(wstep, 99),
+            "graph_p99_ms": pctl(gstep, 99),
             "kernel_ms": round(wk, 4) if wk else None,
-            "hbm_roofline": {"bound": "hbm", "algorithmic_bytes": in_bytes + out_bytes,
-                             "achieved_GBs": round((in_bytes + out_bytes) / (wk * 1e-3) / 1e9, 2) if wk else None,
-                             "peak_GBs": PEAK_HBM_GBS},
+            "roofline": {"bound": "hbm", "achieved": round((in_bytes + out_bytes) / (wk * 1e-3) / 1e9, 2) if wk else None,
+                         "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round((in_bytes + out_bytes) / (wk * 1e-3) / 1e9 / PEAK_HBM_GBS, 5) if wk else None,
+                         "algorithmic_bytes": in_bytes + out_bytes,
+                         "kernel": "wfa_kernel<true, int16_t, true, 1> (smax <= 64 class)"},
             "valu_roofline": wfa_valu_roofline(wf, sc, wk),
             "mismatches": wbad, "checked": int(len(wsample)),
             "tcups_ref_style": round(float((wf.a_len.astype(np.int64) * wf.b_len).sum())
                                      / (wwall / args.steps) / 1e12, 4),
         }
         log(f"wfa: {wfa_line['pairs_per_s']:.0f} pairs/s, {wbad} mismatches")
-        if not args.no_cpu_baseline:
-            wfa_line["cpu_baseline"] = cpu_baseline_wfa(wf, np.ones(wf.n, dtype=np.uint8), args.cpu_seconds,
-                                                        min(os.cpu_count() or 1, 16), O)
+        if cpu_on:
+            wfa_line["cpu_baseline"] = cpu_baseline_wfa(wf, np.ones(wf.n, dtype=np.uint8), threads, O,
+                                                        "wfa_plain 150 bp, smax 64, score + CIGAR")
             log(f"wfa cpu baseline: {wfa_line['cpu_baseline']}")
 
     # ---------------- WFA, mixed lengths / distances ----------------
     mixed_line = None
-    if not args.no_wfa_mixed:
+    if "mixed" in only:
         t0 = time.perf_counter()
         wm = W.wfa_mixed(n=args.wfa_mixed_pairs)
         want = (np.random.default_rng(4).random(wm.n) < 0.7).astype(np.uint8)
@@ -369,7 +529,7 @@ def main():
             osc, ocg = O.wfa(wm.a_seq(i), wm.b_seq(i), int(wm.smax[i]))
             if int(sc[i]) != osc or cg[i].decode() != (ocg if want[i] else ""):
                 mbad += 1
-        mk = float(np.mean([k for k in mkern if k > 0])) if any(k > 0 for k in mkern) else None
+        mk = mean_pos(mkern)
         mixed_line = {
             "workload": f"wfa_mixed {wm.n} pairs, lengths log-uniform 100..10000 bp, divergence "
                         "0..15% (2:1:1), smax min(0.2 len, 2048), 70% CIGAR / 30% score-only",
@@ -382,16 +542,16 @@ def main():
             "autotuned_waves_per_class": tuned_waves,
         }
         log(f"wfa_mixed: {mixed_line}")
-        if not args.no_cpu_baseline:
-            mixed_line["cpu_baseline"] = cpu_baseline_wfa(wm, want, args.cpu_seconds,
-                                                          min(os.cpu_count() or 1, 16), O)
+        if cpu_on:
+            mixed_line["cpu_baseline"] = cpu_baseline_wfa(wm, want, threads, O,
+                                                          "wfa_mixed (70% CIGAR), drawn at random")
             log(f"wfa_mixed cpu baseline: {mixed_line['cpu_baseline']}")
         del wmd
         wfa_set_tuning(None)
 
     # ---------------- WFA routing: plain per-class path vs learned router ----------------
     routing_line = None
-    if not args.no_routing:
+    if "routing" in only:
         from genoaligner_amd import router as R
         t0 = time.perf_counter()
         wr = W.wfa_routing(n=args.routing_pairs)
@@ -400,10 +560,10 @@ def main():
         plain = WfaDeviceBatch(wr, trace=True, want=rwant)
         routed = WfaDeviceBatch(wr, trace=True, want=rwant)
         log(f"wfa_routing workload ready ({time.perf_counter() - t0:.1f} s)")
-        pwall, pstep, _ = time_steps(plain.run, args.steps, args.warmup, lib)
+        pwall, pstep, pkern = time_steps(plain.run, args.steps, args.warmup, lib)
         stats = []
-        rwall, rstep, _ = time_steps(lambda: stats.append(R.run_routed(routed, params, q=args.route_q)),
-                                     args.steps, args.warmup, lib)
+        rwall, rstep, rkern = time_steps(lambda: stats.append(R.run_routed(routed, params, q=args.route_q)),
+                                         args.steps, args.warmup, lib)
         psc, pcg = plain.results()
         rsc, rcg = routed.results()
         diff = int(sum(1 for i in range(wr.n) if psc[i] != rsc[i] or pcg[i] != rcg[i]))
@@ -414,6 +574,7 @@ def main():
             if int(rsc[i]) != osc or rcg[i].decode() != (ocg if rwant[i] else ""):
                 obad += 1
         st = stats[-1]
+        pk, rk = mean_pos(pkern), mean_pos(rkern)
         routing_line = {
             "workload": f"wfa_routing {wr.n} pairs, lengths log-uniform 100..10000 bp, divergence 80% "
                         "U[0,0.03] / 15% U[0.03,0.15] / 5% unrelated, smax min(0.2 len, 2048), "
@@ -422,28 +583,43 @@ def main():
             "routed_pairs_per_s": round(wr.n / (rwall / args.steps), 1),
             "routed_over_plain": round(pwall / rwall, 3),
             "plain_ms": round(pwall * 1000 / args.steps, 3), "routed_ms": round(rwall * 1000 / args.steps, 3),
+            "plain_p99_ms": pctl(pstep, 99),
             "routed_p50_ms": pctl(rstep, 50), "routed_p99_ms": pctl(rstep, 99),
+            "plain_kernel_ms": round(pk, 3) if pk else None,
+            "routed_kernel_ms": round(rk, 3) if rk else None,
+            # the pairs' algorithmic diagonal updates over the routed path's
+            # WFA launches (both passes), against the int32 VALU peak
+            "roofline": wfa_valu_roofline(wr, rsc, rk),
+            "plain_roofline": wfa_valu_roofline(wr, psc, pk),
             "routed_vs_plain_differences": diff, "oracle_mismatches": obad, "oracle_checked": int(len(rsample)),
             "shortcut": st.shortcut, "routed": st.routed, "plain": st.plain, "rerun": st.rerun,
             "miss_rate": round(st.miss_rate, 4), "route_q": args.route_q,
         }
         log(f"wfa_routing: {routing_line}")
+        if cpu_on:
+            routing_line["cpu_baseline"] = cpu_baseline_wfa(wr, rwant, threads, O,
+                                                            "wfa_routing (70% CIGAR), drawn at random")
+            log(f"wfa_routing cpu baseline: {routing_line['cpu_baseline']}")
         del plain, routed
 
     # ---------------- protein SW (BLOSUM62, gap open 11 / extend 1), host entry point ----------------
     protein_line = None
-    if not args.no_protein:
+    m62 = None
+    if only & {"protein", "serving"}:
         import genoaligner_amd as ga
-        pp = W.protein_pairs(n=2000)
         m62 = ga.blosum62(11, 1)
+        tab62 = O.matrix_table(m62.alphabet.decode(), [[m62.score[a][b] for b in range(20)] for a in range(20)], -1)
+    if "protein" in only:
+        import ctypes as C
+        import genoaligner_amd as ga
+        from genoaligner_amd import api as A
+        pp = W.protein_pairs(n=2000)
         preqs = [ga.SWRequest(text=pp.a_seq(i), text_len=int(pp.a_len[i]), pattern=pp.b_seq(i),
                               pattern_len=int(pp.b_len[i]), with_cigar=True) for i in range(pp.n)]
         pcells = int((pp.a_len.astype(np.int64) * pp.b_len).sum())
         ga.align_sw_matrix_batch(preqs[:64], m62)
         # the C entry point as a C++ caller uses it: request array and arena
         # built before the clock, as the caller's own data would be
-        import ctypes as C
-        from genoaligner_amd import api as A, _native as N
         keep: list = []
         parr = (N.SWReq * pp.n)()
         for i, r in enumerate(preqs):
@@ -455,24 +631,28 @@ def main():
         pok, pun, perr = C.c_int32(), C.c_int32(), C.c_char_p()
         # one untimed full-size call (it sizes the library's reusable device
         # workspace), then the median of timed calls
-        pts = []
+        pts, pks = [], []
         for rep in range(1 + max(3, min(args.steps, 5))):
             t0 = time.perf_counter()
             prc = lib.ga_align_sw_matrix_batch(parr, pp.n, C.byref(m62), pres, parena, pneed, C.byref(pok),
                                                C.byref(pun), C.byref(perr))
             if rep > 0:
                 pts.append(time.perf_counter() - t0)
+                pks.append(lib.ga_last_kernel_ms())
         pdt = float(np.median(pts))
-        t0 = time.perf_counter()
-        pb = ga.align_sw_matrix_batch(preqs, m62)
-        pdt_py = time.perf_counter() - t0
-        tab = O.matrix_table(m62.alphabet.decode(), [[m62.score[a][b] for b in range(20)] for a in range(20)], -1)
+        pkm = mean_pos(pks)
+        py_ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            pb = ga.align_sw_matrix_batch(preqs, m62)
+            py_ts.append(time.perf_counter() - t0)
+        pdt_py = float(np.median(py_ts))
         pbad = 0 if prc == N.GA_OK else pp.n
         raw = parena.raw
         for i in rng.choice(pp.n, size=20, replace=False):
             r = pres[i]
             cg = raw[r.cigar_off: r.cigar_off + r.cigar_len].decode("latin-1") if r.cigar_len > 0 else ""
-            e = O.sw_trace(pp.a_seq(i), pp.b_seq(i), tab, 11, 1)
+            e = O.sw_trace(pp.a_seq(i), pp.b_seq(i), tab62, 11, 1)
             if (r.score, r.start_i, r.start_j, r.end_i, r.end_j, cg) != e:
                 pbad += 1
             q = pb.results[i]
@@ -484,19 +664,26 @@ def main():
                         "ga_align_sw_matrix_batch (includes host packing, PCIe copies and CIGAR validation; "
                         "median of timed calls after one untimed full-size call)",
             "gcups": round(pcells / pdt / 1e9, 3), "ms": round(pdt * 1e3, 2), "cells": pcells,
+            "kernel_ms": round(pkm, 3) if pkm else None,
+            "kernel_gcups": round(pcells / (pkm * 1e-3) / 1e9, 1) if pkm else None,
+            # int32 kernel: 12 ops per cell against the int32 VALU peak
+            "roofline": {"bound": "valu", "unit": "TOP/s", "peak": PEAK_INT32_TOPS,
+                         "achieved": round(SW_OPS_PER_CELL * pcells / (pkm * 1e-3) / 1e12, 3) if pkm else None,
+                         "frac": round(SW_OPS_PER_CELL * pcells / (pkm * 1e-3) / 1e12 / PEAK_INT32_TOPS, 4)
+                         if pkm else None,
+                         "kernel": "sw_matrix_mw_kernel<true> (int32, 4-wave strip pipeline)"},
             "python_api_ms": round(pdt_py * 1e3, 2),
             "mismatches": pbad, "checked": 20,
         }
         log(f"protein: {protein_line}")
-        if not args.no_cpu_baseline:
+        if cpu_on:
             protein_line["cpu_baseline"] = cpu_baseline_sw(
-                pp, args.cpu_seconds, min(os.cpu_count() or 1, 16), O, tab=tab, gaps=(11, 1),
-                label="protein, BLOSUM62 11/1, score + span + CIGAR")
+                pp, threads, O, tab=tab62, gaps=(11, 1), label="protein, BLOSUM62 11/1, score + span + CIGAR")
             log(f"protein cpu baseline: {protein_line['cpu_baseline']}")
 
     # ---------------- FASTA ingest on the device ----------------
     fasta_line = None
-    if not args.no_fasta:
+    if "fasta" in only:
         from genoaligner_amd.fasta import fasta_to_device
         fa = W.fasta_genome(args.fasta_mib << 20, seed=11)
         fa_dev = torch.from_numpy(fa).cuda()
@@ -530,31 +717,24 @@ def main():
                          "algorithmic_bytes": moved},
         }
         del fa_dev, d
-        if not args.no_cpu_baseline:
-            sample = W.fasta_genome(64 << 20, seed=12)
-            path = os.path.join(tempfile.gettempdir(), "ga_bench_sample.fa")
-            sample.tofile(path)
-            use_ref = O.ref_available()
-            t0, k = time.perf_counter(), 0
-            while time.perf_counter() - t0 < min(args.cpu_seconds, 5.0) or k == 0:
-                if use_ref:
-                    O.ref_fasta_file(path)
-                else:
-                    O.fasta_parse(sample.tobytes())
-                k += 1
-            el = time.perf_counter() - t0
-            os.remove(path)
+        if cpu_on:
+            # the oracle's reader (a restatement of the reference's io::read_fasta
+            # in Python), one host thread, median of 5 after one warm-up
+            smp = W.fasta_genome(16 << 20, seed=12).tobytes()
+            O.fasta_parse(smp)
+            ts = sorted(_timed(lambda: O.fasta_parse(smp)) for _ in range(5))
             fasta_line["cpu_baseline"] = {
-                "value": round(k * len(sample) / el / 1e9, 3), "unit": "GB/s of FASTA input", "cores": 1,
-                "kind": "reference" if use_ref else "port",
-                "sample": f"{k} x io::read_fasta_file of a 64 MiB file of the same generator"
-                          if use_ref else f"{k} x the oracle's reader over a 64 MiB file of the same generator",
+                "value": round(len(smp) / ts[2] / 1e9, 4), "unit": "GB/s of FASTA input", "cores": 1,
+                "kind": "port", "min": round(len(smp) / ts[-1] / 1e9, 4), "max": round(len(smp) / ts[0] / 1e9, 4),
+                "sample": f"the oracle's reader over a 16 MiB file of the same generator, median of 5 "
+                          f"({ts[2]:.2f} s each) after one warm-up",
+                "host": host_info(),
             }
         log(f"fasta: {fasta_line}")
 
     # ---------------- gene QC task: one bait, many records ----------------
     qc_line = None
-    if not args.no_gene_qc:
+    if "gene_qc" in only:
         import io
         from genoaligner_amd import gene_qc as G
         seqs, refs = W.gene_family(args.gene_qc_records, codons=380, seed=21)
@@ -578,79 +758,40 @@ def main():
             "value": round(n_rec / (med * 1e-3), 1), "unit": "records/s", "ms_per_run": round(med, 2),
             "gcups": round(qc_cells / (med * 1e-3) / 1e9, 1),   # record length x bait length, whole run
         }
-        if os.path.exists(O.REF_GENE_QC):
-            ssub, _ = W.gene_family(args.gene_qc_cpu_records, codons=380, seed=21)
-            ssp = os.path.join(qdir, "sample.fa")
-            open(ssp, "wb").write(ssub)
-            # checked on the sample: the reference tool on the CPU takes ~13 ms a record
-            G.run(ssp, rp, code=2, emit=os.path.join(qdir, "got_s.tsv"), out=io.StringIO(), err=io.StringIO())
-            t0 = time.perf_counter()
-            rc, _, _, ref_tsv = O.ref_gene_qc(ssp, rp, ("--code", "2"), emit=os.path.join(qdir, "ref_s.tsv"))
-            el = time.perf_counter() - t0
-            qc_line["matches_reference_tool"] = bool(
-                rc == 0 and ref_tsv == open(os.path.join(qdir, "got_s.tsv"), encoding="latin-1").read())
-            qc_line["checked_records"] = args.gene_qc_cpu_records
-            if not args.no_cpu_baseline:
-                qc_line["cpu_baseline"] = {
-                    "value": round(args.gene_qc_cpu_records / el, 1), "unit": "records/s", "cores": 1,
-                    "kind": "reference",
-                    "sample": f"the reference gene_qc (tools/gene_qc.cpp on its CPU shim) over the first "
-                              f"{args.gene_qc_cpu_records} records of the same family",
-                }
+        # checked on a sample against the oracle's pipeline (pinned to the
+        # reference tool on the CPU), which is also the CPU baseline
+        nq = args.gene_qc_cpu_records
+        ssub, _ = W.gene_family(nq, codons=380, seed=21)
+        ssp = os.path.join(qdir, "sample.fa")
+        open(ssp, "wb").write(ssub)
+        G.run(ssp, rp, code=2, emit=os.path.join(qdir, "got_s.tsv"), out=io.StringIO(), err=io.StringIO())
+        want_tsv, _ = O.gene_qc_tsv(ssub, refs, code=2)
+        qc_line["matches_oracle"] = want_tsv == open(os.path.join(qdir, "got_s.tsv"), encoding="latin-1").read()
+        qc_line["checked_records"] = nq
+        if cpu_on:
+            ts = sorted(_timed(lambda: O.gene_qc_tsv(ssub, refs, code=2)) for _ in range(5))
+            qc_line["cpu_baseline"] = {
+                "value": round(nq / ts[2], 1), "unit": "records/s", "cores": 1, "kind": "port",
+                "min": round(nq / ts[-1], 1), "max": round(nq / ts[0], 1),
+                "sample": f"the oracle's gene QC pipeline over the first {nq} records of the same family, "
+                          f"median of 5 ({ts[2]:.2f} s each) after the checked run",
+                "host": host_info(),
+            }
         log(f"gene_qc: {qc_line}")
 
     # ---------------- serving: Poisson arrivals, continuous batching ----------------
     serve_line = None
-    if not args.no_serve:
-        import genoaligner_amd as ga
-        from genoaligner_amd.serve import Server, poisson_load
-        srng = np.random.default_rng(5)
-        sreqs = [W.serving_request(srng, ga) for _ in range(args.serve_requests)]
-        # warm-up: a dozen requests through a server of the same shape first
-        # (slot threads' streams, kernels and workspaces), not measured
-        warm = Server(budget_bytes=64 << 30, slots=4)
-        for t in poisson_load(warm, lambda k: sreqs[k], 2 * args.serve_rate, min(12, len(sreqs)), seed=1):
-            t.wait()
-        warm.close()
-        server = Server(budget_bytes=64 << 30, slots=4)
-        t0 = time.perf_counter()
-        tickets = poisson_load(server, lambda k: sreqs[k], args.serve_rate, len(sreqs), seed=9)
-        for t in tickets:
-            t.wait()
-        wall = time.perf_counter() - t0
-        server.close()
-        lat = [t.latency_ms for t in tickets]
-        errors = sum(1 for t in tickets if t.error)
-        by = {"wfa": [t for t in tickets if t.kind == "wfa"], "sw": [t for t in tickets if t.kind == "sw"]}
-        serve_line = {
-            "workload": f"{len(sreqs)} requests, Poisson {args.serve_rate}/s, 1..512 pairs each; 60% WFA DNA "
-                        "100 bp..5 kbp at 0..10% divergence, 40% protein SW (query 100..1000 aa, target "
-                        "100..2000 aa, BLOSUM62 11/1); 70% of requests with CIGAR; 64 GB budget, 4 slots; "
-                        "after 12 warm-up requests",
-            "p50_ms": pctl(lat, 50), "p99_ms": pctl(lat, 99),
-            "wfa_p50_ms": pctl([t.latency_ms for t in by["wfa"]], 50),
-            "sw_p50_ms": pctl([t.latency_ms for t in by["sw"]], 50),
-            "pairs": int(sum(t.pairs() for t in tickets)), "wall_s": round(wall, 3),
-            "pairs_per_s": round(sum(t.pairs() for t in tickets) / wall, 1),
-            "batches": len(server.batches), "peak_reserved_gb": round(server.peak_reserved / 2**30, 3),
-            "errors": errors,
-        }
-        log(f"serving: {serve_line}")
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        threads = min(os.cpu_count() or 1, 16)
-        cpu = cpu_baseline_sw(sw, args.cpu_seconds, threads, O)
-        log(f"cpu baseline: {cpu}")
+    if "serving" in only:
+        serve_line = serving_section(args, W, O, tab62, threads, cpu_on)
 
     line = {
         "metric": BASELINE_METRIC,
-        "value": round(gcups, 3),
+        "value": round(gcups, 3) if gcups else None,
         "unit": "GCUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(ms_per_step, 3) if ms_per_step else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -658,14 +799,15 @@ def main():
         "data": "synthetic (seeded)",
         "config": {"workload": "sw_embedded: 50000 pairs, query 256 bp, target 1024 bp with a "
                                "10%-mutated query copy (2:1:1), scoring 2/-3/3/1, score + span + CIGAR",
-                   "pairs": sw.n, "cells_per_step": cells, "device": dev_name},
-        "p50_ms": pctl(step_ms, 50),
-        "p99_ms": pctl(step_ms, 99),
+                   "pairs": sw.n if sw is not None else None, "cells_per_step": cells, "device": dev_name},
+        "p50_ms": pctl(step_ms, 50) if step_ms else None,
+        "p99_ms": pctl(step_ms, 99) if step_ms else None,
         "mismatches": sw_bad,
         "checked": int(len(sample)),
         "roofline": roof,
         "cpu_baseline": cpu,
         "sw_score_only": sw_score_line,
+        "sw_host_entry": sw_host_line,
         "wfa": wfa_line,
         "wfa_mixed": mixed_line,
         "wfa_routing": routing_line,
@@ -675,6 +817,133 @@ def main():
         "gene_qc": qc_line,
     }
     print(json.dumps(line), flush=True)
+
+
+def _timed(fn):
+    t0 = time.perf_counter()
+    fn()
+    return time.perf_counter() - t0
+
+
+def serving_section(args, W, O, tab62, threads, cpu_on):
+    """Poisson arrivals at a rising rate until p99 passes twice its lowest-load
+    value (the capacity knee), then every request at once (saturation), with
+    per-class work rates against the VALU peaks; the oracle on the same mix."""
+    import genoaligner_amd as ga
+    from genoaligner_amd.serve import Server, poisson_load
+    srng = np.random.default_rng(5)
+    sreqs = [W.serving_request(srng, ga) for _ in range(args.serve_requests)]
+    # warm-up: a dozen requests through a server of the same shape first
+    # (slot threads' streams, kernels and workspaces), not measured
+    warm = Server(budget_bytes=64 << 30, slots=4)
+    for t in poisson_load(warm, lambda k: sreqs[k], 2 * args.serve_rate, min(12, len(sreqs)), seed=1):
+        t.wait()
+    warm.close()
+
+    def run(rate):
+        server = Server(budget_bytes=64 << 30, slots=4)
+        t0 = time.perf_counter()
+        if rate is None:
+            tickets = [server.submit(*sreqs[k]) for k in range(len(sreqs))]
+        else:
+            tickets = poisson_load(server, lambda k: sreqs[k], rate, len(sreqs), seed=9)
+        for t in tickets:
+            t.wait()
+        wall = time.perf_counter() - t0
+        server.close()
+        lat = [t.latency_ms for t in tickets]
+        pairs = int(sum(t.pairs() for t in tickets))
+        return tickets, {"rate_per_s": rate, "p50_ms": pctl(lat, 50), "p99_ms": pctl(lat, 99),
+                         "pairs_per_s": round(pairs / wall, 1), "wall_s": round(wall, 3),
+                         "batches": len(server.batches), "errors": sum(1 for t in tickets if t.error),
+                         "peak_reserved_gb": round(server.peak_reserved / 2**30, 3)}
+
+    base_t, base = run(args.serve_rate)
+    sweep = [base]
+    knee = base
+    rate = args.serve_rate
+    while rate < 64 * args.serve_rate:
+        rate *= 1.5
+        _, r = run(rate)
+        sweep.append(r)
+        if r["p99_ms"] > 2 * base["p99_ms"]:
+            break
+        knee = r
+    flood_t, flood = run(None)
+
+    # per-class work of the mix: WFA diagonal updates ((w + 1)^2, w the score,
+    # or smax when unresolved within the length bound), SW cells
+    def work(tickets):
+        wfa_upd = wfa_pairs = sw_cells = sw_pairs = 0
+        for t in tickets:
+            if t.kind == "wfa":
+                for it, r in zip(t.items, t.results or []):
+                    m, n, s = int(it.pattern_len), int(it.text_len), int(it.smax)
+                    w = r.score if r.score >= 0 else (s if abs(m - n) <= s else -1)
+                    wfa_upd += (w + 1) ** 2
+                    wfa_pairs += 1
+            else:
+                for it in t.items:
+                    sw_cells += int(it.text_len) * int(it.pattern_len)
+                    sw_pairs += 1
+        return wfa_upd, wfa_pairs, sw_cells, sw_pairs
+
+    wfa_upd, wfa_pairs, sw_cells, sw_pairs = work(flood_t)
+    fw = flood["wall_s"]
+    per_class = {
+        "wfa": {"pairs": wfa_pairs, "pairs_per_s": round(wfa_pairs / fw, 1), "diagonal_updates": wfa_upd,
+                "roofline": {"bound": "valu", "unit": "TOP/s", "peak": PEAK_INT32_TOPS,
+                             "achieved": round(WFA_OPS_PER_UPDATE * wfa_upd / fw / 1e12, 4),
+                             "frac": round(WFA_OPS_PER_UPDATE * wfa_upd / fw / 1e12 / PEAK_INT32_TOPS, 5)}},
+        "sw": {"pairs": sw_pairs, "pairs_per_s": round(sw_pairs / fw, 1), "cells": sw_cells,
+               "gcups": round(sw_cells / fw / 1e9, 2),
+               "roofline": {"bound": "valu", "unit": "TOP/s", "peak": PEAK_INT32_TOPS,
+                            "achieved": round(SW_OPS_PER_CELL * sw_cells / fw / 1e12, 4),
+                            "frac": round(SW_OPS_PER_CELL * sw_cells / fw / 1e12 / PEAK_INT32_TOPS, 5)}},
+        "note": "work of each class over the saturation run's wall time (the classes share the GPU)",
+    }
+    by = {"wfa": [t for t in base_t if t.kind == "wfa"], "sw": [t for t in base_t if t.kind == "sw"]}
+    line = {
+        "workload": f"{len(sreqs)} requests, 1..512 pairs each; 60% WFA DNA 100 bp..5 kbp at 0..10% "
+                    "divergence, 40% protein SW (query 100..1000 aa, target 100..2000 aa, BLOSUM62 11/1); "
+                    "70% of requests with CIGAR; 64 GB budget, 4 slots; after 12 warm-up requests",
+        "p50_ms": base["p50_ms"], "p99_ms": base["p99_ms"], "rate_per_s": args.serve_rate,
+        "wfa_p50_ms": pctl([t.latency_ms for t in by["wfa"]], 50),
+        "sw_p50_ms": pctl([t.latency_ms for t in by["sw"]], 50),
+        "pairs_per_s_offered": base["pairs_per_s"],
+        "capacity": {"knee_rate_per_s": knee["rate_per_s"], "knee_p99_ms": knee["p99_ms"],
+                     "knee_pairs_per_s": knee["pairs_per_s"],
+                     "saturation_pairs_per_s": flood["pairs_per_s"], "saturation_p99_ms": flood["p99_ms"],
+                     "rule": "highest Poisson rate (x1.5 steps from the base) whose p99 stays within 2x the "
+                             "base p99; saturation = every request submitted at once"},
+        "sweep": sweep + [flood],
+        "per_class": per_class,
+        "errors": sum(r["errors"] for r in sweep) + flood["errors"],
+    }
+    if cpu_on:
+        # the oracle over the same request mix: each class timed on its own
+        # pairs, combined by the mix's pair counts
+        wfa_items = [it for k, items, cig in sreqs if k == "wfa" for it in items]
+        wfa_want = np.array([1 if cig else 0 for k, items, cig in sreqs if k == "wfa" for _ in items], np.uint8)
+        sw_items = [it for k, items, cig in sreqs if k == "sw" for it in items]
+        wps = wfa_requests_pairset(wfa_items)
+        sps = sw_requests_pairset(sw_items)
+        cw = cpu_baseline_wfa(wps, wfa_want, threads, O, "the serving mix's WFA pairs")
+        cs = cpu_baseline_sw(sps, threads, O, tab=tab62, gaps=(11, 1), label="the serving mix's protein pairs")
+        sw_cells_mix = float((sps.a_len.astype(np.int64) * sps.b_len).sum())
+        t_cpu = len(wfa_items) / cw["value"] + sw_cells_mix / (cs["value"] * 1e9)
+        t1 = len(wfa_items) / cw["single_core"]["value"] + sw_cells_mix / (cs["single_core"]["value"] * 1e9)
+        line["cpu_baseline"] = {
+            "value": round((len(wfa_items) + len(sw_items)) / t_cpu, 1), "unit": "pairs/s",
+            "cores": threads, "kind": "port",
+            "sample": f"the oracle on the mix's {len(wfa_items)} WFA pairs ({cw['value']:.0f} pairs/s) and "
+                      f"{len(sw_items)} protein pairs ({cs['value']:.3f} GCUPS), each a median of 5 after one "
+                      "warm-up, combined by the mix's counts",
+            "single_core": {"value": round((len(wfa_items) + len(sw_items)) / t1, 1)},
+            "host": cw["host"],
+        }
+    log(f"serving: {line}")
+    return line
 
 
 if __name__ == "__main__":

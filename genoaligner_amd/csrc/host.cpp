@@ -816,7 +816,8 @@ const char* run_wfa(const ga_align_req* reqs, int64_t n, const std::vector<int32
         L.cig_len = D.alloc<int32_t>(size_t(n));
     }
     if (!D.ok) err = "hipMalloc/hipMemcpy (WFA buffers)";
-    if (!err) err = enqueue_wfa(L, plan, any_trace, st, nullptr);
+    t_timer.reset();
+    if (!err) err = enqueue_wfa(L, plan, any_trace, st, &t_timer);
     out.score.assign(static_cast<size_t>(n), -1);
     out.cig_len.assign(static_cast<size_t>(n), 0);
     if (!err) {
@@ -1030,6 +1031,7 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
                    const SwMatrixHost* mat = nullptr)
 {
     std::call_once(g_pool_once, init_pool);
+    t_timer.reset();
     // Order on the device: pairs for the int32 kernel first (texts longer than
     // the packed kernel takes, and empty or skipped pairs), then the packed
     // kernel's pairs sorted by shape so each wave's two pairs are alike.
@@ -1151,7 +1153,7 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
                 L.n_total = int32_t(n);
                 L.tcode = d_tcode; L.pcode = d_pcode; L.sub = d_sub;
                 if (mat) { L.go = mat->gap_open; L.ge = mat->gap_extend; L.sub8 = mat->fits8(); }
-                if (ga::sw_launch(L, trace, st) != hipSuccess) err = "SW kernel launch";
+                if (ga::sw_launch(L, trace, st, &t_timer_hooks) != hipSuccess) err = "SW kernel launch";
                 a += len;
             }
         }
@@ -1209,7 +1211,7 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
             L.dirs = d_dirs16; L.dir_stride = trace ? ga::sw16_dir_words_per_wave(cm, cn) : 0;
             L.selbuf = d_sel;
             L.walk = d_walk; L.cigar = d_cig; L.cig_off = d_slot;
-            if (enqueue_sw16_chunk(L, trace, st, nullptr) != hipSuccess) err = "SW kernel launch";
+            if (enqueue_sw16_chunk(L, trace, st, &t_timer_hooks) != hipSuccess) err = "SW kernel launch";
         }
     }
 
@@ -1858,6 +1860,7 @@ int ga_wfa_routed_device(const ga_wfa_dev_batch* b, const float* params, float q
     const int64_t n = b->n_pairs;
     if (n <= 0) return GA_OK;
     std::call_once(g_pool_once, init_pool);
+    t_timer.reset();
     hipStream_t st = static_cast<hipStream_t>(stream);
     std::vector<void*> held;
     auto alloc = [&](size_t bytes) -> void* {
@@ -1914,7 +1917,11 @@ int ga_wfa_routed_device(const ga_wfa_dev_batch* b, const float* params, float q
         L.smax = pass == 0 ? sm1 : sm2;
         WfaDevClasses C;
         err = setup_wfa_dev_classes(L, n, b->smax_max, b->max_len, b->cigar != nullptr, C, alloc);
-        if (!err) err = enqueue_wfa_dev(C, st, side, ev);
+        if (!err) {
+            t_timer.begin(st);
+            err = enqueue_wfa_dev(C, st, side, ev);
+            t_timer.end(st);
+        }
         release(mark);                                    // stream-ordered: reused by the next pass
     }
     unsigned long long hs[4] = {0, 0, 0, 0};

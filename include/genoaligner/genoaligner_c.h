@@ -298,10 +298,12 @@ typedef struct ga_sw_dev_batch {
 int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error);
 
 /* ---- kernel timing (EXTENSION: measurement support) ------------------------
- * When enabled (per calling thread), ga_wfa_device and ga_sw_device record HIP
- * events on their stream around their dominant kernel (the WFA kernel; the SW
- * forward kernel) and ga_last_kernel_ms() returns the summed duration of those
- * launches in the calling thread's last device call, waiting for it to finish.
+ * When enabled (per calling thread), the batch entry points (ga_align_batch,
+ * _ext, ga_align_sw_batch, ga_align_sw_matrix_batch) and the device ones
+ * (ga_wfa_device, ga_wfa_routed_device, ga_sw_device) record HIP events on
+ * their stream around their dominant kernels (the WFA class launches; the SW
+ * forward kernels) and ga_last_kernel_ms() returns the summed duration of
+ * those launches in the calling thread's last such call, waiting for it.
  * Returns a negative value when nothing was recorded. */
 void  ga_set_kernel_timing(int enabled);
 float ga_last_kernel_ms(void);
