@@ -75,24 +75,32 @@ def pctl(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
 
 
-def time_steps(run, steps, warmup, lib):
+def time_steps(run, steps, warmup, lib, kernel_reps=5):
+    """W untimed steps, then exactly `steps` timed steps submitted back to back
+    (no host wait between them, as a caller streaming batches would submit
+    them) between two device synchronisations; per-step spans from events on
+    the launch stream.  Then `kernel_reps` more steps, each waited for, give
+    the dominant kernels' time from the library's own events on their stream."""
     import torch
     for _ in range(warmup):
         run()
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
           for _ in range(steps)]
-    kern = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for a, b in ev:
         a.record()
         run()
         b.record()
-        kern.append(lib.ga_last_kernel_ms())
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     step_ms = [a.elapsed_time(b) for a, b in ev]
+    kern = []
+    for _ in range(kernel_reps):
+        run()
+        kern.append(lib.ga_last_kernel_ms())
+    torch.cuda.synchronize()
     return wall, step_ms, kern
 
 
@@ -561,9 +569,9 @@ def main():
         routed = WfaDeviceBatch(wr, trace=True, want=rwant)
         log(f"wfa_routing workload ready ({time.perf_counter() - t0:.1f} s)")
         pwall, pstep, pkern = time_steps(plain.run, args.steps, args.warmup, lib)
-        stats = []
-        rwall, rstep, rkern = time_steps(lambda: stats.append(R.run_routed(routed, params, q=args.route_q)),
+        rwall, rstep, rkern = time_steps(lambda: R.run_routed(routed, params, q=args.route_q, stats=False),
                                          args.steps, args.warmup, lib)
+        stats = [R.run_routed(routed, params, q=args.route_q)]
         psc, pcg = plain.results()
         rsc, rcg = routed.results()
         diff = int(sum(1 for i in range(wr.n) if psc[i] != rsc[i] or pcg[i] != rcg[i]))

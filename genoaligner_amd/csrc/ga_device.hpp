@@ -45,6 +45,7 @@ struct WfaQueueLaunch {
     bool wide;
     bool staged;
     const int32_t* dev_class; // device-planned: {first, count} of this class (then n, order ignored / offset)
+    bool counter_zeroed;      // host-planned: `next` already zeroed on the stream (else the launcher zeroes it)
 };
 
 // Size classes by smax (device planning uses the same bounds as the host plan).
@@ -67,6 +68,8 @@ size_t wfa_lds_bytes(const WfaQueueLaunch& L);     // dynamic LDS of one workgro
 int wfa_queue_groups(const WfaQueueLaunch& L);     // resident workgroups (<= n)
 size_t wfa_queue_ws_bytes(int smax_cls, int n_groups, bool wide);
 hipError_t wfa_queue_launch(const WfaQueueLaunch& L, hipStream_t st);
+// p[0 .. n) = 0 by a kernel (graph-capturable, unlike a memset node)
+hipError_t zero_i32_launch(int32_t* p, int n, hipStream_t st);
 
 // ---------------------------------------------------------------- router
 constexpr int kSketchDim = 64;

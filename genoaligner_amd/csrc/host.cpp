@@ -534,15 +534,17 @@ int64_t wfa_stack_budget()
 }
 
 struct WfaPlan {
-    std::vector<int32_t> order;                 // pair ids, class by class
+    std::vector<int32_t> order;                 // pair ids, class by class (empty: identity)
     std::vector<int> bound;                     // per class: smax bound
     std::vector<int> max_plen, max_tlen;        // per class: longest sequences
     std::vector<int64_t> first, count;          // per class: slice of `order`
 };
 
 // Classes by smax; inside a class the longest pairs are claimed first (a
-// counting sort on (pattern_len + text_len) / 32, so planning stays O(n)).
-// `ids` (optional) restricts the plan to those pairs.
+// counting sort on (pattern_len + text_len) / 32, so planning stays O(n), over
+// the bins the batch occupies only).  `ids` (optional) restricts the plan to
+// those pairs.  A batch of one class whose pairs share one length bin keeps
+// its own order: `order` stays empty and the launch claims pairs 0 .. n-1.
 WfaPlan plan_wfa(int64_t n, const int32_t* smax, const int32_t* plen, const int32_t* tlen,
                  const std::vector<int32_t>* ids = nullptr)
 {
@@ -550,43 +552,58 @@ WfaPlan plan_wfa(int64_t n, const int32_t* smax, const int32_t* plen, const int3
     WfaPlan P;
     const int64_t cnt = ids ? int64_t(ids->size()) : n;
     auto pair_at = [&](int64_t x) { return ids ? int64_t((*ids)[size_t(x)]) : x; };
-    std::vector<int> cls(static_cast<size_t>(cnt));
-    std::vector<int64_t> hist(size_t(kWfaNumClasses) * kKeys, 0);
-    std::vector<int> cmax(kWfaNumClasses, 0), pmax(kWfaNumClasses, 0), tmax(kWfaNumClasses, 0);
-    std::vector<int64_t> ccount(kWfaNumClasses, 0);
-    auto key = [&](int64_t i) {
-        const int64_t l = (int64_t(std::max(plen[i], 0)) + std::max(tlen[i], 0)) >> 5;
-        return int(kKeys - 1 - std::min<int64_t>(l, kKeys - 1));      // longest first
-    };
+    std::vector<uint16_t> key(static_cast<size_t>(cnt));
+    std::vector<uint8_t> cls(static_cast<size_t>(cnt));
+    int cmax[kWfaNumClasses] = {0, 0, 0, 0}, pmax[kWfaNumClasses] = {0, 0, 0, 0}, tmax[kWfaNumClasses] = {0, 0, 0, 0};
+    int kmin[kWfaNumClasses] = {kKeys, kKeys, kKeys, kKeys}, kmax[kWfaNumClasses] = {-1, -1, -1, -1};
+    int64_t ccount[kWfaNumClasses] = {0, 0, 0, 0};
     for (int64_t x = 0; x < cnt; ++x) {
         const int64_t i = pair_at(x);
         int c = 0;
         while (c < kWfaNumClasses - 1 && smax[i] > kWfaClassBound[c]) ++c;
-        cls[size_t(x)] = c;
-        cmax[size_t(c)] = std::max(cmax[size_t(c)], int(smax[i]));
-        pmax[size_t(c)] = std::max(pmax[size_t(c)], int(plen[i]));
-        tmax[size_t(c)] = std::max(tmax[size_t(c)], int(tlen[i]));
-        ++ccount[size_t(c)];
-        ++hist[size_t(c) * kKeys + size_t(key(i))];
+        const int64_t l = (int64_t(std::max(plen[i], 0)) + std::max(tlen[i], 0)) >> 5;
+        const int k = int(kKeys - 1 - std::min<int64_t>(l, kKeys - 1));      // longest first
+        cls[size_t(x)] = uint8_t(c);
+        key[size_t(x)] = uint16_t(k);
+        cmax[c] = std::max(cmax[c], int(smax[i]));
+        pmax[c] = std::max(pmax[c], int(plen[i]));
+        tmax[c] = std::max(tmax[c], int(tlen[i]));
+        kmin[c] = std::min(kmin[c], k);
+        kmax[c] = std::max(kmax[c], k);
+        ++ccount[c];
     }
+    int used = 0;
+    bool one_bin = true;
+    std::vector<int64_t> hist[kWfaNumClasses];
     int64_t at = 0;
     for (int c = 0; c < kWfaNumClasses; ++c) {
-        if (ccount[size_t(c)] == 0) continue;
-        P.bound.push_back(std::max(cmax[size_t(c)], 0));
-        P.max_plen.push_back(pmax[size_t(c)]);
-        P.max_tlen.push_back(tmax[size_t(c)]);
+        if (ccount[c] == 0) continue;
+        ++used;
+        one_bin = one_bin && kmin[c] == kmax[c];
+        P.bound.push_back(std::max(cmax[c], 0));
+        P.max_plen.push_back(pmax[c]);
+        P.max_tlen.push_back(tmax[c]);
         P.first.push_back(at);
-        P.count.push_back(ccount[size_t(c)]);
-        for (int k = 0; k < kKeys; ++k) {                       // exclusive scan -> slots
-            const int64_t h = hist[size_t(c) * kKeys + size_t(k)];
-            hist[size_t(c) * kKeys + size_t(k)] = at;
-            at += h;
-        }
+        P.count.push_back(ccount[c]);
+        hist[c].assign(size_t(kmax[c] - kmin[c] + 1), 0);
+        at += ccount[c];
     }
+    if (used == 1 && one_bin && !ids) return P;             // identity order
+    for (int64_t x = 0; x < cnt; ++x) {
+        const int c = cls[size_t(x)];
+        ++hist[c][size_t(key[size_t(x)] - kmin[c])];
+    }
+    at = 0;
+    for (int c = 0; c < kWfaNumClasses; ++c)                // exclusive scan -> slots
+        for (auto& h : hist[c]) {
+            const int64_t v = h;
+            h = at;
+            at += v;
+        }
     P.order.resize(static_cast<size_t>(cnt));
     for (int64_t x = 0; x < cnt; ++x) {
-        const int64_t i = pair_at(x);
-        P.order[size_t(hist[size_t(cls[size_t(x)]) * kKeys + size_t(key(i))]++)] = int32_t(i);
+        const int c = cls[size_t(x)];
+        P.order[size_t(hist[c][size_t(key[size_t(x)] - kmin[c])]++)] = int32_t(pair_at(x));
     }
     return P;
 }
@@ -629,12 +646,15 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
         if (d_order) (void)ws_free(d_order, st);
         if (d_next) (void)ws_free(d_next, st);
     };
-    if (ws_alloc(reinterpret_cast<void**>(&d_order), plan.order.size() * 4 + 64, st) != hipSuccess ||
+    const bool identity = plan.order.empty();
+    if ((!identity && ws_alloc(reinterpret_cast<void**>(&d_order), plan.order.size() * 4 + 64, st) != hipSuccess) ||
         ws_alloc(reinterpret_cast<void**>(&d_next), nc * 4 + 64, st) != hipSuccess) {
         drop_queue();
         return "hipMalloc(WFA queue)";
     }
-    if (hipMemcpyAsync(d_order, plan.order.data(), plan.order.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess) {
+    if ((!identity && hipMemcpyAsync(d_order, plan.order.data(), plan.order.size() * 4, hipMemcpyHostToDevice,
+                                     st) != hipSuccess) ||
+        ga::zero_i32_launch(d_next, int(nc), st) != hipSuccess) {
         drop_queue();
         return "hipMemcpy(WFA queue)";
     }
@@ -644,9 +664,10 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
     for (size_t c = 0; c < nc && !err; ++c) {
         ga::WfaQueueLaunch& L = Ls[c];
         L = base;
-        L.order = d_order + plan.first[c];
+        L.order = identity ? nullptr : d_order + plan.first[c];
         L.n = int32_t(plan.count[c]);
         L.next = d_next + c;
+        L.counter_zeroed = true;                       // zeroed above, before the timed region
         L.smax_cls = plan.bound[c];
         L.max_plen = plan.max_plen[c];
         L.max_tlen = plan.max_tlen[c];
@@ -2000,7 +2021,7 @@ int ga_wfa_autotune(const ga_wfa_dev_batch* b, int reps, ga_wfa_tuning* out, voi
         int best_w = t.waves[cls];
         for (int w : {1, 2, 4, 8}) {
             ga::WfaQueueLaunch L = base;
-            L.order = d_order + plan.first[c];
+            L.order = plan.order.empty() ? nullptr : d_order + plan.first[c];
             L.n = int32_t(plan.count[c]);
             L.next = d_next;
             L.smax_cls = plan.bound[c];

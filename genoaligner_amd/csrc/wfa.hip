@@ -655,12 +655,20 @@ hipError_t wfa_queue_launch(const WfaQueueLaunch& L, hipStream_t st)
     // the claim counter is zeroed by a kernel, not a memset node, so this
     // submission can be captured into a graph and replayed (device-planned
     // classes: the plan kernels zero it)
-    if (!L.dev_class) {
+    if (!L.dev_class && !L.counter_zeroed) {
         hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, st, L.next, 1);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return dispatch(L, wfa_lds_bytes(L), st, nullptr);
+}
+
+hipError_t zero_i32_launch(int32_t* p, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(unsigned((n + 255) / 256 < 64 ? (n + 255) / 256 : 64)), dim3(256), 0,
+                       st, p, n);
+    return hipGetLastError();
 }
 
 hipError_t wfa_plan_launch(const WfaDevPlan& P, hipStream_t st)

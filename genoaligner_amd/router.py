@@ -177,15 +177,18 @@ class RouteStats:
         return self.rerun / self.routed if self.routed else 0.0
 
 
-def run_routed(batch: WfaDeviceBatch, params: torch.Tensor, q: float = 0.9, stream=None) -> RouteStats:
+def run_routed(batch: WfaDeviceBatch, params: torch.Tensor, q: float = 0.9, stream=None,
+               stats: bool = True) -> Optional[RouteStats]:
     """ga_wfa_routed_device over a device batch (results in batch.score /
-    batch.cigar as for batch.run())."""
+    batch.cigar as for batch.run()).  With stats the call waits for the stream
+    to report them; without, it only enqueues and returns None."""
     st = N.RouteStats()
     err = C.c_char_p()
-    _check(N.lib().ga_wfa_routed_device(C.byref(batch.b), _p(params), C.c_float(q), C.byref(st),
+    _check(N.lib().ga_wfa_routed_device(C.byref(batch.b), _p(params), C.c_float(q),
+                                        C.byref(st) if stats else None,
                                         C.c_void_p(_stream_handle(stream)), C.byref(err)),
            err, "ga_wfa_routed_device")
-    return RouteStats(st.shortcut, st.routed, st.plain, st.rerun)
+    return RouteStats(st.shortcut, st.routed, st.plain, st.rerun) if stats else None
 
 
 def save(params: np.ndarray, path: str = WEIGHTS):

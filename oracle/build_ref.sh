@@ -32,3 +32,12 @@ g++ -O2 -std=c++17 -pthread -ffp-contract=off \
     -o "$OUT/gene_qc.tmp"
 mv "$OUT/gene_qc.tmp" "$OUT/gene_qc"
 echo "build_ref: wrote $OUT/gene_qc"
+# the reference's MSA building blocks (src/msa/msa_ref.cpp) and its
+# profile-profile kernel body over the same shim
+g++ -O2 -std=c++17 -pthread -fPIC -shared -ffp-contract=off \
+    -DGENOALIGNER_HIP_SHIM \
+    -I "$REF/tests/parity/hip_cpu_shim" -I "$REF" -I "$REF/include" \
+    "$HERE/ref/ref_msa_driver.cpp" "$REF/src/msa/msa_ref.cpp" \
+    -o "$OUT/libga_ref_msa.so.tmp"
+mv "$OUT/libga_ref_msa.so.tmp" "$OUT/libga_ref_msa.so"
+echo "build_ref: wrote $OUT/libga_ref_msa.so"

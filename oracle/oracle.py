@@ -731,3 +731,230 @@ def gene_qc_tsv(seqs: bytes, refs: bytes, code: int = 1, scoring=(1, -1, 2, 1), 
     summary = "summary: %d seqs | OK=%d NO_ALIGN=%d LOW_COV=%d FS=%d STOP=%d | invalid=%d" % (
         len(recs), cnt["ok"], cnt["na"], cnt["lc"], cnt["fs"], cnt["st"], cnt["bad"])
     return "\n".join(lines) + "\n", summary
+
+
+# ---------------------------------------------------------------------------
+# MSA building blocks (src/msa/msa_ref.cpp): profiles, the profile-profile
+# Gotoh, k-mer distances, neighbour joining
+# ---------------------------------------------------------------------------
+REF_MSA_SO = os.path.join(HERE, "_ref", "libga_ref_msa.so")
+
+# msa_ref.cpp:117-138 (rows and columns in ARNDCQEGHILKMFPSTWYV order)
+MSA_BLOSUM62 = np.array([
+    [4, -1, -2, -2, 0, -1, -1, 0, -2, -1, -1, -1, -1, -2, -1, 1, 0, -3, -2, 0],
+    [-1, 5, 0, -2, -3, 1, 0, -2, 0, -3, -2, 2, -1, -3, -2, -1, -1, -3, -2, -3],
+    [-2, 0, 6, 1, -3, 0, 0, 0, 1, -3, -3, 0, -2, -3, -2, 1, 0, -4, -2, -3],
+    [-2, -2, 1, 6, -3, 0, 2, -1, -1, -3, -4, -1, -3, -3, -1, 0, -1, -4, -3, -3],
+    [0, -3, -3, -3, 9, -3, -4, -3, -3, -1, -1, -3, -1, -2, -3, -1, -1, -2, -2, -1],
+    [-1, 1, 0, 0, -3, 5, 2, -2, 0, -3, -2, 1, 0, -3, -1, 0, -1, -2, -1, -2],
+    [-1, 0, 0, 2, -4, 2, 5, -2, 0, -3, -3, 1, -2, -3, -1, 0, -1, -3, -2, -2],
+    [0, -2, 0, -1, -3, -2, -2, 6, -2, -4, -4, -2, -3, -3, -2, 0, -2, -2, -3, -3],
+    [-2, 0, 1, -1, -3, 0, 0, -2, 8, -3, -3, -1, -2, -1, -2, -1, -2, -2, 2, -3],
+    [-1, -3, -3, -3, -1, -3, -3, -4, -3, 4, 2, -3, 1, 0, -3, -2, -1, -3, -1, 3],
+    [-1, -2, -3, -4, -1, -2, -3, -4, -3, 2, 4, -2, 2, 0, -3, -2, -1, -2, -1, 1],
+    [-1, 2, 0, -1, -3, 1, 1, -2, -1, -3, -2, 5, -1, -3, -1, 0, -1, -3, -2, -2],
+    [-1, -1, -2, -3, -1, 0, -2, -3, -1, 1, 2, -1, 5, 0, -2, -1, -1, -1, -1, 1],
+    [-2, -3, -3, -3, -2, -3, -3, -3, -1, 0, 0, -3, 0, 6, -4, -2, -2, 1, 3, -1],
+    [-1, -2, -2, -1, -3, -1, -1, -2, -2, -3, -3, -1, -2, -4, 7, -1, -1, -4, -3, -2],
+    [1, -1, 1, 0, -1, 0, 0, 0, -1, -2, -2, 0, -1, -2, -1, 4, 1, -3, -2, -2],
+    [0, -1, 0, -1, -1, -1, -1, -2, -2, -1, -1, -1, -1, -2, -1, 1, 5, -2, -2, 0],
+    [-3, -3, -4, -4, -2, -2, -3, -2, -2, -3, -2, -3, -1, 1, -4, -3, -2, 11, 2, -3],
+    [-2, -2, -2, -3, -2, -1, -2, -3, 2, -1, -1, -2, -1, 3, -3, -2, -2, 2, 7, -1],
+    [0, -3, -3, -3, -1, -2, -2, -3, -3, 3, 1, -2, 1, -1, -2, -2, 0, -3, -1, 4],
+], dtype=np.float32)
+
+
+class MsaParamsC(C.Structure):
+    _fields_ = [("match", C.c_float), ("ts", C.c_float), ("tv", C.c_float),
+                ("gap_open", C.c_float), ("gap_extend", C.c_float), ("free_end_gaps", C.c_int),
+                ("psgp", C.c_int), ("psgp_scale", C.c_float), ("psgp_min_open", C.c_float),
+                ("psgp_min_ext", C.c_float), ("alpha", C.c_int), ("sub", C.c_void_p)]
+
+
+@dataclass
+class MsaParams:
+    """genomsa::Params fields of the profile DP (msa.hpp:36-60 defaults);
+    protein() is protein_params() (msa_ref.cpp:140-151)."""
+    match: float = 2.0
+    ts: float = -1.0
+    tv: float = -2.0
+    gap_open: float = 3.0
+    gap_extend: float = 1.0
+    free_end_gaps: bool = True
+    psgp: bool = True
+    psgp_scale: float = 0.5
+    psgp_min_open: float = 0.1
+    psgp_min_ext: float = 0.2
+    alpha: int = 4
+    sub: Optional[np.ndarray] = None
+
+    @staticmethod
+    def protein() -> "MsaParams":
+        return MsaParams(gap_open=11.0, gap_extend=1.0, free_end_gaps=False, alpha=20,
+                         sub=MSA_BLOSUM62.copy())
+
+    def c(self):
+        sub = None if self.sub is None else np.ascontiguousarray(self.sub, dtype=np.float32)
+        p = MsaParamsC(self.match, self.ts, self.tv, self.gap_open, self.gap_extend, int(self.free_end_gaps),
+                       int(self.psgp), self.psgp_scale, self.psgp_min_open, self.psgp_min_ext, self.alpha,
+                       sub.ctypes.data if sub is not None else None)
+        p._keep = sub
+        return p
+
+
+def _msa_lib():
+    L = lib()
+    if not getattr(L, "_msa_ready", False):
+        vp = C.c_void_p
+        L.ga_oracle_msa_profile.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, vp, vp]
+        L.ga_oracle_msa_align.argtypes = [vp, vp, C.c_int, vp, vp, C.c_int, C.POINTER(MsaParamsC), vp, vp, vp, vp]
+        L.ga_oracle_msa_align_batch.argtypes = [vp, vp, vp, vp, vp, vp, C.c_int64, C.POINTER(MsaParamsC),
+                                                vp, vp, vp, vp, vp, C.c_int]
+        L.ga_oracle_kmer_distances.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp]
+        L.ga_oracle_nj_tree.argtypes = [vp, C.c_int, vp]
+        L._msa_ready = True
+    return L
+
+
+def msa_profile(rows: Sequence, alpha: int = 4):
+    """(cols float32 [L, alpha], occ float32 [L]) of aligned rows."""
+    rows = [_b(r) for r in rows]
+    L = len(rows[0]) if rows else 0
+    assert all(len(r) == L for r in rows)
+    cols = np.zeros((L, alpha), dtype=np.float32)
+    occ = np.zeros(L, dtype=np.float32)
+    rc = _msa_lib().ga_oracle_msa_profile(b"".join(rows), len(rows), L, alpha, cols.ctypes.data, occ.ctypes.data)
+    assert rc == 0
+    return cols, occ
+
+
+def msa_align(A, B, params: MsaParams):
+    """align_profiles: (score, (ai, aj, bi, bj), cigar) of profiles A = (cols,
+    occ) and B."""
+    ca, oa = (np.ascontiguousarray(x, dtype=np.float32) for x in A)
+    cb, ob = (np.ascontiguousarray(x, dtype=np.float32) for x in B)
+    M, N = len(oa), len(ob)
+    score = np.zeros(1, dtype=np.float32)
+    span = np.zeros(4, dtype=np.int32)
+    cig = C.create_string_buffer(M + N + 1)
+    clen = np.zeros(1, dtype=np.int32)
+    p = params.c()
+    rc = _msa_lib().ga_oracle_msa_align(ca.ctypes.data, oa.ctypes.data, M, cb.ctypes.data, ob.ctypes.data, N,
+                                        C.byref(p), score.ctypes.data, span.ctypes.data, cig, clen.ctypes.data)
+    assert rc == 0
+    return float(score[0]), tuple(int(x) for x in span), cig.raw[: int(clen[0])].decode()
+
+
+def msa_merge_rows(rowsA: Sequence, rowsB: Sequence, span, cigar: str):
+    """The rows of merge_profiles (msa_ref.cpp:1177-1220): A's and B's rows
+    interleaved by the alignment, prefix / suffix columns as gap blocks."""
+    ai, aj, bi, bj = span
+    la, lb = len(rowsA[0]), len(rowsB[0])
+    w = ai + aj + len(cigar) + (la - bi) + (lb - bj)
+    out = []
+    for row in rowsA:
+        s, ca = [row[:ai], "-" * aj], ai
+        for op in cigar:
+            if op in "MI":
+                s.append(row[ca]); ca += 1
+            else:
+                s.append("-")
+        s += [row[ca:], "-" * (lb - bj)]
+        out.append("".join(s).ljust(w, "-"))
+    for row in rowsB:
+        s, cb = ["-" * ai, row[:aj]], aj
+        for op in cigar:
+            if op in "MD":
+                s.append(row[cb]); cb += 1
+            else:
+                s.append("-")
+        s += ["-" * (la - bi), row[cb:]]
+        out.append("".join(s).ljust(w, "-"))
+    return out
+
+
+def kmer_distances(seqs: Sequence, k: int, alpha: int = 4) -> np.ndarray:
+    """Packed upper-triangle k-mer distances (kmer_distances)."""
+    bs = [_b(s) for s in seqs]
+    lens = np.array([len(s) for s in bs], dtype=np.int32)
+    off = np.zeros(len(bs), dtype=np.int64)
+    off[1:] = np.cumsum(lens.astype(np.int64))[:-1] if len(bs) > 1 else off[1:]
+    buf = np.frombuffer(b"".join(bs) + b"\0", dtype=np.uint8)
+    n = len(bs)
+    D = np.zeros(max(n * (n - 1) // 2, 1), dtype=np.float32)
+    rc = _msa_lib().ga_oracle_kmer_distances(buf.ctypes.data, off.ctypes.data, lens.ctypes.data, n, k, alpha,
+                                             D.ctypes.data)
+    assert rc == 0
+    return D[: n * (n - 1) // 2]
+
+
+def nj_tree(D: np.ndarray, n: int) -> np.ndarray:
+    """nj_tree: int32 [n-1, 2] children of internal nodes n .. 2n-2."""
+    D = np.ascontiguousarray(D, dtype=np.float32)
+    ch = np.zeros(2 * (n - 1), dtype=np.int32)
+    rc = _msa_lib().ga_oracle_nj_tree(D.ctypes.data, n, ch.ctypes.data)
+    assert rc == 0
+    return ch.reshape(n - 1, 2)
+
+
+_ref_msa = None
+
+
+def ref_msa():
+    """The reference's MSA code compiled for the CPU (None when not built)."""
+    global _ref_msa
+    if _ref_msa is None and os.path.exists(REF_MSA_SO):
+        R = C.CDLL(REF_MSA_SO)
+        vp, cp = C.c_void_p, C.c_char_p
+        R.ga_ref_msa_profile.argtypes = [cp, C.c_int, C.c_int, C.c_int, vp, vp]
+        for f in (R.ga_ref_msa_align, R.ga_ref_msa_pp_kernel):
+            f.argtypes = [cp, C.c_int, C.c_int, cp, C.c_int, C.c_int, C.POINTER(MsaParamsC), vp, vp, vp, vp]
+        R.ga_ref_kmer_distances.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp]
+        R.ga_ref_nj_tree.argtypes = [vp, C.c_int, vp]
+        _ref_msa = R
+    return _ref_msa
+
+
+def ref_msa_profile(rows: Sequence, alpha: int = 4):
+    rows = [_b(r) for r in rows]
+    L = len(rows[0])
+    cols = np.zeros((L, alpha), dtype=np.float32)
+    occ = np.zeros(L, dtype=np.float32)
+    ref_msa().ga_ref_msa_profile(b"".join(rows), len(rows), L, alpha, cols.ctypes.data, occ.ctypes.data)
+    return cols, occ
+
+
+def ref_msa_align(rowsA: Sequence, rowsB: Sequence, params: MsaParams, kernel: bool = False):
+    """align_profiles (or, with kernel, the shipped msa_pp_trace_kernel body)
+    on the profiles of two row sets."""
+    ra, rb = [_b(r) for r in rowsA], [_b(r) for r in rowsB]
+    M, N = len(ra[0]), len(rb[0])
+    score = np.zeros(1, dtype=np.float32)
+    span = np.zeros(4, dtype=np.int32)
+    cig = C.create_string_buffer(M + N + 8)
+    clen = np.zeros(1, dtype=np.int32)
+    p = params.c()
+    f = ref_msa().ga_ref_msa_pp_kernel if kernel else ref_msa().ga_ref_msa_align
+    f(b"".join(ra), len(ra), M, b"".join(rb), len(rb), N, C.byref(p), score.ctypes.data, span.ctypes.data, cig,
+      clen.ctypes.data)
+    return float(score[0]), tuple(int(x) for x in span), cig.raw[: int(clen[0])].decode()
+
+
+def ref_kmer_distances(seqs: Sequence, k: int, alpha: int = 4) -> np.ndarray:
+    bs = [_b(s) for s in seqs]
+    lens = np.array([len(s) for s in bs], dtype=np.int32)
+    off = np.zeros(len(bs), dtype=np.int64)
+    if len(bs) > 1:
+        off[1:] = np.cumsum(lens.astype(np.int64))[:-1]
+    buf = np.frombuffer(b"".join(bs) + b"\0", dtype=np.uint8)
+    n = len(bs)
+    D = np.zeros(max(n * (n - 1) // 2, 1), dtype=np.float32)
+    ref_msa().ga_ref_kmer_distances(buf.ctypes.data, off.ctypes.data, lens.ctypes.data, n, k, alpha, D.ctypes.data)
+    return D[: n * (n - 1) // 2]
+
+
+def ref_nj_tree(D: np.ndarray, n: int) -> np.ndarray:
+    D = np.ascontiguousarray(D, dtype=np.float32)
+    ch = np.zeros(2 * (n - 1), dtype=np.int32)
+    ref_msa().ga_ref_nj_tree(D.ctypes.data, n, ch.ctypes.data)
+    return ch.reshape(n - 1, 2)

@@ -186,10 +186,154 @@ def write_sw_score():
     return path
 
 
+# ---------------------------------------------------------------- MSA
+MSA_PARAMS = {
+    "default": O.MsaParams(),
+    "legacy": O.MsaParams(psgp=False),
+    "global": O.MsaParams(free_end_gaps=False),
+    "protein": O.MsaParams.protein(),
+    "protein_semi": O.MsaParams(gap_open=11.0, gap_extend=1.0, free_end_gaps=True, alpha=20,
+                                sub=O.MSA_BLOSUM62.copy()),
+}
+AA = "ARNDCQEGHILKMFPSTWYV"
+
+
+def msa_mutate(rng, s, ps, pi, pd, alpha=ALPHA):
+    """substitutions + small indels (percent), in the style of
+    tests/parity/msa_pp_parity.cpp:141-153"""
+    out = []
+    for c in s:
+        r = rng.randrange(100)
+        if r < pd:
+            continue
+        out.append(rng.choice(alpha) if r < pd + ps else c)
+        if rng.randrange(100) < pi:
+            out.append(rng.choice(alpha))
+    return "".join(out)
+
+
+def msa_merged(rng, tag, rows_list):
+    """Progressively merge single-row profiles with the reference's alignment
+    (rows of merge_profiles), giving fractional columns and gap columns."""
+    rows = [rows_list[0]]
+    for r in rows_list[1:]:
+        sc, span, cg = O.ref_msa_align(rows, [r], MSA_PARAMS[tag])
+        rows = O.msa_merge_rows(rows, [r], span, cg)
+    return rows
+
+
+def msa_cases():
+    """(params tag, rows A, rows B): the case families of the reference's
+    tests/parity/msa_pp_parity.cpp:155-237 (hand pairs, merged profiles,
+    global mode, random sweeps of merged and single-sequence profiles,
+    protein), plus longer profiles (several 64-column strips) and many-row
+    profiles."""
+    rng = random.Random(4242)
+    out = []
+    hand = [("ACGTACGT", "ACGTACGT"), ("ACGTACGT", "ACGTTTTACGT"),
+            ("ACGTACGTACGTACGTACGT", "GTACGTAC"), ("ANRYSWKB", "ACGTACGT"), ("A", "T"),
+            ("ACGT", "A"), ("acgtNNNN", "ACGTACGT"), ("ACGTACGT", "TTTT")]
+    for tag in ("default", "legacy", "global"):
+        for a, b in hand:
+            out.append((tag, [a], [b]))
+        ab = msa_merged(rng, tag, ["ACGTACGT", "ACGTTCGT"])
+        out.append((tag, ab, ["ACGTACGTAA"]))
+        out.append((tag, ab, ["TTGCGCGC"]))
+        for t in range(40):
+            anc = rand_seq(rng, 20 + rng.randrange(60))
+            A = msa_merged(rng, tag, [msa_mutate(rng, anc, 10, 5, 5), msa_mutate(rng, anc, 10, 5, 5)])
+            anc2 = rand_seq(rng, 20 + rng.randrange(60))
+            B = msa_merged(rng, tag, [msa_mutate(rng, anc2, 10, 5, 5), msa_mutate(rng, anc2, 10, 5, 5)])
+            out.append((tag, A, B))
+        for t in range(30):
+            out.append((tag, [rand_seq(rng, 5 + rng.randrange(60))], [rand_seq(rng, 5 + rng.randrange(60))]))
+        for t in range(12):                                   # several strips, related
+            anc = rand_seq(rng, 100 + rng.randrange(400))
+            k = 1 + rng.randrange(5)
+            A = msa_merged(rng, tag, [msa_mutate(rng, anc, 8, 3, 3) for _ in range(k)])
+            B = msa_merged(rng, tag, [msa_mutate(rng, anc, 8, 3, 3) for _ in range(1 + rng.randrange(4))])
+            if rng.random() < 0.3:                            # a fragment
+                lo = rng.randrange(len(B[0]) // 2)
+                B = [r[lo: lo + len(B[0]) // 3] for r in B]
+            out.append((tag, A, B))
+    for tag in ("protein", "protein_semi"):
+        out.append((tag, ["MVLSPADKTNVKAAWGKVGAHAGEYG"], ["VHWTAEEKQLITGLWGKVNVAECGA"]))
+        out.append((tag, ["AXBZJUO"], ["ACDEFGH"]))
+        for t in range(40):
+            out.append((tag, [rand_seq(rng, 5 + rng.randrange(60), AA)],
+                        [rand_seq(rng, 5 + rng.randrange(60), AA)]))
+        a = msa_merged(rng, tag, [rand_seq(rng, 30, AA), rand_seq(rng, 30, AA)])
+        out.append((tag, a, [rand_seq(rng, 35, AA)]))
+        for t in range(8):
+            anc = rand_seq(rng, 80 + rng.randrange(300), AA)
+            A = msa_merged(rng, tag, [msa_mutate(rng, anc, 20, 3, 3, AA) for _ in range(1 + rng.randrange(4))])
+            B = msa_merged(rng, tag, [msa_mutate(rng, anc, 20, 3, 3, AA) for _ in range(1 + rng.randrange(3))])
+            out.append((tag, A, B))
+    return out
+
+
+def f32hex(x) -> str:
+    import struct
+    return "%08x" % struct.unpack("<I", struct.pack("<f", float(x)))[0]
+
+
+def write_msa_pp():
+    path = os.path.join(HERE, "msa_pp.tsv")
+    with open(path, "w") as f:
+        f.write("# reference genomsa::align_profiles (src/msa/msa_ref.cpp) via its CPU shim; the shipped\n"
+                "# msa_pp_trace_kernel body agrees on every row (checked when written)\n"
+                "#params\trows_a\trows_b\tscore_f32hex\tai\taj\tbi\tbj\tcigar\n")
+        for tag, A, B in msa_cases():
+            P = MSA_PARAMS[tag]
+            sc, span, cg = O.ref_msa_align(A, B, P)
+            assert O.ref_msa_align(A, B, P, kernel=True) == (sc, span, cg), (tag, A, B)
+            f.write("\t".join([tag, ",".join(A), ",".join(B), f32hex(sc), *map(str, span), esc(cg)]) + "\n")
+    return path
+
+
+def kmer_nj_cases():
+    """(k, alpha, sequences): related families (so NJ has structure), random
+    sets, IUPAC / ambiguous letters that break k-mers, duplicate sequences
+    (distance 0, NJ ties) and short sequences with no k-mer at all."""
+    rng = random.Random(20261129)
+    out = []
+    for t in range(10):
+        anc = rand_seq(rng, 200 + rng.randrange(300))
+        fam = [msa_mutate(rng, anc, 8, 2, 2) for _ in range(4 + rng.randrange(20))]
+        out.append((5, 4, fam))
+    for t in range(6):
+        n = 3 + rng.randrange(30)
+        out.append((rng.choice([3, 4, 5, 6, 8]), 4, [rand_seq(rng, 10 + rng.randrange(120)) for _ in range(n)]))
+    seqs = [rand_seq(rng, 60) for _ in range(5)]
+    seqs += [seqs[0], seqs[2], "ACGNNACGTRYACGT", "AC", "", "acgtacgtacgt", "ACGTACGTACGT"]
+    out.append((5, 4, seqs))
+    out.append((3, 4, seqs))
+    for t in range(6):
+        anc = rand_seq(rng, 100 + rng.randrange(200), AA)
+        fam = [msa_mutate(rng, anc, 15, 2, 2, AA) for _ in range(4 + rng.randrange(16))]
+        fam.append("BZJXUO" + fam[0][:20])
+        out.append((2, 20, fam))
+    return out
+
+
+def write_kmer_nj():
+    path = os.path.join(HERE, "kmer_nj.tsv")
+    with open(path, "w") as f:
+        f.write("# reference genomsa::kmer_distances and genomsa::nj_tree (src/msa/msa_ref.cpp) via its CPU shim\n"
+                "#k\talpha\tsequences\tdistances_f32hex\tchildren (internal nodes n..2n-2)\n")
+        for k, alpha, seqs in kmer_nj_cases():
+            D = O.ref_kmer_distances(seqs, k, alpha)
+            ch = O.ref_nj_tree(D, len(seqs)) if len(seqs) >= 2 else []
+            f.write("\t".join([str(k), str(alpha), ",".join(esc(s) for s in seqs),
+                               ",".join(f32hex(x) for x in D),
+                               ",".join(f"{a}:{b}" for a, b in ch)]) + "\n")
+    return path
+
+
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/libga_ref.so missing: run oracle/build_ref.sh first")
-    for fn in (write_wfa, write_sw_trace, write_sw_score):
+    for fn in (write_wfa, write_sw_trace, write_sw_score, write_msa_pp, write_kmer_nj):
         print("wrote", fn())
 
 

@@ -573,3 +573,26 @@ def test_workspace_blocks_reused_across_streams(gpu):
         assert [int(x) for x in sc] == [int(x) for x in rs]
         assert cg == rc
     N.lib().ga_trim_workspace()
+
+
+def test_wfa_one_class_one_length_bin(gpu, oracle):
+    """A batch of one size class whose pairs all fall in one length bin keeps
+    its own order (no claim-order upload): 150 bp pairs at smax 64 through
+    the host entry point and the device entry point with host-planned
+    classes, against the oracle; plus the same pairs with one long pair
+    appended (a second bin, so a sorted claim order)."""
+    import genoaligner_amd as ga
+    from genoaligner_amd import workloads as W
+    from genoaligner_amd.device import WfaDeviceBatch
+    ps = W.wfa_plain(n=3000, length=150, rate=0.02, smax=64, seed=5)
+    exp = [oracle.wfa(ps.a_seq(i), ps.b_seq(i), 64) for i in range(ps.n)]
+    b = WfaDeviceBatch(ps, trace=True)
+    b.run()
+    sc, cg = b.results()
+    assert [(int(sc[i]), cg[i].decode()) for i in range(ps.n)] == exp
+    reqs = [_areq(ga, ps.a_seq(i).decode(), ps.b_seq(i).decode(), 64) for i in range(300)]
+    reqs.append(_areq(ga, "ACGT" * 300, "ACGT" * 290 + "AGGT" * 10, 64))
+    r = ga.align_batch_ext(reqs)
+    assert r.ok()
+    assert [(x.score, x.cigar) for x in r.results[:300]] == exp[:300]
+    assert (r.results[300].score, r.results[300].cigar) == oracle.wfa("ACGT" * 300, "ACGT" * 290 + "AGGT" * 10, 64)
