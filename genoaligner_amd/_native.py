@@ -33,6 +33,7 @@ EXPORTS = (
     "ga_wfa_autotune", "ga_wfa_set_tuning", "ga_wfa_get_tuning",
     "ga_fasta_parse", "ga_fasta_read_file", "ga_fasta_count", "ga_fasta_record", "ga_fasta_free",
     "ga_fasta_sizes", "ga_fasta_export", "ga_fasta_device", "ga_gene_qc_walk",
+    "ga_msa_align_profiles", "ga_msa_profile_from_rows", "ga_msa_pp_device",
 )
 
 
@@ -104,6 +105,30 @@ class SwDevBatch(C.Structure):
                 ("n_pairs", C.c_int64), ("max_text_len", C.c_int32), ("max_pattern_len", C.c_int32),
                 ("scoring", SWScoringC), ("with_cigar", C.c_int32), ("out", C.c_void_p),
                 ("cigar", C.c_void_p), ("cigar_off", C.c_void_p), ("cigar_len", C.c_void_p)]
+
+
+class MsaParams(C.Structure):
+    _fields_ = [("match", C.c_float), ("ts", C.c_float), ("tv", C.c_float),
+                ("gap_open", C.c_float), ("gap_extend", C.c_float), ("free_end_gaps", C.c_int32),
+                ("psgp", C.c_int32), ("psgp_scale", C.c_float), ("psgp_min_open", C.c_float),
+                ("psgp_min_ext", C.c_float), ("alpha", C.c_int32), ("sub", C.c_float * 400)]
+
+
+class MsaProfile(C.Structure):
+    _fields_ = [("cols", C.c_void_p), ("occ", C.c_void_p), ("len", C.c_int32)]
+
+
+class MsaPPRes(C.Structure):
+    _fields_ = [("score", C.c_float), ("ai", C.c_int32), ("aj", C.c_int32), ("bi", C.c_int32),
+                ("bj", C.c_int32), ("cigar_off", C.c_int64), ("cigar_len", C.c_int32)]
+
+
+class MsaDevBatch(C.Structure):
+    _fields_ = [("cols", C.c_void_p), ("occ", C.c_void_p), ("n_cols", C.c_int64), ("col_off", C.c_void_p),
+                ("len", C.c_void_p), ("n_profiles", C.c_int32), ("a_idx", C.c_void_p), ("b_idx", C.c_void_p),
+                ("n_pairs", C.c_int64), ("score", C.c_void_p), ("span", C.c_void_p), ("cigar", C.c_void_p),
+                ("cigar_off", C.c_void_p), ("cigar_len", C.c_void_p), ("host_len", C.c_void_p),
+                ("host_a_idx", C.c_void_p), ("host_b_idx", C.c_void_p)]
 
 
 _lib = None
@@ -196,6 +221,14 @@ def lib() -> C.CDLL:
         L.ga_gene_qc_walk.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_int32, C.c_int32, C.c_void_p]
         L.ga_gene_qc_walk.restype = C.c_int
+        L.ga_msa_align_profiles.argtypes = [C.POINTER(MsaProfile), C.POINTER(MsaProfile), C.c_int64,
+                                            C.POINTER(MsaParams), C.POINTER(MsaPPRes), C.c_void_p, C.c_int64, cpp]
+        L.ga_msa_align_profiles.restype = C.c_int
+        L.ga_msa_profile_from_rows.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                               C.c_void_p]
+        L.ga_msa_profile_from_rows.restype = C.c_int
+        L.ga_msa_pp_device.argtypes = [C.POINTER(MsaDevBatch), C.POINTER(MsaParams), C.c_void_p, cpp]
+        L.ga_msa_pp_device.restype = C.c_int
         L.ga_route_bucket_bound.argtypes = [C.c_int32]
         L.ga_route_bucket_bound.restype = C.c_int32
         _lib = L

@@ -219,6 +219,43 @@ hipError_t sw_gather(const int32_t* res8, int64_t n, bool trace, int32_t* out5,
 const char* wfa_device_in_region(const ga_wfa_dev_batch* b, hipStream_t st, uint8_t* region, size_t plan_bytes,
                                  size_t stack_bytes);
 
+// ---------------------------------------------------------------- MSA
+// Profile-profile Gotoh (msa.hip).  Profiles live in one column table (all
+// profiles back to back, profile k at columns col_off[k] .. + len[k]); the
+// sparse form holds, per column c, nz_cnt[c] non-zero symbol fractions in
+// symbol order at nz_sym / nz_frq[k * n_cols + c] and the column's gap costs.
+struct MsaParamsDev {
+    float match, ts, tv, go, ge;
+    int free_end, psgp;
+    float psgp_scale, psgp_min_open, psgp_min_ext;
+    int alpha;                   // 4 or 20
+};
+struct MsaSparseLaunch {
+    const float* cols;           // n_cols x alpha fractions
+    const float* occ;            // n_cols
+    int64_t n_cols;
+    MsaParamsDev P;
+    uint8_t* nz_cnt; uint8_t* nz_sym; float* nz_frq;
+    float* gopen; float* gext;
+};
+struct MsaLaunch {
+    const uint8_t* nz_cnt; const uint8_t* nz_sym; const float* nz_frq;
+    const float* gopen; const float* gext;
+    int64_t n_cols;
+    const int64_t* col_off; const int32_t* len;        // per profile
+    const int32_t* a_idx; const int32_t* b_idx;        // per pair
+    int32_t first_pair;
+    const float* sub;            // alpha x alpha substitution scores (device)
+    MsaParamsDev P;
+    uint32_t* dirs; const int64_t* dir_off;            // msa_dir_words(M, N) words per pair
+    float* bnd; const int64_t* bnd_off;                // 3 (M + 1) floats per pair
+    float* score; int32_t* span;                       // span: ai, aj, bi, bj
+    uint8_t* cigar; const int64_t* cig_off; int32_t* cig_len;   // slot of M + N bytes, CIGAR right-aligned
+};
+int64_t msa_dir_words(int M, int N);
+hipError_t msa_sparse_launch(const MsaSparseLaunch& S, hipStream_t st);
+hipError_t msa_pp_launch(const MsaLaunch& L, int n_pairs, hipStream_t st, LaunchHooks* hooks = nullptr);
+
 // Run chunk(k0, k1) over [0, n) on the host work pool, about `grain` indices a
 // chunk; returns when every chunk has run.
 void host_parallel_for(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& chunk);

@@ -24,6 +24,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 namespace {
@@ -2064,6 +2065,338 @@ int ga_wfa_autotune(const ga_wfa_dev_batch* b, int reps, ga_wfa_tuning* out, voi
     if (e1) (void)hipEventDestroy(e1);
     if (err) { if (error) *error = err; return GA_DEVICE_ERROR; }
     *out = t;
+    return GA_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// MSA: profile-profile Gotoh batches (the reference's msa_pp_trace_kernel
+// launches, src/msa/msa_gpu.cpp, over genomsa::align_profiles semantics)
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int64_t kMsaDirBudget = int64_t(8) << 30;     // direction bytes per launch
+
+const char* msa_params_error(const ga_msa_params* p)
+{
+    if (!p) return "null MSA parameters";
+    if (p->alpha != 4 && p->alpha != 20) return "MSA alphabet must be 4 (DNA) or 20 (protein)";
+    return nullptr;
+}
+
+ga::MsaParamsDev msa_dev_params(const ga_msa_params& p)
+{
+    return ga::MsaParamsDev{p.match, p.ts, p.tv, p.gap_open, p.gap_extend, p.free_end_gaps != 0 ? 1 : 0,
+                            p.psgp != 0 ? 1 : 0, p.psgp_scale, p.psgp_min_open, p.psgp_min_ext, p.alpha};
+}
+
+// Substitution scores as the reference's col_score reads them
+// (msa_ref.cpp:1023-1028 for DNA: match, transitions A<->G and C<->T, other
+// transversions; the matrix otherwise).
+std::vector<float> msa_sub_table(const ga_msa_params& p)
+{
+    const int al = p.alpha;
+    std::vector<float> t(size_t(al) * al);
+    for (int a = 0; a < al; ++a)
+        for (int b = 0; b < al; ++b) {
+            float v;
+            if (al == 4) {
+                const bool ts = (a == 0 && b == 2) || (a == 2 && b == 0) || (a == 1 && b == 3) || (a == 3 && b == 1);
+                v = a == b ? p.match : (ts ? p.ts : p.tv);
+            } else {
+                v = p.sub[a * 20 + b];
+            }
+            t[size_t(a) * al + b] = v;
+        }
+    return t;
+}
+
+// Sparse columns of a device profile table, then the pairs in launches of at
+// most kMsaDirBudget direction bytes.  hlen / ha / hb: host copies of the
+// profile lengths and pair indices.
+const char* enqueue_msa(const ga_msa_params& p, const float* cols, const float* occ, int64_t n_cols,
+                        const int64_t* col_off, const int32_t* len, const int32_t* a_idx, const int32_t* b_idx,
+                        int64_t n, const int32_t* hlen, const int32_t* ha, const int32_t* hb, float* score,
+                        int32_t* span, uint8_t* cigar, const int64_t* cig_off, int32_t* cig_len, hipStream_t st)
+{
+    DevArena D;
+    D.st = st;
+    const int al = p.alpha;
+    ga::MsaSparseLaunch S{};
+    S.cols = cols; S.occ = occ; S.n_cols = n_cols; S.P = msa_dev_params(p);
+    S.nz_cnt = D.alloc<uint8_t>(size_t(std::max<int64_t>(n_cols, 1)));
+    S.nz_sym = D.alloc<uint8_t>(size_t(std::max<int64_t>(n_cols, 1)) * al);
+    S.nz_frq = D.alloc<float>(size_t(std::max<int64_t>(n_cols, 1)) * al);
+    S.gopen = D.alloc<float>(size_t(std::max<int64_t>(n_cols, 1)));
+    S.gext = D.alloc<float>(size_t(std::max<int64_t>(n_cols, 1)));
+    const float* d_sub = D.upload(msa_sub_table(p));
+    // per pair: direction words and boundary floats relative to its launch
+    std::vector<int64_t> dir_off(static_cast<size_t>(n)), bnd_off(static_cast<size_t>(n));
+    std::vector<int64_t> first{0};
+    std::vector<int64_t> dwords, bfloats;
+    int64_t dw = 0, bf = 0;
+    for (int64_t q = 0; q < n; ++q) {
+        const int M = std::max(hlen[ha[q]], 0), N = std::max(hlen[hb[q]], 0);
+        const int64_t w = ga::msa_dir_words(M, N);
+        if (dw > 0 && (dw + w) * 4 > kMsaDirBudget) {
+            first.push_back(q);
+            dwords.push_back(dw);
+            bfloats.push_back(bf);
+            dw = bf = 0;
+        }
+        dir_off[size_t(q)] = dw;
+        bnd_off[size_t(q)] = bf;
+        dw += w;
+        bf += 3 * (int64_t(M) + 1);
+    }
+    first.push_back(n);
+    dwords.push_back(dw);
+    bfloats.push_back(bf);
+    const int64_t* d_dir_off = D.upload(dir_off);
+    const int64_t* d_bnd_off = D.upload(bnd_off);
+    if (!D.ok) { D.release(); return "hipMalloc/hipMemcpy (MSA buffers)"; }
+    const char* err = nullptr;
+    if (ga::msa_sparse_launch(S, st) != hipSuccess) err = "MSA sparse kernel launch";
+    ga::MsaLaunch L{};
+    L.nz_cnt = S.nz_cnt; L.nz_sym = S.nz_sym; L.nz_frq = S.nz_frq; L.gopen = S.gopen; L.gext = S.gext;
+    L.n_cols = n_cols; L.col_off = col_off; L.len = len; L.a_idx = a_idx; L.b_idx = b_idx;
+    L.sub = d_sub; L.P = S.P;
+    L.dir_off = d_dir_off; L.bnd_off = d_bnd_off;
+    L.score = score; L.span = span; L.cigar = cigar; L.cig_off = cig_off; L.cig_len = cig_len;
+    for (size_t c = 0; c + 1 < first.size() && !err; ++c) {
+        void* dirs = nullptr;
+        void* bnd = nullptr;
+        if (ws_alloc(&dirs, size_t(std::max<int64_t>(dwords[c], 16)) * 4, st) != hipSuccess ||
+            ws_alloc(&bnd, size_t(std::max<int64_t>(bfloats[c], 16)) * 4, st) != hipSuccess) {
+            if (dirs) (void)ws_free(dirs, st);
+            err = "hipMalloc(MSA direction bytes)";
+            break;
+        }
+        L.dirs = static_cast<uint32_t*>(dirs);
+        L.bnd = static_cast<float*>(bnd);
+        L.first_pair = int32_t(first[c]);
+        if (ga::msa_pp_launch(L, int(first[c + 1] - first[c]), st, &t_timer_hooks) != hipSuccess)
+            err = "MSA kernel launch";
+        (void)ws_free(dirs, st);
+        (void)ws_free(bnd, st);
+    }
+    D.release();
+    return err;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ga_msa_profile_from_rows(const char* rows, int32_t nrows, int32_t L, int32_t alpha, float* cols, float* occ)
+{
+    // msa_ref.cpp:21-106 (symbol fractions) and :773-794 (counts over rows)
+    if ((alpha != 4 && alpha != 20) || nrows <= 0 || L < 0 || (L > 0 && (!rows || !cols || !occ)))
+        return GA_INVALID_ARGUMENT;
+    auto counts = [alpha](char c, float* cnt) -> bool {
+        for (int s = 0; s < alpha; ++s) cnt[s] = 0.f;
+        if (alpha == 4) {
+            int mask = 0;
+            switch (c) {
+            case 'A': case 'a': mask = 1; break;
+            case 'C': case 'c': mask = 2; break;
+            case 'G': case 'g': mask = 4; break;
+            case 'T': case 't': case 'U': case 'u': mask = 8; break;
+            case 'R': mask = 5; break;
+            case 'Y': mask = 10; break;
+            case 'M': mask = 3; break;
+            case 'K': mask = 12; break;
+            case 'S': mask = 6; break;
+            case 'W': mask = 9; break;
+            case 'H': mask = 11; break;
+            case 'B': mask = 14; break;
+            case 'V': mask = 7; break;
+            case 'D': mask = 13; break;
+            case 'N': case 'n': case 'X': mask = 15; break;
+            default: return false;
+            }
+            const float f = 1.0f / float(__builtin_popcount(unsigned(mask)));
+            for (int s = 0; s < 4; ++s)
+                if (mask & (1 << s)) cnt[s] = f;
+            return true;
+        }
+        switch (c) {
+        case 'B': cnt[2] = cnt[3] = 0.5f; return true;
+        case 'Z': cnt[5] = cnt[6] = 0.5f; return true;
+        case 'J': cnt[9] = cnt[10] = 0.5f; return true;
+        case 'U': case 'u': cnt[4] = 1.f; return true;
+        case 'X': case 'x': case 'O': case 'o':
+            for (int s = 0; s < 20; ++s) cnt[s] = 1.f / 20.f;
+            return true;
+        default: break;
+        }
+        static const char aa[] = "ARNDCQEGHILKMFPSTWYV";
+        const char* q = static_cast<const char*>(std::memchr(aa, c >= 'a' && c <= 'z' ? c - 32 : c, 20));
+        if (!q || c == 0) return false;
+        cnt[q - aa] = 1.f;
+        return true;
+    };
+    for (int i = 0; i < L; ++i) {
+        for (int s = 0; s < alpha; ++s) cols[size_t(i) * alpha + s] = 0.f;
+        occ[i] = 0.f;
+    }
+    float cnt[20];
+    for (int r = 0; r < nrows; ++r)
+        for (int i = 0; i < L; ++i)
+            if (counts(rows[size_t(r) * L + i], cnt)) {
+                for (int s = 0; s < alpha; ++s) cols[size_t(i) * alpha + s] += cnt[s];
+                occ[i] += 1.f;
+            }
+    for (int i = 0; i < L; ++i) {
+        for (int s = 0; s < alpha; ++s) cols[size_t(i) * alpha + s] /= float(nrows);
+        occ[i] /= float(nrows);
+    }
+    return GA_OK;
+}
+
+int ga_msa_align_profiles(const ga_msa_profile* A, const ga_msa_profile* B, int64_t n, const ga_msa_params* p,
+                          ga_msa_pp_res* res, char* arena, int64_t arena_bytes, const char** error)
+{
+    if (error) *error = nullptr;
+    auto refuse = [&](const char* m) {
+        if (error) *error = m;
+        return GA_INVALID_ARGUMENT;
+    };
+    if (n <= 0) return GA_OK;
+    if (const char* e = msa_params_error(p)) return refuse(e);
+    if (!A || !B || !res) return refuse("null profile or result array");
+    if (n > INT32_MAX) return refuse("too many pairs");
+    for (int64_t q = 0; q < n; ++q) res[q] = ga_msa_pp_res{0.f, 0, 0, 0, 0, 0, 0};
+    int64_t need = 0;
+    for (int64_t q = 0; q < n; ++q) {
+        for (const ga_msa_profile* f : {A + q, B + q}) {
+            if (f->len < 0) return refuse("negative profile length");
+            if (f->len > 0 && (!f->cols || !f->occ)) return refuse("null profile columns with a positive length");
+        }
+        need += int64_t(A[q].len) + B[q].len;
+    }
+    if (need > 0 && (!arena || arena_bytes < need)) return refuse("CIGAR arena smaller than the batch needs");
+    std::call_once(g_pool_once, init_pool);
+    const int al = p->alpha;
+    // column table: each distinct profile once (a profile shared by several
+    // pairs, one bait against many, is uploaded once)
+    std::vector<const ga_msa_profile*> prof;
+    std::vector<int32_t> ha(static_cast<size_t>(n)), hb(static_cast<size_t>(n));
+    {
+        std::unordered_map<const float*, int32_t> seen;
+        auto id = [&](const ga_msa_profile* f) -> int32_t {
+            if (f->len > 0) {
+                auto it = seen.find(f->cols);
+                if (it != seen.end() && prof[size_t(it->second)]->occ == f->occ &&
+                    prof[size_t(it->second)]->len == f->len)
+                    return it->second;
+            }
+            prof.push_back(f);
+            const int32_t k = int32_t(prof.size() - 1);
+            if (f->len > 0) seen.emplace(f->cols, k);
+            return k;
+        };
+        for (int64_t q = 0; q < n; ++q) {
+            ha[size_t(q)] = id(A + q);
+            hb[size_t(q)] = id(B + q);
+        }
+    }
+    std::vector<int32_t> hlen(prof.size());
+    std::vector<int64_t> col_off(prof.size());
+    int64_t n_cols = 0;
+    for (size_t k = 0; k < prof.size(); ++k) {
+        hlen[k] = prof[k]->len;
+        col_off[k] = n_cols;
+        n_cols += prof[k]->len;
+    }
+    std::vector<float> cols(size_t(std::max<int64_t>(n_cols, 1)) * al), occ(size_t(std::max<int64_t>(n_cols, 1)));
+    for (size_t k = 0; k < prof.size(); ++k)
+        if (prof[k]->len > 0) {
+            std::memcpy(cols.data() + size_t(col_off[k]) * al, prof[k]->cols, size_t(prof[k]->len) * al * 4);
+            std::memcpy(occ.data() + col_off[k], prof[k]->occ, size_t(prof[k]->len) * 4);
+        }
+    std::vector<int64_t> cig_off(static_cast<size_t>(n));
+    int64_t slot = 0;
+    for (int64_t q = 0; q < n; ++q) {
+        cig_off[size_t(q)] = slot;
+        slot += int64_t(A[q].len) + B[q].len;
+    }
+    hipStream_t st = acquire_stream();
+    if (!st) { if (error) *error = "hipStreamCreate"; return GA_DEVICE_ERROR; }
+    DevArena D;
+    D.st = st;
+    const float* d_cols = D.upload(cols);
+    const float* d_occ = D.upload(occ);
+    const int64_t* d_col_off = D.upload(col_off);
+    const int32_t* d_len = D.upload(hlen);
+    const int32_t* d_a = D.upload(ha);
+    const int32_t* d_b = D.upload(hb);
+    const int64_t* d_cig_off = D.upload(cig_off);
+    float* d_score = D.alloc<float>(size_t(n));
+    int32_t* d_span = D.alloc<int32_t>(size_t(n) * 4);
+    int32_t* d_clen = D.alloc<int32_t>(size_t(n));
+    uint8_t* d_cig = D.alloc<uint8_t>(size_t(std::max<int64_t>(slot, 1)));
+    const char* err = D.ok ? nullptr : "hipMalloc/hipMemcpy (MSA buffers)";
+    t_timer.reset();
+    if (!err)
+        err = enqueue_msa(*p, d_cols, d_occ, n_cols, d_col_off, d_len, d_a, d_b, n, hlen.data(), ha.data(), hb.data(),
+                          d_score, d_span, d_cig, d_cig_off, d_clen, st);
+    std::vector<float> score(static_cast<size_t>(n));
+    std::vector<int32_t> span(static_cast<size_t>(n) * 4), clen(static_cast<size_t>(n));
+    if (!err && (hipMemcpyAsync(score.data(), d_score, size_t(n) * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                 hipMemcpyAsync(span.data(), d_span, size_t(n) * 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                 hipMemcpyAsync(clen.data(), d_clen, size_t(n) * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                 (slot > 0 && hipMemcpyAsync(arena, d_cig, size_t(slot), hipMemcpyDeviceToHost, st) != hipSuccess)))
+        err = "hipMemcpy(MSA results)";
+    D.release();
+    const hipError_t se = hipStreamSynchronize(st);
+    if (!err && se != hipSuccess) err = hipGetErrorString(se);
+    release_stream(st);
+    if (err) {
+        if (error) *error = err;
+        for (int64_t q = 0; q < n; ++q) res[q] = ga_msa_pp_res{0.f, 0, 0, 0, 0, 0, 0};
+        return GA_DEVICE_ERROR;
+    }
+    for (int64_t q = 0; q < n; ++q) {
+        ga_msa_pp_res& r = res[q];
+        r.score = score[size_t(q)];
+        r.ai = span[size_t(q) * 4]; r.aj = span[size_t(q) * 4 + 1];
+        r.bi = span[size_t(q) * 4 + 2]; r.bj = span[size_t(q) * 4 + 3];
+        r.cigar_len = clen[size_t(q)];
+        r.cigar_off = cig_off[size_t(q)] + int64_t(A[q].len) + B[q].len - r.cigar_len;
+    }
+    return GA_OK;
+}
+
+int ga_msa_pp_device(const ga_msa_dev_batch* b, const ga_msa_params* p, void* stream, const char** error)
+{
+    if (error) *error = nullptr;
+    if (const char* e = msa_params_error(p)) { if (error) *error = e; return GA_INVALID_ARGUMENT; }
+    if (!b) { if (error) *error = "null batch"; return GA_INVALID_ARGUMENT; }
+    if (b->n_pairs <= 0) return GA_OK;
+    if (b->n_pairs > INT32_MAX) { if (error) *error = "too many pairs"; return GA_INVALID_ARGUMENT; }
+    if (!b->host_len || !b->host_a_idx || !b->host_b_idx) {
+        if (error) *error = "ga_msa_pp_device needs the host copies of the lengths and pair indices";
+        return GA_INVALID_ARGUMENT;
+    }
+    for (int32_t k = 0; k < b->n_profiles; ++k)
+        if (b->host_len[k] < 0) { if (error) *error = "negative profile length"; return GA_INVALID_ARGUMENT; }
+    for (int64_t q = 0; q < b->n_pairs; ++q)
+        if (b->host_a_idx[q] < 0 || b->host_a_idx[q] >= b->n_profiles || b->host_b_idx[q] < 0 ||
+            b->host_b_idx[q] >= b->n_profiles) {
+            if (error) *error = "pair index outside the profile table";
+            return GA_INVALID_ARGUMENT;
+        }
+    std::call_once(g_pool_once, init_pool);
+    t_timer.reset();
+    const char* err = enqueue_msa(*p, b->cols, b->occ, b->n_cols, b->col_off, b->len, b->a_idx, b->b_idx, b->n_pairs,
+                                  b->host_len, b->host_a_idx, b->host_b_idx, b->score, b->span, b->cigar,
+                                  b->cigar_off, b->cigar_len, static_cast<hipStream_t>(stream));
+    if (err) {
+        if (error) *error = err;
+        return GA_DEVICE_ERROR;
+    }
     return GA_OK;
 }
 

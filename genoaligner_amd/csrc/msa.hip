@@ -1,0 +1,363 @@
+// msa.hip -- gfx950 kernels for the profile-profile Gotoh DP of the
+// reference's progressive MSA (include/genoaligner/backend/
+// msa_pp_kernel_impl.hip:35-209, semantics fixed by src/msa/msa_ref.cpp:
+// 1063-1164):
+//   M[i][j]  = col(A_i-1, B_j-1) + max(M, Ix, Iy)[i-1][j-1]   ties M > Ix > Iy
+//   Ix[i][j] = max(M[i-1][j] - open(occA_i-1), Ix[i-1][j] - ext(occA_i-1))
+//   Iy[i][j] = max(M[i][j-1] - open(occB_j-1), Iy[i][j-1] - ext(occB_j-1))
+//   col(a, b) = sum over symbols x of a, then y of b, of fa_x * fb_y * S(x, y)
+// in float, every operation in the reference's order and unfused (the
+// library is built with -ffp-contract=off).  free_end_gaps: row 0 and column 0
+// are 0 and the end is the best cell of the last column (rows ascending, M
+// then Ix) or the last row (columns ascending, M then Iy), first maximum
+// winning; otherwise the end is M[M][N].  A direction byte per cell (M source
+// in bits 0-1, Ix extended 4, Iy extended 8) drives the reference's
+// three-state walk.
+//
+// Mapping to CDNA4: one wave64 per pair.  Profile B's columns are cut into
+// strips of 64, lane l owning column j = 64 s + l + 1; A's columns stream
+// through the wave as rows with a one-step skew per lane (at step t lane l
+// computes row t - l).  A cell's left neighbour (row i, column j-1) and its
+// diagonal (row i-1, column j-1) are what lane l-1 produced one and two steps
+// earlier, moved by one DPP wave_shr:1 per value and step; the row above stays
+// in the lane's registers.  Lane 0 takes its left column from the strip
+// boundary buffer that lane 63 of the previous strip filled (or, in strip 0,
+// from the boundary formulas).  Profile columns come in sparse form (the
+// non-zero symbol fractions in ascending symbol order, from msa_sparse_kernel):
+// a zero fraction adds exactly +0 to a sum that is never -0, so skipping it
+// leaves every bit of the reference's sum unchanged.  Direction bytes are
+// stored in the computation's own skewed order, four steps per 32-bit word, a
+// wave's 64 words one contiguous 256-byte store.
+#include "ga_device.hpp"
+
+namespace ga {
+namespace {
+
+constexpr float kMsaNeg = -1e30f;
+
+__device__ __forceinline__ float dpp_shr1(float old, float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x138, 0xf, 0xf,
+                                                      false));
+}
+
+// Position-specific gap penalties (include/genoaligner/msa/msa.hpp:123-134).
+__device__ __forceinline__ float psgp_open(float occ, const MsaParamsDev& P)
+{
+    if (!P.psgp || occ >= 1.0f) return P.go * occ;
+    const float p = P.psgp_scale * P.go * occ;
+    const float fl = P.psgp_min_open * P.go;
+    return p > fl ? p : fl;
+}
+__device__ __forceinline__ float psgp_ext(float occ, const MsaParamsDev& P)
+{
+    if (!P.psgp || occ >= 1.0f) return P.ge;
+    const float p = P.ge * occ;
+    const float fl = P.psgp_min_ext * P.ge;
+    return p > fl ? p : fl;
+}
+
+// One thread per profile column: its non-zero fractions in symbol order and
+// its gap-open / gap-extend costs.
+__global__ void msa_sparse_kernel(MsaSparseLaunch S)
+{
+    const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= S.n_cols) return;
+    const float* f = S.cols + c * S.P.alpha;
+    int k = 0;
+    for (int x = 0; x < S.P.alpha; ++x) {
+        const float v = f[x];
+        if (v != 0.f) {
+            S.nz_sym[int64_t(k) * S.n_cols + c] = uint8_t(x);
+            S.nz_frq[int64_t(k) * S.n_cols + c] = v;
+            ++k;
+        }
+    }
+    S.nz_cnt[c] = uint8_t(k);
+    S.gopen[c] = psgp_open(S.occ[c], S.P);
+    S.gext[c] = psgp_ext(S.occ[c], S.P);
+}
+
+struct Best {
+    float v;
+    int key;        // position in the reference's end-point scan order
+};
+__device__ __forceinline__ void offer(Best& b, float v, int key)
+{
+    if (v > b.v || (v == b.v && key < b.key)) { b.v = v; b.key = key; }
+}
+
+// The byte of cell (i, j) in the skewed direction layout.
+__device__ __forceinline__ uint32_t dir_at(const uint32_t* d, int tq, int i, int j)
+{
+    const int s = (j - 1) >> 6, l = (j - 1) & 63, t = i + l;
+    return (d[(int64_t(s) * tq + (t >> 2)) * 64 + l] >> (8 * (t & 3))) & 0xffu;
+}
+
+// Three-state walk from the end cell (msa_ref.cpp:1131-1156); emits the
+// column ops right-aligned in the pair's CIGAR slot.
+__device__ void msa_walk(const uint32_t* d, int tq, int ei, int ej, char est, uint8_t* slot_end, int* oi, int* oj,
+                         int* used_out)
+{
+    int i = ei, j = ej, used = 0;
+    char st = est;
+    while (i > 0 || j > 0) {
+        if (st == 'M') {
+            if (i == 0 || j == 0) break;
+            slot_end[-1 - used++] = 'M';
+            const uint32_t h = dir_at(d, tq, i, j) & 3u;
+            --i; --j;
+            st = h == 0 ? 'M' : (h == 1 ? 'I' : 'D');
+        } else if (st == 'I') {
+            if (i == 0) break;
+            slot_end[-1 - used++] = 'I';
+            if (j > 0) st = (dir_at(d, tq, i, j) & 4u) ? 'I' : 'M';
+            --i;
+        } else {
+            if (j == 0) break;
+            slot_end[-1 - used++] = 'D';
+            if (i > 0) st = (dir_at(d, tq, i, j) & 8u) ? 'D' : 'M';
+            --j;
+        }
+    }
+    *oi = i; *oj = j; *used_out = used;
+}
+
+__device__ __forceinline__ float boundary_gap(float open, int k, float ge)
+{
+    return -(open + float(k - 1) * ge);
+}
+
+// A pair with an empty profile: only boundary cells exist (row 0 / column 0).
+__device__ void msa_degenerate(const MsaLaunch& L, int pid, int64_t ca, int M, int64_t cb, int N)
+{
+    const MsaParamsDev& P = L.P;
+    const bool fr = P.free_end != 0;
+    auto Mv = [&](int i, int j) -> float { return (i == 0 && j == 0) ? 0.f : (fr ? 0.f : kMsaNeg); };
+    auto Ixv = [&](int i) -> float { return i == 0 ? kMsaNeg : (fr ? 0.f : boundary_gap(L.gopen[ca + i - 1], i, P.ge)); };
+    auto Iyv = [&](int j) -> float { return j == 0 ? kMsaNeg : (fr ? 0.f : boundary_gap(L.gopen[cb + j - 1], j, P.ge)); };
+    float best = Mv(M, N);
+    int ei = M, ej = N;
+    char est = 'M';
+    if (fr) {
+        best = kMsaNeg;
+        // column N (only column 0 or row 0 cells exist here)
+        for (int i = 0; i <= M; ++i) {
+            const float vm = N == 0 ? Mv(i, 0) : Mv(0, N);
+            const float vx = N == 0 ? Ixv(i) : kMsaNeg;
+            if (N != 0 && i > 0) break;                  // M == 0 here: only row 0
+            if (vm > best) { best = vm; ei = i; ej = N; est = 'M'; }
+            if (vx > best) { best = vx; ei = i; ej = N; est = 'I'; }
+        }
+        for (int j = 0; j <= N; ++j) {
+            const float vm = M == 0 ? Mv(0, j) : Mv(M, 0);
+            const float vy = M == 0 ? Iyv(j) : kMsaNeg;
+            if (M != 0 && j > 0) break;                  // N == 0 here: only column 0
+            if (vm > best) { best = vm; ei = M; ej = j; est = 'M'; }
+            if (vy > best) { best = vy; ei = M; ej = j; est = 'D'; }
+        }
+    }
+    uint8_t* slot_end = L.cigar + L.cig_off[pid] + M + N;
+    int i = ei, j = ej, used = 0;
+    char st = est;
+    while (i > 0 || j > 0) {                             // no interior cell: no direction byte is read
+        if (st == 'M') break;
+        if (st == 'I') {
+            if (i == 0) break;
+            slot_end[-1 - used++] = 'I';
+            --i;
+        } else {
+            if (j == 0) break;
+            slot_end[-1 - used++] = 'D';
+            --j;
+        }
+    }
+    L.score[pid] = best;
+    int32_t* sp = L.span + 4 * int64_t(pid);
+    sp[0] = i; sp[1] = j; sp[2] = ei; sp[3] = ej;
+    L.cig_len[pid] = used;
+}
+
+template <int K>
+__global__ __launch_bounds__(64) void msa_pp_kernel(MsaLaunch L)
+{
+    __shared__ float sub_s[K * K];
+    __shared__ float bfrq[K * 64];
+    __shared__ uint8_t bsym[K * 64];
+    __shared__ Best red[64];
+    __shared__ float mmn;
+    const int pid = L.first_pair + int(blockIdx.x);
+    const int lane = threadIdx.x;
+    const MsaParamsDev P = L.P;
+    for (int q = lane; q < K * K; q += 64) sub_s[q] = L.sub[q];
+    const int a = L.a_idx[pid], b = L.b_idx[pid];
+    const int M = L.len[a], N = L.len[b];
+    const int64_t ca = L.col_off[a], cb = L.col_off[b];
+    __syncthreads();
+    if (M <= 0 || N <= 0) {
+        if (lane == 0) msa_degenerate(L, pid, ca, M > 0 ? M : 0, cb, N > 0 ? N : 0);
+        return;
+    }
+    const bool fr = P.free_end != 0;
+    const float ge = P.ge;
+    const int64_t C = L.n_cols;
+    uint32_t* dirs = L.dirs + L.dir_off[pid];
+    float* bM = L.bnd + L.bnd_off[pid];
+    float* bX = bM + (M + 1);
+    float* bY = bX + (M + 1);
+    const int nstrips = (N + 63) >> 6;
+    const int T = M + 64;                               // steps t = 0 .. M + 63
+    const int tq = (T + 3) >> 2;
+    Best best{-3.0e38f, 0x7fffffff};
+    if (lane == 0 && fr) {
+        // row M, column 0: M[M][0] (key 2(M+1)), Iy[M][0] = NEG
+        offer(best, 0.f, 2 * (M + 1));
+    }
+    for (int s = 0; s < nstrips; ++s) {
+        const int j = (s << 6) + lane + 1;
+        const bool jv = j <= N;
+        const int64_t cB = cb + j - 1;
+        const int nb = jv ? int(L.nz_cnt[cB]) : 0;
+        for (int k = 0; k < nb; ++k) {
+            bsym[k * 64 + lane] = L.nz_sym[int64_t(k) * C + cB];
+            bfrq[k * 64 + lane] = L.nz_frq[int64_t(k) * C + cB];
+        }
+        const float openA = jv ? L.gopen[cB] : 0.f, extA = jv ? L.gext[cB] : 0.f;
+        float prevM = fr ? 0.f : kMsaNeg, prevIx = kMsaNeg;          // M[0][j], Ix[0][j]
+        const float row0Iy = fr ? 0.f : boundary_gap(openA, j, ge);
+        float outM = kMsaNeg, outIx = kMsaNeg, outIy = kMsaNeg;
+        float dM = kMsaNeg, dIx = kMsaNeg, dIy = kMsaNeg;
+        uint32_t acc = 0;
+        for (int t = 0; t < T; ++t) {
+            const int i = t - lane;
+            // lane 0's left column at row t: the column-0 boundary or the
+            // previous strip's last column (wave-uniform values)
+            float lbM, lbX, lbY;
+            if (t <= M) {
+                if (s == 0) {
+                    lbM = t == 0 ? 0.f : (fr ? 0.f : kMsaNeg);
+                    lbX = t == 0 ? kMsaNeg : (fr ? 0.f : boundary_gap(L.gopen[ca + t - 1], t, ge));
+                    lbY = kMsaNeg;
+                } else {
+                    lbM = bM[t]; lbX = bX[t]; lbY = bY[t];
+                }
+            } else {
+                lbM = lbX = lbY = kMsaNeg;
+            }
+            const float lM = dpp_shr1(lbM, outM);
+            const float lX = dpp_shr1(lbX, outIx);
+            const float lY = dpp_shr1(lbY, outIy);
+            uint32_t code = 0;
+            if (i == 0) {
+                outM = prevM; outIx = prevIx; outIy = row0Iy;
+                if (jv && j == N) {                       // column N, row 0
+                    offer(best, prevM, 0);
+                    offer(best, prevIx, 1);
+                }
+            } else if (i >= 1 && i <= M && jv) {
+                const int64_t cA = ca + i - 1;
+                const int na = L.nz_cnt[cA];
+                const float openB = L.gopen[cA], extB = L.gext[cA];
+                float sc = 0.f;
+                for (int ka = 0; ka < na; ++ka) {
+                    const int x = L.nz_sym[int64_t(ka) * C + cA];
+                    const float fa = L.nz_frq[int64_t(ka) * C + cA];
+                    for (int kb = 0; kb < nb; ++kb) {
+                        const int y = bsym[kb * 64 + lane];
+                        const float fb = bfrq[kb * 64 + lane];
+                        sc += fa * fb * sub_s[x * K + y];
+                    }
+                }
+                const float oIx = prevM - openB, eIx = prevIx - extB;
+                const float nIx = oIx >= eIx ? oIx : eIx;
+                const float oIy = lM - openA, eIy = lY - extA;
+                const float cIy = oIy >= eIy ? oIy : eIy;
+                float mx = dM;
+                uint32_t h = 0;
+                if (dIx > mx) { mx = dIx; h = 1; }
+                if (dIy > mx) { mx = dIy; h = 2; }
+                const float cm = sc + mx;
+                code = h | (eIx > oIx ? 4u : 0u) | (eIy > oIy ? 8u : 0u);
+                outM = cm; outIx = nIx; outIy = cIy;
+                prevM = cm; prevIx = nIx;
+                if (fr) {
+                    if (j == N) { offer(best, cm, 2 * i); offer(best, nIx, 2 * i + 1); }
+                    if (i == M) { offer(best, cm, 2 * (M + 1) + 2 * j); offer(best, cIy, 2 * (M + 1) + 2 * j + 1); }
+                } else if (i == M && j == N) {
+                    mmn = cm;
+                }
+            }
+            dM = lM; dIx = lX; dIy = lY;
+            if (lane == 63 && i >= 0 && i <= M) { bM[i] = outM; bX[i] = outIx; bY[i] = outIy; }
+            acc |= code << (8 * (t & 3));
+            if ((t & 3) == 3 || t == T - 1) {
+                dirs[(int64_t(s) * tq + (t >> 2)) * 64 + lane] = acc;
+                acc = 0;
+            }
+        }
+        __syncthreads();
+    }
+    // first maximum of the end-point scan: the greatest value, then the
+    // earliest position in the scan order
+    red[lane] = best;
+    __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) {
+        if (lane < off) {
+            const Best o = red[lane + off];
+            Best m = red[lane];
+            offer(m, o.v, o.key);
+            red[lane] = m;
+        }
+        __syncthreads();
+    }
+    __threadfence();
+    if (lane != 0) return;
+    int ei = M, ej = N;
+    char est = 'M';
+    float sc;
+    if (fr) {
+        const Best w = red[0];
+        sc = w.v;
+        if (w.key < 2 * (M + 1)) {
+            ei = w.key >> 1; ej = N; est = (w.key & 1) ? 'I' : 'M';
+        } else {
+            const int q = w.key - 2 * (M + 1);
+            ei = M; ej = q >> 1; est = (q & 1) ? 'D' : 'M';
+        }
+    } else {
+        sc = mmn;
+    }
+    int oi, oj, used;
+    msa_walk(dirs, tq, ei, ej, est, L.cigar + L.cig_off[pid] + M + N, &oi, &oj, &used);
+    L.score[pid] = sc;
+    int32_t* sp = L.span + 4 * int64_t(pid);
+    sp[0] = oi; sp[1] = oj; sp[2] = ei; sp[3] = ej;
+    L.cig_len[pid] = used;
+}
+
+}  // namespace
+
+int64_t msa_dir_words(int M, int N)
+{
+    if (M <= 0 || N <= 0) return 0;
+    return int64_t((N + 63) >> 6) * int64_t((M + 64 + 3) >> 2) * 64;
+}
+
+hipError_t msa_sparse_launch(const MsaSparseLaunch& S, hipStream_t st)
+{
+    if (S.n_cols <= 0) return hipSuccess;
+    hipLaunchKernelGGL(msa_sparse_kernel, dim3(unsigned((S.n_cols + 255) / 256)), dim3(256), 0, st, S);
+    return hipGetLastError();
+}
+
+hipError_t msa_pp_launch(const MsaLaunch& L, int n_pairs, hipStream_t st, LaunchHooks* hooks)
+{
+    if (n_pairs <= 0) return hipSuccess;
+    if (hooks) hooks->before(st);
+    if (L.P.alpha == 4) hipLaunchKernelGGL(msa_pp_kernel<4>, dim3(unsigned(n_pairs)), dim3(64), 0, st, L);
+    else hipLaunchKernelGGL(msa_pp_kernel<20>, dim3(unsigned(n_pairs)), dim3(64), 0, st, L);
+    if (hooks) hooks->after(st);
+    return hipGetLastError();
+}
+
+}  // namespace ga

@@ -393,6 +393,83 @@ int ga_gene_qc_walk(const ga_sw_res* res, int64_t n, const uint8_t* cigar_arena,
                     const int64_t* text_off, const int32_t* text_len, int32_t frame, int32_t code,
                     ga_qc_row* rows);
 
+/* ---- MSA building blocks (reference: src/msa/msa_ref.cpp, the genomsa
+ * engine; include/genoaligner/backend/msa_pp_kernel_impl.hip,
+ * nj_kernel_impl.hip) ----------------------------------------------------- */
+
+/* genomsa::Params fields of the profile DP (include/genoaligner/msa/msa.hpp:
+ * 36-60): alpha 4 (IUPAC DNA: match / ts / tv scores) or 20 (amino acids:
+ * sub[a * 20 + b], BLOSUM order ARNDCQEGHILKMFPSTWYV). */
+typedef struct ga_msa_params {
+    float   match, ts, tv;
+    float   gap_open, gap_extend;
+    int32_t free_end_gaps;
+    int32_t psgp;
+    float   psgp_scale, psgp_min_open, psgp_min_ext;
+    int32_t alpha;
+    float   sub[20 * 20];
+} ga_msa_params;
+
+/* A profile on the host: len columns of alpha symbol fractions
+ * (cols[c * alpha + s]) and their occupancy (non-gap fraction). */
+typedef struct ga_msa_profile {
+    const float* cols;
+    const float* occ;
+    int32_t      len;
+} ga_msa_profile;
+
+/* genomsa::AlignResult (msa.hpp:339-344): score, aligned span (first
+ * consumed column of A / B, ends exclusive) and the column CIGAR over
+ * {M, I (A column only), D (B column only)} at arena[cigar_off ..). */
+typedef struct ga_msa_pp_res {
+    float   score;
+    int32_t ai, aj, bi, bj;
+    int64_t cigar_off;
+    int32_t cigar_len;
+} ga_msa_pp_res;
+
+/* Replaces genomsa::align_profiles (msa.hpp:345, msa_ref.cpp:1166-1169) for
+ * a batch of profile pairs -- the reference's msa_pp_trace_kernel launch
+ * (msa_pp_kernel_impl.hip:35-209) -- with the same float results bit for
+ * bit.  The arena needs sum(A.len + B.len) bytes.  Refused with
+ * GA_INVALID_ARGUMENT: alpha other than 4 or 20, a negative length, a null
+ * array with a positive length, a small arena. */
+int ga_msa_align_profiles(const ga_msa_profile* A, const ga_msa_profile* B, int64_t n, const ga_msa_params* p,
+                          ga_msa_pp_res* res, char* cigar_arena, int64_t arena_bytes, const char** error);
+
+/* genomsa::profile_from_seq / profile_update_counts (msa.hpp:322-325,
+ * msa_ref.cpp:751-794): the profile of nrows aligned rows of L bytes (row r
+ * at rows + r * L); IUPAC letters (alpha 4) and B / Z / J / X / O / U (alpha
+ * 20) count fractionally, '-' and unknown letters as gap. */
+int ga_msa_profile_from_rows(const char* rows, int32_t nrows, int32_t L, int32_t alpha, float* cols, float* occ);
+
+/* Device-resident batch (EXTENSION: no reference counterpart): profiles in
+ * one column table on the device (profile k at columns col_off[k] ..
+ * + len[k]), pair p aligns a_idx[p] against b_idx[p]; results to device
+ * arrays (span: ai, aj, bi, bj per pair; CIGAR right-aligned in a slot of
+ * len(A) + len(B) bytes at cigar_off[p]).  host_len / host_a_idx / host_b_idx
+ * are host copies the library sizes its workspace with. */
+typedef struct ga_msa_dev_batch {
+    const float*   cols;          /* n_cols x alpha */
+    const float*   occ;
+    int64_t        n_cols;
+    const int64_t* col_off;
+    const int32_t* len;
+    int32_t        n_profiles;
+    const int32_t* a_idx;
+    const int32_t* b_idx;
+    int64_t        n_pairs;
+    float*         score;
+    int32_t*       span;
+    uint8_t*       cigar;
+    const int64_t* cigar_off;
+    int32_t*       cigar_len;
+    const int32_t* host_len;
+    const int32_t* host_a_idx;
+    const int32_t* host_b_idx;
+} ga_msa_dev_batch;
+int ga_msa_pp_device(const ga_msa_dev_batch* b, const ga_msa_params* p, void* stream, const char** error);
+
 #ifdef __cplusplus
 }
 #endif
