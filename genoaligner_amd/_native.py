@@ -33,7 +33,7 @@ EXPORTS = (
     "ga_wfa_autotune", "ga_wfa_set_tuning", "ga_wfa_get_tuning",
     "ga_fasta_parse", "ga_fasta_read_file", "ga_fasta_count", "ga_fasta_record", "ga_fasta_free",
     "ga_fasta_sizes", "ga_fasta_export", "ga_fasta_device", "ga_gene_qc_walk",
-    "ga_msa_align_profiles", "ga_msa_profile_from_rows", "ga_msa_pp_device",
+    "ga_msa_align_profiles", "ga_msa_profile_from_rows", "ga_msa_pp_device", "ga_kmer_distances", "ga_nj_tree",
 )
 
 
@@ -229,6 +229,11 @@ def lib() -> C.CDLL:
         L.ga_msa_profile_from_rows.restype = C.c_int
         L.ga_msa_pp_device.argtypes = [C.POINTER(MsaDevBatch), C.POINTER(MsaParams), C.c_void_p, cpp]
         L.ga_msa_pp_device.restype = C.c_int
+        L.ga_kmer_distances.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                        C.c_void_p, cpp]
+        L.ga_kmer_distances.restype = C.c_int
+        L.ga_nj_tree.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, cpp]
+        L.ga_nj_tree.restype = C.c_int
         L.ga_route_bucket_bound.argtypes = [C.c_int32]
         L.ga_route_bucket_bound.restype = C.c_int32
         _lib = L

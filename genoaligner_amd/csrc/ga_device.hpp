@@ -256,6 +256,32 @@ int64_t msa_dir_words(int M, int N);
 hipError_t msa_sparse_launch(const MsaSparseLaunch& S, hipStream_t st);
 hipError_t msa_pp_launch(const MsaLaunch& L, int n_pairs, hipStream_t st, LaunchHooks* hooks = nullptr);
 
+// k-mer distances and neighbour joining (nj.hip).
+struct KmerLaunch {
+    const uint8_t* seq; const int64_t* off; const int32_t* len;
+    int32_t n, k, alpha;
+    int64_t words;               // bitset words per sequence: ceil(alpha^k / 32)
+    uint32_t* bits;              // n x words, zeroed by the caller
+    int32_t* count;              // n: |K_i|
+    float* dist;                 // n (n - 1) / 2 packed upper triangle
+};
+hipError_t kmer_distances_launch(const KmerLaunch& K, int max_len, hipStream_t st, LaunchHooks* hooks = nullptr);
+
+struct NjBest {                  // a merge candidate; a < 0: none
+    double q;
+    int32_t a, b;
+};
+struct NjLaunch {
+    int32_t n;
+    const float* packed;         // n (n - 1) / 2 distances
+    double* d;                   // n x n
+    double* r;                   // n
+    NjBest* part;                // n per-row first minima
+    int32_t* alive0; int32_t* alive1; int32_t* slot_node;   // n each
+    int32_t* children;           // 2 (n - 1): children of internal nodes n .. 2n-2
+};
+hipError_t nj_launch(const NjLaunch& J, hipStream_t st, LaunchHooks* hooks = nullptr);
+
 // Run chunk(k0, k1) over [0, n) on the host work pool, about `grain` indices a
 // chunk; returns when every chunk has run.
 void host_parallel_for(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& chunk);

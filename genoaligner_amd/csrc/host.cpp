@@ -2401,3 +2401,108 @@ int ga_msa_pp_device(const ga_msa_dev_batch* b, const ga_msa_params* p, void* st
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// guide tree: k-mer distances and neighbour joining
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int ga_kmer_distances(const char* seqs, const int64_t* off, const int32_t* len, int32_t n, int32_t k, int32_t alpha,
+                      float* D, const char** error)
+{
+    if (error) *error = nullptr;
+    auto refuse = [&](const char* m) {
+        if (error) *error = m;
+        return GA_INVALID_ARGUMENT;
+    };
+    if (n <= 1) return GA_OK;
+    if (alpha != 4 && alpha != 20) return refuse("k-mer alphabet must be 4 (DNA) or 20 (protein)");
+    if (n > 65535) return refuse("more than 65,535 sequences");
+    if (!off || !len || !D) return refuse("null sequence or distance array");
+    if (k < 1) return refuse("k must be >= 1");
+    double codes = 1;
+    for (int q = 0; q < k; ++q) codes *= alpha;
+    if (codes > double(1 << 24)) return refuse("alpha^k above 2^24 k-mer codes");
+    int64_t total = 0;
+    int32_t max_len = 0;
+    for (int32_t s = 0; s < n; ++s) {
+        if (len[s] < 0) return refuse("negative sequence length");
+        if (len[s] > 0 && !seqs) return refuse("null sequence with a positive length");
+        total = std::max<int64_t>(total, off[s] + len[s]);
+        max_len = std::max(max_len, len[s]);
+    }
+    std::call_once(g_pool_once, init_pool);
+    const int64_t words = (int64_t(codes) + 31) / 32;
+    hipStream_t st = acquire_stream();
+    if (!st) { if (error) *error = "hipStreamCreate"; return GA_DEVICE_ERROR; }
+    DevArena Da;
+    Da.st = st;
+    std::vector<uint8_t> bytes(static_cast<size_t>(std::max<int64_t>(total, 1)));
+    if (total > 0) std::memcpy(bytes.data(), seqs, size_t(total));
+    ga::KmerLaunch K{};
+    K.seq = Da.upload(bytes);
+    K.off = Da.upload(std::vector<int64_t>(off, off + n));
+    K.len = Da.upload(std::vector<int32_t>(len, len + n));
+    K.n = n; K.k = k; K.alpha = alpha; K.words = words;
+    K.bits = Da.alloc<uint32_t>(size_t(n) * size_t(words));
+    K.count = Da.alloc<int32_t>(size_t(n));
+    const int64_t np = int64_t(n) * (n - 1) / 2;
+    K.dist = Da.alloc<float>(size_t(np));
+    const char* err = Da.ok ? nullptr : "hipMalloc/hipMemcpy (k-mer buffers)";
+    t_timer.reset();
+    if (!err && int64_t(n) * words > INT32_MAX) err = "k-mer bitsets larger than 2^31 words";
+    if (!err && ga::zero_i32_launch(reinterpret_cast<int32_t*>(K.bits), int(int64_t(n) * words), st) != hipSuccess)
+        err = "k-mer kernel launch";
+    if (!err && ga::kmer_distances_launch(K, max_len, st, &t_timer_hooks) != hipSuccess) err = "k-mer kernel launch";
+    if (!err && hipMemcpyAsync(D, K.dist, size_t(np) * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+        err = "hipMemcpy(distances)";
+    Da.release();
+    const hipError_t se = hipStreamSynchronize(st);
+    if (!err && se != hipSuccess) err = hipGetErrorString(se);
+    release_stream(st);
+    if (err) {
+        if (error) *error = err;
+        return GA_DEVICE_ERROR;
+    }
+    return GA_OK;
+}
+
+int ga_nj_tree(const float* D, int32_t n, int32_t* children, const char** error)
+{
+    if (error) *error = nullptr;
+    if (n <= 1) return GA_OK;
+    if (!D || !children) { if (error) *error = "null distance or tree array"; return GA_INVALID_ARGUMENT; }
+    if (n > 65535) { if (error) *error = "more than 65,535 sequences"; return GA_INVALID_ARGUMENT; }
+    std::call_once(g_pool_once, init_pool);
+    hipStream_t st = acquire_stream();
+    if (!st) { if (error) *error = "hipStreamCreate"; return GA_DEVICE_ERROR; }
+    DevArena Da;
+    Da.st = st;
+    const int64_t np = int64_t(n) * (n - 1) / 2;
+    ga::NjLaunch J{};
+    J.n = n;
+    J.packed = Da.upload(std::vector<float>(D, D + np));
+    J.d = Da.alloc<double>(size_t(n) * size_t(n));
+    J.r = Da.alloc<double>(size_t(n));
+    J.part = Da.alloc<ga::NjBest>(size_t(n));
+    J.alive0 = Da.alloc<int32_t>(size_t(n));
+    J.alive1 = Da.alloc<int32_t>(size_t(n));
+    J.slot_node = Da.alloc<int32_t>(size_t(n));
+    J.children = Da.alloc<int32_t>(2 * size_t(n));
+    const char* err = Da.ok ? nullptr : "hipMalloc/hipMemcpy (NJ buffers)";
+    t_timer.reset();
+    if (!err && ga::nj_launch(J, st, &t_timer_hooks) != hipSuccess) err = "NJ kernel launch";
+    if (!err && hipMemcpyAsync(children, J.children, size_t(n - 1) * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
+        err = "hipMemcpy(tree)";
+    Da.release();
+    const hipError_t se = hipStreamSynchronize(st);
+    if (!err && se != hipSuccess) err = hipGetErrorString(se);
+    release_stream(st);
+    if (err) {
+        if (error) *error = err;
+        return GA_DEVICE_ERROR;
+    }
+    return GA_OK;
+}
+
+}  // extern "C"

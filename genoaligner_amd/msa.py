@@ -254,3 +254,54 @@ class MsaDeviceBatch:
             end = int(self.slot_off[q] + self.slot[q])
             out.append(AlignResult(float(sc[q]), raw[end - int(ln[q]): end].decode(), *(int(x) for x in sp[q])))
         return out
+
+
+# ---------------------------------------------------------------- guide tree
+@dataclass
+class Tree:
+    """genomsa::Tree (msa.hpp:338-341): nodes[0 .. n-1] are the leaves
+    (-1, -1), nodes[n ..] the internal nodes' (left, right) children."""
+    nodes: List[Tuple[int, int]]
+    root: int
+
+
+def kmer_distances(seqs: Sequence, k: int = 5, alpha: int = 4) -> np.ndarray:
+    """genomsa::kmer_distances: packed upper-triangle float32 distances."""
+    bs = [s.encode("latin-1") if isinstance(s, str) else bytes(s) for s in seqs]
+    n = len(bs)
+    if n < 2:
+        return np.zeros(0, dtype=np.float32)
+    lens = np.array([len(s) for s in bs], dtype=np.int32)
+    off = np.zeros(n, dtype=np.int64)
+    off[1:] = np.cumsum(lens.astype(np.int64))[:-1]
+    buf = np.frombuffer(b"".join(bs) + b"\0", dtype=np.uint8)
+    D = np.zeros(n * (n - 1) // 2, dtype=np.float32)
+    err = C.c_char_p()
+    rc = N.lib().ga_kmer_distances(buf.ctypes.data, off.ctypes.data, lens.ctypes.data, n, k, alpha, D.ctypes.data,
+                                   C.byref(err))
+    if rc != N.GA_OK:
+        raise MsaError((err.value or b"k-mer distances failed").decode())
+    return D
+
+
+def nj_children(D: np.ndarray, n: int) -> np.ndarray:
+    """ga_nj_tree: int32 [n-1, 2] children of internal nodes n .. 2n-2."""
+    if n < 2:
+        return np.zeros((0, 2), dtype=np.int32)
+    D = np.ascontiguousarray(D, dtype=np.float32)
+    if D.shape[0] != n * (n - 1) // 2:
+        raise MsaError("distance array is not n (n - 1) / 2 long")
+    ch = np.zeros(2 * (n - 1), dtype=np.int32)
+    err = C.c_char_p()
+    rc = N.lib().ga_nj_tree(D.ctypes.data, n, ch.ctypes.data, C.byref(err))
+    if rc != N.GA_OK:
+        raise MsaError((err.value or b"neighbour joining failed").decode())
+    return ch.reshape(n - 1, 2)
+
+
+def nj_tree(D: np.ndarray, n: int) -> Tree:
+    """genomsa::nj_tree (msa_ref.cpp:904-947) on the device."""
+    if n == 1:
+        return Tree([(-1, -1)], 0)
+    ch = nj_children(D, n)
+    return Tree([(-1, -1)] * n + [tuple(int(x) for x in c) for c in ch], 2 * n - 2)

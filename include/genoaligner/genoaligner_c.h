@@ -470,6 +470,22 @@ typedef struct ga_msa_dev_batch {
 } ga_msa_dev_batch;
 int ga_msa_pp_device(const ga_msa_dev_batch* b, const ga_msa_params* p, void* stream, const char** error);
 
+/* Replaces genomsa::kmer_distances / kmer_distances_mt (msa.hpp:329-333,
+ * msa_ref.cpp:802-898): the packed upper triangle (row i, then j > i, index
+ * i*n - i*(i+1)/2 + j - i - 1) of D = 1 - |K_i & K_j| / min(|K_i|, |K_j|)
+ * (1 when the smaller set is empty) over the sets of k-mers of unambiguous
+ * letters, sequence s at seqs[off[s] .. + len[s]).  The same floats bit for
+ * bit.  Limits: alpha 4 or 20, alpha^k <= 2^24, n <= 65,535. */
+int ga_kmer_distances(const char* seqs, const int64_t* off, const int32_t* len, int32_t n, int32_t k,
+                      int32_t alpha, float* D, const char** error);
+
+/* Replaces genomsa::nj_tree / nj_tree_gpu (msa.hpp:341-357, msa_ref.cpp:
+ * 904-947, nj_kernel_impl.hip:84-187): deterministic neighbour joining over
+ * the packed distances, the same tree: children[2*(u-n)] and
+ * children[2*(u-n)+1] are the child node ids of internal node u, n <= u <=
+ * 2n-2 (leaves 0 .. n-1, root 2n-2).  n <= 65,535; n < 2 writes nothing. */
+int ga_nj_tree(const float* D, int32_t n, int32_t* children, const char** error);
+
 #ifdef __cplusplus
 }
 #endif
