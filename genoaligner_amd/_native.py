@@ -225,7 +225,7 @@ def lib() -> C.CDLL:
                                             C.POINTER(MsaParams), C.POINTER(MsaPPRes), C.c_void_p, C.c_int64, cpp]
         L.ga_msa_align_profiles.restype = C.c_int
         L.ga_msa_profile_from_rows.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
-                                               C.c_void_p]
+                                               C.c_void_p, C.c_void_p]
         L.ga_msa_profile_from_rows.restype = C.c_int
         L.ga_msa_pp_device.argtypes = [C.POINTER(MsaDevBatch), C.POINTER(MsaParams), C.c_void_p, cpp]
         L.ga_msa_pp_device.restype = C.c_int

@@ -2189,7 +2189,8 @@ const char* enqueue_msa(const ga_msa_params& p, const float* cols, const float* 
 
 extern "C" {
 
-int ga_msa_profile_from_rows(const char* rows, int32_t nrows, int32_t L, int32_t alpha, float* cols, float* occ)
+int ga_msa_profile_from_rows(const char* rows, int32_t nrows, int32_t L, int32_t alpha, float* cols, float* occ,
+                             float* gap)
 {
     // msa_ref.cpp:21-106 (symbol fractions) and :773-794 (counts over rows)
     if ((alpha != 4 && alpha != 20) || nrows <= 0 || L < 0 || (L > 0 && (!rows || !cols || !occ)))
@@ -2240,6 +2241,7 @@ int ga_msa_profile_from_rows(const char* rows, int32_t nrows, int32_t L, int32_t
     for (int i = 0; i < L; ++i) {
         for (int s = 0; s < alpha; ++s) cols[size_t(i) * alpha + s] = 0.f;
         occ[i] = 0.f;
+        if (gap) gap[i] = 0.f;
     }
     float cnt[20];
     for (int r = 0; r < nrows; ++r)
@@ -2247,10 +2249,13 @@ int ga_msa_profile_from_rows(const char* rows, int32_t nrows, int32_t L, int32_t
             if (counts(rows[size_t(r) * L + i], cnt)) {
                 for (int s = 0; s < alpha; ++s) cols[size_t(i) * alpha + s] += cnt[s];
                 occ[i] += 1.f;
+            } else if (gap) {
+                gap[i] += 1.f;
             }
     for (int i = 0; i < L; ++i) {
         for (int s = 0; s < alpha; ++s) cols[size_t(i) * alpha + s] /= float(nrows);
         occ[i] /= float(nrows);
+        if (gap) gap[i] /= float(nrows);
     }
     return GA_OK;
 }

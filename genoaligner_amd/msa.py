@@ -106,7 +106,7 @@ def profile_from_rows(rows: Sequence[str], alpha: int = 4) -> Profile:
     cols = np.zeros((L, alpha), dtype=np.float32)
     occ = np.zeros(L, dtype=np.float32)
     rc = N.lib().ga_msa_profile_from_rows("".join(rows).encode("latin-1"), len(rows), L, alpha,
-                                          cols.ctypes.data, occ.ctypes.data)
+                                          cols.ctypes.data, occ.ctypes.data, None)
     if rc != N.GA_OK:
         raise MsaError("invalid profile rows or alphabet")
     return Profile(cols, occ, list(rows), alpha)

@@ -440,8 +440,10 @@ int ga_msa_align_profiles(const ga_msa_profile* A, const ga_msa_profile* B, int6
 /* genomsa::profile_from_seq / profile_update_counts (msa.hpp:322-325,
  * msa_ref.cpp:751-794): the profile of nrows aligned rows of L bytes (row r
  * at rows + r * L); IUPAC letters (alpha 4) and B / Z / J / X / O / U (alpha
- * 20) count fractionally, '-' and unknown letters as gap. */
-int ga_msa_profile_from_rows(const char* rows, int32_t nrows, int32_t L, int32_t alpha, float* cols, float* occ);
+ * 20) count fractionally, '-' and unknown letters as gap.  gap (optional)
+ * receives the gap fraction the reference keeps in its slot `alpha`. */
+int ga_msa_profile_from_rows(const char* rows, int32_t nrows, int32_t L, int32_t alpha, float* cols, float* occ,
+                             float* gap);
 
 /* Device-resident batch (EXTENSION: no reference counterpart): profiles in
  * one column table on the device (profile k at columns col_off[k] ..

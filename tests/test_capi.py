@@ -41,7 +41,8 @@ def test_struct_layouts_match_header(tmp_path):
         "ga_align_req": N.AlignReq, "ga_align_res": N.AlignRes, "ga_sw_scoring": N.SWScoringC,
         "ga_sw_req": N.SWReq, "ga_sw_res": N.SWRes, "ga_wfa_dev_batch": N.WfaDevBatch,
         "ga_sw_dev_batch": N.SwDevBatch, "ga_route_stats": N.RouteStats, "ga_wfa_tuning": N.WfaTuning,
-        "ga_fasta_dev_out": N.FastaDevOut, "ga_qc_row": N.QcRow,
+        "ga_fasta_dev_out": N.FastaDevOut, "ga_qc_row": N.QcRow, "ga_msa_params": N.MsaParams,
+        "ga_msa_profile": N.MsaProfile, "ga_msa_pp_res": N.MsaPPRes, "ga_msa_dev_batch": N.MsaDevBatch,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for cname, py in checks.items():

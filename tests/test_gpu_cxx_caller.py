@@ -49,3 +49,39 @@ def test_caller_golden_vectors_and_examples(gpu, oracle):
             assert (int(score), int(si), int(sj), int(ei), int(ej), cigar) == \
                 oracle.sw_trace(t, p, (2, -3, 3, 1))
             assert rs == wf == "1"
+
+
+MSA_CALLER = os.path.join(ROOT, "tests", "cxx", "_build", "msa_caller")
+
+
+def test_msa_caller_links_and_reports_no_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    if not os.path.exists(MSA_CALLER):
+        pytest.fail("tests/cxx/_build/msa_caller is not built (run __graft_entry__.build())")
+    r = subprocess.run([MSA_CALLER, GOLDEN], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 3 and r.stdout.strip() == "no device"
+
+
+@pytest.mark.gpu
+def test_msa_caller_golden_vectors(gpu, oracle):
+    """include/genoaligner/msa/msa.hpp over the MSA golden vectors, then a
+    four-sequence progressive merge checked row by row against the oracle."""
+    r = subprocess.run([MSA_CALLER, GOLDEN], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    lines = r.stdout.splitlines()
+    assert lines[-1].startswith("GOLDEN checked") and lines[-1].endswith(" bad 0")
+    rows = [ln[4:] for ln in lines if ln.startswith("MSA ")]
+    assert len(rows) == 4 and len({len(x) for x in rows}) == 1
+    seqs = ["ACGTACGTTTGACCA", "ACGTACGTTGACCA", "ACGAACGTTTGACGA", "TTACGTACGTTTGA"]
+    assert sorted(x.replace("-", "") for x in rows) == sorted(seqs)
+    # replay the merge with the oracle
+    ch = oracle.nj_tree(oracle.kmer_distances(seqs, 3), 4)
+    node = [[s] for s in seqs]
+    P = oracle.MsaParams()
+    for l, rr in ch.tolist():
+        A, B = node[l], node[rr]
+        sc, span, cg = oracle.msa_align(oracle.msa_profile(A), oracle.msa_profile(B), P)
+        node.append(oracle.msa_merge_rows(A, B, span, cg))
+    assert rows == node[-1]
