@@ -261,7 +261,8 @@ def cpu_baseline_wfa(ps, want, threads, oracle, label):
     return cpu_timed(prepare, ps.n, lambda idx: float(len(idx)), "pairs/s", label, threads, seed=3)
 
 
-SECTIONS = ("sw", "sw_host", "wfa", "mixed", "routing", "protein", "fasta", "gene_qc", "serving")
+SECTIONS = ("sw", "sw_host", "wfa", "mixed", "routing", "protein", "fasta", "gene_qc", "serving", "msa_pp",
+            "guide_tree")
 
 
 def mean_pos(xs):
@@ -315,6 +316,8 @@ def main():
     ap.add_argument("--fasta-mib", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=100.0, help="requests per second (Poisson), lowest load")
     ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--msa-pairs", type=int, default=4096)
+    ap.add_argument("--tree-seqs", type=int, default=1000)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, to scale hardware byte counters)")
@@ -787,6 +790,16 @@ def main():
             }
         log(f"gene_qc: {qc_line}")
 
+    # ---------------- MSA: profile-profile Gotoh over a level of pairs ----------------
+    msa_line = None
+    if "msa_pp" in only:
+        msa_line = msa_section(args, W, O, threads, cpu_on, lib)
+
+    # ---------------- MSA guide tree: k-mer distances + neighbour joining ----------------
+    tree_line = None
+    if "guide_tree" in only:
+        tree_line = guide_tree_section(args, W, O, cpu_on, lib)
+
     # ---------------- serving: Poisson arrivals, continuous batching ----------------
     serve_line = None
     if "serving" in only:
@@ -823,6 +836,8 @@ def main():
         "serving": serve_line,
         "fasta_ingest": fasta_line,
         "gene_qc": qc_line,
+        "msa_pp": msa_line,
+        "guide_tree": tree_line,
     }
     print(json.dumps(line), flush=True)
 
@@ -831,6 +846,148 @@ def _timed(fn):
     t0 = time.perf_counter()
     fn()
     return time.perf_counter() - t0
+
+
+def msa_section(args, W, O, threads, cpu_on, lib):
+    """A level of a progressive MSA: profile pairs of related DNA families
+    (600..1,400 columns, 1..4 rows per profile) through the device entry
+    point, default genomsa parameters; GCUPS (cells = sum of M x N) and the
+    fp32 VALU roofline at 9 ops per cell plus 3 per symbol-fraction term."""
+    import torch
+    from genoaligner_amd import msa as M
+    t0 = time.perf_counter()
+    pairs_rows = W.msa_profile_pairs(n=args.msa_pairs)
+    P = M.Params()
+    profs, idx = [], []
+    for a, b in pairs_rows:
+        profs += [M.profile_from_rows(a), M.profile_from_rows(b)]
+        idx.append((len(profs) - 2, len(profs) - 1))
+    dev = M.MsaDeviceBatch(profs, idx, P)
+    log(f"msa_pp workload ready ({time.perf_counter() - t0:.1f} s)")
+    cells = sum(profs[a].ncols() * profs[b].ncols() for a, b in idx)
+    nz = [np.count_nonzero(p.cols, axis=1) for p in profs]
+    terms = sum(float(nz[a].sum()) * float(nz[b].sum()) for a, b in idx)
+    ops = 9.0 * cells + 3.0 * terms
+    wall, step_ms, kern = time_steps(dev.run, args.steps, args.warmup, lib)
+    km = mean_pos(kern)
+    got = dev.results()
+    rng = np.random.default_rng(7)
+    sample = rng.choice(len(idx), size=min(40, len(idx)), replace=False)
+    OP = O.MsaParams()
+    bad = 0
+    for q in sample:
+        a, b = idx[q]
+        w = O.msa_align((profs[a].cols, profs[a].occ), (profs[b].cols, profs[b].occ), OP)
+        g = got[q]
+        if (np.float32(g.score).tobytes(), (g.ai, g.aj, g.bi, g.bj), g.cigar) != (np.float32(w[0]).tobytes(), w[1], w[2]):
+            bad += 1
+    line = {
+        "workload": f"msa level {len(idx)} profile pairs, DNA families, 600..1400 columns, 1..4 rows per profile, "
+                    "default genomsa parameters (semiglobal, position-specific gap penalties), score + span + "
+                    "column CIGAR; profiles resident in HBM",
+        "gcups": round(cells / (wall / args.steps) / 1e9, 2), "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "cells": cells, "kernel_ms": round(km, 3) if km else None,
+        "roofline": {"bound": "valu", "unit": "TOP/s", "peak": PEAK_INT32_TOPS,
+                     "ops_per_cell": round(ops / cells, 2),
+                     "achieved": round(ops / (km * 1e-3) / 1e12, 3) if km else None,
+                     "frac": round(ops / (km * 1e-3) / 1e12 / PEAK_INT32_TOPS, 4) if km else None,
+                     "kernel": "msa_pp_kernel<4> (one wave per pair, fp32)"},
+        "p50_ms": pctl(step_ms, 50), "p99_ms": pctl(step_ms, 99),
+        "mismatches": bad, "checked": int(len(sample)),
+    }
+    if cpu_on:
+        import ctypes as C
+        L = O._msa_lib()
+        cols = np.concatenate([p.cols for p in profs]).astype(np.float32)
+        occ = np.concatenate([p.occ for p in profs]).astype(np.float32)
+        lens = np.array([p.ncols() for p in profs], dtype=np.int32)
+        coff = np.zeros(len(profs), dtype=np.int64)
+        coff[1:] = np.cumsum(lens.astype(np.int64))[:-1]
+        ha = np.array([a for a, _ in idx], dtype=np.int32)
+        hb = np.array([b for _, b in idx], dtype=np.int32)
+        pc = OP.c()
+
+        def prepare(sel):
+            a, b = np.ascontiguousarray(ha[sel]), np.ascontiguousarray(hb[sel])
+            n = len(sel)
+            score = np.zeros(n, np.float32)
+            span = np.zeros(4 * n, np.int32)
+            slot = lens[a].astype(np.int64) + lens[b]
+            so = np.zeros(n, np.int64)
+            so[1:] = np.cumsum(slot)[:-1]
+            cig = np.zeros(int(slot.sum()) + 1, np.uint8)
+            cl = np.zeros(n, np.int32)
+
+            def go(th):
+                t0 = time.perf_counter()
+                rc = L.ga_oracle_msa_align_batch(cols.ctypes.data, occ.ctypes.data, coff.ctypes.data, lens.ctypes.data,
+                                                 a.ctypes.data, b.ctypes.data, n, C.byref(pc), score.ctypes.data,
+                                                 span.ctypes.data, cig.ctypes.data, so.ctypes.data, cl.ctypes.data, th)
+                assert rc == 0
+                return time.perf_counter() - t0
+            return go
+
+        cw = lambda sel: float((lens[ha[sel]].astype(np.int64) * lens[hb[sel]]).sum()) / 1e9
+        line["cpu_baseline"] = cpu_timed(prepare, len(idx), cw, "GCUPS", "the msa level's profile pairs", threads,
+                                         seed=8)
+    log(f"msa_pp: {line}")
+    del dev
+    torch.cuda.synchronize()
+    return line
+
+
+def guide_tree_section(args, W, O, cpu_on, lib):
+    """The guide-tree step: 5-mer distances of n related DNA sequences and
+    neighbour joining over them (the host entry points: sequences in, tree
+    out), with each kernel's time from the library's events; NJ against the
+    bytes its rounds read (row sums and the candidate scan read the live
+    block of the double matrix, 1.5 m^2 x 8 bytes per round)."""
+    from genoaligner_amd import msa as M
+    n = args.tree_seqs
+    seqs = W.sequence_family_set(n=n)
+    M.nj_children(M.kmer_distances(seqs[:50], 5), 50)            # warm-up
+    ts_k, ts_n, kk, kn = [], [], [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        D = M.kmer_distances(seqs, 5)
+        ts_k.append(time.perf_counter() - t0)
+        kk.append(lib.ga_last_kernel_ms())
+        t0 = time.perf_counter()
+        ch = M.nj_children(D, n)
+        ts_n.append(time.perf_counter() - t0)
+        kn.append(lib.ga_last_kernel_ms())
+    want_D = O.kmer_distances(seqs, 5)
+    t0 = time.perf_counter()
+    want_ch = O.nj_tree(want_D, n)
+    cpu_nj = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.kmer_distances(seqs, 5)
+    cpu_k = time.perf_counter() - t0
+    nj_bytes = sum(1.5 * m * m * 8 for m in range(3, n + 1))
+    knm = mean_pos(kn)
+    line = {
+        "workload": f"guide tree of {n} DNA sequences (40 families, ~1,200 bp, up to 12% divergence): 5-mer "
+                    "distances + neighbour joining, host entry points",
+        "ms": round(float(np.median(ts_k)) * 1e3 + float(np.median(ts_n)) * 1e3, 2),
+        "kmer_ms": round(float(np.median(ts_k)) * 1e3, 3), "kmer_kernel_ms": round(mean_pos(kk), 3) if mean_pos(kk) else None,
+        "nj_ms": round(float(np.median(ts_n)) * 1e3, 2), "nj_kernel_ms": round(knm, 2) if knm else None,
+        "nj_rounds": n - 2,
+        "roofline": {"bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS, "algorithmic_bytes": int(nj_bytes),
+                     "achieved": round(nj_bytes / (knm * 1e-3) / 1e9, 1) if knm else None,
+                     "frac": round(nj_bytes / (knm * 1e-3) / 1e9 / PEAK_HBM_GBS, 4) if knm else None,
+                     "kernel": "nj rounds (rowsum, argmin, merge), latency-bound: 3 launches per round"},
+        "distances_equal_oracle": bool(np.array_equal(D, want_D)),
+        "tree_equal_oracle": bool(np.array_equal(ch, want_ch)),
+    }
+    if cpu_on:
+        line["cpu_baseline"] = {
+            "value": round((cpu_k + cpu_nj) * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+            "kmer_ms": round(cpu_k * 1e3, 1), "nj_ms": round(cpu_nj * 1e3, 1),
+            "sample": f"the oracle's kmer_distances and nj_tree on the same {n} sequences, one host thread, one run",
+            "host": host_info(),
+        }
+    log(f"guide_tree: {line}")
+    return line
 
 
 def serving_section(args, W, O, tab62, threads, cpu_on):

@@ -19,7 +19,7 @@ Workloads (sizes as used by the benchmark and tests):
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import Optional
+from typing import List, Optional
 
 import numpy as np
 
@@ -351,3 +351,76 @@ def gene_family(n: int = 5000, codons: int = 380, seed: int = 0):
     ref = (b">frag stop-riddled fragment\n" + frag + b"\n>bait0 clean\n" + bait + b"\n>bait1 shifted\nC" +
            bait + b"\n")
     return b"\n".join(recs) + b"\n", ref
+
+
+# ---------------------------------------------------------------- MSA
+def aligned_family(rng: np.random.Generator, L: int, rows: int, sub: float = 0.06, gap: float = 0.03,
+                   ins: float = 0.02) -> List[str]:
+    """`rows` aligned rows of one family: an ancestor of L bases, per row and
+    column a substitution (rate `sub`) or a gap ('-', rate `gap`), and
+    insertion columns (rate `ins` per column) where one row holds a base and
+    the others a gap -- the shape of a progressive merge's profile."""
+    anc = ACGT[rng.integers(0, 4, L, dtype=np.uint8)]
+    cols = rng.random(L) < ins
+    width = L + int(cols.sum())
+    out = np.empty((rows, width), dtype=np.uint8)
+    keep = np.ones(width, dtype=bool)
+    pos = np.arange(L) + np.cumsum(cols)           # ancestor column positions in the aligned rows
+    keep[:] = False
+    keep[pos] = True
+    for r in range(rows):
+        row = np.full(width, ord("-"), dtype=np.uint8)
+        b = anc.copy()
+        m = rng.random(L)
+        s = m < sub
+        b[s] = ACGT[(_CODE[b[s]] + rng.integers(1, 4, int(s.sum()), dtype=np.uint8)) & 3]
+        b[(m >= sub) & (m < sub + gap)] = ord("-")
+        row[pos] = b
+        ins_cols = np.flatnonzero(~keep)
+        owner = rng.integers(0, rows, len(ins_cols))
+        mine = ins_cols[owner == r]
+        row[mine] = ACGT[rng.integers(0, 4, len(mine), dtype=np.uint8)]
+        out[r] = row
+    # drop columns that are gaps in every row
+    nz = (out != ord("-")).any(axis=0)
+    return [bytes(x).decode() for x in out[:, nz]]
+
+
+def msa_profile_pairs(n: int = 4096, lo: int = 600, hi: int = 1400, rows=(1, 4), seed: int = 20261208):
+    """Profile pairs of a progressive-MSA level: pair q aligns two profiles
+    of one family (1..4 rows each, lengths uniform lo..hi), returned as row
+    lists [(rows_a, rows_b)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        L = int(rng.integers(lo, hi + 1))
+        ra, rb = int(rng.integers(rows[0], rows[1] + 1)), int(rng.integers(rows[0], rows[1] + 1))
+        fam = aligned_family(rng, L, ra + rb)
+        a = [r for r in fam[:ra]]
+        b = [r for r in fam[ra:]]
+        # each side keeps only its own non-gap columns (a profile of its rows)
+        def squeeze(rs):
+            arr = np.frombuffer("".join(rs).encode(), np.uint8).reshape(len(rs), -1)
+            k = (arr != ord("-")).any(axis=0)
+            return [bytes(x).decode() for x in arr[:, k]]
+        out.append((squeeze(a), squeeze(b)))
+    return out
+
+
+def sequence_family_set(n: int = 1000, L: int = 1200, families: int = 40, rate: float = 0.12,
+                        seed: int = 20261209) -> List[str]:
+    """n DNA sequences from `families` ancestors (substitutions and indels
+    at up to `rate`), the input of a guide tree."""
+    rng = np.random.default_rng(seed)
+    ancs = [ACGT[rng.integers(0, 4, L, dtype=np.uint8)] for _ in range(families)]
+    out = []
+    for q in range(n):
+        a = ancs[q % families]
+        r = rng.uniform(0.0, rate)
+        m = rng.random(L)
+        b = a.copy()
+        s = m < r * 0.6
+        b[s] = ACGT[(_CODE[b[s]] + rng.integers(1, 4, int(s.sum()), dtype=np.uint8)) & 3]
+        keep = ~((m >= r * 0.6) & (m < r * 0.8))
+        out.append(bytes(b[keep]).decode())
+    return out
