@@ -30,6 +30,8 @@
 // wave's 64 words one contiguous 256-byte store.
 #include "ga_device.hpp"
 
+#include <type_traits>
+
 namespace ga {
 namespace {
 
@@ -87,36 +89,57 @@ __device__ __forceinline__ void offer(Best& b, float v, int key)
     if (v > b.v || (v == b.v && key < b.key)) { b.v = v; b.key = key; }
 }
 
-// The byte of cell (i, j) in the skewed direction layout.
-__device__ __forceinline__ uint32_t dir_at(const uint32_t* d, int tq, int i, int j)
+// The walk's window over the skewed direction words: one strip, 16 word
+// rows (64 steps) of all 64 lanes, loaded by the whole wave when the walk
+// leaves it.  The walk itself runs on every lane with wave-uniform state.
+constexpr int kWinRows = 16;
+
+struct WalkWin {
+    uint32_t* w;         // kWinRows x 64 words (LDS)
+    int s = -1, lo = 0;  // strip and first word row held
+};
+
+__device__ __forceinline__ uint32_t dir_at(const uint32_t* d, int tq, WalkWin& W, int lane, int i, int j)
 {
-    const int s = (j - 1) >> 6, l = (j - 1) & 63, t = i + l;
-    return (d[(int64_t(s) * tq + (t >> 2)) * 64 + l] >> (8 * (t & 3))) & 0xffu;
+    const int s = (j - 1) >> 6, l = (j - 1) & 63, t = i + l, wr = t >> 2;
+    if (s != W.s || wr < W.lo || wr >= W.lo + kWinRows) {
+        W.s = s;
+        W.lo = wr - (kWinRows - 1) > 0 ? wr - (kWinRows - 1) : 0;
+        __syncthreads();
+        for (int q = 0; q < kWinRows; ++q)
+            if (W.lo + q < tq) W.w[q * 64 + lane] = d[(int64_t(s) * tq + W.lo + q) * 64 + lane];
+        __syncthreads();
+    }
+    const uint32_t word = W.w[(wr - W.lo) * 64 + l];
+    return (uint32_t(__builtin_amdgcn_readfirstlane(int(word))) >> (8 * (t & 3))) & 0xffu;
 }
 
 // Three-state walk from the end cell (msa_ref.cpp:1131-1156); emits the
-// column ops right-aligned in the pair's CIGAR slot.
-__device__ void msa_walk(const uint32_t* d, int tq, int ei, int ej, char est, uint8_t* slot_end, int* oi, int* oj,
-                         int* used_out)
+// column ops right-aligned in the pair's CIGAR slot (lane 0 stores).
+__device__ void msa_walk(const uint32_t* d, int tq, WalkWin& W, int lane, int ei, int ej, char est,
+                         uint8_t* slot_end, int* oi, int* oj, int* used_out)
 {
     int i = ei, j = ej, used = 0;
     char st = est;
     while (i > 0 || j > 0) {
         if (st == 'M') {
             if (i == 0 || j == 0) break;
-            slot_end[-1 - used++] = 'M';
-            const uint32_t h = dir_at(d, tq, i, j) & 3u;
+            if (lane == 0) slot_end[-1 - used] = 'M';
+            ++used;
+            const uint32_t h = dir_at(d, tq, W, lane, i, j) & 3u;
             --i; --j;
             st = h == 0 ? 'M' : (h == 1 ? 'I' : 'D');
         } else if (st == 'I') {
             if (i == 0) break;
-            slot_end[-1 - used++] = 'I';
-            if (j > 0) st = (dir_at(d, tq, i, j) & 4u) ? 'I' : 'M';
+            if (lane == 0) slot_end[-1 - used] = 'I';
+            ++used;
+            if (j > 0) st = (dir_at(d, tq, W, lane, i, j) & 4u) ? 'I' : 'M';
             --i;
         } else {
             if (j == 0) break;
-            slot_end[-1 - used++] = 'D';
-            if (i > 0) st = (dir_at(d, tq, i, j) & 8u) ? 'D' : 'M';
+            if (lane == 0) slot_end[-1 - used] = 'D';
+            ++used;
+            if (i > 0) st = (dir_at(d, tq, W, lane, i, j) & 8u) ? 'D' : 'M';
             --j;
         }
     }
@@ -178,12 +201,59 @@ __device__ void msa_degenerate(const MsaLaunch& L, int pid, int64_t ca, int M, i
     L.cig_len[pid] = used;
 }
 
+// Rows of A (its columns, with their gap costs) and of the strip boundary
+// column come through an LDS ring of 128 rows: at the start of each 64-step
+// block the wave writes the rows fetched during the previous block into the
+// ring and issues the global loads for the next, so no step waits on HBM.
+// At step t of block t0 the lanes need A rows t - 63 .. t <= t0 + 64 and
+// lane 0 the boundary row t: rows t0 - 63 .. t0 + 64 are in the ring.
+constexpr int kRing = 128;
+
+template <int K>
+struct RowFetch {
+    int cnt;
+    float open, ext;
+    uint32_t sym[(K + 3) / 4];
+    float frq[K];
+    float bm, bx, by;
+};
+
+template <int K>
+__device__ __forceinline__ void fetch_rows(RowFetch<K>& f, const MsaLaunch& L, int64_t ca, int M, int t0, int lane,
+                                           bool strip0, const float* bM, const float* bX, const float* bY)
+{
+    const int r = t0 + 1 + lane;                      // 1-based row of A
+    const int64_t C = L.n_cols;
+    f.cnt = 0; f.open = 0.f; f.ext = 0.f;
+    for (int q = 0; q < (K + 3) / 4; ++q) f.sym[q] = 0;
+    if (r <= M) {
+        const int64_t c = ca + r - 1;
+        f.cnt = L.nz_cnt[c];
+        f.open = L.gopen[c];
+        f.ext = L.gext[c];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            f.sym[k >> 2] |= uint32_t(L.nz_sym[int64_t(k) * C + c]) << (8 * (k & 3));
+            f.frq[k] = L.nz_frq[int64_t(k) * C + c];
+        }
+    }
+    const int rb = t0 + lane;                          // boundary row
+    if (!strip0 && rb <= M) { f.bm = bM[rb]; f.bx = bX[rb]; f.by = bY[rb]; }
+}
+
 template <int K>
 __global__ __launch_bounds__(64) void msa_pp_kernel(MsaLaunch L)
 {
     __shared__ float sub_s[K * K];
     __shared__ float bfrq[K * 64];
     __shared__ uint8_t bsym[K * 64];
+    __shared__ float r_open[kRing], r_ext[kRing];
+    __shared__ int r_cnt[kRing];
+    __shared__ uint8_t r_sym[kRing * K];
+    __shared__ float r_frq[kRing * K];
+    __shared__ float r_bm[kRing], r_bx[kRing], r_by[kRing];
+    __shared__ uint32_t win[kWinRows * 64];
+    __shared__ float gsum[K * 64];
     __shared__ Best red[64];
     __shared__ float mmn;
     const int pid = L.first_pair + int(blockIdx.x);
@@ -209,10 +279,7 @@ __global__ __launch_bounds__(64) void msa_pp_kernel(MsaLaunch L)
     const int T = M + 64;                               // steps t = 0 .. M + 63
     const int tq = (T + 3) >> 2;
     Best best{-3.0e38f, 0x7fffffff};
-    if (lane == 0 && fr) {
-        // row M, column 0: M[M][0] (key 2(M+1)), Iy[M][0] = NEG
-        offer(best, 0.f, 2 * (M + 1));
-    }
+    if (lane == 0 && fr) offer(best, 0.f, 2 * (M + 1));   // row M, column 0: M[M][0]
     for (int s = 0; s < nstrips; ++s) {
         const int j = (s << 6) + lane + 1;
         const bool jv = j <= N;
@@ -228,71 +295,127 @@ __global__ __launch_bounds__(64) void msa_pp_kernel(MsaLaunch L)
         float outM = kMsaNeg, outIx = kMsaNeg, outIy = kMsaNeg;
         float dM = kMsaNeg, dIx = kMsaNeg, dIy = kMsaNeg;
         uint32_t acc = 0;
-        for (int t = 0; t < T; ++t) {
-            const int i = t - lane;
-            // lane 0's left column at row t: the column-0 boundary or the
-            // previous strip's last column (wave-uniform values)
-            float lbM, lbX, lbY;
-            if (t <= M) {
-                if (s == 0) {
-                    lbM = t == 0 ? 0.f : (fr ? 0.f : kMsaNeg);
-                    lbX = t == 0 ? kMsaNeg : (fr ? 0.f : boundary_gap(L.gopen[ca + t - 1], t, ge));
-                    lbY = kMsaNeg;
-                } else {
-                    lbM = bM[t]; lbX = bX[t]; lbY = bY[t];
+        const bool full_strip = (s << 6) + 64 <= N;
+        const bool coln = (s << 6) + 64 == N;               // lane 63 holds column N
+        // G[x] = the substitution sum of a one-symbol A column x (fraction
+        // 1) against the lane's column of B, in the reference's term order
+        for (int x = 0; x < K; ++x) {
+            float g = 0.f;
+            for (int kb = 0; kb < nb; ++kb) g += 1.0f * bfrq[kb * 64 + lane] * sub_s[x * K + bsym[kb * 64 + lane]];
+            gsum[x * 64 + lane] = g;
+        }
+        RowFetch<K> pf;
+        fetch_rows<K>(pf, L, ca, M, 0, lane, s == 0, bM, bX, bY);
+        for (int t0 = 0; t0 < T; t0 += 64) {
+            __syncthreads();                             // the previous block's ring reads are done
+            {
+                const int r = t0 + 1 + lane, q = r & (kRing - 1);
+                r_cnt[q] = pf.cnt; r_open[q] = pf.open; r_ext[q] = pf.ext;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    r_sym[q * K + k] = uint8_t(pf.sym[k >> 2] >> (8 * (k & 3)));
+                    r_frq[q * K + k] = pf.frq[k];
                 }
-            } else {
-                lbM = lbX = lbY = kMsaNeg;
+                const int qb = (t0 + lane) & (kRing - 1);
+                r_bm[qb] = pf.bm; r_bx[qb] = pf.bx; r_by[qb] = pf.by;
             }
-            const float lM = dpp_shr1(lbM, outM);
-            const float lX = dpp_shr1(lbX, outIx);
-            const float lY = dpp_shr1(lbY, outIy);
-            uint32_t code = 0;
-            if (i == 0) {
-                outM = prevM; outIx = prevIx; outIy = row0Iy;
-                if (jv && j == N) {                       // column N, row 0
-                    offer(best, prevM, 0);
-                    offer(best, prevIx, 1);
-                }
-            } else if (i >= 1 && i <= M && jv) {
-                const int64_t cA = ca + i - 1;
-                const int na = L.nz_cnt[cA];
-                const float openB = L.gopen[cA], extB = L.gext[cA];
-                float sc = 0.f;
-                for (int ka = 0; ka < na; ++ka) {
-                    const int x = L.nz_sym[int64_t(ka) * C + cA];
-                    const float fa = L.nz_frq[int64_t(ka) * C + cA];
-                    for (int kb = 0; kb < nb; ++kb) {
-                        const int y = bsym[kb * 64 + lane];
-                        const float fb = bfrq[kb * 64 + lane];
-                        sc += fa * fb * sub_s[x * K + y];
+            __syncthreads();
+            if (t0 + 64 < T) fetch_rows<K>(pf, L, ca, M, t0 + 64, lane, s == 0, bM, bX, bY);
+            const int tend = t0 + 64 < T ? t0 + 64 : T;
+            // One step.  FULL: every lane's row exists and is below the
+            // last row (63 < t0, t0 + 63 < M) and every lane's column
+            // exists, none of them column N (no activity tests, no end-point
+            // bookkeeping).  S0: strip 0 (the left column is the boundary).
+            auto step = [&](int t, auto full_tag, auto s0_tag) {
+                constexpr bool FULL = decltype(full_tag)::value;
+                constexpr bool S0 = decltype(s0_tag)::value;
+                const int i = t - lane;
+                // lane 0's left column at row t: the column-0 boundary or
+                // the previous strip's last column (wave-uniform values)
+                float lbM = kMsaNeg, lbX = kMsaNeg, lbY = kMsaNeg;
+                if (FULL || t <= M) {
+                    const int qt = t & (kRing - 1);
+                    if (S0) {
+                        lbM = (!FULL && t == 0) ? 0.f : (fr ? 0.f : kMsaNeg);
+                        lbX = (!FULL && t == 0) ? kMsaNeg : (fr ? 0.f : boundary_gap(r_open[qt], t, ge));
+                    } else {
+                        lbM = r_bm[qt]; lbX = r_bx[qt]; lbY = r_by[qt];
                     }
                 }
-                const float oIx = prevM - openB, eIx = prevIx - extB;
-                const float nIx = oIx >= eIx ? oIx : eIx;
-                const float oIy = lM - openA, eIy = lY - extA;
-                const float cIy = oIy >= eIy ? oIy : eIy;
-                float mx = dM;
-                uint32_t h = 0;
-                if (dIx > mx) { mx = dIx; h = 1; }
-                if (dIy > mx) { mx = dIy; h = 2; }
-                const float cm = sc + mx;
-                code = h | (eIx > oIx ? 4u : 0u) | (eIy > oIy ? 8u : 0u);
-                outM = cm; outIx = nIx; outIy = cIy;
-                prevM = cm; prevIx = nIx;
-                if (fr) {
-                    if (j == N) { offer(best, cm, 2 * i); offer(best, nIx, 2 * i + 1); }
-                    if (i == M) { offer(best, cm, 2 * (M + 1) + 2 * j); offer(best, cIy, 2 * (M + 1) + 2 * j + 1); }
-                } else if (i == M && j == N) {
-                    mmn = cm;
+                const float lM = dpp_shr1(lbM, outM);
+                const float lX = dpp_shr1(lbX, outIx);
+                const float lY = dpp_shr1(lbY, outIy);
+                uint32_t code = 0;
+                if (!FULL && i == 0) {
+                    outM = prevM; outIx = prevIx; outIy = row0Iy;
+                    if (jv && j == N) {                  // column N, row 0
+                        offer(best, prevM, 0);
+                        offer(best, prevIx, 1);
+                    }
+                } else if (FULL || (i >= 1 && i <= M && jv)) {
+                    const int qi = i & (kRing - 1);
+                    const float openB = r_open[qi], extB = r_ext[qi];
+                    float sc = 0.f;
+                    const int na = r_cnt[qi];
+                    const int x0 = r_sym[qi * K];
+                    const float f0 = r_frq[qi * K];
+                    // A column of one symbol at fraction 1: fa * fb == fb
+                    // exactly, so the sum is the lane's precomputed G[x]
+                    if (__all(na == 1 && f0 == 1.0f)) {
+                        sc = gsum[x0 * 64 + lane];
+                    } else {
+                        for (int ka = 0; ka < na; ++ka) {
+                            const int x = r_sym[qi * K + ka];
+                            const float fa = r_frq[qi * K + ka];
+                            for (int kb = 0; kb < nb; ++kb)
+                                sc += fa * bfrq[kb * 64 + lane] * sub_s[x * K + bsym[kb * 64 + lane]];
+                        }
+                    }
+                    const float oIx = prevM - openB, eIx = prevIx - extB;
+                    const float nIx = oIx >= eIx ? oIx : eIx;
+                    const float oIy = lM - openA, eIy = lY - extA;
+                    const float cIy = oIy >= eIy ? oIy : eIy;
+                    float mx = dM;
+                    uint32_t h = 0;
+                    if (dIx > mx) { mx = dIx; h = 1; }
+                    if (dIy > mx) { mx = dIy; h = 2; }
+                    const float cm = sc + mx;
+                    code = h | (eIx > oIx ? 4u : 0u) | (eIy > oIy ? 8u : 0u);
+                    outM = cm; outIx = nIx; outIy = cIy;
+                    prevM = cm; prevIx = nIx;
+                    if (!FULL) {
+                        if (fr) {
+                            if (j == N) { offer(best, cm, 2 * i); offer(best, nIx, 2 * i + 1); }
+                            if (i == M) { offer(best, cm, 2 * (M + 1) + 2 * j); offer(best, cIy, 2 * (M + 1) + 2 * j + 1); }
+                        } else if (i == M && j == N) {
+                            mmn = cm;
+                        }
+                    }
                 }
-            }
-            dM = lM; dIx = lX; dIy = lY;
-            if (lane == 63 && i >= 0 && i <= M) { bM[i] = outM; bX[i] = outIx; bY[i] = outIy; }
-            acc |= code << (8 * (t & 3));
-            if ((t & 3) == 3 || t == T - 1) {
-                dirs[(int64_t(s) * tq + (t >> 2)) * 64 + lane] = acc;
-                acc = 0;
+                dM = lM; dIx = lX; dIy = lY;
+                if (lane == 63 && (FULL || (i >= 0 && i <= M))) { bM[i] = outM; bX[i] = outIx; bY[i] = outIy; }
+                acc |= code << (8 * (t & 3));
+                if (FULL ? (t & 3) == 3 : ((t & 3) == 3 || t == T - 1)) {
+                    dirs[(int64_t(s) * tq + (t >> 2)) * 64 + lane] = acc;
+                    acc = 0;
+                }
+            };
+            using TT = std::true_type;
+            using FF = std::false_type;
+            if (full_strip && !coln && t0 >= 64 && t0 + 63 < M) {
+                auto run4 = [&](auto s0_tag) {
+                    for (int t = t0; t < tend; t += 4) {
+                        step(t, TT{}, s0_tag);
+                        step(t + 1, TT{}, s0_tag);
+                        step(t + 2, TT{}, s0_tag);
+                        step(t + 3, TT{}, s0_tag);
+                    }
+                };
+                if (s == 0) run4(TT{}); else run4(FF{});
+            } else if (s == 0) {
+                for (int t = t0; t < tend; ++t) step(t, FF{}, TT{});
+            } else {
+                for (int t = t0; t < tend; ++t) step(t, FF{}, FF{});
             }
         }
         __syncthreads();
@@ -311,7 +434,6 @@ __global__ __launch_bounds__(64) void msa_pp_kernel(MsaLaunch L)
         __syncthreads();
     }
     __threadfence();
-    if (lane != 0) return;
     int ei = M, ej = N;
     char est = 'M';
     float sc;
@@ -327,12 +449,16 @@ __global__ __launch_bounds__(64) void msa_pp_kernel(MsaLaunch L)
     } else {
         sc = mmn;
     }
+    WalkWin W;
+    W.w = win;
     int oi, oj, used;
-    msa_walk(dirs, tq, ei, ej, est, L.cigar + L.cig_off[pid] + M + N, &oi, &oj, &used);
-    L.score[pid] = sc;
-    int32_t* sp = L.span + 4 * int64_t(pid);
-    sp[0] = oi; sp[1] = oj; sp[2] = ei; sp[3] = ej;
-    L.cig_len[pid] = used;
+    msa_walk(dirs, tq, W, lane, ei, ej, est, L.cigar + L.cig_off[pid] + M + N, &oi, &oj, &used);
+    if (lane == 0) {
+        L.score[pid] = sc;
+        int32_t* sp = L.span + 4 * int64_t(pid);
+        sp[0] = oi; sp[1] = oj; sp[2] = ei; sp[3] = ej;
+        L.cig_len[pid] = used;
+    }
 }
 
 }  // namespace
