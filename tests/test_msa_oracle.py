@@ -149,3 +149,105 @@ def test_nj_recovers_an_additive_tree(oracle):
     cherries = {tuple(sorted(p)) for p in ch.tolist() if p[0] < n and p[1] < n}
     assert {(0, 1), (2, 3), (5, 6)} <= cherries
     assert ch.shape == (n - 1, 2) and sorted(ch.ravel().tolist()) == list(range(2 * n - 2))
+
+
+
+def _alignments(m, n, last=None):
+    """Every column-op string over M / I (consumes a column of A) / D (of B)
+    with no I next to a D, aligning m columns of A with n of B."""
+    if m == 0 and n == 0:
+        yield ""
+        return
+    for op, dm, dn in (("M", 1, 1), ("I", 1, 0), ("D", 0, 1)):
+        if m < dm or n < dn or {op, last} == {"I", "D"}:
+            continue
+        for rest in _alignments(m - dm, n - dn, op):
+            yield op + rest
+
+
+class _ClosedEndScorer:
+    """Scores an alignment of two profiles under the closed-end rules of
+    msa_ref.cpp:1063-1164, written from the alignment's runs, not from the
+    DP: the alignment ends in an M column and no gap run touches a run of the
+    other kind; an inner run of gaps over columns c0..c1 costs open(c0) plus
+    ext(c) for c0 < c <= c1 (position-specific costs of the columns it
+    spans, msa.hpp:123-134), and a leading run of L columns costs open of its
+    L-th column plus (L - 1) * gap_extend (the boundary row and column).
+    Column scores and costs in float64."""
+
+    def __init__(self, A, B, P):
+        (ca, self.oa), (cb, self.ob) = A, B
+        if P.sub is not None:
+            S = np.array(P.sub, dtype=np.float64).reshape(P.alpha, P.alpha)
+        else:
+            S = np.array([[P.match if x == y else (P.ts if {x, y} in ({0, 2}, {1, 3}) else P.tv)
+                           for y in range(4)] for x in range(4)])
+        self.col = ca.astype(np.float64) @ S @ cb.astype(np.float64).T
+        self.P = P
+
+    def open_(self, occ):
+        P = self.P
+        if not P.psgp or occ >= 1.0:
+            return P.gap_open * occ
+        return max(P.psgp_scale * P.gap_open * occ, P.psgp_min_open * P.gap_open)
+
+    def ext_(self, occ):
+        P = self.P
+        if not P.psgp or occ >= 1.0:
+            return P.gap_extend
+        return max(P.gap_extend * occ, P.psgp_min_ext * P.gap_extend)
+
+    def score(self, al):
+        i = j = k = 0
+        s = 0.0
+        while k < len(al):
+            r = k
+            while r < len(al) and al[r] == al[k]:
+                r += 1
+            L = r - k
+            if al[k] == "M":
+                s += sum(self.col[i + t, j + t] for t in range(L))
+                i += L
+                j += L
+            else:
+                occ, c0 = (self.oa, i) if al[k] == "I" else (self.ob, j)
+                if k == 0:
+                    s -= self.open_(float(occ[c0 + L - 1])) + (L - 1) * self.P.gap_extend
+                else:
+                    s -= self.open_(float(occ[c0])) + sum(self.ext_(float(occ[c0 + t])) for t in range(1, L))
+                if al[k] == "I":
+                    i += L
+                else:
+                    j += L
+            k = r
+        return s
+
+    def best(self):
+        return max(self.score(al) for al in _alignments(len(self.oa), len(self.ob)) if al.endswith("M"))
+
+
+@pytest.mark.parametrize("tag", ["global", "protein"])
+def test_profile_align_brute_force_closed_ends(oracle, tag):
+    """Closed end gaps, tiny profiles (one of 1..3 columns, the other of
+    1..6, so inner gap runs are common; 1..3 rows with gaps and ambiguity
+    codes): the oracle's score is the best over every alignment, enumerated
+    and scored from its runs, and its own CIGAR scores exactly that."""
+    P = msa_params(oracle, tag)
+    letters = ("ACGT" * 3 + "RN--") if P.alpha == 4 else ("ARNDCQEGHILKMFPSTWYV" + "BX---")
+    rng = random.Random(91 if tag == "global" else 92)
+    for it in range(150):
+        def prof(hi):
+            L, n = rng.randint(1, hi), rng.randint(1, 3)
+            return ["".join(rng.choice(letters) for _ in range(L)) for _ in range(n)]
+        rA, rB = prof(3), prof(6)
+        if it % 2:
+            rA, rB = rB, rA
+        A, B = oracle.msa_profile(rA, P.alpha), oracle.msa_profile(rB, P.alpha)
+        score, span, cigar = oracle.msa_align(A, B, P)
+        sc = _ClosedEndScorer(A, B, P)
+        want = sc.best()
+        tol = 1e-4 * max(1.0, abs(want))
+        assert abs(score - want) <= tol, (score, want)
+        assert span == (0, 0, len(A[1]), len(B[1]))
+        assert cigar.endswith("M") and "ID" not in cigar and "DI" not in cigar
+        assert abs(sc.score(cigar) - want) <= tol, (cigar, want)
