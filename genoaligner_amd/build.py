@@ -21,7 +21,7 @@ LIB = os.path.join(LIBDIR, "libgenoaligner_amd.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-SOURCES = ["wfa.hip", "sw.hip", "sw16.hip", "router.hip", "fasta.hip", "noalloc.hip", "msa.hip", "nj.hip", "host.cpp", "api.cpp", "fasta.cpp", "gene_qc.cpp"]
+SOURCES = ["wfa.hip", "sw.hip", "sw16.hip", "sw16m.hip", "router.hip", "fasta.hip", "noalloc.hip", "msa.hip", "nj.hip", "host.cpp", "api.cpp", "fasta.cpp", "gene_qc.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include")]
 # Packed-FP32 instructions stay off in every device pass.

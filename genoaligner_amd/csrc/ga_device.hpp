@@ -160,6 +160,11 @@ struct Sw16Launch {
     uint32_t* selbuf;       // per-wave text selector rows (scratch), sw16_sel_stride dwords per wave
     const uint8_t* walk;    // per pair: emit span + CIGAR; null = all
     uint8_t* cigar; const int64_t* cig_off;
+    // substitution-matrix launches (sw16m_launch): residue codes 0..31 laid
+    // out like the bytes, the 32 x 32 score table (row: text residue), and the
+    // bias the kernel adds to every score (every biased score fits 0..255)
+    const uint8_t* tcode; const uint8_t* pcode; const int32_t* sub;
+    int32_t bias;
 };
 
 // Largest text length the packed kernel takes (LDS: 8 bytes per text row).
@@ -205,6 +210,13 @@ hipError_t sw_launch(const SwLaunch& L, bool trace, hipStream_t st, LaunchHooks*
 // Packed kernel over L.n_waves waves (two pairs each), then -- separately --
 // its walk over 2 * n_waves slots.
 hipError_t sw16_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHooks* hooks = nullptr);
+
+// Substitution-matrix pairs, two per workgroup (packed 16-bit), their strips
+// pipelined over the workgroup's waves; n_waves counts workgroups.  The trace
+// variant then walks every pair not flagged for the int32 re-run.
+constexpr int kSw16mMaxRows = 4096;
+int sw16m_bias(const int32_t* sub32x32, int gap_open);   // -1: some biased score leaves 0..255
+hipError_t sw16m_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHooks* hooks = nullptr);
 
 // Reduce the 8-int per-pair records to the device-resident result layout:
 // out5 = score, start_i, start_j, end_i, end_j (trace: row-major end and span;

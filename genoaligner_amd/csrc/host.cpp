@@ -1028,10 +1028,11 @@ const char* sw_scheme_error(const ga_sw_scoring& s, bool with_cigar)
 // the partner half's packed flags until the packed walk has run -- hence this
 // order.  All on `st`, no host synchronisation.
 hipError_t enqueue_sw16_chunk(const ga::Sw16Launch& P16, bool trace, hipStream_t st,
-                              ga::LaunchHooks* hooks)
+                              ga::LaunchHooks* hooks, const SwMatrixHost* mat = nullptr)
 {
-    // (the packed kernel's trace variant walks its own pairs before it exits)
-    hipError_t e = ga::sw16_launch(P16, trace, st, hooks);
+    // (the packed kernels' trace variants walk their pairs before the re-run
+    // kernel reuses the direction regions)
+    hipError_t e = mat ? ga::sw16m_launch(P16, trace, st, hooks) : ga::sw16_launch(P16, trace, st, hooks);
     if (e != hipSuccess) return e;
     ga::SwLaunch R{};
     R.txt = P16.txt; R.toff = P16.toff; R.tlen = P16.tlen;
@@ -1042,6 +1043,7 @@ hipError_t enqueue_sw16_chunk(const ga::Sw16Launch& P16, bool trace, hipStream_t
     R.walk = P16.walk; R.cigar = P16.cigar; R.cig_off = P16.cig_off; R.bnd = nullptr;
     R.first_pair = P16.first_pair; R.rerun = P16.rerun;
     R.pair_a = P16.pair_a; R.pair_b = P16.pair_b; R.n_total = P16.n_total;
+    if (mat) { R.tcode = P16.tcode; R.pcode = P16.pcode; R.sub = P16.sub; R.sub8 = mat->fits8(); }
     return ga::sw_launch(R, trace, st);
 }
 
@@ -1062,22 +1064,36 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
     std::vector<int64_t> packed;
     slot_off.resize(static_cast<size_t>(n));
     int64_t slot = 0;
+    // substitution-matrix pairs take the packed kernel when every biased
+    // score fits a byte
+    const int mbias = mat ? ga::sw16m_bias(mat->sub, mat->gap_open) : -1;
     for (int64_t i = 0; i < n; ++i) {
         const ga_sw_req& r = reqs[i];
         slot_off[size_t(i)] = slot;
         slot += std::max(r.text_len, 0) + std::max(r.pattern_len, 0);
         const bool sk = skip[size_t(i)] != 0;
-        if (!mat && !sk && r.text_len > 0 && r.pattern_len > 0 && r.text_len <= ga::kSw16MaxRows &&
-            (!trace || r.pattern_len <= ga::kSw16MaxCols))
+        const bool fits = mat ? (mbias >= 0 && r.text_len <= ga::kSw16mMaxRows)
+                              : (r.text_len <= ga::kSw16MaxRows && (!trace || r.pattern_len <= ga::kSw16MaxCols));
+        if (!sk && r.text_len > 0 && r.pattern_len > 0 && fits)
             packed.push_back(i);
         else
             order.push_back(i);
     }
     const int64_t n32 = int64_t(order.size());
-    std::stable_sort(packed.begin(), packed.end(), [&](int64_t a, int64_t b) {
-        if (reqs[a].text_len != reqs[b].text_len) return reqs[a].text_len > reqs[b].text_len;
-        return reqs[a].pattern_len > reqs[b].pattern_len;
-    });
+    if (mat) {
+        // a workgroup computes max(rows) x max(strips) of its two pairs: pair
+        // pairs with the same strip count and close row counts, largest first
+        std::stable_sort(packed.begin(), packed.end(), [&](int64_t a, int64_t b) {
+            const int sa = (reqs[a].pattern_len + 63) >> 6, sb = (reqs[b].pattern_len + 63) >> 6;
+            if (sa != sb) return sa > sb;
+            return reqs[a].text_len > reqs[b].text_len;
+        });
+    } else {
+        std::stable_sort(packed.begin(), packed.end(), [&](int64_t a, int64_t b) {
+            if (reqs[a].text_len != reqs[b].text_len) return reqs[a].text_len > reqs[b].text_len;
+            return reqs[a].pattern_len > reqs[b].pattern_len;
+        });
+    }
     order.insert(order.end(), packed.begin(), packed.end());
 
     std::vector<int32_t> tl(static_cast<size_t>(n)), pl(static_cast<size_t>(n));
@@ -1233,7 +1249,11 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
             L.dirs = d_dirs16; L.dir_stride = trace ? ga::sw16_dir_words_per_wave(cm, cn) : 0;
             L.selbuf = d_sel;
             L.walk = d_walk; L.cigar = d_cig; L.cig_off = d_slot;
-            if (enqueue_sw16_chunk(L, trace, st, &t_timer_hooks) != hipSuccess) err = "SW kernel launch";
+            if (mat) {
+                L.go = mat->gap_open; L.ge = mat->gap_extend;
+                L.tcode = d_tcode; L.pcode = d_pcode; L.sub = d_sub; L.bias = mbias;
+            }
+            if (enqueue_sw16_chunk(L, trace, st, &t_timer_hooks, mat) != hipSuccess) err = "SW kernel launch";
         }
     }
 

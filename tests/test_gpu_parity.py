@@ -459,6 +459,63 @@ def test_protein_sw_long_queries(gpu, oracle):
                 assert (r.score, r.end_i, r.end_j) == oracle.sw_score_lane(t, p, tab, 11, 1), (len(t), len(p))
 
 
+def _sw_scheme(scores, unknown, go, ge, alphabet="ARNDCQEGHILKMFPSTWYV"):
+    import genoaligner_amd._native as N
+    m = N.SwMatrix()
+    m.alphabet = alphabet.encode()
+    for a in range(len(alphabet)):
+        for b in range(len(alphabet)):
+            m.score[a][b] = scores[a][b]
+    m.unknown, m.gap_open, m.gap_extend = unknown, go, ge
+    return m
+
+
+def test_protein_sw_packed_reruns_and_fallbacks(gpu, oracle):
+    """Edges of the packed substitution-matrix kernel (two pairs per
+    workgroup): a pair whose score leaves its 16-bit range (3,100 residues,
+    mostly W, re-run by the int32 kernel) sharing workgroups with ordinary
+    pairs; a text past its 4,096 rows (int32 kernel) in the same batch; empty
+    sides; a scheme whose lowest score is below -gap_open (the extra bias
+    term); and one whose biased scores leave a byte (the whole batch on the
+    int32 kernel).  Trace and score-only results equal the oracle's."""
+    import genoaligner_amd as ga
+    alpha = "ARNDCQEGHILKMFPSTWYV"
+    b62 = ga.blosum62(11, 1)
+    S62 = [[b62.score[a][b] for b in range(20)] for a in range(20)]
+    Sgk = [[(5 if a == b else -((a * 7 + b * 3) % 10)) for b in range(20)] for a in range(20)]
+    Swide = [[(120 if a == b else -20) for b in range(20)] for a in range(20)]
+    cases = [(b62, S62, -1, 11, 1), (_sw_scheme(Sgk, -3, 4, 1), Sgk, -3, 4, 1),
+             (_sw_scheme(Swide, -5, 150, 10), Swide, -5, 150, 10)]
+    rng = random.Random(47)
+    for m, S, unk, go, ge in cases:
+        tab = oracle.matrix_table(alpha, S, unk)
+        pairs = []
+        for k in range(40):
+            p = _protein(rng, rng.randint(1, 500))
+            t = _mutate(rng, p, 0.15, alpha) if k % 2 else _protein(rng, rng.randint(1, 700))
+            pairs.append((t, p))
+        big = "".join(rng.choice("WWWWWWWWWC") for _ in range(3100))
+        pairs.insert(5, (big, _mutate(rng, big, 0.002, alpha)))
+        longt = _protein(rng, 4500)
+        pairs.insert(11, (longt, _mutate(rng, longt[1000:1400], 0.1, alpha)))
+        pairs.insert(20, ("", "ACDE"))
+        pairs.append(("KLM", ""))
+        for cigar in (True, False):
+            reqs = [ga.SWRequest(text=t, text_len=len(t), pattern=p, pattern_len=len(p), with_cigar=cigar)
+                    for t, p in pairs]
+            b = ga.align_sw_matrix_batch(reqs, m)
+            assert b.ok(), b.error
+            for (t, p), r in zip(pairs, b.results):
+                if cigar:
+                    assert (r.score, r.start_i, r.start_j, r.end_i, r.end_j, r.cigar) == \
+                           oracle.sw_trace(t, p, tab, go, ge), (go, len(t), len(p))
+                else:
+                    assert (r.score, r.end_i, r.end_j) == oracle.sw_score_lane(t, p, tab, go, ge), \
+                           (go, len(t), len(p))
+        if m is b62:
+            assert oracle.sw_trace(pairs[5][0], pairs[5][1], tab, go, ge)[0] > 32767 - 255 - 11
+
+
 def test_wfa_graph_replays_match_eager(gpu, oracle):
     """The device-planned submission captured into a HIP graph: every replay
     (outputs reset between replays, eager work of the program in between)
