@@ -216,6 +216,9 @@ hipError_t sw16_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHo
 // variant then walks every pair not flagged for the int32 re-run.
 constexpr int kSw16mMaxRows = 4096;
 int sw16m_bias(const int32_t* sub32x32, int gap_open);   // -1: some biased score leaves 0..255
+// Direction words per workgroup of sw16m_launch (its own layout, or the int32
+// layout of both pairs when they are re-run, whichever is larger).
+int64_t sw16m_dir_words_per_group(int max_m, int max_n);
 hipError_t sw16m_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHooks* hooks = nullptr);
 
 // Reduce the 8-int per-pair records to the device-resident result layout:

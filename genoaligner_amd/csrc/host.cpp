@@ -1202,6 +1202,10 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
     if (npk > 0 && !err) {
         const int64_t n_waves = (npk + 1) / 2;
         // chunk waves by the direction budget (stride from each chunk's largest shape)
+        // direction words per wave (DNA) or per workgroup (matrix kernel)
+        const auto dir_words = [&](int m_, int n_) {
+            return mat ? ga::sw16m_dir_words_per_group(m_, n_) : ga::sw16_dir_words_per_wave(m_, n_);
+        };
         std::vector<std::pair<int64_t, int64_t>> wchunks;
         int64_t max_stride = 0;
         {
@@ -1213,17 +1217,17 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
                     if (k < n) { wm = std::max<int64_t>(wm, tl[size_t(k)]); wn = std::max<int64_t>(wn, pl[size_t(k)]); }
                 }
                 const int64_t nm = std::max(mm, wm), nn = std::max(mn, wn);
-                const int64_t stride = trace ? ga::sw16_dir_words_per_wave(int(nm), int(nn)) : 0;
+                const int64_t stride = trace ? dir_words(int(nm), int(nn)) : 0;
                 if (w > w0 && stride * (w - w0 + 1) * 4 > sw_dir_budget()) {
                     wchunks.emplace_back(w0, w);
-                    max_stride = std::max(max_stride, trace ? ga::sw16_dir_words_per_wave(int(mm), int(mn)) * (w - w0) : 0);
+                    max_stride = std::max(max_stride, trace ? dir_words(int(mm), int(mn)) * (w - w0) : 0);
                     w0 = w; mm = wm; mn = wn;
                 } else {
                     mm = nm; mn = nn;
                 }
             }
             wchunks.emplace_back(w0, n_waves);
-            max_stride = std::max(max_stride, trace ? ga::sw16_dir_words_per_wave(int(mm), int(mn)) * (n_waves - w0) : 0);
+            max_stride = std::max(max_stride, trace ? dir_words(int(mm), int(mn)) * (n_waves - w0) : 0);
         }
         uint32_t* d_dirs16 = trace ? D.alloc<uint32_t>(size_t(max_stride)) : nullptr;
         int32_t pk_max_m = 0;
@@ -1246,7 +1250,7 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
             L.n_waves = int32_t(ch.second - ch.first); L.max_m = cm; L.max_n = cn;
             L.match = sc.match; L.mismatch = sc.mismatch; L.go = sc.gap_open; L.ge = sc.gap_extend;
             L.res = d_res; L.rerun = d_rerun;
-            L.dirs = d_dirs16; L.dir_stride = trace ? ga::sw16_dir_words_per_wave(cm, cn) : 0;
+            L.dirs = d_dirs16; L.dir_stride = trace ? dir_words(cm, cn) : 0;
             L.selbuf = d_sel;
             L.walk = d_walk; L.cigar = d_cig; L.cig_off = d_slot;
             if (mat) {

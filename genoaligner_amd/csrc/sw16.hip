@@ -62,7 +62,6 @@ constexpr uint32_t kSelNone = 0x0c0c0c0cu;   // v_perm selector: both halves u =
 // Flag-word window of a walk (walk16): kWinC columns ending at the anchor
 // column (one per lane of the walk's half-wave), and per column the kWinQ
 // words that hold the kWinR rows ending at the anchor row.
-constexpr int kWinR = 32, kWinC = 32, kWinQ = kWinR / 4 + 1;
 constexpr size_t kWinBytes = size_t(2) * kWinC * kWinQ * 4;   // both walks of a wave
 
 // Offset of the walk windows behind the trace staging area (2 CIGAR slots and

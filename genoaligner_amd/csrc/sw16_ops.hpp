@@ -68,6 +68,11 @@ __device__ __forceinline__ uint32_t wave_pmax(uint32_t v)
     return uint32_t(__builtin_amdgcn_readlane(int(v), 63));
 }
 
+// Flag-word window of a traceback walk: kWinC columns ending at the anchor
+// column (one per lane of the walk's half-wave), and per column the kWinQ
+// words that hold the kWinR rows ending at the anchor row.
+constexpr int kWinR = 32, kWinC = 32, kWinQ = kWinR / 4 + 1;
+
 // Per-lane state of one strip.
 struct Strip {
     uint32_t H, Eh, Fh, diag;      // last row's H, Eh, Fh; H of the left column one row up

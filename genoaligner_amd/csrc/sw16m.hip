@@ -23,7 +23,7 @@
 //
 // Parallelism.  A protein batch has few pairs of many strips, so the strips
 // of a workgroup's two pairs run as a pipeline over its kWaves waves: wave w
-// takes strips w, w + kWaves, ...; strip s starts kLag 8-row chunks after
+// takes strips w, w + kWaves, ...; strip s starts kLag chunks of kChunk rows after
 // strip s - 1 (lane 63 of strip s - 1 hands row r over at step r + 63, lane 0
 // of strip s reads it one chunk ahead of step r) and no earlier than its
 // wave is free; one barrier per chunk keeps the waves in step.  Strip s reads
@@ -39,14 +39,20 @@
 // STOP, before any int32 re-run reuses the workgroup's direction region.
 #include "sw16_ops.hpp"
 
+#include <type_traits>
+
 namespace ga {
 namespace {
 
 constexpr int kWaves = 4;                 // waves per workgroup
-constexpr int kLag = 10;                  // 8-row chunks between consecutive strips
+constexpr int kChunk = 16;               // text rows per pipeline step (one barrier each)
+// Pipeline steps between consecutive strips: the rows a strip reads in a step
+// (and the group it prefetches) were handed over in an earlier step.
+constexpr int kLag = 2 + 66 / kChunk;
 constexpr int kZeroCode = 32;             // table row of u = 0
 constexpr int kTabBytes = 33 * 64;        // one half's table: 33 rows x 64 lanes
 constexpr int kProfBytes = kWaves * 2 * kTabBytes;
+constexpr int kRecBytes = 2 * kWaves * 64 * 3 * 4;   // merge records (in the tables' LDS)
 constexpr uint32_t kSelZero = uint32_t(kZeroCode * 64) | (uint32_t(kTabBytes + kZeroCode * 64) << 16);
 
 // Boundary rows in use: the text rows rounded up to whole chunks of 8.
@@ -83,8 +89,120 @@ __device__ __forceinline__ Duo duo_of(const Sw16Launch& L, int g)
     d.mBp = lb ? d.mB : 0; d.nBp = lb ? d.nB : 0;
     d.M = d.mAp > d.mBp ? d.mAp : d.mBp;
     d.N = d.nAp > d.nBp ? d.nAp : d.nBp;
-    d.nq = ((d.M + 63 + 7) & ~7) >> 2;
+    d.nq = ((d.M + 63 + kChunk - 1) / kChunk) * (kChunk / 4);
     return d;
+}
+
+// The reference's three-state walk (sw_kernel_impl.hip:198-242) over the
+// packed flags, run by wave 0 of each workgroup of the forward kernel once its
+// strips are done (other workgroups' strips hide its latency): lane 0 walks
+// pair A and lane 32 pair B, from the trace end cell with the cell's value
+// carried (score; minus s on a diagonal step, plus ge on an extension, plus
+// go on an opening) and stopping where that value is 0 in the H state.  When
+// a walk leaves its window, its half-wave loads the kWinC columns x kWinR rows
+// up and left of the current cell in one coalesced round: lane c of the half
+// the flag words of column j - c, the text byte and residue code of row i - c
+// and the pattern byte and code of column j - c, so the steps inside the
+// window read only LDS.  The CIGAR goes right-aligned into the pair's slot of
+// m + n bytes.  Flag layout (non-HMOD, sw16.hip): cell (i, j) of half h sits
+// in word (s * nq + (t >> 2)) * 64 + l, s and l the strip and lane of column
+// j - 1 and t = i - 1 + l; with b = 8h + (t & 3), bit b is E extended, b + 4
+// not-from-the-diagonal, b + 16 F extended and b + 20 F over E.
+struct WalkLds {
+    uint32_t win[2][kWinC * kWinQ];
+    uint8_t wt[2][kWinR], wtc[2][kWinR], wp[2][kWinC], wpc[2][kWinC];
+    int16_t sub_s[32 * 32];
+};
+
+// Called by one whole wave after the forward pass of its workgroup's pairs.
+// score / ei / ej: the half's score (> 0 to walk) and 0-based trace end cell.
+__device__ void walk_duo(const Sw16Launch& L, const Duo& D, int g, int lane, int score, int ei, int ej,
+                         bool walk_it, WalkLds& S)
+{
+    auto& win = S.win;
+    auto& wt = S.wt;
+    auto& wtc = S.wtc;
+    auto& wp = S.wp;
+    auto& wpc = S.wpc;
+    auto& sub_s = S.sub_s;
+    const int hw = lane >> 5, c = lane & 31;
+    for (int q = lane; q < 32 * 32; q += 64) sub_s[q] = int16_t(L.sub[q]);
+    __builtin_amdgcn_wave_barrier();
+    const int pid = hw == 0 ? D.ia : D.ib;
+    const int m = hw == 0 ? D.mA : D.mB, n = hw == 0 ? D.nA : D.nB;
+    int i = ei + 1, j = ej + 1;
+    const bool walker = c == 0 && walk_it;
+    const int ps = pid >= 0 ? pid : D.ia;
+    const uint8_t* T = L.txt + L.toff[ps];
+    const uint8_t* P = L.pat + L.poff[ps];
+    const uint8_t* TC = L.tcode + L.toff[ps];
+    const uint8_t* PC = L.pcode + L.poff[ps];
+    uint8_t* cg = L.cigar + (walker ? L.cig_off[pid] : 0);
+    const uint32_t* dirs = L.dirs + int64_t(g) * L.dir_stride;
+    const int nq = D.nq;
+    const int go = L.go, ge = L.ge;
+    uint32_t* w = win[hw];
+    int pos = m + n, state = 0, v = score;
+    bool active = walker, have = false;
+    int iw = 0, jw = 0;                           // window anchor (walker lanes)
+    while (__any(active)) {
+        if (active && (!(i > 0 && j > 0 && pos > 0) || (state == 0 && v == 0))) active = false;
+        const bool need = active && !(have && iw - i < kWinR && jw - j < kWinC);
+        const int need_h = __shfl(int(need), hw * 32);
+        const int ih = __shfl(i, hw * 32), jh = __shfl(j, hw * 32);
+        if (need_h) {
+            const int jj = jh - c;                // 1-based column of this lane
+            const int ii = ih - c;                // 1-based row of this lane
+            if (jj >= 1) {
+                const int l = (jj - 1) & 63, s = (jj - 1) >> 6;
+                const int q0 = (ih - kWinR + l) >> 2;   // floor
+#pragma unroll
+                for (int k = 0; k < kWinQ; ++k) {
+                    const int q = q0 + k;
+                    w[c * kWinQ + k] = (q >= 0 && q < nq) ? dirs[(int64_t(s) * nq + q) * 64 + l] : 0u;
+                }
+                wp[hw][c] = P[jj - 1];
+                wpc[hw][c] = PC[jj - 1];
+            }
+            if (ii >= 1) {
+                wt[hw][c] = T[ii - 1];
+                wtc[hw][c] = TC[ii - 1];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (need) { iw = i; jw = j; have = true; }
+        if (!active) continue;
+        // at most kWinR + kWinC - 1 steps stay inside the window
+        while (i > 0 && j > 0 && pos > 0 && !(state == 0 && v == 0) && iw - i < kWinR && jw - j < kWinC) {
+            const int l = (j - 1) & 63, t = i - 1 + l;
+            const uint32_t word = w[(jw - j) * kWinQ + ((t >> 2) - ((iw - kWinR + l) >> 2))];
+            const int b = 8 * hw + (t & 3);
+            if (state == 0 && !((word >> (b + 4)) & 1u)) {
+                const int ri = iw - i, cj = jw - j;
+                cg[--pos] = wt[hw][ri] == wp[hw][cj] ? 'M' : 'X';
+                v -= sub_s[int(wtc[hw][ri]) * 32 + int(wpc[hw][cj])];
+                --i; --j;
+            } else if (state == 1 || (state == 0 && !((word >> (b + 20)) & 1u))) {
+                cg[--pos] = 'D';
+                const bool ext = ((word >> b) & 1u) != 0;
+                state = ext ? 1 : 0;
+                v += ext ? ge : go;
+                --j;
+            } else {
+                cg[--pos] = 'I';
+                const bool ext = ((word >> (b + 16)) & 1u) != 0;
+                state = ext ? 2 : 0;
+                v += ext ? ge : go;
+                --i;
+            }
+        }
+    }
+    if (walker) {
+        int32_t* res = L.res + int64_t(pid) * 8;
+        res[5] = i;
+        res[6] = j;
+        res[7] = m + n - pos;
+    }
 }
 
 template <bool TRACE, bool GK>
@@ -137,8 +255,8 @@ __global__ __launch_bounds__(64 * kWaves) void sw16m_kernel(Sw16Launch L)
     __syncthreads();
 
     const int nstrips = (N + 63) >> 6;
-    const int nchunks = (M + 63 + 7) >> 3;
-    const int nq = nchunks * 2;
+    const int nchunks = (M + 63 + kChunk - 1) / kChunk;
+    const int nq = nchunks * (kChunk / 4);
     const int period = nchunks > kWaves * kLag ? nchunks : kWaves * kLag;
     const int last = nstrips - 1;
     const int total = (last / kWaves) * period + (last % kWaves) * kLag + nchunks;
@@ -151,16 +269,7 @@ __global__ __launch_bounds__(64 * kWaves) void sw16m_kernel(Sw16Launch L)
 
     int gH[2] = {0, 0}, gI[2] = {-1, -1}, gJ[2] = {-1, -1};
     uint32_t thrm1 = 0;                        // per half: this wave's best so far - 1
-    Strip S{};
-    int j = 0;
-    bool fast = false;
-    // Prefetch sets, indexed by the parity of the group (4 rows): the group
-    // about to run has its u and boundary rows; the selectors run two groups
-    // ahead.  The sets rotate with the group index, so no register copy waits
-    // for a load in flight.
-    uint32_t sv[2][4], uu[2][4];
-    uint2 bv[2][4];
-    const int last_t = nchunks * 8 - 4;        // first row of the strip's last group
+    const int last_t = nchunks * kChunk - 4;   // first row of the strip's last group
     auto ld_sel = [&](int t, uint32_t (&s4)[4]) {
         const int tt = t < last_t ? t : last_t;
 #pragma unroll
@@ -177,38 +286,39 @@ __global__ __launch_bounds__(64 * kWaves) void sw16m_kernel(Sw16Launch L)
         for (int u = 0; u < 4; ++u) b4[u] = b[u];
     };
 
-    for (int G = 0; G < total; ++G) {
-        // this wave's strip and chunk at pipeline step G (wave-uniform)
-        int s = -1, c = 0;
-        if (G >= w * kLag) {
-            const int r = (G - w * kLag) / period;
-            const int cc = G - w * kLag - r * period;
-            if (cc < nchunks && r * kWaves + w < nstrips) { s = r * kWaves + w; c = cc; }
+    // One strip s, nchunks pipeline steps (one barrier each).  FAST: every
+    // column of the strip exists in both pairs (no column masks).
+    auto run_strip = [&](auto fast_tag, int s) {
+        constexpr bool FAST = decltype(fast_tag)::value;
+        const uint2* br = bnd + ((s + 1) & 1) * Mr;
+        uint2* bw = bnd + (s & 1) * Mr;
+        const int j = (s << 6) + lane;
+        const uint32_t pa = j < D.nAp ? PCA[j] : 0u, pbc = j < D.nBp ? PCB[j] : 0u;
+        for (int code = 0; code < 32; ++code) {
+            prof[code * 64 + lane] = uint8_t(j < D.nAp ? L.sub[code * 32 + pa] + bias : 0);
+            prof[kTabBytes + code * 64 + lane] = uint8_t(j < D.nBp ? L.sub[code * 32 + pbc] + bias : 0);
         }
-        if (s >= 0) {
-            const uint2* br = bnd + ((s + 1) & 1) * Mr;
-            uint2* bw = bnd + (s & 1) * Mr;
-            if (c == 0) {
-                j = (s << 6) + lane;
-                const uint32_t pa = j < D.nAp ? PCA[j] : 0u, pbc = j < D.nBp ? PCB[j] : 0u;
-                for (int code = 0; code < 32; ++code) {
-                    prof[code * 64 + lane] = uint8_t(j < D.nAp ? L.sub[code * 32 + pa] + bias : 0);
-                    prof[kTabBytes + code * 64 + lane] = uint8_t(j < D.nBp ? L.sub[code * 32 + pbc] + bias : 0);
-                }
-                prof[kZeroCode * 64 + lane] = 0;
-                prof[kTabBytes + kZeroCode * 64 + lane] = 0;
-                S.H = 0; S.Eh = go2; S.Fh = go2; S.diag = 0;
-                S.bestH = thrm1; S.bestT = 0xffffffffu;   // record at the bound (see strip_group)
-                S.colmask = (j < D.nAp ? 0xffffu : 0u) | (j < D.nBp ? 0xffff0000u : 0u);
-                fast = (s << 6) + 64 <= nmin;
-                ld_sel(0, sv[0]);
-                ld_sel(4, sv[1]);
-                ld_u(sv[0], uu[0]);
-                ld_bnd(br, 0, bv[0]);
-            }
+        prof[kZeroCode * 64 + lane] = 0;
+        prof[kTabBytes + kZeroCode * 64 + lane] = 0;
+        Strip S{};
+        S.H = 0; S.Eh = go2; S.Fh = go2; S.diag = 0;
+        S.bestH = thrm1; S.bestT = 0xffffffffu;   // record at the bound (see strip_group)
+        S.colmask = (j < D.nAp ? 0xffffu : 0u) | (j < D.nBp ? 0xffff0000u : 0u);
+        // Prefetch sets, indexed by the parity of the group (4 rows): the
+        // group about to run has its u and boundary rows; the selectors run
+        // two groups ahead.  The sets rotate with the group index, so no
+        // register copy waits for a load in flight.
+        uint32_t sv[2][4], uu[2][4];
+        uint2 bv[2][4];
+        ld_sel(0, sv[0]);
+        ld_sel(4, sv[1]);
+        ld_u(sv[0], uu[0]);
+        ld_bnd(br, 0, bv[0]);
+        for (int c = 0; c < nchunks; ++c) {
 #pragma unroll
-            for (int p = 0; p < 2; ++p) {
-                const int t0 = c * 8 + 4 * p;
+            for (int k = 0; k < kChunk / 4; ++k) {
+                const int p = k & 1;
+                const int t0 = c * kChunk + 4 * k;
                 // prefetch: selectors of group +2, u and boundary rows of group +1
                 ld_sel(t0 + 8, sv[p]);
                 ld_u(sv[p ^ 1], uu[p ^ 1]);
@@ -216,33 +326,50 @@ __global__ __launch_bounds__(64 * kWaves) void sw16m_kernel(Sw16Launch L)
                 uint32_t acc = 0, hout[4], eout[4];
                 const uint32_t (&u4)[4] = uu[p];
                 const auto mat_u = [&](int u) { return u4[u]; };
-                if (fast)
-                    strip_group<TRACE, false, false, GK>(S, t0, lane, mat_u, bv[p], b2, gk2, ge2, acc, hout, eout, thrm1);
-                else
-                    strip_group<TRACE, false, true, true>(S, t0, lane, mat_u, bv[p], b2, gk2, ge2, acc, hout, eout, thrm1);
+                strip_group<TRACE, false, !FAST, FAST ? GK : true>(S, t0, lane, mat_u, bv[p], b2, gk2, ge2, acc,
+                                                                  hout, eout, thrm1);
                 if (TRACE) dirs[(int64_t(s) * nq + (t0 >> 2)) * 64 + lane] = acc;
-                // lane 63 hands rows t0-63 .. t0-60 to strip s + 1
+                // lane 63 hands rows t0-63 .. t0-60 to strip s + 1; rows
+                // outside 0 .. Mr-1 are dropped (rows M .. Mr-1 only reach
+                // drained cells)
                 if (lane == 63) {
+                    if (t0 >= 64 && t0 - 60 < Mr) {
+                        uint2* b = bw + (t0 - 63);
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int r = t0 + u - 63;
-                        if (r >= 0 && r < Mr) bw[r] = make_uint2(hout[u], eout[u]);
+                        for (int u = 0; u < 4; ++u) b[u] = make_uint2(hout[u], eout[u]);
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int r = t0 + u - 63;
+                            if (r >= 0 && r < Mr) bw[r] = make_uint2(hout[u], eout[u]);
+                        }
                     }
                 }
             }
-            if (c == nchunks - 1) {
-                // fold this strip's first maxima (per half) into the running records
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int bh = int((S.bestH >> (16 * h)) & 0xffffu);
-                    const int bt = int((S.bestT >> (16 * h)) & 0xffffu);
-                    const int bi = bh > 0 ? bt - lane : -1;
-                    if (bh > gH[h] || (bh == gH[h] && bh > 0 && bi < gI[h])) { gH[h] = bh; gI[h] = bi; gJ[h] = j; }
-                }
-            }
+            __syncthreads();
         }
-        __syncthreads();
+        // fold this strip's first maxima (per half) into the running records
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int bh = int((S.bestH >> (16 * h)) & 0xffffu);
+            const int bt = int((S.bestT >> (16 * h)) & 0xffffu);
+            const int bi = bh > 0 ? bt - lane : -1;
+            if (bh > gH[h] || (bh == gH[h] && bh > 0 && bi < gI[h])) { gH[h] = bh; gI[h] = bi; gJ[h] = j; }
+        }
+    };
+
+    // Wave w runs strips w, w + kWaves, ...; strip s occupies pipeline steps
+    // (s / kWaves) * period + w * kLag onwards.  Every wave passes the same
+    // number of barriers (total), idle steps included.
+    int G = 0;
+    for (int s = w; s < nstrips; s += kWaves) {
+        const int start = (s / kWaves) * period + w * kLag;
+        for (; G < start; ++G) __syncthreads();
+        if ((s << 6) + 64 <= nmin) run_strip(std::true_type{}, s);
+        else                       run_strip(std::false_type{}, s);
+        G += nchunks;
     }
+    for (; G < total; ++G) __syncthreads();
 
     // merge the waves' per-lane records (LDS of the tables, free now)
     int* rec = reinterpret_cast<int*>(lds16);
@@ -264,6 +391,8 @@ __global__ __launch_bounds__(64 * kWaves) void sw16m_kernel(Sw16Launch L)
         }
     }
     const int limit = 32767 - 255 - bias;
+    int wS[2] = {0, 0}, wI[2] = {-1, -1}, wJ[2] = {-1, -1};
+    bool wOk[2] = {false, false};
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int pid = h == 0 ? D.ia : D.ib;
@@ -296,64 +425,29 @@ __global__ __launch_bounds__(64 * kWaves) void sw16m_kernel(Sw16Launch L)
                 L.rerun[pid] = rerun ? 1 : 0;
             }
         }
+        const int m = h == 0 ? D.mA : D.mB, n = h == 0 ? D.nA : D.nB;
+        wS[h] = lH; wI[h] = rI; wJ[h] = rJ;
+        wOk[h] = m > 0 && n > 0 && lH > 0 && lH < limit && (!L.walk || L.walk[pid]);
     }
-}
-
-// One thread per pair: the reference's three-state walk (sw_kernel_impl.hip:
-// 198-242) over the packed flags, from the trace end cell with the cell's
-// value carried (score; minus s on a diagonal step, plus ge on an extension,
-// plus go on an opening), stopping where that value is 0 in the H state.
-// Writes the CIGAR right-aligned into the pair's slot of m + n bytes.
-__global__ __launch_bounds__(64) void sw16m_walk_kernel(Sw16Launch L)
-{
-    const int q = int(blockIdx.x) * 64 + int(threadIdx.x);
-    if (q >= 2 * L.n_waves) return;
-    const int g = q >> 1, h = q & 1;
-    const Duo D = duo_of(L, g);
-    const int pid = h == 0 ? D.ia : D.ib;
-    if (pid < 0 || (L.walk && !L.walk[pid])) return;
-    int32_t* res = L.res + int64_t(pid) * 8;
-    const int score = res[0];
-    if (score <= 0) return;                    // nothing to walk, or re-run (kSwRerun)
-    const int m = h == 0 ? D.mA : D.mB, n = h == 0 ? D.nA : D.nB;
-    const uint8_t* T = L.txt + L.toff[pid];
-    const uint8_t* P = L.pat + L.poff[pid];
-    const uint8_t* TC = L.tcode + L.toff[pid];
-    const uint8_t* PC = L.pcode + L.poff[pid];
-    const uint32_t* dirs = L.dirs + int64_t(g) * L.dir_stride;
-    uint8_t* cg = L.cigar + L.cig_off[pid];
-    const int nq = D.nq;
-    const int go = L.go, ge = L.ge;
-    int pos = m + n, state = 0, v = score;
-    int i = res[1] + 1, j = res[2] + 1;
-    while (i > 0 && j > 0 && pos > 0 && !(state == 0 && v == 0)) {
-        const int l = (j - 1) & 63, s = (j - 1) >> 6, t = i - 1 + l;
-        const uint32_t word = dirs[(int64_t(s) * nq + (t >> 2)) * 64 + l];
-        const int b = 8 * h + (t & 3);
-        if (state == 0 && !((word >> (b + 4)) & 1u)) {
-            cg[--pos] = T[i - 1] == P[j - 1] ? 'M' : 'X';
-            v -= L.sub[int(TC[i - 1]) * 32 + int(PC[j - 1])];
-            --i; --j;
-        } else if (state == 1 || (state == 0 && !((word >> (b + 20)) & 1u))) {
-            cg[--pos] = 'D';
-            const bool ext = ((word >> b) & 1u) != 0;
-            state = ext ? 1 : 0;
-            v += ext ? ge : go;
-            --j;
-        } else {
-            cg[--pos] = 'I';
-            const bool ext = ((word >> (b + 16)) & 1u) != 0;
-            state = ext ? 2 : 0;
-            v += ext ? ge : go;
-            --i;
-        }
-    }
-    res[5] = i;
-    res[6] = j;
-    res[7] = m + n - pos;
+    if (!TRACE) return;
+    // the walks (the re-run kernel walks the pairs it re-runs); the tables'
+    // LDS behind the merge records holds the walk's windows
+    WalkLds& WS = *reinterpret_cast<WalkLds*>(reinterpret_cast<uint8_t*>(lds16) + kRecBytes);
+    const int hw = lane >> 5;
+    walk_duo(L, D, g, lane, hw == 0 ? wS[0] : wS[1], hw == 0 ? wI[0] : wI[1], hw == 0 ? wJ[0] : wJ[1],
+             hw == 0 ? wOk[0] : wOk[1], WS);
 }
 
 }  // namespace
+
+int64_t sw16m_dir_words_per_group(int max_m, int max_n)
+{
+    if (max_m <= 0 || max_n <= 0) return 0;
+    const int64_t strips = (max_n + 63) / 64;
+    const int64_t own = strips * ((int64_t(max_m) + 63 + kChunk - 1) / kChunk) * (kChunk / 4) * 64;
+    const int64_t alt = 2 * strips * int64_t((max_m + 63 + 7) / 8) * 64;
+    return own > alt ? own : alt;
+}
 
 int sw16m_bias(const int32_t* sub, int gap_open)
 {
@@ -381,11 +475,6 @@ hipError_t sw16m_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchH
         if (trace) hipLaunchKernelGGL((sw16m_kernel<true, false>), grid, block, lds, st, L);
         else       hipLaunchKernelGGL((sw16m_kernel<false, false>), grid, block, lds, st, L);
     }
-    if (hooks) hooks->after(st);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || !trace) return e;
-    if (hooks) hooks->before(st);
-    hipLaunchKernelGGL(sw16m_walk_kernel, dim3(unsigned((2 * L.n_waves + 63) / 64)), dim3(64), 0, st, L);
     if (hooks) hooks->after(st);
     return hipGetLastError();
 }
