@@ -997,7 +997,7 @@ def serving_section(args, W, O, tab62, threads, cpu_on):
     value (the capacity knee), then every request at once (saturation), with
     per-class work rates against the VALU peaks; the oracle on the same mix."""
     import genoaligner_amd as ga
-    from genoaligner_amd.serve import Server, poisson_load
+    from genoaligner_amd.serve import LibraryBackend, Server, poisson_load
     srng = np.random.default_rng(5)
     sreqs = [W.serving_request(srng, ga) for _ in range(args.serve_requests)]
     # warm-up: a dozen requests through a server of the same shape first
@@ -1007,8 +1007,8 @@ def serving_section(args, W, O, tab62, threads, cpu_on):
         t.wait()
     warm.close()
 
-    def run(rate):
-        server = Server(budget_bytes=64 << 30, slots=4)
+    def run(rate, route=False):
+        server = Server(budget_bytes=64 << 30, slots=4, backend=LibraryBackend(route=route))
         t0 = time.perf_counter()
         if rate is None:
             tickets = [server.submit(*sreqs[k]) for k in range(len(sreqs))]
@@ -1037,6 +1037,11 @@ def serving_section(args, W, O, tab62, threads, cpu_on):
             break
         knee = r
     flood_t, flood = run(None)
+    # the same mix with the shipped router loaded into the library (the slots'
+    # WFA batches take the routed path; results unchanged)
+    _, rbase = run(args.serve_rate, route=True)
+    _, rflood = run(None, route=True)
+    ga.set_router(None)
 
     # per-class work of the mix: WFA diagonal updates ((w + 1)^2, w the score,
     # or smax when unresolved within the length bound), SW cells
@@ -1064,9 +1069,9 @@ def serving_section(args, W, O, tab62, threads, cpu_on):
                              "frac": round(WFA_OPS_PER_UPDATE * wfa_upd / fw / 1e12 / PEAK_INT32_TOPS, 5)}},
         "sw": {"pairs": sw_pairs, "pairs_per_s": round(sw_pairs / fw, 1), "cells": sw_cells,
                "gcups": round(sw_cells / fw / 1e9, 2),
-               "roofline": {"bound": "valu", "unit": "TOP/s", "peak": PEAK_INT32_TOPS,
+               "roofline": {"bound": "valu", "unit": "TOP/s", "peak": PEAK_INT16_TOPS,
                             "achieved": round(SW_OPS_PER_CELL * sw_cells / fw / 1e12, 4),
-                            "frac": round(SW_OPS_PER_CELL * sw_cells / fw / 1e12 / PEAK_INT32_TOPS, 5)}},
+                            "frac": round(SW_OPS_PER_CELL * sw_cells / fw / 1e12 / PEAK_INT16_TOPS, 5)}},
         "note": "work of each class over the saturation run's wall time (the classes share the GPU)",
     }
     by = {"wfa": [t for t in base_t if t.kind == "wfa"], "sw": [t for t in base_t if t.kind == "sw"]}
@@ -1083,6 +1088,10 @@ def serving_section(args, W, O, tab62, threads, cpu_on):
                      "saturation_pairs_per_s": flood["pairs_per_s"], "saturation_p99_ms": flood["p99_ms"],
                      "rule": "highest Poisson rate (x1.5 steps from the base) whose p99 stays within 2x the "
                              "base p99; saturation = every request submitted at once"},
+        "routed": {"p50_ms": rbase["p50_ms"], "p99_ms": rbase["p99_ms"], "rate_per_s": args.serve_rate,
+                   "saturation_pairs_per_s": rflood["pairs_per_s"], "saturation_p99_ms": rflood["p99_ms"],
+                   "errors": rbase["errors"] + rflood["errors"],
+                   "note": "the shipped router network loaded into the library (ga_router_set)"},
         "sweep": sweep + [flood],
         "per_class": per_class,
         "errors": sum(r["errors"] for r in sweep) + flood["errors"],

@@ -9,4 +9,5 @@ from .api import (  # noqa: F401
     AlignRequest, AlignResult, BatchResult, SWAlignResult, SWBatchResult, SWRequest,
     SWScoring, Status, SW_MAX_TRACE_CELLS, align, align_batch, align_batch_ext, align_sw,
     align_sw_batch, align_sw_matrix_batch, backend_name, blosum62, device_available, device_name,
+    router_enabled, set_router,
 )

@@ -15,6 +15,7 @@ from . import build as _build
 LIB_PATH = _build.LIB
 
 GA_OK, GA_INVALID_ARGUMENT, GA_DEVICE_ERROR = 0, 1, 2
+GA_MLP_PARAMS = 26896
 GA_MAX_SMAX = 511
 GA_MAX_SMAX_EXT = 4000
 GA_SW_MAX_TRACE_CELLS = 1 << 26
@@ -26,7 +27,7 @@ EXPORTS = (
     "ga_cigar_arena_bytes_wfa", "ga_cigar_arena_bytes_sw",
     "ga_wfa_device", "ga_wfa_device_noalloc", "ga_sw_device", "ga_set_kernel_timing", "ga_last_kernel_ms",
     "ga_sketch_device", "ga_mlp_forward_device", "ga_mlp_backward_device", "ga_mlp_sgd_device",
-    "ga_mlp_route_device", "ga_route_bucket_bound", "ga_wfa_routed_device",
+    "ga_mlp_route_device", "ga_route_bucket_bound", "ga_wfa_routed_device", "ga_router_set", "ga_router_enabled",
     "ga_align_sw_matrix_batch", "ga_sw_matrix_blosum62", "ga_set_thread_workspace_budget",
     "ga_trim_workspace",
     "ga_wfa_graph_create", "ga_wfa_graph_launch", "ga_wfa_graph_destroy",
@@ -178,6 +179,10 @@ def lib() -> C.CDLL:
         for f in (L.ga_sketch_device, L.ga_mlp_forward_device, L.ga_mlp_backward_device,
                   L.ga_mlp_sgd_device, L.ga_mlp_route_device, L.ga_wfa_routed_device):
             f.restype = C.c_int
+        L.ga_router_set.argtypes = [vp, f32, cpp]
+        L.ga_router_set.restype = C.c_int
+        L.ga_router_enabled.argtypes = []
+        L.ga_router_enabled.restype = C.c_int
         L.ga_align_sw_matrix_batch.argtypes = [C.POINTER(SWReq), C.c_int64, C.POINTER(SwMatrix),
                                                C.POINTER(SWRes), C.c_void_p, C.c_int64, i32p, i32p, cpp]
         L.ga_align_sw_matrix_batch.restype = C.c_int

@@ -305,6 +305,27 @@ def align_sw_matrix_batch(reqs: Sequence[SWRequest], matrix) -> SWBatchResult:
     return _sw_call(reqs, matrix)
 
 
+def set_router(params: Optional[np.ndarray], q: float = 0.95) -> None:
+    """Load a trained router network (GA_MLP_PARAMS float32 values; see
+    genoaligner_amd.router) into the library, or unload it with None.  While
+    one is loaded, align_batch and align_batch_ext run their WFA batches
+    through the routed path (same results pair for pair)."""
+    err = C.c_char_p()
+    if params is None:
+        rc = N.lib().ga_router_set(None, float(q), C.byref(err))
+    else:
+        arr = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        if arr.size != N.GA_MLP_PARAMS:
+            raise ValueError(f"router parameters: {arr.size} values, expected {N.GA_MLP_PARAMS}")
+        rc = N.lib().ga_router_set(arr.ctypes.data, float(q), C.byref(err))
+    if rc != N.GA_OK:
+        raise RuntimeError((err.value or b"ga_router_set failed").decode())
+
+
+def router_enabled() -> bool:
+    return bool(N.lib().ga_router_enabled())
+
+
 def align_sw(req: SWRequest) -> SWAlignResult:
     b = align_sw_batch([req])
     return b.results[0] if b.results else SWAlignResult()

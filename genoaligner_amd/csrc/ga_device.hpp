@@ -8,6 +8,7 @@
 #include <functional>
 
 struct ga_wfa_dev_batch;
+struct ga_route_stats;
 
 namespace ga {
 
@@ -231,6 +232,11 @@ hipError_t sw_gather(const int32_t* res8, int64_t n, bool trace, int32_t* out5,
 // one device region (plan_bytes for the claim order and plan, stack_bytes of
 // wavefront stacks reused by each size class in turn); the classes run one
 // after another on `st`.  Allocates nothing.  Returns an error string or null.
+// The routed WFA batch (router kernels, routed pass, plain re-run) on st.
+const char* wfa_routed_run(const ga_wfa_dev_batch* b, const float* params, float q, ga_route_stats* stats,
+                           hipStream_t st);
+// The network loaded by ga_router_set (device pointer, null when none) and its quantile.
+const float* router_params(float* q);
 const char* wfa_device_in_region(const ga_wfa_dev_batch* b, hipStream_t st, uint8_t* region, size_t plan_bytes,
                                  size_t stack_bytes);
 

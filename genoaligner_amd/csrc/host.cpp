@@ -839,7 +839,24 @@ const char* run_wfa(const ga_align_req* reqs, int64_t n, const std::vector<int32
     }
     if (!D.ok) err = "hipMalloc/hipMemcpy (WFA buffers)";
     t_timer.reset();
-    if (!err) err = enqueue_wfa(L, plan, any_trace, st, &t_timer);
+    float rq = 0.0f;
+    const float* rp = ga::router_params(&rq);
+    if (!err && rp) {
+        // a loaded network chooses each pair's first bound and size class;
+        // every pair it sends too low is re-run with its own smax, so the
+        // results are the plain path's pair for pair
+        ga_wfa_dev_batch b{};
+        b.pattern = L.pat; b.pattern_off = L.poff; b.pattern_len = L.plen;
+        b.text = L.txt; b.text_off = L.toff; b.text_len = L.tlen;
+        b.smax = L.smax; b.n_pairs = n;
+        b.smax_max = *std::max_element(smax.begin(), smax.end());
+        b.max_len = std::max(*std::max_element(plen.begin(), plen.end()), max_text);
+        b.score = L.score;
+        if (any_trace) { b.cigar = L.cigar; b.cigar_off = L.cig_off; b.cigar_len = L.cig_len; b.want_cigar = L.want; }
+        err = ga::wfa_routed_run(&b, rp, rq, nullptr, st);
+    } else if (!err) {
+        err = enqueue_wfa(L, plan, any_trace, st, &t_timer);
+    }
     out.score.assign(static_cast<size_t>(n), -1);
     out.cig_len.assign(static_cast<size_t>(n), 0);
     if (!err) {
@@ -1896,18 +1913,15 @@ int32_t ga_route_bucket_bound(int32_t bucket)
     return (bucket >= 0 && bucket < ga::kBuckets) ? kBucketBound[bucket] : -1;
 }
 
-int ga_wfa_routed_device(const ga_wfa_dev_batch* b, const float* params, float q, ga_route_stats* stats,
-                         void* stream, const char** error)
+// The routed WFA batch (sketch, network, routed pass, plain re-run) enqueued
+// on st; with stats non-null the stream is waited for once at the end.
+extern "C++" const char* ga::wfa_routed_run(const ga_wfa_dev_batch* b, const float* params, float q, ga_route_stats* stats,
+                               hipStream_t st)
 {
-    if (error) *error = nullptr;
-    if (stats) *stats = ga_route_stats{};
-    if (const char* e = check_dev_batch(b)) { if (error) *error = e; return GA_INVALID_ARGUMENT; }
-    if (!params) { if (error) *error = "null MLP parameters"; return GA_INVALID_ARGUMENT; }
     const int64_t n = b->n_pairs;
-    if (n <= 0) return GA_OK;
+    if (n <= 0) return nullptr;
     std::call_once(g_pool_once, init_pool);
     t_timer.reset();
-    hipStream_t st = static_cast<hipStream_t>(stream);
     std::vector<void*> held;
     auto alloc = [&](size_t bytes) -> void* {
         void* p = nullptr;
@@ -1978,17 +1992,63 @@ int ga_wfa_routed_device(const ga_wfa_dev_batch* b, const float* params, float q
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
     for (hipStream_t cs : side) release_stream(cs);
-    if (err) {
-        if (error) *error = err;
-        return GA_DEVICE_ERROR;
-    }
-    if (stats) {
+    if (!err && stats) {
         stats->shortcut = int64_t(hs[0]);
         stats->routed = int64_t(hs[1]);
         stats->plain = int64_t(hs[2]);
         stats->rerun = int64_t(hs[3]);
     }
+    return err;
+}
+
+int ga_wfa_routed_device(const ga_wfa_dev_batch* b, const float* params, float q, ga_route_stats* stats,
+                         void* stream, const char** error)
+{
+    if (error) *error = nullptr;
+    if (stats) *stats = ga_route_stats{};
+    if (const char* e = check_dev_batch(b)) { if (error) *error = e; return GA_INVALID_ARGUMENT; }
+    if (!params) { if (error) *error = "null MLP parameters"; return GA_INVALID_ARGUMENT; }
+    if (const char* e = ga::wfa_routed_run(b, params, q, stats, static_cast<hipStream_t>(stream))) {
+        if (error) *error = e;
+        return GA_DEVICE_ERROR;
+    }
     return GA_OK;
+}
+
+// ------------------------------------------------------ the library's router
+namespace {
+std::atomic<const float*> g_router_params{nullptr};
+std::atomic<float> g_router_q{0.95f};
+}  // namespace
+
+int ga_router_set(const float* params, float q, const char** error)
+{
+    if (error) *error = nullptr;
+    if (!params) { g_router_params.store(nullptr); return GA_OK; }
+    if (!(q <= 1.0f)) { if (error) *error = "routing quantile above 1"; return GA_INVALID_ARGUMENT; }
+    // one device copy per call, kept until the process ends: batches already
+    // running keep reading the network they started with
+    float* d = nullptr;
+    if (hipMalloc(&d, size_t(ga::kMlpParams) * 4) != hipSuccess) {
+        if (error) *error = "hipMalloc(router parameters)";
+        return GA_DEVICE_ERROR;
+    }
+    if (hipMemcpy(d, params, size_t(ga::kMlpParams) * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        if (error) *error = "hipMemcpy(router parameters)";
+        return GA_DEVICE_ERROR;
+    }
+    g_router_q.store(q);
+    g_router_params.store(d);
+    return GA_OK;
+}
+
+int ga_router_enabled(void) { return g_router_params.load() != nullptr ? 1 : 0; }
+
+extern "C++" const float* ga::router_params(float* q)
+{
+    const float* p = g_router_params.load();
+    if (q) *q = g_router_q.load();
+    return p;
 }
 
 void ga_set_thread_workspace_budget(int64_t bytes) { t_workspace_budget = bytes > 0 ? bytes : 0; }

@@ -66,10 +66,15 @@ class Request:
 
 class LibraryBackend:
     """Runs one batch through the library (blocking; the library enqueues on a
-    stream of its own and synchronises it before returning)."""
+    stream of its own and synchronises it before returning).  route=True loads
+    the shipped router network into the library (genoaligner_amd.router.load),
+    so WFA batches take the routed path; results are unchanged."""
 
-    def __init__(self, matrix=None):
+    def __init__(self, matrix=None, route: bool = False, q: float = 0.95):
         self.matrix = matrix if matrix is not None else api.blosum62(11, 1)
+        if route:
+            from . import router
+            api.set_router(router.load(), q)
 
     def set_workspace(self, nbytes: int):
         from . import _native as N

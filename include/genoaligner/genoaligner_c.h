@@ -282,6 +282,17 @@ typedef struct ga_route_stats {
 int ga_wfa_routed_device(const ga_wfa_dev_batch* b, const float* params, float q, ga_route_stats* stats,
                          void* stream, const char** error);
 
+/* The library's router (EXTENSION; no reference counterpart).  With a network
+ * loaded (GA_MLP_PARAMS host floats, copied to the device; q the routing
+ * quantile as for ga_wfa_routed_device), ga_align_batch and
+ * ga_align_batch_ext run their WFA batches through the routed path: same
+ * outputs pair for pair, the network choosing each pair's first bound and
+ * size class.  params NULL unloads it.  Every call keeps its own device copy
+ * until the process ends (batches already running keep the network they
+ * started with). */
+int ga_router_set(const float* params, float q, const char** error);
+int ga_router_enabled(void);
+
 typedef struct ga_sw_dev_batch {
     const uint8_t* text;    const int64_t* text_off;    const int32_t* text_len;
     const uint8_t* pattern; const int64_t* pattern_off; const int32_t* pattern_len;

@@ -145,6 +145,22 @@ def test_refusals_match_reference():
             assert st == 0
 
 
+def test_router_set_refusals_and_unload():
+    """ga_router_set: a quantile above 1 is refused before any device work;
+    NULL unloads (no device needed)."""
+    import genoaligner_amd as ga
+    from genoaligner_amd import _native as N
+    L = N.lib()
+    err = C.c_char_p()
+    params = np.zeros(N.GA_MLP_PARAMS, np.float32)
+    assert L.ga_router_set(params.ctypes.data, 1.5, C.byref(err)) == N.GA_INVALID_ARGUMENT
+    assert b"quantile" in err.value
+    assert L.ga_router_set(None, 0.95, C.byref(err)) == N.GA_OK
+    assert L.ga_router_enabled() == 0
+    with pytest.raises(ValueError):
+        ga.set_router(np.zeros(10, np.float32))
+
+
 def test_blosum62_table():
     """The library's BLOSUM62 equals the reference's table
     (src/msa/msa_ref.cpp:116-137): the published matrix except row M, column H,

@@ -137,6 +137,47 @@ def test_routed_equals_plain_for_any_network(gpu, oracle):
     assert [int(x) for x in sb.results()[0]] == psc
 
 
+def test_library_router_batches_equal_plain(gpu):
+    """With a network loaded into the library (ga_router_set), align_batch_ext
+    and align_batch route their WFA batches and return exactly what they
+    return without one: the shipped network, an untrained one and one that
+    sends every pair to bucket 0 (every pair re-run); mixed CIGAR flags and
+    pairs of up to 10 kbp at smax up to 2048."""
+    import genoaligner_amd as ga
+    from genoaligner_amd import router as R
+    from genoaligner_amd import workloads as W
+    ps = W.wfa_routing(n=1000, lo=60, hi=2500, seed=78)
+    pl = W.wfa_mixed(n=60, lo=5000, hi=10000, seed=79)
+    rng = np.random.default_rng(5)
+
+    def reqs_of(p, smax_cap):
+        return [ga.AlignRequest(pattern=p.a_seq(i), pattern_len=len(p.a_seq(i)), text=p.b_seq(i),
+                                text_len=len(p.b_seq(i)), smax=min(int(p.smax[i]), smax_cap),
+                                with_cigar=bool(rng.random() < 0.7)) for i in range(p.n)]
+
+    ext = reqs_of(ps, 4000) + reqs_of(pl, 4000)
+    short = reqs_of(ps, 300)[:400]
+    fields = ("score", "resolved", "cigar", "rescore_ok", "wellformed_ok")
+
+    def run():
+        a, b = ga.align_batch_ext(ext), ga.align_batch(short)
+        assert a.ok() and b.ok(), (a.error, b.error)
+        return [tuple(getattr(r, f) for f in fields) for r in a.results + b.results]
+
+    ga.set_router(None)
+    want = run()
+    low = np.zeros(R.PARAMS, np.float32)
+    low[-16] = 50.0
+    try:
+        for params in (R.load(), R.init_params(4), low):
+            ga.set_router(params, 0.95)
+            assert ga.router_enabled()
+            assert run() == want
+    finally:
+        ga.set_router(None)
+    assert not ga.router_enabled()
+
+
 def test_training_learns_distance_buckets(gpu):
     """A short GPU training run on exact-aligner labels lowers the loss and beats
     the majority-bucket predictor on held-out pairs; routing with the trained

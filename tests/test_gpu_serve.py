@@ -18,15 +18,25 @@ def _same(a, b):
     return len(a) == len(b)
 
 
-def test_results_alone_equal_results_in_the_mix(gpu, oracle):
+@pytest.mark.parametrize("route", [False, True])
+def test_results_alone_equal_results_in_the_mix(gpu, oracle, route):
+    """route=True: the slots' WFA batches take the library's routed path (the
+    shipped network loaded); the alone results stay unrouted."""
     import genoaligner_amd as ga
     from genoaligner_amd import workloads as W
-    from genoaligner_amd.serve import Server
+    from genoaligner_amd.serve import LibraryBackend, Server
     rng = np.random.default_rng(31)
     reqs = [W.serving_request(rng, ga, max_pairs=48, wfa_len=(100, 1500), sw_q=(50, 300), sw_t=(50, 500))
             for _ in range(30)]
     m = ga.blosum62(11, 1)
-    s = Server(budget_bytes=4 << 30, slots=3, start=False)
+    try:
+        _mix_vs_alone(ga, W, Server(budget_bytes=4 << 30, slots=3, start=False,
+                                    backend=LibraryBackend(route=route)), reqs, m, rng, oracle)
+    finally:
+        ga.set_router(None)
+
+
+def _mix_vs_alone(ga, W, s, reqs, m, rng, oracle):
     tickets = [s.submit(kind, items, with_cigar=cig) for kind, items, cig in reqs[:12]]
     s.start()                                         # 12 queued requests batch together
     for kind, items, cig in reqs[12:]:                # the rest arrive while slots are busy
@@ -35,6 +45,7 @@ def test_results_alone_equal_results_in_the_mix(gpu, oracle):
     for t in tickets:
         assert t.wait(120)
     s.close()
+    ga.set_router(None)                               # the alone runs take the plain path
     assert len(s.batches) < len(reqs)                 # requests were batched together
     for (kind, items, cig), t in zip(reqs, tickets):
         assert t.error is None, t.error
