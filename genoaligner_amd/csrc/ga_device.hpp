@@ -67,6 +67,13 @@ hipError_t wfa_plan_launch(const WfaDevPlan& P, hipStream_t st);
 
 size_t wfa_lds_bytes(const WfaQueueLaunch& L);     // dynamic LDS of one workgroup
 int wfa_queue_groups(const WfaQueueLaunch& L);     // resident workgroups (<= n)
+// True when a launch is scheduled statically throughout (short pairs of the
+// smallest class, at most 8 per workgroup): it never reads its claim counter.
+inline bool wfa_fully_static(const WfaQueueLaunch& L)
+{
+    return L.smax_cls <= kWfaClassBound0 && L.max_plen <= 1024 && L.max_tlen <= 1024 && !L.dev_class &&
+           int64_t(L.n) <= int64_t(8) * L.n_groups;
+}
 size_t wfa_queue_ws_bytes(int smax_cls, int n_groups, bool wide);
 hipError_t wfa_queue_launch(const WfaQueueLaunch& L, hipStream_t st);
 // p[0 .. n) = 0 by a kernel (graph-capturable, unlike a memset node)

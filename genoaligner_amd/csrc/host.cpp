@@ -653,9 +653,8 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
         drop_queue();
         return "hipMalloc(WFA queue)";
     }
-    if ((!identity && hipMemcpyAsync(d_order, plan.order.data(), plan.order.size() * 4, hipMemcpyHostToDevice,
-                                     st) != hipSuccess) ||
-        ga::zero_i32_launch(d_next, int(nc), st) != hipSuccess) {
+    if (!identity && hipMemcpyAsync(d_order, plan.order.data(), plan.order.size() * 4, hipMemcpyHostToDevice,
+                                    st) != hipSuccess) {
         drop_queue();
         return "hipMemcpy(WFA queue)";
     }
@@ -683,6 +682,10 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
         if (trace && ws_alloc(&L.ws, size_t(per_group) * size_t(groups), st) != hipSuccess)
             err = "hipMalloc(wavefront stacks)";
     }
+    // claim counters, unless every class is scheduled statically throughout
+    bool dynamic = false;
+    for (size_t c = 0; c < nc; ++c) dynamic = dynamic || !ga::wfa_fully_static(Ls[c]);
+    if (!err && dynamic && ga::zero_i32_launch(d_next, int(nc), st) != hipSuccess) err = "WFA counter launch";
     if (timer && !err) timer->begin(st);
     hipEvent_t fork = nullptr;
     std::vector<hipStream_t> side;
@@ -725,6 +728,7 @@ struct WfaDevClasses {
     int32_t* order = nullptr;
     int32_t* next = nullptr;
     ga::WfaDevPlan plan{};
+    bool planless = false;      // one class holds every pair: no plan kernels
 };
 
 // `stack_cap`: bytes of wavefront stacks one class launch may hold (its
@@ -746,6 +750,9 @@ const char* setup_wfa_dev_classes(const ga::WfaQueueLaunch& base, int64_t n, int
     C.plan.n = int32_t(n);
     C.plan.order = C.order;
     C.plan.next = C.next;
+    // every smax within the smallest class: one launch over the pairs in
+    // index order, no plan (the claim order only balances long pairs)
+    C.planless = smax_max <= kWfaClassBound[0];
     for (int c = 0; c < ga::kWfaClasses; ++c) {
         if (c > 0 && smax_max <= kWfaClassBound[c - 1]) break;        // no pair can fall in class c
         ga::WfaQueueLaunch L = base;
@@ -756,9 +763,9 @@ const char* setup_wfa_dev_classes(const ga::WfaQueueLaunch& base, int64_t n, int
         L.staged = true;
         if (ga::wfa_lds_bytes(L) > kWfaMaxStagedLds) L.staged = false;
         L.n = int32_t(n);
-        L.order = C.order;
+        L.order = C.planless ? nullptr : C.order;
         L.next = C.next + c;
-        L.dev_class = C.plan.cls + 2 * c;
+        L.dev_class = C.planless ? nullptr : C.plan.cls + 2 * c;
         int groups = ga::wfa_queue_groups(L);
         const int64_t per_group = int64_t(ga::wfa_queue_ws_bytes(L.smax_cls, 1, L.wide));
         if (trace) groups = int(std::max<int64_t>(1, std::min<int64_t>(groups, stack_cap / per_group)));
@@ -776,7 +783,7 @@ const char* setup_wfa_dev_classes(const ga::WfaQueueLaunch& base, int64_t n, int
 const char* enqueue_wfa_dev(const WfaDevClasses& C, hipStream_t st, const std::vector<hipStream_t>& side,
                             std::vector<hipEvent_t>& ev)
 {
-    if (ga::wfa_plan_launch(C.plan, st) != hipSuccess) return "WFA plan launch";
+    if (!C.planless && ga::wfa_plan_launch(C.plan, st) != hipSuccess) return "WFA plan launch";
     const size_t nc = C.L.size();
     // ev[0] forks, ev[1..] join
     if (nc > 1 && hipEventRecord(ev[0], st) != hipSuccess) return "hipEventRecord";
@@ -1604,8 +1611,19 @@ int ga_wfa_device(const ga_wfa_dev_batch* b, void* stream, const char** error)
     t_timer.reset();
     const char* err = nullptr;
     if (b->host_smax && b->host_pattern_len && b->host_text_len) {
-        // host plan: exact class maxima, sizes known before launch
-        const WfaPlan plan = plan_wfa(b->n_pairs, b->host_smax, b->host_pattern_len, b->host_text_len);
+        // host plan: exact class maxima, sizes known before launch; a batch
+        // whose every smax is in the smallest class needs no pass over its
+        // pairs (one class in index order, bounded by smax_max and max_len)
+        WfaPlan plan;
+        if (b->smax_max <= kWfaClassBound[0]) {
+            plan.bound = {b->smax_max};
+            plan.max_plen = {b->max_len};
+            plan.max_tlen = {b->max_len};
+            plan.first = {0};
+            plan.count = {b->n_pairs};
+        } else {
+            plan = plan_wfa(b->n_pairs, b->host_smax, b->host_pattern_len, b->host_text_len);
+        }
         err = enqueue_wfa(L, plan, b->cigar != nullptr, st, &t_timer);
     } else {
         // device plan: classes and claim order computed on the GPU

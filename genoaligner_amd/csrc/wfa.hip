@@ -301,6 +301,9 @@ void wfa_kernel(WfaQueueLaunch L)
         if (q >= n_cls) break;                    // every wave ends here once the queue is empty
         const int pid = order ? order[q] : q;
 
+        // smax < 0: the pair is not in this pass (a launch without a plan runs
+        // every pair in index order; the plans leave such pairs out)
+        if (L.smax[pid] < 0) continue;
         const int m = L.plen[pid];
         const int n = L.tlen[pid];
         const int smax = L.smax[pid] < S ? L.smax[pid] : S;
@@ -655,7 +658,7 @@ hipError_t wfa_queue_launch(const WfaQueueLaunch& L, hipStream_t st)
     // the claim counter is zeroed by a kernel, not a memset node, so this
     // submission can be captured into a graph and replayed (device-planned
     // classes: the plan kernels zero it)
-    if (!L.dev_class && !L.counter_zeroed) {
+    if (!L.dev_class && !L.counter_zeroed && !wfa_fully_static(L)) {
         hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, st, L.next, 1);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
