@@ -106,8 +106,20 @@ __device__ __forceinline__ uint32_t dir_at(const uint32_t* d, int tq, WalkWin& W
         W.s = s;
         W.lo = wr - (kWinRows - 1) > 0 ? wr - (kWinRows - 1) : 0;
         __syncthreads();
-        for (int q = 0; q < kWinRows; ++q)
-            if (W.lo + q < tq) W.w[q * 64 + lane] = d[(int64_t(s) * tq + W.lo + q) * 64 + lane];
+        // every load issued before any is waited for (rows past the table
+        // read its last row; the walk never reads them)
+        const uint32_t* ds = d + int64_t(s) * tq * 64 + lane;
+#pragma unroll
+        for (int q0 = 0; q0 < kWinRows; q0 += 8) {
+            uint32_t val[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int r = W.lo + q0 + q < tq ? W.lo + q0 + q : tq - 1;
+                val[q] = ds[uint32_t(r) * 64u];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) W.w[(q0 + q) * 64 + lane] = val[q];
+        }
         __syncthreads();
     }
     const uint32_t word = W.w[(wr - W.lo) * 64 + l];

@@ -147,10 +147,18 @@ __device__ int walk16(const uint32_t* dirs, int nq, uint32_t* win, bool walker, 
             if (jj >= 1) {
                 const int l = (jj - 1) & 63, s = (jj - 1) >> 6;
                 const int q0 = (ih - kWinR + l) >> 2;   // floor
+                // every load issued before any is waited for (words outside
+                // the table read word 0 and are replaced by 0)
+                uint32_t val[kWinQ];
 #pragma unroll
                 for (int k = 0; k < kWinQ; ++k) {
                     const int q = q0 + k;
-                    w[c * kWinQ + k] = (q >= 0 && q < nq) ? dirs[(int64_t(s) * nq + q) * 64 + l] : 0u;
+                    val[k] = dirs[(int64_t(s) * nq + (q >= 0 && q < nq ? q : 0)) * 64 + l];
+                }
+#pragma unroll
+                for (int k = 0; k < kWinQ; ++k) {
+                    const int q = q0 + k;
+                    w[c * kWinQ + k] = (q >= 0 && q < nq) ? val[k] : 0u;
                 }
             }
         }

@@ -153,20 +153,30 @@ __device__ void walk_duo(const Sw16Launch& L, const Duo& D, int g, int lane, int
         if (need_h) {
             const int jj = jh - c;                // 1-based column of this lane
             const int ii = ih - c;                // 1-based row of this lane
+            // every load issued before any is waited for: out-of-range words
+            // and bytes read index 0 and are discarded
+            const int jc = jj >= 1 ? jj : 1, ic = ii >= 1 ? ii : 1;
+            const int l = (jc - 1) & 63, s = (jc - 1) >> 6;
+            const int q0 = (ih - kWinR + l) >> 2;       // floor
+            uint32_t val[kWinQ];
+#pragma unroll
+            for (int k = 0; k < kWinQ; ++k) {
+                const int q = q0 + k;
+                val[k] = dirs[(int64_t(s) * nq + (q >= 0 && q < nq ? q : 0)) * 64 + l];
+            }
+            const uint8_t pv = P[jc - 1], pcv = PC[jc - 1], tv = T[ic - 1], tcv = TC[ic - 1];
             if (jj >= 1) {
-                const int l = (jj - 1) & 63, s = (jj - 1) >> 6;
-                const int q0 = (ih - kWinR + l) >> 2;   // floor
 #pragma unroll
                 for (int k = 0; k < kWinQ; ++k) {
                     const int q = q0 + k;
-                    w[c * kWinQ + k] = (q >= 0 && q < nq) ? dirs[(int64_t(s) * nq + q) * 64 + l] : 0u;
+                    w[c * kWinQ + k] = (q >= 0 && q < nq) ? val[k] : 0u;
                 }
-                wp[hw][c] = P[jj - 1];
-                wpc[hw][c] = PC[jj - 1];
+                wp[hw][c] = pv;
+                wpc[hw][c] = pcv;
             }
             if (ii >= 1) {
-                wt[hw][c] = T[ii - 1];
-                wtc[hw][c] = TC[ii - 1];
+                wt[hw][c] = tv;
+                wtc[hw][c] = tcv;
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -294,9 +304,21 @@ __global__ __launch_bounds__(64 * kWaves) void sw16m_kernel(Sw16Launch L)
         uint2* bw = bnd + (s & 1) * Mr;
         const int j = (s << 6) + lane;
         const uint32_t pa = j < D.nAp ? PCA[j] : 0u, pbc = j < D.nBp ? PCB[j] : 0u;
-        for (int code = 0; code < 32; ++code) {
-            prof[code * 64 + lane] = uint8_t(j < D.nAp ? L.sub[code * 32 + pa] + bias : 0);
-            prof[kTabBytes + code * 64 + lane] = uint8_t(j < D.nBp ? L.sub[code * 32 + pbc] + bias : 0);
+        // the lane's two score columns: 64 table reads, 16 in flight at a time
+#pragma unroll
+        for (int c0 = 0; c0 < 32; c0 += 8) {
+            int32_t sa[8], sb[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                sa[u] = L.sub[(c0 + u) * 32 + pa];
+                sb[u] = L.sub[(c0 + u) * 32 + pbc];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                prof[(c0 + u) * 64 + lane] = uint8_t(j < D.nAp ? sa[u] + bias : 0);
+                prof[kTabBytes + (c0 + u) * 64 + lane] = uint8_t(j < D.nBp ? sb[u] + bias : 0);
+            }
         }
         prof[kZeroCode * 64 + lane] = 0;
         prof[kTabBytes + kZeroCode * 64 + lane] = 0;

@@ -163,6 +163,7 @@ __host__ __device__ inline int wfa_seq_bytes(int len) { return ((((len > 0 ? len
 // reads and walked out of LDS for up to kTbRows steps.
 constexpr int kTbRows = 16;
 constexpr int kTbCols = 2 * kTbRows + 1;
+constexpr int kTbBatch = 5;          // window loads a lane has in flight at once
 __host__ __device__ inline int wfa_win_bytes(int off_bytes) { return (kTbRows * kTbCols * off_bytes + 15) & ~15; }
 
 // Forward pass over one pair: wavefronts s = 1 .. smax until W[s][n-m] >= n.
@@ -379,12 +380,29 @@ void wfa_kernel(WfaQueueLaunch L)
                             d = 0;
                             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                             __builtin_amdgcn_wave_barrier();
-#pragma unroll 3
-                            for (int e = lane; e < kTbRows * kTbCols; e += 64) {
-                                const int r = top - 1 - e / kTbCols;
-                                const int c = k0 - kTbRows + e % kTbCols;
-                                win[e] = (r >= 0 && c >= -r && c <= r) ? ws[int64_t(r) * r + c + r]
-                                                                        : Off(OffNull<Off>::v);
+                            // kTbBatch unconditional loads in flight at a time (an
+                            // entry outside the stack reads entry 0 and is replaced
+                            // by NULL), not one load per round trip
+#pragma unroll 1
+                            for (int e0 = lane; e0 < kTbRows * kTbCols; e0 += 64 * kTbBatch) {
+                                Off x[kTbBatch];
+#pragma unroll
+                                for (int u = 0; u < kTbBatch; ++u) {
+                                    const int e = e0 + 64 * u;
+                                    const int r = top - 1 - e / kTbCols;
+                                    const int c = k0 - kTbRows + e % kTbCols;
+                                    const bool ok = e < kTbRows * kTbCols && r >= 0 && c >= -r && c <= r;
+                                    x[u] = ws[ok ? uint32_t(r * r + c + r) : 0u];
+                                }
+                                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                                for (int u = 0; u < kTbBatch; ++u) {
+                                    const int e = e0 + 64 * u;
+                                    const int r = top - 1 - e / kTbCols;
+                                    const int c = k0 - kTbRows + e % kTbCols;
+                                    const bool ok = r >= 0 && c >= -r && c <= r;
+                                    if (e < kTbRows * kTbCols) win[e] = ok ? x[u] : Off(OffNull<Off>::v);
+                                }
                             }
                             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                             __builtin_amdgcn_wave_barrier();
