@@ -42,12 +42,16 @@ struct WfaQueueLaunch {
     int32_t n_groups;
     int32_t* score;
     uint8_t* cigar; const int64_t* cig_off; int32_t* cig_len;
-    void* ws;                // n_groups * (smax_cls+1)^2 entries of int16 (int32 if wide)
+    void* ws;                // n_groups * ws_stride entries of int16 (int32 if wide)
+    int64_t ws_stride;       // entries per workgroup, >= (smax_cls+1)^2 (0: exactly that)
     bool wide;
     bool staged;
     const int32_t* dev_class; // device-planned: {first, count} of this class (then n, order ignored / offset)
     bool counter_zeroed;      // host-planned: `next` already zeroed on the stream (else the launcher zeroes it)
 };
+
+// Deferred walks: pairs a workgroup queues before walking them together.
+constexpr int kWfaPend = 32;
 
 // Size classes by smax (device planning uses the same bounds as the host plan).
 constexpr int kWfaClasses = 4;

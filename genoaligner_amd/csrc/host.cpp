@@ -533,6 +533,37 @@ int64_t wfa_stack_budget()
 {
     return t_workspace_budget > 0 ? std::max<int64_t>(t_workspace_budget / 4, 1 << 20) : kWfaStackBudget;
 }
+// Launches with several waves per workgroup and CIGARs defer their walks: each
+// workgroup's stack region holds up to kWfaDeferMult worst-case stacks (as the
+// budget allows), so several pairs' walks run together on all its waves.
+constexpr int64_t kWfaDeferBudget = int64_t(32) << 30;   // bytes of stack regions per class launch
+constexpr int64_t kWfaDeferMult = 4;
+
+// The stack bytes a class launch may take with deferred walks: the default
+// deferral budget unless a thread budget (serving) or a caller's region caps it.
+int64_t wfa_defer_cap(int64_t stack_cap, bool hard_cap)
+{
+    return (hard_cap || t_workspace_budget > 0) ? stack_cap : std::max(stack_cap, kWfaDeferBudget);
+}
+
+// Workgroups and stack region of a class launch (sets n_groups, ws_stride):
+// workgroups capped so one worst-case stack each fits stack_cap, regions of up
+// to kWfaDeferMult stacks within defer_cap.  Returns the region bytes per
+// workgroup.
+int64_t wfa_size_stacks(ga::WfaQueueLaunch& L, bool trace, int64_t stack_cap, int64_t defer_cap)
+{
+    int groups = ga::wfa_queue_groups(L);
+    const int64_t one = int64_t(ga::wfa_queue_ws_bytes(L.smax_cls, 1, L.wide));
+    int64_t mult = 1;
+    if (trace) {
+        groups = int(std::max<int64_t>(1, std::min<int64_t>(groups, stack_cap / one)));
+        if (L.waves > 1)
+            mult = std::max<int64_t>(1, std::min<int64_t>(kWfaDeferMult, defer_cap / (int64_t(groups) * one)));
+    }
+    L.n_groups = groups;
+    L.ws_stride = one * mult / (L.wide ? 4 : 2);
+    return one * mult;
+}
 
 struct WfaPlan {
     std::vector<int32_t> order;                 // pair ids, class by class (empty: identity)
@@ -674,12 +705,9 @@ const char* enqueue_wfa(const ga::WfaQueueLaunch& base, const WfaPlan& plan, boo
         L.waves = wfa_class_waves(L.smax_cls, plan.count[c]);
         L.staged = true;
         if (ga::wfa_lds_bytes(L) > kWfaMaxStagedLds) L.staged = false;
-        int groups = ga::wfa_queue_groups(L);
-        const int64_t per_group = int64_t(ga::wfa_queue_ws_bytes(L.smax_cls, 1, L.wide));
-        if (trace) groups = int(std::max<int64_t>(1, std::min<int64_t>(groups, wfa_stack_budget() / per_group)));
-        L.n_groups = groups;
+        const int64_t per_group = wfa_size_stacks(L, trace, wfa_stack_budget(), wfa_defer_cap(wfa_stack_budget(), false));
         L.ws = nullptr;
-        if (trace && ws_alloc(&L.ws, size_t(per_group) * size_t(groups), st) != hipSuccess)
+        if (trace && ws_alloc(&L.ws, size_t(per_group) * size_t(L.n_groups), st) != hipSuccess)
             err = "hipMalloc(wavefront stacks)";
     }
     // claim counters, unless every class is scheduled statically throughout
@@ -739,6 +767,7 @@ const char* setup_wfa_dev_classes(const ga::WfaQueueLaunch& base, int64_t n, int
                                   int64_t stack_cap = -1,
                                   const std::function<void*(size_t)>* stack_alloc = nullptr)
 {
+    const bool hard_cap = stack_cap >= 0;            // a caller's region: no deferral beyond it
     if (stack_cap < 0) stack_cap = wfa_stack_budget();
     if (!stack_alloc) stack_alloc = &alloc;
     C.order = static_cast<int32_t*>(alloc(size_t(n) * 4 + 64));
@@ -766,11 +795,8 @@ const char* setup_wfa_dev_classes(const ga::WfaQueueLaunch& base, int64_t n, int
         L.order = C.planless ? nullptr : C.order;
         L.next = C.next + c;
         L.dev_class = C.planless ? nullptr : C.plan.cls + 2 * c;
-        int groups = ga::wfa_queue_groups(L);
-        const int64_t per_group = int64_t(ga::wfa_queue_ws_bytes(L.smax_cls, 1, L.wide));
-        if (trace) groups = int(std::max<int64_t>(1, std::min<int64_t>(groups, stack_cap / per_group)));
-        L.n_groups = groups;
-        L.ws = trace ? (*stack_alloc)(size_t(per_group) * size_t(groups)) : nullptr;
+        const int64_t per_group = wfa_size_stacks(L, trace, stack_cap, wfa_defer_cap(stack_cap, hard_cap));
+        L.ws = trace ? (*stack_alloc)(size_t(per_group) * size_t(L.n_groups)) : nullptr;
         if (trace && !L.ws) return "hipMalloc(wavefront stacks)";
         C.L.push_back(L);
     }
@@ -2132,13 +2158,10 @@ int ga_wfa_autotune(const ga_wfa_dev_batch* b, int reps, ga_wfa_tuning* out, voi
             L.max_tlen = plan.max_tlen[c];
             L.waves = w;
             L.staged = ga::wfa_lds_bytes(L) <= kWfaMaxStagedLds;
-            int groups = ga::wfa_queue_groups(L);
-            const int64_t per_group = int64_t(ga::wfa_queue_ws_bytes(L.smax_cls, 1, L.wide));
             const bool trace = b->cigar != nullptr;
-            if (trace) groups = int(std::max<int64_t>(1, std::min<int64_t>(groups, wfa_stack_budget() / per_group)));
-            L.n_groups = groups;
+            const int64_t per_group = wfa_size_stacks(L, trace, wfa_stack_budget(), wfa_defer_cap(wfa_stack_budget(), false));
             L.ws = nullptr;
-            if (trace && ws_alloc(&L.ws, size_t(per_group) * size_t(groups), st) != hipSuccess) {
+            if (trace && ws_alloc(&L.ws, size_t(per_group) * size_t(L.n_groups), st) != hipSuccess) {
                 err = "hipMalloc(wavefront stacks)";
                 break;
             }

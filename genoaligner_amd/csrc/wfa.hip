@@ -235,6 +235,100 @@ __device__ __forceinline__ int wfa_forward(const uint32_t* sP, const uint32_t* s
     return -1;
 }
 
+// The reference's traceback (wfa_kernel.hip:338-437) of one pair on one whole
+// wave (wave-uniform state): from the end of diagonal n - m at `score`, over
+// the pair's stack (row s at entry s^2, diagonal k at s + k), emitting the
+// CIGAR right-aligned into cg[0 .. m + n).  Returns its length.  win:
+// kTbRows x kTbCols entries of LDS for the wave's windows.
+template <typename Off>
+__device__ int wfa_walk(const Off* ws, Off* win, uint8_t* const cg, int m, int n, int score, int lane)
+{
+    const int ak = n - m;
+    int pos = m + n;
+    auto emit = [&](uint8_t c, int cnt) {
+        if (cnt > pos) cnt = pos;
+        if (cnt <= 0) return;
+        for (int x = lane; x < cnt; x += 64) cg[pos - 1 - x] = c;
+        pos -= cnt;
+    };
+    if (score == 0) {
+        emit('M', n);
+    } else {
+        int sc = score, k = ak;
+        int offset = ws_get(ws, score, k);
+        int h = offset, v = offset - k;
+        emit('D', v - m);
+        emit('I', h - n);
+        if (v > m) v = m;
+        if (h > n) h = n;
+        int top = 0, k0 = 0, d = kTbRows;
+        while (v > 0 && h > 0 && sc > 0) {
+            if (d == kTbRows) {
+                // rows sc-1 .. sc-kTbRows, diagonals k-kTbRows .. k+kTbRows
+                top = sc;
+                k0 = k;
+                d = 0;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                // kTbBatch unconditional loads in flight at a time (an
+                // entry outside the stack reads entry 0 and is replaced
+                // by NULL), not one load per round trip
+#pragma unroll 1
+                for (int e0 = lane; e0 < kTbRows * kTbCols; e0 += 64 * kTbBatch) {
+                    Off x[kTbBatch];
+#pragma unroll
+                    for (int u = 0; u < kTbBatch; ++u) {
+                        const int e = e0 + 64 * u;
+                        const int r = top - 1 - e / kTbCols;
+                        const int c = k0 - kTbRows + e % kTbCols;
+                        const bool ok = e < kTbRows * kTbCols && r >= 0 && c >= -r && c <= r;
+                        x[u] = ws[ok ? uint32_t(r * r + c + r) : 0u];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < kTbBatch; ++u) {
+                        const int e = e0 + 64 * u;
+                        const int r = top - 1 - e / kTbCols;
+                        const int c = k0 - kTbRows + e % kTbCols;
+                        const bool ok = r >= 0 && c >= -r && c <= r;
+                        if (e < kTbRows * kTbCols) win[e] = ok ? x[u] : Off(OffNull<Off>::v);
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            const Off* wr = win + d * kTbCols + (k - k0 + kTbRows);
+            const int misms = int(wr[0]) + 1;
+            const int ins = int(wr[-1]) + 1;
+            const int del = int(wr[1]);
+            int best = kWfaNeg, op = 0;
+            if (misms >= 0 && misms >= best) { best = misms; op = 1; }
+            if (ins >= 0 && ins > best) { best = ins; op = 2; }
+            if (del >= 0 && del > best) { best = del; op = 3; }
+            if (op == 0) break;
+            emit('M', offset - best);
+            offset = best;
+            if (op == 1)      { emit('X', 1); --offset; }
+            else if (op == 2) { emit('I', 1); --k; --offset; }
+            else              { emit('D', 1); ++k; }
+            v = offset - k;
+            h = offset;
+            --sc;
+            ++d;
+        }
+        if (v > 0 && h > 0) {
+            const int nm = v < h ? v : h;
+            emit('M', nm);
+            v -= nm;
+            h -= nm;
+        }
+        emit('D', v);
+        emit('I', h);
+    }
+    return m + n - pos;
+}
+
 // Eight waves per SIMD: <= 64 VGPRs and <= 96 SGPRs, so four 8-wave
 // workgroups (or 32 single waves) fit a CU when LDS allows.
 template <bool TRACE, typename Off, bool STAGED, int WAVES>
@@ -258,8 +352,34 @@ void wfa_kernel(WfaQueueLaunch L)
     Off* A = reinterpret_cast<Off*>(wfs);
     Off* B = reinterpret_cast<Off*>(wfs + wfb);
     Off* win = reinterpret_cast<Off*>(wfs + 2 * wfb);
-    Off* ws = TRACE ? static_cast<Off*>(L.ws) + int64_t(blockIdx.x) * (int64_t(S) + 1) * (int64_t(S) + 1)
-                    : nullptr;
+    // the workgroup's stack region: ws_stride entries (at least one worst-case
+    // stack of (S + 1)^2 entries; more when walks are deferred)
+    const int64_t stride = L.ws_stride > 0 ? L.ws_stride : (int64_t(S) + 1) * (int64_t(S) + 1);
+    Off* ws = TRACE ? static_cast<Off*>(L.ws) + int64_t(blockIdx.x) * stride : nullptr;
+    // Deferred walks (several waves per workgroup): each traced pair's stack is
+    // left in the region and its walk queued in LDS; when the region cannot
+    // hold the next pair's worst case (or the queue is full) all waves walk
+    // the queued pairs, wave w taking pairs w, w + WAVES, ..., each with a
+    // window in the wavefront buffers (free between pairs).  So no wave waits
+    // while one walks.
+    const int winb = wfa_win_bytes(int(sizeof(Off)));
+    const bool defer = TRACE && WAVES > 1 && 2 * wfb >= WAVES * winb;
+    int* pend = reinterpret_cast<int*>(wfs + 2 * wfb + winb);
+    int npend = 0;
+    int64_t used = 0;
+    auto walk_pending = [&]() {
+        Off* wwin = reinterpret_cast<Off*>(wfs + wave * winb);
+        for (int i = wave; i < npend; i += WAVES) {
+            const int pid = pend[3 * i];
+            const int64_t off = int64_t(uint32_t(pend[3 * i + 1]));
+            const int sc = pend[3 * i + 2];
+            const int len = wfa_walk<Off>(ws + off, wwin, L.cigar + L.cig_off[pid], L.plen[pid], L.tlen[pid], sc,
+                                          lane);
+            if (lane == 0) L.cig_len[pid] = len;
+        }
+        npend = 0;
+        used = 0;
+    };
     const int G = int(gridDim.x);
     // class size and the start of its slice of the claim order: from the launch,
     // or (device-planned batches) from the plan the device wrote
@@ -333,6 +453,16 @@ void wfa_kernel(WfaQueueLaunch L)
             }
             continue;
         }
+        Off* wsp = ws;
+        if (defer && trace) {
+            // room for this pair's worst case: (smax + 1)^2 entries
+            const int64_t need = ((int64_t(smax) + 1) * (smax + 1) + 1) & ~int64_t(1);
+            if (used + need > stride || npend == kWfaPend) {
+                walk_pending();
+                __syncthreads();                   // the walks' windows live in A and B
+            }
+            wsp = ws + used;
+        }
         for (int x = tid; x < W; x += NT) {
             A[x] = Off(OffNull<Off>::v);
             B[x] = Off(OffNull<Off>::v);
@@ -346,102 +476,33 @@ void wfa_kernel(WfaQueueLaunch L)
             __syncthreads();
         }
         const int ak = n - m;
-        const int score = packed ? wfa_forward<TRACE, Off, true, NT>(sP, sT, P, T, m, n, smax, S, A, B, ws, trace, tid)
-                                 : wfa_forward<TRACE, Off, false, NT>(sP, sT, P, T, m, n, smax, S, A, B, ws, trace, tid);
+        const int score = packed ? wfa_forward<TRACE, Off, true, NT>(sP, sT, P, T, m, n, smax, S, A, B, wsp, trace, tid)
+                                 : wfa_forward<TRACE, Off, false, NT>(sP, sT, P, T, m, n, smax, S, A, B, wsp, trace, tid);
         if (tid == 0) L.score[pid] = score;
         if (TRACE && wave == 0) {
             if (!trace || score < 0) {
                 if (lane == 0 && L.cig_len) L.cig_len[pid] = 0;
-            } else {
-                uint8_t* const cg = L.cigar + L.cig_off[pid];
-                int pos = m + n;
-                auto emit = [&](uint8_t c, int cnt) {
-                    if (cnt > pos) cnt = pos;
-                    if (cnt <= 0) return;
-                    for (int x = lane; x < cnt; x += 64) cg[pos - 1 - x] = c;
-                    pos -= cnt;
-                };
-                if (score == 0) {
-                    emit('M', n);
-                } else {
-                    int sc = score, k = ak;
-                    int offset = ws_get(ws, score, k);
-                    int h = offset, v = offset - k;
-                    emit('D', v - m);
-                    emit('I', h - n);
-                    if (v > m) v = m;
-                    if (h > n) h = n;
-                    int top = 0, k0 = 0, d = kTbRows;
-                    while (v > 0 && h > 0 && sc > 0) {
-                        if (d == kTbRows) {
-                            // rows sc-1 .. sc-kTbRows, diagonals k-kTbRows .. k+kTbRows
-                            top = sc;
-                            k0 = k;
-                            d = 0;
-                            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                            __builtin_amdgcn_wave_barrier();
-                            // kTbBatch unconditional loads in flight at a time (an
-                            // entry outside the stack reads entry 0 and is replaced
-                            // by NULL), not one load per round trip
-#pragma unroll 1
-                            for (int e0 = lane; e0 < kTbRows * kTbCols; e0 += 64 * kTbBatch) {
-                                Off x[kTbBatch];
-#pragma unroll
-                                for (int u = 0; u < kTbBatch; ++u) {
-                                    const int e = e0 + 64 * u;
-                                    const int r = top - 1 - e / kTbCols;
-                                    const int c = k0 - kTbRows + e % kTbCols;
-                                    const bool ok = e < kTbRows * kTbCols && r >= 0 && c >= -r && c <= r;
-                                    x[u] = ws[ok ? uint32_t(r * r + c + r) : 0u];
-                                }
-                                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                                for (int u = 0; u < kTbBatch; ++u) {
-                                    const int e = e0 + 64 * u;
-                                    const int r = top - 1 - e / kTbCols;
-                                    const int c = k0 - kTbRows + e % kTbCols;
-                                    const bool ok = r >= 0 && c >= -r && c <= r;
-                                    if (e < kTbRows * kTbCols) win[e] = ok ? x[u] : Off(OffNull<Off>::v);
-                                }
-                            }
-                            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                            __builtin_amdgcn_wave_barrier();
-                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                        }
-                        const Off* wr = win + d * kTbCols + (k - k0 + kTbRows);
-                        const int misms = int(wr[0]) + 1;
-                        const int ins = int(wr[-1]) + 1;
-                        const int del = int(wr[1]);
-                        int best = kWfaNeg, op = 0;
-                        if (misms >= 0 && misms >= best) { best = misms; op = 1; }
-                        if (ins >= 0 && ins > best) { best = ins; op = 2; }
-                        if (del >= 0 && del > best) { best = del; op = 3; }
-                        if (op == 0) break;
-                        emit('M', offset - best);
-                        offset = best;
-                        if (op == 1)      { emit('X', 1); --offset; }
-                        else if (op == 2) { emit('I', 1); --k; --offset; }
-                        else              { emit('D', 1); ++k; }
-                        v = offset - k;
-                        h = offset;
-                        --sc;
-                        ++d;
-                    }
-                    if (v > 0 && h > 0) {
-                        const int nm = v < h ? v : h;
-                        emit('M', nm);
-                        v -= nm;
-                        h -= nm;
-                    }
-                    emit('D', v);
-                    emit('I', h);
+            } else if (defer && score > 0) {
+                // walked later with other pairs of this workgroup (all waves)
+                if (lane == 0) {
+                    pend[3 * npend] = pid;
+                    pend[3 * npend + 1] = int(uint32_t(used));
+                    pend[3 * npend + 2] = score;
                 }
-                if (lane == 0) L.cig_len[pid] = m + n - pos;
+            } else {
+                const int len = wfa_walk<Off>(wsp, win, L.cigar + L.cig_off[pid], m, n, score, lane);
+                if (lane == 0) L.cig_len[pid] = len;
             }
+        }
+        if (defer && trace && score > 0) {
+            // every wave keeps the same region bookkeeping (uniform values)
+            ++npend;
+            used += ((int64_t(score) + 1) * (score + 1) + 1) & ~int64_t(1);
         }
         // the next pair rewrites the LDS wavefronts, sequences and the stack
         __syncthreads();
     }
+    if (defer) walk_pending();
 }
 
 __global__ void zero_i32_kernel(int32_t* p, int n)
@@ -635,6 +696,7 @@ hipError_t dispatch(const WfaQueueLaunch& L, size_t lds, hipStream_t st, int* gr
 size_t wfa_lds_bytes(const WfaQueueLaunch& L)
 {
     size_t b = 16 + 2 * size_t(wfa_wf_bytes(L.smax_cls, L.wide ? 4 : 2)) + size_t(wfa_win_bytes(L.wide ? 4 : 2));
+    if (L.cigar && L.waves > 1) b += size_t(kWfaPend) * 12;          // the deferred-walk queue
     if (L.staged) b += size_t(wfa_seq_bytes(L.max_plen)) + size_t(wfa_seq_bytes(L.max_tlen));
     return b;
 }
