@@ -307,11 +307,20 @@ __device__ int wfa_walk(const Off* ws, Off* win, uint8_t* const cg, int m, int n
             if (ins >= 0 && ins > best) { best = ins; op = 2; }
             if (del >= 0 && del > best) { best = del; op = 3; }
             if (op == 0) break;
-            emit('M', offset - best);
-            offset = best;
-            if (op == 1)      { emit('X', 1); --offset; }
-            else if (op == 2) { emit('I', 1); --k; --offset; }
-            else              { emit('D', 1); ++k; }
+            // the run of matches down to the predecessor, then the edit, in
+            // one masked store loop (two when the slot would overflow, which
+            // consistent stacks never do)
+            const int run = offset - best > 0 ? offset - best : 0;
+            const uint8_t oc = op == 1 ? uint8_t('X') : op == 2 ? uint8_t('I') : uint8_t('D');
+            if (run + 1 <= pos) {
+                for (int x = lane; x <= run; x += 64) cg[pos - 1 - x] = x < run ? uint8_t('M') : oc;
+                pos -= run + 1;
+            } else {
+                emit('M', run);
+                emit(oc, 1);
+            }
+            offset = best - (op != 3 ? 1 : 0);
+            k += (op == 3 ? 1 : 0) - (op == 2 ? 1 : 0);
             v = offset - k;
             h = offset;
             --sc;
