@@ -256,9 +256,10 @@ __device__ __forceinline__ void fetch_rows(RowFetch<K>& f, const MsaLaunch& L, i
 template <int K>
 __global__ __launch_bounds__(64) void msa_pp_kernel(MsaLaunch L)
 {
+    constexpr bool kRegB = K <= 4;
     __shared__ float sub_s[K * K];
-    __shared__ float bfrq[K * 64];
-    __shared__ uint8_t bsym[K * 64];
+    __shared__ float bfrq[kRegB ? 1 : K * 64];
+    __shared__ uint8_t bsym[kRegB ? 1 : K * 64];
     __shared__ float r_open[kRing], r_ext[kRing];
     __shared__ int r_cnt[kRing];
     __shared__ uint8_t r_sym[kRing * K];
@@ -297,10 +298,6 @@ __global__ __launch_bounds__(64) void msa_pp_kernel(MsaLaunch L)
         const bool jv = j <= N;
         const int64_t cB = cb + j - 1;
         const int nb = jv ? int(L.nz_cnt[cB]) : 0;
-        for (int k = 0; k < nb; ++k) {
-            bsym[k * 64 + lane] = L.nz_sym[int64_t(k) * C + cB];
-            bfrq[k * 64 + lane] = L.nz_frq[int64_t(k) * C + cB];
-        }
         const float openA = jv ? L.gopen[cB] : 0.f, extA = jv ? L.gext[cB] : 0.f;
         float prevM = fr ? 0.f : kMsaNeg, prevIx = kMsaNeg;          // M[0][j], Ix[0][j]
         const float row0Iy = fr ? 0.f : boundary_gap(openA, j, ge);
@@ -309,11 +306,39 @@ __global__ __launch_bounds__(64) void msa_pp_kernel(MsaLaunch L)
         uint32_t acc = 0;
         const bool full_strip = (s << 6) + 64 <= N;
         const bool coln = (s << 6) + 64 == N;               // lane 63 holds column N
+        // the lane's column of B: in registers for DNA (at most 4 symbols),
+        // in the workgroup's LDS for larger alphabets
+        float fbv[kRegB ? K : 1];
+        int ybv[kRegB ? K : 1];
+        if constexpr (kRegB) {
+#pragma unroll
+            for (int kb = 0; kb < K; ++kb) {
+                fbv[kb] = kb < nb ? L.nz_frq[int64_t(kb) * C + cB] : 0.f;
+                ybv[kb] = kb < nb ? int(L.nz_sym[int64_t(kb) * C + cB]) : 0;
+            }
+        } else {
+            for (int k = 0; k < nb; ++k) {
+                bsym[k * 64 + lane] = L.nz_sym[int64_t(k) * C + cB];
+                bfrq[k * 64 + lane] = L.nz_frq[int64_t(k) * C + cB];
+            }
+        }
+        // acc += fa * fb * S[x][y] over the lane's B symbols, in the
+        // reference's term order
+        auto add_col = [&](float& acc, int x, float fa) {
+            if constexpr (kRegB) {
+#pragma unroll
+                for (int kb = 0; kb < K; ++kb)
+                    if (kb < nb) acc += fa * fbv[kb] * sub_s[x * K + ybv[kb]];
+            } else {
+                for (int kb = 0; kb < nb; ++kb)
+                    acc += fa * bfrq[kb * 64 + lane] * sub_s[x * K + bsym[kb * 64 + lane]];
+            }
+        };
         // G[x] = the substitution sum of a one-symbol A column x (fraction
-        // 1) against the lane's column of B, in the reference's term order
+        // 1) against the lane's column of B
         for (int x = 0; x < K; ++x) {
             float g = 0.f;
-            for (int kb = 0; kb < nb; ++kb) g += 1.0f * bfrq[kb * 64 + lane] * sub_s[x * K + bsym[kb * 64 + lane]];
+            add_col(g, x, 1.0f);
             gsum[x * 64 + lane] = g;
         }
         RowFetch<K> pf;
@@ -325,8 +350,8 @@ __global__ __launch_bounds__(64) void msa_pp_kernel(MsaLaunch L)
                 r_cnt[q] = pf.cnt; r_open[q] = pf.open; r_ext[q] = pf.ext;
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    r_sym[q * K + k] = uint8_t(pf.sym[k >> 2] >> (8 * (k & 3)));
-                    r_frq[q * K + k] = pf.frq[k];
+                    r_sym[k * kRing + q] = uint8_t(pf.sym[k >> 2] >> (8 * (k & 3)));
+                    r_frq[k * kRing + q] = pf.frq[k];
                 }
                 const int qb = (t0 + lane) & (kRing - 1);
                 r_bm[qb] = pf.bm; r_bx[qb] = pf.bx; r_by[qb] = pf.by;
@@ -369,19 +394,15 @@ __global__ __launch_bounds__(64) void msa_pp_kernel(MsaLaunch L)
                     const float openB = r_open[qi], extB = r_ext[qi];
                     float sc = 0.f;
                     const int na = r_cnt[qi];
-                    const int x0 = r_sym[qi * K];
-                    const float f0 = r_frq[qi * K];
+                    const int x0 = r_sym[qi];
+                    const float f0 = r_frq[qi];
                     // A column of one symbol at fraction 1: fa * fb == fb
                     // exactly, so the sum is the lane's precomputed G[x]
                     if (__all(na == 1 && f0 == 1.0f)) {
                         sc = gsum[x0 * 64 + lane];
                     } else {
-                        for (int ka = 0; ka < na; ++ka) {
-                            const int x = r_sym[qi * K + ka];
-                            const float fa = r_frq[qi * K + ka];
-                            for (int kb = 0; kb < nb; ++kb)
-                                sc += fa * bfrq[kb * 64 + lane] * sub_s[x * K + bsym[kb * 64 + lane]];
-                        }
+                        for (int ka = 0; ka < na; ++ka)
+                            add_col(sc, r_sym[ka * kRing + qi], r_frq[ka * kRing + qi]);
                     }
                     const float oIx = prevM - openB, eIx = prevIx - extB;
                     const float nIx = oIx >= eIx ? oIx : eIx;
