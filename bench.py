@@ -578,6 +578,18 @@ def main():
         psc, pcg = plain.results()
         rsc, rcg = routed.results()
         diff = int(sum(1 for i in range(wr.n) if psc[i] != rsc[i] or pcg[i] != rcg[i]))
+        # the ceiling of any router that only sets bounds: the plain path again
+        # with every resolved pair's bound at its exact distance (known here
+        # from the plain results, for free), which is the tightest bound that
+        # keeps every result
+        xs = wr.smax.copy()
+        xs[psc >= 0] = psc[psc >= 0]
+        exact = WfaDeviceBatch(W.PairSet(wr.a, wr.a_off, wr.a_len, wr.b, wr.b_off, wr.b_len, xs, dict(wr.meta)),
+                               trace=True, want=rwant)
+        xwall, _, _ = time_steps(exact.run, args.steps, args.warmup, lib, kernel_reps=0)
+        xsc, xcg = exact.results()
+        xdiff = int(sum(1 for i in range(wr.n) if psc[i] != xsc[i] or pcg[i] != xcg[i]))
+        del exact
         rsample = rng.choice(wr.n, size=min(200, wr.n), replace=False)
         obad = 0
         for i in rsample:
@@ -603,6 +615,9 @@ def main():
             "roofline": wfa_valu_roofline(wr, rsc, rk),
             "plain_roofline": wfa_valu_roofline(wr, psc, pk),
             "routed_vs_plain_differences": diff, "oracle_mismatches": obad, "oracle_checked": int(len(rsample)),
+            "exact_bound_ms": round(xwall * 1000 / args.steps, 3),
+            "exact_bound_over_plain": round(pwall / xwall, 3),
+            "exact_bound_differences": xdiff,
             "shortcut": st.shortcut, "routed": st.routed, "plain": st.plain, "rerun": st.rerun,
             "miss_rate": round(st.miss_rate, 4), "route_q": args.route_q,
         }

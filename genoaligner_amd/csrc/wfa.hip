@@ -104,7 +104,10 @@ __device__ __forceinline__ int lds_match_run(const uint32_t* P, int v, const uin
     const uint32_t* p = P + (v >> 4);
     const uint32_t* t = T + (h >> 4);
     const uint32_t sp = (uint32_t(v) & 15u) * 2u, st = (uint32_t(h) & 15u) * 2u;
-    uint32_t x = __builtin_amdgcn_alignbit(p[1], p[0], sp) ^ __builtin_amdgcn_alignbit(t[1], t[0], st);
+    // the four words are read before either is used: one LDS round trip
+    const uint32_t p0 = p[0], p1 = p[1], t0 = t[0], t1 = t[1];
+    __builtin_amdgcn_sched_barrier(0);
+    uint32_t x = __builtin_amdgcn_alignbit(p1, p0, sp) ^ __builtin_amdgcn_alignbit(t1, t0, st);
     // equal bases before the first differing one: ctz / 2, or 16 for x == 0
     uint32_t run = uint32_t(__builtin_ctzg(x, 32)) >> 1;
     if (x == 0 && 16 < lim) {
