@@ -693,13 +693,13 @@ def main():
             "kernel_ms": round(pkm, 3) if pkm else None,
             "kernel_gcups": round(pcells / (pkm * 1e-3) / 1e9, 1) if pkm else None,
             # packed kernel: 12 ops per cell against the packed int16 VALU
-            # peak; kernel_ms covers the forward and the walk kernel
+            # peak; kernel_ms covers the forward pass and the fused walk
             "roofline": {"bound": "valu", "unit": "TOP/s", "peak": PEAK_INT16_TOPS,
                          "achieved": round(SW_OPS_PER_CELL * pcells / (pkm * 1e-3) / 1e12, 3) if pkm else None,
                          "frac": round(SW_OPS_PER_CELL * pcells / (pkm * 1e-3) / 1e12 / PEAK_INT16_TOPS, 4)
                          if pkm else None,
-                         "kernel": "sw16m_kernel<true, false> (packed int16, two pairs per workgroup, "
-                                   "4-wave strip pipeline) + sw16m_walk_kernel"},
+                         "kernel": "sw16m_kernel<true, false, 8> (packed int16, two pairs per workgroup, "
+                                   "8-wave strip pipeline, traceback walk fused into wave 0)"},
             "python_api_ms": round(pdt_py * 1e3, 2),
             "mismatches": pbad, "checked": 20,
         }

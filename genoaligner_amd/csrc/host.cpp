@@ -1138,6 +1138,24 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
             if (sa != sb) return sa > sb;
             return reqs[a].text_len > reqs[b].text_len;
         });
+        // then the workgroups (consecutive pairs of that order; an odd last
+        // pair stays last, alone) by their cost, strips x (rows + 63), largest
+        // first: the longest start first and the short ones fill in behind
+        const size_t nd = packed.size() / 2;
+        std::vector<std::pair<int64_t, size_t>> duo(nd);
+        for (size_t d = 0; d < nd; ++d) {
+            const int64_t rows = std::max(reqs[packed[2 * d]].text_len, reqs[packed[2 * d + 1]].text_len);
+            duo[d] = {int64_t((reqs[packed[2 * d]].pattern_len + 63) >> 6) * (rows + 63), d};
+        }
+        std::stable_sort(duo.begin(), duo.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+        std::vector<int64_t> by_cost;
+        by_cost.reserve(packed.size());
+        for (const auto& d : duo) {
+            by_cost.push_back(packed[2 * d.second]);
+            by_cost.push_back(packed[2 * d.second + 1]);
+        }
+        if (packed.size() & 1) by_cost.push_back(packed.back());
+        packed.swap(by_cost);
     } else {
         std::stable_sort(packed.begin(), packed.end(), [&](int64_t a, int64_t b) {
             if (reqs[a].text_len != reqs[b].text_len) return reqs[a].text_len > reqs[b].text_len;

@@ -22,13 +22,19 @@
 // packed u with one v_perm.
 //
 // Parallelism.  A protein batch has few pairs of many strips, so the strips
-// of a workgroup's two pairs run as a pipeline over its kWaves waves: wave w
-// takes strips w, w + kWaves, ...; strip s starts kLag chunks of kChunk rows after
+// of a workgroup's two pairs run as a pipeline over its WAVES waves (4, or 8
+// when the launch's pairs have 8 strips or more, which halves the longest
+// workgroup's run): wave w takes strips w, w + WAVES, ...; strip s starts kLag
+// chunks of kChunk rows after
 // strip s - 1 (lane 63 of strip s - 1 hands row r over at step r + 63, lane 0
 // of strip s reads it one chunk ahead of step r) and no earlier than its
-// wave is free; one barrier per chunk keeps the waves in step.  Strip s reads
-// the boundary column buffer (s + 1) & 1 and writes buffer s & 1; strip s + 2
-// overwrites a row of that buffer only long after strip s + 1 has read it.
+// wave is free; one barrier per chunk keeps the waves in step.  One boundary
+// column buffer serves every strip: row r is written by strip s at its step
+// r + 63, read by strip s + 1 one group ahead of its step r (kLag chunks
+// after strip s started, so at least two barriers after that write) and
+// overwritten by strip s + 1 only at its step r + 63, after that read.
+// (Reads clamped to the last rows, in the drain past row M, may see the
+// strip's own rows; they feed only drained cells.)
 // Each strip computes exactly the cells, flags and first maxima of the
 // one-wave kernel; the waves' per-lane records merge by (value, row, column),
 // the order in which the one-wave kernel folds strips.
@@ -44,16 +50,20 @@
 namespace ga {
 namespace {
 
-constexpr int kWaves = 4;                 // waves per workgroup
 constexpr int kChunk = 16;               // text rows per pipeline step (one barrier each)
 // Pipeline steps between consecutive strips: the rows a strip reads in a step
 // (and the group it prefetches) were handed over in an earlier step.
 constexpr int kLag = 2 + 66 / kChunk;
 constexpr int kZeroCode = 32;             // table row of u = 0
-constexpr int kTabBytes = 33 * 64;        // one half's table: 33 rows x 64 lanes
-constexpr int kProfBytes = kWaves * 2 * kTabBytes;
-constexpr int kRecBytes = 2 * kWaves * 64 * 3 * 4;   // merge records (in the tables' LDS)
-constexpr uint32_t kSelZero = uint32_t(kZeroCode * 64) | (uint32_t(kTabBytes + kZeroCode * 64) << 16);
+// One half's table: codes 0..35 in 9 rows of 64 lane words, 4 codes per word,
+// so the lanes of a lookup read 64 different words in 64 different banks
+// whatever codes they hold.
+constexpr int kTabBytes = 9 * 64 * 4;
+__host__ __device__ constexpr int prof_bytes(int waves) { return waves * 2 * kTabBytes; }
+__host__ __device__ constexpr uint32_t tab_off(uint32_t c) { return (c >> 2) * 256u + (c & 3u); }
+// merge records (in the tables' LDS)
+__host__ __device__ constexpr int rec_bytes(int waves) { return 2 * waves * 64 * 3 * 4; }
+constexpr uint32_t kSelZero = tab_off(kZeroCode) | ((uint32_t(kTabBytes) + tab_off(kZeroCode)) << 16);
 
 // Boundary rows in use: the text rows rounded up to whole chunks of 8.
 __host__ __device__ inline int bnd_rows(int m)
@@ -62,9 +72,9 @@ __host__ __device__ inline int bnd_rows(int m)
     return r > 8 ? r : 8;
 }
 
-__host__ __device__ inline size_t lds_bytes(int max_m)
+__host__ __device__ inline size_t lds_bytes(int max_m, int waves)
 {
-    return size_t(kProfBytes) + size_t(2) * size_t(bnd_rows(max_m)) * 8;
+    return size_t(prof_bytes(waves)) + size_t(bnd_rows(max_m)) * 8;
 }
 
 // The workgroup's two pairs and the shape they are computed over.
@@ -113,6 +123,7 @@ struct WalkLds {
     uint8_t wt[2][kWinR], wtc[2][kWinR], wp[2][kWinC], wpc[2][kWinC];
     int16_t sub_s[32 * 32];
 };
+static_assert(rec_bytes(4) + sizeof(WalkLds) <= size_t(prof_bytes(4)), "the walk's LDS lies inside the tables' region");
 
 // Called by one whole wave after the forward pass of its workgroup's pairs.
 // score / ei / ej: the half's score (> 0 to walk) and 0-based trace end cell.
@@ -215,8 +226,8 @@ __device__ void walk_duo(const Sw16Launch& L, const Duo& D, int g, int lane, int
     }
 }
 
-template <bool TRACE, bool GK>
-__global__ __launch_bounds__(64 * kWaves) void sw16m_kernel(Sw16Launch L)
+template <bool TRACE, bool GK, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void sw16m_kernel(Sw16Launch L)
 {
     extern __shared__ __align__(16) uint32_t lds16[];
     const int g = blockIdx.x;
@@ -240,42 +251,42 @@ __global__ __launch_bounds__(64 * kWaves) void sw16m_kernel(Sw16Launch L)
     }
 
     uint8_t* prof = reinterpret_cast<uint8_t*>(lds16) + w * 2 * kTabBytes;
-    uint2* bnd = reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(lds16) + kProfBytes);
+    uint2* bnd = reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(lds16) + prof_bytes(WAVES));
     const int Mr = bnd_rows(M);
     const int go = L.go, ge = L.ge;
     const uint32_t go2 = uint32_t(go) * 0x10001u, ge2 = uint32_t(ge) * 0x10001u;
     const int bias = L.bias;
     const uint32_t b2 = uint32_t(bias) * 0x10001u, gk2 = uint32_t(bias - go) * 0x10001u;
 
-    // Text selectors of both pairs (guard rows around them) and both boundary
-    // buffers at the zero boundary (H = 0, E = -go).
+    // Text selectors of both pairs (guard rows around them) and the boundary
+    // buffer at the zero boundary (H = 0, E = -go).
     const uint8_t* TCA = L.tcode + L.toff[D.ia];
     const uint8_t* TCB = D.ib >= 0 ? L.tcode + L.toff[D.ib] : TCA;
     uint32_t* sel = L.selbuf + int64_t(g) * sw16_sel_stride(L.max_m) + 64;
-    for (int i = tid - 64; i < M + 80; i += 64 * kWaves) {
+    for (int i = tid - 64; i < M + 80; i += 64 * WAVES) {
         uint32_t sv = kSelZero;
         if (i >= 0 && i < M) {
             const uint32_t ca = i < D.mAp ? uint32_t(TCA[i]) : uint32_t(kZeroCode);
             const uint32_t cb = i < D.mBp ? uint32_t(TCB[i]) : uint32_t(kZeroCode);
-            sv = (ca * 64u) | ((uint32_t(kTabBytes) + cb * 64u) << 16);
+            sv = tab_off(ca) | ((uint32_t(kTabBytes) + tab_off(cb)) << 16);
         }
         sel[i] = sv;
     }
-    for (int i = tid; i < 2 * Mr; i += 64 * kWaves) bnd[i] = make_uint2(0u, go2);
+    for (int i = tid; i < Mr; i += 64 * WAVES) bnd[i] = make_uint2(0u, go2);
     __syncthreads();
 
     const int nstrips = (N + 63) >> 6;
     const int nchunks = (M + 63 + kChunk - 1) / kChunk;
     const int nq = nchunks * (kChunk / 4);
-    const int period = nchunks > kWaves * kLag ? nchunks : kWaves * kLag;
+    const int period = nchunks > WAVES * kLag ? nchunks : WAVES * kLag;
     const int last = nstrips - 1;
-    const int total = (last / kWaves) * period + (last % kWaves) * kLag + nchunks;
+    const int total = (last / WAVES) * period + (last % WAVES) * kLag + nchunks;
     const int nmin = D.nAp < D.nBp ? D.nAp : D.nBp;
     uint32_t* dirs = TRACE ? L.dirs + int64_t(g) * L.dir_stride : nullptr;
     const uint8_t* PCA = L.pcode + L.poff[D.ia];
     const uint8_t* PCB = D.ib >= 0 ? L.pcode + L.poff[D.ib] : PCA;
     const uint32_t* sp = sel - lane;           // lane l reads row t - l
-    const uint8_t* pb = prof + lane;           // the lane's byte of every table row
+    const uint8_t* pb = prof + lane * 4;       // the lane's word of every table row
 
     int gH[2] = {0, 0}, gI[2] = {-1, -1}, gJ[2] = {-1, -1};
     uint32_t thrm1 = 0;                        // per half: this wave's best so far - 1
@@ -300,11 +311,13 @@ __global__ __launch_bounds__(64 * kWaves) void sw16m_kernel(Sw16Launch L)
     // column of the strip exists in both pairs (no column masks).
     auto run_strip = [&](auto fast_tag, int s) {
         constexpr bool FAST = decltype(fast_tag)::value;
-        const uint2* br = bnd + ((s + 1) & 1) * Mr;
-        uint2* bw = bnd + (s & 1) * Mr;
+        const uint2* br = bnd;
+        uint2* bw = bnd;
         const int j = (s << 6) + lane;
         const uint32_t pa = j < D.nAp ? PCA[j] : 0u, pbc = j < D.nBp ? PCB[j] : 0u;
-        // the lane's two score columns: 64 table reads, 16 in flight at a time
+        // the lane's two score columns: 64 table reads, 16 in flight at a
+        // time, written as words of 4 codes; codes 32..35 (the zero row) are 0
+        uint32_t* pw = reinterpret_cast<uint32_t*>(prof) + lane;
 #pragma unroll
         for (int c0 = 0; c0 < 32; c0 += 8) {
             int32_t sa[8], sb[8];
@@ -315,13 +328,19 @@ __global__ __launch_bounds__(64 * kWaves) void sw16m_kernel(Sw16Launch L)
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                prof[(c0 + u) * 64 + lane] = uint8_t(j < D.nAp ? sa[u] + bias : 0);
-                prof[kTabBytes + (c0 + u) * 64 + lane] = uint8_t(j < D.nBp ? sb[u] + bias : 0);
+            for (int q = 0; q < 2; ++q) {
+                uint32_t wa = 0, wb = 0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    wa |= (uint32_t(sa[4 * q + u] + bias) & 0xffu) << (8 * u);
+                    wb |= (uint32_t(sb[4 * q + u] + bias) & 0xffu) << (8 * u);
+                }
+                pw[(c0 / 4 + q) * 64] = j < D.nAp ? wa : 0u;
+                pw[kTabBytes / 4 + (c0 / 4 + q) * 64] = j < D.nBp ? wb : 0u;
             }
         }
-        prof[kZeroCode * 64 + lane] = 0;
-        prof[kTabBytes + kZeroCode * 64 + lane] = 0;
+        pw[8 * 64] = 0u;
+        pw[kTabBytes / 4 + 8 * 64] = 0u;
         Strip S{};
         S.H = 0; S.Eh = go2; S.Fh = go2; S.diag = 0;
         S.bestH = thrm1; S.bestT = 0xffffffffu;   // record at the bound (see strip_group)
@@ -380,12 +399,12 @@ __global__ __launch_bounds__(64 * kWaves) void sw16m_kernel(Sw16Launch L)
         }
     };
 
-    // Wave w runs strips w, w + kWaves, ...; strip s occupies pipeline steps
-    // (s / kWaves) * period + w * kLag onwards.  Every wave passes the same
+    // Wave w runs strips w, w + WAVES, ...; strip s occupies pipeline steps
+    // (s / WAVES) * period + w * kLag onwards.  Every wave passes the same
     // number of barriers (total), idle steps included.
     int G = 0;
-    for (int s = w; s < nstrips; s += kWaves) {
-        const int start = (s / kWaves) * period + w * kLag;
+    for (int s = w; s < nstrips; s += WAVES) {
+        const int start = (s / WAVES) * period + w * kLag;
         for (; G < start; ++G) __syncthreads();
         if ((s << 6) + 64 <= nmin) run_strip(std::true_type{}, s);
         else                       run_strip(std::false_type{}, s);
@@ -397,15 +416,15 @@ __global__ __launch_bounds__(64 * kWaves) void sw16m_kernel(Sw16Launch L)
     int* rec = reinterpret_cast<int*>(lds16);
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        int* q = rec + ((h * kWaves + w) * 64 + lane) * 3;
+        int* q = rec + ((h * WAVES + w) * 64 + lane) * 3;
         q[0] = gH[h]; q[1] = gI[h]; q[2] = gJ[h];
     }
     __syncthreads();
     if (w != 0) return;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-        for (int v = 1; v < kWaves; ++v) {
-            const int* q = rec + ((h * kWaves + v) * 64 + lane) * 3;
+        for (int v = 1; v < WAVES; ++v) {
+            const int* q = rec + ((h * WAVES + v) * 64 + lane) * 3;
             const int hh = q[0], ii = q[1], jj = q[2];
             if (hh > gH[h] || (hh == gH[h] && hh > 0 && (ii < gI[h] || (ii == gI[h] && jj < gJ[h])))) {
                 gH[h] = hh; gI[h] = ii; gJ[h] = jj;
@@ -454,7 +473,7 @@ __global__ __launch_bounds__(64 * kWaves) void sw16m_kernel(Sw16Launch L)
     if (!TRACE) return;
     // the walks (the re-run kernel walks the pairs it re-runs); the tables'
     // LDS behind the merge records holds the walk's windows
-    WalkLds& WS = *reinterpret_cast<WalkLds*>(reinterpret_cast<uint8_t*>(lds16) + kRecBytes);
+    WalkLds& WS = *reinterpret_cast<WalkLds*>(reinterpret_cast<uint8_t*>(lds16) + rec_bytes(WAVES));
     const int hw = lane >> 5;
     walk_duo(L, D, g, lane, hw == 0 ? wS[0] : wS[1], hw == 0 ? wI[0] : wI[1], hw == 0 ? wJ[0] : wJ[1],
              hw == 0 ? wOk[0] : wOk[1], WS);
@@ -486,16 +505,19 @@ int sw16m_bias(const int32_t* sub, int gap_open)
 hipError_t sw16m_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHooks* hooks)
 {
     if (L.n_waves <= 0) return hipSuccess;
-    const size_t lds = lds_bytes(L.max_m);
-    const dim3 grid(unsigned(L.n_waves)), block(64 * kWaves);
     const bool gk = L.bias > L.go;                 // -min score > gap_open: H needs bias - go
+    const bool wide = (L.max_n + 63) / 64 >= 8;    // 8 strips or more: 8 waves
+    const int waves = wide ? 8 : 4;
+    const size_t lds = lds_bytes(L.max_m, waves);
+    const dim3 grid(unsigned(L.n_waves)), block(64 * waves);
     if (hooks) hooks->before(st);
-    if (gk) {
-        if (trace) hipLaunchKernelGGL((sw16m_kernel<true, true>), grid, block, lds, st, L);
-        else       hipLaunchKernelGGL((sw16m_kernel<false, true>), grid, block, lds, st, L);
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, block, lds, st, L); };
+    if (wide) {
+        if (gk) trace ? go(sw16m_kernel<true, true, 8>) : go(sw16m_kernel<false, true, 8>);
+        else    trace ? go(sw16m_kernel<true, false, 8>) : go(sw16m_kernel<false, false, 8>);
     } else {
-        if (trace) hipLaunchKernelGGL((sw16m_kernel<true, false>), grid, block, lds, st, L);
-        else       hipLaunchKernelGGL((sw16m_kernel<false, false>), grid, block, lds, st, L);
+        if (gk) trace ? go(sw16m_kernel<true, true, 4>) : go(sw16m_kernel<false, true, 4>);
+        else    trace ? go(sw16m_kernel<true, false, 4>) : go(sw16m_kernel<false, false, 4>);
     }
     if (hooks) hooks->after(st);
     return hipGetLastError();
