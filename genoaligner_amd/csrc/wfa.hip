@@ -152,12 +152,17 @@ __device__ __forceinline__ bool stage2(uint32_t* dst, const uint8_t* src, int le
 }
 
 // LDS layout of a workgroup: [claim slots: 4 ints][pattern, 2-bit][text, 2-bit]
-// [wavefront A][wavefront B][traceback window], each region 16-byte aligned
-// (the sequences first: the pattern's base then fits the offset field of the
-// extension's LDS reads, and no add per diagonal is spent on it).
+// [wavefront A][wavefront B][traceback window][deferred-walk queue], each
+// region 16-byte aligned (the sequences first: the pattern's base then fits
+// the offset field of the extension's LDS reads, and no add per diagonal is
+// spent on it).  Budget pruning keeps at most smax + 1 diagonals live (|k| <=
+// s and |k - ak| <= smax - s), so a wavefront buffer holds smax + 5 slots:
+// the live range at slot k - lo + 2 and two guard slots on each side.  With
+// deferred walks the two buffers' region is at least one walk window per
+// wave (the walks use it between pairs).
 __host__ __device__ inline int wfa_wf_bytes(int smax_cls, int off_bytes)
 {
-    return ((2 * smax_cls + 3) * off_bytes + 15) & ~15;
+    return ((smax_cls + 5) * off_bytes + 15) & ~15;
 }
 __host__ __device__ inline int wfa_seq_bytes(int len) { return ((((len > 0 ? len : 0) + 15) / 16 + 2) * 4 + 15) & ~15; }
 
@@ -168,29 +173,43 @@ constexpr int kTbRows = 16;
 constexpr int kTbCols = 2 * kTbRows + 1;
 constexpr int kTbBatch = 5;          // window loads a lane has in flight at once
 __host__ __device__ inline int wfa_win_bytes(int off_bytes) { return (kTbRows * kTbCols * off_bytes + 15) & ~15; }
+__host__ __device__ inline int wfa_region_bytes(int smax_cls, int off_bytes, bool defer, int waves)
+{
+    const int two = 2 * wfa_wf_bytes(smax_cls, off_bytes);
+    const int wins = defer ? waves * wfa_win_bytes(off_bytes) : 0;
+    return two > wins ? two : wins;
+}
 
 // Forward pass over one pair: wavefronts s = 1 .. smax until W[s][n-m] >= n.
 // Returns the score or -1.  PACKED: extend over the 2-bit LDS copies, else over
 // the byte sequences in global memory.
 template <bool TRACE, typename Off, bool PACKED, int NT>
 __device__ __forceinline__ int wfa_forward(const uint32_t* sP, const uint32_t* sT, const uint8_t* P,
-                                           const uint8_t* T, int m, int n, int smax, int S,
+                                           const uint8_t* T, int m, int n, int smax,
                                            Off* A, Off* B, Off* ws, bool trace, int tid)
 {
-    if (tid == 0) {
-        const int o0 = PACKED ? lds_match_run(sP, 0, sT, 0, m < n ? m : n) : match_run(P, T, m < n ? m : n);
-        A[S + 1] = Off(o0);
-        if (trace) ws[0] = Off(o0);
+    // Wavefront s, diagonals lo .. hi, sits at slots k - lo + 2 of its buffer,
+    // and slots 0, 1 (k = lo - 2, lo - 1) and hi - lo + 3, hi - lo + 4 (k =
+    // hi + 1, hi + 2) hold NULL: everything the next score reads (the range
+    // moves by at most one per side per score) was written by this one.
+    // Score 0 is diagonal 0 alone.
+    if (tid < 5) {
+        Off v = Off(OffNull<Off>::v);
+        if (tid == 2) {
+            const int o0 = PACKED ? lds_match_run(sP, 0, sT, 0, m < n ? m : n) : match_run(P, T, m < n ? m : n);
+            v = Off(o0);
+            if (trace) ws[0] = v;
+        }
+        A[tid] = v;
     }
     __syncthreads();
     const int ak = n - m;
-    if (ak == 0 && int(A[S + 1]) >= n) return 0;
+    if (ak == 0 && int(A[2]) >= n) return 0;
     Off* prev = A;
     Off* cur = B;
+    int plo = 0;                                               // lo of the previous score
     for (int s = 1; s <= smax; ++s) {
         Off* wrow = TRACE ? ws + int64_t(s) * s : nullptr;   // row s of the stack (wave-uniform)
-        const Off* pk = prev + S + 1;                          // pk[k] = W[s-1][k]
-        Off* ck = cur + S + 1;
         // Budget pruning: every step changes the diagonal by at most one, so
         // from score s only diagonals with |k - ak| <= smax - s can still
         // reach the end diagonal ak by smax; the others are not computed.  A
@@ -198,14 +217,12 @@ __device__ __forceinline__ int wfa_forward(const uint32_t* sP, const uint32_t* s
         // neighbours are within smax - s + 1 of ak), so kept offsets, the end
         // test and the backtrace (whose candidates are kept too) are exactly
         // those of the full wavefront.  A pair that runs to smax computes
-        // about half the cells.  The next score reads slots lo - 1 .. hi + 1
-        // of this one's buffer only where they were written now or never
-        // written since the pair began (the range first grows with s, then
-        // shrinks by one per side once the budget binds), so no stale value
-        // of score s - 2 is read.
+        // about half the cells.
         const int bud = smax - s;
         const int lo = -s > ak - bud ? -s : ak - bud;
         const int hi = s < ak + bud ? s : ak + bud;
+        const Off* pk = prev + 2 - plo;                        // pk[k] = W[s-1][k]
+        Off* ck = cur + 2 - lo;                                // ck[k] = W[s][k]
         for (int k = lo + tid; k <= hi; k += NT) {
             const int ins = pk[k - 1];
             const int mis = pk[k];
@@ -231,9 +248,11 @@ __device__ __forceinline__ int wfa_forward(const uint32_t* sP, const uint32_t* s
             if (TRACE && trace)
                 *reinterpret_cast<Off*>(reinterpret_cast<char*>(wrow) + uint32_t(k + s) * uint32_t(sizeof(Off))) = Off(mx);
         }
+        if (tid >= NT - 4) ck[tid < NT - 2 ? lo - (NT - tid) + 2 : hi + (tid - (NT - 3))] = Off(OffNull<Off>::v);
         __syncthreads();
-        if (ak >= -s && ak <= s && int(cur[ak + S + 1]) >= n) return s;
+        if (ak >= lo && ak <= hi && int(ck[ak]) >= n) return s;
         Off* t = prev; prev = cur; cur = t;
+        plo = lo;
     }
     return -1;
 }
@@ -353,7 +372,6 @@ void wfa_kernel(WfaQueueLaunch L)
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int S = L.smax_cls;
-    const int W = 2 * S + 3;
     const int wfb = wfa_wf_bytes(S, int(sizeof(Off)));
     int* claim = reinterpret_cast<int*>(smem);
     const int pbytes = STAGED ? wfa_seq_bytes(L.max_plen) : 0;
@@ -363,7 +381,6 @@ void wfa_kernel(WfaQueueLaunch L)
     uint8_t* wfs = smem + 16 + pbytes + tbytes;
     Off* A = reinterpret_cast<Off*>(wfs);
     Off* B = reinterpret_cast<Off*>(wfs + wfb);
-    Off* win = reinterpret_cast<Off*>(wfs + 2 * wfb);
     // the workgroup's stack region: ws_stride entries (at least one worst-case
     // stack of (S + 1)^2 entries; more when walks are deferred)
     const int64_t stride = L.ws_stride > 0 ? L.ws_stride : (int64_t(S) + 1) * (int64_t(S) + 1);
@@ -375,8 +392,10 @@ void wfa_kernel(WfaQueueLaunch L)
     // window in the wavefront buffers (free between pairs).  So no wave waits
     // while one walks.
     const int winb = wfa_win_bytes(int(sizeof(Off)));
-    const bool defer = TRACE && WAVES > 1 && 2 * wfb >= WAVES * winb;
-    int* pend = reinterpret_cast<int*>(wfs + 2 * wfb + winb);
+    constexpr bool defer = TRACE && WAVES > 1;
+    const int region = wfa_region_bytes(S, int(sizeof(Off)), defer, WAVES);
+    Off* win = reinterpret_cast<Off*>(wfs + region);
+    int* pend = reinterpret_cast<int*>(wfs + region + winb);
     int npend = 0;
     int64_t used = 0;
     auto walk_pending = [&]() {
@@ -475,10 +494,6 @@ void wfa_kernel(WfaQueueLaunch L)
             }
             wsp = ws + used;
         }
-        for (int x = tid; x < W; x += NT) {
-            A[x] = Off(OffNull<Off>::v);
-            B[x] = Off(OffNull<Off>::v);
-        }
         // STAGED: 2-bit copies in LDS unless a byte other than A/C/G/T occurs
         bool packed = false;
         if (STAGED) {
@@ -488,8 +503,8 @@ void wfa_kernel(WfaQueueLaunch L)
             __syncthreads();
         }
         const int ak = n - m;
-        const int score = packed ? wfa_forward<TRACE, Off, true, NT>(sP, sT, P, T, m, n, smax, S, A, B, wsp, trace, tid)
-                                 : wfa_forward<TRACE, Off, false, NT>(sP, sT, P, T, m, n, smax, S, A, B, wsp, trace, tid);
+        const int score = packed ? wfa_forward<TRACE, Off, true, NT>(sP, sT, P, T, m, n, smax, A, B, wsp, trace, tid)
+                                 : wfa_forward<TRACE, Off, false, NT>(sP, sT, P, T, m, n, smax, A, B, wsp, trace, tid);
         if (tid == 0) L.score[pid] = score;
         if (TRACE && wave == 0) {
             if (!trace || score < 0) {
@@ -707,8 +722,10 @@ hipError_t dispatch(const WfaQueueLaunch& L, size_t lds, hipStream_t st, int* gr
 
 size_t wfa_lds_bytes(const WfaQueueLaunch& L)
 {
-    size_t b = 16 + 2 * size_t(wfa_wf_bytes(L.smax_cls, L.wide ? 4 : 2)) + size_t(wfa_win_bytes(L.wide ? 4 : 2));
-    if (L.cigar && L.waves > 1) b += size_t(kWfaPend) * 12;          // the deferred-walk queue
+    const bool defer = L.cigar && L.waves > 1;
+    size_t b = 16 + size_t(wfa_region_bytes(L.smax_cls, L.wide ? 4 : 2, defer, L.waves)) +
+               size_t(wfa_win_bytes(L.wide ? 4 : 2));
+    if (defer) b += size_t(kWfaPend) * 12;                          // the deferred-walk queue
     if (L.staged) b += size_t(wfa_seq_bytes(L.max_plen)) + size_t(wfa_seq_bytes(L.max_tlen));
     return b;
 }
