@@ -108,9 +108,14 @@ __device__ __forceinline__ int lds_match_run(const uint32_t* P, int v, const uin
     const uint32_t p0 = p[0], p1 = p[1], t0 = t[0], t1 = t[1];
     __builtin_amdgcn_sched_barrier(0);
     uint32_t x = __builtin_amdgcn_alignbit(p1, p0, sp) ^ __builtin_amdgcn_alignbit(t1, t0, st);
-    // equal bases before the first differing one: ctz / 2, or 16 for x == 0
-    uint32_t run = uint32_t(__builtin_ctzg(x, 32)) >> 1;
-    if (x == 0 && 16 < lim) {
+    // equal bases before the first differing one: (lowest set bit) / 2, where
+    // v_ffbl_b32 gives 0xffffffff for x == 0, so the capped run exceeds 16
+    // exactly when all 16 bases match and the run may go on
+    uint32_t low;
+    asm("v_ffbl_b32 %0, %1" : "=v"(low) : "v"(x));
+    uint32_t run = low >> 1;
+    run = run < uint32_t(lim) ? run : uint32_t(lim);
+    if (run > 16) {
         uint32_t r = 16;
         for (;;) {
             ++p;
@@ -120,8 +125,9 @@ __device__ __forceinline__ int lds_match_run(const uint32_t* P, int v, const uin
             r += 16;
         }
         run = r + (uint32_t(__builtin_ctzg(x, 32)) >> 1);
+        run = run < uint32_t(lim) ? run : uint32_t(lim);
     }
-    return int(run < uint32_t(lim) ? run : uint32_t(lim));
+    return int(run);
 }
 
 // Stage one sequence as 2-bit words (16 bases per thread per step, two spare
