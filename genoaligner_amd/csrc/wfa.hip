@@ -11,17 +11,21 @@
 //                 then deletion (W[s-1][k+1]) >; runs of M between; leading
 //                 matches, then D, then I (:338-437).
 //
-// Mapping to CDNA4: one 64-lane wavefront per pair, one workgroup of exactly one
-// wave, so __syncthreads() costs a single-wave barrier.  Lane q owns diagonals
-// k = -s + q, -s + q + 64, ... of wavefront s.  The two live wavefronts sit in
-// LDS (ping-pong, 2*smax+3 slots each, as the reference lays them out); in the
-// trace variant every wavefront is also stored to a per-pair HBM stack (row s
-// at offset s*s, 2s+1 ints, coalesced across lanes) that the traceback reads
-// back.  The extension compares 8 bytes per step with a 64-bit XOR and a
-// count-trailing-zeros, over raw bytes, so every byte value keeps its literal
-// meaning (N matches only N, case is significant).  The traceback runs in
-// lockstep on all 64 lanes (its state is uniform) and writes each run of M's
-// cooperatively, 64 bytes per store instruction.
+// Mapping to CDNA4: a persistent workgroup of 1, 2, 4 or 8 waves (by size
+// class) aligns one pair at a time and claims the next from a queue.  Thread t
+// owns diagonals lo + t, lo + t + 64 * waves, ... of wavefront s, lo .. hi
+// being the diagonals budget pruning keeps.  The two live wavefronts sit in
+// LDS (ping-pong, smax + 5 slots each: the live range at a per-score origin
+// and two NULL guard slots per side); in the trace variant every wavefront is
+// also stored to an HBM stack (row s at offset s*s, 2s+1 entries, coalesced
+// across lanes) that the traceback reads back.  The extension compares 16
+// bases per probe over 2-bit copies of both sequences staged in LDS, or 8
+// bytes per step over the raw bytes of a pair holding any byte other than
+// A/C/G/T, so every byte value keeps its literal meaning (N matches only N,
+// case is significant).  A traceback runs in lockstep on one wave (its state
+// is uniform) and writes each run of M's cooperatively, 64 bytes per store
+// instruction; with several waves per workgroup the walks are deferred and
+// shared out one pair per wave.
 #include "ga_device.hpp"
 
 #include <atomic>
