@@ -12,14 +12,15 @@
 // kernel (the biased sums could leave 16 bits).
 //
 // Scores.  Lane l of strip s owns pattern column j = 64 s + l of both pairs.
-// Per wave, the LDS holds two byte tables, u_A[c][l] and u_B[c][l] =
-// score(text residue c, the lane's pattern residue) + bias for c = 0..31,
-// and a row c = 32 of zeros for rows past a pair's end (u = 0 means
-// s = -bias <= -1: such cells stay below the pair's maximum, as in sw16.hip).
-// A text row's selector holds 64 c_A in its low and the B table's offset
-// plus 64 c_B in its high 16 bits; a lane adds its own byte address to each
-// half (two adds), reads both bytes (two LDS reads) and merges them into the
-// packed u with one v_perm.
+// Per wave, the LDS holds two byte tables, u_A and u_B = score(text residue
+// c, the lane's pattern residue) + bias for c = 0..31, and codes 32..35 of
+// zeros for rows past a pair's end (u = 0 means s = -bias <= -1: such cells
+// stay below the pair's maximum, as in sw16.hip).  Code c of lane l is byte
+// c % 4 of word (c / 4) * 64 + l, so the 64 lanes of a lookup read 64 words
+// in 64 banks.  A text row's selector holds tab_off(c_A) in its low and the
+// B table's offset plus tab_off(c_B) in its high 16 bits; a lane adds its own
+// byte address to each half (two adds), reads both bytes (two LDS reads) and
+// merges them into the packed u with one v_perm.
 //
 // Parallelism.  A protein batch has few pairs of many strips, so the strips
 // of a workgroup's two pairs run as a pipeline over its WAVES waves (4, or 8
@@ -40,9 +41,10 @@
 // the order in which the one-wave kernel folds strips.
 //
 // Trace: the 4 flags per cell and pair of sw16.hip's non-HMOD layout, one
-// word per lane and 4 rows; a walk kernel (one thread per pair) then runs the
-// reference's three-state walk over them, carrying the cell value to find
-// STOP, before any int32 re-run reuses the workgroup's direction region.
+// word per lane and 4 rows; after the strips, wave 0 runs the reference's
+// three-state walk of both pairs over them (walk_duo), carrying the cell
+// value to find STOP, before any int32 re-run reuses the workgroup's
+// direction region.
 #include "sw16_ops.hpp"
 
 #include <type_traits>
