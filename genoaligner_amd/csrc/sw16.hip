@@ -194,6 +194,135 @@ __device__ int walk16(const uint32_t* dirs, int nq, uint32_t* win, bool walker, 
     return walker ? m + n - pos : 0;
 }
 
+// The walk over the kLayHmod8 record (H mod 8 and fe per cell), in runs: the
+// half-wave of a pair takes up to 31 steps of its current direction per round,
+// lane c testing step c, and a ballot finds the first step that leaves the
+// run.  Only the cell's value v and the record are needed:
+//   H at (i, j), v > 0: the diagonal is taken iff H(i-1, j-1) + s = v (the
+//     reference prefers the diagonal on ties).  Where it is not taken,
+//     v - s - H(i-1, j-1) lies in (0, match - mismatch]: a path that reaches
+//     (i, j) through a gap in row i (or column j) enters that row at some
+//     column c' <= j - 1 from row i - 1, and the same path with the gap moved
+//     to row i - 1 reaches (i-1, j-1) at a cost at most s(c') <= match lower.
+//     So for match - mismatch <= 7, H mod 8 of the diagonal cell decides.
+//     Otherwise fe (F > E, E winning ties) chooses the gap.  Along a diagonal
+//     run, lane c's value is v minus the scores of steps 0 .. c-1: a ballot of
+//     the matches and a count of the bits below the lane.
+//   E at (i, j) with value v (a D): the gap opened iff H(i, j-1) = v + go, and
+//     otherwise (extension, strictly better) v + ge <= H(i, j-1) < v + go; for
+//     go - ge <= 7, H mod 8 decides.  Lane c tests the cell left of step c,
+//     whose value is v + c ge if steps 0 .. c-1 all extended.  F alike with
+//     the cell above (an I).
+// The window is loaded as in walk16 (lane c of the half loads column jw - c,
+// rows iw - 31 .. iw at least) whenever fewer than kRunMin steps of the
+// current direction are left inside it.  Every lane of a walked half takes
+// part; `walk_h` marks them.  Returns the CIGAR length in every lane of the
+// half (0 where the half is not walked).
+constexpr int kRunMin = 8;
+
+template <class CP, class SP>
+__device__ int walk16r(const uint32_t* dirs, int nq, uint32_t* win, bool walk_h, int lane, SP T, SP P,
+                       int m, int n, int i, int j, int score, int match, int mismatch, int go, int ge,
+                       CP cg, int* si, int* sj)
+{
+    const int hw = lane >> 5, c = lane & 31;
+    uint32_t* w = win + hw * (kWinC * kWinQ);
+    int pos = m + n, st = 0, v = score;            // st: 0 H, 1 E (D), 2 F (I)
+    bool active = walk_h;
+    bool have = false;
+    int iw = 0, jw = 0;                            // window anchor (uniform per half)
+    // the record nibble of cell (ii, jj), inside the window
+    auto nibble = [&](int ii, int jj) {
+        const int l = (jj - 1) & 63, t = ii - 1 + l, r = t & 3;
+        const uint32_t wd = w[(jw - jj) * kWinQ + ((t >> 2) - ((iw - kWinR + l) >> 2))];
+        return (wd >> (16 * (r & 1) + 8 * hw + 4 * (r >> 1))) & 0xfu;
+    };
+    while (__any(active)) {
+        if (active && (i <= 0 || j <= 0 || pos <= 0 || (st == 0 && v <= 0))) active = false;
+        if (!active) continue;
+        // steps of the current direction left inside the window (cells up to
+        // row iw - 31 and column jw - 31); a diagonal step also reads the
+        // cell beyond it
+        int R = 0;
+        if (have) {
+            const int ri = i - iw + 31, rj = j - jw + 31;
+            R = st == 0 ? min(ri, rj) : st == 1 ? rj : ri;
+            R = min(R, 31);
+        }
+        if (R < kRunMin) {
+            const int jc = j - c;                  // 1-based column of this lane
+            if (jc >= 1) {
+                const int l = (jc - 1) & 63, s = (jc - 1) >> 6;
+                const int q0 = (i - kWinR + l) >> 2;    // floor
+                uint32_t val[kWinQ];
+#pragma unroll
+                for (int k = 0; k < kWinQ; ++k) {
+                    const int q = q0 + k;
+                    val[k] = dirs[(int64_t(s) * nq + (q >= 0 && q < nq ? q : 0)) * 64 + l];
+                }
+#pragma unroll
+                for (int k = 0; k < kWinQ; ++k) {
+                    const int q = q0 + k;
+                    w[c * kWinQ + k] = (q >= 0 && q < nq) ? val[k] : 0u;
+                }
+            }
+            iw = i; jw = j; have = true;
+            R = 31;
+        }
+        const int k = st == 0 ? c : c + 1;
+        const int ii = st == 1 ? i : i - k;
+        const int jj = st == 2 ? j : j - k;
+        const bool inb = ii >= 1 && jj >= 1;
+        // a diagonal round reads cells 0 .. R (cell c + 1 is the diagonal
+        // cell of step c), a gap round cells 0 .. R - 1
+        const uint32_t nib = (inb && (st == 0 ? c <= R : c < R)) ? nibble(ii, jj) : 0u;
+        const int hm = int(nib & 7u);
+        bool brk;
+        char ch;
+        uint32_t eh = 0;
+        if (st == 0) {
+            // H mod 8 of the diagonal cell: the next lane's cell
+            const int hp = __builtin_amdgcn_update_dpp(0, hm, 0x130, 0xf, 0xf, false);
+            const bool eq = inb && T[ii - 1] == P[jj - 1];
+            const uint64_t eb = __ballot(eq);
+            eh = hw ? uint32_t(eb >> 32) : uint32_t(eb);
+            const int nb = __popc(eh & ((1u << c) - 1u));
+            const int vc = v - match * nb - mismatch * (c - nb);
+            const int sc = eq ? match : mismatch;
+            brk = c >= R || !inb || vc <= 0 || ((hp + sc - vc) & 7) != 0;
+            ch = eq ? 'M' : 'X';
+        } else {
+            brk = c >= R || ((hm - v - c * ge - go) & 7) == 0;
+            ch = st == 1 ? 'D' : 'I';
+        }
+        const uint64_t bb = __ballot(brk);
+        const uint32_t bh = hw ? uint32_t(bb >> 32) : uint32_t(bb);
+        const int L = __builtin_ctz(bh);            // lane 31 always breaks (R <= 31)
+        const int cnt = st == 0 ? L : (L < R ? L + 1 : R);
+        if (c < cnt && pos - 1 - c >= 0) cg[pos - 1 - c] = ch;
+        pos -= cnt;
+        if (st == 0) {
+            const int nb = __popc(eh & ((1u << L) - 1u));
+            const uint32_t nibL = uint32_t(__shfl(int(nib), 32 * hw + L));
+            v -= match * nb + mismatch * (L - nb);
+            i -= L;
+            j -= L;
+            // a break inside the reach: the border, H = 0 (both end the walk
+            // at the loop head) or a gap, chosen by fe
+            if (L < R) st = (nibL & 8u) ? 2 : 1;
+            if (L < R && (i <= 0 || j <= 0 || v <= 0)) st = 0;
+        } else {
+            const bool opened = L < R;
+            v += opened ? L * ge + go : R * ge;
+            if (st == 1) j -= cnt; else i -= cnt;
+            if (opened) st = 0;
+        }
+    }
+    *si = i;
+    *sj = j;
+    return walk_h ? m + n - pos : 0;
+}
+
 // The score bias b = max(gap_open, -mismatch, 0) exceeds gap_open, so H needs
 // the term b - go (same test as the kernel's bias).
 inline bool sw16_gk(const Sw16Launch& L)
@@ -202,8 +331,15 @@ inline bool sw16_gk(const Sw16Launch& L)
     return m0 > L.go;
 }
 
-// H mod 4 decides a gap's open-or-extend step when 0 <= go - ge <= 3.
-inline bool sw16_hmod(const Sw16Launch& L) { return L.go >= L.ge && L.go - L.ge <= 3; }
+// The trace record layout: H mod 8 and fe when match - mismatch <= 7 and
+// 0 <= go - ge <= 7 (walk16r explains why that suffices), else H mod 4 in place
+// of the gap flags when 0 <= go - ge <= 3, else the four flags.
+inline int sw16_layout(const Sw16Launch& L)
+{
+    if (L.go >= L.ge && L.go - L.ge <= 7 && L.match - L.mismatch <= 7) return kLayHmod8;
+    if (L.go >= L.ge && L.go - L.ge <= 3) return kLayHmod4;
+    return kLayFlags;
+}
 
 // At most 96 VGPRs (5 waves per SIMD) for the trace variant.
 template <bool B> struct BoolC { static constexpr bool value = B; };
@@ -211,9 +347,8 @@ constexpr BoolC<true> kTrue{};
 constexpr BoolC<false> kFalse{};
 
 // GK: gap_open < -mismatch, where H needs the extra bias term (sw16_gk).
-// HMOD (trace only): store H mod 4 in place of the two gap flags, for
-// gap_open - gap_extend <= 3 (sw16_hmod; the walk explains why that suffices).
-template <bool TRACE, bool GK, bool HMOD>
+// LAY (trace only): the trace record layout (sw16_layout).
+template <bool TRACE, bool GK, int LAY>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
 void sw16_kernel(Sw16Launch L)
 {
@@ -313,9 +448,9 @@ void sw16_kernel(Sw16Launch L)
             // both pairs' u from the lane's tables by the row's v_perm selector
             const auto dna_u = [&](int u) { return __builtin_amdgcn_perm(S.tabB, S.tabA, sv[u]); };
             if (decltype(fast)::value || t0 < t_fast)
-                strip_group<TRACE, HMOD, false, GK>(S, t0, lane, dna_u, bv, b2, gk2, ge2, acc, hout, eout, thrm1);
+                strip_group<TRACE, LAY, false, GK>(S, t0, lane, dna_u, bv, b2, gk2, ge2, acc, hout, eout, thrm1);
             else
-                strip_group<TRACE, HMOD, true, true>(S, t0, lane, dna_u, bv, b2, gk2, ge2, acc, hout, eout, thrm1);
+                strip_group<TRACE, LAY, true, true>(S, t0, lane, dna_u, bv, b2, gk2, ge2, acc, hout, eout, thrm1);
             if (TRACE) dirs[(int64_t(s) * nq + (t0 >> 2)) * 64 + lane] = acc;
             // lane 63 hands rows t0-63 .. t0-60 to the next strip (lane 0 has
             // read each row before it is written); rows outside 0 .. Mr-1 are
@@ -424,9 +559,18 @@ void sw16_kernel(Sw16Launch L)
     const bool walker = (lane & 31) == 0 && pidw >= 0 && !redo[hw] && endS[hw] > 0 &&
                         (!L.walk || L.walk[pidw]);
     int si = -1, sj = -1;
-    const int len_h = walk16<HMOD>(dirs, nq, win, walker, lane, hw == 0 ? sTA : sTB, hw == 0 ? sPA : sPB,
-                             hw == 0 ? mA : mB, hw == 0 ? nA : nB, endI[hw] + 1, endJ[hw] + 1,
-                             endS[hw], L.match, L.mismatch, go, ge, hw == 0 ? cgA : cgB, &si, &sj);
+    int len_h;
+    if (LAY == kLayHmod8) {
+        const bool walk_h = pidw >= 0 && !redo[hw] && endS[hw] > 0 && (!L.walk || L.walk[pidw]);
+        len_h = walk16r(dirs, nq, win, walk_h, lane, hw == 0 ? sTA : sTB, hw == 0 ? sPA : sPB,
+                        hw == 0 ? mA : mB, hw == 0 ? nA : nB, endI[hw] + 1, endJ[hw] + 1,
+                        endS[hw], L.match, L.mismatch, go, ge, hw == 0 ? cgA : cgB, &si, &sj);
+    } else {
+        len_h = walk16<LAY == kLayHmod4>(dirs, nq, win, walker, lane, hw == 0 ? sTA : sTB,
+                                         hw == 0 ? sPA : sPB, hw == 0 ? mA : mB, hw == 0 ? nA : nB,
+                                         endI[hw] + 1, endJ[hw] + 1, endS[hw], L.match, L.mismatch,
+                                         go, ge, hw == 0 ? cgA : cgB, &si, &sj);
+    }
     if (walker) {
         int32_t* res = L.res + int64_t(pidw) * 8;
         res[5] = si;
@@ -453,15 +597,17 @@ hipError_t sw16_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHo
     const size_t lds = sw16_lds_bytes(L.max_m, trace ? L.max_n : 0);
     const dim3 grid(unsigned(L.n_waves)), block(64);
     if (hooks) hooks->before(st);
-    const bool hm = sw16_hmod(L);
+    const int lay = sw16_layout(L);
     if (sw16_gk(L)) {
-        if (trace && hm) hipLaunchKernelGGL((sw16_kernel<true, true, true>), grid, block, lds, st, L);
-        else if (trace)  hipLaunchKernelGGL((sw16_kernel<true, true, false>), grid, block, lds, st, L);
-        else             hipLaunchKernelGGL((sw16_kernel<false, true, false>), grid, block, lds, st, L);
+        if (!trace)                  hipLaunchKernelGGL((sw16_kernel<false, true, kLayFlags>), grid, block, lds, st, L);
+        else if (lay == kLayHmod8)   hipLaunchKernelGGL((sw16_kernel<true, true, kLayHmod8>), grid, block, lds, st, L);
+        else if (lay == kLayHmod4)   hipLaunchKernelGGL((sw16_kernel<true, true, kLayHmod4>), grid, block, lds, st, L);
+        else                         hipLaunchKernelGGL((sw16_kernel<true, true, kLayFlags>), grid, block, lds, st, L);
     } else {
-        if (trace && hm) hipLaunchKernelGGL((sw16_kernel<true, false, true>), grid, block, lds, st, L);
-        else if (trace)  hipLaunchKernelGGL((sw16_kernel<true, false, false>), grid, block, lds, st, L);
-        else             hipLaunchKernelGGL((sw16_kernel<false, false, false>), grid, block, lds, st, L);
+        if (!trace)                  hipLaunchKernelGGL((sw16_kernel<false, false, kLayFlags>), grid, block, lds, st, L);
+        else if (lay == kLayHmod8)   hipLaunchKernelGGL((sw16_kernel<true, false, kLayHmod8>), grid, block, lds, st, L);
+        else if (lay == kLayHmod4)   hipLaunchKernelGGL((sw16_kernel<true, false, kLayHmod4>), grid, block, lds, st, L);
+        else                         hipLaunchKernelGGL((sw16_kernel<true, false, kLayFlags>), grid, block, lds, st, L);
     }
     if (hooks) hooks->after(st);
     return hipGetLastError();

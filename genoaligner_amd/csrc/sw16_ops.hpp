@@ -81,17 +81,23 @@ struct Strip {
     uint32_t colmask;              // 0xffff per half whose column exists
 };
 
+// Trace record layouts (4 bits per cell and pair, 4 rows per dword):
+//   kLayFlags  nd, fe, ee, fx as flags
+//   kLayHmod4  nd, fe as flags, H mod 4 in place of ee, fx
+//   kLayHmod8  H mod 8 and fe only (the walk derives nd from H mod 8)
+constexpr int kLayFlags = 0, kLayHmod4 = 1, kLayHmod8 = 2;
+
 // Four steps t0 .. t0+3 of a strip.  score(u) gives the biased scores u of
 // both pairs for step t0 + u (one per half), bv the group's boundary rows
 // (prefetched by the caller).  COLS: columns past a pair's end are masked out
-// of the maxima (partial strips only).
+// of the maxima (partial strips only).  LAY: the trace record layout.
 //
 // Rows past a pair's end (the drain, and the shorter pair of a wave) need no
 // mask: their selectors give u = 0, i.e. s = -bias <= -1, and gap costs are at
 // least 1, so each such cell's H, E and F are below a value of an earlier
 // cell: by induction H <= max(0, R - 1), R being the pair's largest real H.
 // They never reach the pair's maximum, and real cells never depend on them.
-template <bool TRACE, bool HMOD, bool COLS, bool GK, class Score>
+template <bool TRACE, int LAY, bool COLS, bool GK, class Score>
 __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, Score score,
                                             const uint2 (&bv)[4], uint32_t b2, uint32_t gk2,
                                             uint32_t ge2, uint32_t& acc, uint32_t (&hout)[4],
@@ -117,14 +123,16 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, Score sc
         uint32_t hb = Hn;
         if (COLS) hb &= S.colmask;
         Hb[u] = hb;
-        if (TRACE) {
+        if (TRACE && LAY == kLayHmod8) {
+            Q[u] = pdiff(En, Fn);                 // sign bit of each half: fe
+        } else if (TRACE) {
             // nd: A2 > A1 (both carry the same bias, so the comparison is exact).
             // v_perm selectors 8..11 replicate the sign bit of a half into a
             // whole byte (8: S1 low half, 9: S1 high, 10: S0 low, 11: S0 high),
             // so each perm yields 0xff/0x00 byte masks for two flags of both
             // pairs; Q[u] keeps nd/fe in the high and ee/fx in the low nibble.
             const uint32_t P1 = __builtin_amdgcn_perm(pdiff(En, Fn), pdiff(A1, A2), 0x0b0a0908u);
-            if (HMOD) {
+            if (LAY == kLayHmod4) {
                 Q[u] = P1;
             } else {
                 const uint32_t P2 = __builtin_amdgcn_perm(pdiff(S.H, xf), pdiff(Hl, xe), 0x0b0a0908u);
@@ -140,13 +148,27 @@ __device__ __forceinline__ void strip_group(Strip& S, int t0, int lane, Score sc
     }
     // Row u of the group goes to bit 4+u (nd, fe) and bit u (ee, fx) of each
     // byte: bits alternate between rows 0/1 and 2/3, then pairs of bits
-    // between the two.  HMOD: the flags give bits u and 4+u copies of nd, fe;
+    // between the two.  kLayHmod4: the flags give bits u and 4+u copies of nd, fe;
     // the high nibbles then take H mod 4 of each row (one shift and one
     // bit-select per row, where the two gap flags took two differences, a
     // perm and a select).
-    if (TRACE) {
+    // kLayHmod8: row r of the group and half h take the nibble at bit
+    // 16 (r & 1) + 8 h + 4 (r >> 1): H mod 8 in its low 3 bits, fe in bit 3.
+    // Two perms gather the low bytes of H (rows 0/1, rows 2/3), two more the
+    // high bytes of the differences (their sign bits: fe); the fe bytes of
+    // rows 0/1 are gathered one byte rotated so that one rotate by 4 bits
+    // puts them at bit 3 of the right byte, next to H mod 8 of rows 2/3.
+    if (TRACE && LAY == kLayHmod8) {
+        const uint32_t h01 = __builtin_amdgcn_perm(hout[1], hout[0], 0x06040200u);
+        const uint32_t h23 = __builtin_amdgcn_perm(hout[3], hout[2], 0x06040200u);
+        const uint32_t d23 = __builtin_amdgcn_perm(Q[3], Q[2], 0x07050301u);
+        const uint32_t d01 = __builtin_amdgcn_perm(Q[1], Q[0], 0x01070503u);
+        const uint32_t lo = bsel(0x07070707u, h01, d23);
+        const uint32_t y = bsel(0x07070707u, h23, d01);
+        acc = bsel(0x87878787u, lo, __builtin_amdgcn_alignbit(y, y, 28));
+    } else if (TRACE) {
         acc = bsel(0x33333333u, bsel(0x55555555u, Q[0], Q[1]), bsel(0x55555555u, Q[2], Q[3]));
-        if (HMOD) {
+        if (LAY == kLayHmod4) {
             acc = bsel(0x00030003u << 4, hout[0] << 4, acc);
             acc = bsel(0x00030003u << 6, hout[1] << 6, acc);
             acc = bsel(0x00030003u << 12, hout[2] << 12, acc);

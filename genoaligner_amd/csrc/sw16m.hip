@@ -369,7 +369,7 @@ __global__ __launch_bounds__(64 * WAVES) void sw16m_kernel(Sw16Launch L)
                 uint32_t acc = 0, hout[4], eout[4];
                 const uint32_t (&u4)[4] = uu[p];
                 const auto mat_u = [&](int u) { return u4[u]; };
-                strip_group<TRACE, false, !FAST, FAST ? GK : true>(S, t0, lane, mat_u, bv[p], b2, gk2, ge2, acc,
+                strip_group<TRACE, kLayFlags, !FAST, FAST ? GK : true>(S, t0, lane, mat_u, bv[p], b2, gk2, ge2, acc,
                                                                   hout, eout, thrm1);
                 if (TRACE) dirs[(int64_t(s) * nq + (t0 >> 2)) * 64 + lane] = acc;
                 // lane 63 hands rows t0-63 .. t0-60 to strip s + 1; rows

@@ -206,14 +206,17 @@ def test_sw_random_vs_oracle(gpu, oracle):
                     exp.end_j, exp.rescore_ok, exp.wellformed_ok, exp.too_large), (sc, t, p)
 
 
-def test_sw_long_gaps_both_trace_layouts(gpu, oracle):
-    """Pairs with inserted and deleted blocks of 2..40 bases, under schemes
-    with gap_open - gap_extend <= 3 (the packed trace stores H mod 4 and the
-    walk derives each gap step from it) and above 3 (it stores the two gap
-    flags), with and without the gk bias term."""
+def test_sw_long_gaps_all_trace_layouts(gpu, oracle):
+    """Pairs with inserted and deleted blocks of 2..40 bases, under the three
+    packed trace records, each with and without the gk bias term:
+    match - mismatch <= 7 and gap_open - gap_extend <= 7 (H mod 8 and fe; the
+    walk derives diagonal and gap steps from H mod 8), match - mismatch > 7
+    with gap_open - gap_extend <= 3 (H mod 4 for the gap steps, H-source flags)
+    and the rest (four flags)."""
     import genoaligner_amd as ga
     rng = random.Random(47)
-    for sc in ((2, -3, 3, 1), (2, -3, 5, 2), (1, -2, 4, 1), (2, -3, 6, 1), (3, -4, 9, 2), (2, -8, 6, 1)):
+    for sc in ((2, -3, 3, 1), (2, -3, 5, 2), (1, -2, 4, 1), (2, -3, 6, 1), (3, -4, 9, 2), (2, -5, 2, 1),
+               (5, -4, 6, 4), (5, -4, 3, 1), (2, -8, 6, 1), (1, -7, 9, 1)):
         reqs, pairs = [], []
         for _ in range(120):
             t = "".join(rng.choice("ACGT") for _ in range(rng.randint(50, 900)))
