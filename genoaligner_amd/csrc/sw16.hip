@@ -405,6 +405,10 @@ void sw16_kernel(Sw16Launch L)
     const int nq = nsteps >> 2;
     const int nmin = nA < nB ? nA : nB;
     uint32_t* dirs = TRACE ? L.dirs + int64_t(w) * L.dir_stride : nullptr;
+    // buffer resource over this wave's trace words (stores beyond them are
+    // dropped by the range check)
+    const __amdgpu_buffer_rsrc_t drs =
+        __builtin_amdgcn_make_buffer_rsrc(TRACE ? static_cast<void*>(dirs) : static_cast<void*>(L.res), 0, TRACE ? int(L.dir_stride * 4) : 0, 0x00020000);
     int gH[2] = {0, 0}, gI[2] = {-1, -1}, gJ[2] = {-1, -1};
     uint32_t thrm1 = 0;                           // per half: the pair's best so far - 1 (uniform)
 
@@ -443,6 +447,12 @@ void sw16_kernel(Sw16Launch L)
         for (int k = 0; k < kDepth; ++k) asm volatile("" ::"v"(sv[k][0]));
         // A strip whose columns all exist in both pairs needs no mask at all.
         const int t_fast = full ? nsteps : 0;
+        // For lane 63, the byte offset of boundary row t0 - 63 in the LDS
+        // column: kept in a VGPR that advances with the groups (an offset the
+        // compiler sees as uniform lives in an SGPR and costs a move into a
+        // VGPR before every store).
+        int boff = -63 * 8;
+        asm volatile("" : "+v"(boff));
         auto group = [&](auto fast, int t0, const uint32_t (&sv)[4], const uint2 (&bv)[4]) {
             uint32_t acc = 0, hout[4], eout[4];
             // both pairs' u from the lane's tables by the row's v_perm selector
@@ -451,20 +461,23 @@ void sw16_kernel(Sw16Launch L)
                 strip_group<TRACE, LAY, false, GK>(S, t0, lane, dna_u, bv, b2, gk2, ge2, acc, hout, eout, thrm1);
             else
                 strip_group<TRACE, LAY, true, true>(S, t0, lane, dna_u, bv, b2, gk2, ge2, acc, hout, eout, thrm1);
-            if (TRACE) dirs[(int64_t(s) * nq + (t0 >> 2)) * 64 + lane] = acc;
+            // the group's trace word: lane offset in a VGPR, group offset in an
+            // SGPR (no per-group address arithmetic)
+            if (TRACE)
+                __builtin_amdgcn_raw_buffer_store_b32(acc, drs, lane * 4, (s * nq + (t0 >> 2)) * 256, 0);
             // lane 63 hands rows t0-63 .. t0-60 to the next strip (lane 0 has
             // read each row before it is written); rows outside 0 .. Mr-1 are
             // dropped, and rows M .. Mr-1 only reach drained cells
+            uint2* b = reinterpret_cast<uint2*>(reinterpret_cast<char*>(bnd) + (boff + t0 * 8));
             if (lane == 63) {
                 if (t0 >= 64 && t0 - 60 < Mr) {
-                    uint2* b = bnd + (t0 - 63);
 #pragma unroll
                     for (int u = 0; u < 4; ++u) b[u] = make_uint2(hout[u], eout[u]);
                 } else {
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const int r = t0 + u - 63;
-                        if (r >= 0 && r < Mr) bnd[r] = make_uint2(hout[u], eout[u]);
+                        if (r >= 0 && r < Mr) b[u] = make_uint2(hout[u], eout[u]);
                     }
                 }
             }
