@@ -306,7 +306,7 @@ def main():
     ap.add_argument("--routing-pairs", type=int, default=200_000)
     ap.add_argument("--router-weights", default=None,
                     help="router parameters (.npy); default: the trained file shipped in the package")
-    ap.add_argument("--route-q", type=float, default=0.95)
+    ap.add_argument("--route-q", type=float, default=0.99)
     ap.add_argument("--no-protein", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-fasta", action="store_true")

@@ -26,7 +26,7 @@ EXPORTS = (
     "ga_backend_name", "ga_device_name", "ga_device_available",
     "ga_cigar_arena_bytes_wfa", "ga_cigar_arena_bytes_sw",
     "ga_wfa_device", "ga_wfa_device_noalloc", "ga_sw_device", "ga_set_kernel_timing", "ga_last_kernel_ms",
-    "ga_sketch_device", "ga_mlp_forward_device", "ga_mlp_backward_device", "ga_mlp_sgd_device",
+    "ga_sketch_device", "ga_mlp_forward_device", "ga_mlp_logits_device", "ga_mlp_backward_device", "ga_mlp_sgd_device",
     "ga_mlp_route_device", "ga_route_bucket_bound", "ga_wfa_routed_device", "ga_router_set", "ga_router_enabled",
     "ga_align_sw_matrix_batch", "ga_sw_matrix_blosum62", "ga_set_thread_workspace_budget",
     "ga_trim_workspace",
@@ -172,11 +172,12 @@ def lib() -> C.CDLL:
         vp, i64, f32 = C.c_void_p, C.c_int64, C.c_float
         L.ga_sketch_device.argtypes = [C.POINTER(WfaDevBatch), vp, vp, vp, cpp]
         L.ga_mlp_forward_device.argtypes = [vp, vp, i64, vp, vp, vp, vp, cpp]
+        L.ga_mlp_logits_device.argtypes = [vp, vp, i64, vp, vp, cpp]
         L.ga_mlp_backward_device.argtypes = [vp, vp, vp, vp, vp, vp, i64, vp, vp, vp, vp, cpp]
         L.ga_mlp_sgd_device.argtypes = [vp, vp, vp, f32, f32, vp, cpp]
         L.ga_mlp_route_device.argtypes = [vp, i64, f32, vp, vp, cpp]
         L.ga_wfa_routed_device.argtypes = [C.POINTER(WfaDevBatch), vp, f32, C.POINTER(RouteStats), vp, cpp]
-        for f in (L.ga_sketch_device, L.ga_mlp_forward_device, L.ga_mlp_backward_device,
+        for f in (L.ga_sketch_device, L.ga_mlp_forward_device, L.ga_mlp_logits_device, L.ga_mlp_backward_device,
                   L.ga_mlp_sgd_device, L.ga_mlp_route_device, L.ga_wfa_routed_device):
             f.restype = C.c_int
         L.ga_router_set.argtypes = [vp, f32, cpp]

@@ -305,7 +305,7 @@ def align_sw_matrix_batch(reqs: Sequence[SWRequest], matrix) -> SWBatchResult:
     return _sw_call(reqs, matrix)
 
 
-def set_router(params: Optional[np.ndarray], q: float = 0.95) -> None:
+def set_router(params: Optional[np.ndarray], q: float = 0.99) -> None:
     """Load a trained router network (GA_MLP_PARAMS float32 values; see
     genoaligner_amd.router) into the library, or unload it with None.  While
     one is loaded, align_batch and align_batch_ext run their WFA batches

@@ -111,6 +111,8 @@ int device_cus();
 hipError_t sketch_launch(const SketchLaunch& L, hipStream_t st);
 hipError_t mlp_forward(const float* params, const float* X, int64_t N, float* H1, float* H2, float* logits,
                        hipStream_t st);
+// logits only, layers fused (inference; params 16-byte aligned)
+hipError_t mlp_logits(const float* params, const float* X, int64_t N, float* logits, hipStream_t st);
 hipError_t mlp_backward(const float* params, const float* X, const float* H1, const float* H2, const float* logits,
                         const int32_t* label, int64_t N, float* grads, float* loss, float* dL, float* dH2,
                         float* dH1, hipStream_t st);

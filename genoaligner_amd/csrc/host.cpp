@@ -806,8 +806,10 @@ const char* setup_wfa_dev_classes(const ga::WfaQueueLaunch& base, int64_t n, int
 // Plan on the device, then the class launches on forked streams joined back
 // into `st`; `side` holds one stream per class beyond the first (so the same
 // streams can serve a graph capture).  No host synchronisation.
+// Classes from `detach` on are not joined: their completion is recorded in
+// ev[c] for the caller to wait on.
 const char* enqueue_wfa_dev(const WfaDevClasses& C, hipStream_t st, const std::vector<hipStream_t>& side,
-                            std::vector<hipEvent_t>& ev)
+                            std::vector<hipEvent_t>& ev, size_t detach = SIZE_MAX)
 {
     if (!C.planless && ga::wfa_plan_launch(C.plan, st) != hipSuccess) return "WFA plan launch";
     const size_t nc = C.L.size();
@@ -817,8 +819,8 @@ const char* enqueue_wfa_dev(const WfaDevClasses& C, hipStream_t st, const std::v
         hipStream_t cs = c == 0 ? st : side[c - 1];
         if (c > 0 && hipStreamWaitEvent(cs, ev[0], 0) != hipSuccess) return "hipStreamWaitEvent";
         if (ga::wfa_queue_launch(C.L[c], cs) != hipSuccess) return "WFA kernel launch";
-        if (c > 0 && (hipEventRecord(ev[c], cs) != hipSuccess || hipStreamWaitEvent(st, ev[c], 0) != hipSuccess))
-            return "hipEventRecord";
+        if (c > 0 && hipEventRecord(ev[c], cs) != hipSuccess) return "hipEventRecord";
+        if (c > 0 && c < detach && hipStreamWaitEvent(st, ev[c], 0) != hipSuccess) return "hipStreamWaitEvent";
     }
     return nullptr;
 }
@@ -1926,6 +1928,25 @@ int ga_mlp_forward_device(const float* params, const float* feat, int64_t n, flo
     return GA_OK;
 }
 
+int ga_mlp_logits_device(const float* params, const float* feat, int64_t n, float* logits, void* stream,
+                         const char** error)
+{
+    if (error) *error = nullptr;
+    if (n < 0 || !params || (n > 0 && (!feat || !logits))) {
+        if (error) *error = "invalid MLP arguments";
+        return GA_INVALID_ARGUMENT;
+    }
+    if (reinterpret_cast<uintptr_t>(params) & 15u) {
+        if (error) *error = "MLP parameters not 16-byte aligned";
+        return GA_INVALID_ARGUMENT;
+    }
+    if (ga::mlp_logits(params, feat, n, logits, static_cast<hipStream_t>(stream)) != hipSuccess) {
+        if (error) *error = "MLP logits launch";
+        return GA_DEVICE_ERROR;
+    }
+    return GA_OK;
+}
+
 int ga_mlp_backward_device(const float* params, const float* feat, const float* h1, const float* h2,
                            const float* logits, const int32_t* label, int64_t n, float* grads, float* loss,
                            float* work, void* stream, const char** error)
@@ -1975,6 +1996,9 @@ int32_t ga_route_bucket_bound(int32_t bucket)
     return (bucket >= 0 && bucket < ga::kBuckets) ? kBucketBound[bucket] : -1;
 }
 
+// Stack bytes per class launch of the routed path's second pass.
+constexpr int64_t kRerunStackCap = int64_t(4) << 30;
+
 // The routed WFA batch (sketch, network, routed pass, plain re-run) enqueued
 // on st; with stats non-null the stream is waited for once at the end.
 extern "C++" const char* ga::wfa_routed_run(const ga_wfa_dev_batch* b, const float* params, float q, ga_route_stats* stats,
@@ -1998,17 +2022,22 @@ extern "C++" const char* ga::wfa_routed_run(const ga_wfa_dev_batch* b, const flo
     const char* err = nullptr;
     float* feat = static_cast<float*>(alloc(size_t(n) * ga::kSketchDim * 4));
     int32_t* lb = static_cast<int32_t*>(alloc(size_t(n) * 4));
-    float* h1 = static_cast<float*>(alloc(size_t(n) * ga::kHidden * 4));
-    float* h2 = static_cast<float*>(alloc(size_t(n) * ga::kHidden * 4));
     float* logits = static_cast<float*>(alloc(size_t(n) * ga::kBuckets * 4));
+    // the fused forward reads the weights as float4: parameters off a 16-byte
+    // boundary take the layer-by-layer forward (activations in scratch)
+    const bool fused = (reinterpret_cast<uintptr_t>(params) & 15u) == 0;
+    float* h1 = fused ? nullptr : static_cast<float*>(alloc(size_t(n) * ga::kHidden * 4));
+    float* h2 = fused ? nullptr : static_cast<float*>(alloc(size_t(n) * ga::kHidden * 4));
     int32_t* bucket = static_cast<int32_t*>(alloc(size_t(n) * 4));
     int32_t* sm1 = static_cast<int32_t*>(alloc(size_t(n) * 4));
     int32_t* sm2 = static_cast<int32_t*>(alloc(size_t(n) * 4));
     auto* dstats = static_cast<unsigned long long*>(alloc(64));
-    if (!feat || !lb || !h1 || !h2 || !logits || !bucket || !sm1 || !sm2 || !dstats) err = "hipMalloc(router buffers)";
+    if (!feat || !lb || !logits || !bucket || !sm1 || !sm2 || !dstats || (!fused && (!h1 || !h2)))
+        err = "hipMalloc(router buffers)";
     ga::SketchLaunch S{b->pattern, b->pattern_off, b->pattern_len, b->text, b->text_off, b->text_len, n, feat, lb};
     if (!err && (ga::zero_u64_launch(dstats, 4, st) != hipSuccess || ga::sketch_launch(S, st) != hipSuccess ||
-                 ga::mlp_forward(params, feat, n, h1, h2, logits, st) != hipSuccess ||
+                 (fused ? ga::mlp_logits(params, feat, n, logits, st)
+                        : ga::mlp_forward(params, feat, n, h1, h2, logits, st)) != hipSuccess ||
                  ga::mlp_route(logits, n, q, bucket, st) != hipSuccess ||
                  ga::route_first_pass_launch(bucket, lb, b->smax, b->pattern_len, b->text_len, int(n), sm1, b->score,
                                              b->cigar ? b->cigar_len : nullptr, dstats, st) != hipSuccess))
@@ -2019,41 +2048,58 @@ extern "C++" const char* ga::wfa_routed_run(const ga_wfa_dev_batch* b, const flo
     L.want = b->want_cigar;
     L.score = b->score; L.cigar = b->cigar; L.cig_off = b->cigar_off; L.cig_len = b->cigar_len;
     L.wide = b->max_len > 32766;
-    std::vector<hipStream_t> side;
-    std::vector<hipEvent_t> ev(ga::kWfaClasses, nullptr);
-    for (int c = 1; !err && c < ga::kWfaClasses; ++c) {
-        hipStream_t cs = acquire_stream();
-        if (!cs) err = "hipStreamCreate";
-        else side.push_back(cs);
+    // each pass forks its classes onto streams of its own
+    std::vector<hipStream_t> side[2];
+    std::vector<hipEvent_t> ev[2] = {std::vector<hipEvent_t>(ga::kWfaClasses, nullptr),
+                                     std::vector<hipEvent_t>(ga::kWfaClasses, nullptr)};
+    for (int p = 0; p < 2; ++p) {
+        for (int c = 1; !err && c < ga::kWfaClasses; ++c) {
+            hipStream_t cs = acquire_stream();
+            if (!cs) err = "hipStreamCreate";
+            else side[p].push_back(cs);
+        }
+        for (auto& e : ev[p])
+            if (!err && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) err = "hipEventCreate";
     }
-    for (auto& e : ev)
-        if (!err && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) err = "hipEventCreate";
-    // two device-planned passes, no host synchronisation: the routed bounds,
-    // then the pairs not resolved under them with their own smax
+    // Two device-planned passes, no host synchronisation: the routed bounds,
+    // then the pairs not resolved under them with their own smax.  A routed
+    // bound in the largest class is the pair's own smax (route_first_pass), so
+    // no pair of that class needs a second try, and the second pass waits only
+    // for the first pass's smaller classes: it runs beside the largest class's
+    // tail.  Both passes' buffers stay held until the end.
+    size_t detached = SIZE_MAX;
+    if (!err) t_timer.begin(st);                          // both passes: one interval
     for (int pass = 0; pass < 2 && !err; ++pass) {
         if (pass == 1 && ga::route_second_pass_launch(sm1, b->smax, b->score, int(n), sm2, dstats, st) != hipSuccess) {
             err = "router kernel launch";
             break;
         }
-        const size_t mark = held.size();
         L.smax = pass == 0 ? sm1 : sm2;
         WfaDevClasses C;
-        err = setup_wfa_dev_classes(L, n, b->smax_max, b->max_len, b->cigar != nullptr, C, alloc);
+        // the second pass holds its stacks beside the first pass's largest
+        // class and runs the few pairs the router sent too low: a small cap
+        err = setup_wfa_dev_classes(L, n, b->smax_max, b->max_len, b->cigar != nullptr, C, alloc,
+                                    pass == 0 ? -1 : kRerunStackCap);
         if (!err) {
-            t_timer.begin(st);
-            err = enqueue_wfa_dev(C, st, side, ev);
-            t_timer.end(st);
+            const size_t detach = pass == 0 ? size_t(ga::kWfaClasses - 1) : SIZE_MAX;
+            err = enqueue_wfa_dev(C, st, side[pass], ev[pass], detach);
+            if (pass == 0 && C.L.size() > detach) detached = detach;
         }
-        release(mark);                                    // stream-ordered: reused by the next pass
     }
+    // join the first pass's largest class
+    if (!err && detached != SIZE_MAX && hipStreamWaitEvent(st, ev[0][detached], 0) != hipSuccess)
+        err = "hipStreamWaitEvent";
+    t_timer.end(st);
     unsigned long long hs[4] = {0, 0, 0, 0};
     if (!err && stats && (hipMemcpyAsync(hs, dstats, sizeof(hs), hipMemcpyDeviceToHost, st) != hipSuccess ||
                           hipStreamSynchronize(st) != hipSuccess))
         err = "hipMemcpy(router stats)";
     release(0);
-    for (auto& e : ev)
-        if (e) (void)hipEventDestroy(e);
-    for (hipStream_t cs : side) release_stream(cs);
+    for (auto& evp : ev)
+        for (auto& e : evp)
+            if (e) (void)hipEventDestroy(e);
+    for (auto& sp : side)
+        for (hipStream_t cs : sp) release_stream(cs);
     if (!err && stats) {
         stats->shortcut = int64_t(hs[0]);
         stats->routed = int64_t(hs[1]);
@@ -2080,7 +2126,7 @@ int ga_wfa_routed_device(const ga_wfa_dev_batch* b, const float* params, float q
 // ------------------------------------------------------ the library's router
 namespace {
 std::atomic<const float*> g_router_params{nullptr};
-std::atomic<float> g_router_q{0.95f};
+std::atomic<float> g_router_q{0.99f};
 }  // namespace
 
 int ga_router_set(const float* params, float q, const char** error)

@@ -163,6 +163,76 @@ __global__ __launch_bounds__(256) void dense_fwd(const float* X, const float* W,
     }
 }
 
+// Inference only (the routed path): the three layers fused, logits out.  A
+// workgroup takes kLRows rows; the layer inputs stay in LDS, transposed
+// (column i of the tile is kLRows consecutive floats), so a thread's 4 rows
+// of one input come in one read; each thread of a hidden layer computes a
+// 4-row x 4-output tile, one weight read serving 16 products.  Each sum runs
+// over the inputs in ascending order with fused multiply-adds, then adds the
+// bias (the training forward above multiplies and adds separately: logits
+// may differ from it in the last bits, which changes no alignment result).
+constexpr int kLRows = 32;
+
+__global__ __launch_bounds__(256) void mlp_logits_kernel(const float* __restrict__ params,
+                                                          const float* __restrict__ X, float* logits, int64_t N)
+{
+    __shared__ __align__(16) float xt[kSketchDim * kLRows];
+    __shared__ __align__(16) float h1t[kHidden * kLRows];
+    __shared__ __align__(16) float h2t[kHidden * kLRows];
+    const MlpLayout P;
+    const int t = threadIdx.x;
+    const int64_t r0 = int64_t(blockIdx.x) * kLRows;
+    for (int x = t; x < kLRows * kSketchDim; x += 256) {
+        const int r = x / kSketchDim, i = x % kSketchDim;
+        xt[i * kLRows + r] = r0 + r < N ? X[(r0 + r) * kSketchDim + i] : 0.0f;
+    }
+    __syncthreads();
+    const int rg = (t >> 5) * 4, og = (t & 31) * 4;
+    auto hidden = [&](const float* in_t, int I, const float* W, const float* b, float* out_t) {
+        float acc[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int o = 0; o < 4; ++o) acc[q][o] = 0.0f;
+        for (int i = 0; i < I; ++i) {
+            const float4 xv = *reinterpret_cast<const float4*>(in_t + i * kLRows + rg);
+            const float4 wv = *reinterpret_cast<const float4*>(W + i * kHidden + og);
+            const float xs[4] = {xv.x, xv.y, xv.z, xv.w};
+            const float ws[4] = {wv.x, wv.y, wv.z, wv.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int o = 0; o < 4; ++o) acc[q][o] = __builtin_fmaf(xs[q], ws[o], acc[q][o]);
+        }
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            const float bo = b[og + o];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float v = acc[q][o] + bo;
+                out_t[(og + o) * kLRows + rg + q] = v > 0.0f ? v : 0.0f;
+            }
+        }
+    };
+    hidden(xt, kSketchDim, params + P.w1, params + P.b1, h1t);
+    __syncthreads();
+    hidden(h1t, kHidden, params + P.w2, params + P.b2, h2t);
+    __syncthreads();
+    // output layer: row t / 8, outputs 2 (t % 8) and the next
+    const int r = t >> 3, o = (t & 7) * 2;
+    float a0 = 0.0f, a1 = 0.0f;
+    for (int i = 0; i < kHidden; ++i) {
+        const float x = h2t[i * kLRows + r];
+        const float2 wv = *reinterpret_cast<const float2*>(params + P.w3 + i * kBuckets + o);
+        a0 = __builtin_fmaf(x, wv.x, a0);
+        a1 = __builtin_fmaf(x, wv.y, a1);
+    }
+    if (r0 + r < N) {
+        logits[(r0 + r) * kBuckets + o] = a0 + params[P.b3 + o];
+        logits[(r0 + r) * kBuckets + o + 1] = a1 + params[P.b3 + o + 1];
+    }
+}
+
 // dX[r, i] = (sum_o dY[r, o] W[i, o]) * (A[r, i] > 0), A the layer input
 // after its ReLU.
 __global__ __launch_bounds__(256) void dense_bwd_data(const float* dY, const float* W, const float* A,
@@ -264,11 +334,25 @@ __global__ void zero_u64(unsigned long long* p, int n)
 
 __constant__ int32_t kBound[kBuckets] = {0, 1, 2, 4, 8, 16, 32, 64, 128, 256, 384, 512, 768, 1024, 2048, 0x7fffffff};
 
-// Adds the number of lanes with `pred` to *ctr: one atomic per wave.
-__device__ __forceinline__ void wave_count(bool pred, unsigned long long* ctr)
+// Adds, for each k, the number of threads of the (256-thread) workgroup with
+// pred[k] to ctr[k]: wave counts gathered in LDS, one atomic per workgroup and
+// counter (atomics from every wave on the same few counters serialise).  Every
+// thread of the workgroup calls it.
+template <int K>
+__device__ __forceinline__ void block_count(const bool (&pred)[K], unsigned long long* ctr)
 {
-    const unsigned long long m = __ballot(pred);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(ctr, (unsigned long long)__popcll(m));
+    __shared__ unsigned int part[4][K];
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const unsigned long long m = __ballot(pred[k]);
+        if ((threadIdx.x & 63) == 0) part[w][k] = unsigned(__popcll(m));
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        const unsigned sum = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+        if (sum) atomicAdd(&ctr[threadIdx.x], (unsigned long long)sum);
+    }
 }
 
 // First pass of a routed batch: each pair's bound = its predicted bucket's
@@ -295,13 +379,15 @@ __global__ __launch_bounds__(256) void route_first_pass(const int32_t* bucket, c
             int v = kBound[b] < s ? kBound[b] : s;
             const int l = lb[i] < s ? lb[i] : s;
             v = v > l ? v : l;
+            // a bound in the largest size class is the pair's own smax: the
+            // second pass then only re-runs pairs of the smaller classes
+            if (v > kWfaClassBound2) v = s;
             sm1[i] = v;
             kind = v < s ? 1 : 2;
         }
     }
-    wave_count(kind == 0, &stats[0]);
-    wave_count(kind == 1, &stats[1]);
-    wave_count(kind == 2, &stats[2]);
+    const bool pred[3] = {kind == 0, kind == 1, kind == 2};
+    block_count(pred, stats);
 }
 
 // Second pass: the routed pairs not resolved under their first bound, with
@@ -315,7 +401,8 @@ __global__ __launch_bounds__(256) void route_second_pass(const int32_t* sm1, con
         again = sm1[i] >= 0 && sm1[i] < smax[i] && score[i] < 0;
         sm2[i] = again ? smax[i] : -1;
     }
-    wave_count(again, &stats[3]);
+    const bool pred[1] = {again};
+    block_count(pred, stats + 3);
 }
 
 inline unsigned blocks(int64_t n, int per) { return unsigned((n + per - 1) / per); }
@@ -342,6 +429,15 @@ hipError_t mlp_forward(const float* params, const float* X, int64_t N, float* H1
                        H1, params + P.w2, params + P.b2, H2, N, kHidden, kHidden);
     hipLaunchKernelGGL((dense_fwd<false>), dim3(g), dim3(256), kRows * kHidden * 4, st,
                        H2, params + P.w3, params + P.b3, logits, N, kHidden, kBuckets);
+    return hipGetLastError();
+}
+
+hipError_t mlp_logits(const float* params, const float* X, int64_t N, float* logits, hipStream_t st)
+{
+    if (N <= 0) return hipSuccess;
+    // the float4 weight reads need 16-byte aligned parameters
+    if (reinterpret_cast<uintptr_t>(params) & 15u) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(mlp_logits_kernel, dim3(blocks(N, kLRows)), dim3(256), 0, st, params, X, logits, N);
     return hipGetLastError();
 }
 

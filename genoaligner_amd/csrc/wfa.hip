@@ -563,11 +563,30 @@ __device__ __forceinline__ int plan_key(int m, int n)
     return kPlanKeys - 1 - (l < kPlanKeys - 1 ? l : kPlanKeys - 1);
 }
 
+// Large batches: each workgroup counts kPlanChunk pairs into an LDS histogram
+// and adds its non-zero bins to the global one (a global atomic per pair
+// serialised on the few bins most pairs share).
+constexpr int kPlanChunk = 4096;
+constexpr int kPlanBins = kWfaClasses * kPlanKeys;
+
+__device__ __forceinline__ int plan_bin(const WfaDevPlan& P, int i)
+{
+    return plan_class(P.smax[i]) * kPlanKeys + plan_key(P.plen[i], P.tlen[i]);
+}
+
 __global__ __launch_bounds__(256) void plan_hist_kernel(WfaDevPlan P)
 {
-    const int i = int(blockIdx.x * 256 + threadIdx.x);
-    if (i >= P.n || P.smax[i] < 0) return;             // smax < 0: not in this plan
-    atomicAdd(&P.hist[plan_class(P.smax[i]) * kPlanKeys + plan_key(P.plen[i], P.tlen[i])], 1);
+    __shared__ int hist[kPlanBins];
+    const int t = threadIdx.x;
+    for (int b = t; b < kPlanBins; b += 256) hist[b] = 0;
+    __syncthreads();
+    const int i0 = int(blockIdx.x) * kPlanChunk;
+    const int i1 = i0 + kPlanChunk < P.n ? i0 + kPlanChunk : P.n;
+    for (int i = i0 + t; i < i1; i += 256)
+        if (P.smax[i] >= 0) atomicAdd(&hist[plan_bin(P, i)], 1);   // smax < 0: not in this plan
+    __syncthreads();
+    for (int b = t; b < kPlanBins; b += 256)
+        if (hist[b]) atomicAdd(&P.hist[b], hist[b]);
 }
 
 // hist (kWfaClasses * kPlanKeys counts) -> exclusive offsets; cls[2c] = first
@@ -647,12 +666,24 @@ __global__ __launch_bounds__(1024) void plan_small_kernel(WfaDevPlan P)
     }
 }
 
+// The same chunks again: each workgroup reserves one range per non-zero bin
+// with a global atomic, then places its pairs in it through LDS counters.
 __global__ __launch_bounds__(256) void plan_scatter_kernel(WfaDevPlan P)
 {
-    const int i = int(blockIdx.x * 256 + threadIdx.x);
-    if (i >= P.n || P.smax[i] < 0) return;
-    const int pos = atomicAdd(&P.hist[plan_class(P.smax[i]) * kPlanKeys + plan_key(P.plen[i], P.tlen[i])], 1);
-    P.order[pos] = i;
+    __shared__ int hist[kPlanBins];
+    const int t = threadIdx.x;
+    for (int b = t; b < kPlanBins; b += 256) hist[b] = 0;
+    __syncthreads();
+    const int i0 = int(blockIdx.x) * kPlanChunk;
+    const int i1 = i0 + kPlanChunk < P.n ? i0 + kPlanChunk : P.n;
+    for (int i = i0 + t; i < i1; i += 256)
+        if (P.smax[i] >= 0) atomicAdd(&hist[plan_bin(P, i)], 1);
+    __syncthreads();
+    for (int b = t; b < kPlanBins; b += 256)
+        if (hist[b]) hist[b] = atomicAdd(&P.hist[b], hist[b]);
+    __syncthreads();
+    for (int i = i0 + t; i < i1; i += 256)
+        if (P.smax[i] >= 0) P.order[atomicAdd(&hist[plan_bin(P, i)], 1)] = i;
 }
 
 }  // namespace
@@ -800,7 +831,7 @@ hipError_t wfa_plan_launch(const WfaDevPlan& P, hipStream_t st)
         hipLaunchKernelGGL(plan_small_kernel, dim3(1), dim3(1024), 0, st, P);
         return hipGetLastError();
     }
-    const unsigned nb = unsigned((P.n + 255) / 256);
+    const unsigned nb = unsigned((P.n + kPlanChunk - 1) / kPlanChunk);
     hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, st, P.next, kWfaClasses);
     hipLaunchKernelGGL(zero_i32_kernel, dim3(16), dim3(256), 0, st, P.hist, kWfaClasses * kPlanKeys);
     hipLaunchKernelGGL(plan_hist_kernel, dim3(nb), dim3(256), 0, st, P);

@@ -101,6 +101,16 @@ class Mlp:
                err, "ga_mlp_forward_device")
         return self.logits[:n]
 
+    def logits_only(self, feat: torch.Tensor, stream=None) -> torch.Tensor:
+        """Inference: the logits by the fused kernel (no activations kept)."""
+        n = feat.shape[0]
+        out = torch.empty((n, BUCKETS), dtype=torch.float32, device=self.dev)
+        err = C.c_char_p()
+        _check(N.lib().ga_mlp_logits_device(_p(self.params), _p(feat), n, _p(out),
+                                            C.c_void_p(_stream_handle(stream)), C.byref(err)),
+               err, "ga_mlp_logits_device")
+        return out
+
     def backward(self, feat: torch.Tensor, label: torch.Tensor, stream=None) -> torch.Tensor:
         """Gradients of the mean cross-entropy of the last forward() (same
         rows) into self.grads; returns the per-sample loss."""

@@ -70,7 +70,7 @@ class LibraryBackend:
     the shipped router network into the library (genoaligner_amd.router.load),
     so WFA batches take the routed path; results are unchanged."""
 
-    def __init__(self, matrix=None, route: bool = False, q: float = 0.95):
+    def __init__(self, matrix=None, route: bool = False, q: float = 0.99):
         self.matrix = matrix if matrix is not None else api.blosum62(11, 1)
         if route:
             from . import router

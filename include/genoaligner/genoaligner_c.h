@@ -252,6 +252,13 @@ int ga_sketch_device(const ga_wfa_dev_batch* b, float* feat, int32_t* lower_boun
 /* h1, h2: n x 128 activations kept for the backward pass; logits n x 16 */
 int ga_mlp_forward_device(const float* params, const float* feat, int64_t n, float* h1, float* h2,
                           float* logits, void* stream, const char** error);
+/* Inference only: logits (n x 16) with the three layers fused and no
+ * activations kept (the routed path's forward; sums by fused multiply-adds, so
+ * the logits may differ from ga_mlp_forward_device's in the last bits).  No
+ * counterpart in the reference: the learned router has none.  params must be
+ * 16-byte aligned. */
+int ga_mlp_logits_device(const float* params, const float* feat, int64_t n, float* logits, void* stream,
+                         const char** error);
 /* mean softmax cross-entropy over the n samples: grads (GA_MLP_PARAMS) and the
  * per-sample loss; work holds n * (16 + 128 + 128) floats */
 int ga_mlp_backward_device(const float* params, const float* feat, const float* h1, const float* h2,

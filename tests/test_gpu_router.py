@@ -62,6 +62,10 @@ def test_mlp_kernels_match_oracle(gpu, oracle):
     h1, h2, elg = oracle.mlp_forward(params, x)
     assert np.allclose(mlp.h1[:300].cpu().numpy(), h1, rtol=1e-6, atol=1e-6)
     assert np.allclose(lg, elg, rtol=1e-6, atol=1e-6)
+    # the fused inference forward (fused multiply-adds: within 1e-5 of the
+    # oracle's separate multiply and add; 300 rows leave a partial tile)
+    fl = mlp.logits_only(xd).cpu().numpy()
+    assert np.allclose(fl, elg, rtol=1e-5, atol=1e-5)
     loss = mlp.backward(xd, yd).cpu().numpy()
     g, eloss = oracle.mlp_backward(params, x, y)
     assert np.allclose(loss, eloss, rtol=1e-5, atol=1e-6)
