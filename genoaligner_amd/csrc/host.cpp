@@ -258,6 +258,13 @@ struct DevArena {
             ok = false;
         return d;
     }
+    // bytes from host memory (pinned, so the copy stays asynchronous)
+    uint8_t* upload_raw(const uint8_t* src, size_t bytes)
+    {
+        uint8_t* d = alloc<uint8_t>(bytes);
+        if (d && bytes > 0 && hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, st) != hipSuccess) ok = false;
+        return d;
+    }
     void release()
     {
         for (void* p : ptrs) (void)ws_free(p, st);
@@ -275,6 +282,12 @@ struct Packed {
     // map[byte] for every byte of the buffer, padding included.
     template <class Get> void build(int64_t n, Get get, const uint8_t* map = nullptr,
                                     std::vector<uint8_t>* codes = nullptr);
+    // The same layout into caller memory: layout() sets the offsets and
+    // returns the buffer size, fill() writes the pads and the sequences (and,
+    // with map, the codes into code_dst).
+    template <class Get> int64_t layout(int64_t n, Get get);
+    template <class Get> void fill(int64_t n, Get get, uint8_t* dst, int64_t size, const uint8_t* map = nullptr,
+                                   uint8_t* code_dst = nullptr) const;
 };
 
 // A process-wide pool of host worker threads (half the host threads, at most
@@ -403,6 +416,98 @@ void Packed::build(int64_t n, Get get, const uint8_t* map, std::vector<uint8_t>*
         }
     }, 16);
 }
+
+template <class Get>
+int64_t Packed::layout(int64_t n, Get get)
+{
+    off.resize(size_t(n));
+    int64_t pos = kSeqPad;
+    for (int64_t k = 0; k < n; ++k) {
+        off[size_t(k)] = pos;
+        const int len = get(k).second;
+        pos += len > 0 ? len : 0;
+    }
+    return pos + kSeqPad;
+}
+
+template <class Get>
+void Packed::fill(int64_t n, Get get, uint8_t* dst, int64_t size, const uint8_t* map, uint8_t* code_dst) const
+{
+    // sequences sit back to back: only the head and tail pads need clearing
+    std::memset(dst, 0, size_t(kSeqPad));
+    std::memset(dst + size - kSeqPad, 0, size_t(kSeqPad));
+    if (code_dst) {
+        std::memset(code_dst, map[0], size_t(kSeqPad));
+        std::memset(code_dst + size - kSeqPad, map[0], size_t(kSeqPad));
+    }
+    parallel_for(n, [&](int64_t k) {
+        const auto sq = get(k);
+        if (sq.second <= 0) return;
+        uint8_t* d = dst + off[size_t(k)];
+        std::memcpy(d, sq.first, size_t(sq.second));
+        if (code_dst) {
+            uint8_t* c = code_dst + off[size_t(k)];
+            for (int x = 0; x < sq.second; ++x) c[x] = map[d[x]];
+        }
+    }, 16);
+}
+
+// ---------------------------------------------------------------------------
+// Pinned host staging for the host entry points' copies.  The runtime copies
+// pageable memory through small bounce buffers, and a device-to-host copy
+// into pageable memory also waits for the stream; from pinned memory
+// (hipHostMalloc) copies run at DMA speed and stay asynchronous.  Blocks are
+// cached process-wide (best fit, at most twice the request) and kept.
+// ---------------------------------------------------------------------------
+struct PinnedBlock {
+    uint8_t* p = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_pin_mu;
+std::vector<PinnedBlock> g_pin_idle;
+
+// Blocks held for one call; given back when it ends.  get() returns nullptr
+// when pinned memory cannot be had (callers then copy from pageable memory).
+struct PinnedHold {
+    std::vector<PinnedBlock> held;
+    PinnedHold() = default;
+    PinnedHold(const PinnedHold&) = delete;              // the blocks go back once
+    PinnedHold& operator=(const PinnedHold&) = delete;
+    uint8_t* get(size_t bytes)
+    {
+        bytes = std::max<size_t>(bytes, 64);
+        {
+            std::lock_guard<std::mutex> lk(g_pin_mu);
+            size_t best = g_pin_idle.size();
+            for (size_t i = 0; i < g_pin_idle.size(); ++i) {
+                const PinnedBlock& b = g_pin_idle[i];
+                if (b.bytes >= bytes && b.bytes / 2 <= bytes &&
+                    (best == g_pin_idle.size() || b.bytes < g_pin_idle[best].bytes))
+                    best = i;
+            }
+            if (best < g_pin_idle.size()) {
+                held.push_back(g_pin_idle[best]);
+                g_pin_idle.erase(g_pin_idle.begin() + std::ptrdiff_t(best));
+                return held.back().p;
+            }
+        }
+        PinnedBlock b;
+        b.bytes = (bytes + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1);
+        void* p = nullptr;
+        if (hipHostMalloc(&p, b.bytes, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        b.p = static_cast<uint8_t*>(p);
+        held.push_back(b);
+        return b.p;
+    }
+    ~PinnedHold()
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        for (const PinnedBlock& b : held) g_pin_idle.push_back(b);
+    }
+};
 
 // ---------------------------------------------------------------------------
 // optional kernel timing (per calling thread)
@@ -828,6 +933,8 @@ const char* enqueue_wfa_dev(const WfaDevClasses& C, hipStream_t st, const std::v
 struct WfaOut {
     std::vector<int32_t> score, cig_len;
     std::vector<int64_t> slot_off;   // slot of pair i in the arena
+    const char* cig_src = nullptr;   // where the CIGAR slots came back (pinned staging or the arena)
+    PinnedHold pin;
 };
 
 // Runs every pair with its own smax; trace[i] == 0 means score only (no CIGAR
@@ -850,9 +957,21 @@ const char* run_wfa(const ga_align_req* reqs, int64_t n, const std::vector<int32
         slot += std::max(r.pattern_len, 0) + std::max(r.text_len, 0);
         any_trace |= trace[size_t(i)] != 0;
     }
+    // both sequence sets packed straight into pinned staging where it can be had
     Packed P, T;
-    P.build(n, [&](int64_t i) { return std::make_pair(reqs[i].pattern, reqs[i].pattern_len); });
-    T.build(n, [&](int64_t i) { return std::make_pair(reqs[i].text, reqs[i].text_len); });
+    const auto pget = [&](int64_t i) { return std::make_pair(reqs[i].pattern, reqs[i].pattern_len); };
+    const auto tget = [&](int64_t i) { return std::make_pair(reqs[i].text, reqs[i].text_len); };
+    const int64_t psz = P.layout(n, pget), tsz = T.layout(n, tget);
+    uint8_t* ppin = out.pin.get(size_t(psz));
+    uint8_t* tpin = ppin ? out.pin.get(size_t(tsz)) : nullptr;
+    const bool pinned = ppin && tpin;
+    if (pinned) {
+        P.fill(n, pget, ppin, psz);
+        T.fill(n, tget, tpin, tsz);
+    } else {
+        P.build(n, pget);
+        T.build(n, tget);
+    }
     const WfaPlan plan = plan_wfa(n, smax.data(), plen.data(), tlen.data());
 
     hipStream_t st = acquire_stream();
@@ -861,8 +980,10 @@ const char* run_wfa(const ga_align_req* reqs, int64_t n, const std::vector<int32
     D.st = st;
     const char* err = nullptr;
     ga::WfaQueueLaunch L{};
-    L.pat = D.upload(P.bytes); L.poff = D.upload(P.off); L.plen = D.upload(plen);
-    L.txt = D.upload(T.bytes); L.toff = D.upload(T.off); L.tlen = D.upload(tlen);
+    L.pat = pinned ? D.upload_raw(ppin, size_t(psz)) : D.upload(P.bytes);
+    L.poff = D.upload(P.off); L.plen = D.upload(plen);
+    L.txt = pinned ? D.upload_raw(tpin, size_t(tsz)) : D.upload(T.bytes);
+    L.toff = D.upload(T.off); L.tlen = D.upload(tlen);
     L.smax = D.upload(smax);
     L.wide = max_text > 32766;
     L.score = D.alloc<int32_t>(size_t(n));
@@ -892,14 +1013,23 @@ const char* run_wfa(const ga_align_req* reqs, int64_t n, const std::vector<int32
     } else if (!err) {
         err = enqueue_wfa(L, plan, any_trace, st, &t_timer);
     }
+    // scores, CIGAR lengths and slots back into pinned staging (pageable
+    // fallback: the vectors and the caller's arena)
     out.score.assign(static_cast<size_t>(n), -1);
     out.cig_len.assign(static_cast<size_t>(n), 0);
+    int32_t* sc_h = reinterpret_cast<int32_t*>(out.pin.get(size_t(n) * 8));
+    int32_t* cl_h = sc_h ? sc_h + n : nullptr;
+    char* cdst = arena;
+    if (any_trace && slot > 0)
+        if (uint8_t* c = out.pin.get(size_t(slot))) cdst = reinterpret_cast<char*>(c);
     if (!err) {
-        if (hipMemcpyAsync(out.score.data(), L.score, size_t(n) * 4, hipMemcpyDeviceToHost, st) != hipSuccess)
+        if (hipMemcpyAsync(sc_h ? sc_h : out.score.data(), L.score, size_t(n) * 4, hipMemcpyDeviceToHost, st) !=
+            hipSuccess)
             err = "hipMemcpy(results)";
         if (!err && any_trace) {
-            if (hipMemcpyAsync(out.cig_len.data(), L.cig_len, size_t(n) * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                (slot > 0 && hipMemcpyAsync(arena, L.cigar, size_t(slot), hipMemcpyDeviceToHost, st) != hipSuccess))
+            if (hipMemcpyAsync(cl_h ? cl_h : out.cig_len.data(), L.cig_len, size_t(n) * 4, hipMemcpyDeviceToHost,
+                               st) != hipSuccess ||
+                (slot > 0 && hipMemcpyAsync(cdst, L.cigar, size_t(slot), hipMemcpyDeviceToHost, st) != hipSuccess))
                 err = "hipMemcpy(results)";
         }
     }
@@ -907,6 +1037,11 @@ const char* run_wfa(const ga_align_req* reqs, int64_t n, const std::vector<int32
     const hipError_t se = hipStreamSynchronize(st);
     if (!err && se != hipSuccess) err = hipGetErrorString(se);
     release_stream(st);
+    out.cig_src = cdst;
+    if (!err && sc_h) {
+        std::copy(sc_h, sc_h + n, out.score.begin());
+        if (any_trace) std::copy(cl_h, cl_h + n, out.cig_len.begin());
+    }
     return err;
 }
 
@@ -977,6 +1112,7 @@ int wfa_common(const ga_align_req* reqs, int64_t n, ga_align_res* res, char* are
         o.cigar_off = out.slot_off[size_t(i)] + slot_len - len;
         if (o.resolved && trace[size_t(i)] && len > 0) {
             o.cigar_len = len;
+            if (out.cig_src != arena) std::memcpy(arena + o.cigar_off, out.cig_src + o.cigar_off, size_t(len));
             wfa_validate(arena + o.cigar_off, len, o.score, r.pattern, r.pattern_len, r.text,
                          r.text_len, &o.rescore_ok, &o.wellformed_ok);
         } else {
@@ -1101,11 +1237,14 @@ hipError_t enqueue_sw16_chunk(const ga::Sw16Launch& P16, bool trace, hipStream_t
 
 // Device SW over pairs given as pointers; res8 receives 8 ints per pair (see
 // ga::SwLaunch).  walk[i] selects pairs whose span + CIGAR are produced.
+// The CIGAR slots come back into pinned staging from `pin` (or, without
+// pinned memory, into arena): *cig_src tells the caller where they are.
 const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>& skip,
                    bool trace, const std::vector<uint8_t>& walk, char* arena,
                    std::vector<int32_t>& res8, std::vector<int64_t>& slot_off,
-                   const SwMatrixHost* mat = nullptr)
+                   PinnedHold& pin, const char** cig_src, const SwMatrixHost* mat = nullptr)
 {
+    *cig_src = arena;
     std::call_once(g_pool_once, init_pool);
     t_timer.reset();
     // Order on the device: pairs for the int32 kernel first (texts longer than
@@ -1181,23 +1320,34 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
         if (k < n32) max_m32 = std::max(max_m32, tl[size_t(k)]);
     }
     // the sequences in launch order, and (matrix scoring) their residue codes,
-    // laid out like the bytes (the padding maps to the code of byte 0)
+    // laid out like the bytes (the padding maps to the code of byte 0); packed
+    // straight into pinned staging where it can be had
     Packed T, P;
     std::vector<uint8_t> tcode, pcode;
-    T.build(n, [&](int64_t k) { return std::make_pair(reqs[order[size_t(k)]].text, tl[size_t(k)]); },
-            mat ? mat->code : nullptr, mat ? &tcode : nullptr);
-    P.build(n, [&](int64_t k) { return std::make_pair(reqs[order[size_t(k)]].pattern, pl[size_t(k)]); },
-            mat ? mat->code : nullptr, mat ? &pcode : nullptr);
+    const auto tget = [&](int64_t k) { return std::make_pair(reqs[order[size_t(k)]].text, tl[size_t(k)]); };
+    const auto pget = [&](int64_t k) { return std::make_pair(reqs[order[size_t(k)]].pattern, pl[size_t(k)]); };
+    const int64_t tsz = T.layout(n, tget), psz = P.layout(n, pget);
+    const int cm = mat ? 2 : 1;                      // bytes (and codes)
+    uint8_t* tpin = pin.get(size_t(tsz) * cm);
+    uint8_t* ppin = tpin ? pin.get(size_t(psz) * cm) : nullptr;
+    if (tpin && ppin) {
+        T.fill(n, tget, tpin, tsz, mat ? mat->code : nullptr, mat ? tpin + tsz : nullptr);
+        P.fill(n, pget, ppin, psz, mat ? mat->code : nullptr, mat ? ppin + psz : nullptr);
+    } else {
+        T.build(n, tget, mat ? mat->code : nullptr, mat ? &tcode : nullptr);
+        P.build(n, pget, mat ? mat->code : nullptr, mat ? &pcode : nullptr);
+    }
+    const bool pinned = tpin && ppin;
 
     hipStream_t st = acquire_stream();
     if (!st) return "hipStreamCreate";
     DevArena D;
     D.st = st;
     const char* err = nullptr;
-    auto* d_txt = D.upload(T.bytes);
-    auto* d_pat = D.upload(P.bytes);
-    const uint8_t* d_tcode = mat ? D.upload(tcode) : nullptr;
-    const uint8_t* d_pcode = mat ? D.upload(pcode) : nullptr;
+    auto* d_txt = pinned ? D.upload_raw(tpin, size_t(tsz)) : D.upload(T.bytes);
+    auto* d_pat = pinned ? D.upload_raw(ppin, size_t(psz)) : D.upload(P.bytes);
+    const uint8_t* d_tcode = !mat ? nullptr : pinned ? D.upload_raw(tpin + tsz, size_t(tsz)) : D.upload(tcode);
+    const uint8_t* d_pcode = !mat ? nullptr : pinned ? D.upload_raw(ppin + psz, size_t(psz)) : D.upload(pcode);
     std::vector<int32_t> subv;
     if (mat) subv.assign(mat->sub, mat->sub + 32 * 32);
     const int32_t* d_sub = mat ? D.upload(subv) : nullptr;
@@ -1331,20 +1481,29 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
         }
     }
 
-    std::vector<int32_t> r8p(static_cast<size_t>(n) * 8, -1);
+    // results and CIGAR slots back into pinned staging (pageable fallback:
+    // the caller's arena and a vector)
+    std::vector<int32_t> r8v;
+    int32_t* r8p = reinterpret_cast<int32_t*>(pin.get(size_t(n) * 32));
+    if (!r8p) { r8v.assign(static_cast<size_t>(n) * 8, -1); r8p = r8v.data(); }
+    char* cdst = arena;
+    if (trace && slot > 0)
+        if (uint8_t* c = pin.get(size_t(slot))) cdst = reinterpret_cast<char*>(c);
     if (!err) {
-        if (hipMemcpyAsync(r8p.data(), d_res, size_t(n) * 32, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        if (hipMemcpyAsync(r8p, d_res, size_t(n) * 32, hipMemcpyDeviceToHost, st) != hipSuccess ||
             (trace && slot > 0 &&
-             hipMemcpyAsync(arena, d_cig, size_t(slot), hipMemcpyDeviceToHost, st) != hipSuccess))
+             hipMemcpyAsync(cdst, d_cig, size_t(slot), hipMemcpyDeviceToHost, st) != hipSuccess))
             err = "hipMemcpy(results)";
     }
     D.release();
     const hipError_t se = hipStreamSynchronize(st);
     if (!err && se != hipSuccess) err = hipGetErrorString(se);
     release_stream(st);
-    res8.assign(static_cast<size_t>(n) * 8, -1);
-    for (int64_t k = 0; k < n; ++k)
-        std::copy(r8p.begin() + k * 8, r8p.begin() + k * 8 + 8, res8.begin() + order[size_t(k)] * 8);
+    *cig_src = cdst;
+    res8.resize(static_cast<size_t>(n) * 8);
+    parallel_for(n, [&](int64_t k) {
+        std::copy(r8p + k * 8, r8p + k * 8 + 8, res8.begin() + order[size_t(k)] * 8);
+    });
     return err;
 }
 
@@ -1498,7 +1657,9 @@ static int sw_common(const ga_sw_req* reqs, int64_t n, ga_sw_res* res, char* are
     }
     std::vector<int32_t> r8;
     std::vector<int64_t> slot_off;
-    const char* err = run_sw(reqs, n, skip, any_cigar, walk, arena, r8, slot_off, mat);
+    PinnedHold pin;
+    const char* cig_src = arena;
+    const char* err = run_sw(reqs, n, skip, any_cigar, walk, arena, r8, slot_off, pin, &cig_src, mat);
     if (err) {
         if (error) *error = err;
         reset_sw(res, n);
@@ -1527,6 +1688,7 @@ static int sw_common(const ga_sw_req* reqs, int64_t n, ga_sw_res* res, char* are
             const int slot_len = std::max(r.text_len, 0) + std::max(r.pattern_len, 0);
             o.cigar_off = slot_off[size_t(i)] + slot_len - len;
             o.cigar_len = len;
+            if (cig_src != arena) std::memcpy(arena + o.cigar_off, cig_src + o.cigar_off, size_t(len));
             if (mat)
                 sw_validate_matrix(arena + o.cigar_off, len, r.text, r.text_len, o.start_i, o.end_i,
                                    r.pattern, r.pattern_len, o.start_j, o.end_j, *mat, o.score,
