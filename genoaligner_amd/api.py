@@ -175,6 +175,97 @@ def _ptr(a: np.ndarray, cstruct):
     return a.ctypes.data_as(C.POINTER(cstruct))
 
 
+def _pack_rows(reqs, wfa: bool):
+    """C request rows of `reqs` (ga_align_req when wfa, else ga_sw_req) and
+    the buffers their sequence pointers address: (rows, keep)."""
+    keep: list = []
+    n = len(reqs)
+    if wfa:
+        arr = np.zeros(n, dtype=_ALIGN_REQ)
+        plen = [int(r.pattern_len) for r in reqs]
+        tlen = [int(r.text_len) for r in reqs]
+        arr["pattern"] = _seq_table([r.pattern for r in reqs], plen, keep)
+        arr["pattern_len"] = plen
+        arr["text"] = _seq_table([r.text for r in reqs], tlen, keep)
+        arr["text_len"] = tlen
+        arr["smax"] = [int(r.smax) for r in reqs]
+    else:
+        arr = np.zeros(n, dtype=_SW_REQ)
+        tlen = [int(r.text_len) for r in reqs]
+        plen = [int(r.pattern_len) for r in reqs]
+        arr["text"] = _seq_table([r.text for r in reqs], tlen, keep)
+        arr["text_len"] = tlen
+        arr["pattern"] = _seq_table([r.pattern for r in reqs], plen, keep)
+        arr["pattern_len"] = plen
+        arr["scoring"] = [(s.match, s.mismatch, s.gap_open, s.gap_extend) if isinstance(s, SWScoring)
+                          else tuple(int(x) for x in s) for s in (r.scoring for r in reqs)]
+    arr["with_cigar"] = [1 if r.with_cigar else 0 for r in reqs]
+    return arr, keep
+
+
+def _call_wfa_rows(arr: np.ndarray, ext: bool):
+    """ga_align_batch(_ext) over packed rows: (result rows, CIGAR arena as a
+    uint8 array, error or None)."""
+    n = len(arr)
+    res = np.zeros(n, dtype=_ALIGN_RES)
+    L = N.lib()
+    pa = _ptr(arr, N.AlignReq)
+    need = L.ga_cigar_arena_bytes_wfa(pa, n)
+    arena = np.empty(max(need, 1), dtype=np.uint8)
+    rc_ok, rc_un, err = C.c_int32(), C.c_int32(), C.c_char_p()
+    fn = L.ga_align_batch_ext if ext else L.ga_align_batch
+    rc = fn(pa, n, _ptr(res, N.AlignRes), arena.ctypes.data_as(C.c_char_p), need, C.byref(rc_ok),
+            C.byref(rc_un), C.byref(err))
+    if rc != N.GA_OK:
+        return res, arena, (err.value.decode() if err.value else f"status {rc}")
+    return res, arena, None
+
+
+def _call_sw_rows(arr: np.ndarray, matrix=None):
+    """ga_align_sw_batch (or the matrix entry point) over packed rows."""
+    n = len(arr)
+    res = np.zeros(n, dtype=_SW_RES)
+    L = N.lib()
+    pa = _ptr(arr, N.SWReq)
+    pr = _ptr(res, N.SWRes)
+    need = L.ga_cigar_arena_bytes_sw(pa, n)
+    arena = np.empty(max(need, 1), dtype=np.uint8)
+    ap = arena.ctypes.data_as(C.c_char_p)
+    rc_ok, rc_un, err = C.c_int32(), C.c_int32(), C.c_char_p()
+    if matrix is None:
+        rc = L.ga_align_sw_batch(pa, n, pr, ap, need, C.byref(rc_ok), C.byref(rc_un), C.byref(err))
+    else:
+        rc = L.ga_align_sw_matrix_batch(pa, n, C.byref(matrix), pr, ap, need, C.byref(rc_ok),
+                                        C.byref(rc_un), C.byref(err))
+    if rc != N.GA_OK:
+        return res, arena, (err.value.decode() if err.value else f"status {rc}")
+    return res, arena, None
+
+
+def _align_results(res: np.ndarray, arena: np.ndarray) -> List[AlignResult]:
+    raw = arena
+    sc = res["score"].tolist()
+    rv = res["resolved"].tolist()
+    co = res["cigar_off"].tolist()
+    cl = res["cigar_len"].tolist()
+    rs = res["rescore_ok"].tolist()
+    wf = res["wellformed_ok"].tolist()
+    return [AlignResult(sc[i], bool(rv[i]),
+                        raw[co[i]: co[i] + cl[i]].tobytes().decode("latin-1") if cl[i] > 0 else "",
+                        bool(rs[i]), bool(wf[i])) for i in range(len(res))]
+
+
+def _sw_results(res: np.ndarray, arena: np.ndarray) -> List["SWAlignResult"]:
+    f = {k: res[k].tolist() for k in ("score", "resolved", "cigar_off", "cigar_len", "start_i", "start_j",
+                                       "end_i", "end_j", "rescore_ok", "wellformed_ok", "too_large")}
+    return [SWAlignResult(
+        f["score"][i], bool(f["resolved"][i]),
+        arena[f["cigar_off"][i]: f["cigar_off"][i] + f["cigar_len"][i]].tobytes().decode("latin-1")
+        if f["cigar_len"][i] > 0 else "",
+        f["start_i"][i], f["start_j"][i], f["end_i"][i], f["end_j"][i], bool(f["rescore_ok"][i]),
+        bool(f["wellformed_ok"][i]), bool(f["too_large"][i])) for i in range(len(res))]
+
+
 def _wfa(reqs: Sequence[AlignRequest], ext: bool) -> BatchResult:
     n = len(reqs)
     out = BatchResult(results=[])

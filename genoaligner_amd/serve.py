@@ -17,16 +17,26 @@ Results never depend on the mix: WFA batches use the per-pair entry point
 (ga_align_batch_ext: each pair keeps its own smax and CIGAR flag), and SW
 batches only join requests with the same CIGAR flag (the reference's SW rules
 make a score-only pair's reported end cell depend on whether its batch traces).
+
+Marshalling is per request, not per batch: at submission a request's pairs are
+packed once (its sequences into one buffer, its C request rows pointing into
+it); a slot joins the batch's rows and calls the C entry point directly, and
+each request gets its slice of the result rows and the batch's CIGAR arena.
+Result objects are built only when `Request.results` is read.
 """
 from __future__ import annotations
 
 import collections
+import ctypes as C
 import threading
 import time
 from dataclasses import dataclass, field
 from typing import Callable, List, Optional, Sequence
 
+import numpy as np
+
 from . import api
+from . import _native as N
 
 
 @dataclass
@@ -37,9 +47,24 @@ class Request:
     arrival: float = 0.0            # perf_counter() at submission
     admitted: float = 0.0
     done_at: float = 0.0
-    results: Optional[list] = None
     error: Optional[str] = None
+    rows: Optional[np.ndarray] = None    # packed C result rows (ga_align_res / ga_sw_res) of the items
+    arena: Optional[np.ndarray] = None   # the CIGAR bytes their cigar_off / cigar_len address
+    _results: Optional[list] = field(default=None, repr=False)
+    _pack: Optional[tuple] = field(default=None, repr=False)
     _done: threading.Event = field(default_factory=threading.Event, repr=False)
+
+    @property
+    def results(self) -> Optional[list]:
+        """Per-item result objects (built from the packed rows on first use)."""
+        if self._results is None and self.rows is not None:
+            self._results = (api._align_results if self.kind == "wfa" else api._sw_results)(
+                self.rows, self.arena)
+        return self._results
+
+    @results.setter
+    def results(self, value):
+        self._results = value
 
     @property
     def latency_ms(self) -> float:
@@ -62,6 +87,20 @@ class Request:
             b = max(int(r.text_len), 0)
             tot += 3 * (a + b) + 96
         return tot
+
+
+def join_rows(parts: Sequence[np.ndarray]) -> np.ndarray:
+    """The C request rows of several requests in one array of the same dtype
+    (np.concatenate would drop the explicit field offsets and padding that
+    match the C struct)."""
+    if len(parts) == 1:
+        return parts[0]
+    rows = np.zeros(sum(len(p) for p in parts), dtype=parts[0].dtype)
+    at = 0
+    for p in parts:
+        rows[at: at + len(p)] = p
+        at += len(p)
+    return rows
 
 
 class LibraryBackend:
@@ -93,6 +132,22 @@ class LibraryBackend:
         if not b.ok():
             raise RuntimeError(b.error or "batch failed")
         return b.results
+
+    def pack(self, kind, items):
+        """The request's C rows and the buffers they point into (built once,
+        at submission)."""
+        return api._pack_rows(items, kind == "wfa")
+
+    def run_packed(self, key, packs):
+        """One batch from packed requests: (result rows, CIGAR arena)."""
+        rows = join_rows([p[0] for p in packs])
+        if key[0] == "wfa":
+            res, arena, err = api._call_wfa_rows(rows, True)
+        else:
+            res, arena, err = api._call_sw_rows(rows, self.matrix)
+        if err is not None:
+            raise RuntimeError(err)
+        return res, arena
 
 
 class Server:
@@ -126,6 +181,9 @@ class Server:
     # ------------------------------------------------------------ client side
     def submit(self, kind: str, items: Sequence, with_cigar: bool = True) -> Request:
         req = Request(kind=kind, items=list(items), with_cigar=with_cigar, arrival=self.clock())
+        pack = getattr(self.backend, "pack", None)
+        if pack is not None and req.items:
+            req._pack = pack(kind, req.items)
         refuse = None
         if kind == "sw" and any(bool(it.with_cigar) != bool(with_cigar) for it in req.items):
             # SW batches are keyed by the flag, and a score-only pair's reported
@@ -203,15 +261,22 @@ class Server:
                 r.admitted = t0
             items = [it for r in batch for it in r.items]
             err = None
+            packed = all(r._pack is not None for r in batch)
             try:
-                res = self.backend.run(key, items)
+                if packed:
+                    res, arena = self.backend.run_packed(key, [r._pack for r in batch])
+                else:
+                    res, arena = self.backend.run(key, items), None
             except Exception as e:                      # reported per request
-                res, err = None, str(e)
+                res, arena, err = None, None, str(e)
             t1 = self.clock()
             at = 0
             for r in batch:
                 if res is not None:
-                    r.results = res[at: at + r.pairs()]
+                    if packed:
+                        r.rows, r.arena = res[at: at + r.pairs()], arena
+                    else:
+                        r.results = res[at: at + r.pairs()]
                 r.error = err
                 at += r.pairs()
                 r.done_at = t1

@@ -163,3 +163,25 @@ def test_sw_request_with_disagreeing_item_flags_is_refused():
     assert ok.wait(5) and ok.error is None
     s.close()
     assert [tags for _, tags in fb.calls] == [["c"]]
+
+
+def test_joined_request_rows_keep_the_c_layout():
+    """A batch's packed rows (several requests joined) keep the C structs'
+    field offsets and size, and every row keeps its request's values."""
+    import ctypes as C
+    import numpy as np
+    from genoaligner_amd import _native as N
+    from genoaligner_amd.serve import join_rows
+    for wfa, cs in ((True, N.AlignReq), (False, N.SWReq)):
+        parts = []
+        for k in range(3):
+            items = [(_wfa if wfa else _sw)(f"r{k}i{i}", length=10 + 7 * k + i) for i in range(k + 2)]
+            parts.append(api._pack_rows(items, wfa)[0])
+        rows = join_rows(parts)
+        assert rows.dtype == parts[0].dtype and rows.dtype.itemsize == C.sizeof(cs)
+        for name, _ in cs._fields_:
+            assert rows.dtype.fields[name][1] == getattr(cs, name).offset, name
+        at = 0
+        for p in parts:
+            assert rows[at: at + len(p)].tobytes() == p.tobytes()
+            at += len(p)
