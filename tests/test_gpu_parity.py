@@ -239,6 +239,60 @@ def test_sw_long_gaps_all_trace_layouts(gpu, oracle):
                    (exp.score, exp.cigar, exp.start_i, exp.start_j, exp.end_i, exp.end_j), (sc, t, p)
 
 
+def test_sw_run_walk_ties_and_borders(gpu, oracle):
+    """The run walk over the H mod 8 record on the inputs where a traceback's
+    tie rules decide the most: homopolymers, dinucleotide and tandem repeats
+    with indels, alignments that start at row or column 1, pairs of 1..40
+    bases, widths at the walk window's and the strip's seams (31..33, 63..65,
+    127..129 columns) and texts up to 2,500 rows; under schemes at the
+    record's limits (match - mismatch = 7, gap_open - gap_extend = 7), a
+    positive mismatch, and gap_open == gap_extend."""
+    import genoaligner_amd as ga
+    rng = random.Random(53)
+    schemes = ((2, -3, 3, 1), (1, -6, 8, 1), (4, -3, 9, 2), (3, 1, 2, 1), (5, -2, 5, 5), (1, -1, 1, 1))
+
+    def rep(unit, n):
+        return (unit * (n // len(unit) + 1))[:n]
+
+    pairs = []
+    for _ in range(12):                                  # homopolymers, repeats with indels
+        for unit in ("A", "AC", "ACG", "AAT", "GATTACA"):
+            t = rep(unit, rng.randint(20, 400))
+            pairs.append((t, _mutate(rng, t[rng.randint(0, len(t) // 3):], 0.08)))
+    for _ in range(40):                                  # alignments touching row / column 1
+        t = "".join(rng.choice("ACGT") for _ in range(rng.randint(5, 300)))
+        q = _mutate(rng, t[: rng.randint(1, len(t))], 0.1)
+        pairs.append((t, q) if rng.random() < 0.5 else (q, t))
+    for _ in range(80):                                  # tiny pairs
+        pairs.append(("".join(rng.choice("ACGT") for _ in range(rng.randint(1, 40))),
+                      "".join(rng.choice("ACGT") for _ in range(rng.randint(1, 40)))))
+    for w in (31, 32, 33, 63, 64, 65, 127, 128, 129):    # window and strip seams
+        t = "".join(rng.choice("ACGT") for _ in range(rng.randint(w, 3 * w)))
+        a = rng.randint(0, len(t) - w)
+        pairs.append((t, _mutate(rng, t[a: a + w], 0.06)[:w]))
+    for _ in range(6):                                   # long texts, indel blocks
+        t = "".join(rng.choice("ACGT") for _ in range(rng.randint(1500, 2500)))
+        a = rng.randint(0, len(t) - 600)
+        q = list(t[a: a + rng.randint(300, 600)])
+        for _ in range(3):
+            at = rng.randint(0, len(q))
+            if rng.random() < 0.5:
+                q[at:at] = [rng.choice("ACGT") for _ in range(rng.randint(2, 30))]
+            else:
+                del q[at: at + rng.randint(2, 30)]
+        pairs.append((t, _mutate(rng, "".join(q), 0.05)))
+    for sc in schemes:
+        reqs = [_sreq(ga, t, p, sc, True) for t, p in pairs]
+        b = ga.align_sw_batch(reqs)
+        want = oracle.align_sw_batch(reqs)
+        assert b.ok(), b.error
+        for got, exp, (t, p) in zip(b.results, want.results, pairs):
+            assert (got.score, got.cigar, got.start_i, got.start_j, got.end_i, got.end_j, got.rescore_ok,
+                    got.wellformed_ok) == \
+                   (exp.score, exp.cigar, exp.start_i, exp.start_j, exp.end_i, exp.end_j, exp.rescore_ok,
+                    exp.wellformed_ok), (sc, t, p)
+
+
 def test_sw_long_text_global_boundary(gpu, oracle):
     """Texts longer than the LDS boundary buffer take the HBM boundary path."""
     import genoaligner_amd as ga
