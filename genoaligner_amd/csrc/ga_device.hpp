@@ -169,7 +169,11 @@ struct Sw16Launch {
     int32_t max_n;          //   and max pattern_len (trace: staging for the walk)
     int32_t match, mismatch, go, ge;
     int32_t* res;           // 8 ints per pair, as SwLaunch
-    uint8_t* rerun;         // out: 1 = needs the int32 kernel
+    uint8_t* rerun;         // out: 1 = needs the int32 kernel (2, inside a launch: the literal-byte pass)
+    // Optional (DNA launches): n_waves + 1 ints of scratch; with it, pairs holding
+    // a byte other than A/C/G/T get a packed literal-byte pass instead of the
+    // int32 kernel (schemes with mismatch < 0)
+    int32_t* bytes_list;
     uint32_t* dirs; int64_t dir_stride;             // words per wave (trace)
     uint32_t* selbuf;       // per-wave text selector rows (scratch), sw16_sel_stride dwords per wave
     const uint8_t* walk;    // per pair: emit span + CIGAR; null = all

@@ -1455,6 +1455,8 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
         int64_t max_cw = 0;
         for (auto& ch : wchunks) max_cw = std::max(max_cw, ch.second - ch.first);
         uint32_t* d_sel = D.alloc<uint32_t>(size_t(max_cw * ga::sw16_sel_stride(pk_max_m)));
+        // the DNA kernel's list of waves for its literal-byte pass
+        int32_t* d_blist = mat ? nullptr : D.alloc<int32_t>(size_t(max_cw + 1));
         if (!D.ok) err = "hipMalloc(dir table) -- batch workspace exceeds device memory; split the batch";
         for (auto& ch : wchunks) {
             if (err) break;
@@ -1472,6 +1474,7 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
             L.res = d_res; L.rerun = d_rerun;
             L.dirs = d_dirs16; L.dir_stride = trace ? dir_words(cm, cn) : 0;
             L.selbuf = d_sel;
+            L.bytes_list = d_blist;
             L.walk = d_walk; L.cigar = d_cig; L.cig_off = d_slot;
             if (mat) {
                 L.go = mat->gap_open; L.ge = mat->gap_extend;
@@ -1994,6 +1997,8 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)
         if (ok && trace) ok = dalloc(reinterpret_cast<void**>(&d_dirs), size_t(cw * stride) * 4);
         uint32_t* d_sel = nullptr;
         if (ok) ok = dalloc(reinterpret_cast<void**>(&d_sel), size_t(cw * ga::sw16_sel_stride(b->max_text_len)) * 4);
+        int32_t* d_blist = nullptr;                        // the literal-byte pass's wave list
+        if (ok) ok = dalloc(reinterpret_cast<void**>(&d_blist), size_t(cw + 1) * 4);
         int rc = GA_OK;
         if (!ok) { if (error) *error = "hipMalloc(SW workspace)"; rc = GA_DEVICE_ERROR; }
         t_timer.reset();
@@ -2007,6 +2012,7 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error)
             L.match = b->scoring.match; L.mismatch = b->scoring.mismatch;
             L.go = b->scoring.gap_open; L.ge = b->scoring.gap_extend;
             L.res = d_res; L.rerun = d_rerun; L.dirs = d_dirs; L.dir_stride = stride; L.selbuf = d_sel;
+            L.bytes_list = d_blist;
             L.walk = nullptr; L.cigar = b->cigar; L.cig_off = b->cigar_off;
             if (enqueue_sw16_chunk(L, trace, st, &t_timer_hooks) != hipSuccess) {
                 if (error) *error = "SW kernel launch";

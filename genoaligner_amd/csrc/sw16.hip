@@ -346,21 +346,32 @@ template <bool B> struct BoolC { static constexpr bool value = B; };
 constexpr BoolC<true> kTrue{};
 constexpr BoolC<false> kFalse{};
 
+// The pass over wave w's two pairs.
 // GK: gap_open < -mismatch, where H needs the extra bias term (sw16_gk).
 // LAY (trace only): the trace record layout (sw16_layout).
-template <bool TRACE, bool GK, int LAY>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
-void sw16_kernel(Sw16Launch L)
+// BYTES: the literal-byte pass: only the wave's pairs the first pass flagged
+// (rerun == 2) are run, the other half is left as it is; text rows stream as
+// raw bytes (0x100 past a pair's end, equal to no byte), each lane keeps its
+// raw pattern bytes, and u = match or mismatch comes from a packed compare
+// (three instructions where the A/C/G/T tables take one perm).  Rows past a
+// pair's end then score the mismatch instead of u = 0: the drained-cell
+// argument (s <= -1) needs mismatch < 0, which the launch requires.
+template <bool TRACE, bool GK, int LAY, bool BYTES>
+__device__ __forceinline__ void sw16_wave(const Sw16Launch& L, const int w)
 {
     constexpr int kDepth = 3;
     extern __shared__ __align__(16) uint32_t lds16[];
-    const int w = blockIdx.x;
     const int lane = threadIdx.x;
     const int ia = L.pair_a ? L.pair_a[w] : L.first_pair + 2 * w;
     int ib = L.pair_b ? L.pair_b[w] : L.first_pair + 2 * w + 1;
     if (ib >= L.n_total) ib = -1;
-    const int mA = L.tlen[ia], nA = L.plen[ia];
-    const int mB = ib >= 0 ? L.tlen[ib] : 0, nB = ib >= 0 ? L.plen[ib] : 0;
+    bool liveA = true, liveB = ib >= 0;
+    if (BYTES) {
+        liveA = L.rerun[ia] == 2;
+        liveB = ib >= 0 && L.rerun[ib] == 2;
+    }
+    const int mA = liveA ? L.tlen[ia] : 0, nA = liveA ? L.plen[ia] : 0;
+    const int mB = liveB ? L.tlen[ib] : 0, nB = liveB ? L.plen[ib] : 0;
     const int mAp = mA > 0 && nA > 0 ? mA : 0, mBp = mB > 0 && nB > 0 ? mB : 0;
     const int M = mAp > mBp ? mAp : mBp;
     const int N = nA > nB ? nA : nB;
@@ -382,22 +393,31 @@ void sw16_kernel(Sw16Launch L)
 
     bool badA = false, badB = false;
     for (int i = lane - 64; i < M + 80; i += 64) {
-        uint32_t sv = kSelNone;
+        uint32_t sv = BYTES ? 0x01000100u : kSelNone;
         if (i >= 0 && i < M) {
-            uint32_t lo = 12u, hi = 12u;
-            if (i < mAp) { const int c = dna_code(TA[i]); badA |= c < 0; lo = uint32_t(c < 0 ? 12 : c); }
-            if (i < mBp) { const int c = dna_code(TB[i]); badB |= c < 0; hi = uint32_t(c < 0 ? 12 : 4 + c); }
-            sv = lo | (12u << 8) | (hi << 16) | (12u << 24);
+            if (BYTES) {
+                sv = (i < mAp ? uint32_t(TA[i]) : 0x100u) | ((i < mBp ? uint32_t(TB[i]) : 0x100u) << 16);
+            } else {
+                uint32_t lo = 12u, hi = 12u;
+                if (i < mAp) { const int c = dna_code(TA[i]); badA |= c < 0; lo = uint32_t(c < 0 ? 12 : c); }
+                if (i < mBp) { const int c = dna_code(TB[i]); badB |= c < 0; hi = uint32_t(c < 0 ? 12 : 4 + c); }
+                sv = lo | (12u << 8) | (hi << 16) | (12u << 24);
+            }
         }
         sel[i] = sv;
     }
     for (int i = lane; i < Mr; i += 64) bnd[i] = make_uint2(0u, go2);
-    for (int j = lane; j < N; j += 64) {
-        if (j < nA && mAp > 0) badA |= dna_code(PA[j]) < 0;
-        if (j < nB && mBp > 0) badB |= dna_code(PB[j]) < 0;
+    if (!BYTES) {
+        for (int j = lane; j < N; j += 64) {
+            if (j < nA && mAp > 0) badA |= dna_code(PA[j]) < 0;
+            if (j < nB && mBp > 0) badB |= dna_code(PB[j]) < 0;
+        }
+        badA = __any(badA);
+        badB = __any(badB);
     }
-    badA = __any(badA);
-    badB = __any(badB);
+    // bytes: u = um + z (ux - um) per half, z = min(text byte ^ pattern byte, 1)
+    const uint32_t um2 = uint32_t(L.match + bias) * 0x10001u;
+    const uint32_t d2 = (uint32_t(L.mismatch - L.match) & 0xffffu) * 0x10001u;
     __syncthreads();
 
     const int nstrips = (N + 63) >> 6;
@@ -415,8 +435,14 @@ void sw16_kernel(Sw16Launch L)
     for (int s = 0; s < nstrips; ++s) {
         const int j = (s << 6) + lane;
         Strip S;
-        S.tabA = u_table(j < nA ? PA[j] : uint8_t(0), L.match, L.mismatch, bias);
-        S.tabB = u_table(j < nB ? PB[j] : uint8_t(0), L.match, L.mismatch, bias);
+        if (BYTES) {
+            // the lane's raw pattern bytes (0x200 past a pattern's end: a masked column)
+            S.tabA = (j < nA ? uint32_t(PA[j]) : 0x200u) | ((j < nB ? uint32_t(PB[j]) : 0x200u) << 16);
+            S.tabB = 0;
+        } else {
+            S.tabA = u_table(j < nA ? PA[j] : uint8_t(0), L.match, L.mismatch, bias);
+            S.tabB = u_table(j < nB ? PB[j] : uint8_t(0), L.match, L.mismatch, bias);
+        }
         S.colmask = (j < nA ? 0xffffu : 0u) | (j < nB ? 0xffff0000u : 0u);
         S.H = 0; S.Eh = go2; S.Fh = go2; S.diag = 0;
         S.bestH = thrm1; S.bestT = 0xffffffffu;     // record at the bound (see strip_group)
@@ -455,8 +481,16 @@ void sw16_kernel(Sw16Launch L)
         asm volatile("" : "+v"(boff));
         auto group = [&](auto fast, int t0, const uint32_t (&sv)[4], const uint2 (&bv)[4]) {
             uint32_t acc = 0, hout[4], eout[4];
-            // both pairs' u from the lane's tables by the row's v_perm selector
-            const auto dna_u = [&](int u) { return __builtin_amdgcn_perm(S.tabB, S.tabA, sv[u]); };
+            // both pairs' u: from the lane's tables by the row's v_perm selector,
+            // or (BYTES) by comparing the row's text bytes with the lane's
+            // pattern bytes
+            const auto dna_u = [&](int u) {
+                if (BYTES) {
+                    const uint32_t z = as_u(__builtin_elementwise_min(as_v(sv[u] ^ S.tabA), as_v(0x00010001u)));
+                    return as_u(as_v(z) * as_v(d2) + as_v(um2));
+                }
+                return __builtin_amdgcn_perm(S.tabB, S.tabA, sv[u]);
+            };
             if (decltype(fast)::value || t0 < t_fast)
                 strip_group<TRACE, LAY, false, GK>(S, t0, lane, dna_u, bv, b2, gk2, ge2, acc, hout, eout, thrm1);
             else
@@ -513,10 +547,14 @@ void sw16_kernel(Sw16Launch L)
     const int limit = 32767 - 255 - bias;
     int endI[2] = {-1, -1}, endJ[2] = {-1, -1}, endS[2] = {0, 0};
     bool redo[2] = {false, false};
+    // literal-byte pass for pairs holding another byte (the launch provides
+    // the list and a scheme with mismatch < 0); else the int32 kernel
+    const bool to_bytes = !BYTES && L.bytes_list != nullptr && L.mismatch < 0;
+    bool listed = false;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int pid = h == 0 ? ia : ib;
-        if (pid < 0) continue;
+        if (pid < 0 || !(h == 0 ? liveA : liveB)) continue;
         int lH = gH[h], lI = gI[h], lJ = gJ[h];
         int rH = gH[h], rI = gI[h], rJ = gJ[h];
         for (int off = 32; off > 0; off >>= 1) {
@@ -528,7 +566,10 @@ void sw16_kernel(Sw16Launch L)
             }
         }
         endI[h] = rI; endJ[h] = rJ; endS[h] = lH;
-        redo[h] = (h == 0 ? badA : badB) || lH >= limit;
+        const bool bad = h == 0 ? badA : badB;
+        redo[h] = bad || lH >= limit;
+        const uint8_t flag = !redo[h] ? 0 : (bad && to_bytes) ? 2 : 1;
+        listed |= flag == 2;
         if (lane == 0) {
             int32_t* res = L.res + int64_t(pid) * 8;
             const bool rerun = redo[h];
@@ -542,10 +583,11 @@ void sw16_kernel(Sw16Launch L)
                 res[0] = rerun ? kSwRerun : lH;
                 res[1] = rI; res[2] = rJ; res[3] = lI; res[4] = lJ;
                 res[5] = -1; res[6] = -1; res[7] = 0;
-                L.rerun[pid] = rerun ? 1 : 0;
+                L.rerun[pid] = flag;
             }
         }
     }
+    if (listed && lane == 0) L.bytes_list[1 + atomicAdd(&L.bytes_list[0], 1)] = w;
     if (!TRACE) return;
     // The walks, fused: lane 0 walks pair A, lane 32 pair B, while the wave's
     // slot would otherwise sit idle; their load latency overlaps the forward
@@ -602,26 +644,63 @@ void sw16_kernel(Sw16Launch L)
     }
 }
 
+template <bool TRACE, bool GK, int LAY>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
+void sw16_kernel(Sw16Launch L)
+{
+    sw16_wave<TRACE, GK, LAY, false>(L, int(blockIdx.x));
+}
+
+// The literal-byte pass over the waves the first pass listed: a small grid
+// takes them in turn (most launches list none, and a full grid of waves that
+// exit at once would still cost its dispatch).
+template <bool TRACE, bool GK, int LAY>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
+void sw16_bytes_kernel(Sw16Launch L)
+{
+    const int n = __builtin_amdgcn_readfirstlane(L.bytes_list[0]);
+    for (int k = int(blockIdx.x); k < n; k += int(gridDim.x)) {
+        sw16_wave<TRACE, GK, LAY, true>(L, __builtin_amdgcn_readfirstlane(L.bytes_list[1 + k]));
+        __syncthreads();                         // the LDS is the next wave's
+    }
+}
+
+__global__ void sw16_zero_kernel(int32_t* p) { if (threadIdx.x == 0) p[0] = 0; }
+
+template <bool GK, int LAY>
+void sw16_launch_t(const Sw16Launch& L, bool trace, unsigned bytes_grid, const dim3& grid, size_t lds,
+                   hipStream_t st)
+{
+    const dim3 block(64);
+    if (trace) hipLaunchKernelGGL((sw16_kernel<true, GK, LAY>), grid, block, lds, st, L);
+    else       hipLaunchKernelGGL((sw16_kernel<false, GK, kLayFlags>), grid, block, lds, st, L);
+    if (bytes_grid == 0) return;
+    if (trace) hipLaunchKernelGGL((sw16_bytes_kernel<true, GK, LAY>), dim3(bytes_grid), block, lds, st, L);
+    else       hipLaunchKernelGGL((sw16_bytes_kernel<false, GK, kLayFlags>), dim3(bytes_grid), block, lds, st, L);
+}
+
 }  // namespace
 
 hipError_t sw16_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHooks* hooks)
 {
     if (L.n_waves <= 0) return hipSuccess;
     const size_t lds = sw16_lds_bytes(L.max_m, trace ? L.max_n : 0);
-    const dim3 grid(unsigned(L.n_waves)), block(64);
+    const dim3 grid(unsigned(L.n_waves));
     if (hooks) hooks->before(st);
     const int lay = sw16_layout(L);
-    if (sw16_gk(L)) {
-        if (!trace)                  hipLaunchKernelGGL((sw16_kernel<false, true, kLayFlags>), grid, block, lds, st, L);
-        else if (lay == kLayHmod8)   hipLaunchKernelGGL((sw16_kernel<true, true, kLayHmod8>), grid, block, lds, st, L);
-        else if (lay == kLayHmod4)   hipLaunchKernelGGL((sw16_kernel<true, true, kLayHmod4>), grid, block, lds, st, L);
-        else                         hipLaunchKernelGGL((sw16_kernel<true, true, kLayFlags>), grid, block, lds, st, L);
-    } else {
-        if (!trace)                  hipLaunchKernelGGL((sw16_kernel<false, false, kLayFlags>), grid, block, lds, st, L);
-        else if (lay == kLayHmod8)   hipLaunchKernelGGL((sw16_kernel<true, false, kLayHmod8>), grid, block, lds, st, L);
-        else if (lay == kLayHmod4)   hipLaunchKernelGGL((sw16_kernel<true, false, kLayHmod4>), grid, block, lds, st, L);
-        else                         hipLaunchKernelGGL((sw16_kernel<true, false, kLayFlags>), grid, block, lds, st, L);
-    }
+    // the literal-byte pass runs when the launch gives its list and the scheme
+    // allows it; its counter is cleared first by a kernel (a memset node is
+    // not used, so the sequence stays safe to capture)
+    const bool bytes = L.bytes_list != nullptr && L.mismatch < 0;
+    const unsigned bytes_grid = bytes ? unsigned(std::min(L.n_waves, 1024)) : 0u;
+    if (bytes) hipLaunchKernelGGL(sw16_zero_kernel, dim3(1), dim3(64), 0, st, L.bytes_list);
+    const bool gk = sw16_gk(L);
+    if (gk && lay == kLayHmod8)       sw16_launch_t<true, kLayHmod8>(L, trace, bytes_grid, grid, lds, st);
+    else if (gk && lay == kLayHmod4)  sw16_launch_t<true, kLayHmod4>(L, trace, bytes_grid, grid, lds, st);
+    else if (gk)                      sw16_launch_t<true, kLayFlags>(L, trace, bytes_grid, grid, lds, st);
+    else if (lay == kLayHmod8)        sw16_launch_t<false, kLayHmod8>(L, trace, bytes_grid, grid, lds, st);
+    else if (lay == kLayHmod4)        sw16_launch_t<false, kLayHmod4>(L, trace, bytes_grid, grid, lds, st);
+    else                              sw16_launch_t<false, kLayFlags>(L, trace, bytes_grid, grid, lds, st);
     if (hooks) hooks->after(st);
     return hipGetLastError();
 }

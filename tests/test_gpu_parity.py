@@ -358,6 +358,79 @@ def test_sw_device_resident_embedded(gpu, oracle):
         assert tuple(int(x) for x in out2[i]) == (s, -1, -1, ei, ej)
 
 
+def test_sw_literal_bytes_pass(gpu, oracle):
+    """Pairs holding bytes other than A/C/G/T (N runs, soft-masked lowercase,
+    IUPAC codes, punctuation) mixed with plain pairs, so waves hold one of
+    each: under schemes with mismatch < 0 the packed kernel's literal-byte
+    pass runs them (every layout, with and without the gk term), under
+    mismatch >= 0 the int32 kernel does; a pair of the literal-byte pass that
+    outgrows 16 bits goes on to the int32 kernel.  Through the batch entry
+    point (score-only and CIGAR batches) and the device entry point."""
+    import genoaligner_amd as ga
+    from genoaligner_amd.device import SwDeviceBatch
+    from genoaligner_amd.workloads import PairSet
+    import numpy as np
+    rng = random.Random(59)
+    pairs = []
+    for k in range(160):
+        t = "".join(rng.choice("ACGT") for _ in range(rng.randint(30, 900)))
+        a = rng.randint(0, len(t) // 2)
+        p = _mutate(rng, t[a: a + rng.randint(20, 250)], 0.08)
+        if k % 2 == 0:                                  # every other pair: other bytes in it
+            kind = k % 8
+            tl, pl = list(t), list(p)
+            for _ in range(rng.randint(1, 4)):
+                if kind == 0:                           # an N run in the text
+                    at = rng.randint(0, len(tl) - 1)
+                    tl[at: at + rng.randint(1, 12)] = ["N"] * rng.randint(1, 12)
+                elif kind == 2:                         # soft-masked run in both
+                    at = rng.randint(0, len(tl) - 1)
+                    tl[at: at + 20] = [c.lower() for c in tl[at: at + 20]]
+                    bt = rng.randint(0, len(pl) - 1)
+                    pl[bt: bt + 10] = [c.lower() for c in pl[bt: bt + 10]]
+                elif kind == 4:                         # IUPAC codes in the pattern
+                    pl[rng.randint(0, len(pl) - 1)] = rng.choice("RYKMSWN")
+                else:                                   # other bytes on both sides
+                    tl[rng.randint(0, len(tl) - 1)] = rng.choice("@#xX-")
+                    pl[rng.randint(0, len(pl) - 1)] = rng.choice("@#xX-")
+            t, p = "".join(tl), "".join(pl)
+        pairs.append((t, p))
+    big = "".join(rng.choice("ACGT") for _ in range(3500))
+    pairs.append((big[:1700] + "N" + big[1701:], big[:1700] + "N" + big[1701:3000]))
+    schemes = ((2, -3, 3, 1), (1, -1, 2, 1), (2, -5, 2, 1), (5, -4, 6, 4), (2, -8, 6, 1), (3, 1, 2, 1),
+               (12, -3, 14, 1))
+    for sc in schemes:
+        for cig in (True, False):
+            reqs = [_sreq(ga, t, p, sc, cig) for t, p in pairs]
+            b = ga.align_sw_batch(reqs)
+            want = oracle.align_sw_batch(reqs)
+            assert b.ok(), b.error
+            for got, exp, (t, p) in zip(b.results, want.results, pairs):
+                assert (got.score, got.cigar, got.start_i, got.start_j, got.end_i, got.end_j, got.rescore_ok,
+                        got.wellformed_ok) == \
+                       (exp.score, exp.cigar, exp.start_i, exp.start_j, exp.end_i, exp.end_j, exp.rescore_ok,
+                        exp.wellformed_ok), (sc, cig, t, p)
+    # the device entry point (sequences in HBM) on the same pairs
+    ta = [t.encode() for t, _ in pairs]
+    pa = [p.encode() for _, p in pairs]
+    tl = np.array([len(x) for x in ta], np.int32)
+    pl = np.array([len(x) for x in pa], np.int32)
+    to = np.zeros(len(ta), np.int64)
+    po = np.zeros(len(pa), np.int64)
+    to[1:] = np.cumsum(tl)[:-1]
+    po[1:] = np.cumsum(pl)[:-1]
+    ps = PairSet(np.frombuffer(b"".join(ta), np.uint8), to, tl, np.frombuffer(b"".join(pa), np.uint8), po, pl,
+                 np.zeros(len(ta), np.int32), {})
+    for sc in ((2, -3, 3, 1), (12, -3, 14, 1)):
+        dev = SwDeviceBatch(ps, scoring=sc, trace=True)
+        dev.run()
+        out, cg = dev.results()
+        for i, (t, p) in enumerate(pairs):
+            r = oracle.sw_trace(t, p, sc)
+            assert tuple(int(x) for x in out[i]) == r[:5], (sc, i)
+            assert cg[i].decode() == r[5], (sc, i)
+
+
 def test_sw_packed_overflow_rerun(gpu, oracle):
     """Scores beyond the packed kernel's 16-bit range are re-run in int32 and
     still match; neighbours in the same wave are unaffected."""

@@ -34,6 +34,10 @@ def test_sketch_kernel_matches_oracle(gpu, oracle):
                     for _ in range(rng.randint(0, 2500)))
         t = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 2500)))
         pairs.append((p, t))
+    # sequences of 500,000+ windows take the kernel's 32-bit-histogram path
+    big = "".join(rng.choice("ACGT") for _ in range(520_000))
+    pairs.append((big, big[: 510_000] + "N" + big[510_001:]))
+    pairs.append(("ACGT" * 10, big))
     ps = _pairset_from(pairs, [10] * len(pairs))
     b = WfaDeviceBatch(ps, trace=False)
     feat, lb = R.sketch(b)
