@@ -22,8 +22,10 @@
 // a 4-byte table of u for its pattern byte against A/C/G/T; the text streams as
 // per-row v_perm selectors (byte 0: code of pair A's text byte, byte 2: 4 + code
 // of pair B's), so one v_perm_b32 yields both u values zero-extended.  A pair
-// containing any other byte is flagged and re-run by the int32 kernel (literal
-// byte semantics: N matches only N, case is significant).
+// containing any other byte is flagged (literal byte semantics: N matches only
+// N, case is significant) and re-run by the literal-byte pass of the same
+// kernel (sw16_bytes_kernel: u by a packed compare of raw bytes) when the
+// scheme's mismatch is negative, else by the int32 kernel.
 //
 // Traceback record (trace variant): 4 flags per cell and pair, taken from sign
 // bits of packed differences --
