@@ -185,11 +185,13 @@ struct Sw16Launch {
     int32_t bias;
 };
 
-// Largest text length the packed kernel takes (LDS: 8 bytes per text row).
-constexpr int kSw16MaxRows = 4096;
+// Largest text length the packed kernel takes (LDS: 8 bytes per text row,
+// 64 KB at this length).
+constexpr int kSw16MaxRows = 8192;
 // Largest pattern length the packed kernel takes in a CIGAR batch (its walk
 // stages both pairs' sequences and CIGARs, 4 bytes per row and column, in LDS:
-// with kSw16MaxRows that stays under 64 KB per wave).
+// with kSw16MaxRows at most 66.3 KB per wave, of the 160 KB a workgroup may
+// hold).
 constexpr int kSw16MaxCols = 8192;
 
 // Selector scratch per wave: 64 guard rows before and 80 after max_m rows
@@ -228,6 +230,8 @@ hipError_t sw_launch(const SwLaunch& L, bool trace, hipStream_t st, LaunchHooks*
 // Packed kernel over L.n_waves waves (two pairs each), then -- separately --
 // its walk over 2 * n_waves slots.
 hipError_t sw16_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHooks* hooks = nullptr);
+// LDS bytes per wave of a packed DNA launch over pairs up to these lengths.
+int64_t sw16_lds_need(int max_m, int max_n, bool trace);
 
 // Substitution-matrix pairs, two per workgroup (packed 16-bit), their strips
 // pipelined over the workgroup's waves; n_waves counts workgroups.  The trace

@@ -1421,15 +1421,22 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
     const int64_t npk = n - n32;
     if (npk > 0 && !err) {
         const int64_t n_waves = (npk + 1) / 2;
-        // chunk waves by the direction budget (stride from each chunk's largest shape)
         // direction words per wave (DNA) or per workgroup (matrix kernel)
         const auto dir_words = [&](int m_, int n_) {
             return mat ? ga::sw16m_dir_words_per_group(m_, n_) : ga::sw16_dir_words_per_wave(m_, n_);
         };
-        std::vector<std::pair<int64_t, int64_t>> wchunks;
-        int64_t max_stride = 0;
+        // Chunks of waves: a chunk's direction table stays within half the
+        // direction budget (two chunks may be in flight, one per stream), and
+        // (DNA) a wave whose LDS need is below half of its chunk's starts a
+        // chunk of its own -- a launch sizes every wave's LDS for its largest
+        // pair, and waves come longest text first, so short pairs would run
+        // at a long pair's occupancy.
+        struct WChunk { int64_t w0, w1; int32_t cm, cn; double steps; };
+        std::vector<WChunk> wchunks;
         {
+            const int64_t budget = std::max<int64_t>(sw_dir_budget() / 2, 1 << 20);
             int64_t w0 = 0, mm = 0, mn = 0;
+            double steps = 0.0;
             for (int64_t w = 0; w < n_waves; ++w) {
                 int64_t wm = 0, wn = 0;
                 for (int h = 0; h < 2; ++h) {
@@ -1438,50 +1445,104 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
                 }
                 const int64_t nm = std::max(mm, wm), nn = std::max(mn, wn);
                 const int64_t stride = trace ? dir_words(int(nm), int(nn)) : 0;
-                if (w > w0 && stride * (w - w0 + 1) * 4 > sw_dir_budget()) {
-                    wchunks.emplace_back(w0, w);
-                    max_stride = std::max(max_stride, trace ? dir_words(int(mm), int(mn)) * (w - w0) : 0);
-                    w0 = w; mm = wm; mn = wn;
+                const bool lds_split = !mat && w > w0 &&
+                                       2 * ga::sw16_lds_need(int(wm), int(wn), trace) <
+                                           ga::sw16_lds_need(int(mm), int(mn), trace);
+                if (w > w0 && (stride * (w - w0 + 1) * 4 > budget || lds_split)) {
+                    wchunks.push_back({w0, w, int32_t(mm), int32_t(mn), steps});
+                    w0 = w; mm = wm; mn = wn; steps = 0.0;
                 } else {
                     mm = nm; mn = nn;
                 }
+                steps += double(wm + 64) * double((wn + 63) / 64);
             }
-            wchunks.emplace_back(w0, n_waves);
-            max_stride = std::max(max_stride, trace ? dir_words(int(mm), int(mn)) * (n_waves - w0) : 0);
+            wchunks.push_back({w0, n_waves, int32_t(mm), int32_t(mn), steps});
         }
-        uint32_t* d_dirs16 = trace ? D.alloc<uint32_t>(size_t(max_stride)) : nullptr;
-        int32_t pk_max_m = 0;
-        for (int64_t k = n32; k < n; ++k) pk_max_m = std::max(pk_max_m, tl[size_t(k)]);
-        int64_t max_cw = 0;
-        for (auto& ch : wchunks) max_cw = std::max(max_cw, ch.second - ch.first);
-        uint32_t* d_sel = D.alloc<uint32_t>(size_t(max_cw * ga::sw16_sel_stride(pk_max_m)));
-        // the DNA kernel's list of waves for its literal-byte pass
-        int32_t* d_blist = mat ? nullptr : D.alloc<int32_t>(size_t(max_cw + 1));
-        if (!D.ok) err = "hipMalloc(dir table) -- batch workspace exceeds device memory; split the batch";
-        for (auto& ch : wchunks) {
-            if (err) break;
-            int32_t cm = 0, cn = 0;
-            for (int64_t k = n32 + 2 * ch.first; k < std::min(n, n32 + 2 * ch.second); ++k) {
-                cm = std::max(cm, tl[size_t(k)]);
-                cn = std::max(cn, pl[size_t(k)]);
+        // Several chunks run on two streams.  A chunk is latency-bound when
+        // its longest wave (a wave runs its strips one after another) outlasts
+        // its wave-steps spread over the device's 1024 SIMDs: measured, a lone
+        // wave's step takes about 72 ns and a full device retires a wave-step
+        // per SIMD about every 44 ns.  Latency-bound chunks (the few longest
+        // texts) take `st`, the others the second stream, so the device is not
+        // left idle behind a few long pairs; chunks all of one kind alternate,
+        // which overlaps each chunk's tail with the next.
+        const int nsq = wchunks.size() > 1 ? 2 : 1;
+        std::vector<int> ch_stream(wchunks.size(), 0);
+        int64_t dir_need[2] = {0, 0}, sel_need[2] = {0, 0}, cw_need[2] = {0, 0};
+        {
+            std::vector<uint8_t> lat_bound(wchunks.size());
+            size_t n_lat = 0;
+            for (size_t c = 0; c < wchunks.size(); ++c) {
+                const WChunk& ch = wchunks[c];
+                const double lone = 72.0 * double(ch.cm + 64) * double((ch.cn + 63) / 64);
+                const double full = 44.0 * ch.steps / 1024.0;
+                lat_bound[c] = lone > full;
+                n_lat += lat_bound[c];
             }
+            const bool mixed = n_lat > 0 && n_lat < wchunks.size();
+            for (size_t c = 0; c < wchunks.size(); ++c) {
+                const WChunk& ch = wchunks[c];
+                const int q = nsq == 1 ? 0 : mixed ? (lat_bound[c] ? 0 : 1) : int(c & 1);
+                ch_stream[c] = q;
+                const int64_t cw = ch.w1 - ch.w0;
+                if (trace) dir_need[q] = std::max(dir_need[q], dir_words(ch.cm, ch.cn) * cw);
+                sel_need[q] = std::max(sel_need[q], cw * ga::sw16_sel_stride(ch.cm));
+                cw_need[q] = std::max(cw_need[q], cw);
+            }
+        }
+        uint32_t* d_dirs16[2] = {nullptr, nullptr};
+        uint32_t* d_sel[2] = {nullptr, nullptr};
+        int32_t* d_blist[2] = {nullptr, nullptr};        // the DNA kernel's literal-byte wave lists
+        for (int q = 0; q < nsq; ++q) {
+            if (trace) d_dirs16[q] = D.alloc<uint32_t>(size_t(dir_need[q]));
+            d_sel[q] = D.alloc<uint32_t>(size_t(sel_need[q]));
+            if (!mat) d_blist[q] = D.alloc<int32_t>(size_t(cw_need[q] + 1));
+        }
+        if (!D.ok) err = "hipMalloc(dir table) -- batch workspace exceeds device memory; split the batch";
+        // the second stream starts behind the uploads and `st` waits for it
+        // before the results are copied back; with two streams the kernel
+        // timer takes one interval around both
+        hipStream_t sq[2] = {st, nullptr};
+        hipEvent_t fork = nullptr, join = nullptr;
+        if (!err && nsq > 1) {
+            sq[1] = acquire_stream();
+            if (!sq[1] || hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&join, hipEventDisableTiming) != hipSuccess ||
+                hipEventRecord(fork, st) != hipSuccess || hipStreamWaitEvent(sq[1], fork, 0) != hipSuccess)
+                err = "hipStreamWaitEvent";
+        }
+        ga::LaunchHooks* hooks = nsq > 1 ? nullptr : &t_timer_hooks;
+        if (!err && nsq > 1) t_timer.begin(st);
+        for (size_t c = 0; c < wchunks.size() && !err; ++c) {
+            const WChunk& ch = wchunks[c];
+            const int q = ch_stream[c];
             ga::Sw16Launch L{};
             L.txt = d_txt; L.toff = d_toff; L.tlen = d_tl;
             L.pat = d_pat; L.poff = d_poff; L.plen = d_pl;
-            L.first_pair = int32_t(n32 + 2 * ch.first); L.n_total = int32_t(n);
-            L.n_waves = int32_t(ch.second - ch.first); L.max_m = cm; L.max_n = cn;
+            L.first_pair = int32_t(n32 + 2 * ch.w0); L.n_total = int32_t(n);
+            L.n_waves = int32_t(ch.w1 - ch.w0); L.max_m = ch.cm; L.max_n = ch.cn;
             L.match = sc.match; L.mismatch = sc.mismatch; L.go = sc.gap_open; L.ge = sc.gap_extend;
             L.res = d_res; L.rerun = d_rerun;
-            L.dirs = d_dirs16; L.dir_stride = trace ? dir_words(cm, cn) : 0;
-            L.selbuf = d_sel;
-            L.bytes_list = d_blist;
+            L.dirs = d_dirs16[q]; L.dir_stride = trace ? dir_words(ch.cm, ch.cn) : 0;
+            L.selbuf = d_sel[q];
+            L.bytes_list = d_blist[q];
             L.walk = d_walk; L.cigar = d_cig; L.cig_off = d_slot;
             if (mat) {
                 L.go = mat->gap_open; L.ge = mat->gap_extend;
                 L.tcode = d_tcode; L.pcode = d_pcode; L.sub = d_sub; L.bias = mbias;
             }
-            if (enqueue_sw16_chunk(L, trace, st, &t_timer_hooks, mat) != hipSuccess) err = "SW kernel launch";
+            if (enqueue_sw16_chunk(L, trace, sq[q], hooks, mat) != hipSuccess) err = "SW kernel launch";
         }
+        if (sq[1]) {
+            // joined even after an error, so `st` never runs ahead of it
+            if ((hipEventRecord(join, sq[1]) != hipSuccess || hipStreamWaitEvent(st, join, 0) != hipSuccess) && !err)
+                err = "hipStreamWaitEvent";
+            if (err) (void)hipStreamSynchronize(sq[1]);
+        }
+        if (nsq > 1) t_timer.end(st);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (sq[1]) release_stream(sq[1]);
     }
 
     // results and CIGAR slots back into pinned staging (pageable fallback:

@@ -674,6 +674,16 @@ void sw16_launch_t(const Sw16Launch& L, bool trace, unsigned bytes_grid, const d
                    hipStream_t st)
 {
     const dim3 block(64);
+    if (lds > 65536) {
+        // above 64 KB of dynamic LDS the kernels are told first (the longest
+        // texts with long patterns in a CIGAR batch)
+        const hipFuncAttribute a = hipFuncAttributeMaxDynamicSharedMemorySize;
+        const void* k[4] = {reinterpret_cast<const void*>(&sw16_kernel<true, GK, LAY>),
+                            reinterpret_cast<const void*>(&sw16_kernel<false, GK, kLayFlags>),
+                            reinterpret_cast<const void*>(&sw16_bytes_kernel<true, GK, LAY>),
+                            reinterpret_cast<const void*>(&sw16_bytes_kernel<false, GK, kLayFlags>)};
+        for (const void* f : k) (void)hipFuncSetAttribute(f, a, int(lds));
+    }
     if (trace) hipLaunchKernelGGL((sw16_kernel<true, GK, LAY>), grid, block, lds, st, L);
     else       hipLaunchKernelGGL((sw16_kernel<false, GK, kLayFlags>), grid, block, lds, st, L);
     if (bytes_grid == 0) return;
@@ -682,6 +692,11 @@ void sw16_launch_t(const Sw16Launch& L, bool trace, unsigned bytes_grid, const d
 }
 
 }  // namespace
+
+int64_t sw16_lds_need(int max_m, int max_n, bool trace)
+{
+    return int64_t(sw16_lds_bytes(max_m, trace ? max_n : 0));
+}
 
 hipError_t sw16_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHooks* hooks)
 {

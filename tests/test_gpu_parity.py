@@ -324,6 +324,34 @@ def test_sw_long_pattern_cigar(gpu, oracle):
                (exp.score, exp.cigar, exp.start_i, exp.end_i, exp.start_j, exp.end_j)
 
 
+def test_sw_long_texts_packed_chunks(gpu, oracle):
+    """Texts of 4097..8192 rows take the packed kernel (up to 64 KB of LDS
+    boundary per wave, more with a long pattern's CIGAR staging), and a batch
+    whose waves differ in LDS need runs as several chunks on two streams."""
+    import genoaligner_amd as ga
+    rng = random.Random(17)
+    reqs = []
+    # 8192 x 8000 with CIGAR: the walk's staging needs more than 64 KB of LDS
+    t = "".join(rng.choice("ACGT") for _ in range(8192))
+    reqs.append(_sreq(ga, t, _mutate(rng, t[100:8100], 0.05), (2, -3, 3, 1)))
+    for lo, hi, k, pl in ((4097, 8192, 6, 1200), (2049, 4096, 40, 800), (100, 1000, 300, 300)):
+        for _ in range(k):
+            t = "".join(rng.choice("ACGT") for _ in range(rng.randint(lo, hi)))
+            a = rng.randint(0, len(t) - 1)
+            p = _mutate(rng, t[a: a + rng.randint(1, pl)], 0.1)
+            if rng.random() < 0.15:
+                p = "".join(rng.choice("ACGT") for _ in range(rng.randint(1, pl)))
+            reqs.append(_sreq(ga, t, p, (2, -3, 3, 1), rng.random() < 0.8))
+    b = ga.align_sw_batch(reqs)
+    want = oracle.align_sw_batch(reqs)
+    assert b.ok(), b.error
+    for k, (got, exp) in enumerate(zip(b.results, want.results)):
+        assert (got.score, got.resolved, got.cigar, got.start_i, got.start_j, got.end_i,
+                got.end_j, got.rescore_ok, got.wellformed_ok) == \
+               (exp.score, exp.resolved, exp.cigar, exp.start_i, exp.start_j, exp.end_i,
+                exp.end_j, exp.rescore_ok, exp.wellformed_ok), (k, len(reqs[k].text), len(reqs[k].pattern))
+
+
 def test_sw_too_large(gpu):
     import genoaligner_amd as ga
     big = "A" * 8192
