@@ -1425,8 +1425,10 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
         const auto dir_words = [&](int m_, int n_) {
             return mat ? ga::sw16m_dir_words_per_group(m_, n_) : ga::sw16_dir_words_per_wave(m_, n_);
         };
-        // Chunks of waves: a chunk's direction table stays within half the
-        // direction budget (two chunks may be in flight, one per stream), and
+        // Chunks of waves: a chunk's direction table stays within the
+        // direction budget (16 GiB by default; with a caller's workspace
+        // budget, half of the direction share, as two chunks may be in
+        // flight, one per stream), and
         // (DNA) a wave whose LDS need is below half of its chunk's starts a
         // chunk of its own -- a launch sizes every wave's LDS for its largest
         // pair, and waves come longest text first, so short pairs would run
@@ -1434,7 +1436,8 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
         struct WChunk { int64_t w0, w1; int32_t cm, cn; double steps; };
         std::vector<WChunk> wchunks;
         {
-            const int64_t budget = std::max<int64_t>(sw_dir_budget() / 2, 1 << 20);
+            const int64_t budget = t_workspace_budget > 0 ? std::max<int64_t>(sw_dir_budget() / 2, 1 << 20)
+                                                          : sw_dir_budget();
             int64_t w0 = 0, mm = 0, mn = 0;
             double steps = 0.0;
             for (int64_t w = 0; w < n_waves; ++w) {
