@@ -33,7 +33,7 @@ EXPORTS = (
     "ga_wfa_graph_create", "ga_wfa_graph_launch", "ga_wfa_graph_destroy",
     "ga_wfa_autotune", "ga_wfa_set_tuning", "ga_wfa_get_tuning",
     "ga_fasta_parse", "ga_fasta_read_file", "ga_fasta_count", "ga_fasta_record", "ga_fasta_free",
-    "ga_fasta_sizes", "ga_fasta_export", "ga_fasta_device", "ga_gene_qc_walk",
+    "ga_fasta_sizes", "ga_fasta_export", "ga_fasta_device", "ga_gene_qc_walk", "ga_gene_qc_tsv",
     "ga_msa_align_profiles", "ga_msa_profile_from_rows", "ga_msa_pp_device", "ga_kmer_distances", "ga_nj_tree",
 )
 
@@ -227,6 +227,9 @@ def lib() -> C.CDLL:
         L.ga_gene_qc_walk.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.c_int32, C.c_int32, C.c_void_p]
         L.ga_gene_qc_walk.restype = C.c_int
+        L.ga_gene_qc_tsv.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_double, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p]
+        L.ga_gene_qc_tsv.restype = C.c_int64
         L.ga_msa_align_profiles.argtypes = [C.POINTER(MsaProfile), C.POINTER(MsaProfile), C.c_int64,
                                             C.POINTER(MsaParams), C.POINTER(MsaPPRes), C.c_void_p, C.c_int64, cpp]
         L.ga_msa_align_profiles.restype = C.c_int

@@ -33,30 +33,63 @@ class FastaRecord:
     sequence: bytes
 
 
-def _records(h: C.c_void_p) -> List[FastaRecord]:
+@dataclass
+class FastaArrays:
+    """Every record of a file as three byte buffers with offsets (n + 1 each):
+    record k's id is ids[id_off[k]:id_off[k + 1]], likewise desc and seq; each
+    buffer holds one zero byte behind its last record."""
+    ids: "np.ndarray"
+    id_off: "np.ndarray"
+    desc: "np.ndarray"
+    desc_off: "np.ndarray"
+    seq: "np.ndarray"
+    seq_off: "np.ndarray"
+
+    def __len__(self) -> int:
+        return len(self.id_off) - 1
+
+
+def _export(h: C.c_void_p) -> FastaArrays:
     """Every record, through one bulk copy (ga_fasta_sizes / ga_fasta_export)."""
     import numpy as np
     L = N.lib()
     n = L.ga_fasta_count(h)
     sz = [C.c_int64() for _ in range(3)]
     L.ga_fasta_sizes(h, C.byref(sz[0]), C.byref(sz[1]), C.byref(sz[2]))
-    bufs = [np.empty(max(x.value, 1), np.uint8) for x in sz]
+    bufs = [np.empty(x.value + 1, np.uint8) for x in sz]
+    for b in bufs:
+        b[-1] = 0                                # one zero byte behind the last record
     offs = [np.empty(n + 1, np.int64) for _ in range(3)]
     L.ga_fasta_export(h, bufs[0].ctypes.data, offs[0].ctypes.data, bufs[1].ctypes.data, offs[1].ctypes.data,
                       bufs[2].ctypes.data, offs[2].ctypes.data)
-    raw = [b.tobytes() for b in bufs]
-    o = [x.tolist() for x in offs]
+    return FastaArrays(bufs[0], offs[0], bufs[1], offs[1], bufs[2], offs[2])
+
+
+def _records(a: FastaArrays) -> List[FastaRecord]:
+    raw = [b.tobytes() for b in (a.ids, a.desc, a.seq)]
+    o = [x.tolist() for x in (a.id_off, a.desc_off, a.seq_off)]
     return [FastaRecord(raw[0][o[0][k]:o[0][k + 1]], raw[1][o[1][k]:o[1][k + 1]], raw[2][o[2][k]:o[2][k + 1]])
-            for k in range(n)]
+            for k in range(len(a))]
 
 
-def _take(rc: int, h: C.c_void_p, err: C.c_char_p) -> List[FastaRecord]:
+def _take_arrays(rc: int, h: C.c_void_p, err: C.c_char_p) -> FastaArrays:
     if rc != N.GA_OK:
         raise FastaError((err.value or b"").decode("latin-1"))
     try:
-        return _records(h)
+        return _export(h)
     finally:
         N.lib().ga_fasta_free(h)
+
+
+def _take(rc: int, h: C.c_void_p, err: C.c_char_p) -> List[FastaRecord]:
+    return _records(_take_arrays(rc, h, err))
+
+
+def read_fasta_file_arrays(path: Union[str, os.PathLike]) -> FastaArrays:
+    """read_fasta_file's records as byte buffers and offsets, with no Python
+    object per record (for callers that hand the sequences to a batch)."""
+    h, err = C.c_void_p(), C.c_char_p()
+    return _take_arrays(N.lib().ga_fasta_read_file(os.fsencode(path), C.byref(h), C.byref(err)), h, err)
 
 
 def read_fasta(data: Union[bytes, bytearray, memoryview, str]) -> List[FastaRecord]:

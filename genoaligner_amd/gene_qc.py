@@ -1,7 +1,9 @@
 """Gene QC tool (reference: tools/gene_qc.cpp): every record of a gene
 FASTA aligned to the gene's bait by Smith-Waterman with CIGAR on the GPU (one
 pattern, many texts), then each CIGAR walked for coverage, frameshifts and
-in-frame stops (``ga_gene_qc_walk``, native, on the host's worker threads).
+in-frame stops (``ga_gene_qc_walk``, native, on the host's worker threads),
+and the TSV written by ``ga_gene_qc_tsv`` (native too).  The records go from
+the FASTA reader's buffers to the batch with no Python object per record.
 
 Same command line, the same TSV rows and the same summary as the reference
 tool: ``python -m genoaligner_amd.gene_qc seqs.fa --ref ref.fa [--code 1|2]
@@ -20,7 +22,7 @@ import numpy as np
 
 from . import _native as N
 from .api import _SW_REQ, _SW_RES, _ptr, backend_name
-from .fasta import FastaError, read_fasta_file
+from .fasta import FastaError, read_fasta_file, read_fasta_file_arrays
 
 # tools/gene_qc.cpp:53-61, by genetic code (any other code: TAA and TAG only)
 _STOPS = {1: (b"TAA", b"TAG", b"TGA"), 2: (b"TAA", b"TAG", b"AGA", b"AGG")}
@@ -54,16 +56,13 @@ def pick_bait(seqs: Sequence[bytes], code: int) -> Tuple[int, int, int]:
     return best, frame, best_stops
 
 
-def _align(texts: List[bytes], bait: bytes, frame: int, code: int, scoring, batch_size: int, err: TextIO):
-    """align_sw_batch in chunks (halved on a refusal) + the native walk.
-    Returns (res structured array, qc rows array) or None after a failure."""
+def _align(buf: np.ndarray, off: np.ndarray, lens: np.ndarray, bait: bytes, frame: int, code: int, scoring,
+           batch_size: int, err: TextIO):
+    """align_sw_batch in chunks (halved on a refusal) + the native walk over
+    the records buf[off[k]:][:lens[k]].  Returns (res structured array, qc rows
+    array) or None after a failure."""
     L = N.lib()
-    n = len(texts)
-    lens = np.array([len(t) for t in texts], np.int32)
-    off = np.zeros(n, np.int64)
-    if n > 1:
-        off[1:] = np.cumsum(lens, dtype=np.int64)[:-1]
-    buf = np.frombuffer(b"".join(texts) + b"\0", np.uint8)
+    n = len(lens)
     pat = np.frombuffer(bait + b"\0", np.uint8)
     res = np.zeros(n, dtype=_SW_RES)
     rows = np.zeros(n, dtype=np.dtype([("cov_text", "<f8"), ("fs_runs", "<i4"), ("fs_net", "<i4"),
@@ -104,19 +103,6 @@ def _align(texts: List[bytes], bait: bytes, frame: int, code: int, scoring, batc
     return res, rows
 
 
-def flag_of(score: int, resolved: bool, cov: float, fs_net: int, stops: int, min_cov: float,
-            min_score: int) -> str:
-    """gene_qc.cpp:219-230."""
-    if not resolved or score <= min_score:
-        return "NO_ALIGN"
-    flag = "LOW_COV" if cov < min_cov else ""
-    if fs_net != 0:
-        flag += ("+" if flag else "") + f"FS={fs_net}"
-    if stops > 0:
-        flag += ("+" if flag else "") + f"STOP={stops}"
-    return flag or "OK"
-
-
 def run(seqs_path: str, ref_path: str, code: int = 1, scoring=(1, -1, 2, 1), min_cov: float = 0.5,
         min_score: int = 0, batch_size: int = 0, emit: Optional[str] = None,
         out: Optional[TextIO] = None, err: Optional[TextIO] = None) -> int:
@@ -125,52 +111,46 @@ def run(seqs_path: str, ref_path: str, code: int = 1, scoring=(1, -1, 2, 1), min
     out = sys.stdout if out is None else out
     err = sys.stderr if err is None else err
     try:
-        recs = read_fasta_file(seqs_path)
+        recs = read_fasta_file_arrays(seqs_path)
         rrecs = read_fasta_file(ref_path)
     except FastaError as e:
         print(f"fasta error: {e}", file=err)
         return 1
     best, frame, best_stops = pick_bait([r.sequence for r in rrecs], code)
     ref = rrecs[best].sequence
-    print(f"gene_qc: {len(recs)} records vs ref {rrecs[best].id.decode('latin-1')} ({len(ref)} bp, frame "
+    n = len(recs)
+    print(f"gene_qc: {n} records vs ref {rrecs[best].id.decode('latin-1')} ({len(ref)} bp, frame "
           f"{frame}, bait_stops {best_stops}, code {code}) on {backend_name()}", file=out)
     if best_stops > 0:
         print(f"WARN: bait itself carries {best_stops} stops in its best frame -- stop counts for this gene "
               "are weak evidence", file=err)
-    got = _align([r.sequence for r in recs], ref, frame, code, scoring, batch_size, err)
+    # the records' sequences straight from the reader's buffer
+    off = np.ascontiguousarray(recs.seq_off[:n])
+    lens = np.diff(recs.seq_off).astype(np.int32)
+    got = _align(recs.seq, off, lens, ref, frame, code, scoring, batch_size, err)
     if got is None:
         return 1
     res, rows = got
     try:
-        dst = open(emit, "w", encoding="latin-1", newline="\n") if emit else out
+        dst = open(emit, "wb") if emit else None
     except OSError:
         print(f"cannot write {emit}", file=err)
         return 1
-    lines = ["acc\tlen\tscore\tsi\tsj\tei\tej\tcov_text\tfs_runs\tfs_net\tstops\tstop_end\tflag"]
-    n_ok = n_noalign = n_lowcov = n_fs = n_stop = n_bad = 0
-    R = {f: res[f].tolist() for f in ("score", "resolved", "start_i", "start_j", "end_i", "end_j",
-                                       "rescore_ok", "wellformed_ok")}
-    Q = {f: rows[f].tolist() for f in ("cov_text", "fs_runs", "fs_net", "stops", "stop_end")}
-    for k, rec in enumerate(recs):
-        score, resolved = R["score"][k], bool(R["resolved"][k])
-        flag = flag_of(score, resolved, Q["cov_text"][k], Q["fs_net"][k], Q["stops"][k], min_cov, min_score)
-        lines.append("%s\t%d\t%d\t%d\t%d\t%d\t%d\t%.3f\t%d\t%d\t%d\t%d\t%s" % (
-            rec.id.decode("latin-1"), len(rec.sequence), score, R["start_i"][k], R["start_j"][k], R["end_i"][k],
-            R["end_j"][k], Q["cov_text"][k], Q["fs_runs"][k], Q["fs_net"][k], Q["stops"][k], Q["stop_end"][k],
-            flag))
-        if flag == "OK":
-            n_ok += 1
-        else:
-            n_noalign += "NO_ALIGN" in flag
-            n_lowcov += "LOW_COV" in flag
-            n_fs += "FS=" in flag
-            n_stop += "STOP=" in flag
-        if resolved and score > 0 and (not R["rescore_ok"][k] or not R["wellformed_ok"][k]):
-            n_bad += 1
-    dst.write("\n".join(lines) + "\n")
-    if emit:
-        dst.close()
-    print(f"summary: {len(recs)} seqs | OK={n_ok} NO_ALIGN={n_noalign} LOW_COV={n_lowcov} FS={n_fs} "
+    L = N.lib()
+    counts = np.zeros(6, np.int64)
+    args = (res.ctypes.data, rows.ctypes.data, n, recs.ids.ctypes.data, recs.id_off.ctypes.data,
+            lens.ctypes.data, float(min_cov), int(min_score))
+    need = L.ga_gene_qc_tsv(*args, None, 0, counts.ctypes.data)
+    tsv = np.empty(max(need, 1), np.uint8)
+    if need < 0 or L.ga_gene_qc_tsv(*args, tsv.ctypes.data, need, counts.ctypes.data) != need:
+        raise RuntimeError("ga_gene_qc_tsv refused its arguments")
+    if dst is not None:
+        with dst:
+            dst.write(memoryview(tsv)[:need])
+    else:
+        out.write(tsv[:need].tobytes().decode("latin-1"))
+    n_ok, n_noalign, n_lowcov, n_fs, n_stop, n_bad = (int(x) for x in counts)
+    print(f"summary: {n} seqs | OK={n_ok} NO_ALIGN={n_noalign} LOW_COV={n_lowcov} FS={n_fs} "
           f"STOP={n_stop} | invalid={n_bad}", file=err)
     return 1 if n_bad else 0
 

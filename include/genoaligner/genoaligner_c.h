@@ -410,6 +410,20 @@ typedef struct ga_qc_row {
 int ga_gene_qc_walk(const ga_sw_res* res, int64_t n, const uint8_t* cigar_arena, const uint8_t* text,
                     const int64_t* text_off, const int32_t* text_len, int32_t frame, int32_t code,
                     ga_qc_row* rows);
+/* The tool's TSV (reference: tools/gene_qc.cpp:168, 239-262): the header and
+ * one row per record -- id (record k's bytes ids[id_off[k] .. id_off[k+1]),
+ * up to a NUL byte), text length, score, si, sj, ei, ej, cov_text (%.3f),
+ * fs_runs, fs_net, stops, stop_end, flag (NO_ALIGN when unresolved or
+ * score <= min_score; else LOW_COV when cov_text < min_cov, +FS=net when
+ * fs_net != 0, +STOP=n when stops > 0, or OK) -- written into out when
+ * out_cap holds it.  Returns the TSV's length in bytes (nothing written when
+ * out is NULL or out_cap is smaller), or -1 for invalid arguments.  counts
+ * (6) receives the summary line's OK, NO_ALIGN, LOW_COV, FS, STOP and invalid
+ * counts (gene_qc.cpp:255-266).  EXTENSION (the reference tool formats inside
+ * its main).  Runs on the host's worker threads. */
+int64_t ga_gene_qc_tsv(const ga_sw_res* res, const ga_qc_row* rows, int64_t n, const char* ids,
+                       const int64_t* id_off, const int32_t* text_len, double min_cov, int32_t min_score,
+                       char* out, int64_t out_cap, int64_t* counts);
 
 /* ---- MSA building blocks (reference: src/msa/msa_ref.cpp, the genomsa
  * engine; include/genoaligner/backend/msa_pp_kernel_impl.hip,

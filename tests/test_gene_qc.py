@@ -114,3 +114,45 @@ def test_tool_argument_and_fasta_errors(tmp_path, capsys):
     err = io.StringIO()
     assert G.run(str(bad), str(bad), err=err) == 1
     assert err.getvalue() == "fasta error: record has a header but no sequence (line 1)\n"
+
+
+def test_library_tsv_matches_oracle(oracle):
+    """ga_gene_qc_tsv (a host call) over the oracle's SW results and walk rows
+    writes the oracle pipeline's TSV byte for byte, with its summary counts,
+    under thresholds that produce every flag and flag combination."""
+    from genoaligner_amd import _native as N
+    from genoaligner_amd import workloads as W
+    from genoaligner_amd.api import _SW_RES
+    seqs, refs = W.gene_family(80, codons=90, seed=11)
+    recs = oracle.fasta_parse(seqs)[0]
+    best, frame, _ = oracle.gene_qc_bait([r[2] for r in oracle.fasta_parse(refs)[0]], 2)
+    flags = set()
+    for min_cov, min_score in ((0.5, 0), (0.95, 100), (0.0, -1), (1.1, 0), (1.1, 10 ** 6)):
+        want_tsv, want_summary = oracle.gene_qc_tsv(seqs, refs, code=2, min_cov=min_cov, min_score=min_score)
+        bait = oracle.fasta_parse(refs)[0][best][2]
+        res = np.zeros(len(recs), dtype=_SW_RES)
+        rows = np.zeros(len(recs), dtype=np.dtype([("cov_text", "<f8"), ("fs_runs", "<i4"), ("fs_net", "<i4"),
+                                                   ("stops", "<i4"), ("stop_end", "<i4")]))
+        for k, r in enumerate(recs):
+            sc, si, sj, ei, ej, cig = oracle.sw_trace(r[2], bait, (1, -1, 2, 1))
+            res[k] = (sc, 1, 0, len(cig), si, sj, ei, ej, 1, 1, 0)
+            rows[k] = oracle.gene_qc_walk(cig.encode(), r[2], sc, True, si, sj, ei, frame, 2)
+        ids = np.frombuffer(b"".join(r[0] for r in recs) + b"\0", np.uint8)
+        id_off = np.concatenate([[0], np.cumsum([len(r[0]) for r in recs])]).astype(np.int64)
+        lens = np.array([len(r[2]) for r in recs], np.int32)
+        counts = np.zeros(6, np.int64)
+        L = N.lib()
+        args = (res.ctypes.data, rows.ctypes.data, len(recs), ids.ctypes.data, id_off.ctypes.data,
+                lens.ctypes.data, min_cov, min_score)
+        need = L.ga_gene_qc_tsv(*args, None, 0, counts.ctypes.data)
+        out = np.zeros(need, np.uint8)
+        assert L.ga_gene_qc_tsv(*args, out.ctypes.data, need - 1, counts.ctypes.data) == need
+        assert not out.any()                     # too small: nothing written
+        assert L.ga_gene_qc_tsv(*args, out.ctypes.data, need, counts.ctypes.data) == need
+        assert out.tobytes().decode("latin-1") == want_tsv
+        assert want_summary.endswith("OK=%d NO_ALIGN=%d LOW_COV=%d FS=%d STOP=%d | invalid=%d" % tuple(counts))
+        flags |= {ln.split("\t")[-1].split("=")[0] for ln in want_tsv.splitlines()[1:]}
+    assert {"OK", "NO_ALIGN", "LOW_COV", "FS", "STOP", "LOW_COV+FS", "LOW_COV+STOP"} <= flags, flags
+    assert L.ga_gene_qc_tsv(None, None, 3, None, None, None, 0.5, 0, None, 0, counts.ctypes.data) == -1
+    header = b"acc\tlen\tscore\tsi\tsj\tei\tej\tcov_text\tfs_runs\tfs_net\tstops\tstop_end\tflag\n"
+    assert L.ga_gene_qc_tsv(None, None, 0, None, None, None, 0.5, 0, None, 0, counts.ctypes.data) == len(header)
