@@ -126,3 +126,48 @@ def test_oracle_pack2(oracle):
     assert int(w[0]) == sum(((j % 4) << (2 * j)) for j in range(16))
     assert int(w[1]) == 3
     assert len(oracle.fasta_pack2(b"")) == 0
+
+
+def _big_fasta(rng: random.Random, nbytes: int) -> bytes:
+    """A valid file of about nbytes: blank and blank-led lines before the first
+    header, headers with and without descriptions, sequence lines ending in
+    \\n, \\r\\n, a lone \\r or a blank line, holding spaces, lowercase, N and '>'
+    (never first on a line)."""
+    out, size, k = [b"\n \t x>y\n\r\n"], 0, 0
+    while size < nbytes:
+        head = b">r%d desc %d\t \n" % (k, k) if rng.random() < 0.9 else b">r%d\r\n" % k
+        lines = []
+        for _ in range(rng.randint(1, 40)):
+            ln = bytes(rng.choices(b"ACGTNacgt >x", k=rng.randint(0, 120)))
+            if ln.startswith(b">"):
+                ln = b"A" + ln
+            lines.append(ln + rng.choice([b"\n", b"\r\n", b"\r", b"\n\n"]))
+        lines.append(b"ACGT\n")
+        rec = head + b"".join(lines)
+        out.append(rec)
+        size += len(rec)
+        k += 1
+    return b"".join(out)
+
+
+def test_library_reader_large_files_match_oracle(oracle):
+    """Files of 4 MiB and more are parsed in segments cut at record starts on
+    the host's worker threads: the same records as the oracle's reader, and
+    for a malformed file its message and line number, wherever the fault
+    lies."""
+    rng = random.Random(21)
+    files = [_big_fasta(rng, 6 << 20), _big_fasta(rng, (4 << 20) + 7)]
+    # one record larger than every segment; a file led by 5 MiB of blank lines
+    files.append(b">only one\n" + b"ACGTN" * (1 << 20) + b"\n")
+    files.append(b"\n" * (5 << 20) + b">late\nACGT\r\n>late2 x\rAC")
+    base = _big_fasta(rng, 5 << 20)
+    for frac in (0.03, 0.5, 0.97):
+        cut = base.index(b"\n>", int(len(base) * frac)) + 1
+        files.append(base[:cut] + b">empty_record\n" + base[cut:])   # header but no sequence
+        files.append(base[:cut] + b">\nACGT\n" + base[cut:])           # empty identifier
+        files.append(base[:cut] + b"> x\nACGT\n" + base[cut:])         # id starts with a blank
+    files.append(b"\n" * (5 << 20))                                     # no records
+    files.append(base + b">tail_without_sequence\n\n")
+    for t in files:
+        want = oracle.fasta_parse(t)
+        assert _host_read(t) == want, (len(t), want[1])
