@@ -24,7 +24,7 @@ GA_SW_MAX_TRACE_CELLS = 1 << 26
 EXPORTS = (
     "ga_align_batch", "ga_align_batch_ext", "ga_align_sw_batch",
     "ga_backend_name", "ga_device_name", "ga_device_available",
-    "ga_cigar_arena_bytes_wfa", "ga_cigar_arena_bytes_sw",
+    "ga_cigar_arena_bytes_wfa", "ga_cigar_arena_bytes_sw", "ga_sw_validate_cigar",
     "ga_wfa_device", "ga_wfa_device_noalloc", "ga_sw_device", "ga_set_kernel_timing", "ga_last_kernel_ms",
     "ga_sketch_device", "ga_mlp_forward_device", "ga_mlp_logits_device", "ga_mlp_backward_device", "ga_mlp_sgd_device",
     "ga_mlp_route_device", "ga_route_bucket_bound", "ga_wfa_routed_device", "ga_router_set", "ga_router_enabled",
@@ -159,6 +159,10 @@ def lib() -> C.CDLL:
         L.ga_cigar_arena_bytes_wfa.restype = C.c_int64
         L.ga_cigar_arena_bytes_sw.argtypes = [C.POINTER(SWReq), C.c_int64]
         L.ga_cigar_arena_bytes_sw.restype = C.c_int64
+        L.ga_sw_validate_cigar.argtypes = [C.c_char_p, C.c_int32, C.c_char_p, C.c_int32, C.c_int32, C.c_int32,
+                                           C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(SWScoringC),
+                                           C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.ga_sw_validate_cigar.restype = C.c_int
         L.ga_wfa_device.argtypes = [C.POINTER(WfaDevBatch), C.c_void_p, cpp]
         L.ga_wfa_device.restype = C.c_int
         L.ga_wfa_device_noalloc.argtypes = [C.POINTER(WfaDevBatch), C.c_void_p, cpp]

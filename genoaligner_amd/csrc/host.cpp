@@ -577,11 +577,35 @@ void wfa_validate(const char* cg, int len, int score, const char* pattern, int m
     *rescore_ok = (s == score);
 }
 
+// End of the run of cg[c] in cg[0 .. len): eight bytes at a time against the
+// run's byte broadcast, then the first differing byte.
+int cigar_run_end(const char* cg, int c, int len)
+{
+    const uint64_t rep = uint64_t(uint8_t(cg[c])) * 0x0101010101010101ull;
+    int e = c + 1;
+    while (e + 8 <= len) {
+        uint64_t w;
+        std::memcpy(&w, cg + e, 8);
+        const uint64_t x = w ^ rep;
+        if (x) return e + (__builtin_ctzll(x) >> 3);
+        e += 8;
+    }
+    while (e < len && cg[e] == cg[c]) ++e;
+    return e;
+}
+
+// The same checks as the reference's SW validator (api.cpp:114-156), a run
+// of equal ops at a time: every op inside the span [start, end] and the
+// sequences, M on equal and X on unequal bytes, the walk ending at (end_i,
+// end_j), and the affine re-score (a gap run of L costs go + (L-1) ge) equal
+// to `score`.  Positions only grow along a run, so a run's ops lie in range
+// exactly up to the first that does not, and its byte tests are one
+// branch-free loop.
 void sw_validate(const char* cg, int len, const char* text, int text_len, int start_i, int end_i,
                  const char* pattern, int pattern_len, int start_j, int end_j,
                  const ga_sw_scoring& sc, int score, int32_t* rescore_ok, int32_t* wellformed_ok)
 {
-    int i = start_i, j = start_j, s = 0, run = 0;
+    int i = start_i, j = start_j, s = 0, run = 0;    // run: open gap, > 0 inserted, < 0 deleted
     bool wf = true;
     auto close_gap = [&] {
         if (run != 0) {
@@ -590,26 +614,43 @@ void sw_validate(const char* cg, int len, const char* text, int text_len, int st
             run = 0;
         }
     };
-    for (int c = 0; c < len; ++c) {
+    const int lim_i = std::min(end_i + 1, text_len), lim_j = std::min(end_j + 1, pattern_len);
+    for (int c = 0; c < len;) {
+        const int e = cigar_run_end(cg, c, len);
+        const int L = e - c;
         switch (cg[c]) {
             case 'M':
+            case 'X': {
                 close_gap();
-                if (i > end_i || j > end_j || i >= text_len || j >= pattern_len || text[i] != pattern[j]) wf = false;
-                ++i; ++j; s += sc.match; break;
-            case 'X':
-                close_gap();
-                if (i > end_i || j > end_j || i >= text_len || j >= pattern_len || text[i] == pattern[j]) wf = false;
-                ++i; ++j; s += sc.mismatch; break;
+                const int in = std::max(0, std::min(L, std::min(lim_i - i, lim_j - j)));
+                if (in < L) wf = false;
+                const uint8_t* a = reinterpret_cast<const uint8_t*>(text) + i;
+                const uint8_t* b = reinterpret_cast<const uint8_t*>(pattern) + j;
+                uint8_t bad = 0;
+                if (cg[c] == 'M') {
+                    for (int t = 0; t < in; ++t) bad |= uint8_t(a[t] ^ b[t]);
+                    s += L * sc.match;
+                } else {
+                    for (int t = 0; t < in; ++t) bad |= uint8_t(a[t] == b[t]);
+                    s += L * sc.mismatch;
+                }
+                if (bad) wf = false;
+                i += L; j += L;
+                break;
+            }
             case 'I':
                 if (run < 0) close_gap();
-                if (i > end_i || i >= text_len) wf = false;
-                ++i; ++run; break;
+                if (i + L - 1 > end_i || i + L - 1 >= text_len) wf = false;
+                i += L; run += L;
+                break;
             case 'D':
                 if (run > 0) close_gap();
-                if (j > end_j || j >= pattern_len) wf = false;
-                ++j; --run; break;
+                if (j + L - 1 > end_j || j + L - 1 >= pattern_len) wf = false;
+                j += L; run -= L;
+                break;
             default: wf = false; break;
         }
+        c = e;
     }
     close_gap();
     if (i != end_i + 1 || j != end_j + 1) wf = false;
@@ -1770,6 +1811,19 @@ static int sw_common(const ga_sw_req* reqs, int64_t n, ga_sw_res* res, char* are
     for (int64_t i = 0; i < n; ++i) rc += res[i].resolved ? 1 : 0;
     if (resolved_count) *resolved_count = rc;
     if (unresolved_count) *unresolved_count = int32_t(n) - rc;
+    return GA_OK;
+}
+
+int ga_sw_validate_cigar(const char* cigar, int32_t cigar_len, const char* text, int32_t text_len,
+                         int32_t start_i, int32_t end_i, const char* pattern, int32_t pattern_len,
+                         int32_t start_j, int32_t end_j, const ga_sw_scoring* scoring, int32_t score,
+                         int32_t* rescore_ok, int32_t* wellformed_ok)
+{
+    if (!scoring || !rescore_ok || !wellformed_ok || cigar_len < 0 || text_len < 0 || pattern_len < 0 ||
+        (cigar_len > 0 && !cigar) || (text_len > 0 && !text) || (pattern_len > 0 && !pattern))
+        return GA_INVALID_ARGUMENT;
+    sw_validate(cigar, cigar_len, text, text_len, start_i, end_i, pattern, pattern_len, start_j, end_j, *scoring,
+                score, rescore_ok, wellformed_ok);
     return GA_OK;
 }
 

@@ -156,6 +156,18 @@ void ga_sw_matrix_blosum62(ga_sw_matrix* out, int32_t gap_open, int32_t gap_exte
 int64_t ga_cigar_arena_bytes_wfa(const ga_align_req* reqs, int64_t n);
 int64_t ga_cigar_arena_bytes_sw(const ga_sw_req* reqs, int64_t n);
 
+/* The CIGAR check ga_align_sw_batch applies to each result (the reference's
+ * sw_validate, src/api/api.cpp:114-156): wellformed_ok when every op stays in
+ * [start, end] and inside both sequences, M pairs equal and X unequal bytes,
+ * and the walk ends at (end_i, end_j); rescore_ok when also the affine
+ * re-score (a gap run of L costs gap_open + (L-1) gap_extend) equals score.
+ * EXTENSION (static in the reference), for callers that check CIGARs of their
+ * own.  GA_INVALID_ARGUMENT for a null pointer with a non-zero length. */
+int ga_sw_validate_cigar(const char* cigar, int32_t cigar_len, const char* text, int32_t text_len,
+                         int32_t start_i, int32_t end_i, const char* pattern, int32_t pattern_len,
+                         int32_t start_j, int32_t end_j, const ga_sw_scoring* scoring, int32_t score,
+                         int32_t* rescore_ok, int32_t* wellformed_ok);
+
 /* ---- device-resident batches (EXTENSION: no reference counterpart) ---------
  * For callers whose sequences already sit in device memory (the benchmark,
  * the request loop).  All pointers are device pointers.  Sequences are packed

@@ -318,3 +318,56 @@ int main(int argc, char** argv) {
     assert out[4] == "0 cannot open '%s'" % missing
     assert out[5] == "1 1 x"
 
+
+
+def test_sw_validate_cigar_matches_oracle(oracle):
+    """The library's CIGAR check (ga_sw_validate_cigar, the one the batch
+    applies) against the oracle's restatement of the reference validator, on
+    traced CIGARs and on mutations of them: ops flipped, inserted, dropped or
+    replaced by a foreign byte, runs cut, ends and starts moved, the score
+    changed, spans past the sequences."""
+    import random
+    from genoaligner_amd import _native as N
+    L = N.lib()
+    rng = random.Random(41)
+    ok_r, ok_w = C.c_int32(), C.c_int32()
+    seen = set()
+    for it in range(2500):
+        sc = rng.choice([(2, -3, 3, 1), (1, -1, 2, 1), (2, -5, 2, 1), (3, -6, 4, 2)])
+        t = "".join(rng.choice("ACGT") for _ in range(rng.randint(1, 90)))
+        a = rng.randint(0, len(t) - 1)
+        p = "".join(c if rng.random() > 0.12 else rng.choice("ACGT") for c in t[a:a + rng.randint(1, 70)])
+        if rng.random() < 0.3:
+            p = p[:len(p) // 2] + "".join(rng.choice("ACGT") for _ in range(rng.randint(1, 6))) + p[len(p) // 2:]
+        score, si, sj, ei, ej, cig = oracle.sw_trace(t, p, sc)
+        cig = list(cig)
+        k = rng.randrange(9)
+        if k == 1 and cig:
+            cig[rng.randrange(len(cig))] = rng.choice("MXID")
+        elif k == 2:
+            cig.insert(rng.randint(0, len(cig)), rng.choice("MXID"))
+        elif k == 3 and cig:
+            del cig[rng.randrange(len(cig))]
+        elif k == 4 and cig:
+            cig[rng.randrange(len(cig))] = rng.choice("Z=\0m")
+        elif k == 5:
+            ei, ej = ei + rng.randint(-2, 2), ej + rng.randint(-2, 2)
+        elif k == 6:
+            si, sj = si + rng.randint(-1, 2), sj + rng.randint(-1, 2)
+        elif k == 7:
+            score += rng.choice([-1, 1])
+        elif k == 8:
+            cig += ["M"] * rng.randint(1, 20)            # past both sequences
+        cg = "".join(cig)
+        if min(si, sj) < 0:
+            continue                                     # negative starts are no SW result
+        want = oracle.sw_validate(cg, t, si, ei, p, sj, ej, sc, score)
+        scc = N.SWScoringC(*sc)
+        assert L.ga_sw_validate_cigar(cg.encode("latin-1"), len(cg), t.encode(), len(t), si, ei, p.encode(),
+                                      len(p), sj, ej, C.byref(scc), score, C.byref(ok_r), C.byref(ok_w)) == 0
+        got = (bool(ok_r.value), bool(ok_w.value))
+        assert got == tuple(bool(x) for x in want), (it, k, cg, t, p, si, ei, sj, ej, score)
+        seen.add(got)
+    assert seen == {(True, True), (False, True), (False, False)}
+    assert L.ga_sw_validate_cigar(None, 3, b"A", 1, 0, 0, b"A", 1, 0, 0, C.byref(N.SWScoringC(1, -1, 1, 1)), 1,
+                                  C.byref(ok_r), C.byref(ok_w)) == N.GA_INVALID_ARGUMENT
