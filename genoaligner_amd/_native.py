@@ -25,7 +25,7 @@ EXPORTS = (
     "ga_align_batch", "ga_align_batch_ext", "ga_align_sw_batch",
     "ga_backend_name", "ga_device_name", "ga_device_available",
     "ga_cigar_arena_bytes_wfa", "ga_cigar_arena_bytes_sw", "ga_sw_validate_cigar",
-    "ga_wfa_device", "ga_wfa_device_noalloc", "ga_sw_device", "ga_set_kernel_timing", "ga_last_kernel_ms",
+    "ga_wfa_device", "ga_wfa_device_noalloc", "ga_sw_device", "ga_sw_validate_device", "ga_set_kernel_timing", "ga_last_kernel_ms",
     "ga_sketch_device", "ga_mlp_forward_device", "ga_mlp_logits_device", "ga_mlp_backward_device", "ga_mlp_sgd_device",
     "ga_mlp_route_device", "ga_route_bucket_bound", "ga_wfa_routed_device", "ga_router_set", "ga_router_enabled",
     "ga_align_sw_matrix_batch", "ga_sw_matrix_blosum62", "ga_set_thread_workspace_budget",
@@ -169,6 +169,8 @@ def lib() -> C.CDLL:
         L.ga_wfa_device_noalloc.restype = C.c_int
         L.ga_sw_device.argtypes = [C.POINTER(SwDevBatch), C.c_void_p, cpp]
         L.ga_sw_device.restype = C.c_int
+        L.ga_sw_validate_device.argtypes = [C.POINTER(SwDevBatch), C.c_void_p, C.c_void_p, cpp]
+        L.ga_sw_validate_device.restype = C.c_int
         L.ga_set_kernel_timing.argtypes = [C.c_int]
         L.ga_set_kernel_timing.restype = None
         L.ga_last_kernel_ms.argtypes = []

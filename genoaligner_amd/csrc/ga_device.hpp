@@ -243,6 +243,26 @@ int sw16m_bias(const int32_t* sub32x32, int gap_open);   // -1: some biased scor
 int64_t sw16m_dir_words_per_group(int max_m, int max_n);
 hipError_t sw16m_launch(const Sw16Launch& L, bool trace, hipStream_t st, LaunchHooks* hooks = nullptr);
 
+// CIGAR checks on the device (the reference's sw_validate, api.cpp:114-156,
+// one wave per pair): pair k's fields are read at score[k * stride] etc.,
+// its CIGAR at cigar + cig_off[k] + text_len[k] + pattern_len[k] - len (the
+// right-aligned slot every SW path writes).  flags[k] = rescore_ok |
+// wellformed_ok << 1 for pairs with score > 0, len > 0 and walk[k] (walk may
+// be null: all), 0 for the others.
+struct SwValidate {
+    const uint8_t* txt; const int64_t* toff; const int32_t* tlen;
+    const uint8_t* pat; const int64_t* poff; const int32_t* plen;
+    const uint8_t* cigar; const int64_t* cig_off;
+    const int32_t* score; const int32_t* si; const int32_t* sj; const int32_t* ei; const int32_t* ej;
+    int stride;                                   // ints between one pair's fields and the next's
+    const int32_t* len; int len_stride;
+    const uint8_t* walk;
+    uint8_t* flags;
+    int64_t n;
+    int match, mismatch, go, ge;
+};
+hipError_t sw_validate_launch(const SwValidate& V, hipStream_t st);
+
 // Reduce the 8-int per-pair records to the device-resident result layout:
 // out5 = score, start_i, start_j, end_i, end_j (trace: row-major end and span;
 // score-only: the wave64 lane-rule end), cig_len = CIGAR length (may be null).

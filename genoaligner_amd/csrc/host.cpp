@@ -1283,7 +1283,8 @@ hipError_t enqueue_sw16_chunk(const ga::Sw16Launch& P16, bool trace, hipStream_t
 const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>& skip,
                    bool trace, const std::vector<uint8_t>& walk, char* arena,
                    std::vector<int32_t>& res8, std::vector<int64_t>& slot_off,
-                   PinnedHold& pin, const char** cig_src, const SwMatrixHost* mat = nullptr)
+                   PinnedHold& pin, const char** cig_src, const SwMatrixHost* mat = nullptr,
+                   std::vector<uint8_t>* vflags = nullptr)
 {
     *cig_src = arena;
     std::call_once(g_pool_once, init_pool);
@@ -1589,16 +1590,35 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
         if (sq[1]) release_stream(sq[1]);
     }
 
-    // results and CIGAR slots back into pinned staging (pageable fallback:
-    // the caller's arena and a vector)
+    // the CIGAR checks on the device (DNA trace batches), once every walk
+    // has written its CIGAR
+    uint8_t* d_vf = nullptr;
+    if (vflags && trace && !mat && !err) {
+        d_vf = D.alloc<uint8_t>(size_t(n));
+        ga::SwValidate V{};
+        V.txt = d_txt; V.toff = d_toff; V.tlen = d_tl;
+        V.pat = d_pat; V.poff = d_poff; V.plen = d_pl;
+        V.cigar = d_cig; V.cig_off = d_slot;
+        V.score = d_res; V.ei = d_res + 1; V.ej = d_res + 2; V.si = d_res + 5; V.sj = d_res + 6; V.stride = 8;
+        V.len = d_res + 7; V.len_stride = 8;
+        V.walk = d_walk; V.flags = d_vf; V.n = n;
+        V.match = sc.match; V.mismatch = sc.mismatch; V.go = sc.gap_open; V.ge = sc.gap_extend;
+        if (!D.ok || ga::sw_validate_launch(V, st) != hipSuccess) err = "SW validate launch";
+    }
+    // results, flags and CIGAR slots back into pinned staging (pageable
+    // fallback: the caller's arena and vectors)
     std::vector<int32_t> r8v;
     int32_t* r8p = reinterpret_cast<int32_t*>(pin.get(size_t(n) * 32));
     if (!r8p) { r8v.assign(static_cast<size_t>(n) * 8, -1); r8p = r8v.data(); }
+    std::vector<uint8_t> vfv;
+    uint8_t* vfp = d_vf ? pin.get(size_t(n)) : nullptr;
+    if (d_vf && !vfp) { vfv.assign(static_cast<size_t>(n), 0); vfp = vfv.data(); }
     char* cdst = arena;
     if (trace && slot > 0)
         if (uint8_t* c = pin.get(size_t(slot))) cdst = reinterpret_cast<char*>(c);
     if (!err) {
         if (hipMemcpyAsync(r8p, d_res, size_t(n) * 32, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            (d_vf && hipMemcpyAsync(vfp, d_vf, size_t(n), hipMemcpyDeviceToHost, st) != hipSuccess) ||
             (trace && slot > 0 &&
              hipMemcpyAsync(cdst, d_cig, size_t(slot), hipMemcpyDeviceToHost, st) != hipSuccess))
             err = "hipMemcpy(results)";
@@ -1609,9 +1629,12 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
     release_stream(st);
     *cig_src = cdst;
     res8.resize(static_cast<size_t>(n) * 8);
+    if (d_vf) vflags->resize(static_cast<size_t>(n));
     parallel_for(n, [&](int64_t k) {
         std::copy(r8p + k * 8, r8p + k * 8 + 8, res8.begin() + order[size_t(k)] * 8);
+        if (d_vf) (*vflags)[size_t(order[size_t(k)])] = vfp[k];
     });
+    if (vflags && !d_vf) vflags->clear();
     return err;
 }
 
@@ -1767,7 +1790,9 @@ static int sw_common(const ga_sw_req* reqs, int64_t n, ga_sw_res* res, char* are
     std::vector<int64_t> slot_off;
     PinnedHold pin;
     const char* cig_src = arena;
-    const char* err = run_sw(reqs, n, skip, any_cigar, walk, arena, r8, slot_off, pin, &cig_src, mat);
+    std::vector<uint8_t> vflags;                  // the device's CIGAR checks (DNA batches)
+    const char* err = run_sw(reqs, n, skip, any_cigar, walk, arena, r8, slot_off, pin, &cig_src, mat, &vflags);
+    const bool dev_checks = !vflags.empty();
     if (err) {
         if (error) *error = err;
         reset_sw(res, n);
@@ -1797,7 +1822,10 @@ static int sw_common(const ga_sw_req* reqs, int64_t n, ga_sw_res* res, char* are
             o.cigar_off = slot_off[size_t(i)] + slot_len - len;
             o.cigar_len = len;
             if (cig_src != arena) std::memcpy(arena + o.cigar_off, cig_src + o.cigar_off, size_t(len));
-            if (mat)
+            if (dev_checks) {
+                o.rescore_ok = vflags[size_t(i)] & 1;
+                o.wellformed_ok = (vflags[size_t(i)] >> 1) & 1;
+            } else if (mat)
                 sw_validate_matrix(arena + o.cigar_off, len, r.text, r.text_len, o.start_i, o.end_i,
                                    r.pattern, r.pattern_len, o.start_j, o.end_j, *mat, o.score,
                                    &o.rescore_ok, &o.wellformed_ok);
@@ -2072,6 +2100,33 @@ int ga_wfa_graph_launch(ga_wfa_graph* g, void* stream, const char** error)
     if (!g || !g->exec) { if (error) *error = "null graph"; return GA_INVALID_ARGUMENT; }
     if (hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)) != hipSuccess) {
         if (error) *error = "hipGraphLaunch";
+        return GA_DEVICE_ERROR;
+    }
+    return GA_OK;
+}
+
+int ga_sw_validate_device(const ga_sw_dev_batch* b, uint8_t* flags, void* stream, const char** error)
+{
+    if (error) *error = nullptr;
+    auto refuse = [&](const char* m) { if (error) *error = m; return GA_INVALID_ARGUMENT; };
+    if (!b) return refuse("null batch");
+    if (b->n_pairs <= 0) return GA_OK;
+    if (!b->with_cigar) return refuse("the batch holds no CIGARs (with_cigar 0)");
+    if (const char* e = sw_scheme_error(b->scoring, true)) return refuse(e);
+    if (!flags || !b->out || !b->cigar || !b->cigar_off || !b->cigar_len || !b->text || !b->text_off ||
+        !b->text_len || !b->pattern || !b->pattern_off || !b->pattern_len)
+        return refuse("null device pointer");
+    ga::SwValidate V{};
+    V.txt = b->text; V.toff = b->text_off; V.tlen = b->text_len;
+    V.pat = b->pattern; V.poff = b->pattern_off; V.plen = b->pattern_len;
+    V.cigar = b->cigar; V.cig_off = b->cigar_off;
+    V.score = b->out; V.si = b->out + 1; V.sj = b->out + 2; V.ei = b->out + 3; V.ej = b->out + 4; V.stride = 5;
+    V.len = b->cigar_len; V.len_stride = 1;
+    V.walk = nullptr; V.flags = flags; V.n = b->n_pairs;
+    V.match = b->scoring.match; V.mismatch = b->scoring.mismatch;
+    V.go = b->scoring.gap_open; V.ge = b->scoring.gap_extend;
+    if (ga::sw_validate_launch(V, static_cast<hipStream_t>(stream)) != hipSuccess) {
+        if (error) *error = "SW validate launch";
         return GA_DEVICE_ERROR;
     }
     return GA_OK;

@@ -32,6 +32,7 @@
 // lanes holding the maximum.
 #include "ga_device.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace ga {
@@ -408,7 +409,80 @@ __global__ __launch_bounds__(256) void sw_gather_kernel(const int32_t* res8, int
     if (cig_len) cig_len[i] = trace ? q[7] : 0;
 }
 
+// One wave per pair, 64 CIGAR ops per step.  Lane l takes op c0 + l; its
+// cell (i, j) is the carried position plus the text and pattern advances of
+// the ops before it in the step (ballot + mbcnt), so every op's range and
+// byte tests run at once.  The re-score counts the step's M, X and gap ops
+// and its gap-run starts (a gap op whose predecessor is another op): a run
+// of L gap ops costs go + (L-1) ge = L ge + (go - ge).  The host validator
+// keeps a run open across a foreign byte ("IZI" is one run there, two here);
+// such a CIGAR is not well formed either way, so both flags agree.
+__global__ __launch_bounds__(256) void sw_validate_kernel(SwValidate V)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t(blockIdx.x) * 256 + threadIdx.x) >> 6;
+    const int64_t nwaves = int64_t(gridDim.x) * 4;
+    for (int64_t k = wave; k < V.n; k += nwaves) {
+        const int64_t q = k * V.stride;
+        const int score = V.score[q], len = V.len[k * V.len_stride];
+        if (score <= 0 || len <= 0 || (V.walk && !V.walk[k])) {
+            if (lane == 0) V.flags[k] = 0;
+            continue;
+        }
+        const int m = V.tlen[k], n = V.plen[k];
+        if (len > m + n) {                                // no such slot: not well formed
+            if (lane == 0) V.flags[k] = 0;
+            continue;
+        }
+        const int end_i = V.ei[q], end_j = V.ej[q];
+        const uint8_t* cg = V.cigar + V.cig_off[k] + (m + n - len);
+        const uint8_t* T = V.txt + V.toff[k];
+        const uint8_t* P = V.pat + V.poff[k];
+        const int lim_i = end_i + 1 < m ? end_i + 1 : m, lim_j = end_j + 1 < n ? end_j + 1 : n;
+        int i0 = V.si[q], j0 = V.sj[q];
+        int s = 0;
+        bool bad = false;
+        uint8_t prev = 0;                                 // op before the step
+        for (int c0 = 0; c0 < len; c0 += 64) {
+            const bool live = c0 + lane < len;
+            const uint8_t op = live ? cg[c0 + lane] : uint8_t(0);
+            const bool isM = op == 'M', isX = op == 'X', isI = op == 'I', isD = op == 'D';
+            const uint64_t bi = __ballot(isM || isX || isI), bj = __ballot(isM || isX || isD);
+            const int di = int(__builtin_amdgcn_mbcnt_hi(uint32_t(bi >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(bi), 0u)));
+            const int dj = int(__builtin_amdgcn_mbcnt_hi(uint32_t(bj >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(bj), 0u)));
+            const int i = i0 + di, j = j0 + dj;
+            bool b = live && !(isM || isX || isI || isD);
+            if (isM || isX) {
+                if (i >= 0 && j >= 0 && i < lim_i && j < lim_j) b |= isM ? T[i] != P[j] : T[i] == P[j];
+                else b = true;                            // (a negative start is no SW result)
+            }
+            if (isI) b |= i >= lim_i;
+            if (isD) b |= j >= lim_j;
+            bad |= __any(b) != 0;
+            // predecessor op: the lane below, or the last op of the step before
+            const uint8_t before = uint8_t(__shfl_up(int(op), 1));
+            const uint8_t pr = lane == 0 ? prev : before;
+            const bool start = (isI || isD) && pr != op;
+            s += int(__popcll(__ballot(isM))) * V.match + int(__popcll(__ballot(isX))) * V.mismatch -
+                 int(__popcll(__ballot(isI || isD))) * V.ge - int(__popcll(__ballot(start))) * (V.go - V.ge);
+            i0 += int(__popcll(bi));
+            j0 += int(__popcll(bj));
+            prev = uint8_t(__shfl(int(op), 63));
+        }
+        const bool wf = !bad && i0 == end_i + 1 && j0 == end_j + 1;
+        if (lane == 0) V.flags[k] = uint8_t((wf && s == score ? 1 : 0) | (wf ? 2 : 0));
+    }
+}
+
 }  // namespace
+
+hipError_t sw_validate_launch(const SwValidate& V, hipStream_t st)
+{
+    if (V.n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((V.n + 3) / 4, 8192);
+    hipLaunchKernelGGL(sw_validate_kernel, dim3(unsigned(blocks)), dim3(256), 0, st, V);
+    return hipGetLastError();
+}
 
 hipError_t sw_gather(const int32_t* res8, int64_t n, bool trace, int32_t* out5, int32_t* cig_len,
                      hipStream_t st)

@@ -205,6 +205,20 @@ class SwDeviceBatch:
         _check(N.lib().ga_sw_device(C.byref(self.b), C.c_void_p(_stream_handle(stream)),
                                     C.byref(err)), err, "ga_sw_device")
 
+    def validate(self, stream: Optional[torch.cuda.Stream] = None) -> np.ndarray:
+        """The CIGAR checks on the device (ga_sw_validate_device) over the
+        batch's current results: per pair rescore_ok | wellformed_ok << 1 (0 for
+        pairs with score 0)."""
+        if not self.trace:
+            raise ValueError("a score-only batch has no CIGARs to check")
+        flags = torch.empty(max(self.n, 1), dtype=torch.uint8, device=self.out.device)
+        err = C.c_char_p()
+        _check(N.lib().ga_sw_validate_device(C.byref(self.b), C.c_void_p(flags.data_ptr()),
+                                             C.c_void_p(_stream_handle(stream)), C.byref(err)),
+               err, "ga_sw_validate_device")
+        torch.cuda.synchronize()
+        return flags[:self.n].cpu().numpy()
+
     def results(self):
         """(out[n,5] = score, start_i, start_j, end_i, end_j; cigars or None)."""
         torch.cuda.synchronize()

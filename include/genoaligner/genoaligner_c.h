@@ -326,6 +326,14 @@ typedef struct ga_sw_dev_batch {
 } ga_sw_dev_batch;
 
 int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error);
+/* ga_sw_validate_cigar on the device for a batch ga_sw_device has filled
+ * (with_cigar 1): pair k's CIGAR is the cigar_len[k] bytes ending its slot
+ * (cigar + cigar_off[k] + text_len[k] + pattern_len[k]), checked against
+ * out[5k .. 5k+5) and the sequences; flags[k] (device, n_pairs bytes) =
+ * rescore_ok | wellformed_ok << 1 for pairs with score > 0 and a CIGAR, 0
+ * for the others.  The checks ga_align_sw_batch runs on its own DNA results.
+ * Enqueued on `stream`.  EXTENSION: no reference counterpart. */
+int ga_sw_validate_device(const ga_sw_dev_batch* b, uint8_t* flags, void* stream, const char** error);
 
 /* ---- kernel timing (EXTENSION: measurement support) ------------------------
  * When enabled (per calling thread), the batch entry points (ga_align_batch,

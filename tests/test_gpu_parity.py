@@ -811,3 +811,73 @@ def test_wfa_one_class_one_length_bin(gpu, oracle):
     assert r.ok()
     assert [(x.score, x.cigar) for x in r.results[:300]] == exp[:300]
     assert (r.results[300].score, r.results[300].cigar) == oracle.wfa("ACGT" * 300, "ACGT" * 290 + "AGGT" * 10, 64)
+
+
+def test_sw_validate_device_matches_host_checks(gpu, oracle):
+    """ga_sw_validate_device (the CIGAR checks ga_align_sw_batch now runs on
+    the device) against the host's ga_sw_validate_cigar, on a device batch's
+    own results and on mutations of them written back into its buffers: ops
+    flipped, inserted, dropped or foreign, ends and starts moved, scores
+    changed, CIGARs running past both sequences."""
+    import ctypes as C
+    import torch
+    from genoaligner_amd import _native as N
+    from genoaligner_amd import workloads as W
+    from genoaligner_amd.device import SwDeviceBatch
+    sc = (2, -3, 3, 1)
+    ps = W.sw_embedded(n=600)
+    dev = SwDeviceBatch(ps, scoring=sc, trace=True)
+    dev.run()
+    out, cig = dev.results()
+    flags = dev.validate()
+    assert all(f == (3 if out[k, 0] > 0 else 0) for k, f in enumerate(flags))
+    rng = random.Random(77)
+    L = N.lib()
+    scc = N.SWScoringC(*sc)
+    out2 = out.copy()
+    raw = bytearray(dev.cigar.cpu().numpy().tobytes())
+    lens = np.zeros(ps.n, np.int32)
+    for k in range(ps.n):
+        c = list(cig[k].decode())
+        score, si, sj, ei, ej = (int(x) for x in out[k])
+        kind = rng.randrange(9)
+        if kind == 1 and c:
+            c[rng.randrange(len(c))] = rng.choice("MXID")
+        elif kind == 2:
+            c.insert(rng.randint(0, len(c)), rng.choice("MXID"))
+        elif kind == 3 and c:
+            del c[rng.randrange(len(c))]
+        elif kind == 4 and c:
+            c[rng.randrange(len(c))] = rng.choice("Z=m")
+        elif kind == 5:
+            ei, ej = ei + rng.randint(-2, 2), ej + rng.randint(-2, 2)
+        elif kind == 6:
+            si, sj = si + rng.randint(0, 2), sj + rng.randint(0, 2)
+        elif kind == 7:
+            score += rng.choice([-1, 1])
+        elif kind == 8:
+            c += ["M"] * rng.randint(1, 20)
+        cg = "".join(c)[: int(ps.a_len[k] + ps.b_len[k])]
+        end = int(dev.slot_off_np[k] + dev.slot[k])
+        raw[end - len(cg): end] = cg.encode("latin-1")
+        lens[k] = len(cg)
+        out2[k] = (score, si, sj, ei, ej)
+    dev.cigar.copy_(torch.frombuffer(raw, dtype=torch.uint8).to(dev.cigar.device))
+    dev.out.copy_(torch.from_numpy(out2.reshape(-1).astype(np.int32)).to(dev.out.device))
+    dev.cig_len.copy_(torch.from_numpy(lens).to(dev.cig_len.device))
+    got = dev.validate()
+    r, w = C.c_int32(), C.c_int32()
+    seen = set()
+    for k in range(ps.n):
+        score, si, sj, ei, ej = (int(x) for x in out2[k])
+        if score <= 0 or lens[k] == 0:
+            assert got[k] == 0, k
+            continue
+        end = int(dev.slot_off_np[k] + dev.slot[k])
+        cg = bytes(raw[end - lens[k]: end])
+        t, p = (x if isinstance(x, bytes) else x.encode() for x in (ps.a_seq(k), ps.b_seq(k)))
+        assert L.ga_sw_validate_cigar(cg, len(cg), t, len(t), si, ei, p, len(p), sj, ej, C.byref(scc), score,
+                                      C.byref(r), C.byref(w)) == 0
+        assert got[k] == (r.value | (w.value << 1)), (k, cg, si, ei, sj, ej, score)
+        seen.add(int(got[k]))
+    assert {0, 2, 3} <= seen and 1 not in seen          # rescore_ok implies wellformed_ok
