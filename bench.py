@@ -396,7 +396,7 @@ def main():
             achieved = SW_OPS_PER_CELL * cells / (kern_ms * 1e-3) / 1e12
             roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_INT16_TOPS,
                     "unit": "TOP/s", "frac": round(achieved / PEAK_INT16_TOPS, 4),
-                    "traffic": traffic, "kernel": "sw16_kernel<true, false, true> (packed forward + fused walk)",
+                    "traffic": traffic, "kernel": "sw16_kernel<true, false, 2> (packed forward + fused walk, H mod 8 trace record)",
                     "kernel_ms": round(kern_ms, 3)}
 
         # score-only on the same pairs: score + end cell (the lane-rule end)
