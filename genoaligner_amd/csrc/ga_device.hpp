@@ -262,6 +262,12 @@ struct SwValidate {
     int match, mismatch, go, ge;
 };
 hipError_t sw_validate_launch(const SwValidate& V, hipStream_t st);
+// Pair k's CIGAR (the coff[k+1] - coff[k] bytes ending its slot at
+// cigar + slot_off[k] + tlen[k] + plen[k]) copied to out + coff[k]: the
+// CIGARs back to back for a short copy to the host.
+hipError_t sw_cigar_compact_launch(const uint8_t* cigar, const int64_t* slot_off, const int32_t* tlen,
+                                   const int32_t* plen, const int64_t* coff, int64_t n, uint8_t* out,
+                                   hipStream_t st);
 
 // Reduce the 8-int per-pair records to the device-resident result layout:
 // out5 = score, start_i, start_j, end_i, end_j (trace: row-major end and span;

@@ -288,6 +288,14 @@ struct Packed {
     template <class Get> int64_t layout(int64_t n, Get get);
     template <class Get> void fill(int64_t n, Get get, uint8_t* dst, int64_t size, const uint8_t* map = nullptr,
                                    uint8_t* code_dst = nullptr) const;
+    // Sequences [k0, k1) of the same layout (and the pads at its ends), and
+    // the byte range they fill: a piece of the buffer to upload on its own.
+    template <class Get> void fill_range(int64_t k0, int64_t k1, int64_t n, Get get, uint8_t* dst,
+                                         int64_t size) const;
+    std::pair<int64_t, int64_t> piece(int64_t k0, int64_t k1, int64_t n, int64_t size) const
+    {
+        return {k0 == 0 ? 0 : off[size_t(k0)], k1 == n ? size : off[size_t(k1)]};
+    }
 };
 
 // A process-wide pool of host worker threads (half the host threads, at most
@@ -428,6 +436,17 @@ int64_t Packed::layout(int64_t n, Get get)
         pos += len > 0 ? len : 0;
     }
     return pos + kSeqPad;
+}
+
+template <class Get>
+void Packed::fill_range(int64_t k0, int64_t k1, int64_t n, Get get, uint8_t* dst, int64_t size) const
+{
+    if (k0 == 0) std::memset(dst, 0, size_t(kSeqPad));
+    if (k1 == n) std::memset(dst + size - kSeqPad, 0, size_t(kSeqPad));
+    parallel_for(k1 - k0, [&](int64_t q) {
+        const auto sq = get(k0 + q);
+        if (sq.second > 0) std::memcpy(dst + off[size_t(k0 + q)], sq.first, size_t(sq.second));
+    }, 16);
 }
 
 template <class Get>
@@ -1284,7 +1303,7 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
                    bool trace, const std::vector<uint8_t>& walk, char* arena,
                    std::vector<int32_t>& res8, std::vector<int64_t>& slot_off,
                    PinnedHold& pin, const char** cig_src, const SwMatrixHost* mat = nullptr,
-                   std::vector<uint8_t>* vflags = nullptr)
+                   std::vector<uint8_t>* vflags = nullptr, std::vector<int64_t>* csrc_off = nullptr)
 {
     *cig_src = arena;
     std::call_once(g_pool_once, init_pool);
@@ -1372,9 +1391,13 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
     const int cm = mat ? 2 : 1;                      // bytes (and codes)
     uint8_t* tpin = pin.get(size_t(tsz) * cm);
     uint8_t* ppin = tpin ? pin.get(size_t(psz) * cm) : nullptr;
+    // large DNA batches are packed and uploaded in pieces below
+    const bool pipe = tpin && ppin && !mat && n >= 4096;
     if (tpin && ppin) {
-        T.fill(n, tget, tpin, tsz, mat ? mat->code : nullptr, mat ? tpin + tsz : nullptr);
-        P.fill(n, pget, ppin, psz, mat ? mat->code : nullptr, mat ? ppin + psz : nullptr);
+        if (!pipe) {
+            T.fill(n, tget, tpin, tsz, mat ? mat->code : nullptr, mat ? tpin + tsz : nullptr);
+            P.fill(n, pget, ppin, psz, mat ? mat->code : nullptr, mat ? ppin + psz : nullptr);
+        }
     } else {
         T.build(n, tget, mat ? mat->code : nullptr, mat ? &tcode : nullptr);
         P.build(n, pget, mat ? mat->code : nullptr, mat ? &pcode : nullptr);
@@ -1386,8 +1409,27 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
     DevArena D;
     D.st = st;
     const char* err = nullptr;
-    auto* d_txt = pinned ? D.upload_raw(tpin, size_t(tsz)) : D.upload(T.bytes);
-    auto* d_pat = pinned ? D.upload_raw(ppin, size_t(psz)) : D.upload(P.bytes);
+    uint8_t* d_txt = nullptr;
+    uint8_t* d_pat = nullptr;
+    if (pipe) {
+        // four pieces: each piece's copy runs while the host packs the next
+        d_txt = D.alloc<uint8_t>(size_t(tsz));
+        d_pat = D.alloc<uint8_t>(size_t(psz));
+        for (int q = 0; q < 4 && D.ok; ++q) {
+            const int64_t k0 = n * q / 4, k1 = n * (q + 1) / 4;
+            T.fill_range(k0, k1, n, tget, tpin, tsz);
+            P.fill_range(k0, k1, n, pget, ppin, psz);
+            const auto tb = T.piece(k0, k1, n, tsz), pb = P.piece(k0, k1, n, psz);
+            if (hipMemcpyAsync(d_txt + tb.first, tpin + tb.first, size_t(tb.second - tb.first),
+                               hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(d_pat + pb.first, ppin + pb.first, size_t(pb.second - pb.first),
+                               hipMemcpyHostToDevice, st) != hipSuccess)
+                D.ok = false;
+        }
+    } else {
+        d_txt = pinned ? D.upload_raw(tpin, size_t(tsz)) : D.upload(T.bytes);
+        d_pat = pinned ? D.upload_raw(ppin, size_t(psz)) : D.upload(P.bytes);
+    }
     const uint8_t* d_tcode = !mat ? nullptr : pinned ? D.upload_raw(tpin + tsz, size_t(tsz)) : D.upload(tcode);
     const uint8_t* d_pcode = !mat ? nullptr : pinned ? D.upload_raw(ppin + psz, size_t(psz)) : D.upload(pcode);
     std::vector<int32_t> subv;
@@ -1614,14 +1656,48 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
     uint8_t* vfp = d_vf ? pin.get(size_t(n)) : nullptr;
     if (d_vf && !vfp) { vfv.assign(static_cast<size_t>(n), 0); vfp = vfv.data(); }
     char* cdst = arena;
-    if (trace && slot > 0)
+    // (DNA, with the device's checks) the CIGARs come back packed: the slots
+    // are sized m + n per pair, the CIGARs a fraction of that
+    const bool compact = d_vf && csrc_off && slot > 0;
+    std::vector<int64_t> coff;                            // compact offsets, launch order
+    if (!compact && trace && slot > 0)
         if (uint8_t* c = pin.get(size_t(slot))) cdst = reinterpret_cast<char*>(c);
     if (!err) {
         if (hipMemcpyAsync(r8p, d_res, size_t(n) * 32, hipMemcpyDeviceToHost, st) != hipSuccess ||
             (d_vf && hipMemcpyAsync(vfp, d_vf, size_t(n), hipMemcpyDeviceToHost, st) != hipSuccess) ||
-            (trace && slot > 0 &&
+            (!compact && trace && slot > 0 &&
              hipMemcpyAsync(cdst, d_cig, size_t(slot), hipMemcpyDeviceToHost, st) != hipSuccess))
             err = "hipMemcpy(results)";
+    }
+    if (compact && !err) {
+        // the lengths from the results' host copy, the offsets uploaded, one
+        // kernel packs the CIGARs, one short copy brings them back
+        const hipError_t e1 = hipStreamSynchronize(st);
+        if (e1 != hipSuccess) err = hipGetErrorString(e1);
+        if (!err) {
+            coff.resize(size_t(n) + 1);
+            int64_t tot = 0;
+            for (int64_t k = 0; k < n; ++k) {
+                coff[size_t(k)] = tot;
+                const int32_t* q = r8p + k * 8;
+                if (walk_p[size_t(k)] && q[0] > 0 && q[7] > 0) tot += q[7];
+            }
+            coff[size_t(n)] = tot;
+            uint8_t* cc = tot > 0 ? pin.get(size_t(tot)) : nullptr;
+            if (tot > 0 && !cc) {
+                // no pinned memory: the slots straight into the caller's arena
+                coff.clear();
+                if (hipMemcpyAsync(arena, d_cig, size_t(slot), hipMemcpyDeviceToHost, st) != hipSuccess)
+                    err = "hipMemcpy(CIGARs)";
+            } else if (tot > 0) {
+                const int64_t* d_coff = D.upload(coff);
+                uint8_t* d_cc = D.alloc<uint8_t>(size_t(tot));
+                if (!D.ok || ga::sw_cigar_compact_launch(d_cig, d_slot, d_tl, d_pl, d_coff, n, d_cc, st) != hipSuccess ||
+                    hipMemcpyAsync(cc, d_cc, size_t(tot), hipMemcpyDeviceToHost, st) != hipSuccess)
+                    err = "hipMemcpy(CIGARs)";
+                cdst = reinterpret_cast<char*>(cc);
+            }
+        }
     }
     D.release();
     const hipError_t se = hipStreamSynchronize(st);
@@ -1630,9 +1706,11 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
     *cig_src = cdst;
     res8.resize(static_cast<size_t>(n) * 8);
     if (d_vf) vflags->resize(static_cast<size_t>(n));
+    if (csrc_off) csrc_off->assign(coff.empty() ? 0 : static_cast<size_t>(n), 0);
     parallel_for(n, [&](int64_t k) {
         std::copy(r8p + k * 8, r8p + k * 8 + 8, res8.begin() + order[size_t(k)] * 8);
         if (d_vf) (*vflags)[size_t(order[size_t(k)])] = vfp[k];
+        if (!coff.empty()) (*csrc_off)[size_t(order[size_t(k)])] = coff[size_t(k)];
     });
     if (vflags && !d_vf) vflags->clear();
     return err;
@@ -1791,7 +1869,9 @@ static int sw_common(const ga_sw_req* reqs, int64_t n, ga_sw_res* res, char* are
     PinnedHold pin;
     const char* cig_src = arena;
     std::vector<uint8_t> vflags;                  // the device's CIGAR checks (DNA batches)
-    const char* err = run_sw(reqs, n, skip, any_cigar, walk, arena, r8, slot_off, pin, &cig_src, mat, &vflags);
+    std::vector<int64_t> csrc;                    // where each CIGAR sits in cig_src (packed), if so
+    const char* err = run_sw(reqs, n, skip, any_cigar, walk, arena, r8, slot_off, pin, &cig_src, mat, &vflags,
+                             &csrc);
     const bool dev_checks = !vflags.empty();
     if (err) {
         if (error) *error = err;
@@ -1821,7 +1901,8 @@ static int sw_common(const ga_sw_req* reqs, int64_t n, ga_sw_res* res, char* are
             const int slot_len = std::max(r.text_len, 0) + std::max(r.pattern_len, 0);
             o.cigar_off = slot_off[size_t(i)] + slot_len - len;
             o.cigar_len = len;
-            if (cig_src != arena) std::memcpy(arena + o.cigar_off, cig_src + o.cigar_off, size_t(len));
+            if (cig_src != arena)
+                std::memcpy(arena + o.cigar_off, cig_src + (csrc.empty() ? o.cigar_off : csrc[size_t(i)]), size_t(len));
             if (dev_checks) {
                 o.rescore_ok = vflags[size_t(i)] & 1;
                 o.wellformed_ok = (vflags[size_t(i)] >> 1) & 1;

@@ -474,7 +474,35 @@ __global__ __launch_bounds__(256) void sw_validate_kernel(SwValidate V)
     }
 }
 
+// One wave per pair, one byte per lane per step.
+__global__ __launch_bounds__(256) void sw_cigar_compact_kernel(const uint8_t* cigar, const int64_t* slot_off,
+                                                               const int32_t* tlen, const int32_t* plen,
+                                                               const int64_t* coff, int64_t n, uint8_t* out)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t(blockIdx.x) * 256 + threadIdx.x) >> 6;
+    const int64_t nwaves = int64_t(gridDim.x) * 4;
+    for (int64_t k = wave; k < n; k += nwaves) {
+        const int64_t len = coff[k + 1] - coff[k];
+        if (len <= 0) continue;
+        const uint8_t* src = cigar + slot_off[k] + tlen[k] + plen[k] - len;
+        uint8_t* dst = out + coff[k];
+        for (int64_t q = lane; q < len; q += 64) dst[q] = src[q];
+    }
+}
+
 }  // namespace
+
+hipError_t sw_cigar_compact_launch(const uint8_t* cigar, const int64_t* slot_off, const int32_t* tlen,
+                                   const int32_t* plen, const int64_t* coff, int64_t n, uint8_t* out,
+                                   hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 3) / 4, 8192);
+    hipLaunchKernelGGL(sw_cigar_compact_kernel, dim3(unsigned(blocks)), dim3(256), 0, st, cigar, slot_off, tlen,
+                       plen, coff, n, out);
+    return hipGetLastError();
+}
 
 hipError_t sw_validate_launch(const SwValidate& V, hipStream_t st)
 {
