@@ -260,6 +260,9 @@ struct SwValidate {
     uint8_t* flags;
     int64_t n;
     int match, mismatch, go, ge;
+    // substitution-matrix batches: M and X score sub[tcode * 32 + pcode]
+    // (residue codes laid out like the bytes) instead of match / mismatch
+    const uint8_t* tcode; const uint8_t* pcode; const int32_t* sub;
 };
 hipError_t sw_validate_launch(const SwValidate& V, hipStream_t st);
 // Pair k's CIGAR (the coff[k+1] - coff[k] bytes ending its slot at

@@ -1215,49 +1215,6 @@ struct SwMatrixHost {
     }
 };
 
-// rescore_ok / wellformed_ok of a CIGAR under a substitution matrix: the same
-// walk as sw_validate (affine runs: go + (L-1) ge), M/X scored by the table and
-// still required on equal / unequal bytes.
-void sw_validate_matrix(const char* cg, int len, const char* text, int text_len, int start_i, int end_i,
-                        const char* pattern, int pattern_len, int start_j, int end_j,
-                        const SwMatrixHost& mt, int score, int32_t* rescore_ok, int32_t* wellformed_ok)
-{
-    int i = start_i, j = start_j, s = 0, run = 0;
-    bool wf = true;
-    auto close_gap = [&] {
-        if (run != 0) {
-            const int L = run > 0 ? run : -run;
-            s -= mt.gap_open + (L - 1) * mt.gap_extend;
-            run = 0;
-        }
-    };
-    for (int c = 0; c < len; ++c) {
-        const bool in = i <= end_i && j <= end_j && i < text_len && j < pattern_len;
-        switch (cg[c]) {
-            case 'M':
-            case 'X':
-                close_gap();
-                if (!in || ((text[i] == pattern[j]) != (cg[c] == 'M'))) wf = false;
-                if (in) s += mt.score(uint8_t(text[i]), uint8_t(pattern[j]));
-                ++i; ++j; break;
-            case 'I':
-                if (run < 0) close_gap();
-                if (i > end_i || i >= text_len) wf = false;
-                ++i; ++run; break;
-            case 'D':
-                if (run > 0) close_gap();
-                if (j > end_j || j >= pattern_len) wf = false;
-                ++j; --run; break;
-            default: wf = false; break;
-        }
-    }
-    close_gap();
-    if (i != end_i + 1 || j != end_j + 1) wf = false;
-    *wellformed_ok = wf;
-    *rescore_ok = wf && (s == score);
-}
-
-
 const char* sw_scheme_error(const ga_sw_scoring& s, bool with_cigar)
 {
     if (s.match <= 0) return "SW scoring: match must be > 0 (the score is 0 for every input otherwise)";
@@ -1632,10 +1589,10 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
         if (sq[1]) release_stream(sq[1]);
     }
 
-    // the CIGAR checks on the device (DNA trace batches), once every walk
+    // the CIGAR checks on the device (trace batches), once every walk
     // has written its CIGAR
     uint8_t* d_vf = nullptr;
-    if (vflags && trace && !mat && !err) {
+    if (vflags && trace && !err) {
         d_vf = D.alloc<uint8_t>(size_t(n));
         ga::SwValidate V{};
         V.txt = d_txt; V.toff = d_toff; V.tlen = d_tl;
@@ -1645,6 +1602,10 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
         V.len = d_res + 7; V.len_stride = 8;
         V.walk = d_walk; V.flags = d_vf; V.n = n;
         V.match = sc.match; V.mismatch = sc.mismatch; V.go = sc.gap_open; V.ge = sc.gap_extend;
+        if (mat) {
+            V.go = mat->gap_open; V.ge = mat->gap_extend;
+            V.tcode = d_tcode; V.pcode = d_pcode; V.sub = d_sub;
+        }
         if (!D.ok || ga::sw_validate_launch(V, st) != hipSuccess) err = "SW validate launch";
     }
     // results, flags and CIGAR slots back into pinned staging (pageable
@@ -1656,7 +1617,7 @@ const char* run_sw(const ga_sw_req* reqs, int64_t n, const std::vector<uint8_t>&
     uint8_t* vfp = d_vf ? pin.get(size_t(n)) : nullptr;
     if (d_vf && !vfp) { vfv.assign(static_cast<size_t>(n), 0); vfp = vfv.data(); }
     char* cdst = arena;
-    // (DNA, with the device's checks) the CIGARs come back packed: the slots
+    // (with the device's checks) the CIGARs come back packed: the slots
     // are sized m + n per pair, the CIGARs a fraction of that
     const bool compact = d_vf && csrc_off && slot > 0;
     std::vector<int64_t> coff;                            // compact offsets, launch order
@@ -1903,17 +1864,12 @@ static int sw_common(const ga_sw_req* reqs, int64_t n, ga_sw_res* res, char* are
             o.cigar_len = len;
             if (cig_src != arena)
                 std::memcpy(arena + o.cigar_off, cig_src + (csrc.empty() ? o.cigar_off : csrc[size_t(i)]), size_t(len));
+            // the device's checks (every CIGAR batch runs them; without
+            // their flags both stay 0)
             if (dev_checks) {
                 o.rescore_ok = vflags[size_t(i)] & 1;
                 o.wellformed_ok = (vflags[size_t(i)] >> 1) & 1;
-            } else if (mat)
-                sw_validate_matrix(arena + o.cigar_off, len, r.text, r.text_len, o.start_i, o.end_i,
-                                   r.pattern, r.pattern_len, o.start_j, o.end_j, *mat, o.score,
-                                   &o.rescore_ok, &o.wellformed_ok);
-            else
-                sw_validate(arena + o.cigar_off, len, r.text, r.text_len, o.start_i, o.end_i,
-                            r.pattern, r.pattern_len, o.start_j, o.end_j, r.scoring, o.score,
-                            &o.rescore_ok, &o.wellformed_ok);
+            }
         }
     });
     int32_t rc = 0;

@@ -438,6 +438,8 @@ __global__ __launch_bounds__(256) void sw_validate_kernel(SwValidate V)
         const uint8_t* cg = V.cigar + V.cig_off[k] + (m + n - len);
         const uint8_t* T = V.txt + V.toff[k];
         const uint8_t* P = V.pat + V.poff[k];
+        const uint8_t* TC = V.sub ? V.tcode + V.toff[k] : nullptr;
+        const uint8_t* PC = V.sub ? V.pcode + V.poff[k] : nullptr;
         const int lim_i = end_i + 1 < m ? end_i + 1 : m, lim_j = end_j + 1 < n ? end_j + 1 : n;
         int i0 = V.si[q], j0 = V.sj[q];
         int s = 0;
@@ -452,9 +454,14 @@ __global__ __launch_bounds__(256) void sw_validate_kernel(SwValidate V)
             const int dj = int(__builtin_amdgcn_mbcnt_hi(uint32_t(bj >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(bj), 0u)));
             const int i = i0 + di, j = j0 + dj;
             bool b = live && !(isM || isX || isI || isD);
+            int sub_s = 0;                                // (matrix) this op's table score
             if (isM || isX) {
-                if (i >= 0 && j >= 0 && i < lim_i && j < lim_j) b |= isM ? T[i] != P[j] : T[i] == P[j];
-                else b = true;                            // (a negative start is no SW result)
+                if (i >= 0 && j >= 0 && i < lim_i && j < lim_j) {
+                    b |= isM ? T[i] != P[j] : T[i] == P[j];
+                    if (V.sub) sub_s = V.sub[int(TC[i]) * 32 + int(PC[j])];
+                } else {
+                    b = true;                             // (a negative start is no SW result)
+                }
             }
             if (isI) b |= i >= lim_i;
             if (isD) b |= j >= lim_j;
@@ -463,8 +470,13 @@ __global__ __launch_bounds__(256) void sw_validate_kernel(SwValidate V)
             const uint8_t before = uint8_t(__shfl_up(int(op), 1));
             const uint8_t pr = lane == 0 ? prev : before;
             const bool start = (isI || isD) && pr != op;
-            s += int(__popcll(__ballot(isM))) * V.match + int(__popcll(__ballot(isX))) * V.mismatch -
-                 int(__popcll(__ballot(isI || isD))) * V.ge - int(__popcll(__ballot(start))) * (V.go - V.ge);
+            if (V.sub) {
+                for (int off = 32; off > 0; off >>= 1) sub_s += __shfl_xor(sub_s, off);
+                s += sub_s;
+            } else {
+                s += int(__popcll(__ballot(isM))) * V.match + int(__popcll(__ballot(isX))) * V.mismatch;
+            }
+            s -= int(__popcll(__ballot(isI || isD))) * V.ge + int(__popcll(__ballot(start))) * (V.go - V.ge);
             i0 += int(__popcll(bi));
             j0 += int(__popcll(bj));
             prev = uint8_t(__shfl(int(op), 63));

@@ -331,7 +331,8 @@ int ga_sw_device(const ga_sw_dev_batch* b, void* stream, const char** error);
  * (cigar + cigar_off[k] + text_len[k] + pattern_len[k]), checked against
  * out[5k .. 5k+5) and the sequences; flags[k] (device, n_pairs bytes) =
  * rescore_ok | wellformed_ok << 1 for pairs with score > 0 and a CIGAR, 0
- * for the others.  The checks ga_align_sw_batch runs on its own DNA results.
+ * for the others.  The checks ga_align_sw_batch (and, with the table,
+ * ga_align_sw_matrix_batch) runs on its own results.
  * Enqueued on `stream`.  EXTENSION: no reference counterpart. */
 int ga_sw_validate_device(const ga_sw_dev_batch* b, uint8_t* flags, void* stream, const char** error);
 
