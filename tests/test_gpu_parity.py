@@ -352,6 +352,34 @@ def test_sw_long_texts_packed_chunks(gpu, oracle):
                 exp.end_j, exp.rescore_ok, exp.wellformed_ok), (k, len(reqs[k].text), len(reqs[k].pattern))
 
 
+def test_sw_large_batch_piecewise_upload(gpu, oracle):
+    """5,000 pairs through ga_align_sw_batch: the host packs and uploads the
+    sequences in four pieces, the device checks the CIGARs and packs them for
+    the copy back; every result, CIGAR and validator flag against the oracle,
+    with score-only pairs, empty sequences and non-ACGT bytes mixed in."""
+    import genoaligner_amd as ga
+    rng = random.Random(2024)
+    sc = (2, -3, 3, 1)
+    reqs = []
+    for k in range(5000):
+        t = "".join(rng.choice("ACGT") for _ in range(rng.randint(0, 420)))
+        a = rng.randint(0, max(len(t) - 1, 0))
+        p = _mutate(rng, t[a: a + rng.randint(0, 160)], 0.08)
+        if rng.random() < 0.1:
+            p = "".join(rng.choice("ACGTN") for _ in range(rng.randint(0, 120)))
+        if rng.random() < 0.03:
+            t = t[:len(t) // 2] + "N" * 5 + t[len(t) // 2:]
+        reqs.append(_sreq(ga, t, p, sc, rng.random() < 0.8))
+    b = ga.align_sw_batch(reqs)
+    want = oracle.align_sw_batch(reqs)
+    assert b.ok(), b.error
+    for k, (got, exp) in enumerate(zip(b.results, want.results)):
+        assert (got.score, got.resolved, got.cigar, got.start_i, got.start_j, got.end_i, got.end_j,
+                got.rescore_ok, got.wellformed_ok, got.too_large) == \
+               (exp.score, exp.resolved, exp.cigar, exp.start_i, exp.start_j, exp.end_i, exp.end_j,
+                exp.rescore_ok, exp.wellformed_ok, exp.too_large), k
+
+
 def test_sw_too_large(gpu):
     import genoaligner_amd as ga
     big = "A" * 8192
