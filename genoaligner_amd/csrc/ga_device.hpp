@@ -263,6 +263,10 @@ struct SwValidate {
     // substitution-matrix batches: M and X score sub[tcode * 32 + pcode]
     // (residue codes laid out like the bytes) instead of match / mismatch
     const uint8_t* tcode; const uint8_t* pcode; const int32_t* sub;
+    // global edit CIGARs (the WFA check, api.cpp:84-106): from (0, 0) to the
+    // ends of both sequences, the score = X + I + D ops, rescore_ok on the
+    // score alone; si/sj/ei/ej unused, pairs with score >= 0 checked
+    int edit;
 };
 hipError_t sw_validate_launch(const SwValidate& V, hipStream_t st);
 // Pair k's CIGAR (the coff[k+1] - coff[k] bytes ending its slot at

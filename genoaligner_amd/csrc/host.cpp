@@ -575,27 +575,9 @@ struct TimerHooks : ga::LaunchHooks {
 thread_local TimerHooks t_timer_hooks;
 
 // ---------------------------------------------------------------------------
-// the library-side CIGAR checks (api.cpp:84-106 and :114-156)
+// the library-side SW CIGAR check (api.cpp:114-156) on the host, for
+// ga_sw_validate_cigar (the batches run their checks on the device)
 // ---------------------------------------------------------------------------
-void wfa_validate(const char* cg, int len, int score, const char* pattern, int m,
-                  const char* text, int n, int32_t* rescore_ok, int32_t* wellformed_ok)
-{
-    int s = 0, pi = 0, ti = 0;
-    bool wf = true;
-    for (int c = 0; c < len; ++c) {
-        switch (cg[c]) {
-            case 'M': if (pi >= m || ti >= n || pattern[pi] != text[ti]) wf = false; ++pi; ++ti; break;
-            case 'X': if (pi >= m || ti >= n || pattern[pi] == text[ti]) wf = false; ++pi; ++ti; ++s; break;
-            case 'I': ++ti; ++s; break;
-            case 'D': ++pi; ++s; break;
-            default: wf = false; break;
-        }
-    }
-    if (pi != m || ti != n) wf = false;
-    *wellformed_ok = wf;
-    *rescore_ok = (s == score);
-}
-
 // End of the run of cg[c] in cg[0 .. len): eight bytes at a time against the
 // run's byte broadcast, then the first differing byte.
 int cigar_run_end(const char* cg, int c, int len)
@@ -992,6 +974,7 @@ const char* enqueue_wfa_dev(const WfaDevClasses& C, hipStream_t st, const std::v
 
 struct WfaOut {
     std::vector<int32_t> score, cig_len;
+    std::vector<uint8_t> vflags;     // the device's CIGAR checks: rescore_ok | wellformed_ok << 1
     std::vector<int64_t> slot_off;   // slot of pair i in the arena
     const char* cig_src = nullptr;   // where the CIGAR slots came back (pinned staging or the arena)
     PinnedHold pin;
@@ -1073,10 +1056,23 @@ const char* run_wfa(const ga_align_req* reqs, int64_t n, const std::vector<int32
     } else if (!err) {
         err = enqueue_wfa(L, plan, any_trace, st, &t_timer);
     }
-    // scores, CIGAR lengths and slots back into pinned staging (pageable
-    // fallback: the vectors and the caller's arena)
+    // the CIGAR checks on the device, once every CIGAR is written
+    uint8_t* d_vf = nullptr;
+    if (!err && any_trace) {
+        d_vf = D.alloc<uint8_t>(size_t(n));
+        ga::SwValidate V{};
+        V.txt = L.txt; V.toff = L.toff; V.tlen = L.tlen;
+        V.pat = L.pat; V.poff = L.poff; V.plen = L.plen;
+        V.cigar = L.cigar; V.cig_off = L.cig_off;
+        V.score = L.score; V.stride = 1; V.len = L.cig_len; V.len_stride = 1;
+        V.walk = L.want; V.flags = d_vf; V.n = n; V.edit = 1;
+        if (!D.ok || ga::sw_validate_launch(V, st) != hipSuccess) err = "WFA validate launch";
+    }
+    // scores, CIGAR lengths, check flags and slots back into pinned staging
+    // (pageable fallback: the vectors and the caller's arena)
     out.score.assign(static_cast<size_t>(n), -1);
     out.cig_len.assign(static_cast<size_t>(n), 0);
+    if (d_vf) out.vflags.assign(static_cast<size_t>(n), 0);
     int32_t* sc_h = reinterpret_cast<int32_t*>(out.pin.get(size_t(n) * 8));
     int32_t* cl_h = sc_h ? sc_h + n : nullptr;
     char* cdst = arena;
@@ -1089,6 +1085,7 @@ const char* run_wfa(const ga_align_req* reqs, int64_t n, const std::vector<int32
         if (!err && any_trace) {
             if (hipMemcpyAsync(cl_h ? cl_h : out.cig_len.data(), L.cig_len, size_t(n) * 4, hipMemcpyDeviceToHost,
                                st) != hipSuccess ||
+                (d_vf && hipMemcpyAsync(out.vflags.data(), d_vf, size_t(n), hipMemcpyDeviceToHost, st) != hipSuccess) ||
                 (slot > 0 && hipMemcpyAsync(cdst, L.cigar, size_t(slot), hipMemcpyDeviceToHost, st) != hipSuccess))
                 err = "hipMemcpy(results)";
         }
@@ -1173,8 +1170,12 @@ int wfa_common(const ga_align_req* reqs, int64_t n, ga_align_res* res, char* are
         if (o.resolved && trace[size_t(i)] && len > 0) {
             o.cigar_len = len;
             if (out.cig_src != arena) std::memcpy(arena + o.cigar_off, out.cig_src + o.cigar_off, size_t(len));
-            wfa_validate(arena + o.cigar_off, len, o.score, r.pattern, r.pattern_len, r.text,
-                         r.text_len, &o.rescore_ok, &o.wellformed_ok);
+            // the device's checks (every CIGAR batch runs them; without their
+            // flags both stay 0)
+            if (!out.vflags.empty()) {
+                o.rescore_ok = out.vflags[size_t(i)] & 1;
+                o.wellformed_ok = (out.vflags[size_t(i)] >> 1) & 1;
+            }
         } else {
             o.cigar_len = 0;
             o.cigar_off = out.slot_off[size_t(i)];

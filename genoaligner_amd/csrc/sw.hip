@@ -425,7 +425,7 @@ __global__ __launch_bounds__(256) void sw_validate_kernel(SwValidate V)
     for (int64_t k = wave; k < V.n; k += nwaves) {
         const int64_t q = k * V.stride;
         const int score = V.score[q], len = V.len[k * V.len_stride];
-        if (score <= 0 || len <= 0 || (V.walk && !V.walk[k])) {
+        if (score < (V.edit ? 0 : 1) || len <= 0 || (V.walk && !V.walk[k])) {
             if (lane == 0) V.flags[k] = 0;
             continue;
         }
@@ -434,14 +434,14 @@ __global__ __launch_bounds__(256) void sw_validate_kernel(SwValidate V)
             if (lane == 0) V.flags[k] = 0;
             continue;
         }
-        const int end_i = V.ei[q], end_j = V.ej[q];
+        const int end_i = V.edit ? m - 1 : V.ei[q], end_j = V.edit ? n - 1 : V.ej[q];
         const uint8_t* cg = V.cigar + V.cig_off[k] + (m + n - len);
         const uint8_t* T = V.txt + V.toff[k];
         const uint8_t* P = V.pat + V.poff[k];
         const uint8_t* TC = V.sub ? V.tcode + V.toff[k] : nullptr;
         const uint8_t* PC = V.sub ? V.pcode + V.poff[k] : nullptr;
         const int lim_i = end_i + 1 < m ? end_i + 1 : m, lim_j = end_j + 1 < n ? end_j + 1 : n;
-        int i0 = V.si[q], j0 = V.sj[q];
+        int i0 = V.edit ? 0 : V.si[q], j0 = V.edit ? 0 : V.sj[q];
         int s = 0;
         bool bad = false;
         uint8_t prev = 0;                                 // op before the step
@@ -463,26 +463,30 @@ __global__ __launch_bounds__(256) void sw_validate_kernel(SwValidate V)
                     b = true;                             // (a negative start is no SW result)
                 }
             }
-            if (isI) b |= i >= lim_i;
-            if (isD) b |= j >= lim_j;
+            if (isI && !V.edit) b |= i >= lim_i;          // (edit: only the ends are checked)
+            if (isD && !V.edit) b |= j >= lim_j;
             bad |= __any(b) != 0;
             // predecessor op: the lane below, or the last op of the step before
             const uint8_t before = uint8_t(__shfl_up(int(op), 1));
             const uint8_t pr = lane == 0 ? prev : before;
             const bool start = (isI || isD) && pr != op;
-            if (V.sub) {
+            if (V.edit) {
+                s += int(__popcll(__ballot(isX || isI || isD)));
+            } else if (V.sub) {
                 for (int off = 32; off > 0; off >>= 1) sub_s += __shfl_xor(sub_s, off);
                 s += sub_s;
             } else {
                 s += int(__popcll(__ballot(isM))) * V.match + int(__popcll(__ballot(isX))) * V.mismatch;
             }
-            s -= int(__popcll(__ballot(isI || isD))) * V.ge + int(__popcll(__ballot(start))) * (V.go - V.ge);
+            if (!V.edit)
+                s -= int(__popcll(__ballot(isI || isD))) * V.ge + int(__popcll(__ballot(start))) * (V.go - V.ge);
             i0 += int(__popcll(bi));
             j0 += int(__popcll(bj));
             prev = uint8_t(__shfl(int(op), 63));
         }
         const bool wf = !bad && i0 == end_i + 1 && j0 == end_j + 1;
-        if (lane == 0) V.flags[k] = uint8_t((wf && s == score ? 1 : 0) | (wf ? 2 : 0));
+        const bool rs = s == score && (wf || V.edit);
+        if (lane == 0) V.flags[k] = uint8_t((rs ? 1 : 0) | (wf ? 2 : 0));
     }
 }
 
